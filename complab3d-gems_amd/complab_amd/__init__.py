@@ -1,0 +1,250 @@
+"""Python front of libclb.so, the MI355X-native coupled reaction step.
+
+Everything here is plumbing over the C-ABI of include/clb.h: the device work
+is done by the HIP kernels in ../csrc.  Importing this module does not touch
+the GPU; `Context()` does, and raises if the HIP library is missing or no
+device is present -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import params as P
+from .params import CaseParams, default_params, copy_params  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libclb.so")
+_lib = None
+
+
+class ClbError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("clb error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """Loads libclb.so; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libclb.so is not built (run __graft_entry__.build()); "
+                           "the HIP extension is required, there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    V = C.c_void_p
+    I = C.c_int
+    D = C.c_double
+    sig = {
+        "clb_abi_version": ([], I),
+        "clb_ctx_create": ([I, C.POINTER(V)], I),
+        "clb_ctx_destroy": ([V], I),
+        "clb_last_error": ([V], C.c_char_p),
+        "clb_device_name": ([V, C.c_char_p, C.c_size_t], I),
+        "clb_sync": ([V], I),
+        "clb_case_create": ([V, V, V, V, C.POINTER(I)], I),
+        "clb_case_destroy": ([V, I], I),
+        "clb_case_footprint": ([V], C.c_size_t),
+        "clb_ns_init": ([V, I, D], I),
+        "clb_ns_solve": ([V, I, I, I, D, I, C.POINTER(I), C.POINTER(I)], I),
+        "clb_ns_couple": ([V, I], I),
+        "clb_ns_stats": ([V, I, V], I),
+        "clb_ns_update_dynamics": ([V, I, D], I),
+        "clb_set_ade_relaxation": ([V, I, V, V, D], I),
+        "clb_ade_stabilize": ([V, I, I], I),
+        "clb_run_steps": ([V, I, I, C.POINTER(I)], I),
+        "clb_ade_steps": ([V, I, I], I),
+        "clb_equilibrium": ([V, I], I),
+        "clb_ca_spill": ([V, I, C.POINTER(I)], I),
+        "clb_read_field": ([V, I, I, I, V], I),
+        "clb_write_field": ([V, I, I, I, V], I),
+        "clb_species_totals": ([V, I, V], I),
+        "clb_counters": ([V, I, V], I),
+        "clb_kernel_time": ([V, I, I, C.POINTER(D), C.POINTER(C.c_int64)], I),
+        "clb_rng_draws": ([C.c_uint64, I, V], I),
+        "clb_geom_spheres": ([I, I, I, D, I, C.c_uint64, I, V], I),
+        "clb_geom_pack": ([I, I, I, D, D, C.c_uint64, I, V, C.POINTER(D)], I),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names of the entry points declared in include/clb.h."""
+    return [
+        "clb_abi_version", "clb_ctx_create", "clb_ctx_destroy", "clb_last_error", "clb_device_name", "clb_sync",
+        "clb_case_create", "clb_case_destroy", "clb_case_footprint", "clb_ns_init", "clb_ns_solve",
+        "clb_ns_couple", "clb_ns_stats", "clb_ns_update_dynamics", "clb_set_ade_relaxation",
+        "clb_ade_stabilize", "clb_run_steps", "clb_ade_steps", "clb_equilibrium", "clb_ca_spill",
+        "clb_read_field", "clb_write_field", "clb_species_totals", "clb_counters", "clb_kernel_time",
+        "clb_rng_draws", "clb_geom_spheres", "clb_geom_pack",
+    ]
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One device, one HIP stream (clb_ctx)."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.clb_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise ClbError(rc, "no usable HIP device %d" % device)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().clb_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc < 0:
+            raise ClbError(rc, lib().clb_last_error(self.h).decode())
+        return rc
+
+    def device_name(self):
+        buf = C.create_string_buffer(256)
+        self.check(lib().clb_device_name(self.h, buf, 256))
+        return buf.value.decode()
+
+    def sync(self):
+        self.check(lib().clb_sync(self.h))
+
+
+class Case:
+    """One simulation on the device; the same methods as the oracle's case."""
+
+    def __init__(self, ctx, params, geometry, biomass0=None):
+        self.ctx = ctx
+        self.p = params
+        self.shape = (params.nz, params.ny, params.nx)
+        self.N = params.nx * params.ny * params.nz
+        g = np.ascontiguousarray(geometry, dtype=np.int8).reshape(-1)
+        if g.size != self.N:
+            raise ValueError("geometry has %d voxels, the lattice %d" % (g.size, self.N))
+        b = None if biomass0 is None else np.ascontiguousarray(biomass0, dtype=np.float64).reshape(-1)
+        cid = C.c_int()
+        ctx.check(lib().clb_case_create(ctx.h, C.byref(params), _p(g), _p(b) if b is not None else None,
+                                        C.byref(cid)))
+        self.id = cid.value
+
+    def close(self):
+        if getattr(self, "id", None) is not None and self.ctx.h:
+            lib().clb_case_destroy(self.ctx.h, self.id)
+            self.id = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        return self.ctx.check(rc)
+
+    def set_omegas(self, omp, omb, dt):
+        a = np.ascontiguousarray(omp, dtype=np.float64)
+        b = np.ascontiguousarray(omb, dtype=np.float64)
+        self._c(lib().clb_set_ade_relaxation(self.ctx.h, self.id, _p(a), _p(b), dt))
+
+    def ns_init(self, delta_p):
+        self._c(lib().clb_ns_init(self.ctx.h, self.id, delta_p))
+
+    def ns_solve(self, max_iter, rule, eps, reset=False):
+        it = C.c_int(); cv = C.c_int()
+        self._c(lib().clb_ns_solve(self.ctx.h, self.id, max_iter, rule, eps, int(reset), C.byref(it), C.byref(cv)))
+        return it.value, bool(cv.value)
+
+    def ns_couple(self):
+        self._c(lib().clb_ns_couple(self.ctx.h, self.id))
+
+    def ns_stats(self):
+        out = np.zeros(4)
+        self._c(lib().clb_ns_stats(self.ctx.h, self.id, _p(out)))
+        return out
+
+    def ns_update_dynamics(self, perm_ratio):
+        self._c(lib().clb_ns_update_dynamics(self.ctx.h, self.id, perm_ratio))
+
+    def ade_stabilize(self, steps):
+        self._c(lib().clb_ade_stabilize(self.ctx.h, self.id, steps))
+
+    def run_steps(self, n):
+        d = C.c_int()
+        rc = lib().clb_run_steps(self.ctx.h, self.id, n, C.byref(d))
+        return rc, d.value
+
+    def ade_steps(self, n):
+        self._c(lib().clb_ade_steps(self.ctx.h, self.id, n))
+
+    def equilibrium(self):
+        self._c(lib().clb_equilibrium(self.ctx.h, self.id))
+
+    def ca_spill(self):
+        sw = C.c_int()
+        rc = lib().clb_ca_spill(self.ctx.h, self.id, C.byref(sw))
+        return rc, sw.value
+
+    def read(self, field, index=0):
+        out = np.empty(self.N)
+        self._c(lib().clb_read_field(self.ctx.h, self.id, field, index, _p(out)))
+        return out.reshape(self.shape)
+
+    def write(self, field, index, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        self._c(lib().clb_write_field(self.ctx.h, self.id, field, index, _p(a)))
+
+    def species_totals(self):
+        out = np.zeros(self.p.n_species)
+        self._c(lib().clb_species_totals(self.ctx.h, self.id, _p(out)))
+        return out
+
+    def counters(self):
+        out = np.zeros(8, dtype=np.int64)
+        self._c(lib().clb_counters(self.ctx.h, self.id, _p(out)))
+        return out
+
+    def kernel_time(self, reset=False):
+        ms = C.c_double(); n = C.c_int64()
+        self._c(lib().clb_kernel_time(self.ctx.h, self.id, int(reset), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
+def footprint(params):
+    return lib().clb_case_footprint(C.byref(params))
+
+
+def geom_pack(nx_file, ny, nz, radius, porosity, seed, kind=P.GEOM_SPHERES):
+    out = np.zeros((nx_file + 2) * ny * nz, dtype=np.int8)
+    phi = C.c_double()
+    rc = lib().clb_geom_pack(nx_file, ny, nz, radius, porosity, seed, kind, _p(out), C.byref(phi))
+    return rc, out.reshape(nz, ny, nx_file + 2), phi.value
+
+
+def geom_spheres(nx, ny, nz, radius, count, seed, walls_yz):
+    out = np.zeros(nx * ny * nz, dtype=np.int8)
+    rc = lib().clb_geom_spheres(nx, ny, nz, radius, count, seed, int(walls_yz), _p(out))
+    if rc:
+        raise ClbError(rc, "clb_geom_spheres")
+    return out.reshape(nz, ny, nx)
+
+
+def rng_draws(seed, n):
+    out = np.zeros(n, dtype=np.uint64)
+    lib().clb_rng_draws(seed, n, _p(out))
+    return out

@@ -1,0 +1,128 @@
+"""The workloads: parameter blocks + seeded geometry for each configuration.
+
+Rates given "per step" are passed with a kinetic time step of 1, so the
+increment of a step is rate * 1 exactly as the reference's R * ade_dt with
+R = k_step / ade_dt.
+"""
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import params as P
+from .driver import FlowSetup, TransportSetup
+
+
+@dataclass
+class Workload:
+    name: str
+    params: P.CaseParams
+    geometry: np.ndarray          # (nz, ny, nx) int8 lattice incl. ghost columns
+    porosity: float
+    flow: FlowSetup
+    transport: TransportSetup
+    steps: int
+    rates_per_step: bool = True
+
+
+def _pack(nx_file, ny, nz, radius, porosity, seed, kind=P.GEOM_SPHERES, packer=None):
+    if packer is None:
+        from . import geom_pack as packer
+    for k in range(64):                          # a seed that does not percolate is skipped
+        rc, lat, phi = packer(nx_file, ny, nz, radius, porosity, seed + k, kind)
+        if rc == 0:
+            return lat, phi
+    raise RuntimeError("no percolating packing found")
+
+
+def config1(scale=1.0, precision=P.FP64, packer=None, seed=1):
+    """Plain method, FP64: 64x32x32, spheres r=4 to porosity 0.45, walls on y/z,
+    tau 0.8, Pe 1, two D3Q7 species A->B (k = 1e-3 per step), D 0.1 in pore,
+    A Dirichlet 1 at the inlet, Neumann outlets, 2000 steps."""
+    nxf, ny, nz = max(8, int(64 * scale)), max(6, int(32 * scale)), max(6, int(32 * scale))
+    radius = max(1.5, 4.0 * scale)
+    lat, phi = _pack(nxf, ny, nz, radius, 0.45, seed, packer=packer)
+    p = P.default_params(nxf + 2, ny, nz, 2)
+    p.precision = precision
+    p.c0[0] = 0.0; p.c0[1] = 0.0
+    p.left_btype[0] = P.BC_DIRICHLET; p.left_bc[0] = 1.0; p.right_btype[0] = P.BC_NEUMANN
+    p.left_btype[1] = P.BC_DIRICHLET; p.left_bc[1] = 0.0; p.right_btype[1] = P.BC_NEUMANN
+    p.n_rxn = 1
+    p.rxn_r1[0] = 0; p.rxn_r2[0] = -1
+    p.rxn_k[0] = 1e-3
+    p.rxn_stoich[0][0] = -1.0; p.rxn_stoich[0][1] = 1.0
+    p.ns_tau = 0.8
+    p.flow_on = 1
+    p.ns_conv_rule = P.CONV_RELDELTA
+    p.ns_conv_eps = 1e-8
+    p.ns_max_iter = 100000
+    fs = FlowSetup(pe=1.0, char_length=2 * radius, delta_p=1e-5 * (nxf + 1), ns_tau=0.8,
+                   rule=P.CONV_RELDELTA, eps=1e-8, max_iter=100000)
+    ts = TransportSetup(d_pore=[0.1, 0.1], d_biofilm=[0.1, 0.1])
+    return Workload("plain-fp64 %dx%dx%d spheres r%g phi0.45 A->B" % (nxf, ny, nz, radius), p, lat, phi, fs, ts, 2000)
+
+
+def seed_biofilm(lat, fraction, seed, code=P.BIOFILM):
+    """Marks `fraction` of the grain-surface pore voxels (pore with a bounce-back
+    face neighbour, x = 1..nx-2) as biofilm seeds; deterministic for a seed."""
+    nz, ny, nx = lat.shape
+    pore = lat == P.PORE
+    wall = lat == P.WALL
+    touch = np.zeros_like(pore)
+    for ax, sh in ((0, 1), (0, -1), (1, 1), (1, -1), (2, 1), (2, -1)):
+        touch |= np.roll(wall, sh, axis=ax)
+    surf = pore & touch
+    surf[:, :, 0] = False; surf[:, :, nx - 1] = False
+    surf[:, 0, :] = False; surf[:, ny - 1, :] = False; surf[0, :, :] = False; surf[nz - 1, :, :] = False
+    idx = np.flatnonzero(surf.reshape(-1))
+    rng = np.random.default_rng(seed)
+    k = max(1, int(round(fraction * idx.size)))
+    pick = rng.choice(idx, size=k, replace=False)
+    out = lat.copy().reshape(-1)
+    out[pick] = code
+    return out.reshape(lat.shape), k
+
+
+def config2(scale=1.0, precision=P.FP64, packer=None, seed=2):
+    """Biotic plain method on GPU: 256x128x128, spheres r=6 to porosity 0.35,
+    flow re-solved every 50 steps, four species (substrate, acceptor, two
+    products), D 0.1 pore / 0.02 biofilm, dual Monod mu_max 1e-3, K 0.1, one
+    biofilm seeded on 2 % of grain-surface voxels at 50, CA fraction rule,
+    Bmax 100, viscosity ratio 10, 10k steps."""
+    nxf, ny, nz = max(12, int(256 * scale)), max(8, int(128 * scale)), max(8, int(128 * scale))
+    radius = max(1.5, 6.0 * scale)
+    lat, phi = _pack(nxf, ny, nz, radius, 0.35, seed, packer=packer)
+    lat, nseed = seed_biofilm(lat, 0.02, seed)
+    p = P.default_params(nxf + 2, ny, nz, 4)
+    p.precision = precision
+    c0 = [0.5, 0.5, 0.0, 0.0]
+    inlet = [1.0, 1.0, 0.0, 0.0]
+    for s in range(4):
+        p.c0[s] = c0[s]
+        p.left_btype[s] = P.BC_DIRICHLET; p.left_bc[s] = inlet[s]
+        p.right_btype[s] = P.BC_NEUMANN
+    p.bio_law = P.BIO_DUAL_MONOD
+    p.donor = 0; p.acceptor = 1
+    p.n_microbes = 1
+    p.mu_max[0] = 1e-3; p.ks[0] = 0.1; p.ke[0] = 0.1; p.kdecay[0] = 0.0
+    for s, nu in enumerate((-0.02, -0.01, 0.01, 0.005)):
+        p.bio_stoich[0][s] = nu
+    p.b0[0] = 50.0
+    p.bmax = 100.0
+    p.thrd_bfilm_frac = 0.1
+    p.ca_method = P.CA_FRACTION
+    p.ca_seed = 1234 + seed
+    p.perm_ratio = 0.1                 # viscosity ratio 10
+    p.ns_tau = 0.8
+    p.flow_on = 1
+    p.ns_update_interval = 50
+    p.ade_update_interval = 1
+    p.ns_max_iter = 20000
+    p.ns_conv_rule = P.CONV_TRACER
+    p.ns_conv_eps = 1e-6
+    # a reduced domain keeps the full Peclet number only at full scale: tiny grains
+    # would need a pressure drop beyond the incompressible limit
+    fs = FlowSetup(pe=min(1.0, scale), char_length=2 * radius, delta_p=1e-5 * (nxf + 1), ns_tau=0.8,
+                   rule=P.CONV_TRACER, eps=1e-8, max_iter=100000, has_biofilm=True)
+    ts = TransportSetup(d_pore=[0.1] * 4, d_biofilm=[0.02] * 4)
+    return Workload("biotic-fp64 %dx%dx%d spheres r%g phi0.35 4 species CA" % (nxf, ny, nz, radius),
+                    p, lat, phi, fs, ts, 10000)

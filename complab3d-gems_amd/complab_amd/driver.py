@@ -1,0 +1,127 @@
+"""Set-up sequence of a CompLaB3D run on any case object (GPU or oracle).
+
+The order and formulas are those of complab.cpp: the flow solve, the
+permeability and the pressure-drop correction for the target Peclet number
+(complab.cpp:1111-1183), the lattice relaxation of every advection-diffusion
+lattice (complab.cpp:1220-1283), the initial mask update (1981-2000), the
+velocity coupling and the 10 000-step stabilization with its reset
+(2009-2042).  `case` is anything with the methods of complab_amd.Case.
+"""
+from dataclasses import dataclass, field
+from typing import List
+
+from . import params as P
+
+CS2_ADE = 0.25        # D3Q7: rest 1/4, faces 1/8  =>  cs^2 = 1/4
+CS2_NS = 1.0 / 3.0
+
+
+@dataclass
+class FlowSetup:
+    pe: float                       # target Peclet number (0: no flow)
+    char_length: float              # characteristic length, lattice units
+    delta_p: float                  # first pressure drop
+    ns_tau: float = 0.8
+    max_iter: int = 100000          # ns_max_iT1
+    rule: int = P.CONV_TRACER
+    eps: float = 1e-8               # ns_converge_iT1
+    tau_ade_fixed: float = 0.8      # complab.cpp:332
+    has_biofilm: bool = False       # bfilm_count > 0 (pore + biofilm mean velocity)
+
+
+@dataclass
+class TransportSetup:
+    d_pore: List[float]             # physical pore diffusivity per species
+    d_biofilm: List[float]
+    dx: float = 1.0
+    tau_if_no_flow: float = 0.8     # refTau = tau when Pe = 0
+    stabilize_steps: int = 10000
+
+
+@dataclass
+class SetupReport:
+    delta_p: float = 0.0
+    iters: List[int] = field(default_factory=list)
+    converged: List[bool] = field(default_factory=list)
+    pore_mean_u: float = 0.0
+    max_ux: float = 0.0
+    outlet_ux: float = 0.0
+    permeability: float = 0.0
+    pe_achieved: float = 0.0
+    ref_tau: float = 0.0
+    dt: float = 0.0
+    omega_pore: List[float] = field(default_factory=list)
+    omega_biofilm: List[float] = field(default_factory=list)
+
+
+def solve_flow(case, fs: FlowSetup, rep: SetupReport):
+    """NS solve + Pe rescale (complab.cpp:1098-1183)."""
+    nu = CS2_NS * (fs.ns_tau - 0.5)
+    dp = fs.delta_p
+    case.ns_init(dp)
+    it, cv = case.ns_solve(fs.max_iter, fs.rule, fs.eps, reset=True)
+    rep.iters.append(it); rep.converged.append(cv)
+    st = case.ns_stats()
+    pore_mean = st[3] if fs.has_biofilm else st[0]
+    outlet = st[2]
+    d_lat = CS2_ADE * (fs.tau_ade_fixed - 0.5)
+    perm = outlet * nu * fs.char_length / dp
+    u_t = fs.pe * d_lat / fs.char_length
+    dp_new = u_t * nu * fs.char_length / perm if perm > 0 else dp
+    max_ux = st[1]
+    if abs(dp_new - dp) / dp > 0.01:
+        case.ns_init(dp_new)
+        it, cv = case.ns_solve(fs.max_iter, fs.rule, fs.eps, reset=True)
+        rep.iters.append(it); rep.converged.append(cv)
+        st = case.ns_stats()
+        pore_mean = st[0]                      # complab.cpp:1175 averages the box
+        max_ux = st[1]
+        outlet = st[2]
+        dp = dp_new
+    rep.delta_p = dp
+    rep.pore_mean_u = pore_mean
+    rep.max_ux = max_ux
+    rep.outlet_ux = outlet
+    rep.permeability = perm
+    rep.pe_achieved = outlet * fs.char_length / d_lat
+    return pore_mean
+
+
+def ade_relaxation(p, pore_mean_u, fs: FlowSetup, ts: TransportSetup, rep: SetupReport):
+    """Lattice viscosity, relaxation and time step per species (complab.cpp:1220-1252)."""
+    if fs.pe > 1e-12:
+        ref_nu = pore_mean_u * fs.char_length / fs.pe
+        ref_tau = ref_nu / CS2_ADE + 0.5
+        if ref_tau > 2 or ref_tau <= 0.5:
+            raise ValueError("ADE tau=%g invalid (complab.cpp:1224)" % ref_tau)
+    else:
+        ref_tau = ts.tau_if_no_flow
+        ref_nu = CS2_ADE * (ref_tau - 0.5)
+    if not ts.d_pore[0] > 0:
+        raise ValueError("substrate 0 needs a positive pore diffusivity (complab.cpp:1239)")
+    dt = ref_nu * ts.dx * ts.dx / ts.d_pore[0]
+    omp, omb = [], []
+    for s in range(p.n_species):
+        nu_s = ref_nu * ts.d_pore[s] / ts.d_pore[0]
+        omp.append(1.0 / (nu_s / CS2_ADE + 0.5) if s > 0 else 1.0 / ref_tau)
+        omb.append(1.0 / (ref_nu * ts.d_biofilm[s] / ts.d_pore[0] / CS2_ADE + 0.5))
+    rep.ref_tau = ref_tau
+    rep.dt = dt
+    rep.omega_pore = omp
+    rep.omega_biofilm = omb
+    return omp, omb, dt
+
+
+def setup(case, p, fs: FlowSetup, ts: TransportSetup):
+    """Runs the whole set-up on `case`; returns the report."""
+    rep = SetupReport()
+    pore_mean = 0.0
+    if fs.pe > 0 and p.flow_on:
+        pore_mean = solve_flow(case, fs, rep)
+    omp, omb, dt = ade_relaxation(p, pore_mean, fs, ts, rep)
+    case.set_omegas(omp, omb, dt)
+    if fs.pe > 0 and p.flow_on:
+        case.ns_couple()
+        if ts.stabilize_steps > 0:
+            case.ade_stabilize(ts.stabilize_steps)
+    return rep

@@ -1,0 +1,151 @@
+"""ctypes mirror of ``clb_case_params`` (include/clb.h) and the constants.
+
+The field order and types here must match the C struct exactly; a layout
+check against ``sizeof(clb_case_params)`` as reported by the library runs at
+load time.
+"""
+import ctypes as C
+
+MAX_SPECIES = 8
+MAX_MICROBES = 2
+MAX_RXN = 4
+MAX_COMP = 8
+
+OK = 0
+E_ARG, E_HIP, E_NOMEM, E_STATE, E_NONFINITE, E_CA_STALL, E_CA_SWEEPS, E_AGE_SWEEPS, E_GEOMETRY = (
+    -1, -2, -3, -4, -5, -6, -7, -8, -9)
+
+SOLID, WALL, PORE, BIOFILM = 0, 1, 2, 3
+FP64, FP32 = 0, 1
+BC_DIRICHLET, BC_NEUMANN, BC_CLOSED = 0, 1, 2
+CONV_TRACER, CONV_RELDELTA = 0, 1
+BIO_NONE, BIO_DUAL_MONOD = 0, 1
+CA_FRACTION, CA_HALF = 0, 1
+EQ_AAPCF, EQ_NEWTON = 0, 1
+
+FIELD_CONC, FIELD_BIOMASS, FIELD_BTOTAL, FIELD_MASK, FIELD_VEL, FIELD_AGE, FIELD_PHASE, FIELD_NSRHO = range(8)
+GEOM_SPHERES, GEOM_RANDOM, GEOM_CYLINDERS, GEOM_FRACTURE = range(4)
+
+_d = C.c_double
+_i = C.c_int32
+
+
+class CaseParams(C.Structure):
+    _fields_ = [
+        ("nx", _i), ("ny", _i), ("nz", _i),
+        ("precision", _i),
+        ("n_species", _i),
+        ("mobile", _i * MAX_SPECIES),
+        ("c0", _d * MAX_SPECIES),
+        ("omega_pore", _d * MAX_SPECIES),
+        ("omega_biofilm", _d * MAX_SPECIES),
+        ("left_btype", _i * MAX_SPECIES), ("right_btype", _i * MAX_SPECIES),
+        ("left_bc", _d * MAX_SPECIES), ("right_bc", _d * MAX_SPECIES),
+        ("dt", _d),
+        ("clamp", _i),
+        ("bio_law", _i),
+        ("donor", _i), ("acceptor", _i),
+        ("mu_max", _d * MAX_MICROBES), ("ks", _d * MAX_MICROBES), ("ke", _d * MAX_MICROBES),
+        ("kdecay", _d * MAX_MICROBES),
+        ("bio_stoich", (_d * MAX_SPECIES) * MAX_MICROBES),
+        ("n_rxn", _i),
+        ("rxn_r1", _i * MAX_RXN), ("rxn_r2", _i * MAX_RXN),
+        ("rxn_k", _d * MAX_RXN),
+        ("rxn_stoich", (_d * MAX_SPECIES) * MAX_RXN),
+        ("rxn_cap_frac", _d), ("rxn_cap_dt", _d), ("rxn_min_conc", _d),
+        ("abiotic_rate_scale", _d),
+        ("surface_only", _i),
+        ("n_microbes", _i),
+        ("b0", _d * MAX_MICROBES),
+        ("bmax", _d),
+        ("thrd_bfilm_frac", _d),
+        ("ca_method", _i),
+        ("ca_seed", C.c_uint64),
+        ("ade_update_interval", _i),
+        ("ns_tau", _d),
+        ("delta_p", _d),
+        ("perm_ratio", _d),
+        ("precip_perm_ratio", _d),
+        ("ns_update_interval", _i),
+        ("ns_max_iter", _i),
+        ("ns_conv_rule", _i),
+        ("ns_conv_eps", _d),
+        ("flow_on", _i),
+        ("eq_on", _i),
+        ("eq_solver", _i),
+        ("eq_nc", _i),
+        ("eq_comp_species", _i * MAX_COMP),
+        ("eq_stoich", (_d * MAX_COMP) * MAX_SPECIES),
+        ("eq_logk", _d * MAX_SPECIES),
+        ("eq_tol", _d),
+        ("eq_max_iter", _i),
+        ("eq_anderson_depth", _i),
+        ("precip_on", _i),
+        ("precip_species", _i),
+        ("precip_max_rho", _d),
+        ("precip_interval", _i),
+        ("precip_phase_id", _i),
+        ("dissol_on", _i),
+        ("dissol_species", _i),
+        ("dissol_k", _d),
+        ("dissol_product", _i),
+        ("dissol_full", _d),
+        ("dissol_reopen_frac", _d),
+        ("dissol_interval", _i),
+        ("dissol_phase_id", _i),
+    ]
+
+
+def default_params(nx, ny, nz, n_species):
+    """A parameter block with the reference's defaults
+    (complab_functions.hh:1254-1265, part4_eqsolver.hh:204-215)."""
+    p = CaseParams()
+    p.nx, p.ny, p.nz = nx, ny, nz
+    p.precision = FP64
+    p.n_species = n_species
+    for s in range(MAX_SPECIES):
+        p.mobile[s] = 1
+        p.omega_pore[s] = 1.0
+        p.omega_biofilm[s] = 1.0
+        p.left_btype[s] = BC_NEUMANN
+        p.right_btype[s] = BC_NEUMANN
+    p.dt = 1.0
+    p.clamp = 1
+    p.bio_law = BIO_NONE
+    p.donor = -1
+    p.acceptor = -1
+    for r in range(MAX_RXN):
+        p.rxn_r1[r] = -1
+        p.rxn_r2[r] = -1
+    p.abiotic_rate_scale = 1.0
+    p.bmax = 999999999.0
+    p.thrd_bfilm_frac = 0.0
+    p.ca_method = CA_FRACTION
+    p.ca_seed = 12345
+    p.ade_update_interval = 1
+    p.ns_tau = 1.0
+    p.perm_ratio = 1.0
+    p.precip_perm_ratio = 1.0
+    p.ns_update_interval = 1
+    p.ns_max_iter = 100000
+    p.ns_conv_rule = CONV_TRACER
+    p.ns_conv_eps = 1e-6
+    p.flow_on = 0
+    p.eq_solver = EQ_AAPCF
+    for j in range(MAX_COMP):
+        p.eq_comp_species[j] = -1
+    p.eq_tol = 1e-8
+    p.eq_max_iter = 200
+    p.eq_anderson_depth = 4
+    p.precip_species = -1
+    p.precip_interval = 1
+    p.dissol_species = -1
+    p.dissol_product = -1
+    p.dissol_interval = 1
+    return p
+
+
+def copy_params(p):
+    q = CaseParams()
+    C.memmove(C.byref(q), C.byref(p), C.sizeof(CaseParams))
+    return q

@@ -1,0 +1,137 @@
+// clb_device.h -- argument blocks shared by the kernels and the host driver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define CLB_DEV_MAXS 8
+#define CLB_DEV_MAXM 2
+#define CLB_DEV_MAXR 4
+#define NS_TRACER_WINDOW 500
+
+namespace clb {
+
+enum { ADE_BB = 0, ADE_PORE = 1, ADE_BFILM = 2 };
+enum { NS_BGK = 0, NS_BIO = 1, NS_BB = 2, NS_NODYN = 3 };
+enum { BC_DIRICHLET_DEV = 0, BC_NEUMANN_DEV = 1, BC_CLOSED_DEV = 2 };
+
+// the advection-diffusion lattices of one case
+template <typename R>
+struct AdeDev {
+    int nx, ny, nz;
+    long long N;
+    int S, Sm, Smax, M;
+    int mobile[CLB_DEV_MAXS];      // per species
+    int slot[CLB_DEV_MAXS];        // per species: mobile slot or immobile slot
+    int mobSpecies[CLB_DEV_MAXS];  // per mobile slot: species index
+    R *cur, *out;                  // [Smax][7][N]
+    R *imm;                        // [Si][N]
+    R *B;                          // [M][N]
+    const R *u;                    // [3][N] or null
+    int8_t *mask;
+    const int8_t *geom;
+    const uint8_t *cls, *clsPull;
+    double *rhoN;                  // [2][Smax][ny*nz]
+    int anyNeumann;
+    int first, reset;
+    double c0[CLB_DEV_MAXS], omP[CLB_DEV_MAXS], omB[CLB_DEV_MAXS];
+    int lbt[CLB_DEV_MAXS], rbt[CLB_DEV_MAXS];
+    double lbc[CLB_DEV_MAXS], rbc[CLB_DEV_MAXS];
+    unsigned long long *clampCount;
+};
+
+// rate-law parameters (biotic dual Monod, abiotic mass action)
+struct KinDev {
+    int on, bio, S, M;
+    double dt;
+    int clamp;
+    int donor, acceptor;
+    double mu_max[CLB_DEV_MAXM], ks[CLB_DEV_MAXM], ke[CLB_DEV_MAXM], kdecay[CLB_DEV_MAXM];
+    double bio_stoich[CLB_DEV_MAXM][CLB_DEV_MAXS];
+    int nrxn;
+    int r1[CLB_DEV_MAXR], r2[CLB_DEV_MAXR];
+    double rk[CLB_DEV_MAXR];
+    double rxn_stoich[CLB_DEV_MAXR][CLB_DEV_MAXS];
+    double cap_frac, cap_dt, min_conc, rate_scale;
+    int surface_only;
+};
+
+template <typename R>
+struct NsDev {
+    int nx, ny, nz;
+    long long N;
+    R *cur, *out;                  // [19][N]
+    const uint8_t *cls, *clsPull;
+    double om, omBio, rhoBarRight;
+    int first;
+    int *done;
+};
+
+struct NsTracer {
+    double ring[NS_TRACER_WINDOW];
+    int head, count;
+    long long iters;
+    int converged, nonfinite;
+    double last, prevE;
+};
+
+template <typename R>
+struct CaDev {
+    int nx, ny, nz;
+    long long N;
+    int M;
+    R *B;                          // [M][N]
+    R *Btot;
+    int8_t *mask;
+    const int *age;
+    const int8_t *dist;
+    R *outbox;                     // [M][6][N]
+    uint8_t *pushed;
+    double bmax, thrdB;
+    int half;
+    uint64_t seed;
+};
+
+#define EQ_MAXS 8
+#define EQ_MAXC 8
+#define EQ_MAXD 4
+#define EQ_MIN_CONC 1e-30
+#define EQ_MAX_CONC 10.0
+
+struct EqDev {
+    int ns, nc;
+    int solver;                    // 0 AA-PCF, 1 Newton
+    double S[EQ_MAXS][EQ_MAXC];
+    double logk[EQ_MAXS];
+    int isEq[EQ_MAXS];
+    int compSpecies[EQ_MAXC];
+    double mu0[EQ_MAXC];           // smallest positive coefficient per component (PCF scale)
+    double tol;
+    int maxIter, depth;
+};
+
+// host launchers (clb_kernels.hip)
+template <typename R> hipError_t launchAdeStep(const AdeDev<R> &, const KinDev &, hipStream_t);
+template <typename R> hipError_t launchAdeDensity(const AdeDev<R> &, int, double *, hipStream_t);
+template <typename R> hipError_t launchAdeSet(const AdeDev<R> &, int, const double *, hipStream_t);
+template <typename R> hipError_t launchAdeReopen(const AdeDev<R> &, const uint8_t *, hipStream_t);
+template <typename R> hipError_t launchNsStep(const NsDev<R> &, double *, double *, NsTracer *, int, double, hipStream_t);
+template <typename R> hipError_t launchNsVelocity(const NsDev<R> &, R *, const int8_t *, double *, hipStream_t);
+template <typename R> hipError_t launchNsInit(const NsDev<R> &, double, hipStream_t);
+hipError_t launchNsUpdateDyn(const int8_t *, uint8_t *, long long, double, unsigned int *, hipStream_t);
+template <typename R> hipError_t launchNsReopen(const NsDev<R> &, const uint8_t *, hipStream_t);
+template <typename R> hipError_t launchEquilibrium(const AdeDev<R> &, const EqDev &, unsigned int *, hipStream_t);
+template <typename R> hipError_t launchCaMaskTotal(const CaDev<R> &, double *, long long *, hipStream_t);
+template <typename R> hipError_t launchCaPushGather(const CaDev<R> &, uint64_t, hipStream_t);
+template <typename R> hipError_t launchCaAge(const CaDev<R> &, const int *, int *, long long *, hipStream_t);
+hipError_t launchOmegaSwitch(const int8_t *, uint8_t *, long long, hipStream_t);
+hipError_t launchSolidSwitch(const int8_t *, uint8_t *, uint8_t *, long long, unsigned int *, hipStream_t);
+template <typename R> hipError_t launchPrecip(const AdeDev<R> &, int, double, int, int8_t *, unsigned int *, hipStream_t);
+template <typename R> hipError_t launchSpeciesTotal(const AdeDev<R> &, int, double *, hipStream_t);
+hipError_t launchFinalSum(const double *, long long, int, int, int, double *, hipStream_t);
+hipError_t launchFinalSumLL(const long long *, long long, long long *, hipStream_t);
+template <typename R> hipError_t launchToDouble(const R *, double *, long long, hipStream_t);
+template <typename R> hipError_t launchFromDouble(const double *, R *, long long, hipStream_t);
+hipError_t launchI8ToDouble(const int8_t *, double *, long long, hipStream_t);
+hipError_t launchI32ToDouble(const int *, double *, long long, hipStream_t);
+
+}  // namespace clb

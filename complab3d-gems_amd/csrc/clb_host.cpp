@@ -1,0 +1,954 @@
+// clb_host.cpp -- host driver of the coupled reaction step and the C-ABI (include/clb.h).
+//
+// A context owns one HIP stream on one device; a case owns every device field
+// of one simulation.  The driver reproduces the order of the CompLaB3D main
+// loop (complab.cpp:2178-3262) with device kernels; host code only makes the
+// loop decisions that the reference makes on reduced scalars (the CA trigger,
+// the mask-change trigger, the flow re-solve and its convergence).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "clb.h"
+#include "clb_device.h"
+
+using namespace clb;
+
+namespace {
+
+struct HipFail {
+    hipError_t e;
+    std::string what;
+};
+
+#define HIPT(call)                                                     \
+    do {                                                               \
+        hipError_t e_ = (call);                                        \
+        if (e_ != hipSuccess) throw HipFail{e_, std::string(#call)};   \
+    } while (0)
+
+struct Status {
+    int code;
+    std::string msg;
+};
+
+struct CaseBase {
+    clb_case_params p;
+    int nx = 0, ny = 0, nz = 0;
+    long long N = 0;
+    std::vector<void *> allocs;
+    virtual ~CaseBase() {}
+    virtual void nsInit(hipStream_t, double dP) = 0;
+    virtual Status nsSolve(hipStream_t, int maxIter, int rule, double eps, int reset, int *iters, int *conv) = 0;
+    virtual void nsCouple(hipStream_t) = 0;
+    virtual void nsStats(hipStream_t, double out[4]) = 0;
+    virtual void nsUpdateDynamics(hipStream_t, double perm) = 0;
+    virtual void adeStabilize(hipStream_t, int steps) = 0;
+    virtual Status runSteps(hipStream_t, int n, int *done) = 0;
+    virtual void adeSteps(hipStream_t, int n) = 0;
+    virtual void equilibrium(hipStream_t) = 0;
+    virtual Status caBlock(hipStream_t, int *sweeps) = 0;
+    virtual Status readField(hipStream_t, int field, int index, double *dst) = 0;
+    virtual Status writeField(hipStream_t, int field, int index, const double *src) = 0;
+    virtual void speciesTotals(hipStream_t, double *out) = 0;
+    virtual void counters(hipStream_t, int64_t out[8]) = 0;
+    virtual void kernelTime(hipStream_t, int reset, double *ms, int64_t *n) = 0;
+    virtual void setRelaxation(const double *omp, const double *omb, double dt) = 0;
+};
+
+template <typename T>
+T *dalloc(std::vector<void *> &allocs, size_t n) {
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) throw HipFail{e, "hipMalloc"};
+    allocs.push_back(p);
+    return (T *) p;
+}
+
+inline unsigned int nblk(long long n, int b) { return (unsigned int) ((n + b - 1) / b); }
+
+template <typename R>
+struct Case : CaseBase {
+    // fields
+    int8_t *mask = nullptr, *geom = nullptr, *dist = nullptr, *phase = nullptr;
+    uint8_t *adeCls = nullptr, *adeClsPull = nullptr, *nsCls = nullptr, *nsClsPull = nullptr, *reopened = nullptr;
+    R *fA = nullptr, *fB = nullptr, *imm = nullptr, *B = nullptr, *Btot = nullptr, *outbox = nullptr;
+    uint8_t *pushed = nullptr;
+    int *age = nullptr, *age2 = nullptr;
+    R *gA = nullptr, *gB = nullptr, *u = nullptr;
+    double *rhoN = nullptr, *parts = nullptr, *scal = nullptr, *stage = nullptr;
+    long long *partsLL = nullptr, *scalLL = nullptr;
+    unsigned long long *clampCount = nullptr;
+    unsigned int *ucnt = nullptr;       // [0] changed, [1] reopened, [2] eq nonconverged
+    int *nsDone = nullptr;
+    NsTracer *tracer = nullptr;
+    // state
+    int Sm = 0, Si = 0;
+    int mobSpecies[CLB_DEV_MAXS], slot[CLB_DEV_MAXS];
+    bool fCurIsA = true, gCurIsA = true;
+    bool adeFirst = true, adeReset = false, adeClsDirty = false;
+    bool nsFirst = true, nsClsDirty = false;
+    double nsOmega = 1, nsBioOmega = 1, deltaP = 0;
+    long long iT = 0;
+    int nsSaturate = 0;
+    long long oldTotMask = 0;
+    uint64_t caSweep = 0;
+    int64_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> evpool;
+    size_t evUsed = 0;
+    double kms = 0;
+    int64_t klaunch = 0;
+    unsigned int nparts = 0;
+
+    ~Case() override {
+        for (auto &e : evpool) { (void) hipEventDestroy(e.first); (void) hipEventDestroy(e.second); }
+        for (void *q : allocs) (void) hipFree(q);
+    }
+
+    R *fCur() const { return fCurIsA ? fA : fB; }
+    R *fOut() const { return fCurIsA ? fB : fA; }
+    R *gCur() const { return gCurIsA ? gA : gB; }
+    R *gOut() const { return gCurIsA ? gB : gA; }
+
+    AdeDev<R> ade() const {
+        AdeDev<R> a;
+        memset(&a, 0, sizeof(a));
+        a.nx = nx; a.ny = ny; a.nz = nz; a.N = N;
+        a.S = p.n_species; a.Sm = Sm; a.Smax = Sm > 0 ? Sm : 1; a.M = p.n_microbes;
+        for (int s = 0; s < CLB_DEV_MAXS; ++s) {
+            a.mobile[s] = s < p.n_species ? p.mobile[s] : 0;
+            a.slot[s] = slot[s];
+            a.mobSpecies[s] = mobSpecies[s];
+            a.c0[s] = p.c0[s]; a.omP[s] = p.omega_pore[s]; a.omB[s] = p.omega_biofilm[s];
+            a.lbt[s] = p.left_btype[s]; a.rbt[s] = p.right_btype[s];
+            a.lbc[s] = p.left_bc[s]; a.rbc[s] = p.right_bc[s];
+        }
+        a.cur = fCur(); a.out = fOut(); a.imm = imm; a.B = B;
+        a.u = u; a.mask = mask; a.geom = geom; a.cls = adeCls; a.clsPull = adeClsPull;
+        a.rhoN = rhoN;
+        a.anyNeumann = 0;
+        for (int ms = 0; ms < Sm; ++ms) {
+            int s = mobSpecies[ms];
+            if (p.left_btype[s] == CLB_BC_NEUMANN || p.right_btype[s] == CLB_BC_NEUMANN) a.anyNeumann = 1;
+        }
+        a.first = adeFirst ? 1 : 0; a.reset = adeReset ? 1 : 0;
+        a.clampCount = clampCount;
+        return a;
+    }
+    KinDev kin(bool on) const {
+        KinDev k;
+        memset(&k, 0, sizeof(k));
+        k.on = on ? 1 : 0;
+        k.bio = (p.bio_law != CLB_BIO_NONE && p.n_microbes > 0) ? 1 : 0;
+        k.S = p.n_species; k.M = p.n_microbes;
+        k.dt = p.dt; k.clamp = p.clamp;
+        k.donor = p.donor; k.acceptor = p.acceptor;
+        for (int m = 0; m < CLB_DEV_MAXM; ++m) {
+            k.mu_max[m] = p.mu_max[m]; k.ks[m] = p.ks[m]; k.ke[m] = p.ke[m]; k.kdecay[m] = p.kdecay[m];
+            for (int s = 0; s < CLB_DEV_MAXS; ++s) k.bio_stoich[m][s] = p.bio_stoich[m][s];
+        }
+        k.nrxn = p.n_rxn;
+        for (int r = 0; r < CLB_DEV_MAXR; ++r) {
+            k.r1[r] = p.rxn_r1[r]; k.r2[r] = p.rxn_r2[r]; k.rk[r] = p.rxn_k[r];
+            for (int s = 0; s < CLB_DEV_MAXS; ++s) k.rxn_stoich[r][s] = p.rxn_stoich[r][s];
+        }
+        k.cap_frac = p.rxn_cap_frac; k.cap_dt = p.rxn_cap_dt; k.min_conc = p.rxn_min_conc;
+        k.rate_scale = p.abiotic_rate_scale; k.surface_only = p.surface_only;
+        if (!k.bio && k.nrxn == 0) k.on = 0;
+        return k;
+    }
+    NsDev<R> ns() const {
+        NsDev<R> a;
+        memset(&a, 0, sizeof(a));
+        a.nx = nx; a.ny = ny; a.nz = nz; a.N = N;
+        a.cur = gCur(); a.out = gOut();
+        a.cls = nsCls; a.clsPull = nsClsPull;
+        a.om = nsOmega; a.omBio = nsBioOmega; a.rhoBarRight = -deltaP * 3.0;
+        a.first = nsFirst ? 1 : 0;
+        a.done = nsDone;
+        return a;
+    }
+    CaDev<R> ca() const {
+        CaDev<R> c;
+        memset(&c, 0, sizeof(c));
+        c.nx = nx; c.ny = ny; c.nz = nz; c.N = N; c.M = p.n_microbes;
+        c.B = B; c.Btot = Btot; c.mask = mask; c.age = age; c.dist = dist;
+        c.outbox = outbox; c.pushed = pushed;
+        c.bmax = p.bmax; c.thrdB = p.thrd_bfilm_frac * p.bmax;
+        c.half = p.ca_method == CLB_CA_HALF ? 1 : 0;
+        c.seed = p.ca_seed;
+        return c;
+    }
+    EqDev eqdev() const {
+        EqDev e;
+        memset(&e, 0, sizeof(e));
+        e.ns = p.n_species; e.nc = p.eq_nc; e.solver = p.eq_solver;
+        for (int i = 0; i < p.n_species; ++i) {
+            e.logk[i] = p.eq_logk[i];
+            e.isEq[i] = 0;
+            for (int j = 0; j < p.eq_nc; ++j) { e.S[i][j] = p.eq_stoich[i][j]; if (std::fabs(p.eq_stoich[i][j]) > 1e-10) e.isEq[i] = 1; }
+        }
+        for (int j = 0; j < p.eq_nc; ++j) {
+            e.compSpecies[j] = p.eq_comp_species[j];
+            double mu0 = 1.0;                                   // pcf_residual, part4:702-712
+            for (int i = 0; i < p.n_species; ++i) {
+                if (!e.isEq[i]) continue;
+                double mu = p.eq_stoich[i][j];
+                if (mu > 1e-10 && mu < mu0) mu0 = mu;
+            }
+            e.mu0[j] = mu0;
+        }
+        e.tol = p.eq_tol; e.maxIter = p.eq_max_iter; e.depth = p.eq_anderson_depth;
+        return e;
+    }
+
+    void create(const clb_case_params &pp, const int8_t *geometry, const double *biomass0, hipStream_t st) {
+        p = pp;
+        nx = p.nx; ny = p.ny; nz = p.nz; N = (long long) nx * ny * nz;
+        Sm = 0; Si = 0;
+        for (int s = 0; s < CLB_DEV_MAXS; ++s) { mobSpecies[s] = 0; slot[s] = 0; }
+        for (int s = 0; s < p.n_species; ++s) {
+            if (p.mobile[s]) { slot[s] = Sm; mobSpecies[Sm] = s; ++Sm; }
+            else { slot[s] = Si; ++Si; }
+        }
+        mask = dalloc<int8_t>(allocs, N); geom = dalloc<int8_t>(allocs, N);
+        dist = dalloc<int8_t>(allocs, N); phase = dalloc<int8_t>(allocs, N);
+        adeCls = dalloc<uint8_t>(allocs, N); adeClsPull = dalloc<uint8_t>(allocs, N);
+        nsCls = dalloc<uint8_t>(allocs, N); nsClsPull = dalloc<uint8_t>(allocs, N);
+        reopened = dalloc<uint8_t>(allocs, N);
+        fA = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * 7 * N);
+        fB = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * 7 * N);
+        imm = dalloc<R>(allocs, (size_t) std::max(Si, 1) * N);
+        const int M = p.n_microbes;
+        B = dalloc<R>(allocs, (size_t) std::max(M, 1) * N);
+        Btot = dalloc<R>(allocs, N);
+        if (M > 0) {
+            outbox = dalloc<R>(allocs, (size_t) M * 6 * N);
+            pushed = dalloc<uint8_t>(allocs, N);
+        }
+        age = dalloc<int>(allocs, N); age2 = dalloc<int>(allocs, N);
+        if (p.flow_on) { gA = dalloc<R>(allocs, 19 * (size_t) N); gB = dalloc<R>(allocs, 19 * (size_t) N); }
+        u = dalloc<R>(allocs, 3 * (size_t) N);
+        rhoN = dalloc<double>(allocs, 2 * (size_t) std::max(Sm, 1) * ny * nz);
+        nparts = nblk(N, 128);
+        parts = dalloc<double>(allocs, (size_t) nparts * 5);
+        partsLL = dalloc<long long>(allocs, nparts);
+        scal = dalloc<double>(allocs, 16);
+        scalLL = dalloc<long long>(allocs, 4);
+        stage = dalloc<double>(allocs, N);
+        clampCount = dalloc<unsigned long long>(allocs, 1);
+        ucnt = dalloc<unsigned int>(allocs, 4);
+        nsDone = dalloc<int>(allocs, 1);
+        tracer = dalloc<NsTracer>(allocs, 1);
+
+        // host-side construction of the voxel bytes and scalar fields
+        std::vector<int8_t> g(geometry, geometry + N), dh(N), ph(N, 0);
+        std::vector<uint8_t> ac(N), nc(N);
+        std::vector<int> ag(N);
+        for (long long i = 0; i < N; ++i) {
+            int gv = g[i];
+            int x = (int) (i % nx), y = (int) ((i / nx) % ny), z = (int) (i / ((long long) nx * ny));
+            bool closed = gv == CLB_SOLID || gv == CLB_WALL;
+            ac[i] = closed ? ADE_BB : (gv == CLB_PORE ? ADE_PORE : ADE_BFILM);
+            ag[i] = closed ? -1 : (gv == CLB_PORE ? 0 : 1);                      // createAgeDomain3D
+            bool in = x < nx - 1 && y < ny - 1 && z < nz - 1;                   // calculateDistanceFromSolid
+            dh[i] = (in && !closed) ? 1 : ((gv == CLB_SOLID && in) ? -1 : 0);
+            if (gv == CLB_WALL) nc[i] = NS_BB;
+            else if (gv == CLB_SOLID) nc[i] = NS_NODYN;
+            else if (gv == CLB_PORE) nc[i] = NS_BGK;
+            else nc[i] = p.perm_ratio > 1e-14 ? NS_BIO : NS_BB;
+        }
+        HIPT(hipMemcpyAsync(mask, g.data(), N, hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpyAsync(geom, g.data(), N, hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpyAsync(dist, dh.data(), N, hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpyAsync(phase, ph.data(), N, hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpyAsync(adeCls, ac.data(), N, hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpyAsync(adeClsPull, ac.data(), N, hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpyAsync(nsCls, nc.data(), N, hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpyAsync(nsClsPull, nc.data(), N, hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpyAsync(age, ag.data(), N * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPT(hipMemsetAsync(reopened, 0, N, st));
+        // populations at rest with c0 (initializeAtEquilibrium + stabilizeADElattice3D seeding)
+        {
+            std::vector<R> fl((size_t) 7 * N);
+            for (int ms = 0; ms < Sm; ++ms) {
+                int s = mobSpecies[ms];
+                for (int q = 0; q < 7; ++q) {
+                    R v = (R) ((q == 0 ? 0.25 : 0.125) * p.c0[s]);
+                    std::fill(fl.begin() + (size_t) q * N, fl.begin() + (size_t) (q + 1) * N, v);
+                }
+                HIPT(hipMemcpy(fA + (size_t) ms * 7 * N, fl.data(), fl.size() * sizeof(R), hipMemcpyHostToDevice));
+                HIPT(hipMemcpy(fB + (size_t) ms * 7 * N, fl.data(), fl.size() * sizeof(R), hipMemcpyHostToDevice));
+            }
+            std::vector<R> sc(N);
+            for (int s = 0; s < p.n_species; ++s) {
+                if (p.mobile[s]) continue;
+                std::fill(sc.begin(), sc.end(), (R) p.c0[s]);
+                HIPT(hipMemcpy(imm + (size_t) slot[s] * N, sc.data(), N * sizeof(R), hipMemcpyHostToDevice));
+            }
+            std::vector<double> tot(N, 0.0);
+            for (int m = 0; m < M; ++m) {
+                for (long long i = 0; i < N; ++i) {
+                    double v = biomass0 ? biomass0[(size_t) m * N + i] : (g[i] == CLB_BIOFILM + m ? p.b0[m] : 0.0);
+                    sc[i] = (R) v;
+                    tot[i] += (double) sc[i];
+                }
+                HIPT(hipMemcpy(B + (size_t) m * N, sc.data(), N * sizeof(R), hipMemcpyHostToDevice));
+            }
+            for (long long i = 0; i < N; ++i) sc[i] = (R) tot[i];
+            HIPT(hipMemcpy(Btot, sc.data(), N * sizeof(R), hipMemcpyHostToDevice));
+            long long ms = 0;
+            for (long long i = 0; i < N; ++i) ms += g[i];
+            oldTotMask = ms;
+        }
+        HIPT(hipMemsetAsync(u, 0, 3 * (size_t) N * sizeof(R), st));
+        HIPT(hipMemsetAsync(clampCount, 0, sizeof(unsigned long long), st));
+        HIPT(hipMemsetAsync(ucnt, 0, 4 * sizeof(unsigned int), st));
+        HIPT(hipMemsetAsync(nsDone, 0, sizeof(int), st));
+        HIPT(hipMemsetAsync(tracer, 0, sizeof(NsTracer), st));
+        nsOmega = 1.0 / p.ns_tau;
+        nsBioOmega = 1.0 / (p.perm_ratio * (1.0 / nsOmega - 0.5) + 0.5);
+        HIPT(hipStreamSynchronize(st));
+    }
+
+    void setRelaxation(const double *omp, const double *omb, double dt) override {
+        for (int s = 0; s < p.n_species; ++s) { p.omega_pore[s] = omp[s]; p.omega_biofilm[s] = omb[s]; }
+        p.dt = dt;
+    }
+
+    // ------------------------------------------------------------------ flow
+    void nsInit(hipStream_t st, double dP) override {
+        if (!p.flow_on) return;
+        deltaP = dP;
+        // NSdomainSetup: dynamics from the static geometry
+        std::vector<int8_t> g(N);
+        HIPT(hipMemcpy(g.data(), geom, N, hipMemcpyDeviceToHost));
+        std::vector<uint8_t> nc(N);
+        for (long long i = 0; i < N; ++i) {
+            int gv = g[i];
+            if (gv == CLB_WALL) nc[i] = NS_BB;
+            else if (gv == CLB_SOLID) nc[i] = NS_NODYN;
+            else if (gv == CLB_PORE) nc[i] = NS_BGK;
+            else nc[i] = p.perm_ratio > 1e-14 ? NS_BIO : NS_BB;
+        }
+        HIPT(hipMemcpyAsync(nsCls, nc.data(), N, hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpyAsync(nsClsPull, nc.data(), N, hipMemcpyHostToDevice, st));
+        nsBioOmega = 1.0 / (p.perm_ratio * (1.0 / nsOmega - 0.5) + 0.5);
+        gCurIsA = true;
+        HIPT(launchNsInit<R>(ns(), dP, st));
+        nsFirst = true;
+        nsClsDirty = false;
+        HIPT(hipStreamSynchronize(st));
+    }
+
+    Status nsSolve(hipStream_t st, int maxIter, int rule, double eps, int reset, int *iters, int *conv) override {
+        if (!p.flow_on) { if (iters) *iters = 0; if (conv) *conv = 1; return {CLB_OK, ""}; }
+        if (reset) HIPT(hipMemsetAsync(tracer, 0, sizeof(NsTracer), st));
+        HIPT(hipMemsetAsync(nsDone, 0, sizeof(int), st));
+        // the tracer keeps its window; only the per-call fields restart
+        NsTracer th;
+        HIPT(hipMemcpyAsync(&th, tracer, sizeof(NsTracer), hipMemcpyDeviceToHost, st));
+        HIPT(hipStreamSynchronize(st));
+        long long base = th.iters;
+        th.converged = 0; th.nonfinite = 0;
+        HIPT(hipMemcpyAsync(tracer, &th, sizeof(NsTracer), hipMemcpyHostToDevice, st));
+        const unsigned int g = nblk(N, 256);
+        double *partE = parts, *partN = parts + g;
+        int done = 0;
+        long long ran = 0;
+        bool parity = gCurIsA;
+        bool launched = false;
+        while (ran < maxIter) {
+            const int batch = (int) std::min<long long>(100, maxIter - ran);
+            bool curA = parity;
+            for (int k = 0; k < batch; ++k) {
+                NsDev<R> a = ns();
+                a.cur = curA ? gA : gB; a.out = curA ? gB : gA;
+                HIPT(launchNsStep<R>(a, partE, partN, tracer, rule, eps, st));
+                nsFirst = false;
+                if (!launched && nsClsDirty) { HIPT(hipMemcpyAsync(nsClsPull, nsCls, N, hipMemcpyDeviceToDevice, st)); nsClsDirty = false; }
+                launched = true;
+                curA = !curA;
+            }
+            HIPT(hipMemcpyAsync(&th, tracer, sizeof(NsTracer), hipMemcpyDeviceToHost, st));
+            HIPT(hipMemcpyAsync(&done, nsDone, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPT(hipStreamSynchronize(st));
+            const long long eff = th.iters - base;   // launches that did work
+            parity = ((eff & 1) == 0) ? gCurIsA : !gCurIsA;
+            if (!done && eff < ran + batch) return {CLB_E_HIP, "flow iterations were lost"};
+            ran = eff;
+            if (done) break;
+        }
+        gCurIsA = parity;
+        if (iters) *iters = (int) ran;
+        if (conv) *conv = th.converged ? 1 : 0;
+        if (th.nonfinite) return {CLB_E_NONFINITE, "non-finite flow energy"};
+        return {CLB_OK, ""};
+    }
+
+    void nsCouple(hipStream_t st) override {
+        if (!p.flow_on) return;
+        HIPT(launchNsVelocity<R>(ns(), u, geom, nullptr, st));
+    }
+    void nsStats(hipStream_t st, double out[4]) override {
+        for (int k = 0; k < 4; ++k) out[k] = 0;
+        if (!p.flow_on) return;
+        const unsigned int g = nblk(N, 256);
+        HIPT(launchNsVelocity<R>(ns(), nullptr, geom, parts, st));
+        for (int c = 0; c < 5; ++c) HIPT(launchFinalSum(parts, g, 5, c, c == 1, scal + c, st));
+        double h[5];
+        HIPT(hipMemcpyAsync(h, scal, 5 * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPT(hipStreamSynchronize(st));
+        out[0] = h[0] / ((double) (nx - 2) * ny * nz);
+        out[1] = h[1];
+        out[2] = h[2] / ((double) ny * nz);
+        out[3] = h[4] > 0 ? h[3] / h[4] : 0.0;
+    }
+    void nsUpdateDynamics(hipStream_t st, double perm) override {
+        if (!p.flow_on) return;
+        nsBioOmega = 1.0 / (perm * (1.0 / nsOmega - 0.5) + 0.5);
+        HIPT(hipMemsetAsync(ucnt, 0, sizeof(unsigned int), st));
+        HIPT(launchNsUpdateDyn(mask, nsCls, N, perm, ucnt, st));
+        unsigned int ch = 0;
+        HIPT(hipMemcpyAsync(&ch, ucnt, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        HIPT(hipStreamSynchronize(st));
+        if (ch) {
+            HIPT(launchNsReopen<R>(ns(), nsClsPull, st));
+            nsClsDirty = true;
+        }
+    }
+
+    // ------------------------------------------------------------------ transport
+    // device time of every fused launch, from a pair of events around it on the
+    // case's stream; read back when the stream has drained
+    void flushTimes(hipStream_t st) {
+        if (!evUsed) return;
+        HIPT(hipStreamSynchronize(st));
+        for (size_t k = 0; k < evUsed; ++k) {
+            float ms = 0;
+            HIPT(hipEventElapsedTime(&ms, evpool[k].first, evpool[k].second));
+            kms += ms; ++klaunch;
+        }
+        evUsed = 0;
+    }
+    void fusedLaunch(hipStream_t st, bool kinOn) {
+        if (evUsed == evpool.size()) {
+            if (evpool.size() >= 256) flushTimes(st);
+            else { hipEvent_t e0, e1; HIPT(hipEventCreate(&e0)); HIPT(hipEventCreate(&e1)); evpool.push_back({e0, e1}); }
+        }
+        AdeDev<R> a = ade();
+        KinDev k = kin(kinOn);
+        HIPT(hipEventRecord(evpool[evUsed].first, st));
+        HIPT(launchAdeStep<R>(a, k, st));
+        HIPT(hipEventRecord(evpool[evUsed].second, st));
+        ++evUsed;
+        fCurIsA = !fCurIsA;
+        adeFirst = false;
+        adeReset = false;
+        if (adeClsDirty) { HIPT(hipMemcpyAsync(adeClsPull, adeCls, N, hipMemcpyDeviceToDevice, st)); adeClsDirty = false; }
+    }
+    void adeStabilize(hipStream_t st, int steps) override {
+        for (int t = 0; t < steps; ++t) fusedLaunch(st, false);
+        flushTimes(st);
+        adeReset = true;
+    }
+    void adeSteps(hipStream_t st, int n) override {
+        for (int t = 0; t < n; ++t) fusedLaunch(st, true);
+        flushTimes(st);
+    }
+    void equilibrium(hipStream_t st) override {
+        if (!p.eq_on || p.eq_nc <= 0) return;
+        HIPT(launchEquilibrium<R>(ade(), eqdev(), ucnt + 2, st));
+    }
+
+    // mask/total update + global max of the total + mask sum
+    void maskTotal(hipStream_t st, double *gmax, long long *msum) {
+        const unsigned int g = nblk(N, 256);
+        HIPT(launchCaMaskTotal<R>(ca(), parts, partsLL, st));
+        HIPT(launchFinalSum(parts, g, 1, 0, 1, scal, st));
+        HIPT(launchFinalSumLL(partsLL, g, scalLL, st));
+        HIPT(hipMemcpyAsync(gmax, scal, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPT(hipMemcpyAsync(msum, scalLL, sizeof(long long), hipMemcpyDeviceToHost, st));
+        HIPT(hipStreamSynchronize(st));
+    }
+    // updateAgeDistance3D sweeps; to the fixed point (sum of increments 0) or once
+    Status ageRelax(hipStream_t st, bool toFixedPoint) {
+        const unsigned int g = nblk(N, 256);
+        int wc1 = 0;
+        while (true) {
+            HIPT(launchCaAge<R>(ca(), age, age2, partsLL, st));
+            HIPT(launchFinalSumLL(partsLL, g, scalLL, st));
+            long long diff = 0;
+            HIPT(hipMemcpyAsync(&diff, scalLL, sizeof(long long), hipMemcpyDeviceToHost, st));
+            HIPT(hipStreamSynchronize(st));
+            std::swap(age, age2);
+            if (!toFixedPoint || diff == 0) break;
+            if (++wc1 > 1000) return {CLB_E_AGE_SWEEPS, "age relaxation did not settle"};
+        }
+        return {CLB_OK, ""};
+    }
+    Status caBlock(hipStream_t st, int *sweepsOut) override {
+        int sweeps = 0;
+        if (sweepsOut) *sweepsOut = 0;
+        if (p.n_microbes <= 0) return {CLB_OK, ""};
+        double gmax; long long msum;
+        maskTotal(st, &gmax, &msum);
+        if (!std::isfinite(gmax)) return {CLB_E_NONFINITE, "non-finite biomass"};
+        if (gmax - p.bmax > 1e-12) {
+            double prev = gmax; int stall = 0, wc = 0;
+            while (gmax - p.bmax > 1e-12) {
+                HIPT(launchCaPushGather<R>(ca(), caSweep++, st));
+                maskTotal(st, &gmax, &msum);
+                ++sweeps; cnt[6]++;
+                if (!std::isfinite(gmax)) { if (sweepsOut) *sweepsOut = sweeps; return {CLB_E_NONFINITE, "non-finite biomass"}; }
+                if (gmax > prev - 1e-12) { if (++stall >= 100) { if (sweepsOut) *sweepsOut = sweeps; return {CLB_E_CA_STALL, "CA spill stalled"}; } }
+                else stall = 0;
+                prev = gmax;
+                if (wc % 50 == 0) { Status s = ageRelax(st, true); if (s.code) return s; }
+                if (wc > 1000) { if (sweepsOut) *sweepsOut = sweeps; return {CLB_E_CA_SWEEPS, "CA spill did not settle"}; }
+                ++wc;
+            }
+        }
+        lastMaskSum = msum;
+        if (sweepsOut) *sweepsOut = sweeps;
+        return {CLB_OK, ""};
+    }
+    long long lastMaskSum = 0;
+
+    Status resolve(hipStream_t st, double perm) {
+        nsUpdateDynamics(st, perm);
+        int it = 0, cv = 0;
+        Status s = nsSolve(st, p.ns_max_iter, p.ns_conv_rule, p.ns_conv_eps, 0, &it, &cv);
+        if (s.code) return s;
+        cnt[4]++;
+        if (!cv) {
+            nsSaturate = 1;
+            double stt[4];
+            nsStats(st, stt);
+            if (stt[2] > 1e-12) nsSaturate = 0;
+        }
+        nsCouple(st);
+        return {CLB_OK, ""};
+    }
+
+    Status coupledStep(hipStream_t st) {
+        fusedLaunch(st, true);
+        if (p.eq_on) equilibrium(st);
+        if (p.n_microbes > 0) {
+            int sw = 0;
+            Status s = caBlock(st, &sw);
+            if (s.code) return s;
+            if (lastMaskSum != oldTotMask) {
+                oldTotMask = lastMaskSum;
+                cnt[5]++;
+                Status a = ageRelax(st, false);
+                if (a.code) return a;
+                if (p.ade_update_interval > 0 && iT % p.ade_update_interval == 0) HIPT(launchOmegaSwitch(mask, adeCls, N, st));
+                if (p.ns_update_interval > 0 && iT % p.ns_update_interval == 0 && p.flow_on && !nsSaturate) {
+                    Status r = resolve(st, p.perm_ratio);
+                    if (r.code) return r;
+                }
+            }
+        }
+        if (p.precip_on && p.precip_interval > 0 && iT % p.precip_interval == 0 && p.precip_species >= 0) {
+            HIPT(hipMemsetAsync(ucnt, 0, 2 * sizeof(unsigned int), st));
+            HIPT(launchPrecip<R>(ade(), slot[p.precip_species], p.precip_max_rho, p.precip_phase_id, phase, ucnt, st));
+            unsigned int ch = 0;
+            HIPT(hipMemcpyAsync(&ch, ucnt, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+            HIPT(hipStreamSynchronize(st));
+            if (ch) {
+                cnt[5]++;
+                if (p.flow_on && !nsSaturate) { Status r = resolve(st, p.precip_perm_ratio); if (r.code) return r; }
+                unsigned int nre = 0;
+                HIPT(hipMemsetAsync(ucnt + 1, 0, sizeof(unsigned int), st));
+                HIPT(launchSolidSwitch(mask, adeCls, reopened, N, ucnt + 1, st));
+                HIPT(hipMemcpyAsync(&nre, ucnt + 1, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+                HIPT(hipStreamSynchronize(st));
+                if (nre) HIPT(launchAdeReopen<R>(ade(), reopened, st));
+                adeClsDirty = true;
+            }
+        }
+        ++iT; cnt[7]++;
+        return {CLB_OK, ""};
+    }
+    Status runSteps(hipStream_t st, int n, int *done) override {
+        int k = 0;
+        Status s{CLB_OK, ""};
+        for (; k < n; ++k) { s = coupledStep(st); if (s.code) break; }
+        flushTimes(st);
+        if (done) *done = k;
+        return s;
+    }
+
+    // ------------------------------------------------------------------ fields
+    Status readField(hipStream_t st, int field, int index, double *dst) override {
+        switch (field) {
+        case CLB_FIELD_CONC:
+            if (index < 0 || index >= p.n_species) return {CLB_E_ARG, "species index out of range"};
+            if (p.mobile[index]) {
+                AdeDev<R> a = ade();
+                HIPT(launchAdeDensity<R>(a, slot[index], stage, st));
+            } else HIPT(launchToDouble<R>(imm + (size_t) slot[index] * N, stage, N, st));
+            break;
+        case CLB_FIELD_BIOMASS:
+            if (index < 0 || index >= p.n_microbes) return {CLB_E_ARG, "microbe index out of range"};
+            HIPT(launchToDouble<R>(B + (size_t) index * N, stage, N, st));
+            break;
+        case CLB_FIELD_BTOTAL: HIPT(launchToDouble<R>(Btot, stage, N, st)); break;
+        case CLB_FIELD_MASK: HIPT(launchI8ToDouble(mask, stage, N, st)); break;
+        case CLB_FIELD_VEL:
+            if (index < 0 || index > 2) return {CLB_E_ARG, "velocity component out of range"};
+            HIPT(launchToDouble<R>(u + (size_t) index * N, stage, N, st));
+            break;
+        case CLB_FIELD_AGE: HIPT(launchI32ToDouble(age, stage, N, st)); break;
+        case CLB_FIELD_PHASE: HIPT(launchI8ToDouble(phase, stage, N, st)); break;
+        default: return {CLB_E_ARG, "unknown field"};
+        }
+        HIPT(hipMemcpyAsync(dst, stage, N * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPT(hipStreamSynchronize(st));
+        return {CLB_OK, ""};
+    }
+    Status writeField(hipStream_t st, int field, int index, const double *src) override {
+        HIPT(hipMemcpyAsync(stage, src, N * sizeof(double), hipMemcpyHostToDevice, st));
+        if (field == CLB_FIELD_CONC) {
+            if (index < 0 || index >= p.n_species) return {CLB_E_ARG, "species index out of range"};
+            if (p.mobile[index]) {
+                if (!adeFirst) return {CLB_E_STATE, "a concentration field can only be written before the first step"};
+                HIPT(launchAdeSet<R>(ade(), slot[index], stage, st));
+            } else HIPT(launchFromDouble<R>(stage, imm + (size_t) slot[index] * N, N, st));
+        } else if (field == CLB_FIELD_BIOMASS) {
+            if (index < 0 || index >= p.n_microbes) return {CLB_E_ARG, "microbe index out of range"};
+            HIPT(launchFromDouble<R>(stage, B + (size_t) index * N, N, st));
+            std::vector<R> tot(N, (R) 0), one(N);
+            std::vector<double> acc(N, 0.0);
+            for (int m = 0; m < p.n_microbes; ++m) {
+                HIPT(hipMemcpyAsync(one.data(), B + (size_t) m * N, N * sizeof(R), hipMemcpyDeviceToHost, st));
+                HIPT(hipStreamSynchronize(st));
+                for (long long i = 0; i < N; ++i) acc[i] += (double) one[i];
+            }
+            for (long long i = 0; i < N; ++i) tot[i] = (R) acc[i];
+            HIPT(hipMemcpyAsync(Btot, tot.data(), N * sizeof(R), hipMemcpyHostToDevice, st));
+        } else if (field == CLB_FIELD_VEL) {
+            if (index < 0 || index > 2) return {CLB_E_ARG, "velocity component out of range"};
+            HIPT(launchFromDouble<R>(stage, u + (size_t) index * N, N, st));
+        } else return {CLB_E_ARG, "field cannot be written"};
+        HIPT(hipStreamSynchronize(st));
+        return {CLB_OK, ""};
+    }
+    void speciesTotals(hipStream_t st, double *out) override {
+        const unsigned int g = nblk(N, 256);
+        for (int s = 0; s < p.n_species; ++s) {
+            HIPT(launchSpeciesTotal<R>(ade(), s, parts, st));
+            HIPT(launchFinalSum(parts, g, 1, 0, 0, scal + s, st));
+        }
+        HIPT(hipMemcpyAsync(out, scal, p.n_species * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPT(hipStreamSynchronize(st));
+    }
+    void counters(hipStream_t st, int64_t out[8]) override {
+        unsigned long long cc = 0; unsigned int u3[4];
+        HIPT(hipMemcpyAsync(&cc, clampCount, sizeof(cc), hipMemcpyDeviceToHost, st));
+        HIPT(hipMemcpyAsync(u3, ucnt, sizeof(u3), hipMemcpyDeviceToHost, st));
+        HIPT(hipStreamSynchronize(st));
+        for (int k = 0; k < 8; ++k) out[k] = cnt[k];
+        out[2] = (int64_t) cc;
+        out[3] = (int64_t) u3[2];
+    }
+    void kernelTime(hipStream_t, int reset, double *ms, int64_t *n) override {
+        if (ms) *ms = klaunch ? kms / (double) klaunch : 0.0;
+        if (n) *n = klaunch;
+        if (reset) { kms = 0; klaunch = 0; }
+    }
+};
+
+}  // namespace
+
+struct clb_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<int, std::unique_ptr<CaseBase>> cases;
+    int next = 1;
+};
+
+namespace {
+int fail(clb_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+CaseBase *getCase(clb_ctx *ctx, int id) {
+    auto it = ctx->cases.find(id);
+    return it == ctx->cases.end() ? nullptr : it->second.get();
+}
+
+#define GUARD_BEGIN try {
+#define GUARD_END                                                                              \
+    }                                                                                          \
+    catch (const HipFail &h) {                                                                 \
+        return fail(ctx, h.e == hipErrorOutOfMemory ? CLB_E_NOMEM : CLB_E_HIP,                 \
+                    h.what + ": " + hipGetErrorString(h.e));                                   \
+    }                                                                                          \
+    catch (const std::exception &ex) { return fail(ctx, CLB_E_HIP, ex.what()); }
+
+#define CASE_OR_FAIL(var)                                                                      \
+    if (!ctx) return CLB_E_ARG;                                                                \
+    CaseBase *var = getCase(ctx, case_id);                                                     \
+    if (!var) return fail(ctx, CLB_E_ARG, "unknown case id");                                  \
+    HIPT(hipSetDevice(ctx->device));
+
+int checkParams(clb_ctx *ctx, const clb_case_params *p, const int8_t *geometry) {
+    if (!p || !geometry) return fail(ctx, CLB_E_ARG, "null parameters or geometry");
+    if (p->nx < 3 || p->ny < 1 || p->nz < 1) return fail(ctx, CLB_E_ARG, "lattice must be at least 3 x 1 x 1");
+    if (p->precision != CLB_FP64 && p->precision != CLB_FP32) return fail(ctx, CLB_E_ARG, "unknown precision");
+    if (p->n_species < 0 || p->n_species > CLB_MAX_SPECIES) return fail(ctx, CLB_E_ARG, "n_species out of range");
+    if (p->n_microbes < 0 || p->n_microbes > CLB_MAX_MICROBES) return fail(ctx, CLB_E_ARG, "n_microbes out of range");
+    if (p->n_rxn < 0 || p->n_rxn > CLB_MAX_RXN) return fail(ctx, CLB_E_ARG, "n_rxn out of range");
+    if (p->eq_on && (p->eq_nc < 0 || p->eq_nc > CLB_MAX_COMP)) return fail(ctx, CLB_E_ARG, "eq_nc out of range");
+    if (p->eq_on && (p->eq_anderson_depth < 1 || p->eq_anderson_depth > 4)) return fail(ctx, CLB_E_ARG, "eq_anderson_depth must be 1..4");
+    for (int s = 0; s < p->n_species; ++s) {
+        if (p->left_btype[s] < 0 || p->left_btype[s] > 2 || p->right_btype[s] < 0 || p->right_btype[s] > 2)
+            return fail(ctx, CLB_E_ARG, "boundary type must be 0, 1 or 2");
+    }
+    if (p->precip_on && (p->precip_species < 0 || p->precip_species >= p->n_species || p->mobile[p->precip_species]))
+        return fail(ctx, CLB_E_ARG, "precip_species must be an immobile species");
+    if (p->dissol_on) return fail(ctx, CLB_E_ARG, "dissolution is not available in this build");
+    if (p->bio_law != CLB_BIO_NONE) {
+        if (p->donor >= p->n_species || p->acceptor >= p->n_species) return fail(ctx, CLB_E_ARG, "donor/acceptor index out of range");
+    }
+    const long long N = (long long) p->nx * p->ny * p->nz;
+    // a no-dynamics grain voxel must not touch open water through a face
+    for (long long i = 0; i < N; ++i) {
+        int g = geometry[i];
+        if (g < 0 || g > CLB_BIOFILM + CLB_MAX_MICROBES + 2) return fail(ctx, CLB_E_GEOMETRY, "unknown material code in geometry");
+    }
+    for (int z = 0; z < p->nz; ++z) for (int y = 0; y < p->ny; ++y) for (int x = 0; x < p->nx; ++x) {
+        long long i = x + (long long) p->nx * (y + (long long) p->ny * z);
+        if (geometry[i] != CLB_SOLID) continue;
+        const int d[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
+        for (int k = 0; k < 6; ++k) {
+            int X = x + d[k][0], Y = y + d[k][1], Z = z + d[k][2];
+            if (X < 0 || Y < 0 || Z < 0 || X >= p->nx || Y >= p->ny || Z >= p->nz) continue;
+            if (geometry[X + (long long) p->nx * (Y + (long long) p->ny * Z)] >= CLB_PORE)
+                return fail(ctx, CLB_E_GEOMETRY, "a no-dynamics (0) voxel touches open water; grains need a bounce-back (1) shell");
+        }
+    }
+    return CLB_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int clb_abi_version(void) { return CLB_ABI_VERSION; }
+
+int clb_ctx_create(int device, clb_ctx **out) {
+    if (!out) return CLB_E_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return CLB_E_HIP;
+    if (device < 0 || device >= n) return CLB_E_ARG;
+    clb_ctx *c = new clb_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) { delete c; return CLB_E_HIP; }
+    *out = c;
+    return CLB_OK;
+}
+int clb_ctx_destroy(clb_ctx *ctx) {
+    if (!ctx) return CLB_E_ARG;
+    (void) hipSetDevice(ctx->device);
+    ctx->cases.clear();
+    if (ctx->stream) (void) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CLB_OK;
+}
+const char *clb_last_error(const clb_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+int clb_device_name(clb_ctx *ctx, char *dst, size_t n) {
+    if (!ctx || !dst || n == 0) return CLB_E_ARG;
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, ctx->device) != hipSuccess) return fail(ctx, CLB_E_HIP, "hipGetDeviceProperties");
+    snprintf(dst, n, "%s (%s, %d CUs)", pr.name, pr.gcnArchName, pr.multiProcessorCount);
+    return CLB_OK;
+}
+int clb_sync(clb_ctx *ctx) {
+    if (!ctx) return CLB_E_ARG;
+    return hipStreamSynchronize(ctx->stream) == hipSuccess ? CLB_OK : fail(ctx, CLB_E_HIP, "hipStreamSynchronize");
+}
+
+size_t clb_case_footprint(const clb_case_params *p) {
+    if (!p) return 0;
+    const size_t N = (size_t) p->nx * p->ny * p->nz;
+    const size_t r = p->precision == CLB_FP32 ? 4 : 8;
+    int sm = 0, si = 0;
+    for (int s = 0; s < p->n_species && s < CLB_MAX_SPECIES; ++s) (p->mobile[s] ? sm : si)++;
+    size_t b = 0;
+    b += N * 9;                                         // mask, geom, dist, phase, classes, reopened
+    b += 2 * (size_t) std::max(sm, 1) * 7 * N * r;      // ADE populations
+    b += (size_t) std::max(si, 1) * N * r;
+    b += (size_t) std::max(p->n_microbes, 1) * N * r + N * r;
+    if (p->n_microbes > 0) b += (size_t) p->n_microbes * 6 * N * r + N;
+    b += 2 * N * sizeof(int);
+    if (p->flow_on) b += 2 * 19 * N * r;
+    b += 3 * N * r;
+    b += N * sizeof(double);                            // staging
+    b += ((N + 127) / 128) * (5 * sizeof(double) + sizeof(long long));
+    return b;
+}
+
+int clb_case_create(clb_ctx *ctx, const clb_case_params *p, const int8_t *geometry, const double *biomass0, int *case_id) {
+    if (!ctx || !case_id) return CLB_E_ARG;
+    int rc = checkParams(ctx, p, geometry);
+    if (rc) return rc;
+    GUARD_BEGIN
+    HIPT(hipSetDevice(ctx->device));
+    std::unique_ptr<CaseBase> c;
+    if (p->precision == CLB_FP32) { auto *k = new Case<float>(); c.reset(k); k->create(*p, geometry, biomass0, ctx->stream); }
+    else { auto *k = new Case<double>(); c.reset(k); k->create(*p, geometry, biomass0, ctx->stream); }
+    int id = ctx->next++;
+    ctx->cases[id] = std::move(c);
+    *case_id = id;
+    return CLB_OK;
+    GUARD_END
+}
+int clb_case_destroy(clb_ctx *ctx, int case_id) {
+    if (!ctx) return CLB_E_ARG;
+    auto it = ctx->cases.find(case_id);
+    if (it == ctx->cases.end()) return fail(ctx, CLB_E_ARG, "unknown case id");
+    (void) hipSetDevice(ctx->device);
+    ctx->cases.erase(it);
+    return CLB_OK;
+}
+
+int clb_set_ade_relaxation(clb_ctx *ctx, int case_id, const double *omp, const double *omb, double dt) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (!omp || !omb) return fail(ctx, CLB_E_ARG, "null relaxation arrays");
+    c->setRelaxation(omp, omb, dt);
+    return CLB_OK;
+    GUARD_END
+}
+
+int clb_ns_init(clb_ctx *ctx, int case_id, double delta_p) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (!c->p.flow_on) return fail(ctx, CLB_E_STATE, "case was created with flow_on = 0");
+    c->nsInit(ctx->stream, delta_p);
+    return CLB_OK;
+    GUARD_END
+}
+int clb_ns_solve(clb_ctx *ctx, int case_id, int max_iter, int rule, double eps, int reset_tracer, int *iters, int *converged) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (!c->p.flow_on) return fail(ctx, CLB_E_STATE, "case was created with flow_on = 0");
+    if (max_iter < 0 || (rule != CLB_CONV_TRACER && rule != CLB_CONV_RELDELTA)) return fail(ctx, CLB_E_ARG, "bad solve arguments");
+    Status s = c->nsSolve(ctx->stream, max_iter, rule, eps, reset_tracer, iters, converged);
+    return s.code ? fail(ctx, s.code, s.msg) : CLB_OK;
+    GUARD_END
+}
+int clb_ns_couple(clb_ctx *ctx, int case_id) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    c->nsCouple(ctx->stream);
+    HIPT(hipStreamSynchronize(ctx->stream));
+    return CLB_OK;
+    GUARD_END
+}
+int clb_ns_stats(clb_ctx *ctx, int case_id, double out[4]) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (!out) return fail(ctx, CLB_E_ARG, "null output");
+    c->nsStats(ctx->stream, out);
+    return CLB_OK;
+    GUARD_END
+}
+int clb_ns_update_dynamics(clb_ctx *ctx, int case_id, double perm_ratio) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    c->nsUpdateDynamics(ctx->stream, perm_ratio);
+    return CLB_OK;
+    GUARD_END
+}
+int clb_ade_stabilize(clb_ctx *ctx, int case_id, int steps) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (steps < 0) return fail(ctx, CLB_E_ARG, "negative step count");
+    c->adeStabilize(ctx->stream, steps);
+    return CLB_OK;
+    GUARD_END
+}
+int clb_run_steps(clb_ctx *ctx, int case_id, int nsteps, int *done) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (nsteps < 0) return fail(ctx, CLB_E_ARG, "negative step count");
+    Status s = c->runSteps(ctx->stream, nsteps, done);
+    return s.code ? fail(ctx, s.code, s.msg) : CLB_OK;
+    GUARD_END
+}
+int clb_ade_steps(clb_ctx *ctx, int case_id, int nsteps) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (nsteps < 0) return fail(ctx, CLB_E_ARG, "negative step count");
+    c->adeSteps(ctx->stream, nsteps);
+    return CLB_OK;
+    GUARD_END
+}
+int clb_equilibrium(clb_ctx *ctx, int case_id) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    c->equilibrium(ctx->stream);
+    HIPT(hipStreamSynchronize(ctx->stream));
+    return CLB_OK;
+    GUARD_END
+}
+int clb_ca_spill(clb_ctx *ctx, int case_id, int *sweeps) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    Status s = c->caBlock(ctx->stream, sweeps);
+    return s.code ? fail(ctx, s.code, s.msg) : CLB_OK;
+    GUARD_END
+}
+int clb_read_field(clb_ctx *ctx, int case_id, int field, int index, double *dst) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (!dst) return fail(ctx, CLB_E_ARG, "null destination");
+    Status s = c->readField(ctx->stream, field, index, dst);
+    return s.code ? fail(ctx, s.code, s.msg) : CLB_OK;
+    GUARD_END
+}
+int clb_write_field(clb_ctx *ctx, int case_id, int field, int index, const double *src) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (!src) return fail(ctx, CLB_E_ARG, "null source");
+    Status s = c->writeField(ctx->stream, field, index, src);
+    return s.code ? fail(ctx, s.code, s.msg) : CLB_OK;
+    GUARD_END
+}
+int clb_species_totals(clb_ctx *ctx, int case_id, double *out) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (!out) return fail(ctx, CLB_E_ARG, "null output");
+    c->speciesTotals(ctx->stream, out);
+    return CLB_OK;
+    GUARD_END
+}
+int clb_counters(clb_ctx *ctx, int case_id, int64_t out[8]) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (!out) return fail(ctx, CLB_E_ARG, "null output");
+    c->counters(ctx->stream, out);
+    return CLB_OK;
+    GUARD_END
+}
+int clb_kernel_time(clb_ctx *ctx, int case_id, int reset, double *avg_ms, int64_t *launches) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    c->kernelTime(ctx->stream, reset, avg_ms, launches);
+    return CLB_OK;
+    GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,276 @@
+/*
+ * clb.h -- C-ABI of the MI355X-native coupled reaction step.
+ *
+ * One library (libclb.so, HIP for gfx950) owns the device memory of every case
+ * and runs the coupled step of the CompLaB3D main loop on it:
+ *   D3Q19 incompressible BGK flow with bounce-back on the live mask,
+ *   one D3Q7 advection-diffusion lattice per mobile species,
+ *   per-voxel kinetics with the clamp dC = max(R*dt, -C),
+ *   per-voxel equilibrium speciation,
+ *   the biofilm cellular-automaton spill,
+ *   precipitation / dissolution mask updates.
+ *
+ * Plain C types only: ints, doubles, pointers, sizes.  Every entry point
+ * returns 0 on success and a negative CLB_E_* code on failure; the message of
+ * the last failure on a context is clb_last_error(ctx).  Nothing aborts the
+ * process.  Calls on one context are not thread-safe; use one context per
+ * host thread.
+ *
+ * Each declaration cites the reference interface it replaces (paths relative
+ * to the CompLaB3D-GEMS source tree).
+ */
+#ifndef CLB_H
+#define CLB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLB_ABI_VERSION 1
+
+#define CLB_MAX_SPECIES   8   /* substrates (mobile + immobile)              */
+#define CLB_MAX_MICROBES  2   /* CA biofilm populations                      */
+#define CLB_MAX_RXN       4   /* abiotic mass-action reactions               */
+#define CLB_MAX_COMP      8   /* equilibrium components                      */
+
+/* status codes */
+#define CLB_OK              0
+#define CLB_E_ARG          -1   /* bad argument / inconsistent parameters    */
+#define CLB_E_HIP          -2   /* HIP runtime failure                       */
+#define CLB_E_NOMEM        -3   /* device allocation failed                  */
+#define CLB_E_STATE        -4   /* call not valid in the case's state        */
+#define CLB_E_NONFINITE    -5   /* NaN/Inf detected (biomass or flow energy) */
+#define CLB_E_CA_STALL     -6   /* CA spill made no progress in 100 sweeps   */
+#define CLB_E_CA_SWEEPS    -7   /* CA spill did not settle in 1000 sweeps    */
+#define CLB_E_AGE_SWEEPS   -8   /* age relaxation did not settle             */
+#define CLB_E_GEOMETRY     -9   /* geometry does not percolate / invalid     */
+
+/* material codes, as in complab3d_geometry.hh:61 and ABC make_geometries.py */
+#define CLB_SOLID   0   /* no-dynamics grain interior                         */
+#define CLB_WALL    1   /* bounce-back                                        */
+#define CLB_PORE    2   /* open pore water                                    */
+#define CLB_BIOFILM 3   /* first CA microbe's material number (3 + m)         */
+
+/* precisions */
+#define CLB_FP64 0
+#define CLB_FP32 1
+
+/* boundary types of the x planes, complab_functions.hh:382-392 */
+#define CLB_BC_DIRICHLET 0
+#define CLB_BC_NEUMANN   1
+#define CLB_BC_CLOSED    2
+
+/* flow convergence rules */
+#define CLB_CONV_TRACER   0  /* Palabos ValueTracer(1,1000,eps) on the mean energy */
+#define CLB_CONV_RELDELTA 1  /* |E_t - E_{t-1}| / E_t < eps                          */
+
+/* biotic rate laws (defineRxnKinetics, config/kinetics/defineKinetics.biotic.hh:90-126) */
+#define CLB_BIO_NONE       0
+#define CLB_BIO_DUAL_MONOD 1  /* mu = mu_max S/(Ks+S) * E/(Ke+E); E term off if acceptor<0 */
+
+/* CA rules, complab_functions.hh:1414-1439 */
+#define CLB_CA_FRACTION 0   /* pushExcessBiomass3D      */
+#define CLB_CA_HALF     1   /* halfPushExcessBiomass3D  */
+
+/* equilibrium solvers */
+#define CLB_EQ_AAPCF  0   /* Anderson-accelerated PCF, part4_eqsolver.hh:960-1180 */
+#define CLB_EQ_NEWTON 1   /* Newton on log-components (NR_reserve.hh:512)         */
+
+typedef struct clb_case_params {
+    /* lattice, including the two ghost x columns (complab_functions.hh:999) */
+    int32_t nx, ny, nz;
+    int32_t precision;                 /* CLB_FP64 / CLB_FP32                 */
+
+    /* ---- species (substrates) -------------------------------------------- */
+    int32_t n_species;
+    int32_t mobile[CLB_MAX_SPECIES];   /* 0 = immobile scalar (never streams) */
+    double  c0[CLB_MAX_SPECIES];       /* initial concentration               */
+    double  omega_pore[CLB_MAX_SPECIES];
+    double  omega_biofilm[CLB_MAX_SPECIES];
+    int32_t left_btype[CLB_MAX_SPECIES], right_btype[CLB_MAX_SPECIES];
+    double  left_bc[CLB_MAX_SPECIES],    right_bc[CLB_MAX_SPECIES];
+    double  dt;                        /* ade_dt, seconds per step            */
+    int32_t clamp;                     /* 1: dC = max(R dt, -C) on every path */
+
+    /* ---- biotic kinetics ------------------------------------------------- */
+    int32_t bio_law;                   /* CLB_BIO_*                           */
+    int32_t donor, acceptor;           /* species indices (-1 = unused)       */
+    double  mu_max[CLB_MAX_MICROBES], ks[CLB_MAX_MICROBES], ke[CLB_MAX_MICROBES];
+    double  kdecay[CLB_MAX_MICROBES];
+    double  bio_stoich[CLB_MAX_MICROBES][CLB_MAX_SPECIES]; /* dC_s = nu * growth */
+
+    /* ---- abiotic mass-action kinetics (defineAbioticRxnKinetics) ---------- */
+    int32_t n_rxn;
+    int32_t rxn_r1[CLB_MAX_RXN], rxn_r2[CLB_MAX_RXN];      /* reactants (-1 none) */
+    double  rxn_k[CLB_MAX_RXN];                            /* 1/s or L/(mol s)    */
+    double  rxn_stoich[CLB_MAX_RXN][CLB_MAX_SPECIES];
+    double  rxn_cap_frac, rxn_cap_dt, rxn_min_conc;        /* ABC per-step cap    */
+    double  abiotic_rate_scale;                            /* g_abioticRateScale  */
+    int32_t surface_only;                                  /* precipitationVOP.hh:53 */
+
+    /* ---- CA biofilm ------------------------------------------------------ */
+    int32_t n_microbes;                /* CA biofilm populations              */
+    double  b0[CLB_MAX_MICROBES];      /* density written on seed voxels      */
+    double  bmax;                      /* maximum_biomass_density             */
+    double  thrd_bfilm_frac;           /* thrd_biofilm_fraction               */
+    int32_t ca_method;                 /* CLB_CA_*                            */
+    uint64_t ca_seed;                  /* counter-based RNG for the spill     */
+    int32_t ade_update_interval;       /* solute omega switch interval        */
+
+    /* ---- flow ------------------------------------------------------------ */
+    double  ns_tau;                    /* tau of the D3Q19 lattice            */
+    double  delta_p;                   /* pressure drop (lattice)             */
+    double  perm_ratio;                /* 1/viscosity_ratio in biofilm; <=1e-14 -> BB */
+    double  precip_perm_ratio;
+    int32_t ns_update_interval;        /* re-solve interval (steps)           */
+    int32_t ns_max_iter;               /* ns_max_iT2 cap per re-solve         */
+    int32_t ns_conv_rule;              /* CLB_CONV_*                          */
+    double  ns_conv_eps;
+    int32_t flow_on;                   /* 0: Pe = 0, velocity stays zero      */
+
+    /* ---- equilibrium ----------------------------------------------------- */
+    int32_t eq_on;
+    int32_t eq_solver;                 /* CLB_EQ_*                            */
+    int32_t eq_nc;                     /* components                          */
+    int32_t eq_comp_species[CLB_MAX_COMP];  /* species holding each component's guess (-1 none) */
+    double  eq_stoich[CLB_MAX_SPECIES][CLB_MAX_COMP];
+    double  eq_logk[CLB_MAX_SPECIES];
+    double  eq_tol;
+    int32_t eq_max_iter;
+    int32_t eq_anderson_depth;
+
+    /* ---- precipitation (precipitationVOP.hh) ---------------------------- */
+    int32_t precip_on;
+    int32_t precip_species;            /* immobile mineral species            */
+    double  precip_max_rho;
+    int32_t precip_interval;
+    int32_t precip_phase_id;
+
+    /* ---- dissolution (dissolutionVOP.hh) -------------------------------- */
+    int32_t dissol_on;
+    int32_t dissol_species;            /* immobile mineral species            */
+    double  dissol_k;                  /* rate constant, 1/s (first order in mineral) */
+    int32_t dissol_product;            /* released species                    */
+    double  dissol_full;               /* full_density                        */
+    double  dissol_reopen_frac;        /* reopen_fraction                     */
+    int32_t dissol_interval;
+    int32_t dissol_phase_id;
+} clb_case_params;
+
+typedef struct clb_ctx clb_ctx;
+
+/* ---- context ------------------------------------------------------------- */
+/* plbInit (complab.cpp:261) */
+int  clb_ctx_create(int device, clb_ctx **out);
+int  clb_ctx_destroy(clb_ctx *ctx);
+const char *clb_last_error(const clb_ctx *ctx);
+int  clb_abi_version(void);
+/* fills dst with a short text naming the device; for reports */
+int  clb_device_name(clb_ctx *ctx, char *dst, size_t n);
+int  clb_sync(clb_ctx *ctx);
+
+/* ---- cases --------------------------------------------------------------- */
+/* the lattices built in complab.cpp:1380-1711 (soluteDomainSetup,
+ * bmassDomainSetup, NSdomainSetup, CopyGeometryScalar2maskLattice3D).
+ * geometry: nx*ny*nz material codes, x fastest (ghost columns included).
+ * biomass0: n_microbes*nx*ny*nz initial biofilm densities or NULL. */
+int  clb_case_create(clb_ctx *ctx, const clb_case_params *p, const int8_t *geometry,
+                     const double *biomass0, int *case_id);
+int  clb_case_destroy(clb_ctx *ctx, int case_id);
+/* device bytes held by a case built from these parameters (admission control) */
+size_t clb_case_footprint(const clb_case_params *p);
+
+/* ---- flow ---------------------------------------------------------------- */
+/* NSdomainSetup(...) + initializeAtEquilibrium(PressureGradient)
+ * (complab_functions.hh:82-94, 296-345) with the current delta_p */
+int  clb_ns_init(clb_ctx *ctx, int case_id, double delta_p);
+/* collideAndStream loop with a ValueTracer (complab.cpp:1111-1124); the
+ * tracer keeps its window across calls unless reset != 0 (ns_convg2 at
+ * complab.cpp:3090-3093 is never reset).  *iters = iterations run,
+ * *converged = 1 if the rule was met before max_iter. */
+int  clb_ns_solve(clb_ctx *ctx, int case_id, int max_iter, int rule, double eps,
+                  int reset_tracer, int *iters, int *converged);
+/* latticeToPassiveAdvDiff (complab.cpp:2012-2032): velocity of the current
+ * flow state copied into the field every ADE lattice reads */
+int  clb_ns_couple(clb_ctx *ctx, int case_id);
+/* reductions of complab.cpp:1136-1157: box-mean |u| over x=1..nx-2, max u_x,
+ * outlet-plane mean u_x at x=nx-2, and the pore+biofilm mean |u| */
+int  clb_ns_stats(clb_ctx *ctx, int case_id, double out[4]);
+/* updateNsLatticesDynamics3D (part2:748-818) from the live mask */
+int  clb_ns_update_dynamics(clb_ctx *ctx, int case_id, double perm_ratio);
+
+/* ---- advection-diffusion + reactions -------------------------------------- */
+/* the relaxation rates and time step of the ADE lattices after the Peclet
+ * mapping of complab.cpp:1220-1283 (omega per species in pore / biofilm) */
+int  clb_set_ade_relaxation(clb_ctx *ctx, int case_id, const double *omega_pore,
+                            const double *omega_biofilm, double dt);
+/* the 10 000-step stabilization pre-loop and stabilizeADElattice3D reset of
+ * pore/biofilm voxels to c0 (complab.cpp:2033-2041; part3:261-310) */
+int  clb_ade_stabilize(clb_ctx *ctx, int case_id, int steps);
+/* nsteps coupled reaction steps of the main loop (complab.cpp:2178-3262):
+ * collide, biotic kinetics + apply, abiotic kinetics (+ dissolution) + apply,
+ * equilibrium, CA spill, mask-driven dynamics updates and flow re-solves,
+ * precipitation / dissolution node conversion, stream.  *done = steps run
+ * (fewer when a stop condition is met; the status says which). */
+int  clb_run_steps(clb_ctx *ctx, int case_id, int nsteps, int *done);
+/* only the fused transport + kinetics kernel, nsteps times (no CA, no
+ * equilibrium, no conversions); the timed unit of the roofline */
+int  clb_ade_steps(clb_ctx *ctx, int case_id, int nsteps);
+/* run_equilibrium_biotic (part4:1359-1560) once over the reaction box */
+int  clb_equilibrium(clb_ctx *ctx, int case_id);
+/* the CA block of complab.cpp:2793-3055 once; *sweeps = push/pull sweeps */
+int  clb_ca_spill(clb_ctx *ctx, int case_id, int *sweeps);
+
+/* ---- fields ---------------------------------------------------------------- */
+#define CLB_FIELD_CONC     0  /* species concentration (density after stream) */
+#define CLB_FIELD_BIOMASS  1  /* CA biomass of microbe `index`                 */
+#define CLB_FIELD_BTOTAL   2  /* total biofilm biomass                          */
+#define CLB_FIELD_MASK     3  /* live material code (as double)                 */
+#define CLB_FIELD_VEL      4  /* velocity component `index`                     */
+#define CLB_FIELD_AGE      5  /* CA age                                         */
+#define CLB_FIELD_PHASE    6  /* solid phase id                                 */
+#define CLB_FIELD_NSRHO    7  /* flow density                                   */
+/* copies nx*ny*nz doubles (x fastest) of one field into dst */
+int  clb_read_field(clb_ctx *ctx, int case_id, int field, int index, double *dst);
+/* overwrites a species concentration field with equilibrium populations at
+ * rest (or a biomass field); for tests and restarts */
+int  clb_write_field(clb_ctx *ctx, int case_id, int field, int index, const double *src);
+
+/* ---- reductions (PopulationSum3D, MaskedScalarCounts3D, computeSum) --------- */
+/* per species: sum of concentration over open voxels (mask not solid/wall)
+ * of x = 1..nx-2 on the live mask; out has n_species entries */
+int  clb_species_totals(clb_ctx *ctx, int case_id, double *out);
+/* counters: [0] open voxels, [1] biofilm voxels, [2] clamp binds,
+ * [3] equilibrium non-converged voxels, [4] flow re-solves, [5] mask changes,
+ * [6] CA sweeps, [7] steps run */
+int  clb_counters(clb_ctx *ctx, int case_id, int64_t out[8]);
+
+/* ---- profiling ------------------------------------------------------------- */
+/* average device time (ms) of the fused transport kernel over the launches
+ * since the last reset, measured with HIP events on the case's stream */
+int  clb_kernel_time(clb_ctx *ctx, int case_id, int reset, double *avg_ms, int64_t *launches);
+
+/* ---- geometry (complab3d_geometry.hh) -------------------------------------- */
+/* Rng of complab3d_geometry.hh:77-86: next `n` raw 64-bit draws from seed */
+int  clb_rng_draws(uint64_t seed, int n, uint64_t *out);
+/* complab_geom::generate "spheres" with `count` spheres (x fastest, no ghosts) */
+int  clb_geom_spheres(int nx, int ny, int nz, double radius, int count, uint64_t seed,
+                      int walls_yz, int8_t *out);
+/* sphere pack inserted until porosity <= target, grain shell -> wall, y/z
+ * walls, isolated pockets -> wall, percolation checked; writes the lattice
+ * with ghost columns (nx_file+2) * ny * nz.  *porosity = open fraction of the
+ * file box.  Returns CLB_E_GEOMETRY when the pore space does not span x. */
+int  clb_geom_pack(int nx_file, int ny, int nz, double radius, double porosity,
+                   uint64_t seed, int kind, int8_t *out_lattice, double *porosity_out);
+#define CLB_GEOM_SPHERES   0
+#define CLB_GEOM_RANDOM    1  /* smoothed random field thresholded at the quantile */
+#define CLB_GEOM_CYLINDERS 2
+#define CLB_GEOM_FRACTURE  3
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLB_H */

@@ -1,0 +1,231 @@
+"""HIP path against the CPU oracle on the same seeded inputs (FP64: 1e-10
+relative on fields, bit-exact masks and CA topology)."""
+import numpy as np
+import pytest
+
+import complab_amd as clb
+from complab_amd import configs, driver
+from complab_amd import params as P
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+def relerr(a, b, mask=None):
+    if mask is not None:
+        a = a[mask]; b = b[mask]
+    scale = max(np.max(np.abs(b)), 1e-300)
+    return float(np.max(np.abs(a - b)) / scale)
+
+
+def open_mask(case):
+    m = case.read(P.FIELD_MASK)
+    return (m >= 2)
+
+
+def both(ctx, w):
+    g = clb.Case(ctx, w.params, w.geometry)
+    o = O.OracleCase(w.params, w.geometry)
+    return g, o
+
+
+def test_geometry_matches_oracle():
+    w1 = configs.config1(scale=0.25)
+    w2 = configs.config1(scale=0.25, packer=O.geom_pack)
+    assert np.array_equal(w1.geometry, w2.geometry)
+
+
+def test_flow_and_transport_fp64(ctx):
+    w = configs.config1(scale=0.25)
+    w.transport.stabilize_steps = 50
+    g, o = both(ctx, w)
+    rg = driver.setup(g, w.params, w.flow, w.transport)
+    ro = driver.setup(o, w.params, w.flow, w.transport)
+    print("flow iters gpu", rg.iters, "oracle", ro.iters)
+    assert rg.iters == ro.iters
+    assert abs(rg.delta_p - ro.delta_p) <= 1e-10 * abs(ro.delta_p)
+    for k in range(3):
+        assert relerr(g.read(P.FIELD_VEL, k), o.read(P.FIELD_VEL, k)) <= REL
+    for c in (g, o):
+        c.set_omegas(ro.omega_pore, ro.omega_biofilm, 1.0)
+    rc, n = g.run_steps(150)
+    assert rc == 0 and n == 150
+    rc, n = o.run_steps(150)
+    assert rc == 0 and n == 150
+    om = open_mask(o)
+    for s in range(2):
+        e = relerr(g.read(P.FIELD_CONC, s), o.read(P.FIELD_CONC, s), om)
+        print("species", s, "rel err", e)
+        assert e <= REL
+    tg, to = g.species_totals(), o.species_totals()
+    assert np.allclose(tg, to, rtol=1e-10, atol=0)
+
+
+def test_transport_only_steps(ctx):
+    w = configs.config1(scale=0.25)
+    g, o = both(ctx, w)
+    rng = np.random.default_rng(3)
+    u = rng.uniform(-0.02, 0.02, size=(3,) + w.geometry.shape)
+    closed = (w.geometry == P.SOLID) | (w.geometry == P.WALL)
+    for k in range(3):
+        u[k][closed] = 0.0
+        g.write(P.FIELD_VEL, k, u[k]); o.write(P.FIELD_VEL, k, u[k])
+    c0 = rng.uniform(0.0, 1.0, size=w.geometry.shape)
+    for c in (g, o):
+        c.set_omegas([1.3, 1.1], [1.3, 1.1], 1.0)
+        c.write(P.FIELD_CONC, 0, c0)
+    g.ade_steps(40); o.ade_steps(40)
+    om = open_mask(o)
+    for s in range(2):
+        assert relerr(g.read(P.FIELD_CONC, s), o.read(P.FIELD_CONC, s), om) <= REL
+
+
+def test_biotic_coupled_fp64(ctx):
+    w = configs.config2(scale=0.12)
+    w.transport.stabilize_steps = 30
+    w.params.ns_update_interval = 5
+    w.params.ns_max_iter = 3000
+    w.params.b0[0] = 95.0              # seeds near the cap and fast growth, so the
+    w.params.mu_max[0] = 2e-2          # spill and the mask-driven re-solves run here
+    g, o = both(ctx, w)
+    ro = driver.setup(o, w.params, w.flow, w.transport)
+    rg = driver.setup(g, w.params, w.flow, w.transport)
+    assert rg.iters == ro.iters
+    for c in (g, o):
+        c.set_omegas(ro.omega_pore, ro.omega_biofilm, 1.0)
+    # grow the biofilm fast so the spill runs within the test
+    rc_g, ng = g.run_steps(60)
+    rc_o, no = o.run_steps(60)
+    assert rc_g == rc_o and ng == no
+    mg, mo = g.read(P.FIELD_MASK), o.read(P.FIELD_MASK)
+    assert np.array_equal(mg, mo), "mask topology differs"
+    assert np.array_equal(g.read(P.FIELD_AGE), o.read(P.FIELD_AGE))
+    om = mo >= 2
+    assert relerr(g.read(P.FIELD_BIOMASS, 0), o.read(P.FIELD_BIOMASS, 0)) <= REL
+    for s in range(4):
+        assert relerr(g.read(P.FIELD_CONC, s), o.read(P.FIELD_CONC, s), om) <= REL
+    cg, co = g.counters(), o.counters()
+    print("counters gpu", cg, "oracle", co)
+    assert cg[4] == co[4] and cg[5] == co[5] and cg[6] == co[6]
+
+
+def test_ca_spill_invariants_and_parity(ctx):
+    w = configs.config2(scale=0.1)
+    p = w.params
+    p.flow_on = 0
+    g, o = both(ctx, w)
+    rng = np.random.default_rng(5)
+    b = np.zeros(w.geometry.shape)
+    openv = w.geometry >= 2
+    openv[:, :, 0] = False; openv[:, :, -1] = False
+    b[openv] = rng.uniform(0, 60, size=openv.sum())
+    hot = np.flatnonzero(openv.reshape(-1))[:: 7]
+    b.reshape(-1)[hot] = rng.uniform(150, 400, size=hot.size)
+    for c in (g, o):
+        c.write(P.FIELD_BIOMASS, 0, b)
+    rg, sg = g.ca_spill()
+    ro, so = o.ca_spill()
+    assert rg == ro == 0 and sg == so
+    bg, bo = g.read(P.FIELD_BIOMASS, 0), o.read(P.FIELD_BIOMASS, 0)
+    assert relerr(bg, bo) <= REL
+    assert np.array_equal(g.read(P.FIELD_MASK), o.read(P.FIELD_MASK))
+    assert abs(bg.sum() - b.sum()) <= 1e-9 * b.sum()          # mass conserved
+    assert bg.max() <= p.bmax + 1e-9
+    assert np.all(bg[~(w.geometry >= 2)] == 0)
+
+
+def eq_params(p, solver):
+    """examples/04_equilibrium/CompLaB.xml: components HCO3, H; species
+    HCO3, H, CO3 (= HCO3 - H, logK -10.33), H2CO3 (= HCO3 + H, logK 6.35)."""
+    p.eq_on = 1
+    p.eq_solver = solver
+    p.eq_nc = 2
+    S = [[1, 0], [0, 1], [1, -1], [1, 1]]
+    lk = [0.0, 0.0, -10.33, 6.35]
+    for i in range(4):
+        for j in range(2):
+            p.eq_stoich[i][j] = S[i][j]
+        p.eq_logk[i] = lk[i]
+    p.eq_comp_species[0] = 0
+    p.eq_comp_species[1] = 1
+    p.eq_tol = 1e-12 if solver == P.EQ_NEWTON else 1e-8
+    p.eq_max_iter = 200
+
+
+@pytest.mark.parametrize("solver,tol", [(P.EQ_NEWTON, 1e-10), (P.EQ_AAPCF, 1e-6)])
+def test_equilibrium_parity(ctx, solver, tol):
+    w = configs.config1(scale=0.25)
+    p = w.params
+    p.n_species = 4
+    for s in range(4):
+        p.mobile[s] = 1
+        p.left_btype[s] = P.BC_NEUMANN; p.right_btype[s] = P.BC_NEUMANN
+        p.c0[s] = 0.0
+    p.n_rxn = 0
+    p.flow_on = 0
+    eq_params(p, solver)
+    g, o = both(ctx, w)
+    rng = np.random.default_rng(11)
+    for s in range(4):
+        f = 10 ** rng.uniform(-6, -2, size=w.geometry.shape)
+        g.write(P.FIELD_CONC, s, f); o.write(P.FIELD_CONC, s, f)
+    for c in (g, o):
+        c.set_omegas([1.2] * 4, [1.2] * 4, 1.0)
+    for _ in range(3):
+        g.ade_steps(1); o.ade_steps(1)
+        g.equilibrium(); o.equilibrium()
+    om = open_mask(o)
+    for s in range(4):
+        e = relerr(g.read(P.FIELD_CONC, s), o.read(P.FIELD_CONC, s), om)
+        print("eq species", s, e)
+        assert e <= tol
+
+
+def test_precipitation_parity(ctx):
+    w = configs.config1(scale=0.3)
+    p = w.params
+    p.n_species = 3
+    p.mobile[2] = 0
+    p.c0[0] = 0.2; p.c0[1] = 0.2; p.c0[2] = 0.0
+    for s in range(2):
+        p.left_btype[s] = P.BC_DIRICHLET; p.left_bc[s] = 0.5; p.right_btype[s] = P.BC_NEUMANN
+    p.n_rxn = 1
+    p.rxn_r1[0] = 0; p.rxn_r2[0] = 1; p.rxn_k[0] = 0.5
+    p.rxn_stoich[0][0] = -1; p.rxn_stoich[0][1] = -1; p.rxn_stoich[0][2] = 1
+    p.surface_only = 1
+    p.precip_on = 1; p.precip_species = 2; p.precip_max_rho = 0.05; p.precip_interval = 5
+    p.flow_on = 0
+    g, o = both(ctx, w)
+    for c in (g, o):
+        c.set_omegas([1.4, 1.4, 1.0], [1.4, 1.4, 1.0], 1.0)
+    rg, ng = g.run_steps(80)
+    ro, no = o.run_steps(80)
+    assert rg == ro == 0
+    mg, mo = g.read(P.FIELD_MASK), o.read(P.FIELD_MASK)
+    assert np.array_equal(mg, mo)
+    assert (mo == P.SOLID).sum() > (w.geometry == P.SOLID).sum()      # something precipitated
+    om = mo >= 2
+    for s in range(2):
+        assert relerr(g.read(P.FIELD_CONC, s), o.read(P.FIELD_CONC, s), om) <= REL
+    assert relerr(g.read(P.FIELD_CONC, 2), o.read(P.FIELD_CONC, 2)) <= REL
+
+
+def test_fp32_close_to_fp64(ctx):
+    w = configs.config1(scale=0.25)
+    w.transport.stabilize_steps = 20
+    w32 = configs.config1(scale=0.25, precision=P.FP32)
+    g64 = clb.Case(ctx, w.params, w.geometry)
+    g32 = clb.Case(ctx, w32.params, w32.geometry)
+    r64 = driver.setup(g64, w.params, w.flow, w.transport)
+    w32.transport.stabilize_steps = 20
+    r32 = driver.setup(g32, w32.params, w32.flow, w32.transport)
+    for c, r in ((g64, r64), (g32, r32)):
+        c.set_omegas(r.omega_pore, r.omega_biofilm, 1.0)
+        assert c.run_steps(100)[0] == 0
+    om = open_mask(g64)
+    for s in range(2):
+        e = relerr(g32.read(P.FIELD_CONC, s), g64.read(P.FIELD_CONC, s), om)
+        print("fp32 species", s, e)
+        assert e < 1e-3

@@ -1,0 +1,271 @@
+"""The oracle's restatement of the Palabos pieces (BGK advection-diffusion with
+cs^2 = 1/4, bounce-back, regularized planes, incompressible BGK flow, the
+pressure planes) and of the reference processors, pinned by closed forms and
+invariants.  Each check is chosen so a plausible mistake fails it: cs^2 = 1/3
+instead of 1/4 misses the diffusion decay by 33 %, a pressure plane without
+the 1/cs^2 factor misses Poiseuille by 3x, a lost deposit weight breaks the
+exact first-order decay, a skipped clamp drives a concentration negative."""
+import numpy as np
+import pytest
+
+import oracle as O
+from complab_amd import configs, driver
+from complab_amd import params as P
+
+
+def channel_params(nx, species=1):
+    p = P.default_params(nx, 1, 1, species)
+    for s in range(species):
+        p.left_btype[s] = P.BC_CLOSED
+        p.right_btype[s] = P.BC_CLOSED
+        p.omega_pore[s] = 1.0
+        p.omega_biofilm[s] = 1.0
+    return p
+
+
+def test_diffusion_mode_decay_cs2_quarter():
+    nx = 66                                     # fluid x = 1..64, closed planes at 0 and 65
+    p = channel_params(nx)
+    geom = np.full((1, 1, nx), P.PORE, np.int8)
+    o = O.OracleCase(p, geom)
+    L = nx - 2
+    x = np.arange(nx)
+    k = np.pi / L
+    c = 1.0 + 0.1 * np.cos(k * (x - 0.5))
+    o.write(P.FIELD_CONC, 0, c.reshape(1, 1, nx))
+    T = 2000
+    o.ade_steps(T)
+    r = o.read(P.FIELD_CONC, 0).reshape(-1)[1:-1]
+    xs = x[1:-1]
+    amp = 2.0 / L * np.sum((r - r.mean()) * np.cos(k * (xs - 0.5)))
+    D = 0.25 * (1.0 / 1.0 - 0.5)              # cs^2 (tau - 1/2), cs^2 = 1/4
+    expect = 0.1 * np.exp(-D * k * k * T)
+    assert abs(amp / expect - 1) < 5e-3
+    assert abs(r.mean() - 1.0) < 1e-12          # closed box keeps its mass
+
+
+def test_closed_box_conserves_mass_with_flow():
+    w = configs.config1(scale=0.25, packer=O.geom_pack)
+    p = w.params
+    for s in range(2):
+        p.left_btype[s] = P.BC_CLOSED; p.right_btype[s] = P.BC_CLOSED
+    p.n_rxn = 0
+    o = O.OracleCase(p, w.geometry)
+    rng = np.random.default_rng(0)
+    closed = w.geometry < 2
+    for a in range(3):
+        u = rng.uniform(-0.03, 0.03, w.geometry.shape)
+        u[closed] = 0
+        o.write(P.FIELD_VEL, a, u)
+    c = rng.uniform(0, 2, w.geometry.shape)
+    o.write(P.FIELD_CONC, 0, c)
+    o.set_omegas([1.5, 1.5], [1.5, 1.5], 1.0)
+    tot0 = o.read(P.FIELD_CONC, 0).sum()        # every cell, bounce-back links included
+    o.ade_steps(300)
+    tot1 = o.read(P.FIELD_CONC, 0).sum()
+    assert abs(tot1 - tot0) <= 1e-12 * tot0
+
+
+def test_dirichlet_planes_give_linear_steady_profile():
+    nx = 34
+    p = channel_params(nx)
+    p.left_btype[0] = P.BC_DIRICHLET; p.left_bc[0] = 1.0
+    p.right_btype[0] = P.BC_DIRICHLET; p.right_bc[0] = 0.0
+    geom = np.full((1, 1, nx), P.PORE, np.int8)
+    o = O.OracleCase(p, geom)
+    o.ade_steps(30000)
+    r = o.read(P.FIELD_CONC, 0).reshape(-1)
+    xs = np.arange(1, nx - 1)
+    a, b = np.polyfit(xs, r[1:-1], 1)
+    assert np.max(np.abs(np.polyval([a, b], xs) - r[1:-1])) < 1e-9
+    # node-based planes: the line passes through 1 at x = 0 and 0 at x = nx-1
+    assert abs(b - 1.0) < 1e-6 and abs(a * (nx - 1) + b) < 1e-6
+
+
+def test_neumann_outlet_copies_inner_density():
+    nx = 20
+    p = channel_params(nx)
+    p.left_btype[0] = P.BC_DIRICHLET; p.left_bc[0] = 1.0
+    p.right_btype[0] = P.BC_NEUMANN
+    geom = np.full((1, 1, nx), P.PORE, np.int8)
+    o = O.OracleCase(p, geom)
+    o.ade_steps(40000)
+    r = o.read(P.FIELD_CONC, 0).reshape(-1)
+    # no flow, zero-gradient outlet, constant inlet: the whole channel fills to 1
+    assert np.max(np.abs(r[1:] - 1.0)) < 1e-6
+
+
+def first_order_box(k, clamp=1):
+    nx = 6
+    p = channel_params(nx, species=2)
+    p.ny = p.nz = 4
+    p.c0[0] = 1.0
+    p.n_rxn = 1
+    p.rxn_r1[0] = 0; p.rxn_k[0] = k
+    p.rxn_stoich[0][0] = -1; p.rxn_stoich[0][1] = 1
+    p.clamp = clamp
+    geom = np.full((4, 4, nx), P.PORE, np.int8)
+    return p, O.OracleCase(p, geom)
+
+
+def test_first_order_decay_exact():
+    """One step of A -> B from a uniform field: every voxel of the reaction box
+    that is not next to a plane holds exactly 1 - k (the deposit weights sum to
+    the increment); over many steps A + B stays exactly 1 in every voxel (both
+    lattices relax alike and the reaction moves mass between them)."""
+    k = 1e-2
+    p, o = first_order_box(k)
+    p.nx = 10
+    o = O.OracleCase(p, np.full((4, 4, 10), P.PORE, np.int8))
+    o.ade_steps(1)
+    A = o.read(P.FIELD_CONC, 0)
+    np.testing.assert_allclose(A[:, :, 2:-2], 1 - k, rtol=1e-14)
+    o.ade_steps(49)
+    A = o.read(P.FIELD_CONC, 0)[:, :, 1:-1]
+    B = o.read(P.FIELD_CONC, 1)[:, :, 1:-1]
+    np.testing.assert_allclose(A + B, 1.0, rtol=1e-13)
+    assert np.all(A < 1 - k)
+
+
+def test_clamp_caps_consumption_at_availability():
+    """test_abiotic.cpp:119-125 / test_symbolic_processor.cpp:125-148: a debit
+    larger than the voxel holds takes exactly what is there; production is
+    never clamped."""
+    p, o = first_order_box(3.0, clamp=1)
+    p.nx = 10
+    o = O.OracleCase(p, np.full((4, 4, 10), P.PORE, np.int8))
+    o.ade_steps(1)
+    A = o.read(P.FIELD_CONC, 0)[:, :, 2:-2]             # away from the closed planes
+    B = o.read(P.FIELD_CONC, 1)[:, :, 2:-2]
+    assert np.all(A == 0.0)
+    np.testing.assert_allclose(B, 3.0, rtol=1e-12)        # the product term is R dt, unclamped
+    assert o.counters()[2] > 0
+    p2, o2 = first_order_box(3.0, clamp=0)
+    p2.nx = 10
+    o2 = O.OracleCase(p2, np.full((4, 4, 10), P.PORE, np.int8))
+    o2.ade_steps(1)
+    np.testing.assert_allclose(o2.read(P.FIELD_CONC, 0)[:, :, 2:-2], -2.0, rtol=1e-12)
+
+
+def square_duct_mean_velocity(G, mu, W):
+    s = 0.0
+    for n in range(1, 200, 2):
+        s += np.tanh(n * np.pi / 2) / n ** 5
+    Q = G * W ** 4 / (12 * mu) * (1 - 192 / np.pi ** 5 * s)
+    return Q / W ** 2
+
+
+def test_poiseuille_square_duct():
+    nxf, W = 30, 12
+    nx, ny, nz = nxf + 2, W + 2, W + 2
+    geom = np.full((nz, ny, nx), P.PORE, np.int8)
+    geom[:, 0, :] = P.WALL; geom[:, -1, :] = P.WALL; geom[0, :, :] = P.WALL; geom[-1, :, :] = P.WALL
+    p = P.default_params(nx, ny, nz, 1)
+    p.flow_on = 1
+    p.ns_tau = 0.8
+    o = O.OracleCase(p, geom)
+    dP = 1e-4
+    o.ns_init(dP)
+    it, cv = o.ns_solve(60000, P.CONV_RELDELTA, 1e-12, reset=True)
+    assert cv
+    o.ns_couple()
+    ux = o.read(P.FIELD_VEL, 0)
+    mid = ux[1:-1, 1:-1, nx // 2]
+    G = dP / (nx - 1)                      # p = rho/3: the planes differ by dP over nx-1
+    mu = (0.8 - 0.5) / 3.0
+    expect = square_duct_mean_velocity(G, mu, W)
+    assert abs(mid.mean() / expect - 1) < 0.03
+    # the profile is symmetric in y and z
+    np.testing.assert_allclose(mid, mid[::-1, :], rtol=1e-9)
+    np.testing.assert_allclose(mid, mid.T, rtol=1e-9)
+
+
+def test_peclet_rescale_hits_target():
+    w = configs.config1(scale=0.25, packer=O.geom_pack)
+    w.transport.stabilize_steps = 0
+    o = O.OracleCase(w.params, w.geometry)
+    rep = driver.setup(o, w.params, w.flow, w.transport)
+    assert abs(rep.pe_achieved - w.flow.pe) / w.flow.pe < 0.02
+    assert 0.5 < rep.ref_tau <= 2
+
+
+def test_ca_spill_invariants():
+    w = configs.config2(scale=0.1, packer=O.geom_pack)
+    p = w.params
+    p.flow_on = 0
+    o = O.OracleCase(p, w.geometry)
+    rng = np.random.default_rng(1)
+    openv = w.geometry >= 2
+    openv[:, :, 0] = False; openv[:, :, -1] = False
+    b = np.zeros(w.geometry.shape)
+    b[openv] = rng.uniform(0, 60, openv.sum())
+    hot = np.flatnonzero(openv.reshape(-1))[::5]
+    b.reshape(-1)[hot] = rng.uniform(120, 300, hot.size)
+    o.write(P.FIELD_BIOMASS, 0, b)
+    rc, sweeps = o.ca_spill()
+    assert rc == 0 and sweeps > 0
+    a = o.read(P.FIELD_BIOMASS, 0)
+    assert abs(a.sum() - b.sum()) <= 1e-11 * b.sum()
+    assert a.max() <= p.bmax + 1e-9
+    assert np.all(a[w.geometry < 2] == 0)
+    m = o.read(P.FIELD_MASK)
+    thr = p.thrd_bfilm_frac * p.bmax
+    assert np.all(m[(a >= thr) & openv] == P.BIOFILM)
+    assert np.all(m[(a < thr) & openv] == P.PORE)
+
+
+def gauss_seidel_age(age, dist, btot, bmax):
+    """updateAgeDistance3D as the reference runs it: in place, x outermost,
+    then y, then z (part2:868-955); repeated until the age sum stops changing."""
+    nz, ny, nx = age.shape
+    age = age.copy()
+    T = 1e-14
+    while True:
+        s0 = age.sum()
+        for x in range(nx):
+            for y in range(ny):
+                for z in range(nz):
+                    B = btot[z, y, x]
+                    if not B > T:
+                        continue
+                    nb = []
+                    for dx, dy, dz in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+                        X, Y, Z = x + dx, y + dy, z + dz
+                        if 0 <= X < nx and 0 <= Y < ny and 0 <= Z < nz and dist[Z, Y, X] > 0:
+                            nb.append((Z, Y, X))
+                    a = age[z, y, x]
+                    if a == 0:
+                        age[z, y, x] = 1
+                    elif a == 1 and (B - bmax) > -T:
+                        if not any(age[q] == 0 for q in nb):
+                            age[z, y, x] = 2
+                    else:
+                        if all(age[q] >= a and (btot[q] - bmax) > -T for q in nb):
+                            age[z, y, x] = a + 1
+        if age.sum() == s0:
+            return age
+
+
+def test_age_relaxation_reaches_the_reference_fixed_point():
+    """The restatement sweeps against a snapshot; the reference sweeps in
+    place.  The rule only raises ages and is monotone in the neighbours'
+    ages, so both reach the same fixed point."""
+    nx, ny, nz = 10, 7, 6
+    rng = np.random.default_rng(4)
+    geom = np.full((nz, ny, nx), P.PORE, np.int8)
+    geom[:, 0, :] = P.WALL; geom[:, -1, :] = P.WALL; geom[0, :, :] = P.WALL; geom[-1, :, :] = P.WALL
+    p = P.default_params(nx, ny, nz, 1)
+    p.n_microbes = 1
+    p.bmax = 10.0
+    o = O.OracleCase(p, geom)
+    b = np.where(geom >= 2, rng.choice([0.0, 5.0, 10.0, 10.0], size=geom.shape), 0.0)
+    o.write(P.FIELD_BIOMASS, 0, b)
+    age0 = np.where(geom >= 2, 0, -1)
+    o.write(P.FIELD_AGE, 0, age0)
+    o.age_relax(True)
+    jac = o.read(P.FIELD_AGE).astype(int)
+    dist = np.zeros_like(age0)
+    inner = np.zeros(geom.shape, bool); inner[:-1, :-1, :-1] = True
+    dist[(geom >= 2) & inner] = 1
+    gs = gauss_seidel_age(age0, dist, b, p.bmax)
+    assert np.array_equal(jac, gs)
