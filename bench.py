@@ -1,0 +1,146 @@
+"""Benchmark of the coupled reaction step on one MI355X.
+
+Workload: the biotic plain-method configuration -- 256x128x128 voxels (+2 ghost
+x columns) seeded sphere pack at porosity 0.35 (r = 6), four D3Q7 species,
+dual-Monod biofilm on 2 % of the grain surface, CA spill, flow re-solved on
+mask changes every 50 steps, FP64.  The set-up (flow solve, Peclet rescale,
+stabilization) is untimed; W warm-up steps, then K coupled steps are timed
+between two device synchronisations.  Prints one JSON line.
+
+value: coupled MLUPS = lattice voxels x K / seconds / 1e6.
+roofline: the fused transport + kinetics kernel, algorithmic bytes per launch
+(per open voxel S_m*14*p + 3p + (S_i+M)*2p, plus 1 byte of mask per voxel)
+over its HIP-event-timed average duration, against 8 TB/s.
+cpu_baseline: the CPU oracle (a scalar C++ restatement) on the same case for a
+few steps, velocity copied from the GPU flow solve.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "complab3d-gems_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(p, n_open, n_vox):
+    r = 4 if p.precision == 1 else 8
+    sm = sum(1 for s in range(p.n_species) if p.mobile[s])
+    si = p.n_species - sm
+    per_open = sm * 14 * r + 3 * r + (si + p.n_microbes) * 2 * r
+    return per_open * n_open + n_vox, per_open
+
+
+def cpu_baseline(w, g, rep, steps):
+    import oracle as O
+    from complab_amd import params as P
+    o = O.OracleCase(w.params, w.geometry)
+    for a in range(3):
+        o.write(P.FIELD_VEL, a, g.read(P.FIELD_VEL, a))
+    o.set_omegas(rep.omega_pore, rep.omega_biofilm, 1.0)
+    o.run_steps(1)
+    t0 = time.perf_counter()
+    rc, n = o.run_steps(steps)
+    dt = time.perf_counter() - t0
+    V = w.params.nx * w.params.ny * w.params.nz
+    o.close()
+    return V * n / dt / 1e6, n, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stabilize", type=int, default=10000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(each case is one device's work; see the sharding design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import complab_amd as clb
+    from complab_amd import configs, driver
+    from complab_amd import params as P
+
+    prec = P.FP32 if args.precision == "fp32" else P.FP64
+    w = configs.config2(scale=args.scale, precision=prec)
+    w.transport.stabilize_steps = args.stabilize
+    ctx = clb.Context(0)
+    dev = ctx.device_name()
+    g = clb.Case(ctx, w.params, w.geometry)
+    t0 = time.perf_counter()
+    rep = driver.setup(g, w.params, w.flow, w.transport)
+    g.set_omegas(rep.omega_pore, rep.omega_biofilm, 1.0)
+    setup_s = time.perf_counter() - t0
+    print("setup: %.1f s, flow iterations %s, dP %.4g, Pe %.3f, dt %.4g" %
+          (setup_s, rep.iters, rep.delta_p, rep.pe_achieved, rep.dt), file=sys.stderr, flush=True)
+
+    rc, n = g.run_steps(args.warmup)
+    if rc:
+        raise SystemExit("warm-up stopped with status %d after %d steps" % (rc, n))
+    ctx.sync()
+    g.kernel_time(reset=True)
+    c0 = g.counters()
+    t0 = time.perf_counter()
+    rc, n = g.run_steps(args.steps)
+    ctx.sync()
+    t = time.perf_counter() - t0
+    if rc or n != args.steps:
+        raise SystemExit("timed run stopped with status %d after %d steps" % (rc, n))
+    c1 = g.counters()
+    kms, launches = g.kernel_time()
+
+    p = w.params
+    V = p.nx * p.ny * p.nz
+    mask = g.read(P.FIELD_MASK)
+    n_open = int((mask >= 2).sum())
+    bytes_launch, per_open = algorithmic_bytes(p, n_open, V)
+    achieved = bytes_launch / (kms * 1e-3) / 1e9
+    mlups = V * args.steps / t / 1e6
+
+    cpu = None
+    if not args.no_cpu:
+        cv, cn, cdt = cpu_baseline(w, g, rep, args.cpu_steps)
+        cpu = {"value": round(cv, 4), "unit": "MLUPS", "cores": 1, "kind": "port",
+               "sample": "%d coupled steps of the same %dx%dx%d case by the scalar C++ oracle, "
+                         "velocity copied from the GPU flow solve (%.1f s)" % (cn, p.nx, p.ny, p.nz, cdt)}
+    out = {
+        "metric": "coupled MLUPS + %HBM roofline; sweep cases/hr and p50/p99 vs plain FP64",
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == P.FP64 else "f32",
+        "data": "synthetic: seeded sphere pack and biofilm seeds",
+        "config": {"workload": w.name, "lattice": [p.nx, p.ny, p.nz], "open_voxels": n_open,
+                   "species": p.n_species, "device": dev,
+                   "flow_resolves": int(c1[4] - c0[4]), "ca_sweeps": int(c1[6] - c0[6]),
+                   "mask_changes": int(c1[5] - c0[5])},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_ade_step (fused stream+collide+kinetics)",
+                     "kernel_avg_ms": round(kms, 5), "launches": int(launches),
+                     "bytes_per_open_voxel": per_open, "bytes_per_launch": int(bytes_launch)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -161,12 +161,15 @@ __global__ void k_ade_neumann(AdeDev<R> a) {
         const int s = a.mobSpecies[ms];
         const int bt = side == 0 ? a.lbt[s] : a.rbt[s];
         if (bt != BC_NEUMANN_DEV) continue;
-        double fl[7];
-        const R *cur = a.cur + (long long) ms * 7 * a.N;
-        const R *old = a.out + (long long) ms * 7 * a.N;
-        pull7<R>(cur, old, a.clsPull, a.N, a.nx, a.ny, a.nz, idx, x, y, z, a.lbt[s], a.rbt[s], a.first, fl);
         double r = 0;
-        for (int q = 0; q < 7; ++q) r += fl[q];
+        // a bounce-back inner cell reports its stored density, 0
+        if (!adeClosed(a.cls[idx], x, a.nx, a.lbt[s], a.rbt[s])) {
+            double fl[7];
+            const R *cur = a.cur + (long long) ms * 7 * a.N;
+            const R *old = a.out + (long long) ms * 7 * a.N;
+            pull7<R>(cur, old, a.clsPull, a.N, a.nx, a.ny, a.nz, idx, x, y, z, a.lbt[s], a.rbt[s], a.first, fl);
+            for (int q = 0; q < 7; ++q) r += fl[q];
+        }
         a.rhoN[(long long) (side * a.Smax + ms) * plane + k] = r;
     }
 }

@@ -187,11 +187,14 @@ void adeStream(Case &c, int s) {
     }
     c.f[s].swap(nf);
     /* FlatAdiabaticBoundaryFunctional3D (complab_functions.hh:384,389): after the
-     * stream, the Neumann plane takes the density of the layer inside it. */
+     * stream, the Neumann plane takes the density of the layer inside it.  An
+     * inner cell that is bounce-back reports BounceBack's stored density, 0
+     * (updateSoluteSolidDynamics3D installs BounceBack() with no density). */
     for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) {
         size_t k = (size_t) y + (size_t) c.ny * z;
-        if (c.p.left_btype[s] == CLB_BC_NEUMANN) c.rhoL[s][k] = densityF(c, s, c.id(1, y, z));
-        if (c.p.right_btype[s] == CLB_BC_NEUMANN) c.rhoR[s][k] = densityF(c, s, c.id(c.nx - 2, y, z));
+        size_t il = c.id(1, y, z), ir = c.id(c.nx - 2, y, z);
+        if (c.p.left_btype[s] == CLB_BC_NEUMANN) c.rhoL[s][k] = adeClassFor(c, s, 1, il) == ADE_BB ? 0.0 : densityF(c, s, il);
+        if (c.p.right_btype[s] == CLB_BC_NEUMANN) c.rhoR[s][k] = adeClassFor(c, s, c.nx - 2, ir) == ADE_BB ? 0.0 : densityF(c, s, ir);
     }
 }
 
@@ -809,8 +812,10 @@ double nsCollideStream(Case &c) {
         size_t i = c.id(x, y, z);
         int cls = c.nsCls[i];
         double fl[19]; for (int q = 0; q < Q19; ++q) fl[q] = c.g[(size_t) q * N + i];
-        if (cls == NS_NODYN) continue;
-        if (cls == NS_BB) { for (int q = 1; q < Q19; q += 2) std::swap(fl[q], fl[q + 1]); }
+        /* a no-dynamics grain cell reflects like a bounce-back cell: Palabos would let
+         * populations stream through it, which leaks flow into grain interiors
+         * along the D3Q19 edge links wherever a grain shell is only face-closed */
+        if (cls == NS_BB || cls == NS_NODYN) { for (int q = 1; q < Q19; q += 2) std::swap(fl[q], fl[q + 1]); }
         else {
             double om = cls == NS_BIO ? c.nsBioOmega : c.nsOmega;
             double rb = 0, j[3] = {0,0,0};
@@ -1210,8 +1215,9 @@ int orc_write_field(void *h, int field, int index, const double *src) {
             // a Neumann plane imposes the density its inner layer holds now
             for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) {
                 size_t k = (size_t) y + (size_t) c.ny * z;
-                if (c.p.left_btype[index] == CLB_BC_NEUMANN) c.rhoL[index][k] = src[c.id(1, y, z)];
-                if (c.p.right_btype[index] == CLB_BC_NEUMANN) c.rhoR[index][k] = src[c.id(c.nx - 2, y, z)];
+                size_t il = c.id(1, y, z), ir = c.id(c.nx - 2, y, z);
+                if (c.p.left_btype[index] == CLB_BC_NEUMANN) c.rhoL[index][k] = adeClassFor(c, index, 1, il) == ADE_BB ? 0.0 : src[il];
+                if (c.p.right_btype[index] == CLB_BC_NEUMANN) c.rhoR[index][k] = adeClassFor(c, index, c.nx - 2, ir) == ADE_BB ? 0.0 : src[ir];
             }
         }
         else for (size_t i = 0; i < N; ++i) c.imm[index][i] = src[i];

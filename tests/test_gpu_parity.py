@@ -173,9 +173,8 @@ def test_equilibrium_parity(ctx, solver, tol):
         g.write(P.FIELD_CONC, s, f); o.write(P.FIELD_CONC, s, f)
     for c in (g, o):
         c.set_omegas([1.2] * 4, [1.2] * 4, 1.0)
-    for _ in range(3):
-        g.ade_steps(1); o.ade_steps(1)
-        g.equilibrium(); o.equilibrium()
+        rc, n = c.run_steps(3)              # collide, speciation, stream (complab.cpp:2307-2569)
+        assert rc == 0 and n == 3
     om = open_mask(o)
     for s in range(4):
         e = relerr(g.read(P.FIELD_CONC, s), o.read(P.FIELD_CONC, s), om)
