@@ -8,7 +8,7 @@ REPO = os.path.dirname(ROOT)
 CSRC = os.path.join(ROOT, "csrc")
 LIB = os.path.join(ROOT, "libclb.so")
 
-SOURCES = ["clb_kernels.hip", "clb_host.cpp", "clb_geometry.cpp"]
+SOURCES = ["clb_lattice.hip", "clb_aux.hip", "clb_host.cpp", "clb_geometry.cpp"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -55,7 +55,9 @@ def config1(scale=1.0, precision=P.FP64, packer=None, seed=1):
     p.ns_conv_rule = P.CONV_RELDELTA
     p.ns_conv_eps = 1e-8
     p.ns_max_iter = 100000
-    fs = FlowSetup(pe=1.0, char_length=2 * radius, delta_p=1e-5 * (nxf + 1), ns_tau=0.8,
+    # characteristic length = the domain length, the convention of the reference's
+    # ABC sweep (ABC_sweep_package/tools/make_campaign.py:236, CHAR_LEN = NX*DX)
+    fs = FlowSetup(pe=1.0, char_length=float(nxf), delta_p=1e-5 * (nxf + 1), ns_tau=0.8,
                    rule=P.CONV_RELDELTA, eps=1e-8, max_iter=100000)
     ts = TransportSetup(d_pore=[0.1, 0.1], d_biofilm=[0.1, 0.1])
     return Workload("plain-fp64 %dx%dx%d spheres r%g phi0.45 A->B" % (nxf, ny, nz, radius), p, lat, phi, fs, ts, 2000)
@@ -121,7 +123,7 @@ def config2(scale=1.0, precision=P.FP64, packer=None, seed=2):
     p.ns_conv_eps = 1e-6
     # a reduced domain keeps the full Peclet number only at full scale: tiny grains
     # would need a pressure drop beyond the incompressible limit
-    fs = FlowSetup(pe=min(1.0, scale), char_length=2 * radius, delta_p=1e-5 * (nxf + 1), ns_tau=0.8,
+    fs = FlowSetup(pe=min(1.0, scale), char_length=float(nxf), delta_p=1e-5 * (nxf + 1), ns_tau=0.8,
                    rule=P.CONV_TRACER, eps=1e-8, max_iter=100000, has_biofilm=True)
     ts = TransportSetup(d_pore=[0.1] * 4, d_biofilm=[0.02] * 4)
     return Workload("biotic-fp64 %dx%dx%d spheres r%g phi0.35 4 species CA" % (nxf, ny, nz, radius),

@@ -1,0 +1,396 @@
+// clb_aux.hip -- the dense-grid kernels of the coupled step (gfx950): the CA
+// biofilm spill, the mask-driven dynamics switches, precipitation, the
+// reductions and the host<->device staging conversions.  Scalar fields
+// (mask, biomass, immobile species, age) are dense, index x + nx*(y + ny*z).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "clb_device.h"
+#include "clb_common.h"
+
+namespace clb {
+
+// ------------------------------------------------------------------------------------------
+// CA biofilm spill
+// ------------------------------------------------------------------------------------------
+// updateLocalMaskNtotalLattices3D + the global max of the total and the mask sum
+template <typename R>
+__global__ void __launch_bounds__(256) k_ca_mask_total(CaDev<R> c, double *partMax, long long *partMask) {
+    __shared__ double sMax[4];
+    __shared__ long long sMask[4];
+    const long long idx = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    double bt = -1e300; long long ms = 0;
+    if (idx < c.N) {
+        int mask = c.mask[idx];
+        if (mask != 0 && mask != 1) {
+            double bm = 0; int nm = 0;
+            for (int m = 0; m < c.M; ++m) { const double v = (double) c.B[(long long) m * c.N + idx]; if (v > 1e-14) { bm += v; nm += 3 + m; } }
+            double t = (double) c.Btot[idx];
+            if (fabs(bm - t) > 1e-14) { c.Btot[idx] = (R) bm; t = bm; }
+            const bool pore = (mask == 2);
+            if (!pore && bm < c.thrdB) { mask = 2; c.mask[idx] = 2; }
+            else if (pore && bm >= c.thrdB && nm > 0) { mask = nm; c.mask[idx] = (int8_t) nm; }
+        }
+        bt = (double) c.Btot[idx];
+        if (isnan(bt)) bt = __builtin_nan("");
+        ms = mask;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_down(bt, off, 64);
+        bt = (isnan(bt) || isnan(o)) ? __builtin_nan("") : fmax(bt, o);
+        ms += __shfl_down(ms, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { sMax[wv] = bt; sMask[wv] = ms; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = -1e300; long long s = 0;
+        for (int w = 0; w < (int) (blockDim.x >> 6); ++w) { m = (isnan(m) || isnan(sMax[w])) ? __builtin_nan("") : fmax(m, sMax[w]); s += sMask[w]; }
+        partMax[blockIdx.x] = m; partMask[blockIdx.x] = s;
+    }
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint32_t caDraw(uint64_t seed, uint64_t sweep, uint64_t vox, uint32_t k) {
+    uint64_t h = mix64(seed ^ mix64(sweep * 0x100000001b3ULL + 0x51ed27ULL));
+    h = mix64(h ^ (vox * 0x9e3779b97f4a7c15ULL));
+    h = mix64(h + k);
+    return (uint32_t) (h >> 32);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_ca_push(CaDev<R> c, uint64_t sweep) {
+    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.N) return;
+    c.pushed[i] = 0;
+    const int mask = c.mask[i];
+    if (mask == 0 || mask == 1) return;
+    if (c.half && mask == 2) return;
+    double bMt = (double) c.Btot[i];
+    if (!(bMt > c.bmax)) return;
+    double bMd;
+    if (c.half) bMd = bMt * 0.5;
+    else { bMd = bMt - c.bmax; if (!(bMd > 1e-12)) return; }
+    int x, y, z;
+    coords(i, c.nx, c.ny, x, y, z);
+    int cand[6]; int nc = 0;
+    for (int d = 0; d < 6; ++d) {
+        const int q = d + 1;
+        const int X = x + D7C[q][0], Y = y + D7C[q][1], Z = z + D7C[q][2];
+        if (q == 1 && x == c.nx - 2) continue;
+        if (q == 2 && x == 1) continue;
+        if (q == 3 && y == c.ny - 1) continue;
+        if (q == 4 && y == 0) continue;
+        if (q == 5 && z == c.nz - 1) continue;
+        if (q == 6 && z == 0) continue;
+        if (X < 0 || Y < 0 || Z < 0 || X >= c.nx || Y >= c.ny || Z >= c.nz) continue;
+        const int nm = c.mask[X + (long long) c.nx * (Y + (long long) c.ny * Z)];
+        if (nm == 0 || nm == 1) continue;
+        cand[nc++] = d;
+    }
+    if (nc == 0) return;
+    uint32_t kk = 0;
+    int perm[6];
+    for (int a = 0; a < nc; ++a) perm[a] = a;
+    for (int a = nc - 1; a > 0; --a) {
+        const int b = (int) (caDraw(c.seed, sweep, (uint64_t) i, kk++) % (uint32_t) (a + 1));
+        const int t = perm[a]; perm[a] = perm[b]; perm[b] = t;
+    }
+    double Bl[CLB_DEV_MAXM], ob[CLB_DEV_MAXM][6];
+    for (int m = 0; m < c.M; ++m) { Bl[m] = (double) c.B[(long long) m * c.N + i]; for (int d = 0; d < 6; ++d) ob[m][d] = 0; }
+    bool done = false;
+    for (int a = 0; a < nc; ++a) {
+        const int d = cand[perm[a]], q = d + 1;
+        const long long j = i + D7C[q][0] + (long long) c.nx * (D7C[q][1] + (long long) c.ny * D7C[q][2]);
+        const int nm = c.mask[j];
+        const double nb = (nm == 2) ? 0.0 : (double) c.Btot[j];
+        if (nb < c.bmax) {
+            const double cap = c.bmax - nb;
+            double part = bMd;
+            if (bMd > cap) { part = cap; bMd -= cap; } else { bMd = 0; done = true; }
+            for (int m = 0; m < c.M; ++m) {
+                const double sh = Bl[m] / bMt * part;
+                if (sh > 1e-12) { Bl[m] -= sh; ob[m][d] += sh; }
+            }
+            bMt -= part;
+        }
+        if (done) break;
+    }
+    if (!done) {
+        const int id0 = c.age[i];
+        int young[6]; int ny = 0;
+        for (int a = 0; a < nc; ++a) {
+            const int q = cand[a] + 1;
+            const int id1 = c.age[i + D7C[q][0] + (long long) c.nx * (D7C[q][1] + (long long) c.ny * D7C[q][2])];
+            if (id0 > id1) young[ny++] = cand[a];
+        }
+        int d;
+        if (ny > 1) d = young[caDraw(c.seed, sweep, (uint64_t) i, kk++) % (uint32_t) ny];
+        else if (ny == 1) d = young[0];
+        else d = cand[nc > 1 ? caDraw(c.seed, sweep, (uint64_t) i, kk++) % (uint32_t) nc : 0];
+        for (int m = 0; m < c.M; ++m) {
+            const double sh = Bl[m] / bMt * bMd;
+            Bl[m] -= sh; ob[m][d] += sh;
+        }
+    }
+    for (int m = 0; m < c.M; ++m) {
+        c.B[(long long) m * c.N + i] = (R) Bl[m];
+        for (int d = 0; d < 6; ++d) c.outbox[((long long) m * 6 + d) * c.N + i] = (R) ob[m][d];
+    }
+    c.pushed[i] = 1;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_ca_gather(CaDev<R> c) {
+    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.N) return;
+    int x, y, z;
+    coords(i, c.nx, c.ny, x, y, z);
+    double add[CLB_DEV_MAXM];
+    for (int m = 0; m < c.M; ++m) add[m] = 0;
+    bool any = false;
+    for (int d = 0; d < 6; ++d) {
+        const int q = d + 1;
+        const int X = x - D7C[q][0], Y = y - D7C[q][1], Z = z - D7C[q][2];
+        if (X < 0 || Y < 0 || Z < 0 || X >= c.nx || Y >= c.ny || Z >= c.nz) continue;
+        const long long j = X + (long long) c.nx * (Y + (long long) c.ny * Z);
+        if (!c.pushed[j]) continue;
+        any = true;
+        for (int m = 0; m < c.M; ++m) add[m] += (double) c.outbox[((long long) m * 6 + d) * c.N + j];
+    }
+    if (!any) return;
+    for (int m = 0; m < c.M; ++m)
+        if (add[m] != 0.0) c.B[(long long) m * c.N + i] = (R) ((double) c.B[(long long) m * c.N + i] + add[m]);
+}
+
+// updateAgeDistance3D, one Jacobi sweep (ageIn -> ageOut) + the sum of the increments
+template <typename R>
+__global__ void __launch_bounds__(256) k_ca_age(CaDev<R> c, const int *ageIn, int *ageOut, long long *partSum) {
+    __shared__ long long sS[4];
+    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    long long sum = 0;
+    if (i < c.N) {
+        const int a = ageIn[i];
+        int na = a;
+        const double B = (double) c.Btot[i];
+        if (B > 1e-14) {
+            int x, y, z;
+            coords(i, c.nx, c.ny, x, y, z);
+            long long del[6]; int nd = 0;
+            for (int q = 1; q < 7; ++q) {
+                const int X = x + D7C[q][0], Y = y + D7C[q][1], Z = z + D7C[q][2];
+                if (X < 0 || Y < 0 || Z < 0 || X >= c.nx || Y >= c.ny || Z >= c.nz) continue;
+                const long long j = X + (long long) c.nx * (Y + (long long) c.ny * Z);
+                if (c.dist[j] > 0) del[nd++] = j;
+            }
+            if (a == 0) na = 1;
+            else if (a == 1 && (B - c.bmax) > -1e-14) {
+                bool upd = true;
+                for (int t = 0; t < nd; ++t) if (c.dist[del[t]] > 0 && ageIn[del[t]] == 0) { upd = false; break; }
+                if (upd) na = 2;
+            } else {
+                int cnt = 0;
+                for (int t = 0; t < nd; ++t) if (ageIn[del[t]] >= a && ((double) c.Btot[del[t]] - c.bmax) > -1e-14) ++cnt;
+                if (cnt == nd) na = a + 1;
+            }
+        }
+        ageOut[i] = na;
+        sum = na - a;
+    }
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) sS[wv] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) { long long s = 0; for (int w = 0; w < (int) (blockDim.x >> 6); ++w) s += sS[w]; partSum[blockIdx.x] = s; }
+}
+
+// ------------------------------------------------------------------------------------------
+// mask-driven switches (compact ADE classes from the dense live mask) and precipitation
+// ------------------------------------------------------------------------------------------
+// updateSoluteDynamics3D (part2:485-553): pore/biofilm relaxation of open voxels
+__global__ void k_ade_omega_switch(const int8_t *mask, const int *list, uint8_t *cls, int Nopen) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nopen) return;
+    const int m = mask[list[i]];
+    if (m == 0 || m == 1) return;
+    if (cls[i] == ADE_BB) return;
+    cls[i] = m == 2 ? ADE_PORE : ADE_BFILM;
+}
+// updateSoluteSolidDynamics3D (part2:585-647): bounce-back <-> BGK from the live mask
+__global__ void k_ade_solid_switch(const int8_t *mask, const int *list, uint8_t *cls, uint8_t *reopened, int Nopen,
+                                   unsigned int *nre) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nopen) return;
+    const int m = mask[list[i]];
+    const bool want = (m == 0 || m == 1);
+    const bool isBB = cls[i] == ADE_BB;
+    reopened[i] = 0;
+    if (want && !isBB) cls[i] = ADE_BB;
+    else if (!want && isBB) { cls[i] = m == 2 ? ADE_PORE : ADE_BFILM; reopened[i] = 1; atomicAdd(nre, 1u); }
+}
+// precipNodeConversion3D (precipitationVOP.hh:166-227) over x = 1..nx-2
+template <typename R>
+__global__ void k_precip(int nx, int ny, long long N, int8_t *mask, const R *P, double maxRho, int phaseId,
+                         int8_t *phase, unsigned int *changed) {
+    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int x, y, z;
+    coords(i, nx, ny, x, y, z);
+    if (x <= 0 || x >= nx - 1) return;
+    const int m = mask[i];
+    if (m == 0 || m == 1) return;
+    if ((double) P[i] >= maxRho) {
+        mask[i] = 0;
+        if (phaseId > 0) phase[i] = (int8_t) phaseId;
+        atomicAdd(changed, 1u);
+    }
+}
+// total of an immobile species over the open voxels of x = 1..nx-2 (per-block partials)
+template <typename R>
+__global__ void __launch_bounds__(256) k_scalar_total(int nx, int ny, long long N, const int8_t *mask, const R *f,
+                                                      double *parts) {
+    __shared__ double sS[4];
+    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0;
+    if (i < N) {
+        int x, y, z;
+        coords(i, nx, ny, x, y, z);
+        const int m = mask[i];
+        if (x > 0 && x < nx - 1 && m != 0 && m != 1) v = (double) f[i];
+    }
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) sS[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) { double s2 = 0; for (int w = 0; w < (int) (blockDim.x >> 6); ++w) s2 += sS[w]; parts[blockIdx.x] = s2; }
+}
+
+// fixed-order final reduction of per-block partials (one block of 64 threads)
+__global__ void k_final_sum(const double *parts, long long n, int stride, int comp, int isMax, double *out) {
+    __shared__ double s[64];
+    const int t = threadIdx.x;
+    double v = isMax ? -1e300 : 0.0;
+    for (long long i = t; i < n; i += 64) {
+        const double p = parts[i * stride + comp];
+        v = isMax ? ((isnan(v) || isnan(p)) ? __builtin_nan("") : fmax(v, p)) : v + p;
+    }
+    s[t] = v;
+    __syncthreads();
+    if (t == 0) {
+        double r = isMax ? -1e300 : 0.0;
+        for (int i = 0; i < 64; ++i) r = isMax ? ((isnan(r) || isnan(s[i])) ? __builtin_nan("") : fmax(r, s[i])) : r + s[i];
+        *out = r;
+    }
+}
+__global__ void k_final_sum_ll(const long long *parts, long long n, long long *out) {
+    __shared__ long long s[64];
+    const int t = threadIdx.x;
+    long long v = 0;
+    for (long long i = t; i < n; i += 64) v += parts[i];
+    s[t] = v;
+    __syncthreads();
+    if (t == 0) { long long r = 0; for (int i = 0; i < 64; ++i) r += s[i]; *out = r; }
+}
+
+// scalar field <-> double staging
+template <typename R>
+__global__ void k_to_double(const R *src, double *dst, long long n) {
+    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (double) src[i];
+}
+template <typename R>
+__global__ void k_from_double(const double *src, R *dst, long long n) {
+    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (R) src[i];
+}
+template <typename T>
+__global__ void k_int_to_double(const T *src, double *dst, long long n) {
+    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (double) src[i];
+}
+
+// ------------------------------------------------------------------------------------------
+// host launchers
+// ------------------------------------------------------------------------------------------
+static inline unsigned int nblk(long long n, int b) { return (unsigned int) ((n + b - 1) / b); }
+
+template <typename R>
+hipError_t launchCaMaskTotal(const CaDev<R> &c, double *pm, long long *ps, hipStream_t st) {
+    hipLaunchKernelGGL(k_ca_mask_total<R>, dim3(nblk(c.N, 256)), dim3(256), 0, st, c, pm, ps);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchCaPushGather(const CaDev<R> &c, uint64_t sweep, hipStream_t st) {
+    hipLaunchKernelGGL(k_ca_push<R>, dim3(nblk(c.N, 256)), dim3(256), 0, st, c, sweep);
+    hipLaunchKernelGGL(k_ca_gather<R>, dim3(nblk(c.N, 256)), dim3(256), 0, st, c);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchCaAge(const CaDev<R> &c, const int *in, int *out, long long *ps, hipStream_t st) {
+    hipLaunchKernelGGL(k_ca_age<R>, dim3(nblk(c.N, 256)), dim3(256), 0, st, c, in, out, ps);
+    return hipGetLastError();
+}
+hipError_t launchOmegaSwitch(const int8_t *mask, const int *list, uint8_t *cls, int Nopen, hipStream_t st) {
+    hipLaunchKernelGGL(k_ade_omega_switch, dim3(nblk(Nopen, 256)), dim3(256), 0, st, mask, list, cls, Nopen);
+    return hipGetLastError();
+}
+hipError_t launchSolidSwitch(const int8_t *mask, const int *list, uint8_t *cls, uint8_t *re, int Nopen, unsigned int *nre,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(k_ade_solid_switch, dim3(nblk(Nopen, 256)), dim3(256), 0, st, mask, list, cls, re, Nopen, nre);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchPrecip(int nx, int ny, long long N, int8_t *mask, const R *P, double maxRho, int phaseId, int8_t *phase,
+                        unsigned int *changed, hipStream_t st) {
+    hipLaunchKernelGGL(k_precip<R>, dim3(nblk(N, 256)), dim3(256), 0, st, nx, ny, N, mask, P, maxRho, phaseId, phase, changed);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchScalarTotal(int nx, int ny, long long N, const int8_t *mask, const R *f, double *parts, hipStream_t st) {
+    hipLaunchKernelGGL(k_scalar_total<R>, dim3(nblk(N, 256)), dim3(256), 0, st, nx, ny, N, mask, f, parts);
+    return hipGetLastError();
+}
+hipError_t launchFinalSum(const double *parts, long long n, int stride, int comp, int isMax, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(64), 0, st, parts, n, stride, comp, isMax, out);
+    return hipGetLastError();
+}
+hipError_t launchFinalSumLL(const long long *parts, long long n, long long *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_final_sum_ll, dim3(1), dim3(64), 0, st, parts, n, out);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchToDouble(const R *src, double *dst, long long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_to_double<R>, dim3(nblk(n, 256)), dim3(256), 0, st, src, dst, n);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchFromDouble(const double *src, R *dst, long long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_from_double<R>, dim3(nblk(n, 256)), dim3(256), 0, st, src, dst, n);
+    return hipGetLastError();
+}
+hipError_t launchI8ToDouble(const int8_t *src, double *dst, long long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_int_to_double<int8_t>, dim3(nblk(n, 256)), dim3(256), 0, st, src, dst, n);
+    return hipGetLastError();
+}
+hipError_t launchI32ToDouble(const int *src, double *dst, long long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_int_to_double<int>, dim3(nblk(n, 256)), dim3(256), 0, st, src, dst, n);
+    return hipGetLastError();
+}
+
+#define INST(R) \
+    template hipError_t launchCaMaskTotal<R>(const CaDev<R> &, double *, long long *, hipStream_t); \
+    template hipError_t launchCaPushGather<R>(const CaDev<R> &, uint64_t, hipStream_t); \
+    template hipError_t launchCaAge<R>(const CaDev<R> &, const int *, int *, long long *, hipStream_t); \
+    template hipError_t launchPrecip<R>(int, int, long long, int8_t *, const R *, double, int, int8_t *, unsigned int *, hipStream_t); \
+    template hipError_t launchScalarTotal<R>(int, int, long long, const int8_t *, const R *, double *, hipStream_t); \
+    template hipError_t launchToDouble<R>(const R *, double *, long long, hipStream_t); \
+    template hipError_t launchFromDouble<R>(const double *, R *, long long, hipStream_t);
+INST(double)
+INST(float)
+
+}  // namespace clb

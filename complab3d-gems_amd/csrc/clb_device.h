@@ -14,23 +14,34 @@ enum { ADE_BB = 0, ADE_PORE = 1, ADE_BFILM = 2 };
 enum { NS_BGK = 0, NS_BIO = 1, NS_BB = 2, NS_NODYN = 3 };
 enum { BC_DIRICHLET_DEV = 0, BC_NEUMANN_DEV = 1, BC_CLOSED_DEV = 2 };
 
+// The open voxels (geometry pore or biofilm) of a case in compact order:
+// [0, Nint) the interior x = 1..nx-2, then [Nint, Nint+Nleft) the x = 0 plane,
+// then the x = nx-1 plane; each in x-fastest grid order.  list[i] is the grid
+// index of compact voxel i.  nb[q-1][i] (q = 1..Q-1) is the compact index of
+// the voxel a population of direction q is pulled from (grid index - c_q), or
+// NB_CLOSED when that voxel is closed geometry, NB_OUTSIDE beyond the grid.
+#define NB_CLOSED -1
+#define NB_OUTSIDE -2
+
 // the advection-diffusion lattices of one case
 template <typename R>
 struct AdeDev {
     int nx, ny, nz;
     long long N;
+    int Nopen, Nint, Nleft;
+    const int *list;               // [Nopen]
+    const int *nb;                 // [6][Nopen]
     int S, Sm, Smax, M;
     int mobile[CLB_DEV_MAXS];      // per species
     int slot[CLB_DEV_MAXS];        // per species: mobile slot or immobile slot
     int mobSpecies[CLB_DEV_MAXS];  // per mobile slot: species index
-    R *cur, *out;                  // [Smax][7][N]
-    R *imm;                        // [Si][N]
-    R *B;                          // [M][N]
-    const R *u;                    // [3][N] or null
-    int8_t *mask;
-    const int8_t *geom;
-    const uint8_t *cls, *clsPull;
-    double *rhoN;                  // [2][Smax][ny*nz]
+    R *cur, *out;                  // [Smax][7][Nopen]
+    R *imm;                        // [Si][N] dense
+    R *B;                          // [M][N] dense
+    const R *u;                    // [3][Nopen]
+    int8_t *mask;                  // dense live mask
+    const uint8_t *cls, *clsPull;  // [Nopen]
+    double *rhoN;                  // [Smax][Nopen - Nint]
     int anyNeumann;
     int first, reset;
     double c0[CLB_DEV_MAXS], omP[CLB_DEV_MAXS], omB[CLB_DEV_MAXS];
@@ -59,8 +70,11 @@ template <typename R>
 struct NsDev {
     int nx, ny, nz;
     long long N;
-    R *cur, *out;                  // [19][N]
-    const uint8_t *cls, *clsPull;
+    int Nopen, Nint, Nleft;
+    const int *list;               // [Nopen]
+    const int *nb;                 // [18][Nopen]
+    R *cur, *out;                  // [19][Nopen]
+    const uint8_t *cls, *clsPull;  // [Nopen]
     double om, omBio, rhoBarRight;
     int first;
     int *done;
@@ -109,24 +123,29 @@ struct EqDev {
     int maxIter, depth;
 };
 
-// host launchers (clb_kernels.hip)
+// host launchers: compact lattices (clb_lattice.hip)
 template <typename R> hipError_t launchAdeStep(const AdeDev<R> &, const KinDev &, hipStream_t);
 template <typename R> hipError_t launchAdeDensity(const AdeDev<R> &, int, double *, hipStream_t);
 template <typename R> hipError_t launchAdeSet(const AdeDev<R> &, int, const double *, hipStream_t);
 template <typename R> hipError_t launchAdeReopen(const AdeDev<R> &, const uint8_t *, hipStream_t);
+template <typename R> hipError_t launchAdeTotal(const AdeDev<R> &, int, double *, hipStream_t);
 template <typename R> hipError_t launchNsStep(const NsDev<R> &, double *, double *, NsTracer *, int, double, hipStream_t);
-template <typename R> hipError_t launchNsVelocity(const NsDev<R> &, R *, const int8_t *, double *, hipStream_t);
+template <typename R> hipError_t launchNsVelocity(const NsDev<R> &, R *, double *, hipStream_t);
 template <typename R> hipError_t launchNsInit(const NsDev<R> &, double, hipStream_t);
-hipError_t launchNsUpdateDyn(const int8_t *, uint8_t *, long long, double, unsigned int *, hipStream_t);
+hipError_t launchNsUpdateDyn(const int8_t *, const int *, uint8_t *, int, double, unsigned int *, hipStream_t);
 template <typename R> hipError_t launchNsReopen(const NsDev<R> &, const uint8_t *, hipStream_t);
 template <typename R> hipError_t launchEquilibrium(const AdeDev<R> &, const EqDev &, unsigned int *, hipStream_t);
+template <typename R> hipError_t launchScatterCompact(const R *, const int *, int, double *, hipStream_t);
+template <typename R> hipError_t launchGatherCompact(const double *, const int *, int, R *, hipStream_t);
+unsigned int nsStepBlocks(int Nopen, int Nint);
+// host launchers: dense grids (clb_aux.hip)
 template <typename R> hipError_t launchCaMaskTotal(const CaDev<R> &, double *, long long *, hipStream_t);
 template <typename R> hipError_t launchCaPushGather(const CaDev<R> &, uint64_t, hipStream_t);
 template <typename R> hipError_t launchCaAge(const CaDev<R> &, const int *, int *, long long *, hipStream_t);
-hipError_t launchOmegaSwitch(const int8_t *, uint8_t *, long long, hipStream_t);
-hipError_t launchSolidSwitch(const int8_t *, uint8_t *, uint8_t *, long long, unsigned int *, hipStream_t);
-template <typename R> hipError_t launchPrecip(const AdeDev<R> &, int, double, int, int8_t *, unsigned int *, hipStream_t);
-template <typename R> hipError_t launchSpeciesTotal(const AdeDev<R> &, int, double *, hipStream_t);
+hipError_t launchOmegaSwitch(const int8_t *, const int *, uint8_t *, int, hipStream_t);
+hipError_t launchSolidSwitch(const int8_t *, const int *, uint8_t *, uint8_t *, int, unsigned int *, hipStream_t);
+template <typename R> hipError_t launchPrecip(int, int, long long, int8_t *, const R *, double, int, int8_t *, unsigned int *, hipStream_t);
+template <typename R> hipError_t launchScalarTotal(int, int, long long, const int8_t *, const R *, double *, hipStream_t);
 hipError_t launchFinalSum(const double *, long long, int, int, int, double *, hipStream_t);
 hipError_t launchFinalSumLL(const long long *, long long, long long *, hipStream_t);
 template <typename R> hipError_t launchToDouble(const R *, double *, long long, hipStream_t);

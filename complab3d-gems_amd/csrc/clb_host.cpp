@@ -78,6 +78,9 @@ template <typename R>
 struct Case : CaseBase {
     // fields
     int8_t *mask = nullptr, *geom = nullptr, *dist = nullptr, *phase = nullptr;
+    // compact open-voxel layout (clb_device.h): list, neighbour tables, classes
+    int Nopen = 0, Nint = 0, Nleft = 0;
+    int *list = nullptr, *nb7 = nullptr, *nb19 = nullptr;
     uint8_t *adeCls = nullptr, *adeClsPull = nullptr, *nsCls = nullptr, *nsClsPull = nullptr, *reopened = nullptr;
     R *fA = nullptr, *fB = nullptr, *imm = nullptr, *B = nullptr, *Btot = nullptr, *outbox = nullptr;
     uint8_t *pushed = nullptr;
@@ -121,6 +124,7 @@ struct Case : CaseBase {
         AdeDev<R> a;
         memset(&a, 0, sizeof(a));
         a.nx = nx; a.ny = ny; a.nz = nz; a.N = N;
+        a.Nopen = Nopen; a.Nint = Nint; a.Nleft = Nleft; a.list = list; a.nb = nb7;
         a.S = p.n_species; a.Sm = Sm; a.Smax = Sm > 0 ? Sm : 1; a.M = p.n_microbes;
         for (int s = 0; s < CLB_DEV_MAXS; ++s) {
             a.mobile[s] = s < p.n_species ? p.mobile[s] : 0;
@@ -131,7 +135,7 @@ struct Case : CaseBase {
             a.lbc[s] = p.left_bc[s]; a.rbc[s] = p.right_bc[s];
         }
         a.cur = fCur(); a.out = fOut(); a.imm = imm; a.B = B;
-        a.u = u; a.mask = mask; a.geom = geom; a.cls = adeCls; a.clsPull = adeClsPull;
+        a.u = u; a.mask = mask; a.cls = adeCls; a.clsPull = adeClsPull;
         a.rhoN = rhoN;
         a.anyNeumann = 0;
         for (int ms = 0; ms < Sm; ++ms) {
@@ -168,6 +172,7 @@ struct Case : CaseBase {
         NsDev<R> a;
         memset(&a, 0, sizeof(a));
         a.nx = nx; a.ny = ny; a.nz = nz; a.N = N;
+        a.Nopen = Nopen; a.Nint = Nint; a.Nleft = Nleft; a.list = list; a.nb = nb19;
         a.cur = gCur(); a.out = gOut();
         a.cls = nsCls; a.clsPull = nsClsPull;
         a.om = nsOmega; a.omBio = nsBioOmega; a.rhoBarRight = -deltaP * 3.0;
@@ -218,13 +223,48 @@ struct Case : CaseBase {
             if (p.mobile[s]) { slot[s] = Sm; mobSpecies[Sm] = s; ++Sm; }
             else { slot[s] = Si; ++Si; }
         }
+        // the compact open-voxel order: interior, x = 0 plane, x = nx-1 plane
+        std::vector<int8_t> g(geometry, geometry + N), dh(N), ph(N, 0);
+        std::vector<int> lst, d2c(N, -1);
+        for (int pass = 0; pass < 3; ++pass)
+            for (long long v = 0; v < N; ++v) {
+                const int x = (int) (v % nx);
+                const bool want = pass == 0 ? (x > 0 && x < nx - 1) : (pass == 1 ? x == 0 : x == nx - 1);
+                if (want && g[v] >= CLB_PORE) { d2c[v] = (int) lst.size(); lst.push_back((int) v); }
+                if (pass == 0 && v == N - 1) Nint = (int) lst.size();
+                if (pass == 1 && v == N - 1) Nleft = (int) lst.size() - Nint;
+            }
+        Nopen = (int) lst.size();
+        if (Nopen == 0) throw HipFail{hipErrorInvalidValue, "the geometry has no open voxel"};
+        auto nbTable = [&](int Q, const int (*C)[3]) {
+            std::vector<int> t((size_t) (Q - 1) * Nopen);
+            for (int i = 0; i < Nopen; ++i) {
+                const long long v = lst[i];
+                const int x = (int) (v % nx), y = (int) ((v / nx) % ny), z = (int) (v / ((long long) nx * ny));
+                for (int q = 1; q < Q; ++q) {
+                    const int xs = x - C[q][0], ys = y - C[q][1], zs = z - C[q][2];
+                    int n = -2;
+                    if (xs >= 0 && ys >= 0 && zs >= 0 && xs < nx && ys < ny && zs < nz) {
+                        n = d2c[xs + (long long) nx * (ys + (long long) ny * zs)];
+                        if (n < 0) n = -1;
+                    }
+                    t[(size_t) (q - 1) * Nopen + i] = n;
+                }
+            }
+            return t;
+        };
+        static const int C7h[7][3] = {{0,0,0},{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
+        static const int C19h[19][3] = {{0,0,0},{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1},
+            {1,1,0},{-1,-1,0},{1,-1,0},{-1,1,0},{1,0,1},{-1,0,-1},{1,0,-1},{-1,0,1},{0,1,1},{0,-1,-1},{0,1,-1},{0,-1,1}};
         mask = dalloc<int8_t>(allocs, N); geom = dalloc<int8_t>(allocs, N);
         dist = dalloc<int8_t>(allocs, N); phase = dalloc<int8_t>(allocs, N);
-        adeCls = dalloc<uint8_t>(allocs, N); adeClsPull = dalloc<uint8_t>(allocs, N);
-        nsCls = dalloc<uint8_t>(allocs, N); nsClsPull = dalloc<uint8_t>(allocs, N);
-        reopened = dalloc<uint8_t>(allocs, N);
-        fA = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * 7 * N);
-        fB = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * 7 * N);
+        list = dalloc<int>(allocs, Nopen);
+        nb7 = dalloc<int>(allocs, (size_t) 6 * Nopen);
+        adeCls = dalloc<uint8_t>(allocs, Nopen); adeClsPull = dalloc<uint8_t>(allocs, Nopen);
+        nsCls = dalloc<uint8_t>(allocs, Nopen); nsClsPull = dalloc<uint8_t>(allocs, Nopen);
+        reopened = dalloc<uint8_t>(allocs, Nopen);
+        fA = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * 7 * Nopen);
+        fB = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * 7 * Nopen);
         imm = dalloc<R>(allocs, (size_t) std::max(Si, 1) * N);
         const int M = p.n_microbes;
         B = dalloc<R>(allocs, (size_t) std::max(M, 1) * N);
@@ -234,9 +274,12 @@ struct Case : CaseBase {
             pushed = dalloc<uint8_t>(allocs, N);
         }
         age = dalloc<int>(allocs, N); age2 = dalloc<int>(allocs, N);
-        if (p.flow_on) { gA = dalloc<R>(allocs, 19 * (size_t) N); gB = dalloc<R>(allocs, 19 * (size_t) N); }
-        u = dalloc<R>(allocs, 3 * (size_t) N);
-        rhoN = dalloc<double>(allocs, 2 * (size_t) std::max(Sm, 1) * ny * nz);
+        if (p.flow_on) {
+            gA = dalloc<R>(allocs, 19 * (size_t) Nopen); gB = dalloc<R>(allocs, 19 * (size_t) Nopen);
+            nb19 = dalloc<int>(allocs, (size_t) 18 * Nopen);
+        }
+        u = dalloc<R>(allocs, 3 * (size_t) Nopen);
+        rhoN = dalloc<double>(allocs, (size_t) std::max(Sm, 1) * std::max(Nopen - Nint, 1));
         nparts = nblk(N, 128);
         parts = dalloc<double>(allocs, (size_t) nparts * 5);
         partsLL = dalloc<long long>(allocs, nparts);
@@ -248,44 +291,52 @@ struct Case : CaseBase {
         nsDone = dalloc<int>(allocs, 1);
         tracer = dalloc<NsTracer>(allocs, 1);
 
-        // host-side construction of the voxel bytes and scalar fields
-        std::vector<int8_t> g(geometry, geometry + N), dh(N), ph(N, 0);
-        std::vector<uint8_t> ac(N), nc(N);
+        // voxel bytes and scalar fields
+        std::vector<uint8_t> ac(Nopen), nc(Nopen);
         std::vector<int> ag(N);
         for (long long i = 0; i < N; ++i) {
-            int gv = g[i];
-            int x = (int) (i % nx), y = (int) ((i / nx) % ny), z = (int) (i / ((long long) nx * ny));
-            bool closed = gv == CLB_SOLID || gv == CLB_WALL;
-            ac[i] = closed ? ADE_BB : (gv == CLB_PORE ? ADE_PORE : ADE_BFILM);
+            const int gv = g[i];
+            const int x = (int) (i % nx), y = (int) ((i / nx) % ny), z = (int) (i / ((long long) nx * ny));
+            const bool closed = gv == CLB_SOLID || gv == CLB_WALL;
             ag[i] = closed ? -1 : (gv == CLB_PORE ? 0 : 1);                      // createAgeDomain3D
-            bool in = x < nx - 1 && y < ny - 1 && z < nz - 1;                   // calculateDistanceFromSolid
+            const bool in = x < nx - 1 && y < ny - 1 && z < nz - 1;             // calculateDistanceFromSolid
             dh[i] = (in && !closed) ? 1 : ((gv == CLB_SOLID && in) ? -1 : 0);
-            if (gv == CLB_WALL) nc[i] = NS_BB;
-            else if (gv == CLB_SOLID) nc[i] = NS_NODYN;
-            else if (gv == CLB_PORE) nc[i] = NS_BGK;
-            else nc[i] = p.perm_ratio > 1e-14 ? NS_BIO : NS_BB;
+        }
+        for (int i = 0; i < Nopen; ++i) {
+            const int gv = g[lst[i]];
+            ac[i] = gv == CLB_PORE ? ADE_PORE : ADE_BFILM;
+            nc[i] = gv == CLB_PORE ? NS_BGK : (p.perm_ratio > 1e-14 ? NS_BIO : NS_BB);
         }
         HIPT(hipMemcpyAsync(mask, g.data(), N, hipMemcpyHostToDevice, st));
         HIPT(hipMemcpyAsync(geom, g.data(), N, hipMemcpyHostToDevice, st));
         HIPT(hipMemcpyAsync(dist, dh.data(), N, hipMemcpyHostToDevice, st));
         HIPT(hipMemcpyAsync(phase, ph.data(), N, hipMemcpyHostToDevice, st));
-        HIPT(hipMemcpyAsync(adeCls, ac.data(), N, hipMemcpyHostToDevice, st));
-        HIPT(hipMemcpyAsync(adeClsPull, ac.data(), N, hipMemcpyHostToDevice, st));
-        HIPT(hipMemcpyAsync(nsCls, nc.data(), N, hipMemcpyHostToDevice, st));
-        HIPT(hipMemcpyAsync(nsClsPull, nc.data(), N, hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpy(list, lst.data(), (size_t) Nopen * sizeof(int), hipMemcpyHostToDevice));
+        {
+            std::vector<int> t7 = nbTable(7, C7h);
+            HIPT(hipMemcpy(nb7, t7.data(), t7.size() * sizeof(int), hipMemcpyHostToDevice));
+            if (p.flow_on) {
+                std::vector<int> t19 = nbTable(19, C19h);
+                HIPT(hipMemcpy(nb19, t19.data(), t19.size() * sizeof(int), hipMemcpyHostToDevice));
+            }
+        }
+        HIPT(hipMemcpyAsync(adeCls, ac.data(), Nopen, hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpyAsync(adeClsPull, ac.data(), Nopen, hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpyAsync(nsCls, nc.data(), Nopen, hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpyAsync(nsClsPull, nc.data(), Nopen, hipMemcpyHostToDevice, st));
         HIPT(hipMemcpyAsync(age, ag.data(), N * sizeof(int), hipMemcpyHostToDevice, st));
-        HIPT(hipMemsetAsync(reopened, 0, N, st));
+        HIPT(hipMemsetAsync(reopened, 0, Nopen, st));
         // populations at rest with c0 (initializeAtEquilibrium + stabilizeADElattice3D seeding)
         {
-            std::vector<R> fl((size_t) 7 * N);
+            std::vector<R> fl((size_t) 7 * Nopen);
             for (int ms = 0; ms < Sm; ++ms) {
                 int s = mobSpecies[ms];
                 for (int q = 0; q < 7; ++q) {
                     R v = (R) ((q == 0 ? 0.25 : 0.125) * p.c0[s]);
-                    std::fill(fl.begin() + (size_t) q * N, fl.begin() + (size_t) (q + 1) * N, v);
+                    std::fill(fl.begin() + (size_t) q * Nopen, fl.begin() + (size_t) (q + 1) * Nopen, v);
                 }
-                HIPT(hipMemcpy(fA + (size_t) ms * 7 * N, fl.data(), fl.size() * sizeof(R), hipMemcpyHostToDevice));
-                HIPT(hipMemcpy(fB + (size_t) ms * 7 * N, fl.data(), fl.size() * sizeof(R), hipMemcpyHostToDevice));
+                HIPT(hipMemcpy(fA + (size_t) ms * 7 * Nopen, fl.data(), fl.size() * sizeof(R), hipMemcpyHostToDevice));
+                HIPT(hipMemcpy(fB + (size_t) ms * 7 * Nopen, fl.data(), fl.size() * sizeof(R), hipMemcpyHostToDevice));
             }
             std::vector<R> sc(N);
             for (int s = 0; s < p.n_species; ++s) {
@@ -308,7 +359,7 @@ struct Case : CaseBase {
             for (long long i = 0; i < N; ++i) ms += g[i];
             oldTotMask = ms;
         }
-        HIPT(hipMemsetAsync(u, 0, 3 * (size_t) N * sizeof(R), st));
+        HIPT(hipMemsetAsync(u, 0, 3 * (size_t) Nopen * sizeof(R), st));
         HIPT(hipMemsetAsync(clampCount, 0, sizeof(unsigned long long), st));
         HIPT(hipMemsetAsync(ucnt, 0, 4 * sizeof(unsigned int), st));
         HIPT(hipMemsetAsync(nsDone, 0, sizeof(int), st));
@@ -329,17 +380,16 @@ struct Case : CaseBase {
         deltaP = dP;
         // NSdomainSetup: dynamics from the static geometry
         std::vector<int8_t> g(N);
+        std::vector<int> lst(Nopen);
         HIPT(hipMemcpy(g.data(), geom, N, hipMemcpyDeviceToHost));
-        std::vector<uint8_t> nc(N);
-        for (long long i = 0; i < N; ++i) {
-            int gv = g[i];
-            if (gv == CLB_WALL) nc[i] = NS_BB;
-            else if (gv == CLB_SOLID) nc[i] = NS_NODYN;
-            else if (gv == CLB_PORE) nc[i] = NS_BGK;
-            else nc[i] = p.perm_ratio > 1e-14 ? NS_BIO : NS_BB;
+        HIPT(hipMemcpy(lst.data(), list, (size_t) Nopen * sizeof(int), hipMemcpyDeviceToHost));
+        std::vector<uint8_t> nc(Nopen);
+        for (int i = 0; i < Nopen; ++i) {
+            const int gv = g[lst[i]];
+            nc[i] = gv == CLB_PORE ? NS_BGK : (p.perm_ratio > 1e-14 ? NS_BIO : NS_BB);
         }
-        HIPT(hipMemcpyAsync(nsCls, nc.data(), N, hipMemcpyHostToDevice, st));
-        HIPT(hipMemcpyAsync(nsClsPull, nc.data(), N, hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpyAsync(nsCls, nc.data(), Nopen, hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpyAsync(nsClsPull, nc.data(), Nopen, hipMemcpyHostToDevice, st));
         nsBioOmega = 1.0 / (p.perm_ratio * (1.0 / nsOmega - 0.5) + 0.5);
         gCurIsA = true;
         HIPT(launchNsInit<R>(ns(), dP, st));
@@ -359,7 +409,7 @@ struct Case : CaseBase {
         long long base = th.iters;
         th.converged = 0; th.nonfinite = 0;
         HIPT(hipMemcpyAsync(tracer, &th, sizeof(NsTracer), hipMemcpyHostToDevice, st));
-        const unsigned int g = nblk(N, 256);
+        const unsigned int g = nsStepBlocks(Nopen, Nint);
         double *partE = parts, *partN = parts + g;
         int done = 0;
         long long ran = 0;
@@ -373,7 +423,7 @@ struct Case : CaseBase {
                 a.cur = curA ? gA : gB; a.out = curA ? gB : gA;
                 HIPT(launchNsStep<R>(a, partE, partN, tracer, rule, eps, st));
                 nsFirst = false;
-                if (!launched && nsClsDirty) { HIPT(hipMemcpyAsync(nsClsPull, nsCls, N, hipMemcpyDeviceToDevice, st)); nsClsDirty = false; }
+                if (!launched && nsClsDirty) { HIPT(hipMemcpyAsync(nsClsPull, nsCls, Nopen, hipMemcpyDeviceToDevice, st)); nsClsDirty = false; }
                 launched = true;
                 curA = !curA;
             }
@@ -395,19 +445,20 @@ struct Case : CaseBase {
 
     void nsCouple(hipStream_t st) override {
         if (!p.flow_on) return;
-        HIPT(launchNsVelocity<R>(ns(), u, geom, nullptr, st));
+        HIPT(launchNsVelocity<R>(ns(), u, nullptr, st));
     }
     void nsStats(hipStream_t st, double out[4]) override {
         for (int k = 0; k < 4; ++k) out[k] = 0;
         if (!p.flow_on) return;
-        const unsigned int g = nblk(N, 256);
-        HIPT(launchNsVelocity<R>(ns(), nullptr, geom, parts, st));
+        const unsigned int g = nblk(Nopen, 256);
+        HIPT(launchNsVelocity<R>(ns(), nullptr, parts, st));
         for (int c = 0; c < 5; ++c) HIPT(launchFinalSum(parts, g, 5, c, c == 1, scal + c, st));
         double h[5];
         HIPT(hipMemcpyAsync(h, scal, 5 * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPT(hipStreamSynchronize(st));
         out[0] = h[0] / ((double) (nx - 2) * ny * nz);
-        out[1] = h[1];
+        // closed voxels of the box report u = 0 (BounceBack/NoDynamics computeVelocity)
+        out[1] = (Nint < (long long) (nx - 2) * ny * nz) ? std::max(h[1], 0.0) : h[1];
         out[2] = h[2] / ((double) ny * nz);
         out[3] = h[4] > 0 ? h[3] / h[4] : 0.0;
     }
@@ -415,12 +466,12 @@ struct Case : CaseBase {
         if (!p.flow_on) return;
         nsBioOmega = 1.0 / (perm * (1.0 / nsOmega - 0.5) + 0.5);
         HIPT(hipMemsetAsync(ucnt, 0, sizeof(unsigned int), st));
-        HIPT(launchNsUpdateDyn(mask, nsCls, N, perm, ucnt, st));
+        HIPT(launchNsUpdateDyn(mask, list, nsCls, Nopen, perm, ucnt, st));
         unsigned int ch = 0;
         HIPT(hipMemcpyAsync(&ch, ucnt, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
         HIPT(hipStreamSynchronize(st));
         if (ch) {
-            HIPT(launchNsReopen<R>(ns(), nsClsPull, st));
+            HIPT(launchNsReopen<R>(ns(), nullptr, st));
             nsClsDirty = true;
         }
     }
@@ -452,7 +503,7 @@ struct Case : CaseBase {
         fCurIsA = !fCurIsA;
         adeFirst = false;
         adeReset = false;
-        if (adeClsDirty) { HIPT(hipMemcpyAsync(adeClsPull, adeCls, N, hipMemcpyDeviceToDevice, st)); adeClsDirty = false; }
+        if (adeClsDirty) { HIPT(hipMemcpyAsync(adeClsPull, adeCls, Nopen, hipMemcpyDeviceToDevice, st)); adeClsDirty = false; }
     }
     void adeStabilize(hipStream_t st, int steps) override {
         for (int t = 0; t < steps; ++t) fusedLaunch(st, false);
@@ -550,7 +601,7 @@ struct Case : CaseBase {
                 cnt[5]++;
                 Status a = ageRelax(st, false);
                 if (a.code) return a;
-                if (p.ade_update_interval > 0 && iT % p.ade_update_interval == 0) HIPT(launchOmegaSwitch(mask, adeCls, N, st));
+                if (p.ade_update_interval > 0 && iT % p.ade_update_interval == 0) HIPT(launchOmegaSwitch(mask, list, adeCls, Nopen, st));
                 if (p.ns_update_interval > 0 && iT % p.ns_update_interval == 0 && p.flow_on && !nsSaturate) {
                     Status r = resolve(st, p.perm_ratio);
                     if (r.code) return r;
@@ -559,7 +610,7 @@ struct Case : CaseBase {
         }
         if (p.precip_on && p.precip_interval > 0 && iT % p.precip_interval == 0 && p.precip_species >= 0) {
             HIPT(hipMemsetAsync(ucnt, 0, 2 * sizeof(unsigned int), st));
-            HIPT(launchPrecip<R>(ade(), slot[p.precip_species], p.precip_max_rho, p.precip_phase_id, phase, ucnt, st));
+            HIPT(launchPrecip<R>(nx, ny, N, mask, imm + (size_t) slot[p.precip_species] * N, p.precip_max_rho, p.precip_phase_id, phase, ucnt, st));
             unsigned int ch = 0;
             HIPT(hipMemcpyAsync(&ch, ucnt, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
             HIPT(hipStreamSynchronize(st));
@@ -568,7 +619,7 @@ struct Case : CaseBase {
                 if (p.flow_on && !nsSaturate) { Status r = resolve(st, p.precip_perm_ratio); if (r.code) return r; }
                 unsigned int nre = 0;
                 HIPT(hipMemsetAsync(ucnt + 1, 0, sizeof(unsigned int), st));
-                HIPT(launchSolidSwitch(mask, adeCls, reopened, N, ucnt + 1, st));
+                HIPT(launchSolidSwitch(mask, list, adeCls, reopened, Nopen, ucnt + 1, st));
                 HIPT(hipMemcpyAsync(&nre, ucnt + 1, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
                 HIPT(hipStreamSynchronize(st));
                 if (nre) HIPT(launchAdeReopen<R>(ade(), reopened, st));
@@ -594,6 +645,7 @@ struct Case : CaseBase {
             if (index < 0 || index >= p.n_species) return {CLB_E_ARG, "species index out of range"};
             if (p.mobile[index]) {
                 AdeDev<R> a = ade();
+                HIPT(hipMemsetAsync(stage, 0, N * sizeof(double), st));
                 HIPT(launchAdeDensity<R>(a, slot[index], stage, st));
             } else HIPT(launchToDouble<R>(imm + (size_t) slot[index] * N, stage, N, st));
             break;
@@ -605,7 +657,8 @@ struct Case : CaseBase {
         case CLB_FIELD_MASK: HIPT(launchI8ToDouble(mask, stage, N, st)); break;
         case CLB_FIELD_VEL:
             if (index < 0 || index > 2) return {CLB_E_ARG, "velocity component out of range"};
-            HIPT(launchToDouble<R>(u + (size_t) index * N, stage, N, st));
+            HIPT(hipMemsetAsync(stage, 0, N * sizeof(double), st));
+            HIPT(launchScatterCompact<R>(u + (size_t) index * Nopen, list, Nopen, stage, st));
             break;
         case CLB_FIELD_AGE: HIPT(launchI32ToDouble(age, stage, N, st)); break;
         case CLB_FIELD_PHASE: HIPT(launchI8ToDouble(phase, stage, N, st)); break;
@@ -637,16 +690,20 @@ struct Case : CaseBase {
             HIPT(hipMemcpyAsync(Btot, tot.data(), N * sizeof(R), hipMemcpyHostToDevice, st));
         } else if (field == CLB_FIELD_VEL) {
             if (index < 0 || index > 2) return {CLB_E_ARG, "velocity component out of range"};
-            HIPT(launchFromDouble<R>(stage, u + (size_t) index * N, N, st));
+            HIPT(launchGatherCompact<R>(stage, list, Nopen, u + (size_t) index * Nopen, st));
         } else return {CLB_E_ARG, "field cannot be written"};
         HIPT(hipStreamSynchronize(st));
         return {CLB_OK, ""};
     }
     void speciesTotals(hipStream_t st, double *out) override {
-        const unsigned int g = nblk(N, 256);
         for (int s = 0; s < p.n_species; ++s) {
-            HIPT(launchSpeciesTotal<R>(ade(), s, parts, st));
-            HIPT(launchFinalSum(parts, g, 1, 0, 0, scal + s, st));
+            if (p.mobile[s]) {
+                HIPT(launchAdeTotal<R>(ade(), slot[s], parts, st));
+                HIPT(launchFinalSum(parts, nblk(Nint, 256), 1, 0, 0, scal + s, st));
+            } else {
+                HIPT(launchScalarTotal<R>(nx, ny, N, mask, imm + (size_t) slot[s] * N, parts, st));
+                HIPT(launchFinalSum(parts, nblk(N, 256), 1, 0, 0, scal + s, st));
+            }
         }
         HIPT(hipMemcpyAsync(out, scal, p.n_species * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPT(hipStreamSynchronize(st));

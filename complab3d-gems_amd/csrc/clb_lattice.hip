@@ -1,0 +1,825 @@
+// clb_lattice.hip -- the lattice kernels of the coupled step on the compact
+// open-voxel layout (gfx950).
+//
+// Only open voxels (geometry pore or biofilm) carry populations, one thread
+// each, in three ranges: interior, x = 0 plane, x = nx-1 plane (clb_device.h).
+// Populations are structure-of-arrays: f[slot][q][i] (ADE, full form, density
+// = sum f) and g[q][i] (D3Q19, shifted form, density = 1 + sum g).
+//
+// Streaming is a pull: a launch reads the post-collision populations of the
+// previous launch (`cur`) from its neighbours, through a neighbour table, and
+// writes its own post-collision populations to `out`.  Bounce-back cells
+// have no storage and no thread: a population that would come from one is the
+// voxel's own opposite population from two launches back, still in `out`
+// until this thread overwrites it.  That is the full-way bounce-back of a
+// Palabos BounceBack cell (swap on collide, stream back out) without the cell.
+// A launch pulls with the classes of the previous launch (`clsPull`) and
+// collides with the current ones (`cls`), so a voxel that closes or reopens
+// between two launches streams exactly as the reference's in-step change does.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "clb_device.h"
+#include "clb_common.h"
+#include "clb_eq_device.h"
+
+namespace clb {
+
+// ------------------------------------------------------------------------------------------
+// helpers
+// ------------------------------------------------------------------------------------------
+// compact neighbour n (>= 0) closed for a species: bounce-back class, or a
+// closed x plane (complab_functions.hh:431-432)
+__device__ __forceinline__ bool closedFor(const uint8_t *cls, int n, int Nint, int Nleft, int lbt, int rbt) {
+    if (cls[n] == ADE_BB) return true;
+    if (n >= Nint) return (n < Nint + Nleft) ? (lbt == BC_CLOSED_DEV) : (rbt == BC_CLOSED_DEV);
+    return false;
+}
+
+template <typename R, typename T>
+__device__ __forceinline__ void pull7(const AdeDev<R> &a, const R *__restrict__ cur, const R *__restrict__ old,
+                                      int i, int lbt, int rbt, T fl[7]) {
+    const long long No = a.Nopen;
+    if (a.first) {
+#pragma unroll
+        for (int q = 0; q < 7; ++q) fl[q] = (T) cur[q * No + i];
+        return;
+    }
+    fl[0] = (T) cur[i];
+#pragma unroll
+    for (int q = 1; q < 7; ++q) {
+        const int n = a.nb[(q - 1) * No + i];
+        if (n == NB_OUTSIDE) fl[q] = (T) cur[opp7(q) * No + i];               // edge: own opposite
+        else if (n == NB_CLOSED || closedFor(a.clsPull, n, a.Nint, a.Nleft, lbt, rbt))
+            fl[q] = (T) old[opp7(q) * No + i];                                // bounce-back
+        else fl[q] = (T) cur[q * No + n];
+    }
+}
+
+// first launch after (re)initialisation: the state is pre-collision, and a
+// closed neighbour holds the species' rest population w c0; hand it to the
+// slot the bounce-back rule reads at the next launch
+template <typename R>
+__device__ __forceinline__ void firstHandOver(const AdeDev<R> &a, R *cur, int i, int lbt, int rbt, double c0) {
+    const long long No = a.Nopen;
+#pragma unroll
+    for (int q = 1; q < 7; ++q) {
+        const int n = a.nb[(q - 1) * No + i];
+        if (n == NB_CLOSED || (n >= 0 && closedFor(a.clsPull, n, a.Nint, a.Nleft, lbt, rbt)))
+            cur[opp7(q) * No + i] = (R) (W7F * c0);
+    }
+}
+
+// BGK with the first-order advection-diffusion equilibrium w_i rho (1 + c_i.u / cs^2),
+// in the compute type T (double in FP64 mode, float in FP32 mode)
+template <typename T>
+__device__ __forceinline__ void bgk7(T f[7], T om, T ux, T uy, T uz) {
+    T r = 0;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) r += f[q];
+    const T t0 = (T) 4 * r;
+    const T c1 = (T) 1 - om;
+    const T wr = (T) W7R, wf = (T) W7F;
+    f[0] = f[0] * c1 + om * (wr * r);
+    f[1] = f[1] * c1 + om * (wf * (r + t0 * ux));
+    f[2] = f[2] * c1 + om * (wf * (r - t0 * ux));
+    f[3] = f[3] * c1 + om * (wf * (r + t0 * uy));
+    f[4] = f[4] * c1 + om * (wf * (r - t0 * uy));
+    f[5] = f[5] * c1 + om * (wf * (r + t0 * uz));
+    f[6] = f[6] * c1 + om * (wf * (r - t0 * uz));
+}
+
+// regularized first-order advection-diffusion Dirichlet completion
+template <typename T>
+__device__ __forceinline__ void adeDirichlet(T fl[7], int atLeft, T rhoD) {
+    const int unk = atLeft ? 1 : 2;
+    T s = 0;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) if (q != unk) s += fl[q];
+    fl[unk] = rhoD - s;
+    const T jx = fl[1] - fl[2], jy = fl[3] - fl[4], jz = fl[5] - fl[6];
+    const T wr = (T) W7R, wf = (T) W7F, c = (T) 4;
+    fl[0] = wr * rhoD;
+    fl[1] = wf * (rhoD + c * jx);
+    fl[2] = wf * (rhoD - c * jx);
+    fl[3] = wf * (rhoD + c * jy);
+    fl[4] = wf * (rhoD - c * jy);
+    fl[5] = wf * (rhoD + c * jz);
+    fl[6] = wf * (rhoD - c * jz);
+}
+
+// ------------------------------------------------------------------------------------------
+// kinetics (the rate laws of the reference's compiled-kinetics headers)
+// ------------------------------------------------------------------------------------------
+// dual Monod, config/kinetics/defineKinetics.biotic.hh:90-126
+__device__ void bioticLaw(const KinDev &k, const double *C, const double *B, double *sr, double *br, int mask) {
+    for (int s = 0; s < k.S; ++s) sr[s] = 0;
+    for (int m = 0; m < k.M; ++m) br[m] = 0;
+    if (mask < 2) return;
+    const double S = k.donor >= 0 ? fmax(C[k.donor], 0.0) : 0.0;
+    const double E = k.acceptor >= 0 ? fmax(C[k.acceptor], 0.0) : 1.0;
+    for (int m = 0; m < k.M; ++m) {
+        const double Bm = fmax(B[m], 0.0);
+        if (Bm <= 0.0) continue;
+        double mu = k.mu_max[m] * (S / (k.ks[m] + S));
+        if (k.acceptor >= 0) mu *= E / (k.ke[m] + E);
+        const double growth = mu * Bm;
+        br[m] += growth - k.kdecay[m] * Bm;
+        for (int s = 0; s < k.S; ++s) sr[s] += k.bio_stoich[m][s] * growth;
+    }
+    for (int s = 0; s < k.S; ++s) if (!isfinite(sr[s])) sr[s] = 0;
+    for (int m = 0; m < k.M; ++m) if (!isfinite(br[m])) br[m] = 0;
+}
+
+// mass action with the ABC per-step cap, ABC_sweep_package/kinetics/defineAbioticKinetics.hh:166-210
+__device__ void abioticLaw(const KinDev &k, const double *C, double *sr) {
+    for (int s = 0; s < k.S; ++s) sr[s] = 0;
+    for (int r = 0; r < k.nrxn; ++r) {
+        const int r1 = k.r1[r], r2 = k.r2[r];
+        const double a = r1 >= 0 ? fmax(C[r1], 0.0) : 1.0;
+        const double b = r2 >= 0 ? fmax(C[r2], 0.0) : 1.0;
+        if (k.min_conc > 0 && ((r1 >= 0 && a <= k.min_conc) || (r2 >= 0 && b <= k.min_conc))) continue;
+        double R = k.rk[r] * a * b;
+        if (k.cap_dt > 0 && k.cap_frac > 0) {
+            const double sc = fmin(r1 >= 0 ? a : 1e300, r2 >= 0 ? b : 1e300);
+            const double cap = sc * k.cap_frac / k.cap_dt;
+            if (R > cap) R = cap;
+        }
+        for (int s = 0; s < k.S; ++s) sr[s] += k.rxn_stoich[r][s] * R;
+    }
+    for (int s = 0; s < k.S; ++s) if (!isfinite(sr[s])) sr[s] = 0;
+    if (k.rate_scale != 1.0) for (int s = 0; s < k.S; ++s) sr[s] *= k.rate_scale;
+}
+
+// the clamp dC = max(R dt, -C), only for a debit and only against what is there
+__device__ __forceinline__ double clampInc(int clamp, double d, double cur, unsigned int &binds) {
+    if (!clamp || d >= 0) return d;
+    const double lim = -fmax(cur, 0.0);
+    if (d < lim) { ++binds; return lim; }
+    return d;
+}
+
+// ------------------------------------------------------------------------------------------
+// advection-diffusion
+// ------------------------------------------------------------------------------------------
+// Neumann planes: the density of the inner layer after this launch's stream
+// (FlatAdiabaticBoundaryFunctional3D); a bounce-back inner cell reports 0
+template <typename R>
+__global__ void k_ade_neumann(AdeDev<R> a) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int Np = a.Nopen - a.Nint;
+    if (p >= Np) return;
+    const int i = a.Nint + p;
+    const bool left = i < a.Nint + a.Nleft;
+    const int inner = a.nb[(left ? 1 : 0) * (long long) a.Nopen + i];
+    for (int ms = 0; ms < a.Sm; ++ms) {
+        const int s = a.mobSpecies[ms];
+        if ((left ? a.lbt[s] : a.rbt[s]) != BC_NEUMANN_DEV) continue;
+        double r = 0;
+        if (inner >= 0 && !closedFor(a.cls, inner, a.Nint, a.Nleft, a.lbt[s], a.rbt[s])) {
+            double fl[7];
+            pull7<R, double>(a, a.cur + (long long) ms * 7 * a.Nopen, a.out + (long long) ms * 7 * a.Nopen, inner, a.lbt[s], a.rbt[s], fl);
+            for (int q = 0; q < 7; ++q) r += fl[q];
+        }
+        a.rhoN[(long long) ms * Np + p] = r;
+    }
+}
+
+// interior: pull + BGK collide + biotic and abiotic kinetics with the clamp
+// and the D3Q7 deposit, every mobile species of a voxel in one thread
+template <typename R, int MAXS>
+__global__ void __launch_bounds__(256) k_ade_bulk(AdeDev<R> a, KinDev k) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.Nint) return;
+    const int cls = a.cls[i];
+    if (cls == ADE_BB) return;
+    const long long No = a.Nopen;
+    R f[MAXS][7];
+    const R ux = a.u[i], uy = a.u[No + i], uz = a.u[2 * No + i];
+#pragma unroll
+    for (int ms = 0; ms < MAXS; ++ms) {
+        if (ms >= a.Sm) continue;
+        const int s = a.mobSpecies[ms];
+        R *cur = a.cur + (long long) ms * 7 * No;
+        R *old = a.out + (long long) ms * 7 * No;
+        pull7<R, R>(a, cur, old, i, a.lbt[s], a.rbt[s], f[ms]);
+        if (a.first) firstHandOver<R>(a, cur, i, a.lbt[s], a.rbt[s], a.c0[s]);
+        if (a.reset) {
+            double c0 = a.c0[s];
+            if (c0 < 1e-12 && c0 > -1e-12) c0 = 0;
+#pragma unroll
+            for (int q = 0; q < 7; ++q) f[ms][q] = (R) (w7(q) * c0);
+        }
+        bgk7<R>(f[ms], (R) (cls == ADE_BFILM ? a.omB[s] : a.omP[s]), ux, uy, uz);
+    }
+
+    if (k.on) {
+        const long long v = a.list[i];
+        const int mask = a.mask[v];
+        if (mask != 0 && mask != 1) {
+            unsigned int binds = 0;
+            double C[CLB_DEV_MAXS], sr[CLB_DEV_MAXS];
+            const int S = k.S;
+            const long long N = a.N;
+            auto concOf = [&](int s) -> double {
+                const int sl = a.slot[s];
+                if (a.mobile[s]) {
+                    double r = 0;
+#pragma unroll
+                    for (int m2 = 0; m2 < MAXS; ++m2) if (m2 == sl) {
+#pragma unroll
+                        for (int q = 0; q < 7; ++q) r += (double) f[m2][q];
+                    }
+                    return r;
+                }
+                return (double) a.imm[(long long) sl * N + v];
+            };
+            auto depositTo = [&](int s, double d) {
+                const int sl = a.slot[s];
+                if (a.mobile[s]) {
+#pragma unroll
+                    for (int m2 = 0; m2 < MAXS; ++m2) if (m2 == sl) {
+#pragma unroll
+                        for (int q = 0; q < 7; ++q) f[m2][q] = (R) ((double) f[m2][q] + w7(q) * d);
+                    }
+                } else {
+                    R *p = a.imm + (long long) sl * N + v;
+                    *p = (R) ((double) *p + d);
+                }
+            };
+            if (k.bio && k.M > 0) {
+                double Bv[CLB_DEV_MAXM], br[CLB_DEV_MAXM];
+                for (int s = 0; s < S; ++s) { C[s] = concOf(s); if (C[s] < 1e-12) C[s] = 0; }
+                for (int m = 0; m < k.M; ++m) { Bv[m] = (double) a.B[(long long) m * N + v]; if (Bv[m] < 1e-12) Bv[m] = 0; }
+                bioticLaw(k, C, Bv, sr, br, mask);
+                for (int s = 0; s < S; ++s) {
+                    const double d = sr[s] * k.dt;
+                    if (d > 1e-12 || d < -1e-12) depositTo(s, clampInc(k.clamp, d, concOf(s), binds));
+                }
+                for (int m = 0; m < k.M; ++m) {
+                    double d = br[m] * k.dt;
+                    if (d > 1e-12 || d < -1e-12) {
+                        R *p = a.B + (long long) m * N + v;
+                        const double b = (double) *p;
+                        if (d < -b) d = -b;                               // biomass never negative
+                        *p = (R) (b + d);
+                    }
+                }
+            }
+            if (k.nrxn > 0) {
+                bool react = true;
+                if (k.surface_only) {                                     // precipitationVOP.hh:78-88
+                    react = false;
+#pragma unroll
+                    for (int q = 1; q < 7; ++q) {
+                        const int n = a.nb[(opp7(q) - 1) * No + i];        // the voxel at +c_q
+                        if (n == NB_OUTSIDE) continue;
+                        if (n == NB_CLOSED) { react = true; continue; }
+                        const int nm = a.mask[a.list[n]];
+                        if (nm == 0 || nm == 1) react = true;
+                    }
+                }
+                if (react) {
+                    for (int s = 0; s < S; ++s) { C[s] = concOf(s); if (C[s] < 1e-12) C[s] = 0; }
+                    abioticLaw(k, C, sr);
+                    for (int s = 0; s < S; ++s) {
+                        const double d = sr[s] * k.dt;
+                        if (d > 1e-12 || d < -1e-12) depositTo(s, clampInc(k.clamp, d, concOf(s), binds));
+                    }
+                }
+            }
+            if (binds) atomicAdd(a.clampCount, (unsigned long long) binds);
+        }
+    }
+
+#pragma unroll
+    for (int ms = 0; ms < MAXS; ++ms) {
+        if (ms >= a.Sm) continue;
+        R *o = a.out + (long long) ms * 7 * No;
+#pragma unroll
+        for (int q = 0; q < 7; ++q) o[q * No + i] = f[ms][q];
+    }
+}
+
+// the x planes: pull + regularized Dirichlet (or Neumann) completion + BGK
+template <typename R>
+__global__ void __launch_bounds__(256) k_ade_planes(AdeDev<R> a) {
+    const int i = a.Nint + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.Nopen) return;
+    const int cls = a.cls[i];
+    if (cls == ADE_BB) return;
+    const long long No = a.Nopen;
+    const bool left = i < a.Nint + a.Nleft;
+    const int Np = a.Nopen - a.Nint;
+    const R ux = a.u[i], uy = a.u[No + i], uz = a.u[2 * No + i];
+    for (int ms = 0; ms < a.Sm; ++ms) {
+        const int s = a.mobSpecies[ms];
+        const int bt = left ? a.lbt[s] : a.rbt[s];
+        if (bt == BC_CLOSED_DEV) continue;
+        R *cur = a.cur + (long long) ms * 7 * No;
+        R *old = a.out + (long long) ms * 7 * No;
+        R f[7];
+        pull7<R, R>(a, cur, old, i, a.lbt[s], a.rbt[s], f);
+        if (a.first) firstHandOver<R>(a, cur, i, a.lbt[s], a.rbt[s], a.c0[s]);
+        if (a.reset) {
+            double c0 = a.c0[s];
+            if (c0 < 1e-12 && c0 > -1e-12) c0 = 0;
+            for (int q = 0; q < 7; ++q) f[q] = (R) (w7(q) * c0);
+        }
+        double rhoD = left ? a.lbc[s] : a.rbc[s];
+        if (bt == BC_NEUMANN_DEV) rhoD = a.rhoN[(long long) ms * Np + (i - a.Nint)];
+        adeDirichlet<R>(f, left ? 1 : 0, (R) rhoD);
+        bgk7<R>(f, (R) (cls == ADE_BFILM ? a.omB[s] : a.omP[s]), ux, uy, uz);
+        for (int q = 0; q < 7; ++q) old[q * No + i] = f[q];
+    }
+}
+
+// density after the stream into a dense field (closed voxels keep 0)
+template <typename R>
+__global__ void k_ade_density(AdeDev<R> a, int ms, double *dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.Nopen) return;
+    const int s = a.mobSpecies[ms];
+    const long long v = a.list[i];
+    if (closedFor(a.cls, i, a.Nint, a.Nleft, a.lbt[s], a.rbt[s])) { dst[v] = 0.0; return; }
+    double r = 0;
+    if (a.reset) {
+        r = a.c0[s];
+        if (r < 1e-12 && r > -1e-12) r = 0;
+    } else {
+        double fl[7];
+        pull7<R, double>(a, a.cur + (long long) ms * 7 * a.Nopen, a.out + (long long) ms * 7 * a.Nopen, i, a.lbt[s], a.rbt[s], fl);
+        for (int q = 0; q < 7; ++q) r += fl[q];
+    }
+    dst[v] = r;
+}
+
+// populations at rest for a concentration field (closed voxels: the background c0)
+template <typename R>
+__global__ void k_ade_set(AdeDev<R> a, int ms, const double *src) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.Nopen) return;
+    const int s = a.mobSpecies[ms];
+    R *cur = a.cur + (long long) ms * 7 * a.Nopen;
+    const double c = closedFor(a.cls, i, a.Nint, a.Nleft, a.lbt[s], a.rbt[s]) ? a.c0[s] : src[a.list[i]];
+    for (int q = 0; q < 7; ++q) cur[q * (long long) a.Nopen + i] = (R) (w7(q) * c);
+}
+
+// a voxel reopened this step: rest population, and the rest value on the
+// links from neighbours that were closed at the last collide
+template <typename R>
+__global__ void k_ade_reopen(AdeDev<R> a, const uint8_t *reopened) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.Nopen || !reopened[i]) return;
+    const long long No = a.Nopen;
+    for (int ms = 0; ms < a.Sm; ++ms) {
+        const int s = a.mobSpecies[ms];
+        R *cur = a.cur + (long long) ms * 7 * No;
+        R *old = a.out + (long long) ms * 7 * No;
+        cur[i] = (R) (W7R * a.c0[s]);
+        for (int q = 1; q < 7; ++q) {
+            const int n = a.nb[(q - 1) * No + i];
+            const bool closedSrc = n < 0 || closedFor(a.clsPull, n, a.Nint, a.Nleft, a.lbt[s], a.rbt[s]);
+            if (closedSrc) old[opp7(q) * No + i] = (R) (W7F * a.c0[s]);
+        }
+    }
+}
+
+// total of a mobile species over the open interior (x = 1..nx-2, live mask open)
+template <typename R>
+__global__ void __launch_bounds__(256) k_ade_total(AdeDev<R> a, int ms, double *parts) {
+    __shared__ double sS[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0;
+    if (i < a.Nint) {
+        const int s = a.mobSpecies[ms];
+        const int m = a.mask[a.list[i]];
+        if (m != 0 && m != 1 && !closedFor(a.cls, i, a.Nint, a.Nleft, a.lbt[s], a.rbt[s])) {
+            if (a.reset) { v = a.c0[s]; if (v < 1e-12 && v > -1e-12) v = 0; }
+            else {
+                double fl[7];
+                pull7<R, double>(a, a.cur + (long long) ms * 7 * a.Nopen, a.out + (long long) ms * 7 * a.Nopen, i, a.lbt[s], a.rbt[s], fl);
+                for (int q = 0; q < 7; ++q) v += fl[q];
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) sS[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) { double s2 = 0; for (int w = 0; w < (int) (blockDim.x >> 6); ++w) s2 += sS[w]; parts[blockIdx.x] = s2; }
+}
+
+// ------------------------------------------------------------------------------------------
+// D3Q19 incompressible BGK (IncBGKdynamics) with regularized pressure planes
+// ------------------------------------------------------------------------------------------
+template <typename R, typename T>
+__device__ __forceinline__ void pull19(const NsDev<R> &a, int i, T fl[19]) {
+    const long long No = a.Nopen;
+    if (a.first) {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) fl[q] = (T) a.cur[q * No + i];
+        return;
+    }
+    fl[0] = (T) a.cur[i];
+#pragma unroll
+    for (int q = 1; q < 19; ++q) {
+        const int n = a.nb[(q - 1) * No + i];
+        if (n == NB_OUTSIDE) fl[q] = (T) a.cur[opp19(q) * No + i];
+        else if (n == NB_CLOSED || a.clsPull[n] >= NS_BB) fl[q] = (T) a.out[opp19(q) * No + i];
+        else fl[q] = (T) a.cur[q * No + n];
+    }
+}
+template <typename R>
+__device__ __forceinline__ void nsFirstHandOver(const NsDev<R> &a, int i) {
+    const long long No = a.Nopen;
+#pragma unroll
+    for (int q = 1; q < 19; ++q) {
+        const int n = a.nb[(q - 1) * No + i];
+        if (n == NB_CLOSED || (n >= 0 && a.clsPull[n] >= NS_BB)) a.cur[opp19(q) * No + i] = (R) 0;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T feqInc(int q, T rb, T jx, T jy, T jz, T jsq) {
+    const T cj = (T) D19C[q][0] * jx + (T) D19C[q][1] * jy + (T) D19C[q][2] * jz;
+    return (T) w19(q) * (rb + (T) 3 * cj + (T) 4.5 * cj * cj - (T) 1.5 * jsq);
+}
+
+// dir = +1 at x = 0 (unknowns c_x = +1), -1 at x = nx-1; tangential momentum 0
+__device__ void nsPressurePlane(double fl[19], int dir, double rb, double &jx) {
+    double r0 = 0, rout = 0;
+    for (int q = 0; q < 19; ++q) {
+        if (D19C[q][0] == 0) r0 += fl[q];
+        else if (D19C[q][0] == -dir) rout += fl[q];
+    }
+    jx = dir * (rb - r0 - 2.0 * rout);
+    const double jsq = jx * jx;
+    double feq[19];
+    for (int q = 0; q < 19; ++q) feq[q] = feqInc<double>(q, rb, jx, 0.0, 0.0, jsq);
+    for (int q = 0; q < 19; ++q) if (D19C[q][0] == dir) fl[q] = feq[q] + (fl[opp19(q)] - feq[opp19(q)]);
+    double P[3][3] = {{0,0,0},{0,0,0},{0,0,0}};
+    for (int q = 0; q < 19; ++q) {
+        const double ne = fl[q] - feq[q];
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) P[i][j] += D19C[q][i] * D19C[q][j] * ne;
+    }
+    for (int q = 0; q < 19; ++q) {
+        double s = 0;
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) s += (D19C[q][i] * D19C[q][j] - (i == j ? 1.0 / 3.0 : 0.0)) * P[i][j];
+        fl[q] = feq[q] + w19(q) * 4.5 * s;
+    }
+}
+
+__device__ __forceinline__ void blockEnergy(double e, double cnt, double *partE, double *partN, int slot) {
+    __shared__ double sE[4], sN[4];
+    for (int off = 32; off > 0; off >>= 1) { e += __shfl_down(e, off, 64); cnt += __shfl_down(cnt, off, 64); }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { sE[wv] = e; sN[wv] = cnt; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double E = 0, C = 0;
+        for (int w = 0; w < (int) (blockDim.x >> 6); ++w) { E += sE[w]; C += sN[w]; }
+        partE[slot] = E; partN[slot] = C;
+    }
+}
+
+// interior: pull + BGK; the energy statistic |j|^2/2 per block
+template <typename R>
+__global__ void __launch_bounds__(256) k_ns_bulk(NsDev<R> a, double *partE, double *partN) {
+    if (*a.done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0, cnt = 0;
+    if (i < a.Nint) {
+        const int cls = a.cls[i];
+        if (cls < NS_BB) {
+            R fl[19];                                  // compute type = storage type
+            pull19<R, R>(a, i, fl);
+            if (a.first) nsFirstHandOver<R>(a, i);
+            R rb = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+            for (int q = 0; q < 19; ++q) {
+                rb += fl[q];
+                jx += (R) D19C[q][0] * fl[q]; jy += (R) D19C[q][1] * fl[q]; jz += (R) D19C[q][2] * fl[q];
+            }
+            const R jsq = jx * jx + jy * jy + jz * jz;
+            e = 0.5 * (double) jsq; cnt = 1;
+            const R om = (R) (cls == NS_BIO ? a.omBio : a.om);
+            const long long No = a.Nopen;
+#pragma unroll
+            for (int q = 0; q < 19; ++q) a.out[q * No + i] = fl[q] + om * (feqInc<R>(q, rb, jx, jy, jz, jsq) - fl[q]);
+        }
+    }
+    blockEnergy(e, cnt, partE, partN, blockIdx.x);
+}
+
+// the pressure planes
+template <typename R>
+__global__ void __launch_bounds__(256) k_ns_planes(NsDev<R> a, double *partE, double *partN, int slot0) {
+    if (*a.done) return;
+    const int i = a.Nint + blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0, cnt = 0;
+    if (i < a.Nopen) {
+        const int cls = a.cls[i];
+        if (cls < NS_BB) {
+            double fl[19];
+            pull19<R, double>(a, i, fl);
+            if (a.first) nsFirstHandOver<R>(a, i);
+            const bool left = i < a.Nint + a.Nleft;
+            const double rb = left ? 0.0 : a.rhoBarRight;
+            double jx;
+            nsPressurePlane(fl, left ? +1 : -1, rb, jx);
+            const double jsq = jx * jx;
+            e = 0.5 * jsq; cnt = 1;
+            const double om = cls == NS_BIO ? a.omBio : a.om;
+            const long long No = a.Nopen;
+            for (int q = 0; q < 19; ++q) a.out[q * No + i] = (R) (fl[q] + om * (feqInc<double>(q, rb, jx, 0.0, 0.0, jsq) - fl[q]));
+        }
+    }
+    blockEnergy(e, cnt, partE, partN, slot0 + blockIdx.x);
+}
+
+// the convergence rule, evaluated on the device after every iteration
+__global__ void k_ns_tracer(NsTracer *t, const double *partE, const double *partN, int nparts, int rule, double eps, int *done) {
+    __shared__ double sE[64], sN[64];
+    if (*done) return;
+    const int tid = threadIdx.x;   // 64 threads
+    double E = 0, C = 0;
+    for (int i = tid; i < nparts; i += 64) { E += partE[i]; C += partN[i]; }
+    sE[tid] = E; sN[tid] = C;
+    __syncthreads();
+    if (tid == 0) {
+        double Es = 0, Cs = 0;
+        for (int i = 0; i < 64; ++i) { Es += sE[i]; Cs += sN[i]; }
+        const double val = Cs > 0 ? Es / Cs : 0.0;
+        t->iters += 1;
+        t->last = val;
+        if (!isfinite(val)) { t->nonfinite = 1; *done = 1; return; }
+        if (rule == 0) {                      // ValueTracer(1, 1000, eps): window 500
+            const int W = NS_TRACER_WINDOW;
+            t->ring[t->head] = val;
+            t->head = (t->head + 1) % W;
+            if (t->count < W) t->count += 1;
+            if (t->count >= W) {
+                double m = 0;
+                for (int i = 0; i < W; ++i) m += t->ring[(t->head + i) % W];
+                m /= (double) W;
+                double s = 0;
+                for (int i = 0; i < W; ++i) { const double d = t->ring[(t->head + i) % W] - m; s += d * d; }
+                s = sqrt(s / (double) W);
+                const double r = s / fabs(m);
+                if (isnan(r) || r < eps) { t->converged = 1; *done = 1; }
+            }
+        } else {
+            if (t->prevE > 0 && fabs(val - t->prevE) <= eps * fabs(val)) { t->converged = 1; *done = 1; }
+            t->prevE = val;
+        }
+    }
+}
+
+// velocity of the current flow state (computeVelocity) into the compact field
+// read by the ADE lattices; optionally the box reductions of the Peclet rescale
+template <typename R>
+__global__ void __launch_bounds__(256) k_ns_velocity(NsDev<R> a, R *u, double *parts) {
+    __shared__ double red[4][4], r4[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[5] = {0, 0, 0, 0, 0};      // sum|u| box, max ux box, sum ux outlet, sum|u| open box, count open box
+    bool inBox = false;
+    if (i < a.Nopen) {
+        double ux = 0, uy = 0, uz = 0;
+        if (a.cls[i] < NS_BB) {
+            double fl[19];
+            pull19<R, double>(a, i, fl);
+            if (i >= a.Nint) {
+                const bool left = i < a.Nint + a.Nleft;
+                nsPressurePlane(fl, left ? +1 : -1, left ? 0.0 : a.rhoBarRight, ux);
+            } else {
+                for (int q = 0; q < 19; ++q) { ux += D19C[q][0] * fl[q]; uy += D19C[q][1] * fl[q]; uz += D19C[q][2] * fl[q]; }
+            }
+        }
+        if (u) { u[i] = (R) ux; u[a.Nopen + i] = (R) uy; u[2 * (long long) a.Nopen + i] = (R) uz; }
+        inBox = i < a.Nint;
+        const double n = sqrt(ux * ux + uy * uy + uz * uz);
+        if (inBox) {
+            v[0] = n; v[1] = ux; v[3] = n; v[4] = 1;
+            const int x = (int) (a.list[i] % a.nx);
+            if (x == a.nx - 2) v[2] = ux;
+        }
+    }
+    if (!parts) return;
+    if (!inBox) v[1] = -1e300;
+    for (int off = 32; off > 0; off >>= 1) {
+        v[0] += __shfl_down(v[0], off, 64);
+        v[1] = fmax(v[1], __shfl_down(v[1], off, 64));
+        v[2] += __shfl_down(v[2], off, 64);
+        v[3] += __shfl_down(v[3], off, 64);
+        v[4] += __shfl_down(v[4], off, 64);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { red[0][wv] = v[0]; red[1][wv] = v[1]; red[2][wv] = v[2]; red[3][wv] = v[3]; r4[wv] = v[4]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0, s1 = -1e300, s2 = 0, s3 = 0, s4 = 0;
+        for (int w = 0; w < (int) (blockDim.x >> 6); ++w) { s0 += red[0][w]; s1 = fmax(s1, red[1][w]); s2 += red[2][w]; s3 += red[3][w]; s4 += r4[w]; }
+        parts[blockIdx.x * 5 + 0] = s0; parts[blockIdx.x * 5 + 1] = s1; parts[blockIdx.x * 5 + 2] = s2;
+        parts[blockIdx.x * 5 + 3] = s3; parts[blockIdx.x * 5 + 4] = s4;
+    }
+}
+
+// initializeAtEquilibrium(PressureGradient): rho = 1 - 3 dP x/(nx-1), u = 0
+template <typename R>
+__global__ void k_ns_init(NsDev<R> a, double deltaP) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.Nopen) return;
+    const int x = (int) (a.list[i] % a.nx);
+    const double rb = -deltaP * 3.0 / (double) (a.nx - 1) * (double) x;
+    const bool closed = a.cls[i] >= NS_BB;
+    for (int q = 0; q < 19; ++q) {
+        const R v = closed ? (R) 0 : (R) (w19(q) * rb);
+        a.cur[q * (long long) a.Nopen + i] = v;
+        a.out[q * (long long) a.Nopen + i] = v;
+    }
+}
+
+// updateNsLatticesDynamics3D (part2:748-818) from the live mask
+__global__ void k_ns_update_dyn(const int8_t *mask, const int *list, uint8_t *cls, int Nopen, double permRatio, unsigned int *changed) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nopen) return;
+    const int m = mask[list[i]];
+    if (m == 0 || m == 1) return;
+    const int c = cls[i];
+    const bool isNsOmega = (c == NS_BGK);
+    int nc = c;
+    if (m != 2 && isNsOmega) nc = permRatio <= 1e-14 ? NS_BB : NS_BIO;
+    else if (m == 2 && !isNsOmega) nc = NS_BGK;
+    if (nc != c) { cls[i] = (uint8_t) nc; atomicAdd(changed, 1u); }
+}
+// a cell reopened in the flow lattice: rest (0) on the links from neighbours
+// that were closed at the last collide
+template <typename R>
+__global__ void k_ns_reopen(NsDev<R> a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.Nopen) return;
+    if (!(a.clsPull[i] >= NS_BB && a.cls[i] < NS_BB)) return;
+    const long long No = a.Nopen;
+    a.cur[i] = (R) 0;
+    for (int q = 1; q < 19; ++q) {
+        const int n = a.nb[(q - 1) * No + i];
+        if (n < 0 || a.clsPull[n] >= NS_BB) a.out[opp19(q) * No + i] = (R) 0;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// equilibrium speciation (FP64, one thread per open interior voxel)
+// ------------------------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(128) k_equilibrium(AdeDev<R> a, EqDev e, unsigned int *nonconv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.Nint) return;
+    const long long v = a.list[i];
+    const int mask = a.mask[v];
+    if (mask == 0 || mask == 1) return;
+    const long long No = a.Nopen, N = a.N;
+    const int S = e.ns;
+    double C[EQ_MAXS], E[EQ_MAXS];
+    for (int s = 0; s < S; ++s) {
+        double r;
+        if (a.mobile[s]) {
+            const R *o = a.cur + (long long) a.slot[s] * 7 * No + i;
+            r = 0; for (int q = 0; q < 7; ++q) r += (double) o[q * No];
+        } else r = (double) a.imm[(long long) a.slot[s] * N + v];
+        C[s] = fmax(r, EQ_MIN_CONC);
+    }
+    const bool conv = eqSolve(e, C, E);
+    if (!conv) atomicAdd(nonconv, 1u);
+    for (int s = 0; s < S; ++s) {
+        double dC = E[s] - C[s];
+        const double mx = fmax(1e-9, 0.1 * fabs(C[s]));                 // part4:1472-1485
+        if (fabs(dC) > mx) dC = dC > 0 ? mx : -mx;
+        if (C[s] + dC < 1e-20) dC = 1e-20 - C[s];
+        if (fabs(dC) > 1e-12) {
+            if (a.mobile[s]) {
+                R *o = a.cur + (long long) a.slot[s] * 7 * No + i;
+                for (int q = 0; q < 7; ++q) o[q * No] = (R) ((double) o[q * No] + w7(q) * dC);
+            } else {
+                R *p = a.imm + (long long) a.slot[s] * N + v;
+                *p = (R) ((double) *p + dC);
+            }
+        }
+    }
+}
+
+// compact <-> dense staging
+template <typename R>
+__global__ void k_scatter(const R *src, const int *list, int n, double *dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[list[i]] = (double) src[i];
+}
+template <typename R>
+__global__ void k_gather(const double *src, const int *list, int n, R *dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (R) src[list[i]];
+}
+
+// ------------------------------------------------------------------------------------------
+// host launchers
+// ------------------------------------------------------------------------------------------
+static inline unsigned int nblk(long long n, int b) { return (unsigned int) ((n + b - 1) / b); }
+
+unsigned int nsStepBlocks(int Nopen, int Nint) { return nblk(Nint, 256) + nblk(Nopen - Nint, 256); }
+
+template <typename R>
+hipError_t launchAdeStep(const AdeDev<R> &a, const KinDev &k, hipStream_t st) {
+    const int Np = a.Nopen - a.Nint;
+    if (a.anyNeumann && Np > 0) hipLaunchKernelGGL(k_ade_neumann<R>, dim3(nblk(Np, 256)), dim3(256), 0, st, a);
+    if (a.Nint > 0) {
+        const unsigned int g = nblk(a.Nint, 256);
+        if (a.Sm <= 1) hipLaunchKernelGGL((k_ade_bulk<R, 1>), dim3(g), dim3(256), 0, st, a, k);
+        else if (a.Sm <= 2) hipLaunchKernelGGL((k_ade_bulk<R, 2>), dim3(g), dim3(256), 0, st, a, k);
+        else if (a.Sm <= 3) hipLaunchKernelGGL((k_ade_bulk<R, 3>), dim3(g), dim3(256), 0, st, a, k);
+        else if (a.Sm <= 4) hipLaunchKernelGGL((k_ade_bulk<R, 4>), dim3(g), dim3(256), 0, st, a, k);
+        else if (a.Sm <= 6) hipLaunchKernelGGL((k_ade_bulk<R, 6>), dim3(g), dim3(256), 0, st, a, k);
+        else hipLaunchKernelGGL((k_ade_bulk<R, 8>), dim3(g), dim3(256), 0, st, a, k);
+    }
+    if (Np > 0) hipLaunchKernelGGL(k_ade_planes<R>, dim3(nblk(Np, 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchAdeDensity(const AdeDev<R> &a, int ms, double *dst, hipStream_t st) {
+    hipLaunchKernelGGL(k_ade_density<R>, dim3(nblk(a.Nopen, 256)), dim3(256), 0, st, a, ms, dst);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchAdeSet(const AdeDev<R> &a, int ms, const double *src, hipStream_t st) {
+    hipLaunchKernelGGL(k_ade_set<R>, dim3(nblk(a.Nopen, 256)), dim3(256), 0, st, a, ms, src);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchAdeReopen(const AdeDev<R> &a, const uint8_t *re, hipStream_t st) {
+    hipLaunchKernelGGL(k_ade_reopen<R>, dim3(nblk(a.Nopen, 256)), dim3(256), 0, st, a, re);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchAdeTotal(const AdeDev<R> &a, int ms, double *parts, hipStream_t st) {
+    hipLaunchKernelGGL(k_ade_total<R>, dim3(nblk(a.Nint, 256)), dim3(256), 0, st, a, ms, parts);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchNsStep(const NsDev<R> &a, double *partE, double *partN, NsTracer *t, int rule, double eps, hipStream_t st) {
+    const unsigned int gB = nblk(a.Nint, 256), gP = nblk(a.Nopen - a.Nint, 256);
+    if (gB) hipLaunchKernelGGL(k_ns_bulk<R>, dim3(gB), dim3(256), 0, st, a, partE, partN);
+    if (gP) hipLaunchKernelGGL(k_ns_planes<R>, dim3(gP), dim3(256), 0, st, a, partE, partN, (int) gB);
+    hipLaunchKernelGGL(k_ns_tracer, dim3(1), dim3(64), 0, st, t, partE, partN, (int) (gB + gP), rule, eps, a.done);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchNsVelocity(const NsDev<R> &a, R *u, double *parts, hipStream_t st) {
+    hipLaunchKernelGGL(k_ns_velocity<R>, dim3(nblk(a.Nopen, 256)), dim3(256), 0, st, a, u, parts);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchNsInit(const NsDev<R> &a, double dP, hipStream_t st) {
+    hipLaunchKernelGGL(k_ns_init<R>, dim3(nblk(a.Nopen, 256)), dim3(256), 0, st, a, dP);
+    return hipGetLastError();
+}
+hipError_t launchNsUpdateDyn(const int8_t *mask, const int *list, uint8_t *cls, int Nopen, double pr, unsigned int *changed, hipStream_t st) {
+    hipLaunchKernelGGL(k_ns_update_dyn, dim3(nblk(Nopen, 256)), dim3(256), 0, st, mask, list, cls, Nopen, pr, changed);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchNsReopen(const NsDev<R> &a, const uint8_t *, hipStream_t st) {
+    hipLaunchKernelGGL(k_ns_reopen<R>, dim3(nblk(a.Nopen, 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchEquilibrium(const AdeDev<R> &a, const EqDev &e, unsigned int *nonconv, hipStream_t st) {
+    hipLaunchKernelGGL(k_equilibrium<R>, dim3(nblk(a.Nint, 128)), dim3(128), 0, st, a, e, nonconv);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchScatterCompact(const R *src, const int *list, int n, double *dst, hipStream_t st) {
+    hipLaunchKernelGGL(k_scatter<R>, dim3(nblk(n, 256)), dim3(256), 0, st, src, list, n, dst);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchGatherCompact(const double *src, const int *list, int n, R *dst, hipStream_t st) {
+    hipLaunchKernelGGL(k_gather<R>, dim3(nblk(n, 256)), dim3(256), 0, st, src, list, n, dst);
+    return hipGetLastError();
+}
+
+#define INST(R) \
+    template hipError_t launchAdeStep<R>(const AdeDev<R> &, const KinDev &, hipStream_t); \
+    template hipError_t launchAdeDensity<R>(const AdeDev<R> &, int, double *, hipStream_t); \
+    template hipError_t launchAdeSet<R>(const AdeDev<R> &, int, const double *, hipStream_t); \
+    template hipError_t launchAdeReopen<R>(const AdeDev<R> &, const uint8_t *, hipStream_t); \
+    template hipError_t launchAdeTotal<R>(const AdeDev<R> &, int, double *, hipStream_t); \
+    template hipError_t launchNsStep<R>(const NsDev<R> &, double *, double *, NsTracer *, int, double, hipStream_t); \
+    template hipError_t launchNsVelocity<R>(const NsDev<R> &, R *, double *, hipStream_t); \
+    template hipError_t launchNsInit<R>(const NsDev<R> &, double, hipStream_t); \
+    template hipError_t launchNsReopen<R>(const NsDev<R> &, const uint8_t *, hipStream_t); \
+    template hipError_t launchEquilibrium<R>(const AdeDev<R> &, const EqDev &, unsigned int *, hipStream_t); \
+    template hipError_t launchScatterCompact<R>(const R *, const int *, int, double *, hipStream_t); \
+    template hipError_t launchGatherCompact<R>(const double *, const int *, int, R *, hipStream_t);
+INST(double)
+INST(float)
+
+}  // namespace clb

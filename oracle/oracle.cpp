@@ -1211,7 +1211,13 @@ int orc_write_field(void *h, int field, int index, const double *src) {
     if (field == CLB_FIELD_CONC) {
         if (index < 0 || index >= c.p.n_species) return CLB_E_ARG;
         if (c.p.mobile[index]) {
-            for (int q = 0; q < Q7; ++q) for (size_t i = 0; i < N; ++i) c.f[index][(size_t) q * N + i] = W7[q] * src[i];
+            /* open voxels take the field at rest; bounce-back voxels keep what they
+             * hold (the background c0 they were seeded with) */
+            for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 0; x < c.nx; ++x) {
+                size_t i = c.id(x, y, z);
+                if (adeClassFor(c, index, x, i) == ADE_BB) continue;
+                for (int q = 0; q < Q7; ++q) c.f[index][(size_t) q * N + i] = W7[q] * src[i];
+            }
             // a Neumann plane imposes the density its inner layer holds now
             for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) {
                 size_t k = (size_t) y + (size_t) c.ny * z;

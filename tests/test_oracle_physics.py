@@ -26,6 +26,7 @@ def channel_params(nx, species=1):
 def test_diffusion_mode_decay_cs2_quarter():
     nx = 66                                     # fluid x = 1..64, closed planes at 0 and 65
     p = channel_params(nx)
+    p.c0[0] = 1.0                               # the closed planes hold the background
     geom = np.full((1, 1, nx), P.PORE, np.int8)
     o = O.OracleCase(p, geom)
     L = nx - 2
@@ -41,7 +42,8 @@ def test_diffusion_mode_decay_cs2_quarter():
     D = 0.25 * (1.0 / 1.0 - 0.5)              # cs^2 (tau - 1/2), cs^2 = 1/4
     expect = 0.1 * np.exp(-D * k * k * T)
     assert abs(amp / expect - 1) < 5e-3
-    assert abs(r.mean() - 1.0) < 1e-12          # closed box keeps its mass
+    total = o.read(P.FIELD_CONC, 0).sum()       # fluid and the two bounce-back planes
+    assert abs(total - (c[1:-1].sum() + 2.0)) < 1e-12
 
 
 def test_closed_box_conserves_mass_with_flow():
