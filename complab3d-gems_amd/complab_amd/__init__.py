@@ -62,6 +62,7 @@ def lib():
         "clb_species_totals": ([V, I, V], I),
         "clb_counters": ([V, I, V], I),
         "clb_kernel_time": ([V, I, I, C.POINTER(D), C.POINTER(C.c_int64)], I),
+        "clb_stream_copy": ([V, C.c_size_t, I, C.POINTER(D), C.POINTER(D)], I),
         "clb_rng_draws": ([C.c_uint64, I, V], I),
         "clb_geom_spheres": ([I, I, I, D, I, C.c_uint64, I, V], I),
         "clb_geom_pack": ([I, I, I, D, D, C.c_uint64, I, V, C.POINTER(D)], I),
@@ -82,6 +83,7 @@ def exported_symbols():
         "clb_ns_couple", "clb_ns_stats", "clb_ns_update_dynamics", "clb_set_ade_relaxation",
         "clb_ade_stabilize", "clb_run_steps", "clb_ade_steps", "clb_equilibrium", "clb_ca_spill",
         "clb_read_field", "clb_write_field", "clb_species_totals", "clb_counters", "clb_kernel_time",
+        "clb_stream_copy",
         "clb_rng_draws", "clb_geom_spheres", "clb_geom_pack",
     ]
 
@@ -124,6 +126,12 @@ class Context:
 
     def sync(self):
         self.check(lib().clb_sync(self.h))
+
+    def stream_copy(self, nbytes, reps=20):
+        """Plain device copy bandwidth (GB/s, read + write bytes) and best time (ms)."""
+        g = C.c_double(); ms = C.c_double()
+        self.check(lib().clb_stream_copy(self.h, nbytes, reps, C.byref(g), C.byref(ms)))
+        return g.value, ms.value
 
 
 class Case:

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "clb_device.h"
 #include "clb_common.h"
 
@@ -13,14 +15,25 @@ namespace clb {
 // ------------------------------------------------------------------------------------------
 // CA biofilm spill
 // ------------------------------------------------------------------------------------------
-// updateLocalMaskNtotalLattices3D + the global max of the total and the mask sum
+// NaN-propagating max (a NaN total must reach the host's finiteness check)
+__device__ __forceinline__ double nanMax(double a, double b) {
+    return (isnan(a) || isnan(b)) ? __builtin_nan("") : fmax(a, b);
+}
+
+// updateLocalMaskNtotalLattices3D (part2:71-259) over the open voxels, with
+// per-block partials of the global max of the total and of the mask sum.
+// Voxels outside the compact list are closed geometry: their mask and total
+// never move, so their max and sum are constants of the case.  In a queued
+// batch (`check`) the kernel does nothing once an earlier step halted it.
 template <typename R>
-__global__ void __launch_bounds__(256) k_ca_mask_total(CaDev<R> c, double *partMax, long long *partMask) {
+__global__ void __launch_bounds__(256) k_ca_mask_total(CaDev<R> c, int check) {
     __shared__ double sMax[4];
     __shared__ long long sMask[4];
-    const long long idx = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (check && c.ctl->halt) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double bt = -1e300; long long ms = 0;
-    if (idx < c.N) {
+    if (i < c.Nopen) {
+        const long long idx = c.list[i];
         int mask = c.mask[idx];
         if (mask != 0 && mask != 1) {
             double bm = 0; int nm = 0;
@@ -32,12 +45,10 @@ __global__ void __launch_bounds__(256) k_ca_mask_total(CaDev<R> c, double *partM
             else if (pore && bm >= c.thrdB && nm > 0) { mask = nm; c.mask[idx] = (int8_t) nm; }
         }
         bt = (double) c.Btot[idx];
-        if (isnan(bt)) bt = __builtin_nan("");
         ms = mask;
     }
     for (int off = 32; off > 0; off >>= 1) {
-        const double o = __shfl_down(bt, off, 64);
-        bt = (isnan(bt) || isnan(o)) ? __builtin_nan("") : fmax(bt, o);
+        bt = nanMax(bt, __shfl_down(bt, off, 64));
         ms += __shfl_down(ms, off, 64);
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -45,9 +56,52 @@ __global__ void __launch_bounds__(256) k_ca_mask_total(CaDev<R> c, double *partM
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = -1e300; long long s = 0;
-        for (int w = 0; w < (int) (blockDim.x >> 6); ++w) { m = (isnan(m) || isnan(sMax[w])) ? __builtin_nan("") : fmax(m, sMax[w]); s += sMask[w]; }
-        partMax[blockIdx.x] = m; partMask[blockIdx.x] = s;
+        for (int w = 0; w < (int) (blockDim.x >> 6); ++w) { m = nanMax(m, sMax[w]); s += sMask[w]; }
+        c.partMax[blockIdx.x] = m; c.partMask[blockIdx.x] = s;
     }
+}
+
+// the global max and mask sum from the partials, published in the control
+// block; with `check` also the step's decision: halt the queued batch when
+// the CA must run (max total above Bmax), the mask moved, or the total is not
+// finite (check 2: the step always needs the host, a precipitation step)
+template <typename R>
+__global__ void __launch_bounds__(256) k_ca_mask_final(CaDev<R> c, int nparts, int check) {
+    __shared__ double sMax[4];
+    __shared__ long long sMask[4];
+    StepCtl *ctl = c.ctl;
+    if (check && ctl->halt) return;
+    double m[4] = {-1e300, -1e300, -1e300, -1e300};
+    long long s[4] = {0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nparts; b += 1024) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) if (b + 256 * k < nparts) { m[k] = nanMax(m[k], c.partMax[b + 256 * k]); s[k] += c.partMask[b + 256 * k]; }
+    }
+    double mm = nanMax(nanMax(m[0], m[1]), nanMax(m[2], m[3]));
+    long long ss = (s[0] + s[1]) + (s[2] + s[3]);
+    for (int off = 32; off > 0; off >>= 1) {
+        mm = nanMax(mm, __shfl_down(mm, off, 64));
+        ss += __shfl_down(ss, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { sMax[wv] = mm; sMask[wv] = ss; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double gm = c.gmaxClosed; long long gs = c.msumClosed;
+        for (int w = 0; w < (int) (blockDim.x >> 6); ++w) { gm = nanMax(gm, sMax[w]); gs += sMask[w]; }
+        ctl->gmax = gm;
+        ctl->msum = gs;
+        if (check) {
+            const bool stop = check == 2 || !isfinite(gm) || gm - c.bmax > 1e-12 || gs != ctl->oldMask;
+            if (stop) ctl->halt = 1;
+            else ctl->done += 1;
+        }
+    }
+}
+
+// a new batch: no halt, no finished step, the accepted mask sum
+__global__ void k_step_ctl_reset(StepCtl *ctl, long long oldMask) {
+    ctl->halt = 0; ctl->done = 0; ctl->oldMask = oldMask;
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
@@ -270,31 +324,46 @@ __global__ void __launch_bounds__(256) k_scalar_total(int nx, int ny, long long 
     if (threadIdx.x == 0) { double s2 = 0; for (int w = 0; w < (int) (blockDim.x >> 6); ++w) s2 += sS[w]; parts[blockIdx.x] = s2; }
 }
 
-// fixed-order final reduction of per-block partials (one block of 64 threads)
-__global__ void k_final_sum(const double *parts, long long n, int stride, int comp, int isMax, double *out) {
-    __shared__ double s[64];
+// fixed-order final reduction of per-block partials (one block of 256 threads,
+// four independent accumulators per thread so the loads overlap)
+__global__ void __launch_bounds__(256) k_final_sum(const double *parts, long long n, int stride, int comp, int isMax, double *out) {
+    __shared__ double s[256];
     const int t = threadIdx.x;
-    double v = isMax ? -1e300 : 0.0;
-    for (long long i = t; i < n; i += 64) {
-        const double p = parts[i * stride + comp];
-        v = isMax ? ((isnan(v) || isnan(p)) ? __builtin_nan("") : fmax(v, p)) : v + p;
+    const double id = isMax ? -1e300 : 0.0;
+    double v[4] = {id, id, id, id};
+    for (long long i = t; i < n; i += 1024) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const long long j = i + 256 * k;
+            if (j < n) {
+                const double p = parts[j * stride + comp];
+                v[k] = isMax ? nanMax(v[k], p) : v[k] + p;
+            }
+        }
     }
-    s[t] = v;
+    s[t] = isMax ? nanMax(nanMax(v[0], v[1]), nanMax(v[2], v[3])) : (v[0] + v[1]) + (v[2] + v[3]);
     __syncthreads();
-    if (t == 0) {
-        double r = isMax ? -1e300 : 0.0;
-        for (int i = 0; i < 64; ++i) r = isMax ? ((isnan(r) || isnan(s[i])) ? __builtin_nan("") : fmax(r, s[i])) : r + s[i];
-        *out = r;
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) s[t] = isMax ? nanMax(s[t], s[t + w]) : s[t] + s[t + w];
+        __syncthreads();
     }
+    if (t == 0) *out = s[0];
 }
-__global__ void k_final_sum_ll(const long long *parts, long long n, long long *out) {
-    __shared__ long long s[64];
+__global__ void __launch_bounds__(256) k_final_sum_ll(const long long *parts, long long n, long long *out) {
+    __shared__ long long s[256];
     const int t = threadIdx.x;
-    long long v = 0;
-    for (long long i = t; i < n; i += 64) v += parts[i];
-    s[t] = v;
+    long long v[4] = {0, 0, 0, 0};
+    for (long long i = t; i < n; i += 1024) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) if (i + 256 * k < n) v[k] += parts[i + 256 * k];
+    }
+    s[t] = (v[0] + v[1]) + (v[2] + v[3]);
     __syncthreads();
-    if (t == 0) { long long r = 0; for (int i = 0; i < 64; ++i) r += s[i]; *out = r; }
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) s[t] += s[t + w];
+        __syncthreads();
+    }
+    if (t == 0) *out = s[0];
 }
 
 // scalar field <-> double staging
@@ -314,14 +383,31 @@ __global__ void k_int_to_double(const T *src, double *dst, long long n) {
     if (i < n) dst[i] = (double) src[i];
 }
 
+// plain copy, 16 bytes per lane, grid-stride
+__global__ void __launch_bounds__(256) k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, long long n) {
+    for (long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long) gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // ------------------------------------------------------------------------------------------
 // host launchers
 // ------------------------------------------------------------------------------------------
+hipError_t launchCopy16(const void *src, void *dst, long long n16, hipStream_t st) {
+    const unsigned int g = (unsigned int) std::min<long long>((n16 + 255) / 256, 256LL * 64);
+    hipLaunchKernelGGL(k_copy16, dim3(g), dim3(256), 0, st, (const uint4 *) src, (uint4 *) dst, n16);
+    return hipGetLastError();
+}
 static inline unsigned int nblk(long long n, int b) { return (unsigned int) ((n + b - 1) / b); }
 
 template <typename R>
-hipError_t launchCaMaskTotal(const CaDev<R> &c, double *pm, long long *ps, hipStream_t st) {
-    hipLaunchKernelGGL(k_ca_mask_total<R>, dim3(nblk(c.N, 256)), dim3(256), 0, st, c, pm, ps);
+hipError_t launchCaMaskTotal(const CaDev<R> &c, int check, hipStream_t st) {
+    const unsigned int g = nblk(c.Nopen, 256);
+    hipLaunchKernelGGL(k_ca_mask_total<R>, dim3(g), dim3(256), 0, st, c, check);
+    hipLaunchKernelGGL(k_ca_mask_final<R>, dim3(1), dim3(256), 0, st, c, (int) g, check);
+    return hipGetLastError();
+}
+hipError_t launchStepCtlReset(StepCtl *ctl, long long oldMask, hipStream_t st) {
+    hipLaunchKernelGGL(k_step_ctl_reset, dim3(1), dim3(1), 0, st, ctl, oldMask);
     return hipGetLastError();
 }
 template <typename R>
@@ -356,11 +442,11 @@ hipError_t launchScalarTotal(int nx, int ny, long long N, const int8_t *mask, co
     return hipGetLastError();
 }
 hipError_t launchFinalSum(const double *parts, long long n, int stride, int comp, int isMax, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(64), 0, st, parts, n, stride, comp, isMax, out);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, st, parts, n, stride, comp, isMax, out);
     return hipGetLastError();
 }
 hipError_t launchFinalSumLL(const long long *parts, long long n, long long *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_final_sum_ll, dim3(1), dim3(64), 0, st, parts, n, out);
+    hipLaunchKernelGGL(k_final_sum_ll, dim3(1), dim3(256), 0, st, parts, n, out);
     return hipGetLastError();
 }
 template <typename R>
@@ -383,7 +469,7 @@ hipError_t launchI32ToDouble(const int *src, double *dst, long long n, hipStream
 }
 
 #define INST(R) \
-    template hipError_t launchCaMaskTotal<R>(const CaDev<R> &, double *, long long *, hipStream_t); \
+    template hipError_t launchCaMaskTotal<R>(const CaDev<R> &, int, hipStream_t); \
     template hipError_t launchCaPushGather<R>(const CaDev<R> &, uint64_t, hipStream_t); \
     template hipError_t launchCaAge<R>(const CaDev<R> &, const int *, int *, long long *, hipStream_t); \
     template hipError_t launchPrecip<R>(int, int, long long, int8_t *, const R *, double, int, int8_t *, unsigned int *, hipStream_t); \

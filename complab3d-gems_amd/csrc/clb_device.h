@@ -23,6 +23,18 @@ enum { BC_DIRICHLET_DEV = 0, BC_NEUMANN_DEV = 1, BC_CLOSED_DEV = 2 };
 #define NB_CLOSED -1
 #define NB_OUTSIDE -2
 
+// step-batch control block: a batch of coupled steps is queued without a host
+// round trip; the mask/total reduction of each step decides on the device
+// whether the host must act (CA spill, mask change, non-finite biomass), and
+// then every later kernel of the batch returns at once
+struct StepCtl {
+    int halt;                      // set by the step that needs the host
+    int done;                      // steps of the batch finished without host action
+    double gmax;                   // max total biomass of the last reduction
+    long long msum;                // mask sum of the last reduction
+    long long oldMask;             // the mask sum the host accepted last
+};
+
 // the advection-diffusion lattices of one case
 template <typename R>
 struct AdeDev {
@@ -41,13 +53,14 @@ struct AdeDev {
     const R *u;                    // [3][Nopen]
     int8_t *mask;                  // dense live mask
     const uint8_t *cls, *clsPull;  // [Nopen]
-    double *rhoN;                  // [Smax][Nopen - Nint]
+    double *rhoN;                  // [Smax][Nopen - Nint] Neumann plane densities
     int anyNeumann;
     int first, reset;
     double c0[CLB_DEV_MAXS], omP[CLB_DEV_MAXS], omB[CLB_DEV_MAXS];
     int lbt[CLB_DEV_MAXS], rbt[CLB_DEV_MAXS];
     double lbc[CLB_DEV_MAXS], rbc[CLB_DEV_MAXS];
     unsigned long long *clampCount;
+    const int *halt;               // StepCtl::halt of a queued batch, or null
 };
 
 // rate-law parameters (biotic dual Monod, abiotic mass action)
@@ -93,6 +106,13 @@ struct CaDev {
     int nx, ny, nz;
     long long N;
     int M;
+    int Nopen;
+    const int *list;               // the compact open voxels (the only ones whose mask or total moves)
+    double gmaxClosed;             // max total over the other voxels (fixed at creation)
+    long long msumClosed;          // mask sum over the other voxels (fixed at creation)
+    StepCtl *ctl;
+    double *partMax;               // [blocks]
+    long long *partMask;           // [blocks]
     R *B;                          // [M][N]
     R *Btot;
     int8_t *mask;
@@ -139,7 +159,8 @@ template <typename R> hipError_t launchScatterCompact(const R *, const int *, in
 template <typename R> hipError_t launchGatherCompact(const double *, const int *, int, R *, hipStream_t);
 unsigned int nsStepBlocks(int Nopen, int Nint);
 // host launchers: dense grids (clb_aux.hip)
-template <typename R> hipError_t launchCaMaskTotal(const CaDev<R> &, double *, long long *, hipStream_t);
+template <typename R> hipError_t launchCaMaskTotal(const CaDev<R> &, int check, hipStream_t);
+hipError_t launchStepCtlReset(StepCtl *, long long oldMask, hipStream_t);
 template <typename R> hipError_t launchCaPushGather(const CaDev<R> &, uint64_t, hipStream_t);
 template <typename R> hipError_t launchCaAge(const CaDev<R> &, const int *, int *, long long *, hipStream_t);
 hipError_t launchOmegaSwitch(const int8_t *, const int *, uint8_t *, int, hipStream_t);
@@ -152,5 +173,6 @@ template <typename R> hipError_t launchToDouble(const R *, double *, long long, 
 template <typename R> hipError_t launchFromDouble(const double *, R *, long long, hipStream_t);
 hipError_t launchI8ToDouble(const int8_t *, double *, long long, hipStream_t);
 hipError_t launchI32ToDouble(const int *, double *, long long, hipStream_t);
+hipError_t launchCopy16(const void *, void *, long long, hipStream_t);
 
 }  // namespace clb

@@ -109,10 +109,21 @@ struct Case : CaseBase {
     double kms = 0;
     int64_t klaunch = 0;
     unsigned int nparts = 0;
+    // queued step batches (CA cases): the control block on the device, its
+    // pinned host mirror, the mask/total partials and the closed-voxel constants
+    StepCtl *ctl = nullptr, *ctlHost = nullptr;
+    double *caPartMax = nullptr;
+    long long *caPartMask = nullptr;
+    double gmaxClosed = -1e300;
+    long long msumClosed = 0;
+    int batchLen = 8;
+    static constexpr int kBatchMax = 64;
+    static constexpr size_t kEvPool = 256;
 
     ~Case() override {
         for (auto &e : evpool) { (void) hipEventDestroy(e.first); (void) hipEventDestroy(e.second); }
         for (void *q : allocs) (void) hipFree(q);
+        if (ctlHost) (void) hipHostFree(ctlHost);
     }
 
     R *fCur() const { return fCurIsA ? fA : fB; }
@@ -120,7 +131,7 @@ struct Case : CaseBase {
     R *gCur() const { return gCurIsA ? gA : gB; }
     R *gOut() const { return gCurIsA ? gB : gA; }
 
-    AdeDev<R> ade() const {
+    AdeDev<R> ade(const int *halt = nullptr) const {
         AdeDev<R> a;
         memset(&a, 0, sizeof(a));
         a.nx = nx; a.ny = ny; a.nz = nz; a.N = N;
@@ -137,13 +148,9 @@ struct Case : CaseBase {
         a.cur = fCur(); a.out = fOut(); a.imm = imm; a.B = B;
         a.u = u; a.mask = mask; a.cls = adeCls; a.clsPull = adeClsPull;
         a.rhoN = rhoN;
-        a.anyNeumann = 0;
-        for (int ms = 0; ms < Sm; ++ms) {
-            int s = mobSpecies[ms];
-            if (p.left_btype[s] == CLB_BC_NEUMANN || p.right_btype[s] == CLB_BC_NEUMANN) a.anyNeumann = 1;
-        }
         a.first = adeFirst ? 1 : 0; a.reset = adeReset ? 1 : 0;
         a.clampCount = clampCount;
+        a.halt = halt;
         return a;
     }
     KinDev kin(bool on) const {
@@ -184,6 +191,9 @@ struct Case : CaseBase {
         CaDev<R> c;
         memset(&c, 0, sizeof(c));
         c.nx = nx; c.ny = ny; c.nz = nz; c.N = N; c.M = p.n_microbes;
+        c.Nopen = Nopen; c.list = list;
+        c.gmaxClosed = gmaxClosed; c.msumClosed = msumClosed;
+        c.ctl = ctl; c.partMax = caPartMax; c.partMask = caPartMask;
         c.B = B; c.Btot = Btot; c.mask = mask; c.age = age; c.dist = dist;
         c.outbox = outbox; c.pushed = pushed;
         c.bmax = p.bmax; c.thrdB = p.thrd_bfilm_frac * p.bmax;
@@ -290,6 +300,15 @@ struct Case : CaseBase {
         ucnt = dalloc<unsigned int>(allocs, 4);
         nsDone = dalloc<int>(allocs, 1);
         tracer = dalloc<NsTracer>(allocs, 1);
+        ctl = dalloc<StepCtl>(allocs, 1);
+        caPartMax = dalloc<double>(allocs, nblk(Nopen, 256));
+        caPartMask = dalloc<long long>(allocs, nblk(Nopen, 256));
+        {
+            void *h = nullptr;
+            HIPT(hipHostMalloc(&h, sizeof(StepCtl), hipHostMallocDefault));
+            ctlHost = (StepCtl *) h;
+            memset(ctlHost, 0, sizeof(StepCtl));
+        }
 
         // voxel bytes and scalar fields
         std::vector<uint8_t> ac(Nopen), nc(Nopen);
@@ -358,12 +377,21 @@ struct Case : CaseBase {
             long long ms = 0;
             for (long long i = 0; i < N; ++i) ms += g[i];
             oldTotMask = ms;
+            // voxels outside the compact list keep their mask and total for good
+            gmaxClosed = -1e300; msumClosed = 0;
+            for (long long i = 0; i < N; ++i) {
+                if (d2c[i] >= 0) continue;
+                msumClosed += g[i];
+                const double t = (double) sc[i];
+                gmaxClosed = (std::isnan(gmaxClosed) || std::isnan(t)) ? t + gmaxClosed : std::max(gmaxClosed, t);
+            }
         }
         HIPT(hipMemsetAsync(u, 0, 3 * (size_t) Nopen * sizeof(R), st));
         HIPT(hipMemsetAsync(clampCount, 0, sizeof(unsigned long long), st));
         HIPT(hipMemsetAsync(ucnt, 0, 4 * sizeof(unsigned int), st));
         HIPT(hipMemsetAsync(nsDone, 0, sizeof(int), st));
         HIPT(hipMemsetAsync(tracer, 0, sizeof(NsTracer), st));
+        HIPT(hipMemsetAsync(ctl, 0, sizeof(StepCtl), st));
         nsOmega = 1.0 / p.ns_tau;
         nsBioOmega = 1.0 / (p.perm_ratio * (1.0 / nsOmega - 0.5) + 0.5);
         HIPT(hipStreamSynchronize(st));
@@ -489,12 +517,12 @@ struct Case : CaseBase {
         }
         evUsed = 0;
     }
-    void fusedLaunch(hipStream_t st, bool kinOn) {
+    void fusedLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr) {
         if (evUsed == evpool.size()) {
-            if (evpool.size() >= 256) flushTimes(st);
+            if (evpool.size() >= kEvPool) flushTimes(st);
             else { hipEvent_t e0, e1; HIPT(hipEventCreate(&e0)); HIPT(hipEventCreate(&e1)); evpool.push_back({e0, e1}); }
         }
-        AdeDev<R> a = ade();
+        AdeDev<R> a = ade(halt);
         KinDev k = kin(kinOn);
         HIPT(hipEventRecord(evpool[evUsed].first, st));
         HIPT(launchAdeStep<R>(a, k, st));
@@ -514,43 +542,54 @@ struct Case : CaseBase {
         for (int t = 0; t < n; ++t) fusedLaunch(st, true);
         flushTimes(st);
     }
-    void equilibrium(hipStream_t st) override {
+    void equilibrium(hipStream_t st) override { equilibriumQ(st, nullptr); }
+    void equilibriumQ(hipStream_t st, const int *halt) {
         if (!p.eq_on || p.eq_nc <= 0) return;
-        HIPT(launchEquilibrium<R>(ade(), eqdev(), ucnt + 2, st));
+        HIPT(launchEquilibrium<R>(ade(halt), eqdev(), ucnt + 2, st));
     }
 
     // mask/total update + global max of the total + mask sum
     void maskTotal(hipStream_t st, double *gmax, long long *msum) {
-        const unsigned int g = nblk(N, 256);
-        HIPT(launchCaMaskTotal<R>(ca(), parts, partsLL, st));
-        HIPT(launchFinalSum(parts, g, 1, 0, 1, scal, st));
-        HIPT(launchFinalSumLL(partsLL, g, scalLL, st));
-        HIPT(hipMemcpyAsync(gmax, scal, sizeof(double), hipMemcpyDeviceToHost, st));
-        HIPT(hipMemcpyAsync(msum, scalLL, sizeof(long long), hipMemcpyDeviceToHost, st));
+        HIPT(launchCaMaskTotal<R>(ca(), 0, st));
+        HIPT(hipMemcpyAsync(ctlHost, ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, st));
         HIPT(hipStreamSynchronize(st));
+        *gmax = ctlHost->gmax;
+        *msum = ctlHost->msum;
     }
     // updateAgeDistance3D sweeps; to the fixed point (sum of increments 0) or once
+    // A sweep at the fixed point changes nothing, so sweeps are queued 16 at a
+    // time and only the last one's sum of increments is read back; the result
+    // and the 1001-sweep limit are those of one sweep per read-back.
     Status ageRelax(hipStream_t st, bool toFixedPoint) {
         const unsigned int g = nblk(N, 256);
-        int wc1 = 0;
+        int swept = 0;
         while (true) {
-            HIPT(launchCaAge<R>(ca(), age, age2, partsLL, st));
+            const int batch = toFixedPoint ? std::min(16, 1001 - swept) : 1;
+            for (int k = 0; k < batch; ++k) {
+                HIPT(launchCaAge<R>(ca(), age, age2, partsLL, st));
+                std::swap(age, age2);
+            }
             HIPT(launchFinalSumLL(partsLL, g, scalLL, st));
             long long diff = 0;
             HIPT(hipMemcpyAsync(&diff, scalLL, sizeof(long long), hipMemcpyDeviceToHost, st));
             HIPT(hipStreamSynchronize(st));
-            std::swap(age, age2);
+            swept += batch;
             if (!toFixedPoint || diff == 0) break;
-            if (++wc1 > 1000) return {CLB_E_AGE_SWEEPS, "age relaxation did not settle"};
+            if (swept > 1000) return {CLB_E_AGE_SWEEPS, "age relaxation did not settle"};
         }
         return {CLB_OK, ""};
     }
     Status caBlock(hipStream_t st, int *sweepsOut) override {
-        int sweeps = 0;
         if (sweepsOut) *sweepsOut = 0;
         if (p.n_microbes <= 0) return {CLB_OK, ""};
         double gmax; long long msum;
         maskTotal(st, &gmax, &msum);
+        return caFrom(st, gmax, msum, sweepsOut);
+    }
+    // the CA loop (complab.cpp:2793-3055) from a mask/total reduction already made
+    Status caFrom(hipStream_t st, double gmax, long long msum, int *sweepsOut) {
+        int sweeps = 0;
+        if (sweepsOut) *sweepsOut = 0;
         if (!std::isfinite(gmax)) return {CLB_E_NONFINITE, "non-finite biomass"};
         if (gmax - p.bmax > 1e-12) {
             double prev = gmax; int stall = 0, wc = 0;
@@ -589,12 +628,15 @@ struct Case : CaseBase {
         return {CLB_OK, ""};
     }
 
-    Status coupledStep(hipStream_t st) {
-        fusedLaunch(st, true);
-        if (p.eq_on) equilibrium(st);
+    bool precipDue(long long it) const {
+        return p.precip_on && p.precip_interval > 0 && it % p.precip_interval == 0 && p.precip_species >= 0;
+    }
+    // the host part of a step after its transport, equilibrium and (CA cases)
+    // mask/total reduction: CA spill, mask-change consequences, precipitation
+    Status stepTail(hipStream_t st, double gmax, long long msum) {
         if (p.n_microbes > 0) {
             int sw = 0;
-            Status s = caBlock(st, &sw);
+            Status s = caFrom(st, gmax, msum, &sw);
             if (s.code) return s;
             if (lastMaskSum != oldTotMask) {
                 oldTotMask = lastMaskSum;
@@ -608,7 +650,7 @@ struct Case : CaseBase {
                 }
             }
         }
-        if (p.precip_on && p.precip_interval > 0 && iT % p.precip_interval == 0 && p.precip_species >= 0) {
+        if (precipDue(iT)) {
             HIPT(hipMemsetAsync(ucnt, 0, 2 * sizeof(unsigned int), st));
             HIPT(launchPrecip<R>(nx, ny, N, mask, imm + (size_t) slot[p.precip_species] * N, p.precip_max_rho, p.precip_phase_id, phase, ucnt, st));
             unsigned int ch = 0;
@@ -629,10 +671,66 @@ struct Case : CaseBase {
         ++iT; cnt[7]++;
         return {CLB_OK, ""};
     }
+
+    // one coupled step with the host in the loop
+    Status coupledStep(hipStream_t st) {
+        fusedLaunch(st, true);
+        if (p.eq_on) equilibrium(st);
+        double gmax = 0; long long msum = 0;
+        if (p.n_microbes > 0) maskTotal(st, &gmax, &msum);
+        return stepTail(st, gmax, msum);
+    }
+
+    // CA cases: steps are queued in batches.  Each step's mask/total reduction
+    // decides on the device whether the step needs the host; the first that
+    // does halts the batch (every later kernel of the batch returns at once)
+    // and the host finishes that step.  A precipitation step always ends a
+    // batch and always halts.  The batch length adapts: it doubles after a
+    // batch that ran through and shrinks to the halting position otherwise.
+    Status runBatch(hipStream_t st, int want, int *finished) {
+        *finished = 0;
+        int len = want;
+        for (int j = 0; j < len; ++j) if (precipDue(iT + j)) { len = j + 1; break; }
+        if (evUsed + (size_t) len > kEvPool) flushTimes(st);
+        HIPT(launchStepCtlReset(ctl, oldTotMask, st));
+        const size_t ev0 = evUsed;
+        for (int j = 0; j < len; ++j) {
+            fusedLaunch(st, true, &ctl->halt);
+            if (p.eq_on) equilibriumQ(st, &ctl->halt);
+            HIPT(launchCaMaskTotal<R>(ca(), precipDue(iT + j) ? 2 : 1, st));
+        }
+        HIPT(hipMemcpyAsync(ctlHost, ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, st));
+        HIPT(hipStreamSynchronize(st));
+        const int fin = ctlHost->done;
+        const bool halt = ctlHost->halt != 0;
+        const int ran = fin + (halt ? 1 : 0);               // transport launches that did work
+        if (fin > len || ran > len) return {CLB_E_HIP, "step batch bookkeeping failed"};
+        if ((len - ran) & 1) fCurIsA = !fCurIsA;            // launches that returned at once swapped nothing
+        evUsed = ev0 + (size_t) ran;                        // and are not timed
+        iT += fin; cnt[7] += fin;
+        *finished = fin;
+        batchLen = halt ? std::max(1, fin + 1) : std::min(kBatchMax, 2 * batchLen);
+        if (halt) {
+            Status s = stepTail(st, ctlHost->gmax, ctlHost->msum);
+            if (s.code) return s;
+            *finished = fin + 1;
+        }
+        return {CLB_OK, ""};
+    }
+
     Status runSteps(hipStream_t st, int n, int *done) override {
         int k = 0;
         Status s{CLB_OK, ""};
-        for (; k < n; ++k) { s = coupledStep(st); if (s.code) break; }
+        if (p.n_microbes > 0) {
+            while (k < n) {
+                int fin = 0;
+                s = runBatch(st, std::min(n - k, batchLen), &fin);
+                k += fin;
+                if (s.code) break;
+            }
+        } else {
+            for (; k < n; ++k) { s = coupledStep(st); if (s.code) break; }
+        }
         flushTimes(st);
         if (done) *done = k;
         return s;
@@ -1000,6 +1098,42 @@ int clb_counters(clb_ctx *ctx, int case_id, int64_t out[8]) {
     return CLB_OK;
     GUARD_END
 }
+int clb_stream_copy(clb_ctx *ctx, size_t bytes, int reps, double *gbps, double *msOut) {
+    if (!ctx || bytes < 16 || reps < 1) return CLB_E_ARG;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = CLB_OK;
+    try {
+        HIPT(hipSetDevice(ctx->device));
+        const long long n16 = (long long) (bytes / 16);
+        HIPT(hipMalloc(&a, (size_t) n16 * 16));
+        HIPT(hipMalloc(&b, (size_t) n16 * 16));
+        HIPT(hipMemsetAsync(a, 1, (size_t) n16 * 16, ctx->stream));
+        HIPT(hipEventCreate(&e0));
+        HIPT(hipEventCreate(&e1));
+        HIPT(launchCopy16(a, b, n16, ctx->stream));
+        double best = 1e300;
+        for (int r = 0; r < reps; ++r) {
+            HIPT(hipEventRecord(e0, ctx->stream));
+            HIPT(launchCopy16(a, b, n16, ctx->stream));
+            HIPT(hipEventRecord(e1, ctx->stream));
+            HIPT(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPT(hipEventElapsedTime(&ms, e0, e1));
+            best = std::min(best, (double) ms);
+        }
+        if (gbps) *gbps = 2.0 * (double) n16 * 16 / (best * 1e-3) / 1e9;
+        if (msOut) *msOut = best;
+    } catch (const HipFail &h) {
+        rc = fail(ctx, CLB_E_HIP, h.what + ": " + hipGetErrorString(h.e));
+    }
+    if (e0) (void) hipEventDestroy(e0);
+    if (e1) (void) hipEventDestroy(e1);
+    if (a) (void) hipFree(a);
+    if (b) (void) hipFree(b);
+    return rc;
+}
+
 int clb_kernel_time(clb_ctx *ctx, int case_id, int reset, double *avg_ms, int64_t *launches) {
     GUARD_BEGIN
     CASE_OR_FAIL(c)

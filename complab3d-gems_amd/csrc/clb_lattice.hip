@@ -28,6 +28,23 @@ namespace clb {
 // ------------------------------------------------------------------------------------------
 // helpers
 // ------------------------------------------------------------------------------------------
+// The dispatcher deals workgroups round-robin over the 8 XCDs, each with its
+// own L2.  Renumbering the blocks so that XCD x runs one contiguous run of the
+// voxel order keeps the partly-used cache lines of the neighbour gathers (and
+// the +-y/+-z neighbours themselves) in the L2 that fetched them.  A bijection
+// of [0, gridDim.x) whatever the dispatch order.
+__device__ __forceinline__ int xcdBlock() {
+    const int G = gridDim.x, b = blockIdx.x;
+    const int q = G >> 3, r = G & 7, x = b & 7, k = b >> 3;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
+// a queued step batch was halted by an earlier step: the kernel does nothing
+// (the flag is written by an earlier kernel of the stream, never during this one)
+__device__ __forceinline__ bool halted(const int *h) {
+    return h && *h;
+}
+
 // compact neighbour n (>= 0) closed for a species: bounce-back class, or a
 // closed x plane (complab_functions.hh:431-432)
 __device__ __forceinline__ bool closedFor(const uint8_t *cls, int n, int Nint, int Nleft, int lbt, int rbt) {
@@ -162,38 +179,30 @@ __device__ __forceinline__ double clampInc(int clamp, double d, double cur, unsi
 // ------------------------------------------------------------------------------------------
 // advection-diffusion
 // ------------------------------------------------------------------------------------------
-// Neumann planes: the density of the inner layer after this launch's stream
-// (FlatAdiabaticBoundaryFunctional3D); a bounce-back inner cell reports 0
-template <typename R>
-__global__ void k_ade_neumann(AdeDev<R> a) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    const int Np = a.Nopen - a.Nint;
-    if (p >= Np) return;
-    const int i = a.Nint + p;
-    const bool left = i < a.Nint + a.Nleft;
-    const int inner = a.nb[(left ? 1 : 0) * (long long) a.Nopen + i];
-    for (int ms = 0; ms < a.Sm; ++ms) {
-        const int s = a.mobSpecies[ms];
-        if ((left ? a.lbt[s] : a.rbt[s]) != BC_NEUMANN_DEV) continue;
-        double r = 0;
-        if (inner >= 0 && !closedFor(a.cls, inner, a.Nint, a.Nleft, a.lbt[s], a.rbt[s])) {
-            double fl[7];
-            pull7<R, double>(a, a.cur + (long long) ms * 7 * a.Nopen, a.out + (long long) ms * 7 * a.Nopen, inner, a.lbt[s], a.rbt[s], fl);
-            for (int q = 0; q < 7; ++q) r += fl[q];
-        }
-        a.rhoN[(long long) ms * Np + p] = r;
-    }
-}
-
 // interior: pull + BGK collide + biotic and abiotic kinetics with the clamp
 // and the D3Q7 deposit, every mobile species of a voxel in one thread
 template <typename R, int MAXS>
 __global__ void __launch_bounds__(256) k_ade_bulk(AdeDev<R> a, KinDev k) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (halted(a.halt)) return;
+    const int i = xcdBlock() * blockDim.x + threadIdx.x;
     if (i >= a.Nint) return;
     const int cls = a.cls[i];
     if (cls == ADE_BB) return;
     const long long No = a.Nopen;
+    // where each moving population comes from, shared by every species: a
+    // neighbour's cur slot, the voxel's own opposite cur slot (grid edge), or
+    // its own opposite old slot (bounce-back); a neighbour on an x plane is
+    // closed only for the species whose plane is closed
+    int nbq[6];
+#pragma unroll
+    for (int q = 1; q < 7; ++q) nbq[q - 1] = a.nb[(q - 1) * No + i];
+    int kind[6];                       // 0 neighbour, 1 edge, 2 bounce-back, 3 plane neighbour (species rule)
+#pragma unroll
+    for (int q = 1; q < 7; ++q) {
+        const int n = nbq[q - 1];
+        kind[q - 1] = n == NB_OUTSIDE ? 1 : (n == NB_CLOSED ? 2 : (a.clsPull[n] == ADE_BB ? 2 : (n >= a.Nint ? 3 : 0)));
+    }
+    const int Np = a.Nopen - a.Nint;
     R f[MAXS][7];
     const R ux = a.u[i], uy = a.u[No + i], uz = a.u[2 * No + i];
 #pragma unroll
@@ -202,7 +211,34 @@ __global__ void __launch_bounds__(256) k_ade_bulk(AdeDev<R> a, KinDev k) {
         const int s = a.mobSpecies[ms];
         R *cur = a.cur + (long long) ms * 7 * No;
         R *old = a.out + (long long) ms * 7 * No;
-        pull7<R, R>(a, cur, old, i, a.lbt[s], a.rbt[s], f[ms]);
+        if (a.first) {
+#pragma unroll
+            for (int q = 0; q < 7; ++q) f[ms][q] = cur[q * No + i];
+        } else {
+            f[ms][0] = cur[i];
+#pragma unroll
+            for (int q = 1; q < 7; ++q) {
+                const int n = nbq[q - 1];
+                int kd = kind[q - 1];
+                if (kd == 3) kd = ((n < a.Nint + a.Nleft) ? a.lbt[s] : a.rbt[s]) == BC_CLOSED_DEV ? 2 : 0;
+                const R *base = kd == 2 ? old : cur;
+                const long long off = kd == 0 ? (long long) q * No + n : (long long) opp7(q) * No + i;
+                f[ms][q] = base[off];
+            }
+        }
+        // the Neumann planes impose this voxel's density after the stream
+        if (nbq[0] >= a.Nint && nbq[0] < a.Nint + a.Nleft && a.lbt[s] == BC_NEUMANN_DEV) {
+            double r = 0;
+#pragma unroll
+            for (int q = 0; q < 7; ++q) r += (double) f[ms][q];
+            a.rhoN[(long long) ms * Np + (nbq[0] - a.Nint)] = r;
+        }
+        if (nbq[1] >= a.Nint + a.Nleft && a.rbt[s] == BC_NEUMANN_DEV) {
+            double r = 0;
+#pragma unroll
+            for (int q = 0; q < 7; ++q) r += (double) f[ms][q];
+            a.rhoN[(long long) ms * Np + (nbq[1] - a.Nint)] = r;
+        }
         if (a.first) firstHandOver<R>(a, cur, i, a.lbt[s], a.rbt[s], a.c0[s]);
         if (a.reset) {
             double c0 = a.c0[s];
@@ -301,21 +337,25 @@ __global__ void __launch_bounds__(256) k_ade_bulk(AdeDev<R> a, KinDev k) {
     }
 }
 
-// the x planes: pull + regularized Dirichlet (or Neumann) completion + BGK
+// the x planes: pull + regularized Dirichlet (or Neumann) completion + BGK;
+// one thread per plane voxel and mobile species (blockIdx.y), so the short
+// dependent load chains of the few plane voxels overlap
 template <typename R>
 __global__ void __launch_bounds__(256) k_ade_planes(AdeDev<R> a) {
+    if (halted(a.halt)) return;
     const int i = a.Nint + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.Nopen) return;
+    const int ms = blockIdx.y;
+    if (i >= a.Nopen || ms >= a.Sm) return;
     const int cls = a.cls[i];
     if (cls == ADE_BB) return;
     const long long No = a.Nopen;
     const bool left = i < a.Nint + a.Nleft;
     const int Np = a.Nopen - a.Nint;
     const R ux = a.u[i], uy = a.u[No + i], uz = a.u[2 * No + i];
-    for (int ms = 0; ms < a.Sm; ++ms) {
+    {
         const int s = a.mobSpecies[ms];
         const int bt = left ? a.lbt[s] : a.rbt[s];
-        if (bt == BC_CLOSED_DEV) continue;
+        if (bt == BC_CLOSED_DEV) return;
         R *cur = a.cur + (long long) ms * 7 * No;
         R *old = a.out + (long long) ms * 7 * No;
         R f[7];
@@ -327,7 +367,13 @@ __global__ void __launch_bounds__(256) k_ade_planes(AdeDev<R> a) {
             for (int q = 0; q < 7; ++q) f[q] = (R) (w7(q) * c0);
         }
         double rhoD = left ? a.lbc[s] : a.rbc[s];
-        if (bt == BC_NEUMANN_DEV) rhoD = a.rhoN[(long long) ms * Np + (i - a.Nint)];
+        if (bt == BC_NEUMANN_DEV) {
+            // FlatAdiabaticBoundaryFunctional3D: the inner layer's density after
+            // the stream, written by the interior kernel; a bounce-back (or
+            // closed) inner cell reports 0
+            const int inner = a.nb[(left ? 1 : 0) * No + i];
+            rhoD = (inner >= 0 && a.cls[inner] != ADE_BB) ? a.rhoN[(long long) ms * Np + (i - a.Nint)] : 0.0;
+        }
         adeDirichlet<R>(f, left ? 1 : 0, (R) rhoD);
         bgk7<R>(f, (R) (cls == ADE_BFILM ? a.omB[s] : a.omP[s]), ux, uy, uz);
         for (int q = 0; q < 7; ++q) old[q * No + i] = f[q];
@@ -487,7 +533,8 @@ __device__ __forceinline__ void blockEnergy(double e, double cnt, double *partE,
 template <typename R>
 __global__ void __launch_bounds__(256) k_ns_bulk(NsDev<R> a, double *partE, double *partN) {
     if (*a.done) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int blk = xcdBlock();
+    const int i = blk * blockDim.x + threadIdx.x;
     double e = 0, cnt = 0;
     if (i < a.Nint) {
         const int cls = a.cls[i];
@@ -509,7 +556,7 @@ __global__ void __launch_bounds__(256) k_ns_bulk(NsDev<R> a, double *partE, doub
             for (int q = 0; q < 19; ++q) a.out[q * No + i] = fl[q] + om * (feqInc<R>(q, rb, jx, jy, jz, jsq) - fl[q]);
         }
     }
-    blockEnergy(e, cnt, partE, partN, blockIdx.x);
+    blockEnergy(e, cnt, partE, partN, blk);
 }
 
 // the pressure planes
@@ -538,13 +585,29 @@ __global__ void __launch_bounds__(256) k_ns_planes(NsDev<R> a, double *partE, do
     blockEnergy(e, cnt, partE, partN, slot0 + blockIdx.x);
 }
 
-// the convergence rule, evaluated on the device after every iteration
-__global__ void k_ns_tracer(NsTracer *t, const double *partE, const double *partN, int nparts, int rule, double eps, int *done) {
+// the convergence rule, evaluated on the device after every iteration.  The
+// window statistics run in the ring's order (oldest first) from a copy in LDS.
+__global__ void __launch_bounds__(64) k_ns_tracer(NsTracer *t, const double *partE, const double *partN, int nparts, int rule, double eps, int *done) {
     __shared__ double sE[64], sN[64];
+    __shared__ double win[NS_TRACER_WINDOW];
+    __shared__ double sVal;
+    __shared__ int sHead, sFull;
     if (*done) return;
     const int tid = threadIdx.x;   // 64 threads
+    // thread t adds partials t, t+64, t+128, ... in that order; the loads of a
+    // chunk of 16 are issued together
     double E = 0, C = 0;
-    for (int i = tid; i < nparts; i += 64) { E += partE[i]; C += partN[i]; }
+    for (int i0 = tid; i0 < nparts; i0 += 64 * 16) {
+        double e[16], c[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int i = i0 + 64 * k;
+            e[k] = i < nparts ? partE[i] : 0.0;
+            c[k] = i < nparts ? partN[i] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) if (i0 + 64 * k < nparts) { E += e[k]; C += c[k]; }
+    }
     sE[tid] = E; sN[tid] = C;
     __syncthreads();
     if (tid == 0) {
@@ -553,26 +616,37 @@ __global__ void k_ns_tracer(NsTracer *t, const double *partE, const double *part
         const double val = Cs > 0 ? Es / Cs : 0.0;
         t->iters += 1;
         t->last = val;
-        if (!isfinite(val)) { t->nonfinite = 1; *done = 1; return; }
-        if (rule == 0) {                      // ValueTracer(1, 1000, eps): window 500
+        sVal = val;
+        sFull = 0;
+        if (!isfinite(val)) { t->nonfinite = 1; *done = 1; }
+        else if (rule == 0) {                 // ValueTracer(1, 1000, eps): window 500
             const int W = NS_TRACER_WINDOW;
-            t->ring[t->head] = val;
-            t->head = (t->head + 1) % W;
+            const int h = t->head;
+            t->ring[h] = val;
+            t->head = (h + 1) % W;
             if (t->count < W) t->count += 1;
-            if (t->count >= W) {
-                double m = 0;
-                for (int i = 0; i < W; ++i) m += t->ring[(t->head + i) % W];
-                m /= (double) W;
-                double s = 0;
-                for (int i = 0; i < W; ++i) { const double d = t->ring[(t->head + i) % W] - m; s += d * d; }
-                s = sqrt(s / (double) W);
-                const double r = s / fabs(m);
-                if (isnan(r) || r < eps) { t->converged = 1; *done = 1; }
-            }
+            sHead = (h + 1) % W;
+            sFull = t->count >= W;
         } else {
             if (t->prevE > 0 && fabs(val - t->prevE) <= eps * fabs(val)) { t->converged = 1; *done = 1; }
             t->prevE = val;
         }
+    }
+    __syncthreads();
+    if (!sFull) return;
+    const int W = NS_TRACER_WINDOW;
+    for (int k = tid; k < W - 1; k += 64) win[k] = t->ring[(sHead + k) % W];
+    if (tid == 0) win[W - 1] = sVal;
+    __syncthreads();
+    if (tid == 0) {
+        double m = 0;
+        for (int i = 0; i < W; ++i) m += win[i];
+        m /= (double) W;
+        double sd = 0;
+        for (int i = 0; i < W; ++i) { const double d = win[i] - m; sd += d * d; }
+        sd = sqrt(sd / (double) W);
+        const double r = sd / fabs(m);
+        if (isnan(r) || r < eps) { t->converged = 1; *done = 1; }
     }
 }
 
@@ -673,6 +747,7 @@ __global__ void k_ns_reopen(NsDev<R> a) {
 // ------------------------------------------------------------------------------------------
 template <typename R>
 __global__ void __launch_bounds__(128) k_equilibrium(AdeDev<R> a, EqDev e, unsigned int *nonconv) {
+    if (halted(a.halt)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.Nint) return;
     const long long v = a.list[i];
@@ -730,7 +805,6 @@ unsigned int nsStepBlocks(int Nopen, int Nint) { return nblk(Nint, 256) + nblk(N
 template <typename R>
 hipError_t launchAdeStep(const AdeDev<R> &a, const KinDev &k, hipStream_t st) {
     const int Np = a.Nopen - a.Nint;
-    if (a.anyNeumann && Np > 0) hipLaunchKernelGGL(k_ade_neumann<R>, dim3(nblk(Np, 256)), dim3(256), 0, st, a);
     if (a.Nint > 0) {
         const unsigned int g = nblk(a.Nint, 256);
         if (a.Sm <= 1) hipLaunchKernelGGL((k_ade_bulk<R, 1>), dim3(g), dim3(256), 0, st, a, k);
@@ -740,7 +814,7 @@ hipError_t launchAdeStep(const AdeDev<R> &a, const KinDev &k, hipStream_t st) {
         else if (a.Sm <= 6) hipLaunchKernelGGL((k_ade_bulk<R, 6>), dim3(g), dim3(256), 0, st, a, k);
         else hipLaunchKernelGGL((k_ade_bulk<R, 8>), dim3(g), dim3(256), 0, st, a, k);
     }
-    if (Np > 0) hipLaunchKernelGGL(k_ade_planes<R>, dim3(nblk(Np, 256)), dim3(256), 0, st, a);
+    if (Np > 0 && a.Sm > 0) hipLaunchKernelGGL(k_ade_planes<R>, dim3(nblk(Np, 256), a.Sm), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 template <typename R>

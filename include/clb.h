@@ -253,6 +253,12 @@ int  clb_counters(clb_ctx *ctx, int case_id, int64_t out[8]);
  * since the last reset, measured with HIP events on the case's stream */
 int  clb_kernel_time(clb_ctx *ctx, int case_id, int reset, double *avg_ms, int64_t *launches);
 
+/* device copy bandwidth of a plain vectorised copy kernel over `bytes` bytes,
+ * best of `reps` launches (HIP events); the measured ceiling the roofline
+ * fractions are read against and the calibration of the HBM byte counters.
+ * No counterpart in the reference. */
+int  clb_stream_copy(clb_ctx *ctx, size_t bytes, int reps, double *gbps, double *ms);
+
 /* ---- geometry (complab3d_geometry.hh) -------------------------------------- */
 /* Rng of complab3d_geometry.hh:77-86: next `n` raw 64-bit draws from seed */
 int  clb_rng_draws(uint64_t seed, int n, uint64_t *out);
