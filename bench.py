@@ -10,7 +10,8 @@ between two device synchronisations.  Prints one JSON line.
 value: coupled MLUPS = lattice voxels x K / seconds / 1e6.
 roofline: the fused transport + kinetics kernel, algorithmic bytes per launch
 (per open voxel S_m*14*p + 3p + (S_i+M)*2p, plus 1 byte of mask per voxel)
-over its HIP-event-timed average duration, against 8 TB/s.
+over its HIP-event-timed average duration, against 8 TB/s; `traffic` is the
+HBM bytes per launch of the committed rocprofv3 counter summary in profiles/.
 cpu_baseline: the CPU oracle (a scalar C++ restatement) on the same case for a
 few steps, velocity copied from the GPU flow solve.
 """
@@ -27,6 +28,22 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+PROFILE = os.path.join(ROOT, "profiles", "k_ade_bulk_biotic_258x128x128_fp64.txt")
+
+
+def profiled_traffic(p, n_open):
+    """HBM bytes per launch of the fused kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a copy
+    kernel of known size); None for any other workload."""
+    if p.precision != 0 or (p.nx, p.ny, p.nz) != (258, 128, 128) or n_open != 1485001:
+        return None
+    try:
+        for line in open(PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        return None
+    return None
 
 
 def algorithmic_bytes(p, n_open, n_vox):
@@ -137,7 +154,7 @@ def main():
                    "flow_resolves": int(c1[4] - c0[4]), "ca_sweeps": int(c1[6] - c0[6]),
                    "mask_changes": int(c1[5] - c0[5])},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(p, n_open),
                      "kernel": "k_ade_step (fused stream+collide+kinetics)",
                      "kernel_avg_ms": round(kms, 5), "launches": int(launches),
                      "bytes_per_open_voxel": per_open, "bytes_per_launch": int(bytes_launch),

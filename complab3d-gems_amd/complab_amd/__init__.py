@@ -46,6 +46,7 @@ def lib():
         "clb_case_create": ([V, V, V, V, C.POINTER(I)], I),
         "clb_case_destroy": ([V, I], I),
         "clb_case_footprint": ([V], C.c_size_t),
+        "clb_case_get_params": ([V, I, V], I),
         "clb_ns_init": ([V, I, D], I),
         "clb_ns_solve": ([V, I, I, I, D, I, C.POINTER(I), C.POINTER(I)], I),
         "clb_ns_couple": ([V, I], I),
@@ -57,6 +58,14 @@ def lib():
         "clb_ade_steps": ([V, I, I], I),
         "clb_equilibrium": ([V, I], I),
         "clb_ca_spill": ([V, I, C.POINTER(I)], I),
+        "clb_ade_collide_stream": ([V, I, I], I),
+        "clb_kinetics": ([V, I, I], I),
+        "clb_mask_total": ([V, I, C.POINTER(D), C.POINTER(C.c_int64)], I),
+        "clb_ca_sweep": ([V, I], I),
+        "clb_age_update": ([V, I, I], I),
+        "clb_update_solute_dynamics": ([V, I], I),
+        "clb_update_solute_solid": ([V, I, C.POINTER(I)], I),
+        "clb_precip_convert": ([V, I, C.POINTER(I)], I),
         "clb_read_field": ([V, I, I, I, V], I),
         "clb_write_field": ([V, I, I, I, V], I),
         "clb_species_totals": ([V, I, V], I),
@@ -79,9 +88,11 @@ def exported_symbols():
     """Names of the entry points declared in include/clb.h."""
     return [
         "clb_abi_version", "clb_ctx_create", "clb_ctx_destroy", "clb_last_error", "clb_device_name", "clb_sync",
-        "clb_case_create", "clb_case_destroy", "clb_case_footprint", "clb_ns_init", "clb_ns_solve",
+        "clb_case_create", "clb_case_destroy", "clb_case_footprint", "clb_case_get_params", "clb_ns_init", "clb_ns_solve",
         "clb_ns_couple", "clb_ns_stats", "clb_ns_update_dynamics", "clb_set_ade_relaxation",
         "clb_ade_stabilize", "clb_run_steps", "clb_ade_steps", "clb_equilibrium", "clb_ca_spill",
+        "clb_ade_collide_stream", "clb_kinetics", "clb_mask_total", "clb_ca_sweep", "clb_age_update",
+        "clb_update_solute_dynamics", "clb_update_solute_solid", "clb_precip_convert",
         "clb_read_field", "clb_write_field", "clb_species_totals", "clb_counters", "clb_kernel_time",
         "clb_stream_copy",
         "clb_rng_draws", "clb_geom_spheres", "clb_geom_pack",
@@ -165,6 +176,11 @@ class Case:
     def _c(self, rc):
         return self.ctx.check(rc)
 
+    def get_params(self):
+        out = P.CaseParams()
+        self._c(lib().clb_case_get_params(self.ctx.h, self.id, C.byref(out)))
+        return out
+
     def set_omegas(self, omp, omb, dt):
         a = np.ascontiguousarray(omp, dtype=np.float64)
         b = np.ascontiguousarray(omb, dtype=np.float64)
@@ -207,6 +223,37 @@ class Case:
         sw = C.c_int()
         rc = lib().clb_ca_spill(self.ctx.h, self.id, C.byref(sw))
         return rc, sw.value
+
+    # ---- the processors one at a time
+    def ade_collide_stream(self, n):
+        self._c(lib().clb_ade_collide_stream(self.ctx.h, self.id, n))
+
+    def kinetics(self, which=P.KIN_BIOTIC | P.KIN_ABIOTIC):
+        self._c(lib().clb_kinetics(self.ctx.h, self.id, which))
+
+    def mask_total(self):
+        g = C.c_double(); m = C.c_int64()
+        self._c(lib().clb_mask_total(self.ctx.h, self.id, C.byref(g), C.byref(m)))
+        return g.value, m.value
+
+    def ca_sweep(self):
+        self._c(lib().clb_ca_sweep(self.ctx.h, self.id))
+
+    def age_update(self, to_fixed_point=True):
+        self._c(lib().clb_age_update(self.ctx.h, self.id, int(to_fixed_point)))
+
+    def update_solute_dynamics(self):
+        self._c(lib().clb_update_solute_dynamics(self.ctx.h, self.id))
+
+    def update_solute_solid(self):
+        n = C.c_int()
+        self._c(lib().clb_update_solute_solid(self.ctx.h, self.id, C.byref(n)))
+        return n.value
+
+    def precip_convert(self):
+        ch = C.c_int()
+        self._c(lib().clb_precip_convert(self.ctx.h, self.id, C.byref(ch)))
+        return bool(ch.value)
 
     def read(self, field, index=0):
         out = np.empty(self.N)

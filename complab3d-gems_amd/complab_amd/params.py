@@ -21,6 +21,7 @@ BC_DIRICHLET, BC_NEUMANN, BC_CLOSED = 0, 1, 2
 CONV_TRACER, CONV_RELDELTA = 0, 1
 BIO_NONE, BIO_DUAL_MONOD = 0, 1
 CA_FRACTION, CA_HALF = 0, 1
+KIN_BIOTIC, KIN_ABIOTIC = 1, 2
 EQ_AAPCF, EQ_NEWTON = 0, 1
 
 FIELD_CONC, FIELD_BIOMASS, FIELD_BTOTAL, FIELD_MASK, FIELD_VEL, FIELD_AGE, FIELD_PHASE, FIELD_NSRHO = range(8)

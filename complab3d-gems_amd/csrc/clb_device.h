@@ -145,6 +145,7 @@ struct EqDev {
 
 // host launchers: compact lattices (clb_lattice.hip)
 template <typename R> hipError_t launchAdeStep(const AdeDev<R> &, const KinDev &, hipStream_t);
+template <typename R> hipError_t launchKinetics(const AdeDev<R> &, const KinDev &, hipStream_t);
 template <typename R> hipError_t launchAdeDensity(const AdeDev<R> &, int, double *, hipStream_t);
 template <typename R> hipError_t launchAdeSet(const AdeDev<R> &, int, const double *, hipStream_t);
 template <typename R> hipError_t launchAdeReopen(const AdeDev<R> &, const uint8_t *, hipStream_t);

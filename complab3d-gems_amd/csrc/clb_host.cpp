@@ -60,6 +60,15 @@ struct CaseBase {
     virtual void counters(hipStream_t, int64_t out[8]) = 0;
     virtual void kernelTime(hipStream_t, int reset, double *ms, int64_t *n) = 0;
     virtual void setRelaxation(const double *omp, const double *omb, double dt) = 0;
+    // the processors one at a time (host facade of the reference's processor API)
+    virtual void adeCollideStream(hipStream_t, int n) = 0;
+    virtual void kinetics(hipStream_t, int which) = 0;
+    virtual void maskTotalNow(hipStream_t, double *gmax, long long *msum) = 0;
+    virtual void caSweep(hipStream_t) = 0;
+    virtual Status ageUpdate(hipStream_t, bool toFixedPoint) = 0;
+    virtual void soluteDynamics(hipStream_t) = 0;
+    virtual void soluteSolid(hipStream_t, int *reopened) = 0;
+    virtual void precipConvert(hipStream_t, int *changed) = 0;
 };
 
 template <typename T>
@@ -102,7 +111,7 @@ struct Case : CaseBase {
     long long iT = 0;
     int nsSaturate = 0;
     long long oldTotMask = 0;
-    uint64_t caSweep = 0;
+    uint64_t caSweep_ = 0;
     int64_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> evpool;
     size_t evUsed = 0;
@@ -543,6 +552,47 @@ struct Case : CaseBase {
         flushTimes(st);
     }
     void equilibrium(hipStream_t st) override { equilibriumQ(st, nullptr); }
+
+    // ---- the processors one at a time
+    void adeCollideStream(hipStream_t st, int n) override {
+        for (int t = 0; t < n; ++t) fusedLaunch(st, false);
+        flushTimes(st);
+    }
+    void kinetics(hipStream_t st, int which) override {
+        KinDev k = kin(true);
+        if (!(which & CLB_KIN_BIOTIC)) k.bio = 0;
+        if (!(which & CLB_KIN_ABIOTIC)) k.nrxn = 0;
+        if (!k.bio && k.nrxn == 0) k.on = 0;
+        HIPT(launchKinetics<R>(ade(), k, st));
+    }
+    void maskTotalNow(hipStream_t st, double *gmax, long long *msum) override { maskTotal(st, gmax, msum); }
+    void caSweep(hipStream_t st) override {
+        if (p.n_microbes <= 0) return;
+        HIPT(launchCaPushGather<R>(ca(), caSweep_++, st));
+        cnt[6]++;
+    }
+    Status ageUpdate(hipStream_t st, bool toFixedPoint) override { return ageRelax(st, toFixedPoint); }
+    void soluteDynamics(hipStream_t st) override { HIPT(launchOmegaSwitch(mask, list, adeCls, Nopen, st)); }
+    void soluteSolid(hipStream_t st, int *reopenedOut) override {
+        unsigned int nre = 0;
+        HIPT(hipMemsetAsync(ucnt + 1, 0, sizeof(unsigned int), st));
+        HIPT(launchSolidSwitch(mask, list, adeCls, reopened, Nopen, ucnt + 1, st));
+        HIPT(hipMemcpyAsync(&nre, ucnt + 1, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        HIPT(hipStreamSynchronize(st));
+        if (nre) HIPT(launchAdeReopen<R>(ade(), reopened, st));
+        adeClsDirty = true;
+        if (reopenedOut) *reopenedOut = (int) nre;
+    }
+    void precipConvert(hipStream_t st, int *changed) override {
+        unsigned int ch = 0;
+        if (p.precip_species >= 0 && p.precip_species < p.n_species && !p.mobile[p.precip_species]) {
+            HIPT(hipMemsetAsync(ucnt, 0, sizeof(unsigned int), st));
+            HIPT(launchPrecip<R>(nx, ny, N, mask, imm + (size_t) slot[p.precip_species] * N, p.precip_max_rho, p.precip_phase_id, phase, ucnt, st));
+            HIPT(hipMemcpyAsync(&ch, ucnt, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+            HIPT(hipStreamSynchronize(st));
+        }
+        if (changed) *changed = ch ? 1 : 0;
+    }
     void equilibriumQ(hipStream_t st, const int *halt) {
         if (!p.eq_on || p.eq_nc <= 0) return;
         HIPT(launchEquilibrium<R>(ade(halt), eqdev(), ucnt + 2, st));
@@ -557,14 +607,16 @@ struct Case : CaseBase {
         *msum = ctlHost->msum;
     }
     // updateAgeDistance3D sweeps; to the fixed point (sum of increments 0) or once
-    // A sweep at the fixed point changes nothing, so sweeps are queued 16 at a
-    // time and only the last one's sum of increments is read back; the result
-    // and the 1001-sweep limit are those of one sweep per read-back.
+    // A sweep at the fixed point changes nothing, so sweeps are queued several
+    // at a time and only the last one's sum of increments is read back; the
+    // result and the 1001-sweep limit are those of one sweep per read-back.
+    // Most relaxations settle within three sweeps: the first read-back comes
+    // after three, later ones after sixteen.
     Status ageRelax(hipStream_t st, bool toFixedPoint) {
         const unsigned int g = nblk(N, 256);
         int swept = 0;
         while (true) {
-            const int batch = toFixedPoint ? std::min(16, 1001 - swept) : 1;
+            const int batch = toFixedPoint ? std::min(swept == 0 ? 3 : 16, 1001 - swept) : 1;
             for (int k = 0; k < batch; ++k) {
                 HIPT(launchCaAge<R>(ca(), age, age2, partsLL, st));
                 std::swap(age, age2);
@@ -594,7 +646,7 @@ struct Case : CaseBase {
         if (gmax - p.bmax > 1e-12) {
             double prev = gmax; int stall = 0, wc = 0;
             while (gmax - p.bmax > 1e-12) {
-                HIPT(launchCaPushGather<R>(ca(), caSweep++, st));
+                HIPT(launchCaPushGather<R>(ca(), caSweep_++, st));
                 maskTotal(st, &gmax, &msum);
                 ++sweeps; cnt[6]++;
                 if (!std::isfinite(gmax)) { if (sweepsOut) *sweepsOut = sweeps; return {CLB_E_NONFINITE, "non-finite biomass"}; }
@@ -978,6 +1030,14 @@ int clb_case_destroy(clb_ctx *ctx, int case_id) {
     return CLB_OK;
 }
 
+int clb_case_get_params(clb_ctx *ctx, int case_id, clb_case_params *out) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (!out) return fail(ctx, CLB_E_ARG, "null output");
+    *out = c->p;
+    return CLB_OK;
+    GUARD_END
+}
 int clb_set_ade_relaxation(clb_ctx *ctx, int case_id, const double *omp, const double *omb, double dt) {
     GUARD_BEGIN
     CASE_OR_FAIL(c)
@@ -1048,6 +1108,73 @@ int clb_ade_steps(clb_ctx *ctx, int case_id, int nsteps) {
     CASE_OR_FAIL(c)
     if (nsteps < 0) return fail(ctx, CLB_E_ARG, "negative step count");
     c->adeSteps(ctx->stream, nsteps);
+    return CLB_OK;
+    GUARD_END
+}
+int clb_ade_collide_stream(clb_ctx *ctx, int case_id, int nsteps) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (nsteps < 0) return fail(ctx, CLB_E_ARG, "negative step count");
+    c->adeCollideStream(ctx->stream, nsteps);
+    return CLB_OK;
+    GUARD_END
+}
+int clb_kinetics(clb_ctx *ctx, int case_id, int which) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (which < 0 || which > (CLB_KIN_BIOTIC | CLB_KIN_ABIOTIC)) return fail(ctx, CLB_E_ARG, "bad kinetics selector");
+    c->kinetics(ctx->stream, which);
+    HIPT(hipStreamSynchronize(ctx->stream));
+    return CLB_OK;
+    GUARD_END
+}
+int clb_mask_total(clb_ctx *ctx, int case_id, double *max_total, int64_t *mask_sum) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (c->p.n_microbes <= 0) return fail(ctx, CLB_E_STATE, "case has no CA biomass");
+    double g = 0; long long m = 0;
+    c->maskTotalNow(ctx->stream, &g, &m);
+    if (max_total) *max_total = g;
+    if (mask_sum) *mask_sum = m;
+    return CLB_OK;
+    GUARD_END
+}
+int clb_ca_sweep(clb_ctx *ctx, int case_id) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (c->p.n_microbes <= 0) return fail(ctx, CLB_E_STATE, "case has no CA biomass");
+    c->caSweep(ctx->stream);
+    HIPT(hipStreamSynchronize(ctx->stream));
+    return CLB_OK;
+    GUARD_END
+}
+int clb_age_update(clb_ctx *ctx, int case_id, int to_fixed_point) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    Status s = c->ageUpdate(ctx->stream, to_fixed_point != 0);
+    return s.code ? fail(ctx, s.code, s.msg) : CLB_OK;
+    GUARD_END
+}
+int clb_update_solute_dynamics(clb_ctx *ctx, int case_id) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    c->soluteDynamics(ctx->stream);
+    HIPT(hipStreamSynchronize(ctx->stream));
+    return CLB_OK;
+    GUARD_END
+}
+int clb_update_solute_solid(clb_ctx *ctx, int case_id, int *reopened) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    c->soluteSolid(ctx->stream, reopened);
+    return CLB_OK;
+    GUARD_END
+}
+int clb_precip_convert(clb_ctx *ctx, int case_id, int *changed) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (!c->p.precip_on) return fail(ctx, CLB_E_STATE, "case was created with precip_on = 0");
+    c->precipConvert(ctx->stream, changed);
     return CLB_OK;
     GUARD_END
 }

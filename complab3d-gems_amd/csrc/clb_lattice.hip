@@ -179,6 +179,103 @@ __device__ __forceinline__ double clampInc(int clamp, double d, double cur, unsi
 // ------------------------------------------------------------------------------------------
 // advection-diffusion
 // ------------------------------------------------------------------------------------------
+// biotic and abiotic kinetics of one open interior voxel on its post-collision
+// populations f (every mobile species in registers): rates from the densities,
+// the clamp, the D3Q7 deposit w_i dC, immobile species and biomass in place
+// (run_kinetics + update_rxnLattices, part1:85-187, 406-497; the abiotic pair
+// part1:199-404; surfaceAbioticKinetics3D's face gate)
+template <typename R, int MAXS>
+__device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &k, int i, R (&f)[MAXS][7]) {
+    const long long No = a.Nopen;
+    const long long v = a.list[i];
+    const int mask = a.mask[v];
+    if (mask != 0 && mask != 1) {
+        // species loops run to the static bound so that the slot selection in
+        // concOf / depositTo resolves to fixed registers (a dynamic trip count
+        // left f indexed at run time: 256 VGPRs instead of 164 at 4 species)
+        unsigned int binds = 0;
+        double C[CLB_DEV_MAXS], sr[CLB_DEV_MAXS];
+        const int S = k.S;
+        const long long N = a.N;
+        auto concOf = [&](int s) -> double {
+            const int sl = a.slot[s];
+            if (a.mobile[s]) {
+                double r = 0;
+#pragma unroll
+                for (int m2 = 0; m2 < MAXS; ++m2) if (m2 == sl) {
+#pragma unroll
+                    for (int q = 0; q < 7; ++q) r += (double) f[m2][q];
+                }
+                return r;
+            }
+            return (double) a.imm[(long long) sl * N + v];
+        };
+        auto depositTo = [&](int s, double d) {
+            const int sl = a.slot[s];
+            if (a.mobile[s]) {
+#pragma unroll
+                for (int m2 = 0; m2 < MAXS; ++m2) if (m2 == sl) {
+#pragma unroll
+                    for (int q = 0; q < 7; ++q) f[m2][q] = (R) ((double) f[m2][q] + w7(q) * d);
+                }
+            } else {
+                R *p = a.imm + (long long) sl * N + v;
+                *p = (R) ((double) *p + d);
+            }
+        };
+        if (k.bio && k.M > 0) {
+            double Bv[CLB_DEV_MAXM], br[CLB_DEV_MAXM];
+            
+#pragma unroll
+            for (int s = 0; s < CLB_DEV_MAXS; ++s) { if (s < S) { C[s] = concOf(s); if (C[s] < 1e-12) C[s] = 0; } }
+            for (int m = 0; m < k.M; ++m) { Bv[m] = (double) a.B[(long long) m * N + v]; if (Bv[m] < 1e-12) Bv[m] = 0; }
+            bioticLaw(k, C, Bv, sr, br, mask);
+#pragma unroll
+            for (int s = 0; s < CLB_DEV_MAXS; ++s) {
+                if (s >= S) continue;
+                const double d = sr[s] * k.dt;
+                if (d > 1e-12 || d < -1e-12) depositTo(s, clampInc(k.clamp, d, concOf(s), binds));
+            }
+            for (int m = 0; m < k.M; ++m) {
+                double d = br[m] * k.dt;
+                if (d > 1e-12 || d < -1e-12) {
+                    R *p = a.B + (long long) m * N + v;
+                    const double b = (double) *p;
+                    if (d < -b) d = -b;                               // biomass never negative
+                    *p = (R) (b + d);
+                }
+            }
+        }
+        if (k.nrxn > 0) {
+            bool react = true;
+            if (k.surface_only) {                                     // precipitationVOP.hh:78-88
+                react = false;
+#pragma unroll
+                for (int q = 1; q < 7; ++q) {
+                    const int n = a.nb[(opp7(q) - 1) * No + i];        // the voxel at +c_q
+                    if (n == NB_OUTSIDE) continue;
+                    if (n == NB_CLOSED) { react = true; continue; }
+                    const int nm = a.mask[a.list[n]];
+                    if (nm == 0 || nm == 1) react = true;
+                }
+            }
+            if (react) {
+                
+#pragma unroll
+            for (int s = 0; s < CLB_DEV_MAXS; ++s) { if (s < S) { C[s] = concOf(s); if (C[s] < 1e-12) C[s] = 0; } }
+                abioticLaw(k, C, sr);
+#pragma unroll
+                for (int s = 0; s < CLB_DEV_MAXS; ++s) {
+                    if (s >= S) continue;
+                    const double d = sr[s] * k.dt;
+                    if (d > 1e-12 || d < -1e-12) depositTo(s, clampInc(k.clamp, d, concOf(s), binds));
+                }
+            }
+        }
+        if (binds) atomicAdd(a.clampCount, (unsigned long long) binds);
+    }
+}
+
 // interior: pull + BGK collide + biotic and abiotic kinetics with the clamp
 // and the D3Q7 deposit, every mobile species of a voxel in one thread
 template <typename R, int MAXS>
@@ -249,84 +346,7 @@ __global__ void __launch_bounds__(256) k_ade_bulk(AdeDev<R> a, KinDev k) {
         bgk7<R>(f[ms], (R) (cls == ADE_BFILM ? a.omB[s] : a.omP[s]), ux, uy, uz);
     }
 
-    if (k.on) {
-        const long long v = a.list[i];
-        const int mask = a.mask[v];
-        if (mask != 0 && mask != 1) {
-            unsigned int binds = 0;
-            double C[CLB_DEV_MAXS], sr[CLB_DEV_MAXS];
-            const int S = k.S;
-            const long long N = a.N;
-            auto concOf = [&](int s) -> double {
-                const int sl = a.slot[s];
-                if (a.mobile[s]) {
-                    double r = 0;
-#pragma unroll
-                    for (int m2 = 0; m2 < MAXS; ++m2) if (m2 == sl) {
-#pragma unroll
-                        for (int q = 0; q < 7; ++q) r += (double) f[m2][q];
-                    }
-                    return r;
-                }
-                return (double) a.imm[(long long) sl * N + v];
-            };
-            auto depositTo = [&](int s, double d) {
-                const int sl = a.slot[s];
-                if (a.mobile[s]) {
-#pragma unroll
-                    for (int m2 = 0; m2 < MAXS; ++m2) if (m2 == sl) {
-#pragma unroll
-                        for (int q = 0; q < 7; ++q) f[m2][q] = (R) ((double) f[m2][q] + w7(q) * d);
-                    }
-                } else {
-                    R *p = a.imm + (long long) sl * N + v;
-                    *p = (R) ((double) *p + d);
-                }
-            };
-            if (k.bio && k.M > 0) {
-                double Bv[CLB_DEV_MAXM], br[CLB_DEV_MAXM];
-                for (int s = 0; s < S; ++s) { C[s] = concOf(s); if (C[s] < 1e-12) C[s] = 0; }
-                for (int m = 0; m < k.M; ++m) { Bv[m] = (double) a.B[(long long) m * N + v]; if (Bv[m] < 1e-12) Bv[m] = 0; }
-                bioticLaw(k, C, Bv, sr, br, mask);
-                for (int s = 0; s < S; ++s) {
-                    const double d = sr[s] * k.dt;
-                    if (d > 1e-12 || d < -1e-12) depositTo(s, clampInc(k.clamp, d, concOf(s), binds));
-                }
-                for (int m = 0; m < k.M; ++m) {
-                    double d = br[m] * k.dt;
-                    if (d > 1e-12 || d < -1e-12) {
-                        R *p = a.B + (long long) m * N + v;
-                        const double b = (double) *p;
-                        if (d < -b) d = -b;                               // biomass never negative
-                        *p = (R) (b + d);
-                    }
-                }
-            }
-            if (k.nrxn > 0) {
-                bool react = true;
-                if (k.surface_only) {                                     // precipitationVOP.hh:78-88
-                    react = false;
-#pragma unroll
-                    for (int q = 1; q < 7; ++q) {
-                        const int n = a.nb[(opp7(q) - 1) * No + i];        // the voxel at +c_q
-                        if (n == NB_OUTSIDE) continue;
-                        if (n == NB_CLOSED) { react = true; continue; }
-                        const int nm = a.mask[a.list[n]];
-                        if (nm == 0 || nm == 1) react = true;
-                    }
-                }
-                if (react) {
-                    for (int s = 0; s < S; ++s) { C[s] = concOf(s); if (C[s] < 1e-12) C[s] = 0; }
-                    abioticLaw(k, C, sr);
-                    for (int s = 0; s < S; ++s) {
-                        const double d = sr[s] * k.dt;
-                        if (d > 1e-12 || d < -1e-12) depositTo(s, clampInc(k.clamp, d, concOf(s), binds));
-                    }
-                }
-            }
-            if (binds) atomicAdd(a.clampCount, (unsigned long long) binds);
-        }
-    }
+    if (k.on) voxelKinetics<R, MAXS>(a, k, i, f);
 
 #pragma unroll
     for (int ms = 0; ms < MAXS; ++ms) {
@@ -334,6 +354,33 @@ __global__ void __launch_bounds__(256) k_ade_bulk(AdeDev<R> a, KinDev k) {
         R *o = a.out + (long long) ms * 7 * No;
 #pragma unroll
         for (int q = 0; q < 7; ++q) o[q * No + i] = f[ms][q];
+    }
+}
+
+// the kinetics alone on the stored post-collision state (the split form of
+// the fused kernel, for callers that apply the reference's processors one by
+// one); bit-identical to the fused kernel's kinetics
+template <typename R, int MAXS>
+__global__ void __launch_bounds__(256) k_kinetics(AdeDev<R> a, KinDev k) {
+    const int i = xcdBlock() * blockDim.x + threadIdx.x;
+    if (i >= a.Nint) return;
+    if (a.cls[i] == ADE_BB) return;
+    const long long No = a.Nopen;
+    R f[MAXS][7];
+#pragma unroll
+    for (int ms = 0; ms < MAXS; ++ms) {
+        if (ms >= a.Sm) continue;
+        const R *c = a.cur + (long long) ms * 7 * No;
+#pragma unroll
+        for (int q = 0; q < 7; ++q) f[ms][q] = c[q * No + i];
+    }
+    voxelKinetics<R, MAXS>(a, k, i, f);
+#pragma unroll
+    for (int ms = 0; ms < MAXS; ++ms) {
+        if (ms >= a.Sm) continue;
+        R *c = a.cur + (long long) ms * 7 * No;
+#pragma unroll
+        for (int q = 0; q < 7; ++q) c[q * No + i] = f[ms][q];
     }
 }
 
@@ -818,6 +865,18 @@ hipError_t launchAdeStep(const AdeDev<R> &a, const KinDev &k, hipStream_t st) {
     return hipGetLastError();
 }
 template <typename R>
+hipError_t launchKinetics(const AdeDev<R> &a, const KinDev &k, hipStream_t st) {
+    if (a.Nint <= 0 || !k.on) return hipSuccess;
+    const unsigned int g = nblk(a.Nint, 256);
+    if (a.Sm <= 1) hipLaunchKernelGGL((k_kinetics<R, 1>), dim3(g), dim3(256), 0, st, a, k);
+    else if (a.Sm <= 2) hipLaunchKernelGGL((k_kinetics<R, 2>), dim3(g), dim3(256), 0, st, a, k);
+    else if (a.Sm <= 3) hipLaunchKernelGGL((k_kinetics<R, 3>), dim3(g), dim3(256), 0, st, a, k);
+    else if (a.Sm <= 4) hipLaunchKernelGGL((k_kinetics<R, 4>), dim3(g), dim3(256), 0, st, a, k);
+    else if (a.Sm <= 6) hipLaunchKernelGGL((k_kinetics<R, 6>), dim3(g), dim3(256), 0, st, a, k);
+    else hipLaunchKernelGGL((k_kinetics<R, 8>), dim3(g), dim3(256), 0, st, a, k);
+    return hipGetLastError();
+}
+template <typename R>
 hipError_t launchAdeDensity(const AdeDev<R> &a, int ms, double *dst, hipStream_t st) {
     hipLaunchKernelGGL(k_ade_density<R>, dim3(nblk(a.Nopen, 256)), dim3(256), 0, st, a, ms, dst);
     return hipGetLastError();
@@ -882,6 +941,7 @@ hipError_t launchGatherCompact(const double *src, const int *list, int n, R *dst
 
 #define INST(R) \
     template hipError_t launchAdeStep<R>(const AdeDev<R> &, const KinDev &, hipStream_t); \
+    template hipError_t launchKinetics<R>(const AdeDev<R> &, const KinDev &, hipStream_t); \
     template hipError_t launchAdeDensity<R>(const AdeDev<R> &, int, double *, hipStream_t); \
     template hipError_t launchAdeSet<R>(const AdeDev<R> &, int, const double *, hipStream_t); \
     template hipError_t launchAdeReopen<R>(const AdeDev<R> &, const uint8_t *, hipStream_t); \

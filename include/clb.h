@@ -182,6 +182,9 @@ int  clb_case_create(clb_ctx *ctx, const clb_case_params *p, const int8_t *geome
 int  clb_case_destroy(clb_ctx *ctx, int case_id);
 /* device bytes held by a case built from these parameters (admission control) */
 size_t clb_case_footprint(const clb_case_params *p);
+/* the parameters a case runs with (after clb_set_ade_relaxation); the values
+ * the reference's processors receive as constructor arguments */
+int  clb_case_get_params(clb_ctx *ctx, int case_id, clb_case_params *out);
 
 /* ---- flow ---------------------------------------------------------------- */
 /* NSdomainSetup(...) + initializeAtEquilibrium(PressureGradient)
@@ -223,6 +226,36 @@ int  clb_ade_steps(clb_ctx *ctx, int case_id, int nsteps);
 int  clb_equilibrium(clb_ctx *ctx, int case_id);
 /* the CA block of complab.cpp:2793-3055 once; *sweeps = push/pull sweeps */
 int  clb_ca_spill(clb_ctx *ctx, int case_id, int *sweeps);
+
+/* ---- the processors one at a time -------------------------------------------
+ * For callers that keep the reference's loop and apply its processors one by
+ * one (include/clb_processors.hh).  Between calls a case holds post-collision,
+ * pre-stream populations: the state the reference's processors see between
+ * collide() and stream().  A transport launch streams that state and collides. */
+/* ADE collide + stream without kinetics, nsteps times (complab.cpp:2307-2315,
+ * 3251-3259) */
+int  clb_ade_collide_stream(clb_ctx *ctx, int case_id, int nsteps);
+#define CLB_KIN_BIOTIC  1   /* run_kinetics + update_rxnLattices (part1:85-187, 406-497)            */
+#define CLB_KIN_ABIOTIC 2   /* run_abiotic_kinetics + update_abiotic_rxnLattices (part1:199-404)   */
+/* rates, clamp and D3Q7 deposit on the current post-collision state; biotic
+ * first, then abiotic, as the main loop applies them */
+int  clb_kinetics(clb_ctx *ctx, int case_id, int which);
+/* updateLocalMaskNtotalLattices3D (part2:71-259) + the max of the total biomass
+ * and the mask sum the main loop reduces (complab.cpp:1983, 2793-2800) */
+int  clb_mask_total(clb_ctx *ctx, int case_id, double *max_total, int64_t *mask_sum);
+/* one sweep of pushExcessBiomass3D / halfPushExcessBiomass3D (part1:500-970),
+ * the rule chosen by ca_method */
+int  clb_ca_sweep(clb_ctx *ctx, int case_id);
+/* updateAgeDistance3D (part2:840-965): one sweep, or sweeps to the fixed point */
+int  clb_age_update(clb_ctx *ctx, int case_id, int to_fixed_point);
+/* updateSoluteDynamics3D (part2:485-553): pore / biofilm relaxation per voxel */
+int  clb_update_solute_dynamics(clb_ctx *ctx, int case_id);
+/* updateSoluteSolidDynamics3D (part2:585-647): bounce-back on solid voxels,
+ * reopened voxels rebuilt at rest; *reopened = voxels reopened */
+int  clb_update_solute_solid(clb_ctx *ctx, int case_id, int *reopened);
+/* precipNodeConversion3D (precipitationVOP.hh:166-227) over the reaction box;
+ * *changed = 1 if any voxel turned solid */
+int  clb_precip_convert(clb_ctx *ctx, int case_id, int *changed);
 
 /* ---- fields ---------------------------------------------------------------- */
 #define CLB_FIELD_CONC     0  /* species concentration (density after stream) */

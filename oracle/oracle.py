@@ -162,6 +162,10 @@ class OracleCase:
         rc = lib().orc_ca_spill(self.h, C.byref(sw))
         return rc, sw.value
 
+    def precip_convert(self):
+        """precipNodeConversion3D over the reaction box; True if any voxel converted."""
+        return bool(lib().orc_precip_convert(self.h))
+
     def age_relax(self, to_fixed_point=True):
         k = C.c_int()
         self._chk(lib().orc_age_relax(self.h, int(to_fixed_point), C.byref(k)))

@@ -73,3 +73,54 @@ def test_no_device_is_an_error_code_not_a_crash():
                        env=dict(os.environ, HIP_VISIBLE_DEVICES=os.environ.get("HIP_VISIBLE_DEVICES", "")))
     assert r.returncode == 0, r.stderr
     assert r.stdout.startswith("DEVICE") or r.stdout.startswith("ERR")
+
+
+FACADE_PROGRAM = r'''
+#include "clb_processors.hh"
+#include <cstdio>
+#include <vector>
+using namespace clb_proc;
+struct FakeEqChem {};
+int main() {
+    Target t = {nullptr, 0};           // no context: every process() must report CLB_E_ARG
+    std::vector<plint> pore(1, 2);
+    std::vector<std::vector<plint> > bio(1, std::vector<plint>(1, 3));
+    std::vector<std::vector<double> > Kc(1, std::vector<double>(2, 0.1));
+    FakeEqChem eq;
+    int bad = 0;
+    bad += applyProcessingFunctional(new run_kinetics<double>(66, 2, 1, 1.0, Kc, std::vector<double>(1, 1e-3), 0, 1), t) != CLB_E_ARG;
+    bad += applyProcessingFunctional(new run_abiotic_kinetics<double>(66, 2, 1.0, 0, 1), t) != CLB_E_ARG;
+    bad += applyProcessingFunctional(new surfaceAbioticKinetics3D<double>(66, 32, 32, 2, 1.0, 0, 1, 1), t) != CLB_E_ARG;
+    bad += applyProcessingFunctional(new run_equilibrium_biotic<double, FakeEqChem>(66, 2, eq, 0, 1), t) != CLB_E_ARG;
+    bad += applyProcessingFunctional(new pushExcessBiomass3D<double>(100.0, 66, 32, 32, 1, 66, 0, 1, pore), t) != CLB_E_ARG;
+    bad += applyProcessingFunctional(new halfPushExcessBiomass3D<double>(100.0, 66, 32, 32, 1, 66, 0, 1, pore), t) != CLB_E_ARG;
+    bad += applyProcessingFunctional(new updateAgeDistance3D<double>(100.0, 66, 32, 32), t) != CLB_E_ARG;
+    bad += applyProcessingFunctional(new updateSoluteDynamics3D<double>(2, 1, 0, pore, std::vector<double>(2, 1.0), std::vector<double>(2, 1.0)), t) != CLB_E_ARG;
+    bad += applyProcessingFunctional(new updateSoluteSolidDynamics3D<double>(2, 1, 0, pore, std::vector<double>(2, 1.0), std::vector<double>(2, 1.0), std::vector<bool>(2, false)), t) != CLB_E_ARG;
+    bad += applyProcessingFunctional(new updateNsLatticesDynamics3D<double>(1.25, 0.1, pore, 0, 1), t) != CLB_E_ARG;
+    bad += applyProcessingFunctional(new precipNodeConversion3D<double>(66, 32, 32, 1.0, 0, 1, 0, 3), t) != CLB_E_ARG;
+    updateLocalMaskNtotalLattices3D<double> mt(66, 32, 32, 66, 1, 0, bio, pore, 0.1, 100.0);
+    bad += applyProcessingFunctional(mt, t) != CLB_E_ARG;
+    MaskedPopulationSumFunctional3D<double> sums(pore, 0.0);
+    bad += applyProcessingFunctional(sums, t) != CLB_E_ARG;
+    bad += applyProcessingFunctional(new update_rxnLattices<double>(66, 2, 1, 0, 1), t) != CLB_OK;
+    std::printf("%d\n", bad);
+    return bad;
+}
+'''
+
+
+def test_processor_facade_compiles_and_links(tmp_path):
+    """include/clb_processors.hh builds as C++11 against clb.h, links against
+    libclb.so, and every processor reports a missing context as CLB_E_ARG."""
+    import subprocess
+    src = tmp_path / "facade.cpp"
+    src.write_text(FACADE_PROGRAM)
+    exe = tmp_path / "facade"
+    libdir = os.path.dirname(clb.LIB_PATH)
+    cmd = ["g++", "-std=c++11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
+           "-o", str(exe), "-L", libdir, "-lclb", "-Wl,-rpath," + libdir]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and r.stdout.strip() == "0", r.stdout + r.stderr

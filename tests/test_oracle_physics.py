@@ -271,3 +271,70 @@ def test_age_relaxation_reaches_the_reference_fixed_point():
     dist[(geom >= 2) & inner] = 1
     gs = gauss_seidel_age(age0, dist, b, p.bmax)
     assert np.array_equal(jac, gs)
+
+
+def monod_params(mu, ks, kd, ke=0.1, acceptor=-1):
+    p = P.default_params(3, 1, 1, 3)
+    p.bio_law = P.BIO_DUAL_MONOD
+    p.n_microbes = 1
+    p.donor = 0
+    p.acceptor = acceptor
+    p.mu_max[0] = mu; p.ks[0] = ks; p.ke[0] = ke; p.kdecay[0] = kd
+    p.bio_stoich[0][0] = -2.0
+    p.bio_stoich[0][1] = -0.5
+    p.bio_stoich[0][2] = 0.25
+    return p
+
+
+def test_monod_rates_at_closed_form_points():
+    """defineKinetics.biotic.hh:90-126 (dual Monod).  At S = Ks the Monod factor
+    is exactly 1/2; with S >> Ks growth is zero order (mu_max B); with the
+    acceptor at E = Ke a second exact 1/2; decay enters with a minus sign and
+    the stoichiometry scales growth only; a closed voxel reacts not at all."""
+    mu, ks, kd, b = 0.3, 0.2, 0.01, 5.0
+    p = monod_params(mu, ks, kd)
+    sr, br, _ = O.rates(p, [ks, 1.0, 0.0], [b], 3)
+    assert br[0] == pytest.approx((0.5 * mu - kd) * b, rel=1e-15)
+    np.testing.assert_allclose(sr, np.array([-2.0, -0.5, 0.25]) * 0.5 * mu * b, rtol=1e-15)
+    sr, br, _ = O.rates(p, [1e12 * ks, 1.0, 0.0], [b], 3)
+    assert br[0] == pytest.approx((mu - kd) * b, rel=1e-11)
+    p2 = monod_params(mu, ks, kd, ke=0.4, acceptor=1)
+    sr, br, _ = O.rates(p2, [ks, 0.4, 0.0], [b], 3)
+    assert br[0] == pytest.approx((0.25 * mu - kd) * b, rel=1e-15)
+    assert sr[0] == pytest.approx(-2.0 * 0.25 * mu * b, rel=1e-15)
+    sr, br, _ = O.rates(p, [ks, 1.0, 0.0], [b], P.SOLID)
+    assert np.all(sr == 0) and np.all(br == 0)
+    sr, br, _ = O.rates(p, [ks, 1.0, 0.0], [0.0], 3)
+    assert np.all(sr == 0) and np.all(br == 0)
+
+
+def test_precipitation_converts_at_threshold():
+    """precipNodeConversion3D (precipitationVOP.hh:166-227): an open voxel of the
+    reaction box whose mineral holds at least max_precipRho turns solid and
+    takes the phase id; below the threshold, on the ghost columns and in
+    closed voxels nothing changes."""
+    nx, ny, nz = 8, 3, 3
+    p = P.default_params(nx, ny, nz, 2)
+    p.mobile[1] = 0
+    p.precip_on = 1; p.precip_species = 1; p.precip_max_rho = 2.0
+    p.precip_interval = 1; p.precip_phase_id = 5
+    geom = np.full((nz, ny, nx), P.PORE, np.int8)
+    geom[1, 1, 4] = P.WALL
+    o = O.OracleCase(p, geom)
+    m = np.zeros((nz, ny, nx))
+    m[0, 0, 2] = 2.0                       # exactly at the threshold: converts
+    m[0, 0, 3] = 2.0 - 1e-12               # just below: stays open
+    m[2, 2, 5] = 7.0                       # well above: converts
+    m[1, 1, 0] = 9.0                       # ghost column: outside the reaction box
+    m[1, 1, 4] = 9.0                       # already closed
+    o.write(P.FIELD_CONC, 1, m)
+    assert o.precip_convert()
+    mask = o.read(P.FIELD_MASK)
+    phase = o.read(P.FIELD_PHASE)
+    conv = np.zeros_like(mask, bool)
+    conv[0, 0, 2] = conv[2, 2, 5] = True
+    assert np.all(mask[conv] == P.SOLID) and np.all(phase[conv] == 5)
+    assert mask[0, 0, 3] == P.PORE and mask[1, 1, 0] == P.PORE and mask[1, 1, 4] == P.WALL
+    assert np.all(phase[~conv] == 0)
+    assert not o.precip_convert()          # nothing left above the threshold
+    o.close()
