@@ -66,6 +66,7 @@ def lib():
         "clb_update_solute_dynamics": ([V, I], I),
         "clb_update_solute_solid": ([V, I, C.POINTER(I)], I),
         "clb_precip_convert": ([V, I, C.POINTER(I)], I),
+        "clb_dissol_convert": ([V, I, C.POINTER(I)], I),
         "clb_read_field": ([V, I, I, I, V], I),
         "clb_write_field": ([V, I, I, I, V], I),
         "clb_species_totals": ([V, I, V], I),
@@ -92,7 +93,7 @@ def exported_symbols():
         "clb_ns_couple", "clb_ns_stats", "clb_ns_update_dynamics", "clb_set_ade_relaxation",
         "clb_ade_stabilize", "clb_run_steps", "clb_ade_steps", "clb_equilibrium", "clb_ca_spill",
         "clb_ade_collide_stream", "clb_kinetics", "clb_mask_total", "clb_ca_sweep", "clb_age_update",
-        "clb_update_solute_dynamics", "clb_update_solute_solid", "clb_precip_convert",
+        "clb_update_solute_dynamics", "clb_update_solute_solid", "clb_precip_convert", "clb_dissol_convert",
         "clb_read_field", "clb_write_field", "clb_species_totals", "clb_counters", "clb_kernel_time",
         "clb_stream_copy",
         "clb_rng_draws", "clb_geom_spheres", "clb_geom_pack",
@@ -255,6 +256,11 @@ class Case:
         self._c(lib().clb_precip_convert(self.ctx.h, self.id, C.byref(ch)))
         return bool(ch.value)
 
+    def dissol_convert(self):
+        n = C.c_int()
+        self._c(lib().clb_dissol_convert(self.ctx.h, self.id, C.byref(n)))
+        return n.value
+
     def read(self, field, index=0):
         out = np.empty(self.N)
         self._c(lib().clb_read_field(self.ctx.h, self.id, field, index, _p(out)))
@@ -270,7 +276,7 @@ class Case:
         return out
 
     def counters(self):
-        out = np.zeros(8, dtype=np.int64)
+        out = np.zeros(12, dtype=np.int64)
         self._c(lib().clb_counters(self.ctx.h, self.id, _p(out)))
         return out
 

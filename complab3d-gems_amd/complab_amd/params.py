@@ -88,12 +88,17 @@ class CaseParams(C.Structure):
         ("precip_phase_id", _i),
         ("dissol_on", _i),
         ("dissol_species", _i),
-        ("dissol_k", _d),
-        ("dissol_product", _i),
-        ("dissol_full", _d),
-        ("dissol_reopen_frac", _d),
-        ("dissol_interval", _i),
+        ("dissol_material", _i),
+        ("dissol_is_precipitate", _i),
         ("dissol_phase_id", _i),
+        ("dissol_surface_only", _i),
+        ("dissol_interval", _i),
+        ("dissol_acid", _i),
+        ("dissol_full", _d),
+        ("dissol_initial", _d),
+        ("dissol_reopen_frac", _d),
+        ("dissol_k", _d),
+        ("dissol_stoich", _d * MAX_SPECIES),
     ]
 
 
@@ -141,8 +146,12 @@ def default_params(nx, ny, nz, n_species):
     p.precip_species = -1
     p.precip_interval = 1
     p.dissol_species = -1
-    p.dissol_product = -1
+    p.dissol_material = -1
+    p.dissol_acid = -1
+    p.dissol_phase_id = 1
+    p.dissol_surface_only = 1
     p.dissol_interval = 1
+    p.dissol_reopen_frac = 0.9
     return p
 
 

@@ -77,6 +77,13 @@ struct KinDev {
     double rxn_stoich[CLB_DEV_MAXR][CLB_DEV_MAXS];
     double cap_frac, cap_dt, min_conc, rate_scale;
     int surface_only;
+    // dissolution of one declared solid phase (dissolutionVOP.hh)
+    int dissol;                    // the gather runs with the abiotic increment
+    int dPhase, dMineral, dAcid, dSurfaceOnly;
+    double dK, dStoich[CLB_DEV_MAXS];
+    const int8_t *phase;           // dense phase ids
+    double *share;                 // [S][Nopen] per-face product share parked this step
+    unsigned long long *dcount;    // [0] shares parked, [1] collected, [2] collected and clamped
 };
 
 template <typename R>
@@ -85,6 +92,7 @@ struct NsDev {
     long long N;
     int Nopen, Nint, Nleft;
     const int *list;               // [Nopen]
+    const int8_t *geom;            // dense geometry codes (the pore-space statistics)
     const int *nb;                 // [18][Nopen]
     R *cur, *out;                  // [19][Nopen]
     const uint8_t *cls, *clsPull;  // [Nopen]
@@ -146,6 +154,9 @@ struct EqDev {
 // host launchers: compact lattices (clb_lattice.hip)
 template <typename R> hipError_t launchAdeStep(const AdeDev<R> &, const KinDev &, hipStream_t);
 template <typename R> hipError_t launchKinetics(const AdeDev<R> &, const KinDev &, hipStream_t);
+template <typename R> hipError_t launchDissolPark(const AdeDev<R> &, const KinDev &, hipStream_t);
+template <typename R> hipError_t launchDissolConvert(const AdeDev<R> &, int8_t *phase, int phaseId, int mineralSlot,
+                                                     double threshold, unsigned int *count, hipStream_t);
 template <typename R> hipError_t launchAdeDensity(const AdeDev<R> &, int, double *, hipStream_t);
 template <typename R> hipError_t launchAdeSet(const AdeDev<R> &, int, const double *, hipStream_t);
 template <typename R> hipError_t launchAdeReopen(const AdeDev<R> &, const uint8_t *, hipStream_t);

@@ -57,7 +57,7 @@ struct CaseBase {
     virtual Status readField(hipStream_t, int field, int index, double *dst) = 0;
     virtual Status writeField(hipStream_t, int field, int index, const double *src) = 0;
     virtual void speciesTotals(hipStream_t, double *out) = 0;
-    virtual void counters(hipStream_t, int64_t out[8]) = 0;
+    virtual void counters(hipStream_t, int64_t out[CLB_NCOUNTERS]) = 0;
     virtual void kernelTime(hipStream_t, int reset, double *ms, int64_t *n) = 0;
     virtual void setRelaxation(const double *omp, const double *omb, double dt) = 0;
     // the processors one at a time (host facade of the reference's processor API)
@@ -69,6 +69,7 @@ struct CaseBase {
     virtual void soluteDynamics(hipStream_t) = 0;
     virtual void soluteSolid(hipStream_t, int *reopened) = 0;
     virtual void precipConvert(hipStream_t, int *changed) = 0;
+    virtual Status dissolConvert(hipStream_t, int *reopened) = 0;
 };
 
 template <typename T>
@@ -112,7 +113,7 @@ struct Case : CaseBase {
     int nsSaturate = 0;
     long long oldTotMask = 0;
     uint64_t caSweep_ = 0;
-    int64_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t cnt[CLB_NCOUNTERS] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> evpool;
     size_t evUsed = 0;
     double kms = 0;
@@ -123,6 +124,10 @@ struct Case : CaseBase {
     StepCtl *ctl = nullptr, *ctlHost = nullptr;
     double *caPartMax = nullptr;
     long long *caPartMask = nullptr;
+    // dissolution: per-face product shares [S][Nopen], device counters (parked,
+    // collected, clamped)
+    double *share = nullptr;
+    unsigned long long *dcount = nullptr;
     double gmaxClosed = -1e300;
     long long msumClosed = 0;
     int batchLen = 8;
@@ -181,14 +186,19 @@ struct Case : CaseBase {
         }
         k.cap_frac = p.rxn_cap_frac; k.cap_dt = p.rxn_cap_dt; k.min_conc = p.rxn_min_conc;
         k.rate_scale = p.abiotic_rate_scale; k.surface_only = p.surface_only;
-        if (!k.bio && k.nrxn == 0) k.on = 0;
+        k.dissol = p.dissol_on ? 1 : 0;
+        k.dPhase = p.dissol_phase_id; k.dMineral = p.dissol_species; k.dAcid = p.dissol_acid;
+        k.dSurfaceOnly = p.dissol_surface_only; k.dK = p.dissol_k;
+        for (int s = 0; s < CLB_DEV_MAXS; ++s) k.dStoich[s] = p.dissol_stoich[s];
+        k.phase = phase; k.share = share; k.dcount = dcount;
+        if (!k.bio && k.nrxn == 0 && !k.dissol) k.on = 0;
         return k;
     }
     NsDev<R> ns() const {
         NsDev<R> a;
         memset(&a, 0, sizeof(a));
         a.nx = nx; a.ny = ny; a.nz = nz; a.N = N;
-        a.Nopen = Nopen; a.Nint = Nint; a.Nleft = Nleft; a.list = list; a.nb = nb19;
+        a.Nopen = Nopen; a.Nint = Nint; a.Nleft = Nleft; a.list = list; a.nb = nb19; a.geom = geom;
         a.cur = gCur(); a.out = gOut();
         a.cls = nsCls; a.clsPull = nsClsPull;
         a.om = nsOmega; a.omBio = nsBioOmega; a.rhoBarRight = -deltaP * 3.0;
@@ -233,6 +243,12 @@ struct Case : CaseBase {
         return e;
     }
 
+    // flow dynamics from the static geometry (NSdomainSetup, complab_functions.hh:296-345)
+    int nsClassOf(int gv) const {
+        if (gv == CLB_PORE) return NS_BGK;
+        if (gv > CLB_PORE) return p.perm_ratio > 1e-14 ? NS_BIO : NS_BB;
+        return gv == CLB_WALL ? NS_BB : NS_NODYN;
+    }
     void create(const clb_case_params &pp, const int8_t *geometry, const double *biomass0, hipStream_t st) {
         p = pp;
         nx = p.nx; ny = p.ny; nz = p.nz; N = (long long) nx * ny * nz;
@@ -245,11 +261,15 @@ struct Case : CaseBase {
         // the compact open-voxel order: interior, x = 0 plane, x = nx-1 plane
         std::vector<int8_t> g(geometry, geometry + N), dh(N), ph(N, 0);
         std::vector<int> lst, d2c(N, -1);
+        // voxels that can carry solute: open geometry, and the grain of a
+        // dissolvable phase (closed until it dissolves and reopens)
+        const bool grainPhase = p.dissol_on && !p.dissol_is_precipitate && p.dissol_material >= 0;
+        auto listed = [&](int gv) { return gv >= CLB_PORE || (grainPhase && gv == p.dissol_material); };
         for (int pass = 0; pass < 3; ++pass)
             for (long long v = 0; v < N; ++v) {
                 const int x = (int) (v % nx);
                 const bool want = pass == 0 ? (x > 0 && x < nx - 1) : (pass == 1 ? x == 0 : x == nx - 1);
-                if (want && g[v] >= CLB_PORE) { d2c[v] = (int) lst.size(); lst.push_back((int) v); }
+                if (want && listed(g[v])) { d2c[v] = (int) lst.size(); lst.push_back((int) v); }
                 if (pass == 0 && v == N - 1) Nint = (int) lst.size();
                 if (pass == 1 && v == N - 1) Nleft = (int) lst.size() - Nint;
             }
@@ -310,6 +330,8 @@ struct Case : CaseBase {
         nsDone = dalloc<int>(allocs, 1);
         tracer = dalloc<NsTracer>(allocs, 1);
         ctl = dalloc<StepCtl>(allocs, 1);
+        if (p.dissol_on) share = dalloc<double>(allocs, (size_t) std::max(p.n_species, 1) * Nopen);
+        dcount = dalloc<unsigned long long>(allocs, 3);
         caPartMax = dalloc<double>(allocs, nblk(Nopen, 256));
         caPartMask = dalloc<long long>(allocs, nblk(Nopen, 256));
         {
@@ -332,9 +354,11 @@ struct Case : CaseBase {
         }
         for (int i = 0; i < Nopen; ++i) {
             const int gv = g[lst[i]];
-            ac[i] = gv == CLB_PORE ? ADE_PORE : ADE_BFILM;
-            nc[i] = gv == CLB_PORE ? NS_BGK : (p.perm_ratio > 1e-14 ? NS_BIO : NS_BB);
+            ac[i] = gv == CLB_PORE ? ADE_PORE : (gv > CLB_PORE ? ADE_BFILM : ADE_BB);
+            nc[i] = nsClassOf(gv);
         }
+        if (grainPhase)
+            for (long long i = 0; i < N; ++i) if (g[i] == p.dissol_material) ph[i] = (int8_t) p.dissol_phase_id;
         HIPT(hipMemcpyAsync(mask, g.data(), N, hipMemcpyHostToDevice, st));
         HIPT(hipMemcpyAsync(geom, g.data(), N, hipMemcpyHostToDevice, st));
         HIPT(hipMemcpyAsync(dist, dh.data(), N, hipMemcpyHostToDevice, st));
@@ -370,6 +394,8 @@ struct Case : CaseBase {
             for (int s = 0; s < p.n_species; ++s) {
                 if (p.mobile[s]) continue;
                 std::fill(sc.begin(), sc.end(), (R) p.c0[s]);
+                if (grainPhase && s == p.dissol_species && p.dissol_initial > 0.0)   // complab.cpp:1684-1698
+                    for (long long i = 0; i < N; ++i) if (g[i] == p.dissol_material) sc[i] = (R) p.dissol_initial;
                 HIPT(hipMemcpy(imm + (size_t) slot[s] * N, sc.data(), N * sizeof(R), hipMemcpyHostToDevice));
             }
             std::vector<double> tot(N, 0.0);
@@ -401,6 +427,8 @@ struct Case : CaseBase {
         HIPT(hipMemsetAsync(nsDone, 0, sizeof(int), st));
         HIPT(hipMemsetAsync(tracer, 0, sizeof(NsTracer), st));
         HIPT(hipMemsetAsync(ctl, 0, sizeof(StepCtl), st));
+        HIPT(hipMemsetAsync(dcount, 0, 3 * sizeof(unsigned long long), st));
+        if (share) HIPT(hipMemsetAsync(share, 0, (size_t) std::max(p.n_species, 1) * Nopen * sizeof(double), st));
         nsOmega = 1.0 / p.ns_tau;
         nsBioOmega = 1.0 / (p.perm_ratio * (1.0 / nsOmega - 0.5) + 0.5);
         HIPT(hipStreamSynchronize(st));
@@ -421,10 +449,7 @@ struct Case : CaseBase {
         HIPT(hipMemcpy(g.data(), geom, N, hipMemcpyDeviceToHost));
         HIPT(hipMemcpy(lst.data(), list, (size_t) Nopen * sizeof(int), hipMemcpyDeviceToHost));
         std::vector<uint8_t> nc(Nopen);
-        for (int i = 0; i < Nopen; ++i) {
-            const int gv = g[lst[i]];
-            nc[i] = gv == CLB_PORE ? NS_BGK : (p.perm_ratio > 1e-14 ? NS_BIO : NS_BB);
-        }
+        for (int i = 0; i < Nopen; ++i) nc[i] = nsClassOf(g[lst[i]]);
         HIPT(hipMemcpyAsync(nsCls, nc.data(), Nopen, hipMemcpyHostToDevice, st));
         HIPT(hipMemcpyAsync(nsClsPull, nc.data(), Nopen, hipMemcpyHostToDevice, st));
         nsBioOmega = 1.0 / (p.perm_ratio * (1.0 / nsOmega - 0.5) + 0.5);
@@ -526,13 +551,28 @@ struct Case : CaseBase {
         }
         evUsed = 0;
     }
-    void fusedLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr) {
+    // one transport launch with its kinetics.  With dissolution the abiotic
+    // increment waits for the products the phase voxels park: the fused launch
+    // does transport + biotic kinetics, then the park pass, then the abiotic
+    // kinetics with the gather (the reference's order, complab.cpp:2461-2545).
+    void transportLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr) {
+        if (!kinOn || !p.dissol_on) { fusedLaunch(st, kinOn, halt); return; }
+        fusedLaunch(st, true, halt, false);
+        KinDev k = kin(true);
+        k.bio = 0;
+        HIPT(launchDissolPark<R>(ade(halt), k, st));
+        HIPT(launchKinetics<R>(ade(halt), k, st));
+    }
+    void fusedLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr, bool abioticInFused = true) {
         if (evUsed == evpool.size()) {
             if (evpool.size() >= kEvPool) flushTimes(st);
             else { hipEvent_t e0, e1; HIPT(hipEventCreate(&e0)); HIPT(hipEventCreate(&e1)); evpool.push_back({e0, e1}); }
         }
         AdeDev<R> a = ade(halt);
         KinDev k = kin(kinOn);
+        k.dissol = 0;
+        if (!abioticInFused) k.nrxn = 0;
+        if (!k.bio && k.nrxn == 0) k.on = 0;
         HIPT(hipEventRecord(evpool[evUsed].first, st));
         HIPT(launchAdeStep<R>(a, k, st));
         HIPT(hipEventRecord(evpool[evUsed].second, st));
@@ -548,7 +588,7 @@ struct Case : CaseBase {
         adeReset = true;
     }
     void adeSteps(hipStream_t st, int n) override {
-        for (int t = 0; t < n; ++t) fusedLaunch(st, true);
+        for (int t = 0; t < n; ++t) transportLaunch(st, true);
         flushTimes(st);
     }
     void equilibrium(hipStream_t st) override { equilibriumQ(st, nullptr); }
@@ -561,9 +601,27 @@ struct Case : CaseBase {
     void kinetics(hipStream_t st, int which) override {
         KinDev k = kin(true);
         if (!(which & CLB_KIN_BIOTIC)) k.bio = 0;
-        if (!(which & CLB_KIN_ABIOTIC)) k.nrxn = 0;
-        if (!k.bio && k.nrxn == 0) k.on = 0;
+        if (!(which & CLB_KIN_ABIOTIC)) { k.nrxn = 0; k.dissol = 0; }
+        if (!k.bio && k.nrxn == 0 && !k.dissol) k.on = 0;
+        if (k.dissol) {
+            // the biotic family first, on its own, then park + abiotic + gather
+            if (k.bio) { KinDev kb = k; kb.nrxn = 0; kb.dissol = 0; HIPT(launchKinetics<R>(ade(), kb, st)); }
+            k.bio = 0;
+            HIPT(launchDissolPark<R>(ade(), k, st));
+        }
         HIPT(launchKinetics<R>(ade(), k, st));
+    }
+    Status dissolConvert(hipStream_t st, int *reopenedOut) override {
+        unsigned int n = 0;
+        if (reopenedOut) *reopenedOut = 0;
+        if (!p.dissol_on || !(p.dissol_full > 0.0)) return {CLB_OK, ""};
+        HIPT(hipMemsetAsync(ucnt + 3, 0, sizeof(unsigned int), st));
+        HIPT(launchDissolConvert<R>(ade(), phase, p.dissol_phase_id, slot[p.dissol_species],
+                                    p.dissol_reopen_frac * p.dissol_full, ucnt + 3, st));
+        HIPT(hipMemcpyAsync(&n, ucnt + 3, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        HIPT(hipStreamSynchronize(st));
+        if (reopenedOut) *reopenedOut = (int) n;
+        return {CLB_OK, ""};
     }
     void maskTotalNow(hipStream_t st, double *gmax, long long *msum) override { maskTotal(st, gmax, msum); }
     void caSweep(hipStream_t st) override {
@@ -683,6 +741,10 @@ struct Case : CaseBase {
     bool precipDue(long long it) const {
         return p.precip_on && p.precip_interval > 0 && it % p.precip_interval == 0 && p.precip_species >= 0;
     }
+    bool dissolDue(long long it) const {
+        return p.dissol_on && p.dissol_interval > 0 && it % p.dissol_interval == 0;
+    }
+    bool hostDue(long long it) const { return precipDue(it) || dissolDue(it); }
     // the host part of a step after its transport, equilibrium and (CA cases)
     // mask/total reduction: CA spill, mask-change consequences, precipitation
     Status stepTail(hipStream_t st, double gmax, long long msum) {
@@ -720,13 +782,32 @@ struct Case : CaseBase {
                 adeClsDirty = true;
             }
         }
+        if (dissolDue(iT)) {                                  // complab.cpp:3203-3247
+            int n = 0;
+            Status d = dissolConvert(st, &n);
+            if (d.code) return d;
+            if (n > 0) {
+                cnt[5]++;
+                cnt[11] += n;
+                if (p.flow_on) {
+                    nsUpdateDynamics(st, p.perm_ratio);
+                    nsSaturate = 0;
+                    int it = 0, cv = 0;
+                    Status r = nsSolve(st, p.ns_max_iter, p.ns_conv_rule, p.ns_conv_eps, 0, &it, &cv);
+                    if (r.code) return r;
+                    cnt[4]++;
+                    nsCouple(st);
+                }
+                soluteSolid(st, nullptr);
+            }
+        }
         ++iT; cnt[7]++;
         return {CLB_OK, ""};
     }
 
     // one coupled step with the host in the loop
     Status coupledStep(hipStream_t st) {
-        fusedLaunch(st, true);
+        transportLaunch(st, true);
         if (p.eq_on) equilibrium(st);
         double gmax = 0; long long msum = 0;
         if (p.n_microbes > 0) maskTotal(st, &gmax, &msum);
@@ -742,14 +823,14 @@ struct Case : CaseBase {
     Status runBatch(hipStream_t st, int want, int *finished) {
         *finished = 0;
         int len = want;
-        for (int j = 0; j < len; ++j) if (precipDue(iT + j)) { len = j + 1; break; }
+        for (int j = 0; j < len; ++j) if (hostDue(iT + j)) { len = j + 1; break; }
         if (evUsed + (size_t) len > kEvPool) flushTimes(st);
         HIPT(launchStepCtlReset(ctl, oldTotMask, st));
         const size_t ev0 = evUsed;
         for (int j = 0; j < len; ++j) {
-            fusedLaunch(st, true, &ctl->halt);
+            transportLaunch(st, true, &ctl->halt);
             if (p.eq_on) equilibriumQ(st, &ctl->halt);
-            HIPT(launchCaMaskTotal<R>(ca(), precipDue(iT + j) ? 2 : 1, st));
+            HIPT(launchCaMaskTotal<R>(ca(), hostDue(iT + j) ? 2 : 1, st));
         }
         HIPT(hipMemcpyAsync(ctlHost, ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, st));
         HIPT(hipStreamSynchronize(st));
@@ -858,14 +939,28 @@ struct Case : CaseBase {
         HIPT(hipMemcpyAsync(out, scal, p.n_species * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPT(hipStreamSynchronize(st));
     }
-    void counters(hipStream_t st, int64_t out[8]) override {
-        unsigned long long cc = 0; unsigned int u3[4];
+    void counters(hipStream_t st, int64_t out[CLB_NCOUNTERS]) override {
+        unsigned long long cc = 0, dc[3] = {0, 0, 0};
+        unsigned int u3[4];
+        std::vector<int8_t> m(N);
         HIPT(hipMemcpyAsync(&cc, clampCount, sizeof(cc), hipMemcpyDeviceToHost, st));
         HIPT(hipMemcpyAsync(u3, ucnt, sizeof(u3), hipMemcpyDeviceToHost, st));
+        HIPT(hipMemcpyAsync(dc, dcount, sizeof(dc), hipMemcpyDeviceToHost, st));
+        HIPT(hipMemcpyAsync(m.data(), mask, N, hipMemcpyDeviceToHost, st));
         HIPT(hipStreamSynchronize(st));
-        for (int k = 0; k < 8; ++k) out[k] = cnt[k];
+        for (int k = 0; k < CLB_NCOUNTERS; ++k) out[k] = cnt[k];
+        // open and biofilm voxels of the reaction box on the live mask (MaskedScalarCounts3D)
+        int64_t open = 0, bio = 0;
+        for (long long i = 0; i < N; ++i) {
+            const int x = (int) (i % nx);
+            if (x < 1 || x > nx - 2) continue;
+            if (m[i] >= CLB_PORE) ++open;
+            if (m[i] >= CLB_BIOFILM) ++bio;
+        }
+        out[0] = open; out[1] = bio;
         out[2] = (int64_t) cc;
         out[3] = (int64_t) u3[2];
+        out[8] = (int64_t) dc[0]; out[9] = (int64_t) dc[1]; out[10] = (int64_t) dc[2];
     }
     void kernelTime(hipStream_t, int reset, double *ms, int64_t *n) override {
         if (ms) *ms = klaunch ? kms / (double) klaunch : 0.0;
@@ -924,7 +1019,24 @@ int checkParams(clb_ctx *ctx, const clb_case_params *p, const int8_t *geometry) 
     }
     if (p->precip_on && (p->precip_species < 0 || p->precip_species >= p->n_species || p->mobile[p->precip_species]))
         return fail(ctx, CLB_E_ARG, "precip_species must be an immobile species");
-    if (p->dissol_on) return fail(ctx, CLB_E_ARG, "dissolution is not available in this build");
+    if (p->dissol_on) {
+        const int sM = p->dissol_species;
+        if (sM < 0 || sM >= p->n_species || p->mobile[sM]) return fail(ctx, CLB_E_ARG, "dissol_species must be an immobile species");
+        if (p->dissol_phase_id < 1 || p->dissol_phase_id > 127) return fail(ctx, CLB_E_ARG, "dissol_phase_id must be 1..127");
+        if (!(p->dissol_reopen_frac > 0.0) || p->dissol_reopen_frac >= 1.0)
+            return fail(ctx, CLB_E_ARG, "dissol_reopen_frac must lie in (0, 1)");             // complab.cpp:539-543
+        if (p->dissol_interval < 1) return fail(ctx, CLB_E_ARG, "dissol_interval must be >= 1");
+        if (p->dissol_acid < -1 || p->dissol_acid >= p->n_species) return fail(ctx, CLB_E_ARG, "dissol_acid out of range");
+        if (p->dissol_is_precipitate) {
+            if (!p->precip_on || p->precip_phase_id != p->dissol_phase_id || p->precip_species != sM)
+                return fail(ctx, CLB_E_ARG, "a precipitate phase needs precipitation on, stamping dissol_phase_id into dissol_species");
+        } else if (p->dissol_material < 0 || p->dissol_material >= CLB_PORE) {
+            return fail(ctx, CLB_E_ARG, "a grain phase needs a closed material number (0 or 1)");
+        }
+        for (int s = 0; s < p->n_species; ++s)
+            if (!p->mobile[s] && s != sM && p->dissol_stoich[s] != 0.0)
+                return fail(ctx, CLB_E_ARG, "dissolution products must be mobile species");
+    }
     if (p->bio_law != CLB_BIO_NONE) {
         if (p->donor >= p->n_species || p->acceptor >= p->n_species) return fail(ctx, CLB_E_ARG, "donor/acceptor index out of range");
     }
@@ -934,7 +1046,9 @@ int checkParams(clb_ctx *ctx, const clb_case_params *p, const int8_t *geometry) 
         int g = geometry[i];
         if (g < 0 || g > CLB_BIOFILM + CLB_MAX_MICROBES + 2) return fail(ctx, CLB_E_GEOMETRY, "unknown material code in geometry");
     }
-    for (int z = 0; z < p->nz; ++z) for (int y = 0; y < p->ny; ++y) for (int x = 0; x < p->nx; ++x) {
+    // (a dissolvable grain of solid material is meant to touch the water)
+    const bool solidGrainPhase = p->dissol_on && !p->dissol_is_precipitate && p->dissol_material == CLB_SOLID;
+    for (int z = 0; z < p->nz && !solidGrainPhase; ++z) for (int y = 0; y < p->ny; ++y) for (int x = 0; x < p->nx; ++x) {
         long long i = x + (long long) p->nx * (y + (long long) p->ny * z);
         if (geometry[i] != CLB_SOLID) continue;
         const int d[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
@@ -1170,6 +1284,14 @@ int clb_update_solute_solid(clb_ctx *ctx, int case_id, int *reopened) {
     return CLB_OK;
     GUARD_END
 }
+int clb_dissol_convert(clb_ctx *ctx, int case_id, int *reopened) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (!c->p.dissol_on) return fail(ctx, CLB_E_STATE, "case was created with dissol_on = 0");
+    Status s = c->dissolConvert(ctx->stream, reopened);
+    return s.code ? fail(ctx, s.code, s.msg) : CLB_OK;
+    GUARD_END
+}
 int clb_precip_convert(clb_ctx *ctx, int case_id, int *changed) {
     GUARD_BEGIN
     CASE_OR_FAIL(c)
@@ -1217,7 +1339,7 @@ int clb_species_totals(clb_ctx *ctx, int case_id, double *out) {
     return CLB_OK;
     GUARD_END
 }
-int clb_counters(clb_ctx *ctx, int case_id, int64_t out[8]) {
+int clb_counters(clb_ctx *ctx, int case_id, int64_t out[CLB_NCOUNTERS]) {
     GUARD_BEGIN
     CASE_OR_FAIL(c)
     if (!out) return fail(ctx, CLB_E_ARG, "null output");

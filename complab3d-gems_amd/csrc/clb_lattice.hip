@@ -184,7 +184,7 @@ __device__ __forceinline__ double clampInc(int clamp, double d, double cur, unsi
 // the clamp, the D3Q7 deposit w_i dC, immobile species and biomass in place
 // (run_kinetics + update_rxnLattices, part1:85-187, 406-497; the abiotic pair
 // part1:199-404; surfaceAbioticKinetics3D's face gate)
-template <typename R, int MAXS>
+template <typename R, int MAXS, bool DISSOL>
 __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &k, int i, R (&f)[MAXS][7]) {
     const long long No = a.Nopen;
     const long long v = a.list[i];
@@ -246,30 +246,71 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
                 }
             }
         }
-        if (k.nrxn > 0) {
-            bool react = true;
-            if (k.surface_only) {                                     // precipitationVOP.hh:78-88
-                react = false;
+        if (k.nrxn > 0 || DISSOL) {
+            // one increment per species: the abiotic rates, then (dissolution) the
+            // product shares the face neighbours of the phase parked this step,
+            // applied together (update_abiotic_rxnLattices, part1:306-404)
+            double dAb[CLB_DEV_MAXS];
 #pragma unroll
-                for (int q = 1; q < 7; ++q) {
-                    const int n = a.nb[(opp7(q) - 1) * No + i];        // the voxel at +c_q
-                    if (n == NB_OUTSIDE) continue;
-                    if (n == NB_CLOSED) { react = true; continue; }
-                    const int nm = a.mask[a.list[n]];
-                    if (nm == 0 || nm == 1) react = true;
+            for (int s = 0; s < CLB_DEV_MAXS; ++s) dAb[s] = 0.0;
+            if (k.nrxn > 0) {
+                bool react = true;
+                if (k.surface_only) {                                     // precipitationVOP.hh:78-88
+                    react = false;
+#pragma unroll
+                    for (int q = 1; q < 7; ++q) {
+                        const int n = a.nb[(opp7(q) - 1) * No + i];        // the voxel at +c_q
+                        if (n == NB_OUTSIDE) continue;
+                        if (n == NB_CLOSED) { react = true; continue; }
+                        const int nm = a.mask[a.list[n]];
+                        if (nm == 0 || nm == 1) react = true;
+                    }
+                }
+                if (react) {
+#pragma unroll
+                    for (int s = 0; s < CLB_DEV_MAXS; ++s) { if (s < S) { C[s] = concOf(s); if (C[s] < 1e-12) C[s] = 0; } }
+                    abioticLaw(k, C, sr);
+#pragma unroll
+                    for (int s = 0; s < CLB_DEV_MAXS; ++s) {
+                        if (s >= S) continue;
+                        const double d = sr[s] * k.dt;
+                        if (d > 1e-12 || d < -1e-12) dAb[s] += d;
+                    }
                 }
             }
-            if (react) {
-                
+            if (DISSOL) {                                                 // dissolutionGather3D, dissolutionVOP.hh:607-720
+                unsigned int collected = 0, clamped = 0;
 #pragma unroll
-            for (int s = 0; s < CLB_DEV_MAXS; ++s) { if (s < S) { C[s] = concOf(s); if (C[s] < 1e-12) C[s] = 0; } }
-                abioticLaw(k, C, sr);
+                for (int d = 0; d < 6; ++d) {
+                    const int n = a.nb[(opp7(d + 1) - 1) * No + i];        // the face neighbour at +dloc[d]
+                    if (n < 0 || n >= a.Nint) continue;                    // outside the reaction box or closed geometry
+                    const int pid = k.phase[a.list[n]];
+                    if (pid <= 0 || pid != k.dPhase) continue;
 #pragma unroll
-                for (int s = 0; s < CLB_DEV_MAXS; ++s) {
-                    if (s >= S) continue;
-                    const double d = sr[s] * k.dt;
-                    if (d > 1e-12 || d < -1e-12) depositTo(s, clampInc(k.clamp, d, concOf(s), binds));
+                    for (int s = 0; s < CLB_DEV_MAXS; ++s) {
+                        if (s >= S || s == k.dMineral) continue;
+                        const double share = k.share[(long long) s * No + n];
+                        if (fabs(share) <= 1e-14) continue;
+                        double give = share;
+                        if (give < 0.0) {
+                            const double have = concOf(s);
+                            const double headroom = (have > 0.0 ? have : 0.0) + dAb[s];
+                            if (give + headroom < 0.0) { give = -headroom; ++clamped; }
+                            if (give > 0.0) give = 0.0;
+                        }
+                        ++collected;
+                        if (!(fabs(give) > 1e-14)) continue;
+                        dAb[s] += give;
+                    }
                 }
+                if (collected) atomicAdd(k.dcount + 1, (unsigned long long) collected);
+                if (clamped) atomicAdd(k.dcount + 2, (unsigned long long) clamped);
+            }
+#pragma unroll
+            for (int s = 0; s < CLB_DEV_MAXS; ++s) {
+                if (s >= S) continue;
+                const double d = dAb[s];
+                if (d > 1e-12 || d < -1e-12) depositTo(s, clampInc(k.clamp, d, concOf(s), binds));
             }
         }
         if (binds) atomicAdd(a.clampCount, (unsigned long long) binds);
@@ -346,7 +387,7 @@ __global__ void __launch_bounds__(256) k_ade_bulk(AdeDev<R> a, KinDev k) {
         bgk7<R>(f[ms], (R) (cls == ADE_BFILM ? a.omB[s] : a.omP[s]), ux, uy, uz);
     }
 
-    if (k.on) voxelKinetics<R, MAXS>(a, k, i, f);
+    if (k.on) voxelKinetics<R, MAXS, false>(a, k, i, f);
 
 #pragma unroll
     for (int ms = 0; ms < MAXS; ++ms) {
@@ -360,8 +401,9 @@ __global__ void __launch_bounds__(256) k_ade_bulk(AdeDev<R> a, KinDev k) {
 // the kinetics alone on the stored post-collision state (the split form of
 // the fused kernel, for callers that apply the reference's processors one by
 // one); bit-identical to the fused kernel's kinetics
-template <typename R, int MAXS>
+template <typename R, int MAXS, bool DISSOL>
 __global__ void __launch_bounds__(256) k_kinetics(AdeDev<R> a, KinDev k) {
+    if (halted(a.halt)) return;
     const int i = xcdBlock() * blockDim.x + threadIdx.x;
     if (i >= a.Nint) return;
     if (a.cls[i] == ADE_BB) return;
@@ -374,13 +416,114 @@ __global__ void __launch_bounds__(256) k_kinetics(AdeDev<R> a, KinDev k) {
 #pragma unroll
         for (int q = 0; q < 7; ++q) f[ms][q] = c[q * No + i];
     }
-    voxelKinetics<R, MAXS>(a, k, i, f);
+    voxelKinetics<R, MAXS, DISSOL>(a, k, i, f);
 #pragma unroll
     for (int ms = 0; ms < MAXS; ++ms) {
         if (ms >= a.Sm) continue;
         R *c = a.cur + (long long) ms * 7 * No;
 #pragma unroll
         for (int q = 0; q < 7; ++q) c[q * No + i] = f[ms][q];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// mineral dissolution (dissolutionVOP.hh)
+// ------------------------------------------------------------------------------------------
+// density of species s at compact voxel n on the stored (post-collision) state
+template <typename R>
+__device__ __forceinline__ double storedConc(const AdeDev<R> &a, int s, int n) {
+    const int sl = a.slot[s];
+    if (a.mobile[s]) {
+        const R *c = a.cur + (long long) sl * 7 * a.Nopen + n;
+        double r = 0;
+#pragma unroll
+        for (int q = 0; q < 7; ++q) r += (double) c[(long long) q * a.Nopen];
+        return r;
+    }
+    return (double) a.imm[(long long) sl * a.N + a.list[n]];
+}
+
+// surfaceDissolutionKinetics3D (dissolutionVOP.hh:378-569): a voxel of the
+// declared phase with mineral left averages the water of its open face
+// neighbours in the reaction box, takes the rate law of defineDissolutionRate
+// (rate = k max(C[acid],0) max(mineral,0)), removes the mineral (at most half
+// of it per step; products scaled alike) and parks one share per open face of
+// every product for the gather.  Shares of a phase voxel are rewritten every
+// step, so a voxel that stops dissolving parks zeros.
+template <typename R>
+__global__ void __launch_bounds__(256) k_dissol_park(AdeDev<R> a, KinDev k) {
+    if (halted(a.halt)) return;
+    const int i = xcdBlock() * blockDim.x + threadIdx.x;
+    if (i >= a.Nint) return;
+    const long long v = a.list[i];
+    const int pid = k.phase[v];
+    if (pid <= 0 || pid != k.dPhase) return;
+    const long long No = a.Nopen;
+    const int S = k.S;
+    for (int s = 0; s < S; ++s) k.share[(long long) s * No + i] = 0.0;
+    R *pm = a.imm + (long long) a.slot[k.dMineral] * a.N + v;
+    const double mineral = (double) *pm;
+    if (!(mineral > 1e-12)) return;
+    int openN[6];
+    int nOpen = 0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const int n = a.nb[(opp7(d + 1) - 1) * No + i];
+        if (n < 0 || n >= a.Nint) continue;
+        const int m = a.mask[a.list[n]];
+        if (m == 0 || m == 1) continue;                       // still rock
+        openN[nOpen++] = n;
+    }
+    if (nOpen == 0 && k.dSurfaceOnly) return;
+    double water[CLB_DEV_MAXS], subsR[CLB_DEV_MAXS];
+    for (int s = 0; s < S; ++s) water[s] = 0.0;
+    if (nOpen > 0) {
+        for (int kk = 0; kk < nOpen; ++kk)
+            for (int s = 0; s < S; ++s) water[s] += storedConc<R>(a, s, openN[kk]);
+        for (int s = 0; s < S; ++s) water[s] /= (double) nOpen;
+    }
+    for (int s = 0; s < S; ++s) if (water[s] < 0.0) water[s] = 0.0;
+    const double H = k.dAcid >= 0 ? fmax(water[k.dAcid], 0.0) : 1.0;
+    const double rate = k.dK * H * fmax(mineral, 0.0);
+    double mineralR = -rate;
+    for (int s = 0; s < S; ++s) { subsR[s] = k.dStoich[s] * rate; if (!isfinite(subsR[s])) subsR[s] = 0.0; }
+    if (!isfinite(mineralR)) mineralR = 0.0;
+    if (!(mineralR < 0.0)) return;
+    double dMineral = mineralR * k.dt;
+    double scale = 1.0;
+    const double maxRemove = 0.5 * mineral;                   // MAX_DISSOL_FRACTION, dissolutionVOP.hh:485
+    if (-dMineral > maxRemove) scale = maxRemove / (-dMineral);
+    dMineral *= scale;
+    if (!(-dMineral > 1e-12)) return;
+    *pm = (R) (mineral + dMineral);
+    if (nOpen > 0) {
+        unsigned long long parked = 0;
+        for (int s = 0; s < S; ++s) {
+            if (s == k.dMineral) continue;
+            const double released = subsR[s] * k.dt * scale;
+            if (fabs(released) <= 1e-12) continue;
+            k.share[(long long) s * No + i] = released / (double) nOpen;
+            parked += (unsigned long long) nOpen;
+        }
+        if (parked) atomicAdd(k.dcount, parked);
+    }
+}
+
+// dissolNodeConversion3D (dissolutionVOP.hh:744-812): a voxel of the phase
+// whose inventory is below reopen_fraction x full_density turns pore and
+// loses its phase
+template <typename R>
+__global__ void k_dissol_convert(AdeDev<R> a, int8_t *phase, int phaseId, int mineralSlot, double threshold,
+                                 unsigned int *count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.Nint) return;
+    const long long v = a.list[i];
+    const int pid = phase[v];
+    if (pid <= 0 || pid != phaseId) return;
+    if ((double) a.imm[(long long) mineralSlot * a.N + v] < threshold) {
+        a.mask[v] = 2;
+        phase[v] = 0;
+        atomicAdd(count, 1u);
     }
 }
 
@@ -721,7 +864,8 @@ __global__ void __launch_bounds__(256) k_ns_velocity(NsDev<R> a, R *u, double *p
         inBox = i < a.Nint;
         const double n = sqrt(ux * ux + uy * uy + uz * uz);
         if (inBox) {
-            v[0] = n; v[1] = ux; v[3] = n; v[4] = 1;
+            v[0] = n; v[1] = ux;
+            if (a.geom[a.list[i]] >= 2) { v[3] = n; v[4] = 1; }
             const int x = (int) (a.list[i] % a.nx);
             if (x == a.nx - 2) v[2] = ux;
         }
@@ -864,16 +1008,35 @@ hipError_t launchAdeStep(const AdeDev<R> &a, const KinDev &k, hipStream_t st) {
     if (Np > 0 && a.Sm > 0) hipLaunchKernelGGL(k_ade_planes<R>, dim3(nblk(Np, 256), a.Sm), dim3(256), 0, st, a);
     return hipGetLastError();
 }
+template <typename R, bool DISSOL>
+static void kineticsVariants(const AdeDev<R> &a, const KinDev &k, hipStream_t st) {
+    const unsigned int g = nblk(a.Nint, 256);
+    if (a.Sm <= 1) hipLaunchKernelGGL((k_kinetics<R, 1, DISSOL>), dim3(g), dim3(256), 0, st, a, k);
+    else if (a.Sm <= 2) hipLaunchKernelGGL((k_kinetics<R, 2, DISSOL>), dim3(g), dim3(256), 0, st, a, k);
+    else if (a.Sm <= 3) hipLaunchKernelGGL((k_kinetics<R, 3, DISSOL>), dim3(g), dim3(256), 0, st, a, k);
+    else if (a.Sm <= 4) hipLaunchKernelGGL((k_kinetics<R, 4, DISSOL>), dim3(g), dim3(256), 0, st, a, k);
+    else if (a.Sm <= 6) hipLaunchKernelGGL((k_kinetics<R, 6, DISSOL>), dim3(g), dim3(256), 0, st, a, k);
+    else hipLaunchKernelGGL((k_kinetics<R, 8, DISSOL>), dim3(g), dim3(256), 0, st, a, k);
+}
 template <typename R>
 hipError_t launchKinetics(const AdeDev<R> &a, const KinDev &k, hipStream_t st) {
     if (a.Nint <= 0 || !k.on) return hipSuccess;
-    const unsigned int g = nblk(a.Nint, 256);
-    if (a.Sm <= 1) hipLaunchKernelGGL((k_kinetics<R, 1>), dim3(g), dim3(256), 0, st, a, k);
-    else if (a.Sm <= 2) hipLaunchKernelGGL((k_kinetics<R, 2>), dim3(g), dim3(256), 0, st, a, k);
-    else if (a.Sm <= 3) hipLaunchKernelGGL((k_kinetics<R, 3>), dim3(g), dim3(256), 0, st, a, k);
-    else if (a.Sm <= 4) hipLaunchKernelGGL((k_kinetics<R, 4>), dim3(g), dim3(256), 0, st, a, k);
-    else if (a.Sm <= 6) hipLaunchKernelGGL((k_kinetics<R, 6>), dim3(g), dim3(256), 0, st, a, k);
-    else hipLaunchKernelGGL((k_kinetics<R, 8>), dim3(g), dim3(256), 0, st, a, k);
+    if (k.dissol) kineticsVariants<R, true>(a, k, st);
+    else kineticsVariants<R, false>(a, k, st);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchDissolPark(const AdeDev<R> &a, const KinDev &k, hipStream_t st) {
+    if (a.Nint <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dissol_park<R>, dim3(nblk(a.Nint, 256)), dim3(256), 0, st, a, k);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchDissolConvert(const AdeDev<R> &a, int8_t *phase, int phaseId, int mineralSlot, double threshold,
+                               unsigned int *count, hipStream_t st) {
+    if (a.Nint <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dissol_convert<R>, dim3(nblk(a.Nint, 256)), dim3(256), 0, st, a, phase, phaseId, mineralSlot,
+                       threshold, count);
     return hipGetLastError();
 }
 template <typename R>
@@ -942,6 +1105,8 @@ hipError_t launchGatherCompact(const double *src, const int *list, int n, R *dst
 #define INST(R) \
     template hipError_t launchAdeStep<R>(const AdeDev<R> &, const KinDev &, hipStream_t); \
     template hipError_t launchKinetics<R>(const AdeDev<R> &, const KinDev &, hipStream_t); \
+    template hipError_t launchDissolPark<R>(const AdeDev<R> &, const KinDev &, hipStream_t); \
+    template hipError_t launchDissolConvert<R>(const AdeDev<R> &, int8_t *, int, int, double, unsigned int *, hipStream_t); \
     template hipError_t launchAdeDensity<R>(const AdeDev<R> &, int, double *, hipStream_t); \
     template hipError_t launchAdeSet<R>(const AdeDev<R> &, int, const double *, hipStream_t); \
     template hipError_t launchAdeReopen<R>(const AdeDev<R> &, const uint8_t *, hipStream_t); \

@@ -149,15 +149,20 @@ typedef struct clb_case_params {
     int32_t precip_interval;
     int32_t precip_phase_id;
 
-    /* ---- dissolution (dissolutionVOP.hh) -------------------------------- */
+    /* ---- dissolution (dissolutionVOP.hh): one declared solid phase --------- */
     int32_t dissol_on;
-    int32_t dissol_species;            /* immobile mineral species            */
-    double  dissol_k;                  /* rate constant, 1/s (first order in mineral) */
-    int32_t dissol_product;            /* released species                    */
-    double  dissol_full;               /* full_density                        */
-    double  dissol_reopen_frac;        /* reopen_fraction                     */
-    int32_t dissol_interval;
-    int32_t dissol_phase_id;
+    int32_t dissol_species;            /* <substrate>: immobile species holding the inventory   */
+    int32_t dissol_material;           /* <material_number> of a grain phase; -1: none          */
+    int32_t dissol_is_precipitate;     /* <is_precipitate>: precipitation stamps this phase     */
+    int32_t dissol_phase_id;           /* the phase id (1 for <phase0>)                         */
+    int32_t dissol_surface_only;       /* <surface_only>: enclosed voxels do not react          */
+    int32_t dissol_interval;           /* <update_interval> of the reopen check                 */
+    int32_t dissol_acid;               /* rate law: the species C[acid]; -1: none (factor 1)    */
+    double  dissol_full;               /* <full_density>                                        */
+    double  dissol_initial;            /* <initial_fill> seeded on dissol_material voxels       */
+    double  dissol_reopen_frac;        /* <reopen_fraction>                                     */
+    double  dissol_k;                  /* rate = k max(C[acid],0) max(mineral,0), mineralR = -rate */
+    double  dissol_stoich[CLB_MAX_SPECIES]; /* subsR[s] = stoich[s] rate (defineDissolutionRate) */
 } clb_case_params;
 
 typedef struct clb_ctx clb_ctx;
@@ -236,7 +241,9 @@ int  clb_ca_spill(clb_ctx *ctx, int case_id, int *sweeps);
  * 3251-3259) */
 int  clb_ade_collide_stream(clb_ctx *ctx, int case_id, int nsteps);
 #define CLB_KIN_BIOTIC  1   /* run_kinetics + update_rxnLattices (part1:85-187, 406-497)            */
-#define CLB_KIN_ABIOTIC 2   /* run_abiotic_kinetics + update_abiotic_rxnLattices (part1:199-404)   */
+#define CLB_KIN_ABIOTIC 2   /* run_abiotic_kinetics + update_abiotic_rxnLattices (part1:199-404);
+                               with dissolution also surfaceDissolutionKinetics3D and
+                               dissolutionGather3D (dissolutionVOP.hh:378-720), one increment */
 /* rates, clamp and D3Q7 deposit on the current post-collision state; biotic
  * first, then abiotic, as the main loop applies them */
 int  clb_kinetics(clb_ctx *ctx, int case_id, int which);
@@ -256,6 +263,11 @@ int  clb_update_solute_solid(clb_ctx *ctx, int case_id, int *reopened);
 /* precipNodeConversion3D (precipitationVOP.hh:166-227) over the reaction box;
  * *changed = 1 if any voxel turned solid */
 int  clb_precip_convert(clb_ctx *ctx, int case_id, int *changed);
+/* dissolNodeConversion3D (dissolutionVOP.hh:744-812) over the reaction box:
+ * phase voxels below reopen_fraction x full_density turn pore; *reopened =
+ * voxels converted (follow with clb_ns_update_dynamics / clb_ns_solve and
+ * clb_update_solute_solid, as complab.cpp:3209-3246 does) */
+int  clb_dissol_convert(clb_ctx *ctx, int case_id, int *reopened);
 
 /* ---- fields ---------------------------------------------------------------- */
 #define CLB_FIELD_CONC     0  /* species concentration (density after stream) */
@@ -278,8 +290,12 @@ int  clb_write_field(clb_ctx *ctx, int case_id, int field, int index, const doub
 int  clb_species_totals(clb_ctx *ctx, int case_id, double *out);
 /* counters: [0] open voxels, [1] biofilm voxels, [2] clamp binds,
  * [3] equilibrium non-converged voxels, [4] flow re-solves, [5] mask changes,
- * [6] CA sweeps, [7] steps run */
-int  clb_counters(clb_ctx *ctx, int case_id, int64_t out[8]);
+ * [6] CA sweeps, [7] steps run, [8] dissolution product shares parked,
+ * [9] shares collected (equal to [8]: the deposit invariant of
+ * dissolutionVOP.hh:243-265), [10] collected shares the positivity rule
+ * clamped, [11] voxels reopened by dissolution */
+#define CLB_NCOUNTERS 12
+int  clb_counters(clb_ctx *ctx, int case_id, int64_t out[CLB_NCOUNTERS]);
 
 /* ---- profiling ------------------------------------------------------------- */
 /* average device time (ms) of the fused transport kernel over the launches

@@ -356,6 +356,86 @@ private:
     int anyChange;
 };
 
+/* dissolutionVOP.hh:309-342: the phase ids of the grain phase are seeded from
+ * the geometry when the case is created (dissol_material, dissol_phase_id) */
+template <typename T>
+class initPhaseLattice3D {
+public:
+    initPhaseLattice3D(const std::vector<plint> &materials_, const std::vector<plint> &ids_)
+        : materials(materials_), ids(ids_) {}
+    int process(const Target &t) {
+        clb_case_params p;
+        int rc = detail::params(t, p);
+        if (rc) return rc;
+        for (size_t k = 0; k < materials.size() && k < ids.size(); ++k)
+            if (!p.dissol_on || materials[k] != p.dissol_material || ids[k] != p.dissol_phase_id) return detail::mismatch(t);
+        return CLB_OK;
+    }
+private:
+    std::vector<plint> materials, ids;
+};
+
+/* dissolutionVOP.hh:378-569 and 607-720: the park and gather passes run inside
+ * the abiotic increment (clb_kinetics with CLB_KIN_ABIOTIC), so these two only
+ * check that the case's dissolution matches */
+template <typename T, typename Config>
+class surfaceDissolutionKinetics3D {
+public:
+    surfaceDissolutionKinetics3D(plint nx_, plint ny_, plint nz_, plint subsNum_, T dt_, plint solid_, plint bb_,
+                                 const std::vector<plint> &pore_, const Config *cfg_)
+        : nx(nx_), ny(ny_), nz(nz_), subsNum(subsNum_), dt(dt_), solid(solid_), bb(bb_), pore(pore_), cfg(cfg_) {}
+    int process(const Target &t) {
+        clb_case_params p;
+        int rc = detail::params(t, p);
+        if (rc) return rc;
+        if (nx != p.nx || ny != p.ny || nz != p.nz || subsNum != p.n_species || !detail::same(dt, p.dt) || !p.dissol_on)
+            return detail::mismatch(t);
+        return CLB_OK;
+    }
+private:
+    plint nx, ny, nz, subsNum;
+    T dt;
+    plint solid, bb;
+    std::vector<plint> pore;
+    const Config *cfg;
+};
+template <typename T, typename Config>
+class dissolutionGather3D {
+public:
+    dissolutionGather3D(plint nx_, plint ny_, plint nz_, plint subsNum_, plint solid_, plint bb_, const Config *cfg_)
+        : nx(nx_), ny(ny_), nz(nz_), subsNum(subsNum_), solid(solid_), bb(bb_), cfg(cfg_) {}
+    int process(const Target &t) {
+        clb_case_params p;
+        int rc = detail::params(t, p);
+        if (rc) return rc;
+        if (nx != p.nx || ny != p.ny || nz != p.nz || subsNum != p.n_species || !p.dissol_on) return detail::mismatch(t);
+        return CLB_OK;
+    }
+private:
+    plint nx, ny, nz, subsNum, solid, bb;
+    const Config *cfg;
+};
+
+/* dissolutionVOP.hh:744-812; reopened() = voxels that turned pore */
+template <typename T, typename Config>
+class dissolNodeConversion3D {
+public:
+    dissolNodeConversion3D(plint subsNum_, plint poreCode_, plint solid_, plint bb_, const Config *cfg_)
+        : subsNum(subsNum_), poreCode(poreCode_), solid(solid_), bb(bb_), cfg(cfg_), nReopened(0) {}
+    int process(const Target &t) {
+        clb_case_params p;
+        int rc = detail::params(t, p);
+        if (rc) return rc;
+        if (subsNum != p.n_species || poreCode != CLB_PORE || !p.dissol_on) return detail::mismatch(t);
+        return clb_dissol_convert(t.ctx, t.case_id, &nReopened);
+    }
+    int reopened() const { return nReopened; }
+private:
+    plint subsNum, poreCode, solid, bb;
+    const Config *cfg;
+    int nReopened;
+};
+
 /* complab3d_processors_part3.hh:261-310: the reset of the stabilization
  * pre-loop; the steps themselves are clb_ade_stabilize's */
 template <typename T>

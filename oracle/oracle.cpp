@@ -96,7 +96,8 @@ struct Case {
     int nsSaturate = 0;
     long long oldTotMask = 0;
     long long iT = 0;
-    int64_t counters[8] = {0,0,0,0,0,0,0,0};
+    int64_t counters[CLB_NCOUNTERS] = {0,0,0,0,0,0,0,0,0,0,0,0};
+    std::vector<double> share;              // dissolution: per-face product share parked per species, S*N
     uint64_t caSweepCounter = 0;
     std::string err;
 
@@ -289,27 +290,139 @@ void bioticStep(Case &c) {
 
 /* run_abiotic_kinetics (part1:199-297) or surfaceAbioticKinetics3D
  * (precipitationVOP.hh:53-153), then update_abiotic_rxnLattices (part1:306-404) */
+/* the dissolution rate law of the reference's examples (examples/14_dissolution,
+ * examples/15_precip_and_dissolution kinetics/defineAbioticKinetics.hh,
+ * defineDissolutionRate): rate = k max(C[acid],0) max(mineral,0),
+ * mineralR = -rate, subsR[s] = stoich[s] rate; non-finite rates read 0 */
+void dissolutionLaw(const clb_case_params &p, const double *water, double mineral, double *subsR, double &mineralR) {
+    const int S = p.n_species;
+    const double H = p.dissol_acid >= 0 ? std::max(water[p.dissol_acid], 0.0) : 1.0;
+    const double rate = p.dissol_k * H * std::max(mineral, 0.0);
+    mineralR = -rate;
+    for (int s = 0; s < S; ++s) subsR[s] = p.dissol_stoich[s] * rate;
+    if (!std::isfinite(mineralR)) mineralR = 0.0;
+    for (int s = 0; s < S; ++s) if (!std::isfinite(subsR[s])) subsR[s] = 0.0;
+}
+
+static const int DLOC[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
+
+/* surfaceDissolutionKinetics3D (dissolutionVOP.hh:378-569): every voxel of the
+ * declared phase with mineral left averages the water of its open face
+ * neighbours in the reaction box, takes the rate law, removes the mineral
+ * (at most half of it per step, products scaled alike) into its own increment,
+ * and parks one share per open face of every product */
+void dissolutionPark(Case &c, std::vector<double> &dC) {
+    const clb_case_params &p = c.p;
+    const int S = p.n_species, sM = p.dissol_species;
+    std::fill(c.share.begin(), c.share.end(), 0.0);
+    double water[CLB_MAX_SPECIES], subsR[CLB_MAX_SPECIES];
+    for (int x = 1; x < c.nx - 1; ++x) for (int y = 0; y < c.ny; ++y) for (int z = 0; z < c.nz; ++z) {
+        size_t i = c.id(x, y, z);
+        if (c.phase[i] <= 0 || c.phase[i] != p.dissol_phase_id) continue;
+        const double mineral = c.imm[sM][i];
+        if (!(mineral > THRD)) continue;
+        size_t openN[6]; int nOpen = 0;
+        for (int d = 0; d < 6; ++d) {
+            int X = x + DLOC[d][0], Y = y + DLOC[d][1], Z = z + DLOC[d][2];
+            if (X < 1 || X > c.nx - 2 || Y < 0 || Y > c.ny - 1 || Z < 0 || Z > c.nz - 1) continue;
+            size_t j = c.id(X, Y, Z);
+            if (c.solidOrBB(c.mask[j])) continue;
+            openN[nOpen++] = j;
+        }
+        if (nOpen == 0 && p.dissol_surface_only != 0) continue;
+        for (int s = 0; s < S; ++s) water[s] = 0.0;
+        if (nOpen > 0) {
+            for (int k = 0; k < nOpen; ++k) for (int s = 0; s < S; ++s) water[s] += conc(c, s, openN[k]);
+            for (int s = 0; s < S; ++s) water[s] /= (double) nOpen;
+        }
+        for (int s = 0; s < S; ++s) if (water[s] < 0.0) water[s] = 0.0;
+        double mineralR = 0.0;
+        dissolutionLaw(p, water, mineral, subsR, mineralR);
+        if (!(mineralR < 0.0)) continue;
+        double dMineral = mineralR * p.dt;
+        double scale = 1.0;
+        const double maxRemove = 0.5 * mineral;                     // MAX_DISSOL_FRACTION, dissolutionVOP.hh:485
+        if (-dMineral > maxRemove) scale = maxRemove / (-dMineral);
+        dMineral *= scale;
+        if (!(-dMineral > THRD)) continue;
+        dC[(size_t) sM * c.N + i] += dMineral;
+        if (nOpen > 0) {
+            for (int s = 0; s < S; ++s) {
+                if (s == sM) continue;
+                const double released = subsR[s] * p.dt * scale;
+                if (std::fabs(released) <= THRD) continue;
+                c.share[(size_t) s * c.N + i] = released / (double) nOpen;
+                c.counters[8] += nOpen;
+            }
+        }
+    }
+}
+
+/* dissolutionGather3D (dissolutionVOP.hh:607-720): every open voxel of the
+ * reaction box collects one share from each face neighbour of the phase; a
+ * consuming share may not take more than the voxel holds plus what is already
+ * pending (the positivity rule, 669-683) */
+void dissolutionGather(Case &c, std::vector<double> &dC) {
+    const clb_case_params &p = c.p;
+    const int S = p.n_species, sM = p.dissol_species;
+    const double thrdG = 1e-14;
+    for (int x = 1; x < c.nx - 1; ++x) for (int y = 0; y < c.ny; ++y) for (int z = 0; z < c.nz; ++z) {
+        size_t i = c.id(x, y, z);
+        if (c.solidOrBB(c.mask[i])) continue;
+        for (int d = 0; d < 6; ++d) {
+            int X = x + DLOC[d][0], Y = y + DLOC[d][1], Z = z + DLOC[d][2];
+            if (X < 1 || X > c.nx - 2 || Y < 0 || Y > c.ny - 1 || Z < 0 || Z > c.nz - 1) continue;
+            size_t j = c.id(X, Y, Z);
+            if (c.phase[j] <= 0 || c.phase[j] != p.dissol_phase_id) continue;
+            for (int s = 0; s < S; ++s) {
+                if (s == sM) continue;
+                const double share = c.share[(size_t) s * c.N + j];
+                if (std::fabs(share) <= thrdG) continue;
+                double give = share;
+                if (give < 0.0) {
+                    const double have = conc(c, s, i);
+                    const double pend = dC[(size_t) s * c.N + i];
+                    const double headroom = (have > 0.0 ? have : 0.0) + pend;
+                    if (give + headroom < 0.0) { give = -headroom; c.counters[10]++; }
+                    if (give > 0.0) give = 0.0;
+                }
+                c.counters[9]++;
+                if (!(std::fabs(give) > thrdG)) continue;
+                dC[(size_t) s * c.N + i] += give;
+            }
+        }
+    }
+}
+
+/* run_abiotic_kinetics + (dissolution park and gather) + update_abiotic_rxnLattices
+ * (part1:199-404; complab.cpp:2461-2545): one increment per species and voxel */
 void abioticStep(Case &c) {
     const clb_case_params &p = c.p;
     const int S = p.n_species;
     std::vector<double> dC((size_t) S * c.N, 0.0);
-    for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 1; x < c.nx - 1; ++x) {
-        size_t i = c.id(x, y, z);
-        int mask = c.mask[i];
-        if (c.solidOrBB(mask)) continue;
-        if (p.surface_only) {
-            bool iface = false;
-            for (int q = 1; q < Q7 && !iface; ++q) {
-                int X = x + C7[q][0], Y = y + C7[q][1], Z = z + C7[q][2];
-                if (!c.inside(X, Y, Z)) continue;
-                if (c.solidOrBB(c.mask[c.id(X, Y, Z)])) iface = true;
+    if (p.n_rxn > 0) {
+        for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 1; x < c.nx - 1; ++x) {
+            size_t i = c.id(x, y, z);
+            int mask = c.mask[i];
+            if (c.solidOrBB(mask)) continue;
+            if (p.surface_only) {
+                bool iface = false;
+                for (int q = 1; q < Q7 && !iface; ++q) {
+                    int X = x + C7[q][0], Y = y + C7[q][1], Z = z + C7[q][2];
+                    if (!c.inside(X, Y, Z)) continue;
+                    if (c.solidOrBB(c.mask[c.id(X, Y, Z)])) iface = true;
+                }
+                if (!iface) continue;
             }
-            if (!iface) continue;
+            double C[CLB_MAX_SPECIES], sr[CLB_MAX_SPECIES];
+            for (int s = 0; s < S; ++s) { C[s] = conc(c, s, i); if (C[s] < THRD) C[s] = 0; }
+            abioticLaw(p, C, sr);
+            for (int s = 0; s < S; ++s) { double d = sr[s] * p.dt; if (d > THRD || d < -THRD) dC[(size_t) s * c.N + i] += d; }
         }
-        double C[CLB_MAX_SPECIES], sr[CLB_MAX_SPECIES];
-        for (int s = 0; s < S; ++s) { C[s] = conc(c, s, i); if (C[s] < THRD) C[s] = 0; }
-        abioticLaw(p, C, sr);
-        for (int s = 0; s < S; ++s) { double d = sr[s] * p.dt; if (d > THRD || d < -THRD) dC[(size_t) s * c.N + i] += d; }
+    }
+    if (p.dissol_on) {
+        dissolutionPark(c, dC);
+        dissolutionGather(c, dC);
     }
     for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 1; x < c.nx - 1; ++x) {
         size_t i = c.id(x, y, z);
@@ -1004,6 +1117,25 @@ bool precipConvert(Case &c) {
     return changed;
 }
 
+/* dissolNodeConversion3D (dissolutionVOP.hh:744-812) over the reaction box: a
+ * voxel of the phase whose mineral is below reopen_fraction x full_density
+ * turns pore and loses its phase; returns the voxels reopened */
+int dissolConvert(Case &c) {
+    const clb_case_params &p = c.p;
+    if (!(p.dissol_full > 0.0)) return 0;
+    int n = 0;
+    for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 1; x < c.nx - 1; ++x) {
+        size_t i = c.id(x, y, z);
+        if (c.phase[i] <= 0 || c.phase[i] != p.dissol_phase_id) continue;
+        if (c.imm[p.dissol_species][i] < p.dissol_reopen_frac * p.dissol_full) {
+            c.mask[i] = CLB_PORE;
+            c.phase[i] = 0;
+            ++n;
+        }
+    }
+    return n;
+}
+
 /* re-solve after a mask change (complab.cpp:3085-3130) */
 int nsResolve(Case &c, double permRatio) {
     nsUpdateDynamics(c, permRatio);
@@ -1028,7 +1160,7 @@ int coupledStep(Case &c) {
     c.adeClsCollide = c.adeCls;
     for (int s = 0; s < p.n_species; ++s) if (p.mobile[s]) adeCollide(c, s);
     if (p.bio_law != CLB_BIO_NONE && p.n_microbes > 0) bioticStep(c);
-    if (p.n_rxn > 0) abioticStep(c);
+    if (p.n_rxn > 0 || p.dissol_on) abioticStep(c);
     if (p.eq_on) equilibriumStep(c);
     int rc = CLB_OK;
     if (p.n_microbes > 0) {
@@ -1057,6 +1189,23 @@ int coupledStep(Case &c) {
             soluteSolidSwitch(c);
         }
     }
+    if (p.dissol_on && p.dissol_interval > 0 && c.iT % p.dissol_interval == 0) {  // complab.cpp:3203-3247
+        const int n = dissolConvert(c);
+        if (n > 0) {
+            c.counters[5]++;
+            c.counters[11] += n;
+            if (p.flow_on) {
+                nsUpdateDynamics(c, p.perm_ratio);
+                c.nsSaturate = 0;
+                int it, conv;
+                rc = nsSolve(c, p.ns_max_iter, p.ns_conv_rule, p.ns_conv_eps, 0, &it, &conv);
+                if (rc) return rc;
+                c.counters[4]++;
+                nsCouple(c);
+            }
+            soluteSolidSwitch(c);
+        }
+    }
     for (int s = 0; s < p.n_species; ++s) if (p.mobile[s]) { adeStream(c, s); rebuildReopened(c, s); }
     c.reopened.clear();
     c.iT++; c.counters[7]++;
@@ -1068,7 +1217,7 @@ void transportStep(Case &c) {
     const clb_case_params &p = c.p;
     for (int s = 0; s < p.n_species; ++s) if (p.mobile[s]) adeCollide(c, s);
     if (p.bio_law != CLB_BIO_NONE && p.n_microbes > 0) bioticStep(c);
-    if (p.n_rxn > 0) abioticStep(c);
+    if (p.n_rxn > 0 || p.dissol_on) abioticStep(c);
     for (int s = 0; s < p.n_species; ++s) if (p.mobile[s]) adeStream(c, s);
 }
 
@@ -1122,6 +1271,17 @@ void *orc_case_create(const clb_case_params *p, const int8_t *geometry, const do
         }
         c->rhoL[s].assign((size_t) c->ny * c->nz, p->left_btype[s] == CLB_BC_DIRICHLET ? p->left_bc[s] : c0);
         c->rhoR[s].assign((size_t) c->ny * c->nz, p->right_btype[s] == CLB_BC_DIRICHLET ? p->right_bc[s] : c0);
+    }
+    // dissolution: the grain phase and its inventory (complab.cpp:1667-1699)
+    if (p->dissol_on) {
+        c->share.assign((size_t) p->n_species * N, 0.0);
+        if (!p->dissol_is_precipitate && p->dissol_material >= 0) {
+            for (size_t i = 0; i < N; ++i) {
+                if (geometry[i] != p->dissol_material) continue;
+                c->phase[i] = (int8_t) p->dissol_phase_id;
+                if (p->dissol_initial > 0.0) c->imm[p->dissol_species][i] = p->dissol_initial;
+            }
+        }
     }
     c->B.resize(p->n_microbes);
     c->Btot.assign(N, 0.0);
@@ -1270,7 +1430,18 @@ int orc_species_totals(void *h, double *out) {
     }
     return 0;
 }
-int orc_counters(void *h, int64_t *out) { Case &c = *(Case *) h; for (int k = 0; k < 8; ++k) out[k] = c.counters[k]; return 0; }
+int orc_counters(void *h, int64_t *out) {
+    Case &c = *(Case *) h;
+    for (int k = 0; k < CLB_NCOUNTERS; ++k) out[k] = c.counters[k];
+    int64_t open = 0, bio = 0;                       // MaskedScalarCounts3D over the reaction box
+    for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 1; x < c.nx - 1; ++x) {
+        const int m = c.mask[c.id(x, y, z)];
+        if (m >= CLB_PORE) ++open;
+        if (m >= CLB_BIOFILM) ++bio;
+    }
+    out[0] = open; out[1] = bio;
+    return 0;
+}
 
 /* one equilibrium solve on a composition (for the golden comparisons) */
 int orc_eq_solve(const clb_case_params *p, const double *cin, double *cout, int *iters, double *res, int *conv) {

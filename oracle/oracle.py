@@ -186,7 +186,7 @@ class OracleCase:
         return out
 
     def counters(self):
-        out = np.zeros(8, dtype=np.int64)
+        out = np.zeros(12, dtype=np.int64)
         self._chk(lib().orc_counters(self.h, _ptr(out)))
         return out
 

@@ -103,6 +103,11 @@ int main() {
     bad += applyProcessingFunctional(mt, t) != CLB_E_ARG;
     MaskedPopulationSumFunctional3D<double> sums(pore, 0.0);
     bad += applyProcessingFunctional(sums, t) != CLB_E_ARG;
+    struct FakeDissolCfg {} dcfg;
+    bad += applyProcessingFunctional(new surfaceDissolutionKinetics3D<double, FakeDissolCfg>(66, 32, 32, 3, 1.0, 0, 1, pore, &dcfg), t) != CLB_E_ARG;
+    bad += applyProcessingFunctional(new dissolutionGather3D<double, FakeDissolCfg>(66, 32, 32, 3, 0, 1, &dcfg), t) != CLB_E_ARG;
+    bad += applyProcessingFunctional(new dissolNodeConversion3D<double, FakeDissolCfg>(3, 2, 0, 1, &dcfg), t) != CLB_E_ARG;
+    bad += applyProcessingFunctional(new initPhaseLattice3D<double>(std::vector<plint>(1, 0), std::vector<plint>(1, 1)), t) != CLB_E_ARG;
     bad += applyProcessingFunctional(new update_rxnLattices<double>(66, 2, 1, 0, 1), t) != CLB_OK;
     std::printf("%d\n", bad);
     return bad;
