@@ -76,6 +76,7 @@ def lib():
         "clb_rng_draws": ([C.c_uint64, I, V], I),
         "clb_geom_spheres": ([I, I, I, D, I, C.c_uint64, I, V], I),
         "clb_geom_pack": ([I, I, I, D, D, C.c_uint64, I, V, C.POINTER(D)], I),
+        "clb_geom_generate": ([I, I, I, I, D, I, D, D, D, C.c_uint64, I, V], I),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -96,7 +97,7 @@ def exported_symbols():
         "clb_update_solute_dynamics", "clb_update_solute_solid", "clb_precip_convert", "clb_dissol_convert",
         "clb_read_field", "clb_write_field", "clb_species_totals", "clb_counters", "clb_kernel_time",
         "clb_stream_copy",
-        "clb_rng_draws", "clb_geom_spheres", "clb_geom_pack",
+        "clb_rng_draws", "clb_geom_spheres", "clb_geom_pack", "clb_geom_generate",
     ]
 
 
@@ -302,6 +303,16 @@ def geom_spheres(nx, ny, nz, radius, count, seed, walls_yz):
     rc = lib().clb_geom_spheres(nx, ny, nz, radius, count, seed, int(walls_yz), _p(out))
     if rc:
         raise ClbError(rc, "clb_geom_spheres")
+    return out.reshape(nz, ny, nx)
+
+
+def geom_generate(nx, ny, nz, kind, radius=6.0, count=20, porosity=0.4, aperture=6.0, roughness=2.0,
+                  seed=1, walls=2):
+    """complab_geom::generate of the reference: (nz, ny, nx) material codes."""
+    out = np.zeros(nx * ny * nz, dtype=np.int8)
+    rc = lib().clb_geom_generate(nx, ny, nz, kind, radius, count, porosity, aperture, roughness, seed, walls, _p(out))
+    if rc:
+        raise ClbError(rc, "clb_geom_generate")
     return out.reshape(nz, ny, nx)
 
 

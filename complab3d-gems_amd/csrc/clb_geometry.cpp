@@ -8,6 +8,7 @@
 // grain, bounce-back y/z walls, isolated pockets walled off, percolation along
 // x required, and the duplicated ghost x columns of readGeometry
 // (complab_functions.hh:178-207).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -63,9 +64,103 @@ long carve(Box &b, int cx, int cy, int cz, double r, bool cyl, bool boundedLoops
     return removed;
 }
 
+// 27-point periodic box mean, `passes` times (the smoothing of the random-field kind)
+void smoothField(std::vector<double> &f, int nx, int ny, int nz, int passes) {
+    std::vector<double> t(f.size());
+    for (int p = 0; p < passes; ++p) {
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    double s = 0.0;
+                    int n = 0;
+                    for (int dz = -1; dz <= 1; ++dz)
+                        for (int dy = -1; dy <= 1; ++dy)
+                            for (int dx = -1; dx <= 1; ++dx) {
+                                const int X = (x + dx + nx) % nx, Y = (y + dy + ny) % ny, Z = (z + dz + nz) % nz;
+                                s += f[(size_t) X + (size_t) nx * ((size_t) Y + (size_t) ny * Z)];
+                                ++n;
+                            }
+                    t[(size_t) x + (size_t) nx * ((size_t) y + (size_t) ny * z)] = s / (double) n;
+                }
+        f.swap(t);
+    }
+}
+
+// the seeded media of complab_geom::generate (complab3d_geometry.hh:138-202)
+bool generateInto(Box &b, int kind, double radius, int count, double porosity, double aperture, double roughness,
+                  uint64_t seed) {
+    const int nx = b.nx, ny = b.ny, nz = b.nz;
+    XorShift rng(seed);
+    switch (kind) {
+    case CLB_GEOM_CHANNEL:
+        return true;
+    case CLB_GEOM_SPHERES:
+    case CLB_GEOM_CYLINDERS: {
+        const bool cyl = kind == CLB_GEOM_CYLINDERS;
+        for (int k = 0; k < count; ++k) {
+            const int cx = rng.below(nx), cy = rng.below(ny);
+            const int cz = cyl ? 0 : rng.below(nz);
+            carve(b, cx, cy, cz, radius, cyl, false);
+        }
+        return true;
+    }
+    case CLB_GEOM_RANDOM: {
+        // a smoothed uniform field thresholded at the quantile of the porosity
+        std::vector<double> f(b.v.size());
+        for (double &x : f) x = rng.uniform();
+        smoothField(f, nx, ny, nz, 2);
+        std::vector<double> sorted(f);
+        std::sort(sorted.begin(), sorted.end());
+        const size_t cut = (size_t) ((1.0 - porosity) * (double) sorted.size());
+        const double thr = sorted[cut < sorted.size() ? cut : sorted.size() - 1];
+        for (size_t i = 0; i < f.size(); ++i) b.v[i] = f[i] >= thr ? CLB_PORE : CLB_SOLID;
+        return true;
+    }
+    case CLB_GEOM_FRACTURE: {
+        // a rough aperture around y = ny/2, three sine modes per z slice
+        const double twoPi = 6.283185307179586;
+        for (int8_t &m : b.v) m = CLB_SOLID;
+        const double mid = 0.5 * ny;
+        std::vector<double> wob((size_t) nx);
+        for (int z = 0; z < nz; ++z) {
+            std::fill(wob.begin(), wob.end(), 0.0);
+            for (int k = 1; k <= 3; ++k) {
+                const double ph = rng.uniform() * twoPi;
+                for (int x = 0; x < nx; ++x) wob[(size_t) x] += (roughness / k) * std::sin(twoPi * k * x / nx + ph);
+            }
+            for (int x = 0; x < nx; ++x)
+                for (int y = 0; y < ny; ++y)
+                    if (std::fabs(y - (mid + wob[(size_t) x])) <= 0.5 * aperture) b.v[b.at(x, y, z)] = CLB_PORE;
+        }
+        return true;
+    }
+    }
+    return false;
+}
+
+// addWalls (complab3d_geometry.hh:92-103): bit 1 x, 2 y, 4 z
+void addWalls(Box &b, int axes) {
+    for (int z = 0; z < b.nz; ++z)
+        for (int y = 0; y < b.ny; ++y)
+            for (int x = 0; x < b.nx; ++x) {
+                const bool onX = (x == 0 || x == b.nx - 1), onY = (y == 0 || y == b.ny - 1), onZ = (z == 0 || z == b.nz - 1);
+                if (((axes & 1) && onX) || ((axes & 2) && onY) || ((axes & 4) && onZ)) b.v[b.at(x, y, z)] = CLB_WALL;
+            }
+}
+
 }  // namespace
 
 extern "C" {
+
+int clb_geom_generate(int nx, int ny, int nz, int kind, double radius, int count, double porosity, double aperture,
+                      double roughness, uint64_t seed, int walls, int8_t *out) {
+    if (nx <= 0 || ny <= 0 || nz <= 0 || count < 0 || !out) return CLB_E_ARG;
+    Box b(nx, ny, nz, CLB_PORE);
+    if (!generateInto(b, kind, radius, count, porosity, aperture, roughness, seed)) return CLB_E_ARG;
+    addWalls(b, walls);
+    for (size_t i = 0; i < b.v.size(); ++i) out[i] = b.v[i];
+    return CLB_OK;
+}
 
 int clb_rng_draws(uint64_t seed, int n, uint64_t *out) {
     if (n < 0 || (n > 0 && !out)) return CLB_E_ARG;
@@ -94,20 +189,27 @@ int clb_geom_spheres(int nx, int ny, int nz, double radius, int count, uint64_t 
 int clb_geom_pack(int nxf, int ny, int nz, double radius, double porosity, uint64_t seed, int kind,
                   int8_t *lat, double *porosity_out) {
     if (nxf <= 1 || ny <= 2 || nz <= 2 || radius <= 0 || !lat) return CLB_E_ARG;
-    if (kind != CLB_GEOM_SPHERES && kind != CLB_GEOM_CYLINDERS) return CLB_E_ARG;
+    if (kind != CLB_GEOM_SPHERES && kind != CLB_GEOM_CYLINDERS && kind != CLB_GEOM_RANDOM && kind != CLB_GEOM_FRACTURE)
+        return CLB_E_ARG;
     Box b(nxf, ny, nz, CLB_PORE);
-    for (int z = 0; z < nz; ++z)
-        for (int y = 0; y < ny; ++y)
-            for (int x = 0; x < nxf; ++x)
-                if (y == 0 || y == ny - 1 || z == 0 || z == nz - 1) b.v[b.at(x, y, z)] = CLB_WALL;
-    long open = 0;
-    for (int8_t m : b.v) open += (m == CLB_PORE);
     const double total = (double) b.v.size();
-    XorShift rng(seed);
-    const bool cyl = (kind == CLB_GEOM_CYLINDERS);
-    for (int guard = 0; (double) open / total > porosity && guard < 10000000; ++guard) {
-        const int cx = rng.below(nxf), cy = rng.below(ny), cz = rng.below(nz);
-        open -= carve(b, cx, cy, cz, radius, cyl, true);
+    if (kind == CLB_GEOM_RANDOM) {
+        generateInto(b, kind, radius, 0, porosity, 0.0, 0.0, seed);
+        addWalls(b, 6);
+    } else if (kind == CLB_GEOM_FRACTURE) {
+        // aperture from the porosity of the slab (walls excluded), roughness radius / 2
+        generateInto(b, kind, radius, 0, porosity, std::max(2.0, porosity * (ny - 2)), 0.5 * radius, seed);
+        addWalls(b, 6);
+    } else {
+        addWalls(b, 6);
+        long open = 0;
+        for (int8_t m : b.v) open += (m == CLB_PORE);
+        XorShift rng(seed);
+        const bool cyl = (kind == CLB_GEOM_CYLINDERS);
+        for (int guard = 0; (double) open / total > porosity && guard < 10000000; ++guard) {
+            const int cx = rng.below(nxf), cy = rng.below(ny), cz = rng.below(nz);
+            open -= carve(b, cx, cy, cz, radius, cyl, true);
+        }
     }
     // flood fill: keep the pore cluster(s) touching both x faces
     std::vector<int> label(b.v.size(), 0);

@@ -314,8 +314,9 @@ int  clb_rng_draws(uint64_t seed, int n, uint64_t *out);
 /* complab_geom::generate "spheres" with `count` spheres (x fastest, no ghosts) */
 int  clb_geom_spheres(int nx, int ny, int nz, double radius, int count, uint64_t seed,
                       int walls_yz, int8_t *out);
-/* sphere pack inserted until porosity <= target, grain shell -> wall, y/z
- * walls, isolated pockets -> wall, percolation checked; writes the lattice
+/* seeded medium (spheres / cylinders inserted until porosity <= target, or a
+ * random field / fracture of that porosity), grain shell -> wall, y/z walls,
+ * isolated pockets -> wall, percolation checked; writes the lattice
  * with ghost columns (nx_file+2) * ny * nz.  *porosity = open fraction of the
  * file box.  Returns CLB_E_GEOMETRY when the pore space does not span x. */
 int  clb_geom_pack(int nx_file, int ny, int nz, double radius, double porosity,
@@ -323,7 +324,13 @@ int  clb_geom_pack(int nx_file, int ny, int nz, double radius, double porosity,
 #define CLB_GEOM_SPHERES   0
 #define CLB_GEOM_RANDOM    1  /* smoothed random field thresholded at the quantile */
 #define CLB_GEOM_CYLINDERS 2
-#define CLB_GEOM_FRACTURE  3
+#define CLB_GEOM_FRACTURE  3  /* rough aperture around y = ny/2 */
+#define CLB_GEOM_CHANNEL   4  /* nothing carved */
+/* complab_geom::generate + addWalls (complab3d_geometry.hh:92-202): the
+ * reference's seeded media, x fastest, no ghost columns; walls: bit 1 x, 2 y,
+ * 4 z */
+int clb_geom_generate(int nx, int ny, int nz, int kind, double radius, int count, double porosity,
+                      double aperture, double roughness, uint64_t seed, int walls, int8_t *out);
 
 #ifdef __cplusplus
 }

@@ -77,6 +77,13 @@ def read_dat(path, dims):
     return v
 
 
+def geometry_fracture():
+    """complab_geom::generate "fracture" (aperture 6, roughness 2: GenOptions'
+    defaults, complab3d_geometry.hh:113) with walls on y"""
+    frac = O.ref_geom_generate(40, 24, 8, 3, 3.0, 0, 0.3, 9, "y")
+    np.savez_compressed(os.path.join(HERE, "geometry_fracture.npz"), fracture=frac)
+
+
 def main():
     if O.ref() is None:
         raise SystemExit("oracle/_ref/libref.so is missing: run `make -C oracle ref` where the reference tree exists")
@@ -98,6 +105,7 @@ def main():
     rnd = O.ref_geom_generate(16, 12, 10, 1, 3.0, 0, 0.35, 11, "none")
     np.savez_compressed(os.path.join(HERE, "geometry_generate.npz"), rng_seed=np.uint64(20240601), rng=draws,
                         spheres=sph, cylinders=cyl, random=rnd)
+    geometry_fracture()
     # ---- ABC sweep geometry fixtures: porosity / percolation of the shipped files
     gdir = os.path.join(REF_TREE, "ABC_sweep_package", "geometries")
     insp = {}
@@ -120,4 +128,7 @@ def main():
 
 
 if __name__ == "__main__":
+    if sys.argv[1:] == ["fracture"]:
+        geometry_fracture()
+        sys.exit(0)
     main()

@@ -51,6 +51,38 @@ def test_geometry_generator_matches_reference_vectors():
     d = np.load(os.path.join(GOLD, "geometry_generate.npz"))
     assert np.array_equal(clb.rng_draws(int(d["rng_seed"]), 32), d["rng"])
     assert np.array_equal(clb.geom_spheres(24, 16, 12, 3.0, 14, 77, 1), d["spheres"])
+    # every kind of complab_geom::generate, bit for bit (walls: 2 = y, 4 = z)
+    assert np.array_equal(clb.geom_generate(24, 16, 12, P.GEOM_SPHERES, 3.0, 14, 0.4, seed=77, walls=6), d["spheres"])
+    assert np.array_equal(clb.geom_generate(20, 14, 6, P.GEOM_CYLINDERS, 2.5, 9, 0.4, seed=5, walls=2), d["cylinders"])
+    assert np.array_equal(clb.geom_generate(16, 12, 10, P.GEOM_RANDOM, 3.0, 0, 0.35, seed=11, walls=0), d["random"])
+    f = np.load(os.path.join(GOLD, "geometry_fracture.npz"))["fracture"]
+    assert np.array_equal(clb.geom_generate(40, 24, 8, P.GEOM_FRACTURE, 3.0, 0, 0.3, 6.0, 2.0, seed=9, walls=2), f)
+    ch = clb.geom_generate(8, 6, 5, P.GEOM_CHANNEL, walls=2)
+    assert np.all(ch[:, 1:-1, :] == P.PORE) and np.all(ch[:, 0, :] == P.WALL)
+
+
+@pytest.mark.parametrize("kind", [P.GEOM_RANDOM, P.GEOM_FRACTURE, P.GEOM_CYLINDERS])
+def test_geometry_pack_kinds_percolate(kind):
+    """The pack builder's media: y/z walls, ghost columns duplicated, every
+    open voxel in a cluster that spans x, grains shelled by bounce-back."""
+    for seed in range(7, 27):                  # a seed whose medium does not span x is skipped
+        rc, lat, phi = clb.geom_pack(40, 20, 20, 3.0, 0.35, seed, kind)
+        if rc == 0:
+            break
+    assert rc == 0
+    assert np.array_equal(lat[:, :, 0], lat[:, :, 1]) and np.array_equal(lat[:, :, -1], lat[:, :, -2])
+    assert np.all(lat[:, 0, :] == P.WALL) and np.all(lat[:, :, :][0] == P.WALL)
+    from scipy import ndimage
+    lab, n = ndimage.label(lat == P.PORE, structure=ndimage.generate_binary_structure(3, 1))
+    west, east = set(np.unique(lab[:, :, 0])) - {0}, set(np.unique(lab[:, :, -1])) - {0}
+    assert set(range(1, n + 1)) <= (west & east)
+    solid = lat == P.SOLID
+    touch = np.zeros_like(solid)
+    for ax in range(3):
+        for sh in (1, -1):
+            touch |= np.roll(lat == P.PORE, sh, axis=ax)
+    assert not np.any(solid[1:-1, 1:-1, 1:-1] & touch[1:-1, 1:-1, 1:-1])
+    assert 0.05 < phi < 0.6
 
 
 def test_footprint_counts_populations():
