@@ -14,6 +14,18 @@ over its HIP-event-timed average duration, against 8 TB/s; `traffic` is the
 HBM bytes per launch of the committed rocprofv3 counter summary in profiles/.
 cpu_baseline: the CPU oracle (a scalar C++ restatement) on the same case for a
 few steps, velocity copied from the GPU flow solve.
+sweep: the batched form -- seeded sweep cases of the batched workload (cuboids
+with edges drawn in 32..160, aspect <= 4, spheres / cylinders / fracture,
+2-6 species, Monod growth), each through its full set-up and `--sweep-steps`
+steps: cases/hour of the plain path (FP64, one case at a time) and of the
+batched path (several cases on the device together, FP32 transport with FP64
+flow, and all-FP32), with the largest relative field error against the plain
+path.
+serving: ABC-style requests (A + B -> mineral, precipitation re-solving the
+flow every 100 steps) arriving as a Poisson stream at 2/s, admitted by HBM
+footprint as workers free up, run to steady state or `--serve-steps`; p50/p99
+time to solution (arrival to last step), cases/hour, and the plain path (the
+same requests one at a time) with the error between the two.
 """
 import argparse
 import json
@@ -47,7 +59,7 @@ def profiled_traffic(p, n_open):
 
 
 def algorithmic_bytes(p, n_open, n_vox):
-    r = 4 if p.precision == 1 else 8
+    r = 8 if p.precision == 0 else 4
     sm = sum(1 for s in range(p.n_species) if p.mobile[s])
     si = p.n_species - sm
     per_open = sm * 14 * r + 3 * r + (si + p.n_microbes) * 2 * r
@@ -70,6 +82,58 @@ def cpu_baseline(w, g, rep, steps):
     return V * n / dt / 1e6, n, dt
 
 
+def sweep_measurement(args):
+    from complab_amd import configs, sweep
+    from complab_amd import params as P
+    n, steps, W = args.sweep_cases, args.sweep_steps, args.workers
+    plain = [configs.sweep_case(s, P.FP64, steps=steps) for s in range(n)]
+    r64, t64 = sweep.run_sequential(plain)
+    if any(r.status for r in r64):
+        raise SystemExit("a plain sweep case failed: %s" % [r.error or r.status for r in r64])
+    cph64 = n / t64 * 3600.0
+    res = {"cases": n, "steps": steps, "workers": W,
+           "workload": "seeded batched-form cases: edges 32..160, aspect <= 4, spheres/cylinders/fracture, "
+                       "porosity 0.25-0.5, 2-6 species, flow tau 0.6-1.2, Monod growth; full set-up per case",
+           "plain_fp64": {"wall_s": round(t64, 3), "cases_per_hour": round(cph64, 1)}}
+    for label, prec in (("fp32_ade", P.FP32_ADE), ("fp32", P.FP32)):
+        ws = [configs.sweep_case(s, prec, steps=steps) for s in range(n)]
+        r, t = sweep.run_batched(ws, workers=W)
+        if any(x.status for x in r):
+            raise SystemExit("a batched sweep case failed: %s" % [x.error or x.status for x in r])
+        err = max(sweep.field_error(a, b) for a, b in zip(r, r64))
+        res["batched_" + label] = {"wall_s": round(t, 3), "cases_per_hour": round(n / t * 3600.0, 1),
+                                   "speedup_vs_plain_fp64": round(t64 / t, 3), "max_rel_err": float("%.3g" % err)}
+    print("sweep: plain %.1f s, batched fp32_ade %.1f s, fp32 %.1f s" %
+          (t64, res["batched_fp32_ade"]["wall_s"], res["batched_fp32"]["wall_s"]), file=sys.stderr, flush=True)
+    return res
+
+
+def serving_measurement(args):
+    """Serving runs FP64: precipitation seals a voxel at a threshold, so any
+    lower precision can seal it a check earlier or later and change the pore
+    space; in FP64 every request's result equals its plain one-at-a-time run."""
+    from complab_amd import configs, sweep
+    from complab_amd import params as P
+    n, W = args.serve_requests, args.workers
+    size = (48, max(48, args.serve_size))
+    reqs = [configs.serving_case(100 + s, P.FP64, size=size, max_steps=args.serve_steps) for s in range(n)]
+    r, wall = sweep.serve(reqs, rate_per_s=2.0, workers=W, steady_tol=1e-6, seed=7)
+    if any(x.status for x in r):
+        raise SystemExit("a serving request failed: %s" % [x.error or x.status for x in r])
+    lat = [x.latency for x in r]
+    plain, t_plain = sweep.run_sequential(reqs, steady_tol=1e-6)
+    err = max(sweep.field_error(a, b) for a, b in zip(r, plain))
+    print("serving: wall %.1f s, p50 %.2f s, p99 %.2f s, plain one-at-a-time %.1f s" %
+          (wall, sweep.percentile(lat, 50), sweep.percentile(lat, 99), t_plain), file=sys.stderr, flush=True)
+    return {"requests": n, "rate_per_s": 2.0, "workers": W, "precision": "fp64",
+            "workload": "ABC-style A+B->mineral, precipitation with flow re-solve every 100 steps, edges %d..%d, "
+                        "3-5 species, to steady state (1e-6) or %d steps" % (size[0], size[1], args.serve_steps),
+            "p50_s": round(sweep.percentile(lat, 50), 3), "p99_s": round(sweep.percentile(lat, 99), 3),
+            "wall_s": round(wall, 3), "cases_per_hour": round(n / wall * 3600.0, 1),
+            "plain_fp64_wall_s": round(t_plain, 3), "plain_fp64_cases_per_hour": round(n / t_plain * 3600.0, 1),
+            "steps": [x.steps for x in r], "max_rel_err_vs_plain_fp64": float("%.3g" % err)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +146,13 @@ def main():
     ap.add_argument("--stabilize", type=int, default=10000)
     ap.add_argument("--ns-max-iter", type=int, default=100000,
                     help="cap of the set-up flow solves (profiling runs shorten the untimed set-up)")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched and serving measurements")
+    ap.add_argument("--sweep-cases", type=int, default=16)
+    ap.add_argument("--sweep-steps", type=int, default=1000)
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-steps", type=int, default=5000)
+    ap.add_argument("--serve-size", type=int, default=128, help="largest edge of a serving request")
+    ap.add_argument("--workers", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -136,6 +207,10 @@ def main():
         cpu = {"value": round(cv, 4), "unit": "MLUPS", "cores": 1, "kind": "port",
                "sample": "%d coupled steps of the same %dx%dx%d case by the scalar C++ oracle, "
                          "velocity copied from the GPU flow solve (%.1f s)" % (cn, p.nx, p.ny, p.nz, cdt)}
+    sweep_out = serve_out = None
+    if not args.no_sweep:
+        sweep_out = sweep_measurement(args)
+        serve_out = serving_measurement(args)
     out = {
         "metric": "coupled MLUPS + %HBM roofline; sweep cases/hr and p50/p99 vs plain FP64",
         "value": round(mlups, 2),
@@ -160,6 +235,8 @@ def main():
                      "bytes_per_open_voxel": per_open, "bytes_per_launch": int(bytes_launch),
                      "copy_kernel_gbs": round(copy_gbs, 1)},
         "cpu_baseline": cpu,
+        "sweep": sweep_out,
+        "serving": serve_out,
     }
     print(json.dumps(out), flush=True)
 

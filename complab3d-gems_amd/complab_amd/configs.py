@@ -128,3 +128,105 @@ def config2(scale=1.0, precision=P.FP64, packer=None, seed=2):
     ts = TransportSetup(d_pore=[0.1] * 4, d_biofilm=[0.02] * 4)
     return Workload("biotic-fp64 %dx%dx%d spheres r%g phi0.35 4 species CA" % (nxf, ny, nz, radius),
                     p, lat, phi, fs, ts, 10000)
+
+
+def _draw_box(rng, lo, hi, max_aspect=4.0):
+    """Edge lengths uniform in [lo, hi] with the longest at most max_aspect
+    times the shortest."""
+    while True:
+        e = rng.integers(lo, hi + 1, size=3)
+        if e.max() <= max_aspect * e.min():
+            return int(e[0]), int(e[1]), int(e[2])
+
+
+def sweep_case(seed, precision=P.FP32, size=(32, 160), steps=2000, stabilize=200):
+    """Batched form, one case: a seeded medium (spheres / cylinders / fracture,
+    porosity 0.25-0.5) of a cuboid drawn in size^3 with aspect up to 4:1,
+    2-6 D3Q7 species, flow tau in [0.6, 1.2], Monod growth of a biofilm seeded
+    on 2 % of the grain surface with mu_max and K log-uniform over one decade.
+    The biofilm never spills (no CA threshold is reached), so the case's
+    topology is fixed and its fields measure transport and kinetics alone."""
+    rng = np.random.default_rng(seed)
+    nxf, ny, nz = _draw_box(rng, size[0], size[1])
+    kind = (P.GEOM_SPHERES, P.GEOM_CYLINDERS, P.GEOM_FRACTURE)[int(rng.integers(0, 3))]
+    phi = float(rng.uniform(0.25, 0.5))
+    radius = float(rng.uniform(3.0, 6.0))
+    lat, phi_got = _pack(nxf, ny, nz, radius, phi, int(rng.integers(1, 1 << 30)), kind=kind)
+    lat, _ = seed_biofilm(lat, 0.02, seed)
+    ns = int(rng.integers(2, 7))
+    p = P.default_params(nxf + 2, ny, nz, ns)
+    p.precision = precision
+    for s in range(ns):
+        p.c0[s] = 0.5 if s < 2 else 0.0
+        p.left_btype[s] = P.BC_DIRICHLET; p.left_bc[s] = 1.0 if s < 2 else 0.0
+        p.right_btype[s] = P.BC_NEUMANN
+    p.bio_law = P.BIO_DUAL_MONOD
+    p.donor = 0
+    p.acceptor = 1 if ns >= 3 else -1
+    p.n_microbes = 1
+    p.mu_max[0] = float(10 ** rng.uniform(-3, -2))
+    p.ks[0] = float(10 ** rng.uniform(-1.3, -0.3))
+    p.ke[0] = float(10 ** rng.uniform(-1.3, -0.3))
+    p.bio_stoich[0][0] = -0.02
+    if ns >= 3:
+        p.bio_stoich[0][1] = -0.01
+    for s in range(2 if ns >= 3 else 1, ns):
+        p.bio_stoich[0][s] = 0.01 / (s + 1)
+    p.b0[0] = 10.0
+    p.bmax = 1e30
+    p.thrd_bfilm_frac = 0.0
+    tau = float(rng.uniform(0.6, 1.2))
+    p.ns_tau = tau
+    p.flow_on = 1
+    p.ns_conv_rule = P.CONV_RELDELTA
+    p.ns_conv_eps = 1e-6
+    p.ns_max_iter = 20000
+    fs = FlowSetup(pe=1.0, char_length=float(nxf), delta_p=1e-5 * (nxf + 1), ns_tau=tau,
+                   rule=P.CONV_RELDELTA, eps=1e-6, max_iter=20000, has_biofilm=True)
+    ts = TransportSetup(d_pore=[0.1] * ns, d_biofilm=[0.05] * ns, stabilize_steps=stabilize)
+    names = {P.GEOM_SPHERES: "spheres", P.GEOM_CYLINDERS: "cylinders", P.GEOM_FRACTURE: "fracture"}
+    return Workload("sweep-%d %dx%dx%d %s phi%.2f %d species tau%.2f" % (seed, nxf, ny, nz, names[kind], phi_got, ns, tau),
+                    p, lat, phi_got, fs, ts, steps)
+
+
+def serving_case(seed, precision=P.FP32, size=(48, 256), max_steps=20000, stabilize=200):
+    """Serving form, one request: an ABC-style case (A + B -> P on the grain
+    surface, P an immobile mineral that seals a voxel at 0.5 mol/L, the flow
+    re-solved at every precipitation check, 100 steps apart) in a seeded sphere
+    pack of a cuboid drawn in size^3, 3-5 species (A, B, P and up to two inert
+    tracers), run to steady state or max_steps."""
+    rng = np.random.default_rng(seed)
+    nxf, ny, nz = _draw_box(rng, size[0], size[1])
+    lat, phi_got = _pack(nxf, ny, nz, float(rng.uniform(3.0, 5.0)), float(rng.uniform(0.3, 0.45)),
+                         int(rng.integers(1, 1 << 30)))
+    ns = int(rng.integers(3, 6))
+    A, B, PM = 0, 1, 2
+    p = P.default_params(nxf + 2, ny, nz, ns)
+    p.precision = precision
+    p.mobile[PM] = 0
+    for s in range(ns):
+        p.c0[s] = 0.0
+        p.left_btype[s] = P.BC_DIRICHLET; p.right_btype[s] = P.BC_NEUMANN
+        p.left_bc[s] = 0.0
+    p.left_bc[A] = 1.0
+    p.c0[B] = 1.0
+    for s in range(3, ns):
+        p.left_bc[s] = 1.0                 # inert tracers fed at the inlet
+    p.n_rxn = 1
+    p.rxn_r1[0] = A; p.rxn_r2[0] = B
+    p.rxn_k[0] = float(10 ** rng.uniform(-2.5, -1.5))
+    p.rxn_stoich[0][A] = -1.0; p.rxn_stoich[0][B] = -1.0; p.rxn_stoich[0][PM] = 1.0
+    p.surface_only = 1
+    p.precip_on = 1; p.precip_species = PM; p.precip_max_rho = 0.5
+    p.precip_interval = 100; p.precip_phase_id = 1
+    p.precip_perm_ratio = 0.0
+    p.ns_tau = 0.8
+    p.flow_on = 1
+    p.ns_conv_rule = P.CONV_RELDELTA
+    p.ns_conv_eps = 1e-6
+    p.ns_max_iter = 20000
+    fs = FlowSetup(pe=1.0, char_length=float(nxf), delta_p=1e-5 * (nxf + 1), ns_tau=0.8,
+                   rule=P.CONV_RELDELTA, eps=1e-6, max_iter=20000)
+    ts = TransportSetup(d_pore=[0.1] * ns, d_biofilm=[0.1] * ns, stabilize_steps=stabilize)
+    return Workload("serve-%d %dx%dx%d spheres phi%.2f %d species" % (seed, nxf, ny, nz, phi_got, ns),
+                    p, lat, phi_got, fs, ts, max_steps)

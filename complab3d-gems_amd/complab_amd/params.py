@@ -16,7 +16,7 @@ E_ARG, E_HIP, E_NOMEM, E_STATE, E_NONFINITE, E_CA_STALL, E_CA_SWEEPS, E_AGE_SWEE
     -1, -2, -3, -4, -5, -6, -7, -8, -9)
 
 SOLID, WALL, PORE, BIOFILM = 0, 1, 2, 3
-FP64, FP32 = 0, 1
+FP64, FP32, FP32_ADE = 0, 1, 2
 BC_DIRICHLET, BC_NEUMANN, BC_CLOSED = 0, 1, 2
 CONV_TRACER, CONV_RELDELTA = 0, 1
 BIO_NONE, BIO_DUAL_MONOD = 0, 1

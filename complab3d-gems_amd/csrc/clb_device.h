@@ -162,7 +162,7 @@ template <typename R> hipError_t launchAdeSet(const AdeDev<R> &, int, const doub
 template <typename R> hipError_t launchAdeReopen(const AdeDev<R> &, const uint8_t *, hipStream_t);
 template <typename R> hipError_t launchAdeTotal(const AdeDev<R> &, int, double *, hipStream_t);
 template <typename R> hipError_t launchNsStep(const NsDev<R> &, double *, double *, NsTracer *, int, double, hipStream_t);
-template <typename R> hipError_t launchNsVelocity(const NsDev<R> &, R *, double *, hipStream_t);
+template <typename R, typename RA> hipError_t launchNsVelocity(const NsDev<R> &, RA *, double *, hipStream_t);
 template <typename R> hipError_t launchNsInit(const NsDev<R> &, double, hipStream_t);
 hipError_t launchNsUpdateDyn(const int8_t *, const int *, uint8_t *, int, double, unsigned int *, hipStream_t);
 template <typename R> hipError_t launchNsReopen(const NsDev<R> &, const uint8_t *, hipStream_t);

@@ -84,7 +84,9 @@ T *dalloc(std::vector<void *> &allocs, size_t n) {
 
 inline unsigned int nblk(long long n, int b) { return (unsigned int) ((n + b - 1) / b); }
 
-template <typename R>
+// R: storage of the transport lattices, immobile species and biomass;
+// RN: storage of the flow lattice
+template <typename R, typename RN = R>
 struct Case : CaseBase {
     // fields
     int8_t *mask = nullptr, *geom = nullptr, *dist = nullptr, *phase = nullptr;
@@ -95,7 +97,8 @@ struct Case : CaseBase {
     R *fA = nullptr, *fB = nullptr, *imm = nullptr, *B = nullptr, *Btot = nullptr, *outbox = nullptr;
     uint8_t *pushed = nullptr;
     int *age = nullptr, *age2 = nullptr;
-    R *gA = nullptr, *gB = nullptr, *u = nullptr;
+    RN *gA = nullptr, *gB = nullptr;
+    R *u = nullptr;
     double *rhoN = nullptr, *parts = nullptr, *scal = nullptr, *stage = nullptr;
     long long *partsLL = nullptr, *scalLL = nullptr;
     unsigned long long *clampCount = nullptr;
@@ -142,8 +145,8 @@ struct Case : CaseBase {
 
     R *fCur() const { return fCurIsA ? fA : fB; }
     R *fOut() const { return fCurIsA ? fB : fA; }
-    R *gCur() const { return gCurIsA ? gA : gB; }
-    R *gOut() const { return gCurIsA ? gB : gA; }
+    RN *gCur() const { return gCurIsA ? gA : gB; }
+    RN *gOut() const { return gCurIsA ? gB : gA; }
 
     AdeDev<R> ade(const int *halt = nullptr) const {
         AdeDev<R> a;
@@ -194,8 +197,8 @@ struct Case : CaseBase {
         if (!k.bio && k.nrxn == 0 && !k.dissol) k.on = 0;
         return k;
     }
-    NsDev<R> ns() const {
-        NsDev<R> a;
+    NsDev<RN> ns() const {
+        NsDev<RN> a;
         memset(&a, 0, sizeof(a));
         a.nx = nx; a.ny = ny; a.nz = nz; a.N = N;
         a.Nopen = Nopen; a.Nint = Nint; a.Nleft = Nleft; a.list = list; a.nb = nb19; a.geom = geom;
@@ -314,7 +317,7 @@ struct Case : CaseBase {
         }
         age = dalloc<int>(allocs, N); age2 = dalloc<int>(allocs, N);
         if (p.flow_on) {
-            gA = dalloc<R>(allocs, 19 * (size_t) Nopen); gB = dalloc<R>(allocs, 19 * (size_t) Nopen);
+            gA = dalloc<RN>(allocs, 19 * (size_t) Nopen); gB = dalloc<RN>(allocs, 19 * (size_t) Nopen);
             nb19 = dalloc<int>(allocs, (size_t) 18 * Nopen);
         }
         u = dalloc<R>(allocs, 3 * (size_t) Nopen);
@@ -454,7 +457,7 @@ struct Case : CaseBase {
         HIPT(hipMemcpyAsync(nsClsPull, nc.data(), Nopen, hipMemcpyHostToDevice, st));
         nsBioOmega = 1.0 / (p.perm_ratio * (1.0 / nsOmega - 0.5) + 0.5);
         gCurIsA = true;
-        HIPT(launchNsInit<R>(ns(), dP, st));
+        HIPT(launchNsInit<RN>(ns(), dP, st));
         nsFirst = true;
         nsClsDirty = false;
         HIPT(hipStreamSynchronize(st));
@@ -481,9 +484,9 @@ struct Case : CaseBase {
             const int batch = (int) std::min<long long>(100, maxIter - ran);
             bool curA = parity;
             for (int k = 0; k < batch; ++k) {
-                NsDev<R> a = ns();
+                NsDev<RN> a = ns();
                 a.cur = curA ? gA : gB; a.out = curA ? gB : gA;
-                HIPT(launchNsStep<R>(a, partE, partN, tracer, rule, eps, st));
+                HIPT(launchNsStep<RN>(a, partE, partN, tracer, rule, eps, st));
                 nsFirst = false;
                 if (!launched && nsClsDirty) { HIPT(hipMemcpyAsync(nsClsPull, nsCls, Nopen, hipMemcpyDeviceToDevice, st)); nsClsDirty = false; }
                 launched = true;
@@ -507,13 +510,13 @@ struct Case : CaseBase {
 
     void nsCouple(hipStream_t st) override {
         if (!p.flow_on) return;
-        HIPT(launchNsVelocity<R>(ns(), u, nullptr, st));
+        HIPT((launchNsVelocity<RN, R>(ns(), u, nullptr, st)));
     }
     void nsStats(hipStream_t st, double out[4]) override {
         for (int k = 0; k < 4; ++k) out[k] = 0;
         if (!p.flow_on) return;
         const unsigned int g = nblk(Nopen, 256);
-        HIPT(launchNsVelocity<R>(ns(), nullptr, parts, st));
+        HIPT((launchNsVelocity<RN, R>(ns(), nullptr, parts, st)));
         for (int c = 0; c < 5; ++c) HIPT(launchFinalSum(parts, g, 5, c, c == 1, scal + c, st));
         double h[5];
         HIPT(hipMemcpyAsync(h, scal, 5 * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -533,7 +536,7 @@ struct Case : CaseBase {
         HIPT(hipMemcpyAsync(&ch, ucnt, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
         HIPT(hipStreamSynchronize(st));
         if (ch) {
-            HIPT(launchNsReopen<R>(ns(), nullptr, st));
+            HIPT(launchNsReopen<RN>(ns(), nullptr, st));
             nsClsDirty = true;
         }
     }
@@ -1007,7 +1010,8 @@ CaseBase *getCase(clb_ctx *ctx, int id) {
 int checkParams(clb_ctx *ctx, const clb_case_params *p, const int8_t *geometry) {
     if (!p || !geometry) return fail(ctx, CLB_E_ARG, "null parameters or geometry");
     if (p->nx < 3 || p->ny < 1 || p->nz < 1) return fail(ctx, CLB_E_ARG, "lattice must be at least 3 x 1 x 1");
-    if (p->precision != CLB_FP64 && p->precision != CLB_FP32) return fail(ctx, CLB_E_ARG, "unknown precision");
+    if (p->precision != CLB_FP64 && p->precision != CLB_FP32 && p->precision != CLB_FP32_ADE)
+        return fail(ctx, CLB_E_ARG, "unknown precision");
     if (p->n_species < 0 || p->n_species > CLB_MAX_SPECIES) return fail(ctx, CLB_E_ARG, "n_species out of range");
     if (p->n_microbes < 0 || p->n_microbes > CLB_MAX_MICROBES) return fail(ctx, CLB_E_ARG, "n_microbes out of range");
     if (p->n_rxn < 0 || p->n_rxn > CLB_MAX_RXN) return fail(ctx, CLB_E_ARG, "n_rxn out of range");
@@ -1103,7 +1107,8 @@ int clb_sync(clb_ctx *ctx) {
 size_t clb_case_footprint(const clb_case_params *p) {
     if (!p) return 0;
     const size_t N = (size_t) p->nx * p->ny * p->nz;
-    const size_t r = p->precision == CLB_FP32 ? 4 : 8;
+    const size_t r = p->precision == CLB_FP64 ? 8 : 4;
+    const size_t rn = p->precision == CLB_FP32 ? 4 : 8;
     int sm = 0, si = 0;
     for (int s = 0; s < p->n_species && s < CLB_MAX_SPECIES; ++s) (p->mobile[s] ? sm : si)++;
     size_t b = 0;
@@ -1113,7 +1118,7 @@ size_t clb_case_footprint(const clb_case_params *p) {
     b += (size_t) std::max(p->n_microbes, 1) * N * r + N * r;
     if (p->n_microbes > 0) b += (size_t) p->n_microbes * 6 * N * r + N;
     b += 2 * N * sizeof(int);
-    if (p->flow_on) b += 2 * 19 * N * r;
+    if (p->flow_on) b += 2 * 19 * N * rn;
     b += 3 * N * r;
     b += N * sizeof(double);                            // staging
     b += ((N + 127) / 128) * (5 * sizeof(double) + sizeof(long long));
@@ -1128,6 +1133,7 @@ int clb_case_create(clb_ctx *ctx, const clb_case_params *p, const int8_t *geomet
     HIPT(hipSetDevice(ctx->device));
     std::unique_ptr<CaseBase> c;
     if (p->precision == CLB_FP32) { auto *k = new Case<float>(); c.reset(k); k->create(*p, geometry, biomass0, ctx->stream); }
+    else if (p->precision == CLB_FP32_ADE) { auto *k = new Case<float, double>(); c.reset(k); k->create(*p, geometry, biomass0, ctx->stream); }
     else { auto *k = new Case<double>(); c.reset(k); k->create(*p, geometry, biomass0, ctx->stream); }
     int id = ctx->next++;
     ctx->cases[id] = std::move(c);

@@ -842,8 +842,8 @@ __global__ void __launch_bounds__(64) k_ns_tracer(NsTracer *t, const double *par
 
 // velocity of the current flow state (computeVelocity) into the compact field
 // read by the ADE lattices; optionally the box reductions of the Peclet rescale
-template <typename R>
-__global__ void __launch_bounds__(256) k_ns_velocity(NsDev<R> a, R *u, double *parts) {
+template <typename R, typename RA>
+__global__ void __launch_bounds__(256) k_ns_velocity(NsDev<R> a, RA *u, double *parts) {
     __shared__ double red[4][4], r4[4];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[5] = {0, 0, 0, 0, 0};      // sum|u| box, max ux box, sum ux outlet, sum|u| open box, count open box
@@ -860,7 +860,7 @@ __global__ void __launch_bounds__(256) k_ns_velocity(NsDev<R> a, R *u, double *p
                 for (int q = 0; q < 19; ++q) { ux += D19C[q][0] * fl[q]; uy += D19C[q][1] * fl[q]; uz += D19C[q][2] * fl[q]; }
             }
         }
-        if (u) { u[i] = (R) ux; u[a.Nopen + i] = (R) uy; u[2 * (long long) a.Nopen + i] = (R) uz; }
+        if (u) { u[i] = (RA) ux; u[a.Nopen + i] = (RA) uy; u[2 * (long long) a.Nopen + i] = (RA) uz; }
         inBox = i < a.Nint;
         const double n = sqrt(ux * ux + uy * uy + uz * uz);
         if (inBox) {
@@ -1067,9 +1067,9 @@ hipError_t launchNsStep(const NsDev<R> &a, double *partE, double *partN, NsTrace
     hipLaunchKernelGGL(k_ns_tracer, dim3(1), dim3(64), 0, st, t, partE, partN, (int) (gB + gP), rule, eps, a.done);
     return hipGetLastError();
 }
-template <typename R>
-hipError_t launchNsVelocity(const NsDev<R> &a, R *u, double *parts, hipStream_t st) {
-    hipLaunchKernelGGL(k_ns_velocity<R>, dim3(nblk(a.Nopen, 256)), dim3(256), 0, st, a, u, parts);
+template <typename R, typename RA>
+hipError_t launchNsVelocity(const NsDev<R> &a, RA *u, double *parts, hipStream_t st) {
+    hipLaunchKernelGGL((k_ns_velocity<R, RA>), dim3(nblk(a.Nopen, 256)), dim3(256), 0, st, a, u, parts);
     return hipGetLastError();
 }
 template <typename R>
@@ -1112,7 +1112,7 @@ hipError_t launchGatherCompact(const double *src, const int *list, int n, R *dst
     template hipError_t launchAdeReopen<R>(const AdeDev<R> &, const uint8_t *, hipStream_t); \
     template hipError_t launchAdeTotal<R>(const AdeDev<R> &, int, double *, hipStream_t); \
     template hipError_t launchNsStep<R>(const NsDev<R> &, double *, double *, NsTracer *, int, double, hipStream_t); \
-    template hipError_t launchNsVelocity<R>(const NsDev<R> &, R *, double *, hipStream_t); \
+    template hipError_t launchNsVelocity<R, R>(const NsDev<R> &, R *, double *, hipStream_t); \
     template hipError_t launchNsInit<R>(const NsDev<R> &, double, hipStream_t); \
     template hipError_t launchNsReopen<R>(const NsDev<R> &, const uint8_t *, hipStream_t); \
     template hipError_t launchEquilibrium<R>(const AdeDev<R> &, const EqDev &, unsigned int *, hipStream_t); \
@@ -1120,5 +1120,7 @@ hipError_t launchGatherCompact(const double *src, const int *list, int n, R *dst
     template hipError_t launchGatherCompact<R>(const double *, const int *, int, R *, hipStream_t);
 INST(double)
 INST(float)
+// FP32 transport lattices fed by an FP64 flow solve
+template hipError_t launchNsVelocity<double, float>(const NsDev<double> &, float *, double *, hipStream_t);
 
 }  // namespace clb

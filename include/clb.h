@@ -55,8 +55,9 @@ extern "C" {
 #define CLB_BIOFILM 3   /* first CA microbe's material number (3 + m)         */
 
 /* precisions */
-#define CLB_FP64 0
-#define CLB_FP32 1
+#define CLB_FP64     0
+#define CLB_FP32     1   /* FP32 storage and collide arithmetic, flow included       */
+#define CLB_FP32_ADE 2   /* FP32 transport lattices, FP64 flow lattice               */
 
 /* boundary types of the x planes, complab_functions.hh:382-392 */
 #define CLB_BC_DIRICHLET 0
@@ -82,7 +83,7 @@ extern "C" {
 typedef struct clb_case_params {
     /* lattice, including the two ghost x columns (complab_functions.hh:999) */
     int32_t nx, ny, nz;
-    int32_t precision;                 /* CLB_FP64 / CLB_FP32                 */
+    int32_t precision;                 /* CLB_FP64 / CLB_FP32 / CLB_FP32_ADE  */
 
     /* ---- species (substrates) -------------------------------------------- */
     int32_t n_species;
