@@ -230,3 +230,75 @@ def serving_case(seed, precision=P.FP32, size=(48, 256), max_steps=20000, stabil
     ts = TransportSetup(d_pore=[0.1] * ns, d_biofilm=[0.1] * ns, stabilize_steps=stabilize)
     return Workload("serve-%d %dx%dx%d spheres phi%.2f %d species" % (seed, nxf, ny, nz, phi_got, ns),
                     p, lat, phi_got, fs, ts, max_steps)
+
+
+H_, CO3_, CA_, HCO3_, CACO3AQ_, CALCITE_ = range(6)
+
+
+def config3(scale=1.0, precision=P.FP64, seed=3, steps=5000):
+    """Precision form: 512x256x256 seeded random grain field at porosity 0.3;
+    the carbonate-calcium system as six species -- H+, CO3--, Ca++ (the
+    equilibrium components), HCO3- and CaCO3(aq) (complexes, log K 10.33 and
+    3.22 drawn within +-2 from the seed), all five mobile D3Q7 lattices, and an
+    immobile calcite inventory.  Ca + HCO3 precipitates calcite on the grain
+    surface releasing H (sealing a voxel at 0.05 mol/L), acid dissolves it back
+    (CaCO3 + H -> Ca + HCO3, reopening below 0.9 of full); speciation by
+    per-voxel FP64 Newton to 1e-12.  Acidic inflow at x = 0."""
+    rng = np.random.default_rng(seed)
+    nxf, ny, nz = max(12, int(512 * scale)), max(8, int(256 * scale)), max(8, int(256 * scale))
+    lat, phi = _pack(nxf, ny, nz, 3.0, 0.3, seed, kind=P.GEOM_RANDOM)
+    p = P.default_params(nxf + 2, ny, nz, 6)
+    p.precision = precision
+    p.mobile[CALCITE_] = 0
+    c0 = {H_: 1e-7, CO3_: 1e-5, CA_: 1e-3, HCO3_: 2.1e-2, CACO3AQ_: 1.7e-5, CALCITE_: 0.0}
+    inlet = {H_: 1e-5, CO3_: 1e-7, CA_: 1e-3, HCO3_: 2.1e-2, CACO3AQ_: 1.7e-7}
+    for s in range(6):
+        p.c0[s] = c0[s]
+        if s != CALCITE_:
+            p.left_btype[s] = P.BC_DIRICHLET; p.left_bc[s] = inlet[s]
+            p.right_btype[s] = P.BC_NEUMANN
+    # equilibrium: components H, CO3, Ca; complexes HCO3 = H + CO3, CaCO3(aq) = Ca + CO3
+    p.eq_on = 1
+    p.eq_solver = P.EQ_NEWTON
+    p.eq_nc = 3
+    rows = {H_: (1, 0, 0), CO3_: (0, 1, 0), CA_: (0, 0, 1), HCO3_: (1, 1, 0), CACO3AQ_: (0, 1, 1)}
+    logk = {H_: 0.0, CO3_: 0.0, CA_: 0.0,
+            HCO3_: 10.33 + float(rng.uniform(-2, 2)), CACO3AQ_: 3.22 + float(rng.uniform(-2, 2))}
+    for s, row in rows.items():
+        for j in range(3):
+            p.eq_stoich[s][j] = row[j]
+        p.eq_logk[s] = logk[s]
+    p.eq_comp_species[0] = H_; p.eq_comp_species[1] = CO3_; p.eq_comp_species[2] = CA_
+    p.eq_tol = 1e-12
+    p.eq_max_iter = 200
+    # Ca + HCO3 -> calcite + H on the grain surface
+    p.n_rxn = 1
+    p.rxn_r1[0] = CA_; p.rxn_r2[0] = HCO3_; p.rxn_k[0] = 5.0
+    p.rxn_stoich[0][CA_] = -1.0; p.rxn_stoich[0][HCO3_] = -1.0; p.rxn_stoich[0][CALCITE_] = 1.0
+    p.rxn_stoich[0][H_] = 1.0
+    p.surface_only = 1
+    p.precip_on = 1; p.precip_species = CALCITE_; p.precip_max_rho = 0.05
+    p.precip_interval = 100; p.precip_phase_id = 1; p.precip_perm_ratio = 0.0
+    # CaCO3 + H -> Ca + HCO3 on the wetted faces of the sealed voxels
+    p.dissol_on = 1
+    p.dissol_species = CALCITE_
+    p.dissol_material = -1
+    p.dissol_is_precipitate = 1
+    p.dissol_phase_id = 1
+    p.dissol_full = 0.05
+    p.dissol_reopen_frac = 0.9
+    p.dissol_surface_only = 1
+    p.dissol_interval = 100
+    p.dissol_k = 20.0
+    p.dissol_acid = H_
+    p.dissol_stoich[H_] = -1.0; p.dissol_stoich[CA_] = 1.0; p.dissol_stoich[HCO3_] = 1.0
+    p.ns_tau = 0.8
+    p.flow_on = 1
+    p.ns_conv_rule = P.CONV_RELDELTA
+    p.ns_conv_eps = 1e-8
+    p.ns_max_iter = 100000
+    fs = FlowSetup(pe=1.0, char_length=float(nxf), delta_p=1e-5 * (nxf + 1), ns_tau=0.8,
+                   rule=P.CONV_RELDELTA, eps=1e-8, max_iter=100000)
+    ts = TransportSetup(d_pore=[0.1] * 6, d_biofilm=[0.1] * 6, stabilize_steps=0)
+    return Workload("precision-%s %dx%dx%d random phi%.2f carbonate-calcium eq + calcite precip/dissol"
+                    % ({0: "fp64", 1: "fp32", 2: "fp32_ade"}[precision], nxf, ny, nz, phi), p, lat, phi, fs, ts, steps)

@@ -1,0 +1,86 @@
+"""The precision form: one seeded case run in FP64 and in a reduced precision,
+compared field by field.
+
+    python -m complab_amd.study --scale 0.25 --steps 5000
+
+Reports, per species, the largest relative difference of the final field over
+the open voxels of the reaction box (scaled by the species' largest FP64
+magnitude), the final mask difference (voxels whose material differs), and the
+drift of each species' total -- |total_low - total_fp64| / |total_fp64| at the
+end, scaled to 1e4 steps.  The equilibrium solve, the reductions and the Peclet
+rescale are FP64 in every precision.
+"""
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import Case, Context
+from . import configs, driver
+from . import params as P
+
+
+def run(ctx, w, steps, chunk=500):
+    c = Case(ctx, w.params, w.geometry)
+    t0 = time.perf_counter()
+    rep = driver.setup(c, w.params, w.flow, w.transport)
+    c.set_omegas(rep.omega_pore, rep.omega_biofilm, 1.0)
+    t1 = time.perf_counter()
+    done = 0
+    totals = []
+    while done < steps:
+        rc, n = c.run_steps(min(chunk, steps - done))
+        done += n
+        totals.append((done, c.species_totals()))
+        if rc:
+            raise RuntimeError("%s stopped with status %d after %d steps" % (w.name, rc, done))
+    ctx.sync()
+    t2 = time.perf_counter()
+    out = {"fields": [c.read(P.FIELD_CONC, s) for s in range(w.params.n_species)],
+           "mask": c.read(P.FIELD_MASK), "totals": totals, "counters": c.counters(),
+           "setup_s": t1 - t0, "steps_s": t2 - t1, "flow_iters": list(rep.iters)}
+    c.close()
+    return out
+
+
+def compare(lo, hi, steps):
+    mask = hi["mask"]
+    om = mask >= P.PORE
+    om[:, :, 0] = False
+    om[:, :, -1] = False
+    per = []
+    for s, (a, b) in enumerate(zip(lo["fields"], hi["fields"])):
+        sel = om if s < len(lo["fields"]) else om
+        scale = max(float(np.max(np.abs(b[sel]))), 1e-300)
+        per.append(float(np.max(np.abs(a[sel] - b[sel]))) / scale)
+    tl, th = lo["totals"][-1][1], hi["totals"][-1][1]
+    drift = [float(abs(x - y) / max(abs(y), 1e-300)) * 1e4 / steps for x, y in zip(tl, th)]
+    return {"field_max_rel_err": [float("%.3g" % e) for e in per],
+            "mask_voxels_differ": int(np.sum(lo["mask"] != hi["mask"])),
+            "total_drift_per_1e4_steps": [float("%.3g" % d) for d in drift]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scale", type=float, default=0.25)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--seed", type=int, default=3)
+    args = ap.parse_args()
+    ctx = Context(0)
+    res = {"steps": args.steps, "scale": args.scale}
+    hi = run(ctx, configs.config3(args.scale, P.FP64, args.seed), args.steps)
+    res["workload"] = configs.config3(args.scale, P.FP64, args.seed).name
+    res["fp64"] = {"setup_s": round(hi["setup_s"], 2), "steps_s": round(hi["steps_s"], 2),
+                   "flow_iters": hi["flow_iters"], "counters": [int(x) for x in hi["counters"]]}
+    for label, prec in (("fp32_ade", P.FP32_ADE), ("fp32", P.FP32)):
+        lo = run(ctx, configs.config3(args.scale, prec, args.seed), args.steps)
+        r = compare(lo, hi, args.steps)
+        r.update({"setup_s": round(lo["setup_s"], 2), "steps_s": round(lo["steps_s"], 2),
+                  "counters": [int(x) for x in lo["counters"]]})
+        res[label] = r
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -601,8 +601,16 @@ __global__ void k_ade_set(AdeDev<R> a, int ms, const double *src) {
     for (int q = 0; q < 7; ++q) cur[q * (long long) a.Nopen + i] = (R) (w7(q) * c);
 }
 
-// a voxel reopened this step: rest population, and the rest value on the
-// links from neighbours that were closed at the last collide
+// a voxel reopened this step: rest population, and the rest value on every
+// link to a neighbour that was closed at the last collide, in both directions:
+// what arrives from it at the next launch (read from `out`) and what the
+// closed neighbour returns at the launch after (read from `cur`, the slot the
+// voxel did not write while it was closed); the grid edge alike.  On a link to
+// a neighbour that was open at the last collide, the `cur` slot takes that
+// neighbour's population toward the voxel from the launch before: should the
+// neighbour close in this same step, it returns to the voxel what the voxel
+// reflected to it while closed, which is that population
+
 template <typename R>
 __global__ void k_ade_reopen(AdeDev<R> a, const uint8_t *reopened) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -616,7 +624,12 @@ __global__ void k_ade_reopen(AdeDev<R> a, const uint8_t *reopened) {
         for (int q = 1; q < 7; ++q) {
             const int n = a.nb[(q - 1) * No + i];
             const bool closedSrc = n < 0 || closedFor(a.clsPull, n, a.Nint, a.Nleft, a.lbt[s], a.rbt[s]);
-            if (closedSrc) old[opp7(q) * No + i] = (R) (W7F * a.c0[s]);
+            if (closedSrc) {
+                old[opp7(q) * No + i] = (R) (W7F * a.c0[s]);
+                cur[opp7(q) * No + i] = (R) (W7F * a.c0[s]);
+            } else {
+                cur[opp7(q) * No + i] = old[q * No + n];
+            }
         }
     }
 }
@@ -918,8 +931,9 @@ __global__ void k_ns_update_dyn(const int8_t *mask, const int *list, uint8_t *cl
     else if (m == 2 && !isNsOmega) nc = NS_BGK;
     if (nc != c) { cls[i] = (uint8_t) nc; atomicAdd(changed, 1u); }
 }
-// a cell reopened in the flow lattice: rest (0) on the links from neighbours
-// that were closed at the last collide
+// a cell reopened in the flow lattice: rest (0) on every link to a neighbour
+// that was closed at the last collide, in both directions (as for the
+// advection-diffusion lattices)
 template <typename R>
 __global__ void k_ns_reopen(NsDev<R> a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -929,7 +943,8 @@ __global__ void k_ns_reopen(NsDev<R> a) {
     a.cur[i] = (R) 0;
     for (int q = 1; q < 19; ++q) {
         const int n = a.nb[(q - 1) * No + i];
-        if (n < 0 || a.clsPull[n] >= NS_BB) a.out[opp19(q) * No + i] = (R) 0;
+        if (n < 0 || a.clsPull[n] >= NS_BB) { a.out[opp19(q) * No + i] = (R) 0; a.cur[opp19(q) * No + i] = (R) 0; }
+        else a.cur[opp19(q) * No + i] = a.out[q * No + n];
     }
 }
 

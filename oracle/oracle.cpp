@@ -1021,7 +1021,9 @@ void nsUpdateDynamics(Case &c, double permRatio) {
             int xs = x - C19[q][0], ys = y - C19[q][1], zs = z - C19[q][2];
             bool fromOpen = false;
             if (c.inside(xs, ys, zs)) { int cl = old[c.id(xs, ys, zs)]; fromOpen = (cl == NS_BGK || cl == NS_BIO); }
-            if (!fromOpen) c.g[(size_t) q * c.N + i] = 0.0;
+            if (fromOpen) continue;
+            c.g[(size_t) q * c.N + i] = 0.0;                       // both directions, as rebuildReopened
+            if (c.inside(xs, ys, zs)) c.g[(size_t) opp19(q) * c.N + c.id(xs, ys, zs)] = 0.0;
         }
     }
 }
@@ -1079,8 +1081,15 @@ void soluteSolidSwitch(Case &c) {          // updateSoluteSolidDynamics3D, part2
     }
 }
 /* a voxel reopened during this step keeps, after the stream, the populations
- * its open neighbours sent; those from closed neighbours are set to the
- * species' rest value w_i c0, which closed-closed links hold from set-up */
+ * its open neighbours sent; on every link to a neighbour closed at the
+ * collide the rest value w_i c0 (which closed-closed links hold from set-up)
+ * is set in both directions: the population arriving now, and the one the
+ * closed neighbour returns at the next stream (the neighbour's stored
+ * population that came from this voxel while it was closed).  A neighbour
+ * open at the collide that closes in this same step (sealed by precipitation)
+ * keeps its population toward this voxel, which it returns at the next
+ * stream.  Parity unpinned: Palabos would return the closed neighbour's
+ * stored population itself. */
 void rebuildReopened(Case &c, int s) {
     for (size_t i : c.reopened) {
         int x = (int) (i % c.nx), y = (int) ((i / c.nx) % c.ny), z = (int) (i / ((size_t) c.nx * c.ny));
@@ -1089,7 +1098,9 @@ void rebuildReopened(Case &c, int s) {
             int xs = x - C7[q][0], ys = y - C7[q][1], zs = z - C7[q][2];
             bool fromOpen = c.inside(xs, ys, zs) && c.adeClsCollide[c.id(xs, ys, zs)] != ADE_BB
                             && !(xs == 0 && c.p.left_btype[s] == CLB_BC_CLOSED) && !(xs == c.nx - 1 && c.p.right_btype[s] == CLB_BC_CLOSED);
-            if (!fromOpen) c.f[s][(size_t) q * c.N + i] = W7[q] * c.p.c0[s];
+            if (fromOpen) continue;
+            c.f[s][(size_t) q * c.N + i] = W7[q] * c.p.c0[s];
+            if (c.inside(xs, ys, zs)) c.f[s][(size_t) OPP7[q] * c.N + c.id(xs, ys, zs)] = W7[q] * c.p.c0[s];
         }
     }
 }
