@@ -70,6 +70,7 @@ def lib():
         "clb_read_field": ([V, I, I, I, V], I),
         "clb_write_field": ([V, I, I, I, V], I),
         "clb_species_totals": ([V, I, V], I),
+        "clb_field_stats": ([V, I, V], I),
         "clb_counters": ([V, I, V], I),
         "clb_kernel_time": ([V, I, I, C.POINTER(D), C.POINTER(C.c_int64)], I),
         "clb_stream_copy": ([V, C.c_size_t, I, C.POINTER(D), C.POINTER(D)], I),
@@ -95,7 +96,7 @@ def exported_symbols():
         "clb_ade_stabilize", "clb_run_steps", "clb_ade_steps", "clb_equilibrium", "clb_ca_spill",
         "clb_ade_collide_stream", "clb_kinetics", "clb_mask_total", "clb_ca_sweep", "clb_age_update",
         "clb_update_solute_dynamics", "clb_update_solute_solid", "clb_precip_convert", "clb_dissol_convert",
-        "clb_read_field", "clb_write_field", "clb_species_totals", "clb_counters", "clb_kernel_time",
+        "clb_read_field", "clb_write_field", "clb_species_totals", "clb_field_stats", "clb_counters", "clb_kernel_time",
         "clb_stream_copy",
         "clb_rng_draws", "clb_geom_spheres", "clb_geom_pack", "clb_geom_generate",
     ]
@@ -275,6 +276,12 @@ class Case:
         out = np.zeros(self.p.n_species)
         self._c(lib().clb_species_totals(self.ctx.h, self.id, _p(out)))
         return out
+
+    def field_stats(self):
+        """Rows (species, then microbes) of total, minimum, maximum, held."""
+        out = np.zeros(4 * (self.p.n_species + self.p.n_microbes))
+        self._c(lib().clb_field_stats(self.ctx.h, self.id, _p(out)))
+        return out.reshape(-1, 4)
 
     def counters(self):
         out = np.zeros(12, dtype=np.int64)

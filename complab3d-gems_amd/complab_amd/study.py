@@ -12,6 +12,7 @@ rescale are FP64 in every precision.
 """
 import argparse
 import json
+import sys
 import time
 
 import numpy as np
@@ -70,6 +71,7 @@ def main():
     ctx = Context(0)
     res = {"steps": args.steps, "scale": args.scale}
     hi = run(ctx, configs.config3(args.scale, P.FP64, args.seed), args.steps)
+    print("fp64 done: setup %.1f s, steps %.1f s" % (hi["setup_s"], hi["steps_s"]), file=sys.stderr, flush=True)
     res["workload"] = configs.config3(args.scale, P.FP64, args.seed).name
     res["fp64"] = {"setup_s": round(hi["setup_s"], 2), "steps_s": round(hi["steps_s"], 2),
                    "flow_iters": hi["flow_iters"], "counters": [int(x) for x in hi["counters"]]}
@@ -79,6 +81,7 @@ def main():
         r.update({"setup_s": round(lo["setup_s"], 2), "steps_s": round(lo["steps_s"], 2),
                   "counters": [int(x) for x in lo["counters"]]})
         res[label] = r
+        print("%s done: %s" % (label, json.dumps(r)), file=sys.stderr, flush=True)
     print(json.dumps(res), flush=True)
 
 

@@ -35,6 +35,7 @@ from . import params as P
 class CaseResult:
     name: str
     seconds: float = 0.0            # set-up + steps, on its worker
+    setup_seconds: float = 0.0      # of which the set-up (flow solves, stabilization)
     latency: float = 0.0            # serving: arrival to completion
     steps: int = 0
     flow_iters: List[int] = field(default_factory=list)
@@ -56,6 +57,8 @@ def run_workload(ctx, w, steady_tol=None, check_every=100, keep_fields=True):
         rep = driver.setup(c, w.params, w.flow, w.transport)
         r.flow_iters = list(rep.iters)
         c.set_omegas(rep.omega_pore, rep.omega_biofilm, 1.0)
+        ctx.sync()
+        r.setup_seconds = time.perf_counter() - t0
         if steady_tol is None:
             rc, n = c.run_steps(w.steps)
             r.steps, r.status = n, rc

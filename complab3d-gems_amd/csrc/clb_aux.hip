@@ -305,23 +305,36 @@ __global__ void k_precip(int nx, int ny, long long N, int8_t *mask, const R *P, 
     }
 }
 // total of an immobile species over the open voxels of x = 1..nx-2 (per-block partials)
+// per block, for a dense scalar field (an immobile species, a biomass): the
+// sum over every voxel of the reaction box x = 1..nx-2 (a mineral keeps its
+// inventory in the voxels it sealed), minus the minimum and the maximum there,
+// and the sum over the x = 0 and x = nx-1 planes; parts[4 * b + 0..3]
 template <typename R>
-__global__ void __launch_bounds__(256) k_scalar_total(int nx, int ny, long long N, const int8_t *mask, const R *f,
-                                                      double *parts) {
-    __shared__ double sS[4];
+__global__ void __launch_bounds__(256) k_scalar_stats(int nx, int ny, long long N, const R *f, double *parts) {
+    __shared__ double sv[4][4];
     const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
-    double v = 0;
+    double t = 0, lo = -1e300, hi = -1e300, pl = 0;
     if (i < N) {
         int x, y, z;
         coords(i, nx, ny, x, y, z);
-        const int m = mask[i];
-        if (x > 0 && x < nx - 1 && m != 0 && m != 1) v = (double) f[i];
+        const double v = (double) f[i];
+        if (x == 0 || x == nx - 1) pl = v;
+        else { t = v; lo = -v; hi = v; }
     }
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    for (int off = 32; off > 0; off >>= 1) {
+        t += __shfl_down(t, off, 64);
+        pl += __shfl_down(pl, off, 64);
+        lo = fmax(lo, __shfl_down(lo, off, 64));
+        hi = fmax(hi, __shfl_down(hi, off, 64));
+    }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) sS[wv] = v;
+    if (lane == 0) { sv[wv][0] = t; sv[wv][1] = lo; sv[wv][2] = hi; sv[wv][3] = pl; }
     __syncthreads();
-    if (threadIdx.x == 0) { double s2 = 0; for (int w = 0; w < (int) (blockDim.x >> 6); ++w) s2 += sS[w]; parts[blockIdx.x] = s2; }
+    if (threadIdx.x == 0) {
+        double T = 0, L = -1e300, U = -1e300, P = 0;
+        for (int w = 0; w < (int) (blockDim.x >> 6); ++w) { T += sv[w][0]; L = fmax(L, sv[w][1]); U = fmax(U, sv[w][2]); P += sv[w][3]; }
+        parts[4 * blockIdx.x] = T; parts[4 * blockIdx.x + 1] = L; parts[4 * blockIdx.x + 2] = U; parts[4 * blockIdx.x + 3] = P;
+    }
 }
 
 // fixed-order final reduction of per-block partials (one block of 256 threads,
@@ -437,8 +450,8 @@ hipError_t launchPrecip(int nx, int ny, long long N, int8_t *mask, const R *P, d
     return hipGetLastError();
 }
 template <typename R>
-hipError_t launchScalarTotal(int nx, int ny, long long N, const int8_t *mask, const R *f, double *parts, hipStream_t st) {
-    hipLaunchKernelGGL(k_scalar_total<R>, dim3(nblk(N, 256)), dim3(256), 0, st, nx, ny, N, mask, f, parts);
+hipError_t launchScalarStats(int nx, int ny, long long N, const R *f, double *parts, hipStream_t st) {
+    hipLaunchKernelGGL(k_scalar_stats<R>, dim3(nblk(N, 256)), dim3(256), 0, st, nx, ny, N, f, parts);
     return hipGetLastError();
 }
 hipError_t launchFinalSum(const double *parts, long long n, int stride, int comp, int isMax, double *out, hipStream_t st) {
@@ -473,7 +486,7 @@ hipError_t launchI32ToDouble(const int *src, double *dst, long long n, hipStream
     template hipError_t launchCaPushGather<R>(const CaDev<R> &, uint64_t, hipStream_t); \
     template hipError_t launchCaAge<R>(const CaDev<R> &, const int *, int *, long long *, hipStream_t); \
     template hipError_t launchPrecip<R>(int, int, long long, int8_t *, const R *, double, int, int8_t *, unsigned int *, hipStream_t); \
-    template hipError_t launchScalarTotal<R>(int, int, long long, const int8_t *, const R *, double *, hipStream_t); \
+    template hipError_t launchScalarStats<R>(int, int, long long, const R *, double *, hipStream_t); \
     template hipError_t launchToDouble<R>(const R *, double *, long long, hipStream_t); \
     template hipError_t launchFromDouble<R>(const double *, R *, long long, hipStream_t);
 INST(double)

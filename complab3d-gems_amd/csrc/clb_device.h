@@ -161,6 +161,7 @@ template <typename R> hipError_t launchAdeDensity(const AdeDev<R> &, int, double
 template <typename R> hipError_t launchAdeSet(const AdeDev<R> &, int, const double *, hipStream_t);
 template <typename R> hipError_t launchAdeReopen(const AdeDev<R> &, const uint8_t *, hipStream_t);
 template <typename R> hipError_t launchAdeTotal(const AdeDev<R> &, int, double *, hipStream_t);
+template <typename R> hipError_t launchAdeHeld(const AdeDev<R> &, int, double *, hipStream_t);
 template <typename R> hipError_t launchNsStep(const NsDev<R> &, double *, double *, NsTracer *, int, double, hipStream_t);
 template <typename R, typename RA> hipError_t launchNsVelocity(const NsDev<R> &, RA *, double *, hipStream_t);
 template <typename R> hipError_t launchNsInit(const NsDev<R> &, double, hipStream_t);
@@ -178,7 +179,7 @@ template <typename R> hipError_t launchCaAge(const CaDev<R> &, const int *, int 
 hipError_t launchOmegaSwitch(const int8_t *, const int *, uint8_t *, int, hipStream_t);
 hipError_t launchSolidSwitch(const int8_t *, const int *, uint8_t *, uint8_t *, int, unsigned int *, hipStream_t);
 template <typename R> hipError_t launchPrecip(int, int, long long, int8_t *, const R *, double, int, int8_t *, unsigned int *, hipStream_t);
-template <typename R> hipError_t launchScalarTotal(int, int, long long, const int8_t *, const R *, double *, hipStream_t);
+template <typename R> hipError_t launchScalarStats(int, int, long long, const R *, double *, hipStream_t);
 hipError_t launchFinalSum(const double *, long long, int, int, int, double *, hipStream_t);
 hipError_t launchFinalSumLL(const long long *, long long, long long *, hipStream_t);
 template <typename R> hipError_t launchToDouble(const R *, double *, long long, hipStream_t);

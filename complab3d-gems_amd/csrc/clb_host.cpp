@@ -57,6 +57,7 @@ struct CaseBase {
     virtual Status readField(hipStream_t, int field, int index, double *dst) = 0;
     virtual Status writeField(hipStream_t, int field, int index, const double *src) = 0;
     virtual void speciesTotals(hipStream_t, double *out) = 0;
+    virtual void fieldStats(hipStream_t, double *out) = 0;
     virtual void counters(hipStream_t, int64_t out[CLB_NCOUNTERS]) = 0;
     virtual void kernelTime(hipStream_t, int reset, double *ms, int64_t *n) = 0;
     virtual void setRelaxation(const double *omp, const double *omb, double dt) = 0;
@@ -935,12 +936,40 @@ struct Case : CaseBase {
                 HIPT(launchAdeTotal<R>(ade(), slot[s], parts, st));
                 HIPT(launchFinalSum(parts, nblk(Nint, 256), 1, 0, 0, scal + s, st));
             } else {
-                HIPT(launchScalarTotal<R>(nx, ny, N, mask, imm + (size_t) slot[s] * N, parts, st));
-                HIPT(launchFinalSum(parts, nblk(N, 256), 1, 0, 0, scal + s, st));
+                HIPT(launchScalarStats<R>(nx, ny, N, imm + (size_t) slot[s] * N, parts, st));
+                HIPT(launchFinalSum(parts, nblk(N, 256), 4, 0, 0, scal + s, st));
             }
         }
         HIPT(hipMemcpyAsync(out, scal, p.n_species * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPT(hipStreamSynchronize(st));
+    }
+    // per species, then per microbe: total, minimum, maximum, held
+    void fieldStats(hipStream_t st, double *out) override {
+        const int S = p.n_species;
+        for (int r = 0; r < S + p.n_microbes; ++r) {
+            double v[4] = {0, 0, 0, 0};
+            if (r < S && p.mobile[r]) {
+                HIPT(launchAdeTotal<R>(ade(), slot[r], parts, st));
+                HIPT(launchFinalSum(parts, nblk(Nint, 256), 1, 0, 0, scal, st));
+                const long long g = nblk(Nopen, 256);
+                HIPT(launchAdeHeld<R>(ade(), slot[r], parts, st));
+                HIPT(launchFinalSum(parts, g, 3, 1, 1, scal + 1, st));
+                HIPT(launchFinalSum(parts, g, 3, 2, 1, scal + 2, st));
+                HIPT(launchFinalSum(parts, g, 3, 0, 0, scal + 3, st));
+            } else {
+                const R *f = r < S ? imm + (size_t) slot[r] * N : B + (size_t) (r - S) * N;
+                const long long g = nblk(N, 256);
+                HIPT(launchScalarStats<R>(nx, ny, N, f, parts, st));
+                for (int k = 0; k < 4; ++k) HIPT(launchFinalSum(parts, g, 4, k, k == 1 || k == 2, scal + k, st));
+            }
+            HIPT(hipMemcpyAsync(v, scal, sizeof(v), hipMemcpyDeviceToHost, st));
+            HIPT(hipStreamSynchronize(st));
+            const bool none = v[2] < -1e299;                   // no voxel counted
+            out[4 * r + 0] = v[0];
+            out[4 * r + 1] = none ? 0.0 : -v[1];
+            out[4 * r + 2] = none ? 0.0 : v[2];
+            out[4 * r + 3] = (r < S && !p.mobile[r]) ? 0.0 : v[3];   // nothing of an immobile species streams
+        }
     }
     void counters(hipStream_t st, int64_t out[CLB_NCOUNTERS]) override {
         unsigned long long cc = 0, dc[3] = {0, 0, 0};
@@ -1342,6 +1371,14 @@ int clb_species_totals(clb_ctx *ctx, int case_id, double *out) {
     CASE_OR_FAIL(c)
     if (!out) return fail(ctx, CLB_E_ARG, "null output");
     c->speciesTotals(ctx->stream, out);
+    return CLB_OK;
+    GUARD_END
+}
+int clb_field_stats(clb_ctx *ctx, int case_id, double *out) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (!out) return fail(ctx, CLB_E_ARG, "null output");
+    c->fieldStats(ctx->stream, out);
     return CLB_OK;
     GUARD_END
 }

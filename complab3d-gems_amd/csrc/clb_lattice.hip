@@ -590,6 +590,61 @@ __global__ void k_ade_density(AdeDev<R> a, int ms, double *dst) {
     dst[v] = r;
 }
 
+// per block, for one mobile species: the mass the interior total does not
+// report -- the content of the open x-plane voxels and the populations in
+// flight into closed voxels (written at the last collide by a voxel open at
+// it, toward a voxel closed now: the bounce-back rule returns them two
+// launches later) -- and the smallest and largest concentration over the
+// voxels the total counts.  parts[3 * b + 0] held, + 1 minus the minimum,
+// + 2 the maximum.  Before the first launch a closed neighbour holds the rest
+// population w c0 toward the voxel instead.
+template <typename R>
+__global__ void __launch_bounds__(256) k_ade_held(AdeDev<R> a, int ms, double *parts) {
+    __shared__ double sH[4], sLo[4], sHi[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = a.mobSpecies[ms];
+    const long long No = a.Nopen;
+    const R *cur = a.cur + (long long) ms * 7 * No;
+    double h = 0, lo = -1e300, hi = -1e300;
+    if (i < a.Nopen) {
+        const int lbt = a.lbt[s], rbt = a.rbt[s];
+        if (!closedFor(a.cls, i, a.Nint, a.Nleft, lbt, rbt)) {
+            double v = 0;
+            if (a.reset) { v = a.c0[s]; if (v < 1e-12 && v > -1e-12) v = 0; }
+            else {
+                double fl[7];
+                pull7<R, double>(a, cur, a.out + (long long) ms * 7 * No, i, lbt, rbt, fl);
+                for (int q = 0; q < 7; ++q) v += fl[q];
+            }
+            if (i >= a.Nint) h += v;
+            else {
+                const int m = a.mask[a.list[i]];
+                if (m != 0 && m != 1) { lo = -v; hi = v; }
+            }
+        }
+        if (!closedFor(a.clsPull, i, a.Nint, a.Nleft, lbt, rbt)) {
+            for (int q = 1; q < 7; ++q) {
+                const int d = a.nb[(opp7(q) - 1) * No + i];          // the voxel at x + c_q
+                if (d == NB_CLOSED || (d >= 0 && closedFor(a.cls, d, a.Nint, a.Nleft, lbt, rbt)))
+                    h += a.first ? W7F * a.c0[s] : (double) cur[q * No + i];
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        h += __shfl_down(h, off, 64);
+        lo = fmax(lo, __shfl_down(lo, off, 64));
+        hi = fmax(hi, __shfl_down(hi, off, 64));
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { sH[wv] = h; sLo[wv] = lo; sHi[wv] = hi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double H = 0, L = -1e300, U = -1e300;
+        for (int w = 0; w < (int) (blockDim.x >> 6); ++w) { H += sH[w]; L = fmax(L, sLo[w]); U = fmax(U, sHi[w]); }
+        parts[3 * blockIdx.x] = H; parts[3 * blockIdx.x + 1] = L; parts[3 * blockIdx.x + 2] = U;
+    }
+}
+
 // populations at rest for a concentration field (closed voxels: the background c0)
 template <typename R>
 __global__ void k_ade_set(AdeDev<R> a, int ms, const double *src) {
@@ -1070,6 +1125,11 @@ hipError_t launchAdeReopen(const AdeDev<R> &a, const uint8_t *re, hipStream_t st
     return hipGetLastError();
 }
 template <typename R>
+hipError_t launchAdeHeld(const AdeDev<R> &a, int ms, double *parts, hipStream_t st) {
+    hipLaunchKernelGGL(k_ade_held<R>, dim3(nblk(a.Nopen, 256)), dim3(256), 0, st, a, ms, parts);
+    return hipGetLastError();
+}
+template <typename R>
 hipError_t launchAdeTotal(const AdeDev<R> &a, int ms, double *parts, hipStream_t st) {
     hipLaunchKernelGGL(k_ade_total<R>, dim3(nblk(a.Nint, 256)), dim3(256), 0, st, a, ms, parts);
     return hipGetLastError();
@@ -1126,6 +1186,7 @@ hipError_t launchGatherCompact(const double *src, const int *list, int n, R *dst
     template hipError_t launchAdeSet<R>(const AdeDev<R> &, int, const double *, hipStream_t); \
     template hipError_t launchAdeReopen<R>(const AdeDev<R> &, const uint8_t *, hipStream_t); \
     template hipError_t launchAdeTotal<R>(const AdeDev<R> &, int, double *, hipStream_t); \
+    template hipError_t launchAdeHeld<R>(const AdeDev<R> &, int, double *, hipStream_t); \
     template hipError_t launchNsStep<R>(const NsDev<R> &, double *, double *, NsTracer *, int, double, hipStream_t); \
     template hipError_t launchNsVelocity<R, R>(const NsDev<R> &, R *, double *, hipStream_t); \
     template hipError_t launchNsInit<R>(const NsDev<R> &, double, hipStream_t); \

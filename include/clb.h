@@ -286,9 +286,23 @@ int  clb_read_field(clb_ctx *ctx, int case_id, int field, int index, double *dst
 int  clb_write_field(clb_ctx *ctx, int case_id, int field, int index, const double *src);
 
 /* ---- reductions (PopulationSum3D, MaskedScalarCounts3D, computeSum) --------- */
-/* per species: sum of concentration over open voxels (mask not solid/wall)
- * of x = 1..nx-2 on the live mask; out has n_species entries */
+/* per species, over the reaction box x = 1..nx-2: a mobile species summed
+ * over the voxels open on the live mask (mask not solid/wall), an immobile one
+ * over every voxel (a mineral keeps its inventory in the voxels it sealed,
+ * complab3d_diagnostics.hh:255-261); out has n_species entries */
 int  clb_species_totals(clb_ctx *ctx, int case_id, double *out);
+/* diagnostics row (complab.cpp:2736-2785, complab3d_diagnostics.hh:57-69): for
+ * each species, then each microbe, four values -- total, minimum, maximum and
+ * held.  total is clb_species_totals' (a microbe: its biomass over every voxel
+ * of the reaction box); minimum and maximum are taken over the voxels the total
+ * counts.  held is the mass the total does not see, so that total + held is
+ * what the lattice holds (PopulationSum3D over the whole box, part3:505-536,
+ * minus the total).  A mobile species: the content of the x = 0 and x = nx-1
+ * voxels open to it plus the populations in flight into closed voxels; the rest
+ * content that closed voxels keep among themselves from set-up is not part of
+ * it.  An immobile species: 0 (nothing streams).  A microbe: its biomass in the
+ * x = 0 and x = nx-1 planes.  out has 4 * (n_species + n_microbes) entries. */
+int  clb_field_stats(clb_ctx *ctx, int case_id, double *out);
 /* counters: [0] open voxels, [1] biofilm voxels, [2] clamp binds,
  * [3] equilibrium non-converged voxels, [4] flow re-solves, [5] mask changes,
  * [6] CA sweeps, [7] steps run, [8] dissolution product shares parked,

@@ -1435,9 +1435,63 @@ int orc_species_totals(void *h, double *out) {
         double t = 0;
         for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 1; x < c.nx - 1; ++x) {
             size_t i = c.id(x, y, z);
-            if (!c.solidOrBB(c.mask[i])) t += conc(c, s, i);
+            if (!c.p.mobile[s] || !c.solidOrBB(c.mask[i])) t += conc(c, s, i);   // a mineral stays in the voxels it sealed
         }
         out[s] = t;
+    }
+    return 0;
+}
+/* diagnostics row (complab.cpp:2736-2785): per species then per microbe the
+ * total over the open voxels of x = 1..nx-2, the minimum and maximum there, and
+ * held -- what the lattice holds beyond the total: PopulationSum3D over the
+ * whole box (part3:505-536) counts the x = 0 and x = nx-1 cells and the
+ * populations a bounce-back cell holds.  Here held is the content of the
+ * plane cells open to the species plus, in every closed cell, the populations
+ * that arrived in the last stream from a cell open at its collide; the rest
+ * values closed cells keep among themselves from set-up are left out (they
+ * never meet an open cell).  Immobile species 0; microbes: biomass in the two
+ * planes. */
+int orc_field_stats(void *h, double *out) {
+    Case &c = *(Case *) h;
+    const int S = c.p.n_species;
+    for (int r = 0; r < S + c.p.n_microbes; ++r) {
+        double tot = 0, lo = 1e300, hi = -1e300, held = 0;
+        for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 0; x < c.nx; ++x) {
+            const size_t i = c.id(x, y, z);
+            const double v = r < S ? conc(c, r, i) : c.B[r - S][i];
+            if (x > 0 && x < c.nx - 1) {
+                const bool mob = r < S && c.p.mobile[r];
+                if (mob && (c.solidOrBB(c.mask[i]) || adeClassFor(c, r, x, i) == ADE_BB)) continue;
+                tot += v; lo = std::min(lo, v); hi = std::max(hi, v);
+            }
+        }
+        if (r >= S) {
+            for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) {
+                held += c.B[r - S][c.id(0, y, z)];
+                held += c.B[r - S][c.id(c.nx - 1, y, z)];
+            }
+        } else if (c.p.mobile[r]) {
+            const double *f = c.f[r].data();
+            for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 0; x < c.nx; ++x) {
+                const size_t i = c.id(x, y, z);
+                if (adeClassFor(c, r, x, i) != ADE_BB) {
+                    if (x == 0 || x == c.nx - 1) held += densityF(c, r, i);
+                    continue;
+                }
+                for (int q = 1; q < Q7; ++q) {
+                    const int xs = x - C7[q][0], ys = y - C7[q][1], zs = z - C7[q][2];
+                    if (!c.inside(xs, ys, zs)) continue;
+                    const size_t k = c.id(xs, ys, zs);
+                    bool srcOpen;
+                    if (xs == 0 && c.p.left_btype[r] == CLB_BC_CLOSED) srcOpen = false;
+                    else if (xs == c.nx - 1 && c.p.right_btype[r] == CLB_BC_CLOSED) srcOpen = false;
+                    else srcOpen = (c.adeClsCollide.empty() ? c.adeCls[k] : c.adeClsCollide[k]) != ADE_BB;
+                    if (srcOpen) held += f[(size_t) q * c.N + i];
+                }
+            }
+        }
+        if (hi < lo) { lo = 0; hi = 0; }
+        out[4 * r + 0] = tot; out[4 * r + 1] = lo; out[4 * r + 2] = hi; out[4 * r + 3] = held;
     }
     return 0;
 }

@@ -54,6 +54,7 @@ def lib():
             ("orc_read_field", [P, C.c_int, C.c_int, P]),
             ("orc_write_field", [P, C.c_int, C.c_int, P]),
             ("orc_species_totals", [P, P]),
+            ("orc_field_stats", [P, P]),
             ("orc_counters", [P, P]),
             ("orc_eq_solve", [P, P, P, P, P, P]),
             ("orc_eq_voxel", [P, P, P]),
@@ -184,6 +185,12 @@ class OracleCase:
         out = np.zeros(self.p.n_species)
         self._chk(lib().orc_species_totals(self.h, _ptr(out)))
         return out
+
+    def field_stats(self):
+        """Rows (species, then microbes) of total, minimum, maximum, held."""
+        out = np.zeros(4 * (self.p.n_species + self.p.n_microbes))
+        self._chk(lib().orc_field_stats(self.h, _ptr(out)))
+        return out.reshape(-1, 4)
 
     def counters(self):
         out = np.zeros(12, dtype=np.int64)

@@ -38,6 +38,10 @@ def test_precision_form_fp64_matches_oracle(ctx):
     assert np.array_equal(cg[8:12], co[8:12]) and cg[8] > 0
     assert np.array_equal(g.read(P.FIELD_MASK), o.read(P.FIELD_MASK))
     assert np.array_equal(g.read(P.FIELD_PHASE), o.read(P.FIELD_PHASE))
+    sg, so = g.field_stats(), o.field_stats()            # diagnostics row: total, min, max, held
+    for col in range(4):
+        scale = max(np.max(np.abs(so[:, col])), 1e-300)
+        assert np.max(np.abs(sg[:, col] - so[:, col])) <= 1e-10 * scale, (col, sg[:, col], so[:, col])
     om = o.read(P.FIELD_MASK) >= 2
     om[:, :, 0] = False; om[:, :, -1] = False
     for s in range(6):
