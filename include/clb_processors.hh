@@ -471,6 +471,45 @@ private:
     std::vector<double> sums;
 };
 
+/* complab3d_processors_part3.hh:505-536: everything a species' lattice holds
+ * (PopulationSum3D over the whole box); here the diagnostics' total + held of
+ * clb_field_stats, so getSum(s) - the reaction-box total is the held column of
+ * complab.cpp:2758-2764 */
+class PopulationSumFunctional3D {
+public:
+    explicit PopulationSumFunctional3D(int species_) : species(species_), sum(0.0) {}
+    int process(const Target &t) {
+        clb_case_params p;
+        int rc = detail::params(t, p);
+        if (rc) return rc;
+        if (species < 0 || species >= p.n_species) return CLB_E_ARG;
+        std::vector<double> st((size_t) 4 * (p.n_species + p.n_microbes));
+        rc = clb_field_stats(t.ctx, t.case_id, st.data());
+        if (!rc) sum = st[(size_t) 4 * species] + st[(size_t) 4 * species + 3];
+        return rc;
+    }
+    double getSum() const { return sum; }
+private:
+    int species;
+    double sum;
+};
+inline int PopulationSum3D(const Target &t, int species, double *sum) {
+    PopulationSumFunctional3D f(species);
+    const int rc = f.process(t);
+    if (!rc && sum) *sum = f.getSum();
+    return rc;
+}
+
+/* complab3d_processors_part3.hh:386-427 over the reaction box on the live
+ * mask: MaskedScalarCounts3D for the open materials (pore and biofilm) is
+ * clb_counters' [0], for the biofilm materials alone its [1] */
+inline int MaskedScalarCounts3D(const Target &t, bool biofilmOnly, int64_t *count) {
+    int64_t c[CLB_NCOUNTERS];
+    const int rc = clb_counters(t.ctx, t.case_id, c);
+    if (!rc && count) *count = biofilmOnly ? c[1] : c[0];
+    return rc;
+}
+
 /* Palabos' applyProcessingFunctional: the callee owns the new-ed processor */
 template <class Proc>
 int applyProcessingFunctional(Proc *proc, const Target &t) {

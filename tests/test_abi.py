@@ -140,6 +140,12 @@ int main() {
     bad += applyProcessingFunctional(new dissolutionGather3D<double, FakeDissolCfg>(66, 32, 32, 3, 0, 1, &dcfg), t) != CLB_E_ARG;
     bad += applyProcessingFunctional(new dissolNodeConversion3D<double, FakeDissolCfg>(3, 2, 0, 1, &dcfg), t) != CLB_E_ARG;
     bad += applyProcessingFunctional(new initPhaseLattice3D<double>(std::vector<plint>(1, 0), std::vector<plint>(1, 1)), t) != CLB_E_ARG;
+    PopulationSumFunctional3D held(0);
+    bad += applyProcessingFunctional(held, t) != CLB_E_ARG;
+    double psum = 0;
+    bad += PopulationSum3D(t, 0, &psum) != CLB_E_ARG;
+    int64_t nopen = 0;
+    bad += MaskedScalarCounts3D(t, false, &nopen) != CLB_E_ARG;
     bad += applyProcessingFunctional(new update_rxnLattices<double>(66, 2, 1, 0, 1), t) != CLB_OK;
     std::printf("%d\n", bad);
     return bad;
