@@ -53,12 +53,17 @@ struct AdeDev {
     const R *u;                    // [3][Nopen]
     int8_t *mask;                  // dense live mask
     const uint8_t *cls, *clsPull;  // [Nopen]
+    const uint16_t *kinds;         // [Nint] source kind of each moving population, 2 bits per direction
     double *rhoN;                  // [Smax][Nopen - Nint] Neumann plane densities
     int anyNeumann;
     int first, reset;
     double c0[CLB_DEV_MAXS], omP[CLB_DEV_MAXS], omB[CLB_DEV_MAXS];
     int lbt[CLB_DEV_MAXS], rbt[CLB_DEV_MAXS];
     double lbc[CLB_DEV_MAXS], rbc[CLB_DEV_MAXS];
+    // the same per mobile slot, so that an unrolled slot loop reads them from
+    // fixed argument offsets
+    int mlbt[CLB_DEV_MAXS], mrbt[CLB_DEV_MAXS];
+    double mc0[CLB_DEV_MAXS], momP[CLB_DEV_MAXS], momB[CLB_DEV_MAXS];
     unsigned long long *clampCount;
     const int *halt;               // StepCtl::halt of a queued batch, or null
 };
@@ -162,6 +167,7 @@ template <typename R> hipError_t launchAdeSet(const AdeDev<R> &, int, const doub
 template <typename R> hipError_t launchAdeReopen(const AdeDev<R> &, const uint8_t *, hipStream_t);
 template <typename R> hipError_t launchAdeTotal(const AdeDev<R> &, int, double *, hipStream_t);
 template <typename R> hipError_t launchAdeHeld(const AdeDev<R> &, int, double *, hipStream_t);
+hipError_t launchAdeKinds(const int *nb, const uint8_t *clsPull, int Nopen, int Nint, uint16_t *kinds, hipStream_t);
 template <typename R> hipError_t launchNsStep(const NsDev<R> &, double *, double *, NsTracer *, int, double, hipStream_t);
 template <typename R, typename RA> hipError_t launchNsVelocity(const NsDev<R> &, RA *, double *, hipStream_t);
 template <typename R> hipError_t launchNsInit(const NsDev<R> &, double, hipStream_t);

@@ -95,6 +95,7 @@ struct Case : CaseBase {
     int Nopen = 0, Nint = 0, Nleft = 0;
     int *list = nullptr, *nb7 = nullptr, *nb19 = nullptr;
     uint8_t *adeCls = nullptr, *adeClsPull = nullptr, *nsCls = nullptr, *nsClsPull = nullptr, *reopened = nullptr;
+    uint16_t *adeKinds = nullptr;
     R *fA = nullptr, *fB = nullptr, *imm = nullptr, *B = nullptr, *Btot = nullptr, *outbox = nullptr;
     uint8_t *pushed = nullptr;
     int *age = nullptr, *age2 = nullptr;
@@ -163,8 +164,13 @@ struct Case : CaseBase {
             a.lbt[s] = p.left_btype[s]; a.rbt[s] = p.right_btype[s];
             a.lbc[s] = p.left_bc[s]; a.rbc[s] = p.right_bc[s];
         }
+        for (int ms = 0; ms < Sm; ++ms) {
+            const int s = mobSpecies[ms];
+            a.mlbt[ms] = p.left_btype[s]; a.mrbt[ms] = p.right_btype[s];
+            a.mc0[ms] = p.c0[s]; a.momP[ms] = p.omega_pore[s]; a.momB[ms] = p.omega_biofilm[s];
+        }
         a.cur = fCur(); a.out = fOut(); a.imm = imm; a.B = B;
-        a.u = u; a.mask = mask; a.cls = adeCls; a.clsPull = adeClsPull;
+        a.u = u; a.mask = mask; a.cls = adeCls; a.clsPull = adeClsPull; a.kinds = adeKinds;
         a.rhoN = rhoN;
         a.first = adeFirst ? 1 : 0; a.reset = adeReset ? 1 : 0;
         a.clampCount = clampCount;
@@ -304,6 +310,7 @@ struct Case : CaseBase {
         list = dalloc<int>(allocs, Nopen);
         nb7 = dalloc<int>(allocs, (size_t) 6 * Nopen);
         adeCls = dalloc<uint8_t>(allocs, Nopen); adeClsPull = dalloc<uint8_t>(allocs, Nopen);
+        adeKinds = dalloc<uint16_t>(allocs, std::max(Nint, 1));
         nsCls = dalloc<uint8_t>(allocs, Nopen); nsClsPull = dalloc<uint8_t>(allocs, Nopen);
         reopened = dalloc<uint8_t>(allocs, Nopen);
         fA = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * 7 * Nopen);
@@ -378,6 +385,7 @@ struct Case : CaseBase {
         }
         HIPT(hipMemcpyAsync(adeCls, ac.data(), Nopen, hipMemcpyHostToDevice, st));
         HIPT(hipMemcpyAsync(adeClsPull, ac.data(), Nopen, hipMemcpyHostToDevice, st));
+        HIPT(launchAdeKinds(nb7, adeClsPull, Nopen, Nint, adeKinds, st));
         HIPT(hipMemcpyAsync(nsCls, nc.data(), Nopen, hipMemcpyHostToDevice, st));
         HIPT(hipMemcpyAsync(nsClsPull, nc.data(), Nopen, hipMemcpyHostToDevice, st));
         HIPT(hipMemcpyAsync(age, ag.data(), N * sizeof(int), hipMemcpyHostToDevice, st));
@@ -584,7 +592,11 @@ struct Case : CaseBase {
         fCurIsA = !fCurIsA;
         adeFirst = false;
         adeReset = false;
-        if (adeClsDirty) { HIPT(hipMemcpyAsync(adeClsPull, adeCls, Nopen, hipMemcpyDeviceToDevice, st)); adeClsDirty = false; }
+        if (adeClsDirty) {
+            HIPT(hipMemcpyAsync(adeClsPull, adeCls, Nopen, hipMemcpyDeviceToDevice, st));
+            HIPT(launchAdeKinds(nb7, adeClsPull, Nopen, Nint, adeKinds, st));
+            adeClsDirty = false;
+        }
     }
     void adeStabilize(hipStream_t st, int steps) override {
         for (int t = 0; t < steps; ++t) fusedLaunch(st, false);

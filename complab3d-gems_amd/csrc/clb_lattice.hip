@@ -185,10 +185,9 @@ __device__ __forceinline__ double clampInc(int clamp, double d, double cur, unsi
 // (run_kinetics + update_rxnLattices, part1:85-187, 406-497; the abiotic pair
 // part1:199-404; surfaceAbioticKinetics3D's face gate)
 template <typename R, int MAXS, bool DISSOL>
-__device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &k, int i, R (&f)[MAXS][7]) {
+__device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &k, int i, R (&f)[MAXS][7],
+                                              long long v, int mask, const R (&bpre)[CLB_DEV_MAXM], bool havePre) {
     const long long No = a.Nopen;
-    const long long v = a.list[i];
-    const int mask = a.mask[v];
     if (mask != 0 && mask != 1) {
         // species loops run to the static bound so that the slot selection in
         // concOf / depositTo resolves to fixed registers (a dynamic trip count
@@ -228,7 +227,12 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
             
 #pragma unroll
             for (int s = 0; s < CLB_DEV_MAXS; ++s) { if (s < S) { C[s] = concOf(s); if (C[s] < 1e-12) C[s] = 0; } }
-            for (int m = 0; m < k.M; ++m) { Bv[m] = (double) a.B[(long long) m * N + v]; if (Bv[m] < 1e-12) Bv[m] = 0; }
+#pragma unroll
+            for (int m = 0; m < CLB_DEV_MAXM; ++m) {
+                if (m >= k.M) continue;
+                Bv[m] = havePre ? (double) bpre[m] : (double) a.B[(long long) m * N + v];
+                if (Bv[m] < 1e-12) Bv[m] = 0;
+            }
             bioticLaw(k, C, Bv, sr, br, mask);
 #pragma unroll
             for (int s = 0; s < CLB_DEV_MAXS; ++s) {
@@ -334,60 +338,71 @@ __global__ void __launch_bounds__(256) k_ade_bulk(AdeDev<R> a, KinDev k) {
     int nbq[6];
 #pragma unroll
     for (int q = 1; q < 7; ++q) nbq[q - 1] = a.nb[(q - 1) * No + i];
+    const unsigned int kinds = a.kinds[i];
     int kind[6];                       // 0 neighbour, 1 edge, 2 bounce-back, 3 plane neighbour (species rule)
 #pragma unroll
-    for (int q = 1; q < 7; ++q) {
-        const int n = nbq[q - 1];
-        kind[q - 1] = n == NB_OUTSIDE ? 1 : (n == NB_CLOSED ? 2 : (a.clsPull[n] == ADE_BB ? 2 : (n >= a.Nint ? 3 : 0)));
-    }
+    for (int q = 1; q < 7; ++q) kind[q - 1] = (kinds >> (2 * (q - 1))) & 3;
     const int Np = a.Nopen - a.Nint;
     R f[MAXS][7];
     const R ux = a.u[i], uy = a.u[No + i], uz = a.u[2 * No + i];
+    const long long v = a.list[i];
+    const int vmask = a.mask[v];
+    // the biomass the kinetics reads, loaded with the populations
+    R bpre[CLB_DEV_MAXM];
+    const bool bioOn = k.on && k.bio;
+#pragma unroll
+    for (int m = 0; m < CLB_DEV_MAXM; ++m) bpre[m] = (bioOn && m < k.M) ? a.B[(long long) m * a.N + v] : (R) 0;
+    // every species' populations first: no store comes between the loads, so
+    // they are all in flight together
 #pragma unroll
     for (int ms = 0; ms < MAXS; ++ms) {
         if (ms >= a.Sm) continue;
-        const int s = a.mobSpecies[ms];
-        R *cur = a.cur + (long long) ms * 7 * No;
-        R *old = a.out + (long long) ms * 7 * No;
+        const R *__restrict__ cur = a.cur + (long long) ms * 7 * No;
+        const R *__restrict__ old = a.out + (long long) ms * 7 * No;
         if (a.first) {
 #pragma unroll
             for (int q = 0; q < 7; ++q) f[ms][q] = cur[q * No + i];
         } else {
+            const bool lClosed = a.mlbt[ms] == BC_CLOSED_DEV, rClosed = a.mrbt[ms] == BC_CLOSED_DEV;
             f[ms][0] = cur[i];
 #pragma unroll
             for (int q = 1; q < 7; ++q) {
                 const int n = nbq[q - 1];
                 int kd = kind[q - 1];
-                if (kd == 3) kd = ((n < a.Nint + a.Nleft) ? a.lbt[s] : a.rbt[s]) == BC_CLOSED_DEV ? 2 : 0;
+                if (kd == 3) kd = ((n < a.Nint + a.Nleft) ? lClosed : rClosed) ? 2 : 0;
                 const R *base = kd == 2 ? old : cur;
                 const long long off = kd == 0 ? (long long) q * No + n : (long long) opp7(q) * No + i;
                 f[ms][q] = base[off];
             }
         }
+    }
+#pragma unroll
+    for (int ms = 0; ms < MAXS; ++ms) {
+        if (ms >= a.Sm) continue;
         // the Neumann planes impose this voxel's density after the stream
-        if (nbq[0] >= a.Nint && nbq[0] < a.Nint + a.Nleft && a.lbt[s] == BC_NEUMANN_DEV) {
+        if (nbq[0] >= a.Nint && nbq[0] < a.Nint + a.Nleft && a.mlbt[ms] == BC_NEUMANN_DEV) {
             double r = 0;
 #pragma unroll
             for (int q = 0; q < 7; ++q) r += (double) f[ms][q];
             a.rhoN[(long long) ms * Np + (nbq[0] - a.Nint)] = r;
         }
-        if (nbq[1] >= a.Nint + a.Nleft && a.rbt[s] == BC_NEUMANN_DEV) {
+        if (nbq[1] >= a.Nint + a.Nleft && a.mrbt[ms] == BC_NEUMANN_DEV) {
             double r = 0;
 #pragma unroll
             for (int q = 0; q < 7; ++q) r += (double) f[ms][q];
             a.rhoN[(long long) ms * Np + (nbq[1] - a.Nint)] = r;
         }
-        if (a.first) firstHandOver<R>(a, cur, i, a.lbt[s], a.rbt[s], a.c0[s]);
+        if (a.first) firstHandOver<R>(a, a.cur + (long long) ms * 7 * No, i, a.mlbt[ms], a.mrbt[ms], a.mc0[ms]);
         if (a.reset) {
-            double c0 = a.c0[s];
+            double c0 = a.mc0[ms];
             if (c0 < 1e-12 && c0 > -1e-12) c0 = 0;
 #pragma unroll
             for (int q = 0; q < 7; ++q) f[ms][q] = (R) (w7(q) * c0);
         }
-        bgk7<R>(f[ms], (R) (cls == ADE_BFILM ? a.omB[s] : a.omP[s]), ux, uy, uz);
+        bgk7<R>(f[ms], (R) (cls == ADE_BFILM ? a.momB[ms] : a.momP[ms]), ux, uy, uz);
     }
 
-    if (k.on) voxelKinetics<R, MAXS, false>(a, k, i, f);
+    if (k.on) voxelKinetics<R, MAXS, false>(a, k, i, f, v, vmask, bpre, true);
 
 #pragma unroll
     for (int ms = 0; ms < MAXS; ++ms) {
@@ -416,7 +431,9 @@ __global__ void __launch_bounds__(256) k_kinetics(AdeDev<R> a, KinDev k) {
 #pragma unroll
         for (int q = 0; q < 7; ++q) f[ms][q] = c[q * No + i];
     }
-    voxelKinetics<R, MAXS, DISSOL>(a, k, i, f);
+    const long long v = a.list[i];
+    const R nopre[CLB_DEV_MAXM] = {};
+    voxelKinetics<R, MAXS, DISSOL>(a, k, i, f, v, a.mask[v], nopre, false);
 #pragma unroll
     for (int ms = 0; ms < MAXS; ++ms) {
         if (ms >= a.Sm) continue;
@@ -588,6 +605,23 @@ __global__ void k_ade_density(AdeDev<R> a, int ms, double *dst) {
         for (int q = 0; q < 7; ++q) r += fl[q];
     }
     dst[v] = r;
+}
+
+// the source kind of each moving population of an interior voxel under the
+// pull classes, two bits per direction (0 neighbour, 1 grid edge, 2 bounce-
+// back, 3 x-plane neighbour whose rule depends on the species' boundary type);
+// recomputed whenever the pull classes change, so the collide kernel reads one
+// word instead of gathering its neighbours' classes
+__global__ void __launch_bounds__(256) k_ade_kinds(const int *nb, const uint8_t *clsPull, int Nopen, int Nint, uint16_t *kinds) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nint) return;
+    unsigned int code = 0;
+    for (int q = 1; q < 7; ++q) {
+        const int n = nb[(long long) (q - 1) * Nopen + i];
+        const unsigned int k = n == NB_OUTSIDE ? 1u : (n == NB_CLOSED ? 2u : (clsPull[n] == ADE_BB ? 2u : (n >= Nint ? 3u : 0u)));
+        code |= k << (2 * (q - 1));
+    }
+    kinds[i] = (uint16_t) code;
 }
 
 // per block, for one mobile species: the mass the interior total does not
@@ -1122,6 +1156,10 @@ hipError_t launchAdeSet(const AdeDev<R> &a, int ms, const double *src, hipStream
 template <typename R>
 hipError_t launchAdeReopen(const AdeDev<R> &a, const uint8_t *re, hipStream_t st) {
     hipLaunchKernelGGL(k_ade_reopen<R>, dim3(nblk(a.Nopen, 256)), dim3(256), 0, st, a, re);
+    return hipGetLastError();
+}
+hipError_t launchAdeKinds(const int *nb, const uint8_t *clsPull, int Nopen, int Nint, uint16_t *kinds, hipStream_t st) {
+    if (Nint > 0) hipLaunchKernelGGL(k_ade_kinds, dim3(nblk(Nint, 256)), dim3(256), 0, st, nb, clsPull, Nopen, Nint, kinds);
     return hipGetLastError();
 }
 template <typename R>
