@@ -33,11 +33,11 @@ namespace clb {
 // voxel order keeps the partly-used cache lines of the neighbour gathers (and
 // the +-y/+-z neighbours themselves) in the L2 that fetched them.  A bijection
 // of [0, gridDim.x) whatever the dispatch order.
-__device__ __forceinline__ int xcdBlock() {
-    const int G = gridDim.x, b = blockIdx.x;
+__device__ __forceinline__ int xcdBlockOf(int b, int G) {
     const int q = G >> 3, r = G & 7, x = b & 7, k = b >> 3;
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
 }
+__device__ __forceinline__ int xcdBlock() { return xcdBlockOf(blockIdx.x, gridDim.x); }
 
 // a queued step batch was halted by an earlier step: the kernel does nothing
 // (the flag is written by an earlier kernel of the stream, never during this one)
@@ -823,9 +823,7 @@ __device__ __forceinline__ void blockEnergy(double e, double cnt, double *partE,
 
 // interior: pull + BGK; the energy statistic |j|^2/2 per block
 template <typename R>
-__global__ void __launch_bounds__(256) k_ns_bulk(NsDev<R> a, double *partE, double *partN) {
-    if (*a.done) return;
-    const int blk = xcdBlock();
+__device__ __forceinline__ void nsBulkBlock(const NsDev<R> &a, double *partE, double *partN, int blk) {
     const int i = blk * blockDim.x + threadIdx.x;
     double e = 0, cnt = 0;
     if (i < a.Nint) {
@@ -853,9 +851,8 @@ __global__ void __launch_bounds__(256) k_ns_bulk(NsDev<R> a, double *partE, doub
 
 // the pressure planes
 template <typename R>
-__global__ void __launch_bounds__(256) k_ns_planes(NsDev<R> a, double *partE, double *partN, int slot0) {
-    if (*a.done) return;
-    const int i = a.Nint + blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void nsPlaneBlock(const NsDev<R> &a, double *partE, double *partN, int b, int slot0) {
+    const int i = a.Nint + b * blockDim.x + threadIdx.x;
     double e = 0, cnt = 0;
     if (i < a.Nopen) {
         const int cls = a.cls[i];
@@ -874,37 +871,52 @@ __global__ void __launch_bounds__(256) k_ns_planes(NsDev<R> a, double *partE, do
             for (int q = 0; q < 19; ++q) a.out[q * No + i] = (R) (fl[q] + om * (feqInc<double>(q, rb, jx, 0.0, 0.0, jsq) - fl[q]));
         }
     }
-    blockEnergy(e, cnt, partE, partN, slot0 + blockIdx.x);
+    blockEnergy(e, cnt, partE, partN, slot0 + b);
+}
+
+// one flow iteration in one launch: the first gP blocks take the pressure
+// planes (few blocks with a long serial chain each, so they start first and
+// run beside the interior), the rest the interior in XCD order; block partials
+// at [interior blocks][plane blocks]
+template <typename R>
+__global__ void __launch_bounds__(256) k_ns_step(NsDev<R> a, double *partE, double *partN, int gP) {
+    if (*a.done) return;
+    const int gB = (int) gridDim.x - gP;
+    if ((int) blockIdx.x < gP) nsPlaneBlock<R>(a, partE, partN, blockIdx.x, gB);
+    else nsBulkBlock<R>(a, partE, partN, xcdBlockOf((int) blockIdx.x - gP, gB));
 }
 
 // the convergence rule, evaluated on the device after every iteration.  The
-// window statistics run in the ring's order (oldest first) from a copy in LDS.
-__global__ void __launch_bounds__(64) k_ns_tracer(NsTracer *t, const double *partE, const double *partN, int nparts, int rule, double eps, int *done) {
-    __shared__ double sE[64], sN[64];
+// partials are summed by 1024 threads, thread t taking t, t + 1024, ... (all
+// of its loads issued together), then in a fixed tree; the window statistics
+// run in the ring's order (oldest first) from a copy in LDS.
+__global__ void __launch_bounds__(1024) k_ns_tracer(NsTracer *t, const double *partE, const double *partN, int nparts, int rule, double eps, int *done) {
+    __shared__ double sE[1024], sN[1024];
     __shared__ double win[NS_TRACER_WINDOW];
     __shared__ double sVal;
     __shared__ int sHead, sFull;
     if (*done) return;
-    const int tid = threadIdx.x;   // 64 threads
-    // thread t adds partials t, t+64, t+128, ... in that order; the loads of a
-    // chunk of 16 are issued together
+    const int tid = threadIdx.x;
     double E = 0, C = 0;
-    for (int i0 = tid; i0 < nparts; i0 += 64 * 16) {
-        double e[16], c[16];
+    for (int i0 = tid; i0 < nparts; i0 += 1024 * 8) {
+        double e[8], c[8];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const int i = i0 + 64 * k;
+        for (int k = 0; k < 8; ++k) {
+            const int i = i0 + 1024 * k;
             e[k] = i < nparts ? partE[i] : 0.0;
             c[k] = i < nparts ? partN[i] : 0.0;
         }
 #pragma unroll
-        for (int k = 0; k < 16; ++k) if (i0 + 64 * k < nparts) { E += e[k]; C += c[k]; }
+        for (int k = 0; k < 8; ++k) { E += e[k]; C += c[k]; }
     }
     sE[tid] = E; sN[tid] = C;
     __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (tid < w) { sE[tid] += sE[tid + w]; sN[tid] += sN[tid + w]; }
+        __syncthreads();
+    }
     if (tid == 0) {
-        double Es = 0, Cs = 0;
-        for (int i = 0; i < 64; ++i) { Es += sE[i]; Cs += sN[i]; }
+        const double Es = sE[0], Cs = sN[0];
         const double val = Cs > 0 ? Es / Cs : 0.0;
         t->iters += 1;
         t->last = val;
@@ -927,7 +939,7 @@ __global__ void __launch_bounds__(64) k_ns_tracer(NsTracer *t, const double *par
     __syncthreads();
     if (!sFull) return;
     const int W = NS_TRACER_WINDOW;
-    for (int k = tid; k < W - 1; k += 64) win[k] = t->ring[(sHead + k) % W];
+    for (int k = tid; k < W - 1; k += 1024) win[k] = t->ring[(sHead + k) % W];
     if (tid == 0) win[W - 1] = sVal;
     __syncthreads();
     if (tid == 0) {
@@ -1175,9 +1187,8 @@ hipError_t launchAdeTotal(const AdeDev<R> &a, int ms, double *parts, hipStream_t
 template <typename R>
 hipError_t launchNsStep(const NsDev<R> &a, double *partE, double *partN, NsTracer *t, int rule, double eps, hipStream_t st) {
     const unsigned int gB = nblk(a.Nint, 256), gP = nblk(a.Nopen - a.Nint, 256);
-    if (gB) hipLaunchKernelGGL(k_ns_bulk<R>, dim3(gB), dim3(256), 0, st, a, partE, partN);
-    if (gP) hipLaunchKernelGGL(k_ns_planes<R>, dim3(gP), dim3(256), 0, st, a, partE, partN, (int) gB);
-    hipLaunchKernelGGL(k_ns_tracer, dim3(1), dim3(64), 0, st, t, partE, partN, (int) (gB + gP), rule, eps, a.done);
+    if (gB + gP) hipLaunchKernelGGL(k_ns_step<R>, dim3(gB + gP), dim3(256), 0, st, a, partE, partN, (int) gP);
+    hipLaunchKernelGGL(k_ns_tracer, dim3(1), dim3(1024), 0, st, t, partE, partN, (int) (gB + gP), rule, eps, a.done);
     return hipGetLastError();
 }
 template <typename R, typename RA>
