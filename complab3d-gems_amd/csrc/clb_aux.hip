@@ -66,19 +66,27 @@ __global__ void __launch_bounds__(256) k_ca_mask_total(CaDev<R> c, int check) {
 // the CA must run (max total above Bmax), the mask moved, or the total is not
 // finite (check 2: the step always needs the host, a precipitation step)
 template <typename R>
-__global__ void __launch_bounds__(256) k_ca_mask_final(CaDev<R> c, int nparts, int check) {
-    __shared__ double sMax[4];
-    __shared__ long long sMask[4];
+__global__ void __launch_bounds__(1024) k_ca_mask_final(CaDev<R> c, int nparts, int check) {
+    __shared__ double sMax[16];
+    __shared__ long long sMask[16];
     StepCtl *ctl = c.ctl;
     if (check && ctl->halt) return;
-    double m[4] = {-1e300, -1e300, -1e300, -1e300};
-    long long s[4] = {0, 0, 0, 0};
-    for (int b = threadIdx.x; b < nparts; b += 1024) {
+    // max and integer sum do not depend on the order: every load of a thread
+    // is issued before any is used
+    double mm = -1e300;
+    long long ss = 0;
+    for (int b = threadIdx.x; b < nparts; b += 1024 * 8) {
+        double m[8];
+        long long s[8];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) if (b + 256 * k < nparts) { m[k] = nanMax(m[k], c.partMax[b + 256 * k]); s[k] += c.partMask[b + 256 * k]; }
+        for (int k = 0; k < 8; ++k) {
+            const int j = b + 1024 * k;
+            m[k] = j < nparts ? c.partMax[j] : -1e300;
+            s[k] = j < nparts ? c.partMask[j] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) { mm = nanMax(mm, m[k]); ss += s[k]; }
     }
-    double mm = nanMax(nanMax(m[0], m[1]), nanMax(m[2], m[3]));
-    long long ss = (s[0] + s[1]) + (s[2] + s[3]);
     for (int off = 32; off > 0; off >>= 1) {
         mm = nanMax(mm, __shfl_down(mm, off, 64));
         ss += __shfl_down(ss, off, 64);
@@ -416,7 +424,7 @@ template <typename R>
 hipError_t launchCaMaskTotal(const CaDev<R> &c, int check, hipStream_t st) {
     const unsigned int g = nblk(c.Nopen, 256);
     hipLaunchKernelGGL(k_ca_mask_total<R>, dim3(g), dim3(256), 0, st, c, check);
-    hipLaunchKernelGGL(k_ca_mask_final<R>, dim3(1), dim3(256), 0, st, c, (int) g, check);
+    hipLaunchKernelGGL(k_ca_mask_final<R>, dim3(1), dim3(1024), 0, st, c, (int) g, check);
     return hipGetLastError();
 }
 hipError_t launchStepCtlReset(StepCtl *ctl, long long oldMask, hipStream_t st) {

@@ -321,12 +321,48 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
     }
 }
 
+// an x-plane voxel for one mobile species: pull, the Dirichlet or Neumann
+// plane rule, BGK.  The Neumann plane takes the inner layer's density after
+// the stream (FlatAdiabaticBoundaryFunctional3D; a bounce-back or closed inner
+// cell reports 0), which the interior kernel wrote before this one ran: the
+// inner voxel's own bounce-back slots are overwritten by that kernel, so the
+// planes cannot run in the same launch
+template <typename R>
+__device__ __forceinline__ void adePlaneVoxel(const AdeDev<R> &a, int i, int ms) {
+    const int cls = a.cls[i];
+    if (cls == ADE_BB) return;
+    const long long No = a.Nopen;
+    const bool left = i < a.Nint + a.Nleft;
+    const int Np = a.Nopen - a.Nint;
+    const R ux = a.u[i], uy = a.u[No + i], uz = a.u[2 * No + i];
+    const int s = a.mobSpecies[ms];
+    const int bt = left ? a.lbt[s] : a.rbt[s];
+    if (bt == BC_CLOSED_DEV) return;
+    R *cur = a.cur + (long long) ms * 7 * No;
+    R *old = a.out + (long long) ms * 7 * No;
+    R f[7];
+    pull7<R, R>(a, cur, old, i, a.lbt[s], a.rbt[s], f);
+    if (a.first) firstHandOver<R>(a, cur, i, a.lbt[s], a.rbt[s], a.c0[s]);
+    if (a.reset) {
+        double c0 = a.c0[s];
+        if (c0 < 1e-12 && c0 > -1e-12) c0 = 0;
+        for (int q = 0; q < 7; ++q) f[q] = (R) (w7(q) * c0);
+    }
+    double rhoD = left ? a.lbc[s] : a.rbc[s];
+    if (bt == BC_NEUMANN_DEV) {
+        const int inner = a.nb[(left ? 1 : 0) * No + i];
+        rhoD = 0.0;
+        if (inner >= 0 && a.cls[inner] != ADE_BB) rhoD = a.rhoN[(long long) ms * Np + (i - a.Nint)];
+    }
+    adeDirichlet<R>(f, left ? 1 : 0, (R) rhoD);
+    bgk7<R>(f, (R) (cls == ADE_BFILM ? a.omB[s] : a.omP[s]), ux, uy, uz);
+    for (int q = 0; q < 7; ++q) old[q * No + i] = f[q];
+}
+
 // interior: pull + BGK collide + biotic and abiotic kinetics with the clamp
 // and the D3Q7 deposit, every mobile species of a voxel in one thread
 template <typename R, int MAXS>
-__global__ void __launch_bounds__(256) k_ade_bulk(AdeDev<R> a, KinDev k) {
-    if (halted(a.halt)) return;
-    const int i = xcdBlock() * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void adeBulkVoxel(const AdeDev<R> &a, const KinDev &k, int i) {
     if (i >= a.Nint) return;
     const int cls = a.cls[i];
     if (cls == ADE_BB) return;
@@ -412,6 +448,24 @@ __global__ void __launch_bounds__(256) k_ade_bulk(AdeDev<R> a, KinDev k) {
         for (int q = 0; q < 7; ++q) o[q * No + i] = f[ms][q];
     }
 }
+
+template <typename R, int MAXS>
+__global__ void __launch_bounds__(256) k_ade_bulk(AdeDev<R> a, KinDev k) {
+    if (halted(a.halt)) return;
+    adeBulkVoxel<R, MAXS>(a, k, xcdBlock() * blockDim.x + threadIdx.x);
+}
+
+// the planes after the interior, one thread per plane voxel and mobile species
+// (blockIdx.y), so the short dependent load chains of the few plane voxels overlap
+template <typename R>
+__global__ void __launch_bounds__(256) k_ade_planes(AdeDev<R> a) {
+    if (halted(a.halt)) return;
+    const int i = a.Nint + blockIdx.x * blockDim.x + threadIdx.x;
+    const int ms = blockIdx.y;
+    if (i >= a.Nopen || ms >= a.Sm) return;
+    adePlaneVoxel<R>(a, i, ms);
+}
+
 
 // the kinetics alone on the stored post-collision state (the split form of
 // the fused kernel, for callers that apply the reference's processors one by
@@ -545,48 +599,6 @@ __global__ void k_dissol_convert(AdeDev<R> a, int8_t *phase, int phaseId, int mi
 }
 
 // the x planes: pull + regularized Dirichlet (or Neumann) completion + BGK;
-// one thread per plane voxel and mobile species (blockIdx.y), so the short
-// dependent load chains of the few plane voxels overlap
-template <typename R>
-__global__ void __launch_bounds__(256) k_ade_planes(AdeDev<R> a) {
-    if (halted(a.halt)) return;
-    const int i = a.Nint + blockIdx.x * blockDim.x + threadIdx.x;
-    const int ms = blockIdx.y;
-    if (i >= a.Nopen || ms >= a.Sm) return;
-    const int cls = a.cls[i];
-    if (cls == ADE_BB) return;
-    const long long No = a.Nopen;
-    const bool left = i < a.Nint + a.Nleft;
-    const int Np = a.Nopen - a.Nint;
-    const R ux = a.u[i], uy = a.u[No + i], uz = a.u[2 * No + i];
-    {
-        const int s = a.mobSpecies[ms];
-        const int bt = left ? a.lbt[s] : a.rbt[s];
-        if (bt == BC_CLOSED_DEV) return;
-        R *cur = a.cur + (long long) ms * 7 * No;
-        R *old = a.out + (long long) ms * 7 * No;
-        R f[7];
-        pull7<R, R>(a, cur, old, i, a.lbt[s], a.rbt[s], f);
-        if (a.first) firstHandOver<R>(a, cur, i, a.lbt[s], a.rbt[s], a.c0[s]);
-        if (a.reset) {
-            double c0 = a.c0[s];
-            if (c0 < 1e-12 && c0 > -1e-12) c0 = 0;
-            for (int q = 0; q < 7; ++q) f[q] = (R) (w7(q) * c0);
-        }
-        double rhoD = left ? a.lbc[s] : a.rbc[s];
-        if (bt == BC_NEUMANN_DEV) {
-            // FlatAdiabaticBoundaryFunctional3D: the inner layer's density after
-            // the stream, written by the interior kernel; a bounce-back (or
-            // closed) inner cell reports 0
-            const int inner = a.nb[(left ? 1 : 0) * No + i];
-            rhoD = (inner >= 0 && a.cls[inner] != ADE_BB) ? a.rhoN[(long long) ms * Np + (i - a.Nint)] : 0.0;
-        }
-        adeDirichlet<R>(f, left ? 1 : 0, (R) rhoD);
-        bgk7<R>(f, (R) (cls == ADE_BFILM ? a.omB[s] : a.omP[s]), ux, uy, uz);
-        for (int q = 0; q < 7; ++q) old[q * No + i] = f[q];
-    }
-}
-
 // density after the stream into a dense field (closed voxels keep 0)
 template <typename R>
 __global__ void k_ade_density(AdeDev<R> a, int ms, double *dst) {
