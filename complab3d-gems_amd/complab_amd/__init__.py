@@ -73,6 +73,7 @@ def lib():
         "clb_field_stats": ([V, I, V], I),
         "clb_counters": ([V, I, V], I),
         "clb_kernel_time": ([V, I, I, C.POINTER(D), C.POINTER(C.c_int64)], I),
+        "clb_set_timing_stride": ([V, I, I], I),
         "clb_stream_copy": ([V, C.c_size_t, I, C.POINTER(D), C.POINTER(D)], I),
         "clb_rng_draws": ([C.c_uint64, I, V], I),
         "clb_geom_spheres": ([I, I, I, D, I, C.c_uint64, I, V], I),
@@ -96,7 +97,7 @@ def exported_symbols():
         "clb_ade_stabilize", "clb_run_steps", "clb_ade_steps", "clb_equilibrium", "clb_ca_spill",
         "clb_ade_collide_stream", "clb_kinetics", "clb_mask_total", "clb_ca_sweep", "clb_age_update",
         "clb_update_solute_dynamics", "clb_update_solute_solid", "clb_precip_convert", "clb_dissol_convert",
-        "clb_read_field", "clb_write_field", "clb_species_totals", "clb_field_stats", "clb_counters", "clb_kernel_time",
+        "clb_read_field", "clb_write_field", "clb_species_totals", "clb_field_stats", "clb_counters", "clb_kernel_time", "clb_set_timing_stride",
         "clb_stream_copy",
         "clb_rng_draws", "clb_geom_spheres", "clb_geom_pack", "clb_geom_generate",
     ]
@@ -287,6 +288,11 @@ class Case:
         out = np.zeros(12, dtype=np.int64)
         self._c(lib().clb_counters(self.ctx.h, self.id, _p(out)))
         return out
+
+    def set_timing_stride(self, stride):
+        """Time only every stride-th transport launch (event markers cost the
+        device a dispatch gap each)."""
+        self._c(lib().clb_set_timing_stride(self.ctx.h, self.id, int(stride)))
 
     def kernel_time(self, reset=False):
         ms = C.c_double(); n = C.c_int64()

@@ -230,45 +230,88 @@ __global__ void __launch_bounds__(256) k_ca_gather(CaDev<R> c) {
         if (add[m] != 0.0) c.B[(long long) m * c.N + i] = (R) ((double) c.B[(long long) m * c.N + i] + add[m]);
 }
 
-// updateAgeDistance3D, one Jacobi sweep (ageIn -> ageOut) + the sum of the increments
+// updateAgeDistance3D's rule at one voxel (every voxel that can hold biomass
+// is in the compact list), from the ages it reads
 template <typename R>
-__global__ void __launch_bounds__(256) k_ca_age(CaDev<R> c, const int *ageIn, int *ageOut, long long *partSum) {
-    __shared__ long long sS[4];
-    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
-    long long sum = 0;
-    if (i < c.N) {
-        const int a = ageIn[i];
-        int na = a;
-        const double B = (double) c.Btot[i];
-        if (B > 1e-14) {
-            int x, y, z;
-            coords(i, c.nx, c.ny, x, y, z);
-            long long del[6]; int nd = 0;
-            for (int q = 1; q < 7; ++q) {
-                const int X = x + D7C[q][0], Y = y + D7C[q][1], Z = z + D7C[q][2];
-                if (X < 0 || Y < 0 || Z < 0 || X >= c.nx || Y >= c.ny || Z >= c.nz) continue;
-                const long long j = X + (long long) c.nx * (Y + (long long) c.ny * Z);
-                if (c.dist[j] > 0) del[nd++] = j;
-            }
-            if (a == 0) na = 1;
-            else if (a == 1 && (B - c.bmax) > -1e-14) {
-                bool upd = true;
-                for (int t = 0; t < nd; ++t) if (c.dist[del[t]] > 0 && ageIn[del[t]] == 0) { upd = false; break; }
-                if (upd) na = 2;
-            } else {
-                int cnt = 0;
-                for (int t = 0; t < nd; ++t) if (ageIn[del[t]] >= a && ((double) c.Btot[del[t]] - c.bmax) > -1e-14) ++cnt;
-                if (cnt == nd) na = a + 1;
-            }
-        }
-        ageOut[i] = na;
-        sum = na - a;
+__device__ __forceinline__ int ageRule(const CaDev<R> &c, const int *ageIn, long long i, int a) {
+    const double B = (double) c.Btot[i];
+    if (!(B > 1e-14)) return a;
+    int x, y, z;
+    coords(i, c.nx, c.ny, x, y, z);
+    long long del[6]; int nd = 0;
+    for (int q = 1; q < 7; ++q) {
+        const int X = x + D7C[q][0], Y = y + D7C[q][1], Z = z + D7C[q][2];
+        if (X < 0 || Y < 0 || Z < 0 || X >= c.nx || Y >= c.ny || Z >= c.nz) continue;
+        const long long j = X + (long long) c.nx * (Y + (long long) c.ny * Z);
+        if (c.dist[j] > 0) del[nd++] = j;
     }
+    if (a == 0) return 1;
+    if (a == 1 && (B - c.bmax) > -1e-14) {
+        for (int t = 0; t < nd; ++t) if (ageIn[del[t]] == 0) return a;
+        return 2;
+    }
+    int cnt = 0;
+    for (int t = 0; t < nd; ++t) if (ageIn[del[t]] >= a && ((double) c.Btot[del[t]] - c.bmax) > -1e-14) ++cnt;
+    return cnt == nd ? a + 1 : a;
+}
+
+template <typename R>
+__device__ __forceinline__ void agePartial(long long sum, long long *partSum) {
+    __shared__ long long sS[4];
     for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off, 64);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (lane == 0) sS[wv] = sum;
     __syncthreads();
     if (threadIdx.x == 0) { long long s = 0; for (int w = 0; w < (int) (blockDim.x >> 6); ++w) s += sS[w]; partSum[blockIdx.x] = s; }
+}
+
+// one Jacobi sweep (ageIn -> ageOut) over the compact voxels + the sum of the
+// increments; the other voxels hold the same age in both buffers
+template <typename R>
+__global__ void __launch_bounds__(256) k_ca_age(CaDev<R> c, const int *ageIn, int *ageOut, long long *partSum) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    long long sum = 0;
+    if (i < c.Nopen) {
+        const long long v = c.list[i];
+        const int a = ageIn[v];
+        const int na = ageRule(c, ageIn, v, a);
+        ageOut[v] = na;
+        sum = na - a;
+    }
+    agePartial<R>(sum, partSum);
+}
+
+// the voxels whose age can move while the biomass stands still: those holding
+// biomass (any order; the sweeps below do not depend on it)
+template <typename R>
+__global__ void __launch_bounds__(256) k_age_compact(CaDev<R> c, int *blist, unsigned int *count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.Nopen) return;
+    const long long v = c.list[i];
+    if ((double) c.Btot[v] > 1e-14) blist[atomicAdd(count, 1u)] = (int) v;
+}
+
+// one sweep in place over those voxels, for the relaxation to the fixed
+// point: a voxel may read ages this sweep already raised.  The rule is
+// monotone -- an age only rises, and raising a neighbour never blocks a rise
+// -- so any order of these updates climbs to the same least fixed point above
+// the start as the Jacobi sweeps, and a sweep that raises nothing stands on
+// it.  Sweep k counts the voxels it raised in raised[k]; a sweep queued after
+// one that raised nothing does nothing.
+template <typename R>
+__global__ void __launch_bounds__(256) k_ca_age_inplace(CaDev<R> c, int *age, const int *blist, int n,
+                                                        unsigned int *raised, int k) {
+    if (k > 0 && raised[k - 1] == 0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool up = false;
+    if (i < n) {
+        const long long v = blist[i];
+        const int a = age[v];
+        const int na = ageRule(c, age, v, a);
+        if (na != a) { age[v] = na; up = true; }
+    }
+    const unsigned long long b = __ballot(up);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(raised + k, (unsigned int) __popcll(b));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -370,17 +413,20 @@ __global__ void __launch_bounds__(256) k_final_sum(const double *parts, long lon
     }
     if (t == 0) *out = s[0];
 }
-__global__ void __launch_bounds__(256) k_final_sum_ll(const long long *parts, long long n, long long *out) {
-    __shared__ long long s[256];
+__global__ void __launch_bounds__(1024) k_final_sum_ll(const long long *parts, long long n, long long *out) {
+    __shared__ long long s[1024];
     const int t = threadIdx.x;
-    long long v[4] = {0, 0, 0, 0};
-    for (long long i = t; i < n; i += 1024) {
+    long long acc = 0;
+    for (long long i = t; i < n; i += 1024 * 8) {
+        long long v[8];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) if (i + 256 * k < n) v[k] += parts[i + 256 * k];
+        for (int k = 0; k < 8; ++k) v[k] = i + 1024 * k < n ? parts[i + 1024 * k] : 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += v[k];
     }
-    s[t] = (v[0] + v[1]) + (v[2] + v[3]);
+    s[t] = acc;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = 512; w > 0; w >>= 1) {
         if (t < w) s[t] += s[t + w];
         __syncthreads();
     }
@@ -439,7 +485,18 @@ hipError_t launchCaPushGather(const CaDev<R> &c, uint64_t sweep, hipStream_t st)
 }
 template <typename R>
 hipError_t launchCaAge(const CaDev<R> &c, const int *in, int *out, long long *ps, hipStream_t st) {
-    hipLaunchKernelGGL(k_ca_age<R>, dim3(nblk(c.N, 256)), dim3(256), 0, st, c, in, out, ps);
+    if (c.Nopen > 0) hipLaunchKernelGGL(k_ca_age<R>, dim3(nblk(c.Nopen, 256)), dim3(256), 0, st, c, in, out, ps);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchAgeCompact(const CaDev<R> &c, int *blist, unsigned int *count, hipStream_t st) {
+    if (c.Nopen > 0) hipLaunchKernelGGL(k_age_compact<R>, dim3(nblk(c.Nopen, 256)), dim3(256), 0, st, c, blist, count);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchCaAgeInplace(const CaDev<R> &c, int *age, const int *blist, int n, unsigned int *raised, int k,
+                              hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_ca_age_inplace<R>, dim3(nblk(n, 256)), dim3(256), 0, st, c, age, blist, n, raised, k);
     return hipGetLastError();
 }
 hipError_t launchOmegaSwitch(const int8_t *mask, const int *list, uint8_t *cls, int Nopen, hipStream_t st) {
@@ -467,7 +524,7 @@ hipError_t launchFinalSum(const double *parts, long long n, int stride, int comp
     return hipGetLastError();
 }
 hipError_t launchFinalSumLL(const long long *parts, long long n, long long *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_final_sum_ll, dim3(1), dim3(256), 0, st, parts, n, out);
+    hipLaunchKernelGGL(k_final_sum_ll, dim3(1), dim3(1024), 0, st, parts, n, out);
     return hipGetLastError();
 }
 template <typename R>
@@ -493,6 +550,8 @@ hipError_t launchI32ToDouble(const int *src, double *dst, long long n, hipStream
     template hipError_t launchCaMaskTotal<R>(const CaDev<R> &, int, hipStream_t); \
     template hipError_t launchCaPushGather<R>(const CaDev<R> &, uint64_t, hipStream_t); \
     template hipError_t launchCaAge<R>(const CaDev<R> &, const int *, int *, long long *, hipStream_t); \
+    template hipError_t launchAgeCompact<R>(const CaDev<R> &, int *, unsigned int *, hipStream_t); \
+    template hipError_t launchCaAgeInplace<R>(const CaDev<R> &, int *, const int *, int, unsigned int *, int, hipStream_t); \
     template hipError_t launchPrecip<R>(int, int, long long, int8_t *, const R *, double, int, int8_t *, unsigned int *, hipStream_t); \
     template hipError_t launchScalarStats<R>(int, int, long long, const R *, double *, hipStream_t); \
     template hipError_t launchToDouble<R>(const R *, double *, long long, hipStream_t); \

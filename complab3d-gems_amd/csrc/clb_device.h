@@ -181,6 +181,9 @@ unsigned int nsStepBlocks(int Nopen, int Nint);
 template <typename R> hipError_t launchCaMaskTotal(const CaDev<R> &, int check, hipStream_t);
 hipError_t launchStepCtlReset(StepCtl *, long long oldMask, hipStream_t);
 template <typename R> hipError_t launchCaPushGather(const CaDev<R> &, uint64_t, hipStream_t);
+template <typename R> hipError_t launchAgeCompact(const CaDev<R> &, int *blist, unsigned int *count, hipStream_t);
+template <typename R> hipError_t launchCaAgeInplace(const CaDev<R> &, int *age, const int *blist, int n, unsigned int *raised,
+                                                    int k, hipStream_t);
 template <typename R> hipError_t launchCaAge(const CaDev<R> &, const int *, int *, long long *, hipStream_t);
 hipError_t launchOmegaSwitch(const int8_t *, const int *, uint8_t *, int, hipStream_t);
 hipError_t launchSolidSwitch(const int8_t *, const int *, uint8_t *, uint8_t *, int, unsigned int *, hipStream_t);

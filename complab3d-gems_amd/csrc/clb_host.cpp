@@ -60,6 +60,7 @@ struct CaseBase {
     virtual void fieldStats(hipStream_t, double *out) = 0;
     virtual void counters(hipStream_t, int64_t out[CLB_NCOUNTERS]) = 0;
     virtual void kernelTime(hipStream_t, int reset, double *ms, int64_t *n) = 0;
+    virtual void setTimeStride(int stride) = 0;
     virtual void setRelaxation(const double *omp, const double *omb, double dt) = 0;
     // the processors one at a time (host facade of the reference's processor API)
     virtual void adeCollideStream(hipStream_t, int n) = 0;
@@ -98,7 +99,8 @@ struct Case : CaseBase {
     uint16_t *adeKinds = nullptr;
     R *fA = nullptr, *fB = nullptr, *imm = nullptr, *B = nullptr, *Btot = nullptr, *outbox = nullptr;
     uint8_t *pushed = nullptr;
-    int *age = nullptr, *age2 = nullptr;
+    int *age = nullptr, *age2 = nullptr, *ageList = nullptr;
+    unsigned int *ageCnt = nullptr;             // [0] biomass voxels, [1..16] voxels raised per queued sweep
     RN *gA = nullptr, *gB = nullptr;
     R *u = nullptr;
     double *rhoN = nullptr, *parts = nullptr, *scal = nullptr, *stage = nullptr;
@@ -120,7 +122,10 @@ struct Case : CaseBase {
     uint64_t caSweep_ = 0;
     int64_t cnt[CLB_NCOUNTERS] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> evpool;
+    std::vector<long long> evLaunch;            // the transport launch each event pair brackets
     size_t evUsed = 0;
+    long long launchNo = 0;                     // transport launches queued so far
+    int timeStride = 1;                         // time every timeStride-th launch
     double kms = 0;
     int64_t klaunch = 0;
     unsigned int nparts = 0;
@@ -323,7 +328,8 @@ struct Case : CaseBase {
             outbox = dalloc<R>(allocs, (size_t) M * 6 * N);
             pushed = dalloc<uint8_t>(allocs, N);
         }
-        age = dalloc<int>(allocs, N); age2 = dalloc<int>(allocs, N);
+        age = dalloc<int>(allocs, N); age2 = dalloc<int>(allocs, N); ageList = dalloc<int>(allocs, std::max(Nopen, 1));
+        ageCnt = dalloc<unsigned int>(allocs, 32);
         if (p.flow_on) {
             gA = dalloc<RN>(allocs, 19 * (size_t) Nopen); gB = dalloc<RN>(allocs, 19 * (size_t) Nopen);
             nb19 = dalloc<int>(allocs, (size_t) 18 * Nopen);
@@ -389,6 +395,7 @@ struct Case : CaseBase {
         HIPT(hipMemcpyAsync(nsCls, nc.data(), Nopen, hipMemcpyHostToDevice, st));
         HIPT(hipMemcpyAsync(nsClsPull, nc.data(), Nopen, hipMemcpyHostToDevice, st));
         HIPT(hipMemcpyAsync(age, ag.data(), N * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpyAsync(age2, ag.data(), N * sizeof(int), hipMemcpyHostToDevice, st));
         HIPT(hipMemsetAsync(reopened, 0, Nopen, st));
         // populations at rest with c0 (initializeAtEquilibrium + stabilizeADElattice3D seeding)
         {
@@ -576,19 +583,24 @@ struct Case : CaseBase {
         HIPT(launchKinetics<R>(ade(halt), k, st));
     }
     void fusedLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr, bool abioticInFused = true) {
-        if (evUsed == evpool.size()) {
+        const bool timed = launchNo % timeStride == 0;
+        if (timed && evUsed == evpool.size()) {
             if (evpool.size() >= kEvPool) flushTimes(st);
-            else { hipEvent_t e0, e1; HIPT(hipEventCreate(&e0)); HIPT(hipEventCreate(&e1)); evpool.push_back({e0, e1}); }
+            else {
+                hipEvent_t e0, e1;
+                HIPT(hipEventCreate(&e0)); HIPT(hipEventCreate(&e1));
+                evpool.push_back({e0, e1}); evLaunch.push_back(0);
+            }
         }
         AdeDev<R> a = ade(halt);
         KinDev k = kin(kinOn);
         k.dissol = 0;
         if (!abioticInFused) k.nrxn = 0;
         if (!k.bio && k.nrxn == 0) k.on = 0;
-        HIPT(hipEventRecord(evpool[evUsed].first, st));
+        if (timed) HIPT(hipEventRecord(evpool[evUsed].first, st));
         HIPT(launchAdeStep<R>(a, k, st));
-        HIPT(hipEventRecord(evpool[evUsed].second, st));
-        ++evUsed;
+        if (timed) { HIPT(hipEventRecord(evpool[evUsed].second, st)); evLaunch[evUsed] = launchNo; ++evUsed; }
+        ++launchNo;
         fCurIsA = !fCurIsA;
         adeFirst = false;
         adeReset = false;
@@ -680,30 +692,40 @@ struct Case : CaseBase {
         *gmax = ctlHost->gmax;
         *msum = ctlHost->msum;
     }
-    // updateAgeDistance3D sweeps; to the fixed point (sum of increments 0) or once
-    // A sweep at the fixed point changes nothing, so sweeps are queued several
-    // at a time and only the last one's sum of increments is read back; the
-    // result and the 1001-sweep limit are those of one sweep per read-back.
-    // Most relaxations settle within three sweeps: the first read-back comes
-    // after three, later ones after sixteen.
+    // updateAgeDistance3D: one Jacobi sweep, or sweeps to the fixed point.  To
+    // the fixed point the sweeps run in place over the voxels holding biomass
+    // (the only ages that can move; any order of the monotone updates reaches
+    // the Jacobi fixed point, and faster).  They are queued sixteen at a time,
+    // each skipping itself once the one before raised nothing, and the counts
+    // of one batch are read back together: the first sweep that raised nothing
+    // stands on the fixed point.  More than 1000 sweeps is an error.
     Status ageRelax(hipStream_t st, bool toFixedPoint) {
-        const unsigned int g = nblk(N, 256);
+        if (!toFixedPoint) {
+            HIPT(launchCaAge<R>(ca(), age, age2, partsLL, st));
+            std::swap(age, age2);
+            return {CLB_OK, ""};
+        }
+        const int K = 16;
+        unsigned int *cntDev = ageCnt;
+        int *blist = ageList;
+        HIPT(hipMemsetAsync(cntDev, 0, sizeof(unsigned int), st));
+        HIPT(launchAgeCompact<R>(ca(), blist, cntDev, st));
+        unsigned int n = 0;
+        HIPT(hipMemcpyAsync(&n, cntDev, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        HIPT(hipStreamSynchronize(st));
         int swept = 0;
         while (true) {
-            const int batch = toFixedPoint ? std::min(swept == 0 ? 3 : 16, 1001 - swept) : 1;
-            for (int k = 0; k < batch; ++k) {
-                HIPT(launchCaAge<R>(ca(), age, age2, partsLL, st));
-                std::swap(age, age2);
-            }
-            HIPT(launchFinalSumLL(partsLL, g, scalLL, st));
-            long long diff = 0;
-            HIPT(hipMemcpyAsync(&diff, scalLL, sizeof(long long), hipMemcpyDeviceToHost, st));
+            HIPT(hipMemsetAsync(cntDev + 1, 0, K * sizeof(unsigned int), st));
+            for (int k = 0; k < K; ++k) HIPT(launchCaAgeInplace<R>(ca(), age, blist, (int) n, cntDev + 1, k, st));
+            unsigned int raised[K];
+            HIPT(hipMemcpyAsync(raised, cntDev + 1, sizeof(raised), hipMemcpyDeviceToHost, st));
             HIPT(hipStreamSynchronize(st));
-            swept += batch;
-            if (!toFixedPoint || diff == 0) break;
-            if (swept > 1000) return {CLB_E_AGE_SWEEPS, "age relaxation did not settle"};
+            for (int k = 0; k < K; ++k) {
+                ++swept;
+                if (raised[k] == 0) return {CLB_OK, ""};
+                if (swept > 1000) return {CLB_E_AGE_SWEEPS, "age relaxation did not settle"};
+            }
         }
-        return {CLB_OK, ""};
     }
     Status caBlock(hipStream_t st, int *sweepsOut) override {
         if (sweepsOut) *sweepsOut = 0;
@@ -842,7 +864,7 @@ struct Case : CaseBase {
         for (int j = 0; j < len; ++j) if (hostDue(iT + j)) { len = j + 1; break; }
         if (evUsed + (size_t) len > kEvPool) flushTimes(st);
         HIPT(launchStepCtlReset(ctl, oldTotMask, st));
-        const size_t ev0 = evUsed;
+        const long long l0 = launchNo;
         for (int j = 0; j < len; ++j) {
             transportLaunch(st, true, &ctl->halt);
             if (p.eq_on) equilibriumQ(st, &ctl->halt);
@@ -855,7 +877,7 @@ struct Case : CaseBase {
         const int ran = fin + (halt ? 1 : 0);               // transport launches that did work
         if (fin > len || ran > len) return {CLB_E_HIP, "step batch bookkeeping failed"};
         if ((len - ran) & 1) fCurIsA = !fCurIsA;            // launches that returned at once swapped nothing
-        evUsed = ev0 + (size_t) ran;                        // and are not timed
+        while (evUsed > 0 && evLaunch[evUsed - 1] >= l0 + ran) --evUsed;   // and are not timed
         iT += fin; cnt[7] += fin;
         *finished = fin;
         batchLen = halt ? std::max(1, fin + 1) : std::min(kBatchMax, 2 * batchLen);
@@ -1006,6 +1028,7 @@ struct Case : CaseBase {
         out[3] = (int64_t) u3[2];
         out[8] = (int64_t) dc[0]; out[9] = (int64_t) dc[1]; out[10] = (int64_t) dc[2];
     }
+    void setTimeStride(int stride) override { timeStride = stride; }
     void kernelTime(hipStream_t, int reset, double *ms, int64_t *n) override {
         if (ms) *ms = klaunch ? kms / (double) klaunch : 0.0;
         if (n) *n = klaunch;
@@ -1438,6 +1461,14 @@ int clb_stream_copy(clb_ctx *ctx, size_t bytes, int reps, double *gbps, double *
     return rc;
 }
 
+int clb_set_timing_stride(clb_ctx *ctx, int case_id, int stride) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (stride < 1) return fail(ctx, CLB_E_ARG, "timing stride below 1");
+    c->setTimeStride(stride);
+    return CLB_OK;
+    GUARD_END
+}
 int clb_kernel_time(clb_ctx *ctx, int case_id, int reset, double *avg_ms, int64_t *launches) {
     GUARD_BEGIN
     CASE_OR_FAIL(c)

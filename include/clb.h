@@ -316,6 +316,10 @@ int  clb_counters(clb_ctx *ctx, int case_id, int64_t out[CLB_NCOUNTERS]);
 /* average device time (ms) of the fused transport kernel over the launches
  * since the last reset, measured with HIP events on the case's stream */
 int  clb_kernel_time(clb_ctx *ctx, int case_id, int reset, double *avg_ms, int64_t *launches);
+/* time every stride-th transport launch only (default 1: every launch); each
+ * timed launch puts two event markers on the stream, which the device waits
+ * on.  No counterpart in the reference. */
+int  clb_set_timing_stride(clb_ctx *ctx, int case_id, int stride);
 
 /* device copy bandwidth of a plain vectorised copy kernel over `bytes` bytes,
  * best of `reps` launches (HIP events); the measured ceiling the roofline
