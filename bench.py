@@ -62,10 +62,17 @@ def profiled_traffic(p, n_open):
 
 
 def algorithmic_bytes(p, n_open, n_vox):
+    """Bytes one transport launch must move: per open voxel 7 populations read
+    and 7 written per mobile species, the velocity, each immobile species and
+    each biomass read and written, and -- with a CA microbe, whose mask and
+    total update the launch makes -- the biomass total read and written; plus
+    the 1-byte mask of every voxel."""
     r = 8 if p.precision == 0 else 4
     sm = sum(1 for s in range(p.n_species) if p.mobile[s])
     si = p.n_species - sm
     per_open = sm * 14 * r + 3 * r + (si + p.n_microbes) * 2 * r
+    if p.n_microbes > 0:
+        per_open += 2 * r
     return per_open * n_open + n_vox, per_open
 
 

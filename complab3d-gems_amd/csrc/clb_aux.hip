@@ -15,11 +15,6 @@ namespace clb {
 // ------------------------------------------------------------------------------------------
 // CA biofilm spill
 // ------------------------------------------------------------------------------------------
-// NaN-propagating max (a NaN total must reach the host's finiteness check)
-__device__ __forceinline__ double nanMax(double a, double b) {
-    return (isnan(a) || isnan(b)) ? __builtin_nan("") : fmax(a, b);
-}
-
 // updateLocalMaskNtotalLattices3D (part2:71-259) over the open voxels, with
 // per-block partials of the global max of the total and of the mask sum.
 // Voxels outside the compact list are closed geometry: their mask and total
@@ -473,6 +468,11 @@ hipError_t launchCaMaskTotal(const CaDev<R> &c, int check, hipStream_t st) {
     hipLaunchKernelGGL(k_ca_mask_final<R>, dim3(1), dim3(1024), 0, st, c, (int) g, check);
     return hipGetLastError();
 }
+template <typename R>
+hipError_t launchCaMaskFinal(const CaDev<R> &c, int nparts, int check, hipStream_t st) {
+    hipLaunchKernelGGL(k_ca_mask_final<R>, dim3(1), dim3(1024), 0, st, c, nparts, check);
+    return hipGetLastError();
+}
 hipError_t launchStepCtlReset(StepCtl *ctl, long long oldMask, hipStream_t st) {
     hipLaunchKernelGGL(k_step_ctl_reset, dim3(1), dim3(1), 0, st, ctl, oldMask);
     return hipGetLastError();
@@ -548,6 +548,7 @@ hipError_t launchI32ToDouble(const int *src, double *dst, long long n, hipStream
 
 #define INST(R) \
     template hipError_t launchCaMaskTotal<R>(const CaDev<R> &, int, hipStream_t); \
+    template hipError_t launchCaMaskFinal<R>(const CaDev<R> &, int, int, hipStream_t); \
     template hipError_t launchCaPushGather<R>(const CaDev<R> &, uint64_t, hipStream_t); \
     template hipError_t launchCaAge<R>(const CaDev<R> &, const int *, int *, long long *, hipStream_t); \
     template hipError_t launchAgeCompact<R>(const CaDev<R> &, int *, unsigned int *, hipStream_t); \

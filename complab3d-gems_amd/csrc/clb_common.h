@@ -17,6 +17,10 @@ static __device__ __constant__ int D19C[19][3] = {
 #define W7R 0.25
 #define W7F 0.125
 __device__ __forceinline__ double w7(int q) { return q == 0 ? W7R : W7F; }
+// NaN-propagating max (a NaN total must reach the host's finiteness check)
+__device__ __forceinline__ double nanMax(double a, double b) {
+    return (isnan(a) || isnan(b)) ? __builtin_nan("") : fmax(a, b);
+}
 __device__ __forceinline__ int opp7(int q) { return q == 0 ? 0 : ((q & 1) ? q + 1 : q - 1); }
 __device__ __forceinline__ int opp19(int q) { return q == 0 ? 0 : ((q & 1) ? q + 1 : q - 1); }
 __device__ __forceinline__ double w19(int q) { return q == 0 ? (1.0 / 3.0) : (q < 7 ? (1.0 / 18.0) : (1.0 / 36.0)); }

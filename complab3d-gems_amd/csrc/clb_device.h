@@ -66,6 +66,13 @@ struct AdeDev {
     double mc0[CLB_DEV_MAXS], momP[CLB_DEV_MAXS], momB[CLB_DEV_MAXS];
     unsigned long long *clampCount;
     const int *halt;               // StepCtl::halt of a queued batch, or null
+    // the per-voxel mask and total update of the step (k_ca_mask_total's) done
+    // by the transport kernels, with block partials for the final reduction
+    int maskTotal;
+    R *Btot;
+    double thrdB;
+    double *partMax;
+    long long *partMask;
 };
 
 // rate-law parameters (biotic dual Monod, abiotic mass action)
@@ -179,6 +186,7 @@ template <typename R> hipError_t launchGatherCompact(const double *, const int *
 unsigned int nsStepBlocks(int Nopen, int Nint);
 // host launchers: dense grids (clb_aux.hip)
 template <typename R> hipError_t launchCaMaskTotal(const CaDev<R> &, int check, hipStream_t);
+template <typename R> hipError_t launchCaMaskFinal(const CaDev<R> &, int nparts, int check, hipStream_t);
 hipError_t launchStepCtlReset(StepCtl *, long long oldMask, hipStream_t);
 template <typename R> hipError_t launchCaPushGather(const CaDev<R> &, uint64_t, hipStream_t);
 template <typename R> hipError_t launchAgeCompact(const CaDev<R> &, int *blist, unsigned int *count, hipStream_t);

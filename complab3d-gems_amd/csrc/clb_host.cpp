@@ -180,6 +180,9 @@ struct Case : CaseBase {
         a.first = adeFirst ? 1 : 0; a.reset = adeReset ? 1 : 0;
         a.clampCount = clampCount;
         a.halt = halt;
+        a.maskTotal = 0;
+        a.Btot = Btot; a.thrdB = p.thrd_bfilm_frac * p.bmax;
+        a.partMax = caPartMax; a.partMask = caPartMask;
         return a;
     }
     KinDev kin(bool on) const {
@@ -349,8 +352,8 @@ struct Case : CaseBase {
         ctl = dalloc<StepCtl>(allocs, 1);
         if (p.dissol_on) share = dalloc<double>(allocs, (size_t) std::max(p.n_species, 1) * Nopen);
         dcount = dalloc<unsigned long long>(allocs, 3);
-        caPartMax = dalloc<double>(allocs, nblk(Nopen, 256));
-        caPartMask = dalloc<long long>(allocs, nblk(Nopen, 256));
+        caPartMax = dalloc<double>(allocs, nblk(Nint, 256) + nblk(Nopen - Nint, 256) + 1);
+        caPartMask = dalloc<long long>(allocs, nblk(Nint, 256) + nblk(Nopen - Nint, 256) + 1);
         {
             void *h = nullptr;
             HIPT(hipHostMalloc(&h, sizeof(StepCtl), hipHostMallocDefault));
@@ -574,15 +577,16 @@ struct Case : CaseBase {
     // increment waits for the products the phase voxels park: the fused launch
     // does transport + biotic kinetics, then the park pass, then the abiotic
     // kinetics with the gather (the reference's order, complab.cpp:2461-2545).
-    void transportLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr) {
-        if (!kinOn || !p.dissol_on) { fusedLaunch(st, kinOn, halt); return; }
-        fusedLaunch(st, true, halt, false);
+    void transportLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr, bool maskTotal = false) {
+        if (!kinOn || !p.dissol_on) { fusedLaunch(st, kinOn, halt, true, maskTotal); return; }
+        fusedLaunch(st, true, halt, false, maskTotal);
         KinDev k = kin(true);
         k.bio = 0;
         HIPT(launchDissolPark<R>(ade(halt), k, st));
         HIPT(launchKinetics<R>(ade(halt), k, st));
     }
-    void fusedLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr, bool abioticInFused = true) {
+    void fusedLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr, bool abioticInFused = true,
+                     bool maskTotal = false) {
         const bool timed = launchNo % timeStride == 0;
         if (timed && evUsed == evpool.size()) {
             if (evpool.size() >= kEvPool) flushTimes(st);
@@ -593,6 +597,7 @@ struct Case : CaseBase {
             }
         }
         AdeDev<R> a = ade(halt);
+        a.maskTotal = maskTotal ? 1 : 0;
         KinDev k = kin(kinOn);
         k.dissol = 0;
         if (!abioticInFused) k.nrxn = 0;
@@ -865,10 +870,16 @@ struct Case : CaseBase {
         if (evUsed + (size_t) len > kEvPool) flushTimes(st);
         HIPT(launchStepCtlReset(ctl, oldTotMask, st));
         const long long l0 = launchNo;
+        // the transport kernels make the step's mask and total update and its
+        // block partials (interior, then planes) when every open voxel passes
+        // through them; the final reduction takes the step's decision
+        const bool fused = Sm > 0 && Nint > 0;
+        const int nparts = (int) (nblk(Nint, 256) + nblk(Nopen - Nint, 256));
         for (int j = 0; j < len; ++j) {
-            transportLaunch(st, true, &ctl->halt);
+            transportLaunch(st, true, &ctl->halt, fused);
             if (p.eq_on) equilibriumQ(st, &ctl->halt);
-            HIPT(launchCaMaskTotal<R>(ca(), hostDue(iT + j) ? 2 : 1, st));
+            if (fused) HIPT(launchCaMaskFinal<R>(ca(), nparts, hostDue(iT + j) ? 2 : 1, st));
+            else HIPT(launchCaMaskTotal<R>(ca(), hostDue(iT + j) ? 2 : 1, st));
         }
         HIPT(hipMemcpyAsync(ctlHost, ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, st));
         HIPT(hipStreamSynchronize(st));

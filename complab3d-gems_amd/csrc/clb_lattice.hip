@@ -186,7 +186,7 @@ __device__ __forceinline__ double clampInc(int clamp, double d, double cur, unsi
 // part1:199-404; surfaceAbioticKinetics3D's face gate)
 template <typename R, int MAXS, bool DISSOL>
 __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &k, int i, R (&f)[MAXS][7],
-                                              long long v, int mask, const R (&bpre)[CLB_DEV_MAXM], bool havePre) {
+                                              long long v, int mask, R (&bpre)[CLB_DEV_MAXM], bool havePre) {
     const long long No = a.Nopen;
     if (mask != 0 && mask != 1) {
         // species loops run to the static bound so that the slot selection in
@@ -240,13 +240,16 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
                 const double d = sr[s] * k.dt;
                 if (d > 1e-12 || d < -1e-12) depositTo(s, clampInc(k.clamp, d, concOf(s), binds));
             }
-            for (int m = 0; m < k.M; ++m) {
+#pragma unroll
+            for (int m = 0; m < CLB_DEV_MAXM; ++m) {
+                if (m >= k.M) continue;
                 double d = br[m] * k.dt;
                 if (d > 1e-12 || d < -1e-12) {
                     R *p = a.B + (long long) m * N + v;
-                    const double b = (double) *p;
+                    const double b = havePre ? (double) bpre[m] : (double) *p;
                     if (d < -b) d = -b;                               // biomass never negative
                     *p = (R) (b + d);
+                    if (havePre) bpre[m] = (R) (b + d);
                 }
             }
         }
@@ -321,6 +324,47 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
     }
 }
 
+// the step's mask and total update of one voxel from its biomass b
+// (updateLocalMaskNtotalLattices3D: the total, then pore <-> biofilm code at
+// the threshold), and the total and mask it leaves for the reductions
+template <typename R>
+__device__ __forceinline__ void voxelMaskTotal(const AdeDev<R> &a, long long idx, int mask, const R (&b)[CLB_DEV_MAXM],
+                                               double &bt, long long &ms) {
+    if (mask != 0 && mask != 1) {
+        double bm = 0; int nm = 0;
+#pragma unroll
+        for (int m = 0; m < CLB_DEV_MAXM; ++m) {
+            if (m >= a.M) continue;
+            const double v = (double) b[m];
+            if (v > 1e-14) { bm += v; nm += 3 + m; }
+        }
+        double t = (double) a.Btot[idx];
+        if (fabs(bm - t) > 1e-14) { a.Btot[idx] = (R) bm; t = (double) (R) bm; }
+        const bool pore = (mask == 2);
+        if (!pore && bm < a.thrdB) { mask = 2; a.mask[idx] = 2; }
+        else if (pore && bm >= a.thrdB && nm > 0) { mask = nm; a.mask[idx] = (int8_t) nm; }
+        bt = t;
+    } else bt = (double) a.Btot[idx];
+    ms = mask;
+}
+// a block's max total (NaN wins) and mask sum into partMax/partMask[slot]
+__device__ __forceinline__ void blockMaskTotal(double bt, long long ms, double *partMax, long long *partMask, int slot) {
+    __shared__ double sMax[4];
+    __shared__ long long sMask[4];
+    for (int off = 32; off > 0; off >>= 1) {
+        bt = nanMax(bt, __shfl_down(bt, off, 64));
+        ms += __shfl_down(ms, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { sMax[wv] = bt; sMask[wv] = ms; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = -1e300; long long s = 0;
+        for (int w = 0; w < (int) (blockDim.x >> 6); ++w) { m = nanMax(m, sMax[w]); s += sMask[w]; }
+        partMax[slot] = m; partMask[slot] = s;
+    }
+}
+
 // an x-plane voxel for one mobile species: pull, the Dirichlet or Neumann
 // plane rule, BGK.  The Neumann plane takes the inner layer's density after
 // the stream (FlatAdiabaticBoundaryFunctional3D; a bounce-back or closed inner
@@ -362,10 +406,13 @@ __device__ __forceinline__ void adePlaneVoxel(const AdeDev<R> &a, int i, int ms)
 // interior: pull + BGK collide + biotic and abiotic kinetics with the clamp
 // and the D3Q7 deposit, every mobile species of a voxel in one thread
 template <typename R, int MAXS>
-__device__ __forceinline__ void adeBulkVoxel(const AdeDev<R> &a, const KinDev &k, int i) {
+__device__ __forceinline__ void adeBulkVoxel(const AdeDev<R> &a, const KinDev &k, int i, double &bt, long long &mt) {
     if (i >= a.Nint) return;
     const int cls = a.cls[i];
-    if (cls == ADE_BB) return;
+    if (cls == ADE_BB) {
+        if (a.maskTotal) { const long long v = a.list[i]; bt = (double) a.Btot[v]; mt = a.mask[v]; }
+        return;
+    }
     const long long No = a.Nopen;
     // where each moving population comes from, shared by every species: a
     // neighbour's cur slot, the voxel's own opposite cur slot (grid edge), or
@@ -387,7 +434,7 @@ __device__ __forceinline__ void adeBulkVoxel(const AdeDev<R> &a, const KinDev &k
     R bpre[CLB_DEV_MAXM];
     const bool bioOn = k.on && k.bio;
 #pragma unroll
-    for (int m = 0; m < CLB_DEV_MAXM; ++m) bpre[m] = (bioOn && m < k.M) ? a.B[(long long) m * a.N + v] : (R) 0;
+    for (int m = 0; m < CLB_DEV_MAXM; ++m) bpre[m] = ((bioOn || a.maskTotal) && m < a.M) ? a.B[(long long) m * a.N + v] : (R) 0;
     // every species' populations first: no store comes between the loads, so
     // they are all in flight together
 #pragma unroll
@@ -447,12 +494,16 @@ __device__ __forceinline__ void adeBulkVoxel(const AdeDev<R> &a, const KinDev &k
 #pragma unroll
         for (int q = 0; q < 7; ++q) o[q * No + i] = f[ms][q];
     }
+    if (a.maskTotal) voxelMaskTotal<R>(a, v, vmask, bpre, bt, mt);
 }
 
 template <typename R, int MAXS>
 __global__ void __launch_bounds__(256) k_ade_bulk(AdeDev<R> a, KinDev k) {
     if (halted(a.halt)) return;
-    adeBulkVoxel<R, MAXS>(a, k, xcdBlock() * blockDim.x + threadIdx.x);
+    const int blk = xcdBlock();
+    double bt = -1e300; long long mt = 0;
+    adeBulkVoxel<R, MAXS>(a, k, blk * blockDim.x + threadIdx.x, bt, mt);
+    if (a.maskTotal) blockMaskTotal(bt, mt, a.partMax, a.partMask, blk);
 }
 
 // the planes after the interior, one thread per plane voxel and mobile species
@@ -462,8 +513,18 @@ __global__ void __launch_bounds__(256) k_ade_planes(AdeDev<R> a) {
     if (halted(a.halt)) return;
     const int i = a.Nint + blockIdx.x * blockDim.x + threadIdx.x;
     const int ms = blockIdx.y;
-    if (i >= a.Nopen || ms >= a.Sm) return;
-    adePlaneVoxel<R>(a, i, ms);
+    if (i < a.Nopen && ms < a.Sm) adePlaneVoxel<R>(a, i, ms);
+    if (a.maskTotal && blockIdx.y == 0) {        // the planes' mask and total, after the interior's partials
+        double bt = -1e300; long long mt = 0;
+        if (i < a.Nopen) {
+            const long long v = a.list[i];
+            R b[CLB_DEV_MAXM];
+#pragma unroll
+            for (int m = 0; m < CLB_DEV_MAXM; ++m) b[m] = m < a.M ? a.B[(long long) m * a.N + v] : (R) 0;
+            voxelMaskTotal<R>(a, v, a.mask[v], b, bt, mt);
+        }
+        blockMaskTotal(bt, mt, a.partMax, a.partMask, (int) ((a.Nint + 255) / 256) + (int) blockIdx.x);
+    }
 }
 
 
@@ -486,7 +547,7 @@ __global__ void __launch_bounds__(256) k_kinetics(AdeDev<R> a, KinDev k) {
         for (int q = 0; q < 7; ++q) f[ms][q] = c[q * No + i];
     }
     const long long v = a.list[i];
-    const R nopre[CLB_DEV_MAXM] = {};
+    R nopre[CLB_DEV_MAXM] = {};
     voxelKinetics<R, MAXS, DISSOL>(a, k, i, f, v, a.mask[v], nopre, false);
 #pragma unroll
     for (int ms = 0; ms < MAXS; ++ms) {
