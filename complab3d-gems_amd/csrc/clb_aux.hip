@@ -277,13 +277,21 @@ __global__ void __launch_bounds__(256) k_ca_age(CaDev<R> c, const int *ageIn, in
 }
 
 // the voxels whose age can move while the biomass stands still: those holding
-// biomass (any order; the sweeps below do not depend on it)
+// biomass (in any order: the sweeps below do not depend on it); one atomic
+// per wave reserves the wave's slots
 template <typename R>
 __global__ void __launch_bounds__(256) k_age_compact(CaDev<R> c, int *blist, unsigned int *count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.Nopen) return;
-    const long long v = c.list[i];
-    if ((double) c.Btot[v] > 1e-14) blist[atomicAdd(count, 1u)] = (int) v;
+    long long v = 0;
+    bool keep = false;
+    if (i < c.Nopen) { v = c.list[i]; keep = (double) c.Btot[v] > 1e-14; }
+    const unsigned long long b = __ballot(keep);
+    if (!b) return;
+    const int lane = threadIdx.x & 63;
+    unsigned int base = 0;
+    if (lane == __ffsll((long long) b) - 1) base = atomicAdd(count, (unsigned int) __popcll(b));
+    base = __shfl(base, __ffsll((long long) b) - 1, 64);
+    if (keep) blist[base + __popcll(b & ((1ull << lane) - 1ull))] = (int) v;
 }
 
 // one sweep in place over those voxels, for the relaxation to the fixed

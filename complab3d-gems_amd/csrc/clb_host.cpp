@@ -100,7 +100,7 @@ struct Case : CaseBase {
     R *fA = nullptr, *fB = nullptr, *imm = nullptr, *B = nullptr, *Btot = nullptr, *outbox = nullptr;
     uint8_t *pushed = nullptr;
     int *age = nullptr, *age2 = nullptr, *ageList = nullptr;
-    unsigned int *ageCnt = nullptr;             // [0] biomass voxels, [1..16] voxels raised per queued sweep
+    unsigned int *ageCnt = nullptr;             // [0] biomass voxels, [1..8] voxels raised per queued sweep
     RN *gA = nullptr, *gB = nullptr;
     R *u = nullptr;
     double *rhoN = nullptr, *parts = nullptr, *scal = nullptr, *stage = nullptr;
@@ -700,7 +700,7 @@ struct Case : CaseBase {
     // updateAgeDistance3D: one Jacobi sweep, or sweeps to the fixed point.  To
     // the fixed point the sweeps run in place over the voxels holding biomass
     // (the only ages that can move; any order of the monotone updates reaches
-    // the Jacobi fixed point, and faster).  They are queued sixteen at a time,
+    // the Jacobi fixed point, and faster).  They are queued eight at a time,
     // each skipping itself once the one before raised nothing, and the counts
     // of one batch are read back together: the first sweep that raised nothing
     // stands on the fixed point.  More than 1000 sweeps is an error.
@@ -710,7 +710,7 @@ struct Case : CaseBase {
             std::swap(age, age2);
             return {CLB_OK, ""};
         }
-        const int K = 16;
+        const int K = 8;
         unsigned int *cntDev = ageCnt;
         int *blist = ageList;
         HIPT(hipMemsetAsync(cntDev, 0, sizeof(unsigned int), st));
