@@ -277,21 +277,59 @@ __global__ void __launch_bounds__(256) k_ca_age(CaDev<R> c, const int *ageIn, in
 }
 
 // the voxels whose age can move while the biomass stands still: those holding
-// biomass (in any order: the sweeps below do not depend on it); one atomic
-// per wave reserves the wave's slots
+// biomass, compacted in list order without atomics (per-block counts, one
+// block's exclusive scan of them, an ordered scatter)
 template <typename R>
-__global__ void __launch_bounds__(256) k_age_compact(CaDev<R> c, int *blist, unsigned int *count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    long long v = 0;
-    bool keep = false;
-    if (i < c.Nopen) { v = c.list[i]; keep = (double) c.Btot[v] > 1e-14; }
+__device__ __forceinline__ bool holdsBiomass(const CaDev<R> &c, int i, long long &v) {
+    if (i >= c.Nopen) return false;
+    v = c.list[i];
+    return (double) c.Btot[v] > 1e-14;
+}
+// the kept voxels before this wave in the block, and the block's total
+__device__ __forceinline__ unsigned int blockRank(bool keep, unsigned int &blockTotal) {
+    __shared__ unsigned int sW[4];
     const unsigned long long b = __ballot(keep);
-    if (!b) return;
-    const int lane = threadIdx.x & 63;
-    unsigned int base = 0;
-    if (lane == __ffsll((long long) b) - 1) base = atomicAdd(count, (unsigned int) __popcll(b));
-    base = __shfl(base, __ffsll((long long) b) - 1, 64);
-    if (keep) blist[base + __popcll(b & ((1ull << lane) - 1ull))] = (int) v;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) sW[wv] = (unsigned int) __popcll(b);
+    __syncthreads();
+    unsigned int before = 0, total = 0;
+    for (int w = 0; w < (int) (blockDim.x >> 6); ++w) { if (w < wv) before += sW[w]; total += sW[w]; }
+    blockTotal = total;
+    return before + (unsigned int) __popcll(b & ((1ull << lane) - 1ull));
+}
+template <typename R>
+__global__ void __launch_bounds__(256) k_age_count(CaDev<R> c, unsigned int *blockCount) {
+    long long v = 0;
+    const bool keep = holdsBiomass(c, blockIdx.x * blockDim.x + threadIdx.x, v);
+    unsigned int total = 0;
+    blockRank(keep, total);
+    if (threadIdx.x == 0) blockCount[blockIdx.x] = total;
+}
+// exclusive scan of the block counts in place; the sum into *total
+__global__ void __launch_bounds__(1024) k_age_scan(unsigned int *blockCount, int nb, unsigned int *total) {
+    __shared__ unsigned int sS[1024];
+    const int t = threadIdx.x, per = (nb + 1023) / 1024, b0 = t * per;
+    unsigned int s = 0;
+    for (int k = 0; k < per; ++k) if (b0 + k < nb) s += blockCount[b0 + k];
+    sS[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {             // inclusive Hillis-Steele scan
+        const unsigned int add = t >= off ? sS[t - off] : 0u;
+        __syncthreads();
+        sS[t] += add;
+        __syncthreads();
+    }
+    unsigned int run = t > 0 ? sS[t - 1] : 0u;
+    for (int k = 0; k < per; ++k) if (b0 + k < nb) { const unsigned int x = blockCount[b0 + k]; blockCount[b0 + k] = run; run += x; }
+    if (t == 1023) *total = sS[1023];
+}
+template <typename R>
+__global__ void __launch_bounds__(256) k_age_scatter(CaDev<R> c, const unsigned int *blockOffset, int *blist) {
+    long long v = 0;
+    const bool keep = holdsBiomass(c, blockIdx.x * blockDim.x + threadIdx.x, v);
+    unsigned int total = 0;
+    const unsigned int r = blockRank(keep, total);
+    if (keep) blist[blockOffset[blockIdx.x] + r] = (int) v;
 }
 
 // one sweep in place over those voxels, for the relaxation to the fixed
@@ -497,8 +535,12 @@ hipError_t launchCaAge(const CaDev<R> &c, const int *in, int *out, long long *ps
     return hipGetLastError();
 }
 template <typename R>
-hipError_t launchAgeCompact(const CaDev<R> &c, int *blist, unsigned int *count, hipStream_t st) {
-    if (c.Nopen > 0) hipLaunchKernelGGL(k_age_compact<R>, dim3(nblk(c.Nopen, 256)), dim3(256), 0, st, c, blist, count);
+hipError_t launchAgeCompact(const CaDev<R> &c, int *blist, unsigned int *blockScratch, unsigned int *count, hipStream_t st) {
+    const unsigned int nb = nblk(c.Nopen, 256);
+    if (nb == 0) return hipGetLastError();
+    hipLaunchKernelGGL(k_age_count<R>, dim3(nb), dim3(256), 0, st, c, blockScratch);
+    hipLaunchKernelGGL(k_age_scan, dim3(1), dim3(1024), 0, st, blockScratch, (int) nb, count);
+    hipLaunchKernelGGL(k_age_scatter<R>, dim3(nb), dim3(256), 0, st, c, blockScratch, blist);
     return hipGetLastError();
 }
 template <typename R>
@@ -559,7 +601,7 @@ hipError_t launchI32ToDouble(const int *src, double *dst, long long n, hipStream
     template hipError_t launchCaMaskFinal<R>(const CaDev<R> &, int, int, hipStream_t); \
     template hipError_t launchCaPushGather<R>(const CaDev<R> &, uint64_t, hipStream_t); \
     template hipError_t launchCaAge<R>(const CaDev<R> &, const int *, int *, long long *, hipStream_t); \
-    template hipError_t launchAgeCompact<R>(const CaDev<R> &, int *, unsigned int *, hipStream_t); \
+    template hipError_t launchAgeCompact<R>(const CaDev<R> &, int *, unsigned int *, unsigned int *, hipStream_t); \
     template hipError_t launchCaAgeInplace<R>(const CaDev<R> &, int *, const int *, int, unsigned int *, int, hipStream_t); \
     template hipError_t launchPrecip<R>(int, int, long long, int8_t *, const R *, double, int, int8_t *, unsigned int *, hipStream_t); \
     template hipError_t launchScalarStats<R>(int, int, long long, const R *, double *, hipStream_t); \

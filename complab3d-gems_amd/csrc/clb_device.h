@@ -189,7 +189,8 @@ template <typename R> hipError_t launchCaMaskTotal(const CaDev<R> &, int check, 
 template <typename R> hipError_t launchCaMaskFinal(const CaDev<R> &, int nparts, int check, hipStream_t);
 hipError_t launchStepCtlReset(StepCtl *, long long oldMask, hipStream_t);
 template <typename R> hipError_t launchCaPushGather(const CaDev<R> &, uint64_t, hipStream_t);
-template <typename R> hipError_t launchAgeCompact(const CaDev<R> &, int *blist, unsigned int *count, hipStream_t);
+template <typename R> hipError_t launchAgeCompact(const CaDev<R> &, int *blist, unsigned int *blockScratch, unsigned int *count,
+                                                  hipStream_t);
 template <typename R> hipError_t launchCaAgeInplace(const CaDev<R> &, int *age, const int *blist, int n, unsigned int *raised,
                                                     int k, hipStream_t);
 template <typename R> hipError_t launchCaAge(const CaDev<R> &, const int *, int *, long long *, hipStream_t);

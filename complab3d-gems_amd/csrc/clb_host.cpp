@@ -713,8 +713,7 @@ struct Case : CaseBase {
         const int K = 8;
         unsigned int *cntDev = ageCnt;
         int *blist = ageList;
-        HIPT(hipMemsetAsync(cntDev, 0, sizeof(unsigned int), st));
-        HIPT(launchAgeCompact<R>(ca(), blist, cntDev, st));
+        HIPT(launchAgeCompact<R>(ca(), blist, reinterpret_cast<unsigned int *>(partsLL), cntDev, st));
         unsigned int n = 0;
         HIPT(hipMemcpyAsync(&n, cntDev, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
         HIPT(hipStreamSynchronize(st));
