@@ -164,7 +164,8 @@ struct EqDev {
 };
 
 // host launchers: compact lattices (clb_lattice.hip)
-template <typename R> hipError_t launchAdeStep(const AdeDev<R> &, const KinDev &, hipStream_t);
+// the interior kernel, then (when given) an event, then the planes
+template <typename R> hipError_t launchAdeStep(const AdeDev<R> &, const KinDev &, hipStream_t, hipEvent_t afterInterior);
 template <typename R> hipError_t launchKinetics(const AdeDev<R> &, const KinDev &, hipStream_t);
 template <typename R> hipError_t launchDissolPark(const AdeDev<R> &, const KinDev &, hipStream_t);
 template <typename R> hipError_t launchDissolConvert(const AdeDev<R> &, int8_t *phase, int phaseId, int mineralSlot,

@@ -561,8 +561,9 @@ struct Case : CaseBase {
     }
 
     // ------------------------------------------------------------------ transport
-    // device time of every fused launch, from a pair of events around it on the
-    // case's stream; read back when the stream has drained
+    // device time of the interior transport kernel (the fused stream, collide
+    // and kinetics of every interior voxel), from a pair of events around it on
+    // the case's stream; read back when the stream has drained
     void flushTimes(hipStream_t st) {
         if (!evUsed) return;
         HIPT(hipStreamSynchronize(st));
@@ -603,8 +604,8 @@ struct Case : CaseBase {
         if (!abioticInFused) k.nrxn = 0;
         if (!k.bio && k.nrxn == 0) k.on = 0;
         if (timed) HIPT(hipEventRecord(evpool[evUsed].first, st));
-        HIPT(launchAdeStep<R>(a, k, st));
-        if (timed) { HIPT(hipEventRecord(evpool[evUsed].second, st)); evLaunch[evUsed] = launchNo; ++evUsed; }
+        HIPT(launchAdeStep<R>(a, k, st, timed ? evpool[evUsed].second : nullptr));
+        if (timed) { evLaunch[evUsed] = launchNo; ++evUsed; }
         ++launchNo;
         fCurIsA = !fCurIsA;
         adeFirst = false;

@@ -1183,7 +1183,7 @@ static inline unsigned int nblk(long long n, int b) { return (unsigned int) ((n 
 unsigned int nsStepBlocks(int Nopen, int Nint) { return nblk(Nint, 256) + nblk(Nopen - Nint, 256); }
 
 template <typename R>
-hipError_t launchAdeStep(const AdeDev<R> &a, const KinDev &k, hipStream_t st) {
+hipError_t launchAdeStep(const AdeDev<R> &a, const KinDev &k, hipStream_t st, hipEvent_t afterInterior) {
     const int Np = a.Nopen - a.Nint;
     if (a.Nint > 0) {
         const unsigned int g = nblk(a.Nint, 256);
@@ -1193,6 +1193,10 @@ hipError_t launchAdeStep(const AdeDev<R> &a, const KinDev &k, hipStream_t st) {
         else if (a.Sm <= 4) hipLaunchKernelGGL((k_ade_bulk<R, 4>), dim3(g), dim3(256), 0, st, a, k);
         else if (a.Sm <= 6) hipLaunchKernelGGL((k_ade_bulk<R, 6>), dim3(g), dim3(256), 0, st, a, k);
         else hipLaunchKernelGGL((k_ade_bulk<R, 8>), dim3(g), dim3(256), 0, st, a, k);
+    }
+    if (afterInterior) {
+        const hipError_t e = hipEventRecord(afterInterior, st);
+        if (e != hipSuccess) return e;
     }
     if (Np > 0 && a.Sm > 0) hipLaunchKernelGGL(k_ade_planes<R>, dim3(nblk(Np, 256), a.Sm), dim3(256), 0, st, a);
     return hipGetLastError();
@@ -1300,7 +1304,7 @@ hipError_t launchGatherCompact(const double *src, const int *list, int n, R *dst
 }
 
 #define INST(R) \
-    template hipError_t launchAdeStep<R>(const AdeDev<R> &, const KinDev &, hipStream_t); \
+    template hipError_t launchAdeStep<R>(const AdeDev<R> &, const KinDev &, hipStream_t, hipEvent_t); \
     template hipError_t launchKinetics<R>(const AdeDev<R> &, const KinDev &, hipStream_t); \
     template hipError_t launchDissolPark<R>(const AdeDev<R> &, const KinDev &, hipStream_t); \
     template hipError_t launchDissolConvert<R>(const AdeDev<R> &, int8_t *, int, int, double, unsigned int *, hipStream_t); \

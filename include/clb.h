@@ -313,8 +313,10 @@ int  clb_field_stats(clb_ctx *ctx, int case_id, double *out);
 int  clb_counters(clb_ctx *ctx, int case_id, int64_t out[CLB_NCOUNTERS]);
 
 /* ---- profiling ------------------------------------------------------------- */
-/* average device time (ms) of the fused transport kernel over the launches
- * since the last reset, measured with HIP events on the case's stream */
+/* average device time (ms) of the fused transport kernel of the interior
+ * voxels (stream, collide, kinetics; the x planes run after it in a kernel of
+ * their own) over the timed launches since the last reset, measured with HIP
+ * events on the case's stream */
 int  clb_kernel_time(clb_ctx *ctx, int case_id, int reset, double *avg_ms, int64_t *launches);
 /* time every stride-th transport launch only (default 1: every launch); each
  * timed launch puts two event markers on the stream, which the device waits
