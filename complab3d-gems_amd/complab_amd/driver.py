@@ -54,12 +54,31 @@ class SetupReport:
     omega_biofilm: List[float] = field(default_factory=list)
 
 
-def solve_flow(case, fs: FlowSetup, rep: SetupReport):
+def _solve(case, fs: FlowSetup, chunk=0, progress=None):
+    """One flow solve to fs.eps or fs.max_iter iterations.  With `chunk` the
+    iterations run in calls of at most `chunk` (the convergence tracer carries
+    over between calls, so the decisions are those of one call) and
+    `progress(iterations so far)` is called after each."""
+    if not chunk:
+        return case.ns_solve(fs.max_iter, fs.rule, fs.eps, reset=True)
+    total, cv = 0, False
+    while total < fs.max_iter:
+        want = min(chunk, fs.max_iter - total)
+        it, cv = case.ns_solve(want, fs.rule, fs.eps, reset=(total == 0))
+        total += it
+        if progress:
+            progress(total)
+        if cv or it < want:
+            break
+    return total, cv
+
+
+def solve_flow(case, fs: FlowSetup, rep: SetupReport, chunk=0, progress=None):
     """NS solve + Pe rescale (complab.cpp:1098-1183)."""
     nu = CS2_NS * (fs.ns_tau - 0.5)
     dp = fs.delta_p
     case.ns_init(dp)
-    it, cv = case.ns_solve(fs.max_iter, fs.rule, fs.eps, reset=True)
+    it, cv = _solve(case, fs, chunk, progress)
     rep.iters.append(it); rep.converged.append(cv)
     st = case.ns_stats()
     pore_mean = st[3] if fs.has_biofilm else st[0]
@@ -71,7 +90,7 @@ def solve_flow(case, fs: FlowSetup, rep: SetupReport):
     max_ux = st[1]
     if abs(dp_new - dp) / dp > 0.01:
         case.ns_init(dp_new)
-        it, cv = case.ns_solve(fs.max_iter, fs.rule, fs.eps, reset=True)
+        it, cv = _solve(case, fs, chunk, progress)
         rep.iters.append(it); rep.converged.append(cv)
         st = case.ns_stats()
         pore_mean = st[0]                      # complab.cpp:1175 averages the box
@@ -112,12 +131,13 @@ def ade_relaxation(p, pore_mean_u, fs: FlowSetup, ts: TransportSetup, rep: Setup
     return omp, omb, dt
 
 
-def setup(case, p, fs: FlowSetup, ts: TransportSetup):
-    """Runs the whole set-up on `case`; returns the report."""
+def setup(case, p, fs: FlowSetup, ts: TransportSetup, chunk=0, progress=None):
+    """Runs the whole set-up on `case`; returns the report.  `chunk` and
+    `progress` split the flow solves as `_solve` does."""
     rep = SetupReport()
     pore_mean = 0.0
     if fs.pe > 0 and p.flow_on:
-        pore_mean = solve_flow(case, fs, rep)
+        pore_mean = solve_flow(case, fs, rep, chunk, progress)
     omp, omb, dt = ade_relaxation(p, pore_mean, fs, ts, rep)
     case.set_omegas(omp, omb, dt)
     if fs.pe > 0 and p.flow_on:

@@ -1,0 +1,356 @@
+"""A CompLaB3D run from its own input files: CompLaB.xml and geometry.dat.
+
+Reads the tags of the coupled reaction step the way the reference's readers
+do -- initialize_complab (complab_functions.hh:825-1600: domain, numerics,
+chemistry, microbiology, equilibrium), the immobile / precipitation /
+dissolution blocks (complab.cpp:440-574) and the diagnostics block
+(complab3d_integration.hh:375-423) -- with the same defaults, and fills a
+parameter block, the flow and transport set-up and the geometry.
+
+Two things the files do not carry: the rate laws (the reference compiles them
+from defineKinetics.hh / defineAbioticKinetics.hh / defineDissolutionRate into
+the executable), which the caller sets on `params` (bio_law, mu_max, rxn_*,
+dissol_k, ...); and the kinetics families this build runs (dual Monod,
+mass action, the examples' dissolution law) bound what those can be.  A run the
+path cannot take -- a finite-difference or LB biomass solver, FBA or surrogate
+reactions, more than one dissolvable phase -- is refused with the reason.
+
+readGeometry reads one x slice at a time through Palabos' stream operator,
+which walks y and then z fastest; the reference's generator (writeDat,
+complab3d_geometry.hh:260-270) writes x fastest.  `read_geometry_dat` follows
+the reader by default (order="slices") and takes the generator's order on
+request (order="x_fastest").
+"""
+import os
+import xml.etree.ElementTree as ET
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import params as P
+from .driver import FlowSetup, TransportSetup
+
+_MISSING = object()
+_TRUE = ("true", "1", "yes", "on")
+
+
+class InputError(ValueError):
+    """The input files describe a run this path cannot take, or are malformed."""
+
+
+class _Doc:
+    def __init__(self, root):
+        self.root = root
+
+    def text(self, *path, default=_MISSING):
+        node = self.root
+        for tag in path:
+            node = node.find(tag) if node is not None else None
+        if node is None or node.text is None or not node.text.strip():
+            if default is _MISSING:
+                raise InputError("<%s> is required" % "/".join(path))
+            return default
+        return node.text.strip()
+
+    def has(self, *path):
+        return self.text(*path, default=None) is not None
+
+    def num(self, *path, default=_MISSING, cast=float):
+        t = self.text(*path, default=None)
+        if t is None:
+            if default is _MISSING:
+                raise InputError("<%s> is required" % "/".join(path))
+            return default
+        try:
+            return cast(float(t.split()[0])) if cast is int else cast(t.split()[0])
+        except ValueError:
+            raise InputError("<%s> is not a number: %r" % ("/".join(path), t))
+
+    def nums(self, *path, default=_MISSING):
+        t = self.text(*path, default=None)
+        if t is None:
+            if default is _MISSING:
+                raise InputError("<%s> is required" % "/".join(path))
+            return default
+        return [float(x) for x in t.split()]
+
+    def flag(self, *path, default=False):
+        t = self.text(*path, default=None)
+        return default if t is None else t.lower() in _TRUE
+
+
+def read_geometry_dat(path, nx, ny, nz, order="slices"):
+    """The material field of an nx x ny x nz geometry.dat as an int8 array
+    (nz, ny, nx + 2): the two ghost x columns copy the first and last slices,
+    as readGeometry does (complab_functions.hh:178-207)."""
+    with open(path) as f:
+        vals = np.array(f.read().split(), dtype=np.int64)
+    if vals.size != nx * ny * nz:
+        raise InputError("%s holds %d values, the domain %dx%dx%d needs %d" % (path, vals.size, nx, ny, nz, nx * ny * nz))
+    if order == "slices":                       # x slowest, z fastest
+        core = vals.reshape(nx, ny, nz).transpose(2, 1, 0)
+    elif order == "x_fastest":
+        core = vals.reshape(nz, ny, nx)
+    else:
+        raise InputError("order must be 'slices' or 'x_fastest'")
+    g = np.empty((nz, ny, nx + 2), dtype=np.int64)
+    g[:, :, 1:-1] = core
+    g[:, :, 0] = core[:, :, 0]
+    g[:, :, -1] = core[:, :, -1]
+    return g
+
+
+def _btype(s, where):
+    s = s.lower()
+    if s == "dirichlet":
+        return P.BC_DIRICHLET
+    if s == "neumann":
+        return P.BC_NEUMANN
+    if s in ("closed", "noflux", "no_flux", "wall"):
+        return P.BC_CLOSED
+    raise InputError("%s (%s) should be Dirichlet, Neumann or closed" % (where, s))
+
+
+@dataclass
+class Run:
+    params: P.CaseParams
+    geometry: np.ndarray                        # (nz, ny, nx + 2) in this build's material codes
+    flow: FlowSetup
+    transport: TransportSetup
+    steps: int                                  # ade_max_iT
+    species: List[str]
+    microbes: List[str]
+    dx: float                                   # metres
+    diag_enabled: bool = False
+    diag_interval: int = 0
+    diag_tol: float = 1e-6
+    conserve: List[str] = field(default_factory=list)
+    materials: dict = field(default_factory=dict)   # file material number -> this build's code
+
+
+def load(xml_path, geometry_path: Optional[str] = None, precision=P.FP64, order="slices") -> Run:
+    """Reads CompLaB.xml (and the geometry file it names, under its
+    <input_path>, unless `geometry_path` is given) into a Run."""
+    doc = _Doc(ET.parse(xml_path).getroot())
+    base = os.path.dirname(os.path.abspath(xml_path))
+    biotic = doc.flag("simulation_mode", "biotic_mode", default=True)
+    if doc.flag("simulation_mode", "enable_fba_glpk") or doc.flag("simulation_mode", "enable_fba_cobrapy") \
+            or doc.flag("simulation_mode", "enable_surrogate"):
+        raise InputError("FBA and surrogate reactions are not on this path")
+
+    # ---- domain (complab_functions.hh:999-1003, 1195-1233)
+    nx = doc.num("LB_numerics", "domain", "nx", cast=int)
+    ny = doc.num("LB_numerics", "domain", "ny", cast=int)
+    nz = doc.num("LB_numerics", "domain", "nz", cast=int)
+    dx = doc.num("LB_numerics", "domain", "dx")
+    pe = doc.num("LB_numerics", "Peclet", default=0.0)
+    char_len = doc.num("LB_numerics", "domain", "characteristic_length", default=None)
+    if char_len is None:
+        if pe > 1e-14:
+            raise InputError("<characteristic_length> must be given when Peclet > 0")
+        char_len = 0.0
+    unit = doc.text("LB_numerics", "domain", "unit", default="um")
+    scale = {"m": 1.0, "mm": 1e-3, "um": 1e-6}.get(unit)
+    if scale is None:
+        raise InputError("unit (%s) must be either m, mm, or um" % unit)
+    char_len /= dx
+    dx_m = dx * scale
+    delta_p = doc.num("LB_numerics", "delta_P", default=0.0)
+    tau = doc.num("LB_numerics", "tau", default=0.8)
+    it = ("LB_numerics", "iteration")
+    ns_max1 = doc.num(*it, "ns_max_iT1", default=100000, cast=int)
+    ns_max2 = doc.num(*it, "ns_max_iT2", default=100000, cast=int)
+    ns_eps1 = doc.num(*it, "ns_converge_iT1", default=1e-8)
+    ns_eps2 = doc.num(*it, "ns_converge_iT2", default=1e-6)
+    ns_upd = doc.num(*it, "ns_update_interval", default=1, cast=int)
+    ade_upd = doc.num(*it, "ade_update_interval", default=1, cast=int)
+    steps = doc.num(*it, "ade_max_iT", default=10000000, cast=int)
+
+    # ---- chemistry (complab_functions.hh:1006-1027, 1280-1295; complab.cpp:440-458)
+    S = doc.num("chemistry", "number_of_substrates", cast=int)
+    if not 1 <= S <= P.MAX_SPECIES:
+        raise InputError("number_of_substrates %d outside 1..%d" % (S, P.MAX_SPECIES))
+    p = P.default_params(nx + 2, ny, nz, S)
+    p.precision = precision
+    names, d_pore, d_bio = [], [], []
+    for s in range(S):
+        tag = "substrate%d" % s
+        names.append(doc.text("chemistry", tag, "name_of_substrates", default=tag))
+        p.c0[s] = doc.num("chemistry", tag, "initial_concentration", default=0.0)
+        d_pore.append(doc.num("chemistry", tag, "substrate_diffusion_coefficients", "in_pore", default=0.0))
+        d_bio.append(doc.num("chemistry", tag, "substrate_diffusion_coefficients", "in_biofilm", default=0.0))
+        p.left_btype[s] = _btype(doc.text("chemistry", tag, "left_boundary_type"), tag + " left_boundary_type")
+        p.right_btype[s] = _btype(doc.text("chemistry", tag, "right_boundary_type"), tag + " right_boundary_type")
+        p.left_bc[s] = doc.num("chemistry", tag, "left_boundary_condition")
+        p.right_bc[s] = doc.num("chemistry", tag, "right_boundary_condition")
+        p.mobile[s] = 0 if doc.flag("chemistry", tag, "immobile") else 1
+
+    # ---- materials (complab_functions.hh:1217-1233)
+    pore_mats = [int(x) for x in doc.nums("LB_numerics", "domain", "material_numbers", "pore", default=[2])]
+    solid_mat = doc.num("LB_numerics", "domain", "material_numbers", "solid", default=0, cast=int)
+    bb_mat = doc.num("LB_numerics", "domain", "material_numbers", "bounce_back", default=1, cast=int)
+    mats = {solid_mat: P.SOLID, bb_mat: P.WALL}
+    for m in pore_mats:
+        mats[m] = P.PORE
+
+    # ---- microbiology (complab_functions.hh:1036-1440)
+    microbes = []
+    M = doc.num("microbiology", "number_of_microbes", default=0, cast=int) if biotic else 0
+    if M > P.MAX_MICROBES:
+        raise InputError("%d microbes, this path holds %d" % (M, P.MAX_MICROBES))
+    perm = []
+    for m in range(M):
+        tag = "microbe%d" % m
+        solver = doc.text("microbiology", tag, "solver_type").lower()
+        if solver not in ("ca", "cellular automata", "cellular_automata"):
+            raise InputError("%s: solver_type %s; only cellular-automaton biofilms are on this path" % (tag, solver))
+        rt = doc.text("microbiology", tag, "reaction_type", default="kinetics").lower()
+        if rt not in ("kinetics", "none"):
+            raise InputError("%s: reaction_type %s is not on this path" % (tag, rt))
+        microbes.append(doc.text("microbiology", tag, "name_of_microbes", default=tag))
+        b0 = doc.nums("microbiology", tag, "initial_densities")
+        bio_mats = doc.nums("LB_numerics", "domain", "material_numbers", tag, default=None)
+        if bio_mats is None:
+            raise InputError("%s has no material number: planktonic biomass is not on this path" % tag)
+        if len(bio_mats) != len(b0):
+            raise InputError("%s: %d material numbers for %d initial densities" % (tag, len(bio_mats), len(b0)))
+        if len(set(b0)) != 1:
+            raise InputError("%s: one initial density per microbe on this path" % tag)
+        for bm in bio_mats:
+            mats[int(bm)] = P.BIOFILM + m
+        p.b0[m] = b0[0]
+        p.kdecay[m] = doc.num("microbiology", tag, "decay_coefficient", default=0.0)
+        ratio = doc.num("microbiology", tag, "viscosity_ratio_in_biofilm", default=None)
+        if ratio is None:
+            raise InputError("%s: viscosity_ratio_in_biofilm must be given for a CA biofilm" % tag)
+        perm.append(1.0 / ratio if ratio > 1e-14 else -99.0)
+    p.n_microbes = M
+    if M > 0:
+        if len(set(perm)) != 1:
+            raise InputError("one biofilm viscosity ratio for every microbe on this path")
+        p.perm_ratio = perm[0] if perm[0] > 0 else 0.0
+        p.thrd_bfilm_frac = doc.num("microbiology", "thrd_biofilm_fraction")
+        p.bmax = doc.num("microbiology", "maximum_biomass_density")
+        ca = doc.text("microbiology", "CA_method", default="fraction").lower()
+        if ca in ("fraction", "0", "no"):
+            p.ca_method = P.CA_FRACTION
+        elif ca in ("half", "1", "yes"):
+            p.ca_method = P.CA_HALF
+        else:
+            raise InputError("CA_method (%s) should be either half or fraction" % ca)
+
+    # ---- flow
+    p.ns_tau = tau
+    p.delta_p = delta_p
+    p.flow_on = 1 if pe > 1e-14 else 0
+    p.ns_update_interval = ns_upd
+    p.ade_update_interval = ade_upd
+    p.ns_max_iter = ns_max2
+    p.ns_conv_rule = P.CONV_TRACER
+    p.ns_conv_eps = ns_eps2
+    flow = FlowSetup(pe=pe, char_length=char_len, delta_p=delta_p, ns_tau=tau, max_iter=ns_max1,
+                     rule=P.CONV_TRACER, eps=ns_eps1, has_biofilm=M > 0)
+    transport = TransportSetup(d_pore=d_pore, d_biofilm=d_bio, dx=dx_m)
+
+    # ---- simulation mode (complab_functions.hh:865-990)
+    p.abiotic_rate_scale = doc.num("simulation_mode", "abiotic_rate_scale", default=1.0)
+
+    # ---- equilibrium (complab_functions.hh:1484-1582; complab.cpp:609-618)
+    if doc.flag("equilibrium", "enabled"):
+        comps = doc.text("equilibrium", "components", default="").split()
+        nc = len(comps)
+        if not 1 <= nc <= P.MAX_COMP:
+            raise InputError("equilibrium needs 1..%d components" % P.MAX_COMP)
+        p.eq_on = 1
+        p.eq_solver = P.EQ_AAPCF
+        p.eq_nc = nc
+        p.eq_tol = 1e-10
+        p.eq_max_iter = 200
+        p.eq_anderson_depth = 4
+        rows = []
+        for s in range(S):
+            row = doc.nums("equilibrium", "stoichiometry", "species%d" % s, default=None)
+            if row is None or len(row) != nc:
+                row = [0.0] * nc
+            rows.append(row)
+            for j in range(nc):
+                p.eq_stoich[s][j] = row[j]
+            p.eq_logk[s] = doc.num("equilibrium", "logK", "species%d" % s, default=0.0)
+        for j, cname in enumerate(comps):
+            k = names.index(cname) if cname in names else -1
+            if k < 0:
+                unit_rows = [s for s in range(S) if all(rows[s][i] == (1.0 if i == j else 0.0) for i in range(nc))]
+                k = unit_rows[0] if unit_rows else -1
+            p.eq_comp_species[j] = k
+
+    # ---- precipitation (complab.cpp:485-503)
+    if doc.flag("precipitation", "enabled"):
+        p.precip_on = 1
+        p.precip_species = doc.num("precipitation", "solid_substrate", default=-1, cast=int)
+        p.precip_max_rho = doc.num("precipitation", "max_precipRho", default=1e30)
+        p.surface_only = doc.num("precipitation", "surface_only", default=1, cast=int)
+        p.precip_perm_ratio = doc.num("precipitation", "perm_ratio", default=0.0)
+        p.precip_interval = doc.num("precipitation", "update_interval", default=200, cast=int)
+
+    # ---- dissolution (complab.cpp:516-574): one declared phase on this path
+    if doc.flag("dissolution", "enabled"):
+        phases = []
+        for k in range(64):
+            if not doc.has("dissolution", "phase%d" % k, "material_number"):
+                break
+            phases.append(k)
+        if not phases:
+            raise InputError("<dissolution> is enabled but no <phase0> block was given")
+        if len(phases) > 1:
+            raise InputError("%d dissolvable phases; this path takes one" % len(phases))
+        ph = ("dissolution", "phase0")
+        p.dissol_on = 1
+        mat = doc.num(*ph, "material_number", cast=int)
+        p.dissol_is_precipitate = 1 if doc.flag(*ph, "is_precipitate") else 0
+        if not p.dissol_is_precipitate and mats.get(mat) not in (P.SOLID, P.WALL):
+            raise InputError("a grain phase lies on the solid or bounce-back material on this path")
+        p.dissol_material = mats.get(mat, -1)
+        p.dissol_species = doc.num(*ph, "substrate", default=-1, cast=int)
+        p.dissol_full = doc.num(*ph, "full_density", default=0.0)
+        p.dissol_initial = doc.num(*ph, "initial_fill", default=0.0)
+        p.dissol_phase_id = 1
+        p.dissol_reopen_frac = doc.num("dissolution", "reopen_fraction", default=0.9)
+        p.dissol_surface_only = doc.num("dissolution", "surface_only", default=1, cast=int)
+        p.dissol_interval = doc.num("dissolution", "update_interval", default=100, cast=int)
+        if not 0 <= p.dissol_species < S:
+            raise InputError("dissolution substrate %d is outside 0..%d" % (p.dissol_species, S - 1))
+        if not p.dissol_full > 0:
+            raise InputError("<full_density> must be positive")
+        if not 0 < p.dissol_reopen_frac < 1:
+            raise InputError("<reopen_fraction> must be strictly between 0 and 1")
+        if p.dissol_is_precipitate:
+            p.precip_phase_id = 1
+
+    # ---- diagnostics (complab3d_integration.hh:375-423)
+    run_diag = doc.flag("diagnostics", "enabled")
+    conserve = []
+    if run_diag:
+        for k in range(32):
+            e = doc.text("diagnostics", "conserve" if k == 0 else "conserve%d" % k, default=None)
+            if e is None:
+                break
+            conserve.append(e)
+
+    # ---- geometry
+    if geometry_path is None:
+        gdir = doc.text("path", "input_path", default="input")
+        geometry_path = os.path.join(base, gdir, doc.text("LB_numerics", "domain", "filename"))
+    raw = read_geometry_dat(geometry_path, nx, ny, nz, order=order)
+    unknown = sorted(set(np.unique(raw).tolist()) - set(mats))
+    if unknown:
+        raise InputError("geometry holds material numbers %s that the XML does not declare" % unknown)
+    geo = np.empty(raw.shape, dtype=np.int8)
+    for src, dst in mats.items():
+        geo[raw == src] = dst
+
+    return Run(params=p, geometry=geo, flow=flow, transport=transport, steps=steps, species=names,
+               microbes=microbes, dx=dx_m, diag_enabled=run_diag,
+               diag_interval=doc.num("diagnostics", "interval", default=0, cast=int),
+               diag_tol=doc.num("diagnostics", "tolerance", default=1e-6), conserve=conserve,
+               materials=dict(mats))

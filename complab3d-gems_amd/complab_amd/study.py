@@ -22,10 +22,14 @@ from . import configs, driver
 from . import params as P
 
 
-def run(ctx, w, steps, chunk=500):
+def run(ctx, w, steps, chunk=500, log=None):
+    """Set-up and `steps` coupled steps of workload w; with `log`, a line per
+    20 000 flow iterations and per `chunk` steps."""
     c = Case(ctx, w.params, w.geometry)
     t0 = time.perf_counter()
-    rep = driver.setup(c, w.params, w.flow, w.transport)
+    say = (lambda m: print("%s: %s" % (w.name.split()[0], m), file=log, flush=True)) if log else None
+    rep = driver.setup(c, w.params, w.flow, w.transport, chunk=20000 if log else 0,
+                       progress=(lambda n: say("flow %d iterations" % n)) if log else None)
     c.set_omegas(rep.omega_pore, rep.omega_biofilm, 1.0)
     t1 = time.perf_counter()
     done = 0
@@ -34,6 +38,8 @@ def run(ctx, w, steps, chunk=500):
         rc, n = c.run_steps(min(chunk, steps - done))
         done += n
         totals.append((done, c.species_totals()))
+        if say:
+            say("%d steps" % done)
         if rc:
             raise RuntimeError("%s stopped with status %d after %d steps" % (w.name, rc, done))
     ctx.sync()
@@ -67,16 +73,20 @@ def main():
     ap.add_argument("--scale", type=float, default=0.25)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--precisions", default="fp32_ade,fp32", help="reduced precisions to compare, of fp32_ade and fp32")
     args = ap.parse_args()
     ctx = Context(0)
     res = {"steps": args.steps, "scale": args.scale}
-    hi = run(ctx, configs.config3(args.scale, P.FP64, args.seed), args.steps)
+    hi = run(ctx, configs.config3(args.scale, P.FP64, args.seed), args.steps, log=sys.stderr)
     print("fp64 done: setup %.1f s, steps %.1f s" % (hi["setup_s"], hi["steps_s"]), file=sys.stderr, flush=True)
     res["workload"] = configs.config3(args.scale, P.FP64, args.seed).name
     res["fp64"] = {"setup_s": round(hi["setup_s"], 2), "steps_s": round(hi["steps_s"], 2),
                    "flow_iters": hi["flow_iters"], "counters": [int(x) for x in hi["counters"]]}
+    chosen = [x for x in args.precisions.split(",") if x]
     for label, prec in (("fp32_ade", P.FP32_ADE), ("fp32", P.FP32)):
-        lo = run(ctx, configs.config3(args.scale, prec, args.seed), args.steps)
+        if label not in chosen:
+            continue
+        lo = run(ctx, configs.config3(args.scale, prec, args.seed), args.steps, log=sys.stderr)
         r = compare(lo, hi, args.steps)
         r.update({"setup_s": round(lo["setup_s"], 2), "steps_s": round(lo["steps_s"], 2),
                   "counters": [int(x) for x in lo["counters"]]})
