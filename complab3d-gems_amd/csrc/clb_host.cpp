@@ -1121,10 +1121,13 @@ int checkParams(clb_ctx *ctx, const clb_case_params *p, const int8_t *geometry) 
     }
     const long long N = (long long) p->nx * p->ny * p->nz;
     // a no-dynamics grain voxel must not touch open water through a face
+    long long open = 0;
     for (long long i = 0; i < N; ++i) {
         int g = geometry[i];
         if (g < 0 || g > CLB_BIOFILM + CLB_MAX_MICROBES + 2) return fail(ctx, CLB_E_GEOMETRY, "unknown material code in geometry");
+        if (g >= CLB_PORE) ++open;
     }
+    if (open == 0) return fail(ctx, CLB_E_GEOMETRY, "the geometry has no open voxel");
     // (a dissolvable grain of solid material is meant to touch the water)
     const bool solidGrainPhase = p->dissol_on && !p->dissol_is_precipitate && p->dissol_material == CLB_SOLID;
     for (int z = 0; z < p->nz && !solidGrainPhase; ++z) for (int y = 0; y < p->ny; ++y) for (int x = 0; x < p->nx; ++x) {
