@@ -108,6 +108,7 @@ struct NsDev {
     const int *nb;                 // [18][Nopen]
     R *cur, *out;                  // [19][Nopen]
     const uint8_t *cls, *clsPull;  // [Nopen]
+    const unsigned long long *kinds;  // [Nint] source kind of each moving population, 2 bits per direction
     double om, omBio, rhoBarRight;
     int first;
     int *done;
@@ -176,6 +177,7 @@ template <typename R> hipError_t launchAdeReopen(const AdeDev<R> &, const uint8_
 template <typename R> hipError_t launchAdeTotal(const AdeDev<R> &, int, double *, hipStream_t);
 template <typename R> hipError_t launchAdeHeld(const AdeDev<R> &, int, double *, hipStream_t);
 hipError_t launchAdeKinds(const int *nb, const uint8_t *clsPull, int Nopen, int Nint, uint16_t *kinds, hipStream_t);
+hipError_t launchNsKinds(const int *nb, const uint8_t *clsPull, int Nopen, int Nint, unsigned long long *kinds, hipStream_t);
 template <typename R> hipError_t launchNsStep(const NsDev<R> &, double *, double *, NsTracer *, int, double, hipStream_t);
 template <typename R, typename RA> hipError_t launchNsVelocity(const NsDev<R> &, RA *, double *, hipStream_t);
 template <typename R> hipError_t launchNsInit(const NsDev<R> &, double, hipStream_t);

@@ -97,6 +97,7 @@ struct Case : CaseBase {
     int *list = nullptr, *nb7 = nullptr, *nb19 = nullptr;
     uint8_t *adeCls = nullptr, *adeClsPull = nullptr, *nsCls = nullptr, *nsClsPull = nullptr, *reopened = nullptr;
     uint16_t *adeKinds = nullptr;
+    unsigned long long *nsKinds = nullptr;
     R *fA = nullptr, *fB = nullptr, *imm = nullptr, *B = nullptr, *Btot = nullptr, *outbox = nullptr;
     uint8_t *pushed = nullptr;
     int *age = nullptr, *age2 = nullptr, *ageList = nullptr;
@@ -218,7 +219,7 @@ struct Case : CaseBase {
         a.nx = nx; a.ny = ny; a.nz = nz; a.N = N;
         a.Nopen = Nopen; a.Nint = Nint; a.Nleft = Nleft; a.list = list; a.nb = nb19; a.geom = geom;
         a.cur = gCur(); a.out = gOut();
-        a.cls = nsCls; a.clsPull = nsClsPull;
+        a.cls = nsCls; a.clsPull = nsClsPull; a.kinds = nsKinds;
         a.om = nsOmega; a.omBio = nsBioOmega; a.rhoBarRight = -deltaP * 3.0;
         a.first = nsFirst ? 1 : 0;
         a.done = nsDone;
@@ -336,6 +337,7 @@ struct Case : CaseBase {
         if (p.flow_on) {
             gA = dalloc<RN>(allocs, 19 * (size_t) Nopen); gB = dalloc<RN>(allocs, 19 * (size_t) Nopen);
             nb19 = dalloc<int>(allocs, (size_t) 18 * Nopen);
+            nsKinds = dalloc<unsigned long long>(allocs, std::max(Nint, 1));
         }
         u = dalloc<R>(allocs, 3 * (size_t) Nopen);
         rhoN = dalloc<double>(allocs, (size_t) std::max(Sm, 1) * std::max(Nopen - Nint, 1));
@@ -397,6 +399,7 @@ struct Case : CaseBase {
         HIPT(launchAdeKinds(nb7, adeClsPull, Nopen, Nint, adeKinds, st));
         HIPT(hipMemcpyAsync(nsCls, nc.data(), Nopen, hipMemcpyHostToDevice, st));
         HIPT(hipMemcpyAsync(nsClsPull, nc.data(), Nopen, hipMemcpyHostToDevice, st));
+        if (p.flow_on) HIPT(launchNsKinds(nb19, nsClsPull, Nopen, Nint, nsKinds, st));
         HIPT(hipMemcpyAsync(age, ag.data(), N * sizeof(int), hipMemcpyHostToDevice, st));
         HIPT(hipMemcpyAsync(age2, ag.data(), N * sizeof(int), hipMemcpyHostToDevice, st));
         HIPT(hipMemsetAsync(reopened, 0, Nopen, st));
@@ -474,6 +477,7 @@ struct Case : CaseBase {
         for (int i = 0; i < Nopen; ++i) nc[i] = nsClassOf(g[lst[i]]);
         HIPT(hipMemcpyAsync(nsCls, nc.data(), Nopen, hipMemcpyHostToDevice, st));
         HIPT(hipMemcpyAsync(nsClsPull, nc.data(), Nopen, hipMemcpyHostToDevice, st));
+        HIPT(launchNsKinds(nb19, nsClsPull, Nopen, Nint, nsKinds, st));
         nsBioOmega = 1.0 / (p.perm_ratio * (1.0 / nsOmega - 0.5) + 0.5);
         gCurIsA = true;
         HIPT(launchNsInit<RN>(ns(), dP, st));
@@ -507,7 +511,11 @@ struct Case : CaseBase {
                 a.cur = curA ? gA : gB; a.out = curA ? gB : gA;
                 HIPT(launchNsStep<RN>(a, partE, partN, tracer, rule, eps, st));
                 nsFirst = false;
-                if (!launched && nsClsDirty) { HIPT(hipMemcpyAsync(nsClsPull, nsCls, Nopen, hipMemcpyDeviceToDevice, st)); nsClsDirty = false; }
+                if (!launched && nsClsDirty) {
+                    HIPT(hipMemcpyAsync(nsClsPull, nsCls, Nopen, hipMemcpyDeviceToDevice, st));
+                    HIPT(launchNsKinds(nb19, nsClsPull, Nopen, Nint, nsKinds, st));
+                    nsClsDirty = false;
+                }
                 launched = true;
                 curA = !curA;
             }

@@ -841,6 +841,30 @@ __device__ __forceinline__ void pull19(const NsDev<R> &a, int i, T fl[19]) {
         else fl[q] = (T) a.cur[q * No + n];
     }
 }
+// the interior's pull from the precomputed source kinds: every neighbour
+// index first, then every population, so that the loads are in flight together
+template <typename R, typename T>
+__device__ __forceinline__ void pull19k(const NsDev<R> &a, int i, T fl[19]) {
+    const long long No = a.Nopen;
+    if (a.first) {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) fl[q] = (T) a.cur[q * No + i];
+        return;
+    }
+    int nbq[18];
+#pragma unroll
+    for (int q = 1; q < 19; ++q) nbq[q - 1] = a.nb[(q - 1) * No + i];
+    const unsigned long long kd = a.kinds[i];
+    fl[0] = (T) a.cur[i];
+#pragma unroll
+    for (int q = 1; q < 19; ++q) {
+        const unsigned int k = (unsigned int) (kd >> (2 * (q - 1))) & 3u;
+        const R *base = k == 2 ? a.out : a.cur;
+        const long long off = k == 0 ? (long long) q * No + nbq[q - 1] : (long long) opp19(q) * No + i;
+        fl[q] = (T) base[off];
+    }
+}
+
 template <typename R>
 __device__ __forceinline__ void nsFirstHandOver(const NsDev<R> &a, int i) {
     const long long No = a.Nopen;
@@ -903,7 +927,7 @@ __device__ __forceinline__ void nsBulkBlock(const NsDev<R> &a, double *partE, do
         const int cls = a.cls[i];
         if (cls < NS_BB) {
             R fl[19];                                  // compute type = storage type
-            pull19<R, R>(a, i, fl);
+            pull19k<R, R>(a, i, fl);
             if (a.first) nsFirstHandOver<R>(a, i);
             R rb = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
@@ -1245,6 +1269,24 @@ hipError_t launchAdeSet(const AdeDev<R> &a, int ms, const double *src, hipStream
 template <typename R>
 hipError_t launchAdeReopen(const AdeDev<R> &a, const uint8_t *re, hipStream_t st) {
     hipLaunchKernelGGL(k_ade_reopen<R>, dim3(nblk(a.Nopen, 256)), dim3(256), 0, st, a, re);
+    return hipGetLastError();
+}
+// the flow's counterpart: 0 neighbour, 1 grid edge, 2 bounce-back (a closed
+// neighbour or one whose pull class is bounce-back), 2 bits per D3Q19 direction
+__global__ void __launch_bounds__(256) k_ns_kinds(const int *nb, const uint8_t *clsPull, int Nopen, int Nint,
+                                                  unsigned long long *kinds) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nint) return;
+    unsigned long long code = 0;
+    for (int q = 1; q < 19; ++q) {
+        const int n = nb[(long long) (q - 1) * Nopen + i];
+        const unsigned long long k = n == NB_OUTSIDE ? 1ull : ((n == NB_CLOSED || clsPull[n] >= NS_BB) ? 2ull : 0ull);
+        code |= k << (2 * (q - 1));
+    }
+    kinds[i] = code;
+}
+hipError_t launchNsKinds(const int *nb, const uint8_t *clsPull, int Nopen, int Nint, unsigned long long *kinds, hipStream_t st) {
+    if (Nint > 0) hipLaunchKernelGGL(k_ns_kinds, dim3(nblk(Nint, 256)), dim3(256), 0, st, nb, clsPull, Nopen, Nint, kinds);
     return hipGetLastError();
 }
 hipError_t launchAdeKinds(const int *nb, const uint8_t *clsPull, int Nopen, int Nint, uint16_t *kinds, hipStream_t st) {
