@@ -77,19 +77,29 @@ def algorithmic_bytes(p, n_open, n_vox):
 
 
 def cpu_baseline(w, g, rep, steps):
+    """The oracle on the same case, velocity copied from the GPU flow solve:
+    `steps` coupled steps on every host thread the process has, and one step
+    on one thread.  Returns (MLUPS all threads, threads, steps, seconds,
+    MLUPS one thread)."""
     import oracle as O
     from complab_amd import params as P
     o = O.OracleCase(w.params, w.geometry)
     for a in range(3):
         o.write(P.FIELD_VEL, a, g.read(P.FIELD_VEL, a))
     o.set_omegas(rep.omega_pore, rep.omega_biofilm, 1.0)
+    threads = O.set_threads(0)
     o.run_steps(1)
+    V = w.params.nx * w.params.ny * w.params.nz
     t0 = time.perf_counter()
     rc, n = o.run_steps(steps)
     dt = time.perf_counter() - t0
-    V = w.params.nx * w.params.ny * w.params.nz
+    O.set_threads(1)
+    t1 = time.perf_counter()
+    o.run_steps(1)
+    dt1 = time.perf_counter() - t1
+    O.set_threads(threads)
     o.close()
-    return V * n / dt / 1e6, n, dt
+    return V * n / dt / 1e6, threads, n, dt, V / dt1 / 1e6
 
 
 def sweep_measurement(args):
@@ -151,7 +161,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stabilize", type=int, default=10000)
     ap.add_argument("--ns-max-iter", type=int, default=100000,
@@ -216,10 +226,12 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cv, cn, cdt = cpu_baseline(w, g, rep, args.cpu_steps)
-        cpu = {"value": round(cv, 4), "unit": "MLUPS", "cores": 1, "kind": "port",
-               "sample": "%d coupled steps of the same %dx%dx%d case by the scalar C++ oracle, "
-                         "velocity copied from the GPU flow solve (%.1f s)" % (cn, p.nx, p.ny, p.nz, cdt)}
+        cv, threads, cn, cdt, cv1 = cpu_baseline(w, g, rep, args.cpu_steps)
+        cpu = {"value": round(cv, 4), "unit": "MLUPS", "cores": threads, "kind": "port",
+               "value_one_thread": round(cv1, 4),
+               "sample": "%d coupled steps of the same %dx%dx%d case by the C++ oracle (OpenMP over voxels, "
+                         "%d threads; %.1f s), and one step on one thread; velocity copied from the GPU flow "
+                         "solve" % (cn, p.nx, p.ny, p.nz, threads, cdt)}
     sweep_out = serve_out = None
     if not args.no_sweep:
         sweep_out = sweep_measurement(args)

@@ -28,6 +28,7 @@
 #include <limits>
 #include <string>
 #include <vector>
+#include <omp.h>
 
 namespace {
 
@@ -147,6 +148,7 @@ void adeDirichlet(double *fl, int inward, double rhoD) {
 void adeCollide(Case &c, int s) {
     const size_t N = c.N;
     double *f = c.f[s].data();
+#pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 0; x < c.nx; ++x) {
         size_t i = c.id(x, y, z);
         int cls = adeClassFor(c, s, x, i);
@@ -178,6 +180,7 @@ void adeStream(Case &c, int s) {
     const size_t N = c.N;
     std::vector<double> nf(7 * N);
     double *f = c.f[s].data();
+#pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 0; x < c.nx; ++x) {
         size_t i = c.id(x, y, z);
         for (int q = 0; q < Q7; ++q) {
@@ -191,6 +194,7 @@ void adeStream(Case &c, int s) {
      * stream, the Neumann plane takes the density of the layer inside it.  An
      * inner cell that is bounce-back reports BounceBack's stored density, 0
      * (updateSoluteSolidDynamics3D installs BounceBack() with no density). */
+#pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) {
         size_t k = (size_t) y + (size_t) c.ny * z;
         size_t il = c.id(1, y, z), ir = c.id(c.nx - 2, y, z);
@@ -250,7 +254,11 @@ void abioticLaw(const clb_case_params &p, const double *C, double *subsR) {
 double clampInc(Case &c, double dC, double cur) {
     if (!c.p.clamp || dC >= 0) return dC;
     double lim = -std::max(cur, 0.0);
-    if (dC < lim) { c.counters[2]++; return lim; }
+    if (dC < lim) {
+#pragma omp atomic
+        c.counters[2]++;
+        return lim;
+    }
     return dC;
 }
 
@@ -260,6 +268,7 @@ void bioticStep(Case &c) {
     const clb_case_params &p = c.p;
     const int S = p.n_species, M = p.n_microbes;
     std::vector<double> dC((size_t) S * c.N, 0.0), dB((size_t) std::max(M,1) * c.N, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 1; x < c.nx - 1; ++x) {
         size_t i = c.id(x, y, z);
         int mask = c.mask[i];
@@ -271,6 +280,7 @@ void bioticStep(Case &c) {
         for (int s = 0; s < S; ++s) { double d = sr[s] * p.dt; if (d > THRD || d < -THRD) dC[(size_t) s * c.N + i] += d; }
         for (int m = 0; m < M; ++m) { double d = br[m] * p.dt; if (d > THRD || d < -THRD) dB[(size_t) m * c.N + i] += d; }
     }
+#pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 1; x < c.nx - 1; ++x) {
         size_t i = c.id(x, y, z);
         if (c.solidOrBB(c.mask[i])) continue;
@@ -401,6 +411,7 @@ void abioticStep(Case &c) {
     const int S = p.n_species;
     std::vector<double> dC((size_t) S * c.N, 0.0);
     if (p.n_rxn > 0) {
+#pragma omp parallel for collapse(2) schedule(static)
         for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 1; x < c.nx - 1; ++x) {
             size_t i = c.id(x, y, z);
             int mask = c.mask[i];
@@ -424,6 +435,7 @@ void abioticStep(Case &c) {
         dissolutionPark(c, dC);
         dissolutionGather(c, dC);
     }
+#pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 1; x < c.nx - 1; ++x) {
         size_t i = c.id(x, y, z);
         int mask = c.mask[i];
@@ -654,6 +666,7 @@ struct Eq {
 void equilibriumStep(Case &c) {
     const clb_case_params &p = c.p;
     Eq eq{p.n_species, p.eq_nc, &p};
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
     for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 1; x < c.nx - 1; ++x) {
         size_t i = c.id(x, y, z);
         if (c.solidOrBB(c.mask[i])) continue;
@@ -661,7 +674,10 @@ void equilibriumStep(Case &c) {
         for (int s = 0; s < p.n_species; ++s) C[s] = std::max(conc(c, s, i), MIN_CONC);
         int it; double res; bool conv;
         eq.solve(C, E, it, res, conv);
-        if (!conv) c.counters[3]++;
+        if (!conv) {
+#pragma omp atomic
+            c.counters[3]++;
+        }
         for (int s = 0; s < p.n_species; ++s) {
             double dC = E[s] - C[s];
             double mx = std::max(1e-9, 0.1 * std::fabs(C[s]));
@@ -920,7 +936,11 @@ double nsRhoBarRight(const Case &c) { return -c.deltaP * 3.0; }   // 1 - deltaP*
  * (the statistic getStoredAverageEnergy reports) */
 double nsCollideStream(Case &c) {
     const size_t N = c.N;
-    double esum = 0; long long en = 0;
+    // the energy of each cell, summed afterwards in cell order, so that threads
+    // do not change the sum
+    std::vector<double> ecell(N, 0.0);
+    std::vector<char> ehas(N, 0);
+#pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 0; x < c.nx; ++x) {
         size_t i = c.id(x, y, z);
         int cls = c.nsCls[i];
@@ -938,12 +958,15 @@ double nsCollideStream(Case &c) {
                 for (int q = 0; q < Q19; ++q) { rb += fl[q]; for (int a = 0; a < 3; ++a) j[a] += C19[q][a] * fl[q]; }
             }
             double jsq = j[0]*j[0] + j[1]*j[1] + j[2]*j[2];
-            esum += 0.5 * jsq; ++en;
+            ecell[i] = 0.5 * jsq; ehas[i] = 1;
             for (int q = 0; q < Q19; ++q) fl[q] += om * (feqInc(q, rb, j, jsq) - fl[q]);
         }
         for (int q = 0; q < Q19; ++q) c.g[(size_t) q * N + i] = fl[q];
     }
+    double esum = 0; long long en = 0;
+    for (size_t i = 0; i < N; ++i) if (ehas[i]) { esum += ecell[i]; ++en; }
     std::vector<double> ng(19 * N);
+#pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 0; x < c.nx; ++x) {
         size_t i = c.id(x, y, z);
         for (int q = 0; q < Q19; ++q) {
@@ -1544,6 +1567,10 @@ int orc_rates(const clb_case_params *p, const double *C, const double *B, int ma
     return 0;
 }
 
+/* threads of the per-cell loops (n <= 0 leaves them); returns the count in use.
+ * Results do not depend on it: every loop writes per cell, and the sums run
+ * in cell order on one thread */
+int orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); return omp_get_max_threads(); }
 int orc_rng_draws(uint64_t seed, int n, uint64_t *out) { Rng r(seed); for (int k = 0; k < n; ++k) out[k] = r.next(); return 0; }
 
 /* complab_geom::generate "spheres" (complab3d_geometry.hh:138-160) + addWalls (89-102) */

@@ -60,6 +60,7 @@ def lib():
             ("orc_eq_voxel", [P, P, P]),
             ("orc_rates", [P, P, P, C.c_int, P, P, P]),
             ("orc_rng_draws", [C.c_uint64, C.c_int, P]),
+            ("orc_set_threads", [C.c_int]),
             ("orc_geom_spheres", [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64, C.c_int, P]),
             ("orc_geom_pack", [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint64, C.c_int, P, P]),
         ]:
@@ -196,6 +197,11 @@ class OracleCase:
         out = np.zeros(12, dtype=np.int64)
         self._chk(lib().orc_counters(self.h, _ptr(out)))
         return out
+
+
+def set_threads(n=0):
+    """Threads of the oracle's per-cell loops (n <= 0 only reports the count)."""
+    return int(lib().orc_set_threads(int(n)))
 
 
 def eq_solve(params, conc):

@@ -338,3 +338,34 @@ def test_precipitation_converts_at_threshold():
     assert np.all(phase[~conv] == 0)
     assert not o.precip_convert()          # nothing left above the threshold
     o.close()
+
+
+def test_threads_do_not_change_results():
+    """The oracle's per-cell loops run on every thread: one thread and all of
+    them give bit-identical fields, totals and counters (biotic with CA and
+    flow re-solves, so every threaded loop and the CA run)."""
+    w = configs.config2(scale=0.08, packer=O.geom_pack)
+    w.transport.stabilize_steps = 5
+    w.params.ns_update_interval = 5
+    w.params.ns_max_iter = 500
+    w.params.b0[0] = 95.0
+    w.params.mu_max[0] = 2e-2
+    w.flow.max_iter = 500
+    out = []
+    n_all = O.set_threads(0)
+    for n in (1, n_all):
+        O.set_threads(n)
+        o = O.OracleCase(w.params, w.geometry)
+        rep = driver.setup(o, w.params, w.flow, w.transport)
+        o.set_omegas(rep.omega_pore, rep.omega_biofilm, 1.0)
+        assert o.run_steps(30) == (0, 30)
+        out.append((rep.iters, [o.read(P.FIELD_CONC, s) for s in range(4)], o.read(P.FIELD_BIOMASS, 0),
+                    o.read(P.FIELD_MASK), o.counters(), o.read(P.FIELD_VEL, 0)))
+    O.set_threads(n_all)
+    a, b = out
+    assert a[4][6] > 0 and a[4][4] > 0                    # the CA and flow re-solves ran
+    assert a[0] == b[0]
+    for x, y in zip(a[1], b[1]):
+        assert np.array_equal(x, y)
+    for k in (2, 3, 4, 5):
+        assert np.array_equal(a[k], b[k])
