@@ -25,6 +25,14 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    # the C++ example that drives a case through clb_driver.hh alone
+    exe = os.path.join(ROOT, "clb_run_case")
+    cmd = ["g++", "-O2", "-std=c++11", "-Wall", "-I" + os.path.join(REPO, "include"),
+           os.path.join(CSRC, "clb_run_case.cpp"), "-o", exe + ".tmp", "-L" + ROOT, "-lclb", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("clb_run_case failed to build:\n" + r.stdout + r.stderr)
+    os.replace(exe + ".tmp", exe)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", exe)
     return LIB

@@ -167,3 +167,13 @@ def test_processor_facade_compiles_and_links(tmp_path):
     assert r.returncode == 0, r.stderr
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and r.stdout.strip() == "0", r.stdout + r.stderr
+
+
+def test_cpp_example_is_built_and_reports_usage():
+    """The C++ driver example builds with the library and refuses bad
+    arguments with a usage line (no device needed)."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(clb.LIB_PATH), "clb_run_case")
+    assert os.path.exists(exe), "clb_run_case is built with the library"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "usage" in r.stderr
