@@ -102,6 +102,46 @@ def cpu_baseline(w, g, rep, steps):
     return V * n / dt / 1e6, threads, n, dt, V / dt1 / 1e6
 
 
+def abc_one_thread(steps=20):
+    """The oracle on one thread on an ABC-style transport step -- 32^3 voxels
+    (sphere pack with y/z walls), 3 D3Q7 species, A + B -> C by second-order
+    mass action -- for the reference's published 0.050 s per step of that
+    case on one core.  Seconds per coupled step."""
+    import oracle as O
+    from complab_amd import configs
+    from complab_amd import params as P
+    lat, _ = configs._pack(32, 32, 32, 3.0, 0.36, 1036)
+    p = P.default_params(34, 32, 32, 3)
+    for s in range(3):
+        p.left_btype[s] = P.BC_DIRICHLET; p.right_btype[s] = P.BC_NEUMANN
+    p.left_bc[0] = 1.0; p.c0[1] = 1.0
+    p.n_rxn = 1; p.rxn_r1[0] = 0; p.rxn_r2[0] = 1; p.rxn_k[0] = 1e-2
+    p.rxn_stoich[0][0] = -1.0; p.rxn_stoich[0][1] = -1.0; p.rxn_stoich[0][2] = 1.0
+    threads = O.set_threads(0)
+    O.set_threads(1)
+    try:
+        o = O.OracleCase(p, lat)
+        o.set_omegas([1.2] * 3, [1.2] * 3, 1.0)
+        o.run_steps(2)
+        t0 = time.perf_counter()
+        o.run_steps(steps)
+        dt = (time.perf_counter() - t0) / steps
+        o.close()
+    finally:
+        O.set_threads(threads)
+    return dt
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def sweep_measurement(args):
     from complab_amd import configs, sweep
     from complab_amd import params as P
@@ -227,8 +267,11 @@ def main():
     cpu = None
     if not args.no_cpu:
         cv, threads, cn, cdt, cv1 = cpu_baseline(w, g, rep, args.cpu_steps)
+        abc = abc_one_thread()
         cpu = {"value": round(cv, 4), "unit": "MLUPS", "cores": threads, "kind": "port",
-               "value_one_thread": round(cv1, 4),
+               "value_one_thread": round(cv1, 4), "cpu_model": cpu_model(),
+               "abc_32cube_3species_s_per_step_one_thread": round(abc, 5),
+               "reference_published_abc_s_per_step_one_core": 0.050,
                "sample": "%d coupled steps of the same %dx%dx%d case by the C++ oracle (OpenMP over voxels, "
                          "%d threads; %.1f s), and one step on one thread; velocity copied from the GPU flow "
                          "solve" % (cn, p.nx, p.ny, p.nz, threads, cdt)}
@@ -253,6 +296,8 @@ def main():
                    "species": p.n_species, "device": dev,
                    "flow_resolves": int(c1[4] - c0[4]), "ca_sweeps": int(c1[6] - c0[6]),
                    "mask_changes": int(c1[5] - c0[5])},
+        "units": {"mflups_open": round(n_open * args.steps / t / 1e6, 2),
+                  "lattice_mlups": round(V * args.steps * sum(1 for s in range(p.n_species) if p.mobile[s]) / t / 1e6, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(p, n_open),
                      "kernel": "k_ade_bulk (fused stream+collide+kinetics of the interior voxels)",
