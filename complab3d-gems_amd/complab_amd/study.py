@@ -73,11 +73,23 @@ def main():
     ap.add_argument("--scale", type=float, default=0.25)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--precisions", default="fp32_ade,fp32", help="reduced precisions to compare, of fp32_ade and fp32")
+    ap.add_argument("--precisions", default="fp32_ade,fp32",
+                    help="reduced precisions to compare, of fp32_ade and fp32 (empty: the FP64 run alone)")
+    ap.add_argument("--flow-max-iter", type=int, default=0,
+                    help="cap on every flow solve, the first and the re-solves (0: the workload's own); "
+                         "for kernel profiles, where the converged flow does not matter")
     args = ap.parse_args()
+
+    def workload(prec):
+        w = configs.config3(args.scale, prec, args.seed)
+        if args.flow_max_iter > 0:
+            w.flow.max_iter = args.flow_max_iter
+            w.params.ns_max_iter = args.flow_max_iter
+        return w
+
     ctx = Context(0)
-    res = {"steps": args.steps, "scale": args.scale}
-    hi = run(ctx, configs.config3(args.scale, P.FP64, args.seed), args.steps, log=sys.stderr)
+    res = {"steps": args.steps, "scale": args.scale, "flow_max_iter": args.flow_max_iter}
+    hi = run(ctx, workload(P.FP64), args.steps, log=sys.stderr)
     print("fp64 done: setup %.1f s, steps %.1f s" % (hi["setup_s"], hi["steps_s"]), file=sys.stderr, flush=True)
     res["workload"] = configs.config3(args.scale, P.FP64, args.seed).name
     res["fp64"] = {"setup_s": round(hi["setup_s"], 2), "steps_s": round(hi["steps_s"], 2),
@@ -86,7 +98,7 @@ def main():
     for label, prec in (("fp32_ade", P.FP32_ADE), ("fp32", P.FP32)):
         if label not in chosen:
             continue
-        lo = run(ctx, configs.config3(args.scale, prec, args.seed), args.steps, log=sys.stderr)
+        lo = run(ctx, workload(prec), args.steps, log=sys.stderr)
         r = compare(lo, hi, args.steps)
         r.update({"setup_s": round(lo["setup_s"], 2), "steps_s": round(lo["steps_s"], 2),
                   "counters": [int(x) for x in lo["counters"]]})

@@ -21,6 +21,11 @@ __device__ __forceinline__ double w7(int q) { return q == 0 ? W7R : W7F; }
 __device__ __forceinline__ double nanMax(double a, double b) {
     return (isnan(a) || isnan(b)) ? __builtin_nan("") : fmax(a, b);
 }
+// a queued step batch was halted by an earlier step: the kernel does nothing
+// (the flag is written by an earlier kernel of the stream, never during this one)
+__device__ __forceinline__ bool halted(const int *h) {
+    return h && *h;
+}
 __device__ __forceinline__ int opp7(int q) { return q == 0 ? 0 : ((q & 1) ? q + 1 : q - 1); }
 __device__ __forceinline__ int opp19(int q) { return q == 0 ? 0 : ((q & 1) ? q + 1 : q - 1); }
 __device__ __forceinline__ double w19(int q) { return q == 0 ? (1.0 / 3.0) : (q < 7 ? (1.0 / 18.0) : (1.0 / 36.0)); }

@@ -151,10 +151,12 @@ struct CaDev {
 #define EQ_MAXD 4
 #define EQ_MIN_CONC 1e-30
 #define EQ_MAX_CONC 10.0
+#define EQ_SOLVER_AAPCF 0
+#define EQ_SOLVER_NEWTON 1
 
 struct EqDev {
     int ns, nc;
-    int solver;                    // 0 AA-PCF, 1 Newton
+    int solver;                    // EQ_SOLVER_AAPCF or EQ_SOLVER_NEWTON
     double S[EQ_MAXS][EQ_MAXC];
     double logk[EQ_MAXS];
     int isEq[EQ_MAXS];

@@ -21,7 +21,6 @@
 
 #include "clb_device.h"
 #include "clb_common.h"
-#include "clb_eq_device.h"
 
 namespace clb {
 
@@ -38,12 +37,6 @@ __device__ __forceinline__ int xcdBlockOf(int b, int G) {
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
 }
 __device__ __forceinline__ int xcdBlock() { return xcdBlockOf(blockIdx.x, gridDim.x); }
-
-// a queued step batch was halted by an earlier step: the kernel does nothing
-// (the flag is written by an earlier kernel of the stream, never during this one)
-__device__ __forceinline__ bool halted(const int *h) {
-    return h && *h;
-}
 
 // compact neighbour n (>= 0) closed for a species: bounce-back class, or a
 // closed x plane (complab_functions.hh:431-432)
@@ -1146,47 +1139,6 @@ __global__ void k_ns_reopen(NsDev<R> a) {
     }
 }
 
-// ------------------------------------------------------------------------------------------
-// equilibrium speciation (FP64, one thread per open interior voxel)
-// ------------------------------------------------------------------------------------------
-template <typename R>
-__global__ void __launch_bounds__(128) k_equilibrium(AdeDev<R> a, EqDev e, unsigned int *nonconv) {
-    if (halted(a.halt)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.Nint) return;
-    const long long v = a.list[i];
-    const int mask = a.mask[v];
-    if (mask == 0 || mask == 1) return;
-    const long long No = a.Nopen, N = a.N;
-    const int S = e.ns;
-    double C[EQ_MAXS], E[EQ_MAXS];
-    for (int s = 0; s < S; ++s) {
-        double r;
-        if (a.mobile[s]) {
-            const R *o = a.cur + (long long) a.slot[s] * 7 * No + i;
-            r = 0; for (int q = 0; q < 7; ++q) r += (double) o[q * No];
-        } else r = (double) a.imm[(long long) a.slot[s] * N + v];
-        C[s] = fmax(r, EQ_MIN_CONC);
-    }
-    const bool conv = eqSolve(e, C, E);
-    if (!conv) atomicAdd(nonconv, 1u);
-    for (int s = 0; s < S; ++s) {
-        double dC = E[s] - C[s];
-        const double mx = fmax(1e-9, 0.1 * fabs(C[s]));                 // part4:1472-1485
-        if (fabs(dC) > mx) dC = dC > 0 ? mx : -mx;
-        if (C[s] + dC < 1e-20) dC = 1e-20 - C[s];
-        if (fabs(dC) > 1e-12) {
-            if (a.mobile[s]) {
-                R *o = a.cur + (long long) a.slot[s] * 7 * No + i;
-                for (int q = 0; q < 7; ++q) o[q * No] = (R) ((double) o[q * No] + w7(q) * dC);
-            } else {
-                R *p = a.imm + (long long) a.slot[s] * N + v;
-                *p = (R) ((double) *p + dC);
-            }
-        }
-    }
-}
-
 // compact <-> dense staging
 template <typename R>
 __global__ void k_scatter(const R *src, const int *list, int n, double *dst) {
@@ -1330,11 +1282,6 @@ hipError_t launchNsReopen(const NsDev<R> &a, const uint8_t *, hipStream_t st) {
     return hipGetLastError();
 }
 template <typename R>
-hipError_t launchEquilibrium(const AdeDev<R> &a, const EqDev &e, unsigned int *nonconv, hipStream_t st) {
-    hipLaunchKernelGGL(k_equilibrium<R>, dim3(nblk(a.Nint, 128)), dim3(128), 0, st, a, e, nonconv);
-    return hipGetLastError();
-}
-template <typename R>
 hipError_t launchScatterCompact(const R *src, const int *list, int n, double *dst, hipStream_t st) {
     hipLaunchKernelGGL(k_scatter<R>, dim3(nblk(n, 256)), dim3(256), 0, st, src, list, n, dst);
     return hipGetLastError();
@@ -1359,7 +1306,6 @@ hipError_t launchGatherCompact(const double *src, const int *list, int n, R *dst
     template hipError_t launchNsVelocity<R, R>(const NsDev<R> &, R *, double *, hipStream_t); \
     template hipError_t launchNsInit<R>(const NsDev<R> &, double, hipStream_t); \
     template hipError_t launchNsReopen<R>(const NsDev<R> &, const uint8_t *, hipStream_t); \
-    template hipError_t launchEquilibrium<R>(const AdeDev<R> &, const EqDev &, unsigned int *, hipStream_t); \
     template hipError_t launchScatterCompact<R>(const R *, const int *, int, double *, hipStream_t); \
     template hipError_t launchGatherCompact<R>(const double *, const int *, int, R *, hipStream_t);
 INST(double)
