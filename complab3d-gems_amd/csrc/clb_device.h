@@ -96,6 +96,11 @@ struct KinDev {
     const int8_t *phase;           // dense phase ids
     double *share;                 // [S][Nopen] per-face product share parked this step
     unsigned long long *dcount;    // [0] shares parked, [1] collected, [2] collected and clamped
+    // the voxels next to the phase (a face neighbour in the reaction box of
+    // phase dPhase) take the gather: 0 every voxel as usual; 1 (the transport
+    // launch) their abiotic increment is left for the pass after the park;
+    // 2 (that pass) only they are visited
+    int nearPhase;
 };
 
 template <typename R>

@@ -583,19 +583,23 @@ struct Case : CaseBase {
         evUsed = 0;
     }
     // one transport launch with its kinetics.  With dissolution the abiotic
-    // increment waits for the products the phase voxels park: the fused launch
-    // does transport + biotic kinetics, then the park pass, then the abiotic
-    // kinetics with the gather (the reference's order, complab.cpp:2461-2545).
+    // increment of a voxel next to the dissolving phase waits for the products
+    // the phase voxels park: the fused launch does transport, the biotic
+    // kinetics and every other voxel's abiotic kinetics, then the park pass
+    // (which reads only the waiting voxels' water), then the abiotic kinetics
+    // with the gather at the waiting voxels (the reference's order,
+    // complab.cpp:2461-2545).
     void transportLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr, bool maskTotal = false) {
         if (!kinOn || !p.dissol_on) { fusedLaunch(st, kinOn, halt, true, maskTotal); return; }
-        fusedLaunch(st, true, halt, false, maskTotal);
+        fusedLaunch(st, true, halt, true, maskTotal, 1);
         KinDev k = kin(true);
         k.bio = 0;
+        k.nearPhase = 2;
         HIPT(launchDissolPark<R>(ade(halt), k, st));
         HIPT(launchKinetics<R>(ade(halt), k, st));
     }
     void fusedLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr, bool abioticInFused = true,
-                     bool maskTotal = false) {
+                     bool maskTotal = false, int nearPhase = 0) {
         const bool timed = launchNo % timeStride == 0;
         if (timed && evUsed == evpool.size()) {
             if (evpool.size() >= kEvPool) flushTimes(st);
@@ -609,6 +613,7 @@ struct Case : CaseBase {
         a.maskTotal = maskTotal ? 1 : 0;
         KinDev k = kin(kinOn);
         k.dissol = 0;
+        k.nearPhase = nearPhase;
         if (!abioticInFused) k.nrxn = 0;
         if (!k.bio && k.nrxn == 0) k.on = 0;
         if (timed) HIPT(hipEventRecord(evpool[evUsed].first, st));

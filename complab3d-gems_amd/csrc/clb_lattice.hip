@@ -172,6 +172,25 @@ __device__ __forceinline__ double clampInc(int clamp, double d, double cur, unsi
 // ------------------------------------------------------------------------------------------
 // advection-diffusion
 // ------------------------------------------------------------------------------------------
+// a face neighbour of compact voxel i in the reaction box belongs to the
+// dissolving phase: i collects that neighbour's parked products (the
+// condition of the gather below and, seen from the phase voxel, the open
+// neighbours whose water the park pass averages).  A voxel of the phase
+// itself counts too: the park reads its mineral.
+template <typename R>
+__device__ __forceinline__ bool nextToPhase(const AdeDev<R> &a, const KinDev &k, int i) {
+    const int own = k.phase[a.list[i]];
+    bool near = own > 0 && own == k.dPhase;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const int n = a.nb[(opp7(d + 1) - 1) * a.Nopen + i];
+        if (n < 0 || n >= a.Nint) continue;
+        const int pid = k.phase[a.list[n]];
+        if (pid > 0 && pid == k.dPhase) near = true;
+    }
+    return near;
+}
+
 // biotic and abiotic kinetics of one open interior voxel on its post-collision
 // populations f (every mobile species in registers): rates from the densities,
 // the clamp, the D3Q7 deposit w_i dC, immobile species and biomass in place
@@ -246,7 +265,30 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
                 }
             }
         }
-        if (k.nrxn > 0 || DISSOL) {
+        // the surface gate of the abiotic rates (precipitationVOP.hh:78-88) and,
+        // in the transport launch of a dissolution step, whether the voxel is
+        // next to the dissolving phase (nextToPhase): there the abiotic
+        // increment waits for the parked products, elsewhere the gather adds
+        // nothing and the increment is final here.  One pass over the faces.
+        const bool wantNear = !DISSOL && k.nearPhase == 1;
+        bool react = k.nrxn > 0, deferred = false;
+        if (react && (k.surface_only || wantNear)) {
+            bool surf = false, near = false;
+            if (wantNear) { const int own = k.phase[v]; near = own > 0 && own == k.dPhase; }
+#pragma unroll
+            for (int q = 1; q < 7; ++q) {
+                const int n = a.nb[(opp7(q) - 1) * No + i];                // the voxel at +c_q
+                if (n == NB_OUTSIDE) continue;
+                if (n == NB_CLOSED) { surf = true; continue; }
+                const long long vn = a.list[n];
+                const int nm = a.mask[vn];
+                if (nm == 0 || nm == 1) surf = true;
+                if (wantNear && n < a.Nint) { const int pid = k.phase[vn]; if (pid > 0 && pid == k.dPhase) near = true; }
+            }
+            if (k.surface_only) react = surf;
+            deferred = react && near;
+        }
+        if ((k.nrxn > 0 || DISSOL) && !deferred) {
             // one increment per species: the abiotic rates, then (dissolution) the
             // product shares the face neighbours of the phase parked this step,
             // applied together (update_abiotic_rxnLattices, part1:306-404)
@@ -254,18 +296,6 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
 #pragma unroll
             for (int s = 0; s < CLB_DEV_MAXS; ++s) dAb[s] = 0.0;
             if (k.nrxn > 0) {
-                bool react = true;
-                if (k.surface_only) {                                     // precipitationVOP.hh:78-88
-                    react = false;
-#pragma unroll
-                    for (int q = 1; q < 7; ++q) {
-                        const int n = a.nb[(opp7(q) - 1) * No + i];        // the voxel at +c_q
-                        if (n == NB_OUTSIDE) continue;
-                        if (n == NB_CLOSED) { react = true; continue; }
-                        const int nm = a.mask[a.list[n]];
-                        if (nm == 0 || nm == 1) react = true;
-                    }
-                }
                 if (react) {
 #pragma unroll
                     for (int s = 0; s < CLB_DEV_MAXS; ++s) { if (s < S) { C[s] = concOf(s); if (C[s] < 1e-12) C[s] = 0; } }
@@ -530,6 +560,10 @@ __global__ void __launch_bounds__(256) k_kinetics(AdeDev<R> a, KinDev k) {
     const int i = xcdBlock() * blockDim.x + threadIdx.x;
     if (i >= a.Nint) return;
     if (a.cls[i] == ADE_BB) return;
+    const long long v = a.list[i];
+    const int mask = a.mask[v];
+    if (mask == 0 || mask == 1) return;                  // no kinetics on solid / bounce-back voxels
+    if (k.nearPhase == 2 && !nextToPhase(a, k, i)) return;
     const long long No = a.Nopen;
     R f[MAXS][7];
 #pragma unroll
@@ -539,9 +573,8 @@ __global__ void __launch_bounds__(256) k_kinetics(AdeDev<R> a, KinDev k) {
 #pragma unroll
         for (int q = 0; q < 7; ++q) f[ms][q] = c[q * No + i];
     }
-    const long long v = a.list[i];
     R nopre[CLB_DEV_MAXM] = {};
-    voxelKinetics<R, MAXS, DISSOL>(a, k, i, f, v, a.mask[v], nopre, false);
+    voxelKinetics<R, MAXS, DISSOL>(a, k, i, f, v, mask, nopre, false);
 #pragma unroll
     for (int ms = 0; ms < MAXS; ++ms) {
         if (ms >= a.Sm) continue;
