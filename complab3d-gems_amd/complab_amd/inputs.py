@@ -127,6 +127,11 @@ class Run:
     diag_tol: float = 1e-6
     conserve: List[str] = field(default_factory=list)
     materials: dict = field(default_factory=dict)   # file material number -> this build's code
+    # output (complab_functions.hh:1166-1178, 1465-1474; complab3d_integration.hh:383-384)
+    output_path: str = "output"                 # absolute, or as the XML names it under its folder
+    vtk_interval: int = 1000                    # <IO><save_VTK_interval>
+    mask_filename: str = "maskLattice"
+    summary_csv: str = "summary.csv"
 
 
 def load(xml_path, geometry_path: Optional[str] = None, precision=P.FP64, order="slices") -> Run:
@@ -353,4 +358,8 @@ def load(xml_path, geometry_path: Optional[str] = None, precision=P.FP64, order=
                microbes=microbes, dx=dx_m, diag_enabled=run_diag,
                diag_interval=doc.num("diagnostics", "interval", default=0, cast=int),
                diag_tol=doc.num("diagnostics", "tolerance", default=1e-6), conserve=conserve,
-               materials=dict(mats))
+               materials=dict(mats),
+               output_path=os.path.join(base, doc.text("path", "output_path", default="output")),
+               vtk_interval=doc.num("IO", "save_VTK_interval", default=1000, cast=int),
+               mask_filename=doc.text("IO", "mask_filename", default="maskLattice"),
+               summary_csv=doc.text("diagnostics", "summary_csv", default="summary.csv"))

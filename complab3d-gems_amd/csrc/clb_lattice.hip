@@ -68,15 +68,20 @@ __device__ __forceinline__ void pull7(const AdeDev<R> &a, const R *__restrict__ 
 
 // first launch after (re)initialisation: the state is pre-collision, and a
 // closed neighbour holds the species' rest population w c0; hand it to the
-// slot the bounce-back rule reads at the next launch
+// slot the bounce-back rule reads at the next launch.  After the
+// stabilization reset (planesOnly) that holds for the closed x planes alone:
+// the reset rewrites every pore voxel, the planes' included, and leaves the
+// walls with what they hold (stabilizeADElattice3D, part3:261-310).
 template <typename R>
-__device__ __forceinline__ void firstHandOver(const AdeDev<R> &a, R *cur, int i, int lbt, int rbt, double c0) {
+__device__ __forceinline__ void firstHandOver(const AdeDev<R> &a, R *cur, int i, int lbt, int rbt, double c0,
+                                              bool planesOnly) {
     const long long No = a.Nopen;
 #pragma unroll
     for (int q = 1; q < 7; ++q) {
         const int n = a.nb[(q - 1) * No + i];
-        if (n == NB_CLOSED || (n >= 0 && closedFor(a.clsPull, n, a.Nint, a.Nleft, lbt, rbt)))
-            cur[opp7(q) * No + i] = (R) (W7F * c0);
+        const bool closed = planesOnly ? (n >= a.Nint && closedFor(a.clsPull, n, a.Nint, a.Nleft, lbt, rbt))
+                                       : (n == NB_CLOSED || (n >= 0 && closedFor(a.clsPull, n, a.Nint, a.Nleft, lbt, rbt)));
+        if (closed) cur[opp7(q) * No + i] = (R) (W7F * c0);
     }
 }
 
@@ -409,7 +414,7 @@ __device__ __forceinline__ void adePlaneVoxel(const AdeDev<R> &a, int i, int ms)
     R *old = a.out + (long long) ms * 7 * No;
     R f[7];
     pull7<R, R>(a, cur, old, i, a.lbt[s], a.rbt[s], f);
-    if (a.first) firstHandOver<R>(a, cur, i, a.lbt[s], a.rbt[s], a.c0[s]);
+    if (a.first) firstHandOver<R>(a, cur, i, a.lbt[s], a.rbt[s], a.c0[s], false);   // a plane voxel has no plane neighbour
     if (a.reset) {
         double c0 = a.c0[s];
         if (c0 < 1e-12 && c0 > -1e-12) c0 = 0;
@@ -498,10 +503,11 @@ __device__ __forceinline__ void adeBulkVoxel(const AdeDev<R> &a, const KinDev &k
             for (int q = 0; q < 7; ++q) r += (double) f[ms][q];
             a.rhoN[(long long) ms * Np + (nbq[1] - a.Nint)] = r;
         }
-        if (a.first) firstHandOver<R>(a, a.cur + (long long) ms * 7 * No, i, a.mlbt[ms], a.mrbt[ms], a.mc0[ms]);
+        if (a.first) firstHandOver<R>(a, a.cur + (long long) ms * 7 * No, i, a.mlbt[ms], a.mrbt[ms], a.mc0[ms], false);
         if (a.reset) {
             double c0 = a.mc0[ms];
             if (c0 < 1e-12 && c0 > -1e-12) c0 = 0;
+            if (!a.first) firstHandOver<R>(a, a.cur + (long long) ms * 7 * No, i, a.mlbt[ms], a.mrbt[ms], c0, true);
 #pragma unroll
             for (int q = 0; q < 7; ++q) f[ms][q] = (R) (w7(q) * c0);
         }
@@ -756,10 +762,17 @@ __global__ void __launch_bounds__(256) k_ade_held(AdeDev<R> a, int ms, double *p
             }
         }
         if (!closedFor(a.clsPull, i, a.Nint, a.Nleft, lbt, rbt)) {
+            // a pending stabilization reset has rewritten the closed x planes
+            // (pore voxels) with the rest populations, not the walls
+            double c0r = a.c0[s];
+            if (c0r < 1e-12 && c0r > -1e-12) c0r = 0;
             for (int q = 1; q < 7; ++q) {
                 const int d = a.nb[(opp7(q) - 1) * No + i];          // the voxel at x + c_q
-                if (d == NB_CLOSED || (d >= 0 && closedFor(a.cls, d, a.Nint, a.Nleft, lbt, rbt)))
-                    h += a.first ? W7F * a.c0[s] : (double) cur[q * No + i];
+                if (d == NB_CLOSED || (d >= 0 && closedFor(a.cls, d, a.Nint, a.Nleft, lbt, rbt))) {
+                    if (a.first) h += W7F * a.c0[s];
+                    else if (a.reset && d >= a.Nint) h += W7F * c0r;
+                    else h += (double) cur[q * No + i];
+                }
             }
         }
     }

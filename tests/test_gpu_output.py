@@ -1,0 +1,43 @@
+"""A CompLaB.xml run through the runner on the device against the same run on
+the oracle: every snapshot file and the summary rows."""
+import os
+
+import numpy as np
+import pytest
+
+from complab_amd import Case, inputs, output, runner
+import oracle as O
+from test_output import write_run
+
+pytestmark = pytest.mark.gpu
+
+
+def test_device_run_writes_the_oracle_files(ctx, tmp_path):
+    print("case: runner on the 10x6x5 abiotic XML with flow, 25 steps", flush=True)
+    r = inputs.load(write_run(str(tmp_path)))
+    g = runner.run(r, case=Case(ctx, r.params, r.geometry), out_dir=str(tmp_path / "gpu"))
+    o = runner.run(r, case=O.OracleCase(r.params, r.geometry), out_dir=str(tmp_path / "cpu"))
+    assert g.steps == o.steps == 25 and g.flow_iters == o.flow_iters
+    names = sorted(os.listdir(g.out_dir))
+    assert names == sorted(os.listdir(o.out_dir)) and len(names) == 17
+    for n in names:
+        if not n.endswith(".vti"):
+            continue
+        ga, gd = output.read_vti(os.path.join(g.out_dir, n))
+        oa, od = output.read_vti(os.path.join(o.out_dir, n))
+        assert gd == od and sorted(ga) == sorted(oa)
+        for k in ga:
+            a, b = ga[k].astype(np.float64), oa[k].astype(np.float64)
+            if n.startswith("maskL"):
+                assert np.array_equal(a, b), n
+            else:
+                tol = 1e-10 if ga[k].dtype == np.float64 else 2e-6   # FP64 fields; Float32 files round once
+                scale = max(float(np.max(np.abs(b))), 1e-300)
+                assert float(np.max(np.abs(a - b))) / scale <= tol, (n, k)
+    gc, grows = output.read_summary(g.summary_csv)
+    oc, orows = output.read_summary(o.summary_csv)
+    assert gc == oc and grows.shape == orows.shape == (5, 3 + 5 * 2)
+    assert np.array_equal(grows[:, :3], orows[:, :3])
+    rel = np.abs(grows - orows) / np.maximum(np.abs(orows), 1e-300)
+    assert float(np.max(rel[:, 3:][np.abs(orows[:, 3:]) > 1e-14])) <= 1e-10
+    assert g.verdict == o.verdict == "PASS"
