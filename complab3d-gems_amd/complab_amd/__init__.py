@@ -74,6 +74,9 @@ def lib():
         "clb_counters": ([V, I, V], I),
         "clb_kernel_time": ([V, I, I, C.POINTER(D), C.POINTER(C.c_int64)], I),
         "clb_set_timing_stride": ([V, I, I], I),
+        "clb_checkpoint_size": ([V, I, C.POINTER(C.c_size_t)], I),
+        "clb_checkpoint_save": ([V, I, V, C.c_size_t], I),
+        "clb_checkpoint_load": ([V, I, V, C.c_size_t], I),
         "clb_stream_copy": ([V, C.c_size_t, I, C.POINTER(D), C.POINTER(D)], I),
         "clb_rng_draws": ([C.c_uint64, I, V], I),
         "clb_geom_spheres": ([I, I, I, D, I, C.c_uint64, I, V], I),
@@ -98,6 +101,7 @@ def exported_symbols():
         "clb_ade_collide_stream", "clb_kinetics", "clb_mask_total", "clb_ca_sweep", "clb_age_update",
         "clb_update_solute_dynamics", "clb_update_solute_solid", "clb_precip_convert", "clb_dissol_convert",
         "clb_read_field", "clb_write_field", "clb_species_totals", "clb_field_stats", "clb_counters", "clb_kernel_time", "clb_set_timing_stride",
+        "clb_checkpoint_size", "clb_checkpoint_save", "clb_checkpoint_load",
         "clb_stream_copy",
         "clb_rng_draws", "clb_geom_spheres", "clb_geom_pack", "clb_geom_generate",
     ]
@@ -288,6 +292,30 @@ class Case:
         out = np.zeros(12, dtype=np.int64)
         self._c(lib().clb_counters(self.ctx.h, self.id, _p(out)))
         return out
+
+    def checkpoint(self) -> bytes:
+        """The case's state as one block (clb_checkpoint_save)."""
+        n = C.c_size_t(0)
+        self._c(lib().clb_checkpoint_size(self.ctx.h, self.id, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        self._c(lib().clb_checkpoint_save(self.ctx.h, self.id, buf, n))
+        return buf.raw
+
+    def restore(self, data: bytes):
+        """Loads a block from checkpoint() of a case with the same parameters
+        and geometry; the case then continues bit-identically."""
+        buf = C.create_string_buffer(bytes(data), len(data))
+        self._c(lib().clb_checkpoint_load(self.ctx.h, self.id, buf, C.c_size_t(len(data))))
+
+    def save_checkpoint(self, path):
+        tmp = path + ".part"
+        with open(tmp, "wb") as f:
+            f.write(self.checkpoint())
+        os.replace(tmp, path)
+
+    def load_checkpoint(self, path):
+        with open(path, "rb") as f:
+            self.restore(f.read())
 
     def set_timing_stride(self, stride):
         """Time only every stride-th transport launch (event markers cost the

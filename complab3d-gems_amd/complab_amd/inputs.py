@@ -130,6 +130,7 @@ class Run:
     # output (complab_functions.hh:1166-1178, 1465-1474; complab3d_integration.hh:383-384)
     output_path: str = "output"                 # absolute, or as the XML names it under its folder
     vtk_interval: int = 1000                    # <IO><save_VTK_interval>
+    chk_interval: int = 1000000                 # <IO><save_CHK_interval>
     mask_filename: str = "maskLattice"
     summary_csv: str = "summary.csv"
 
@@ -361,5 +362,6 @@ def load(xml_path, geometry_path: Optional[str] = None, precision=P.FP64, order=
                materials=dict(mats),
                output_path=os.path.join(base, doc.text("path", "output_path", default="output")),
                vtk_interval=doc.num("IO", "save_VTK_interval", default=1000, cast=int),
+               chk_interval=doc.num("IO", "save_CHK_interval", default=1000000, cast=int),
                mask_filename=doc.text("IO", "mask_filename", default="maskLattice"),
                summary_csv=doc.text("diagnostics", "summary_csv", default="summary.csv"))

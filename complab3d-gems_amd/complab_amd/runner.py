@@ -1,10 +1,12 @@
 """A CompLaB.xml run end to end over the library: the set-up sequence, the
 coupled steps, a snapshot of every field every save_VTK_interval steps and
-after the last step, and the diagnostics rows with summary.csv
-(complab.cpp:1013-1283 set-up; 2178-2290 the loop's output block, taken at
-the top of step iT, i.e. after iT whole steps; 3270-3310 the final output).
+after the last step, the diagnostics rows with summary.csv, and a checkpoint
+every save_CHK_interval steps (complab.cpp:1013-1283 set-up; 2178-2302 the
+loop's output block, taken at the top of step iT, i.e. after iT whole steps;
+3270-3310 the final output).  A run restarted from a checkpoint skips the
+set-up and continues from the step the checkpoint holds, bit-identically.
 
-    python -m complab_amd.runner path/to/CompLaB.xml [--out DIR] [--steps N]
+    python -m complab_amd.runner path/to/CompLaB.xml [--out DIR] [--steps N] [--restart FILE]
 """
 import argparse
 import os
@@ -29,22 +31,37 @@ class Result:
     verdict: str = ""
     flow_iters: List[int] = field(default_factory=list)
     seconds: float = 0.0
+    checkpoints: List[str] = field(default_factory=list)
+
+
+CHK_STEP = 7                                    # counters(): steps run
+
+
+def checkpoint_name(iteration: int) -> str:
+    return "checkpoint_%07d.clbchk" % iteration
 
 
 def run(r: inputs.Run, ctx=None, case=None, out_dir: Optional[str] = None, steps: Optional[int] = None,
-        log=None) -> Result:
+        log=None, restart: Optional[str] = None) -> Result:
     """Runs `r` on a new device case (or on `case`, created from the same
     parameters and geometry) and writes its output under `out_dir` (default:
-    the XML's output path)."""
+    the XML's output path).  With `restart`, the case is loaded from that
+    checkpoint file instead of being set up."""
     t0 = time.perf_counter()
     steps = r.steps if steps is None else steps
     out = out_dir or r.output_path
     if case is None:
         case = Case(ctx or Context(0), r.params, r.geometry)
     say = (lambda m: print(m, file=log, flush=True)) if log else (lambda m: None)
-    rep = driver.setup(case, r.params, r.flow, r.transport)
-    case.set_omegas(rep.omega_pore, rep.omega_biofilm, 1.0)
-    say("set-up done: flow iterations %s, dt %.6g s" % (list(rep.iters), rep.dt))
+    iters = []
+    if restart:
+        case.load_checkpoint(restart)
+        say("restarted from %s at step %d" % (restart, int(case.counters()[CHK_STEP])))
+    else:
+        rep = driver.setup(case, r.params, r.flow, r.transport)
+        case.set_omegas(rep.omega_pore, rep.omega_biofilm, 1.0)
+        iters = list(rep.iters)
+        say("set-up done: flow iterations %s, dt %.6g s" % (iters, rep.dt))
 
     flow = bool(r.params.flow_on) and r.flow.pe > 0
     snaps = Snapshots(out, r.species, r.microbes, r.mask_filename, flow=flow)
@@ -55,11 +72,14 @@ def run(r: inputs.Run, ctx=None, case=None, out_dir: Optional[str] = None, steps
     csv = SummaryCsv(csv_path) if r.diag_enabled else None
     diag_every = r.diag_interval if r.diag_interval > 0 else max(r.vtk_interval, 0)
     vtk_every = max(r.vtk_interval, 0)
+    chk_every = max(r.chk_interval, 0)
+    checkpoints = []
 
     def due(it, every):
         return every > 0 and it % every == 0
 
-    it = 0
+    it = int(case.counters()[CHK_STEP]) if restart else 0
+    first = it
     while True:
         if it < steps:
             if due(it, vtk_every):
@@ -70,10 +90,14 @@ def run(r: inputs.Run, ctx=None, case=None, out_dir: Optional[str] = None, steps
                 csv.append(row)
                 if msg:
                     say(msg.rstrip("\n"))
+            if due(it, chk_every) and it > first:          # complab.cpp:2293 (iT > 0)
+                path = os.path.join(out, checkpoint_name(it))
+                case.save_checkpoint(path)
+                checkpoints.append(path)
         if it >= steps:
             break
         nxt = steps
-        for every in (vtk_every, diag_every if r.diag_enabled else 0):
+        for every in (vtk_every, diag_every if r.diag_enabled else 0, chk_every):
             if every > 0:
                 nxt = min(nxt, (it // every + 1) * every)
         rc, n = case.run_steps(nxt - it)
@@ -83,7 +107,8 @@ def run(r: inputs.Run, ctx=None, case=None, out_dir: Optional[str] = None, steps
         say("step %d" % it)
     snaps.write(case, it)                           # the final output, after the last step
     return Result(steps=it, out_dir=out, files=list(snaps.written), summary_csv=csv_path if csv else None,
-                  verdict=diag.verdict(), flow_iters=list(rep.iters), seconds=time.perf_counter() - t0)
+                  verdict=diag.verdict(), flow_iters=iters, seconds=time.perf_counter() - t0,
+                  checkpoints=checkpoints)
 
 
 def main():
@@ -93,9 +118,10 @@ def main():
     ap.add_argument("--out", default=None, help="output folder (default: the XML's output path)")
     ap.add_argument("--steps", type=int, default=None, help="coupled steps (default: the XML's ade_max_iT)")
     ap.add_argument("--order", default="slices", choices=["slices", "x_fastest"])
+    ap.add_argument("--restart", default=None, help="continue from this checkpoint file")
     args = ap.parse_args()
     r = inputs.load(args.xml, args.geometry, order=args.order)
-    res = run(r, out_dir=args.out, steps=args.steps, log=sys.stderr)
+    res = run(r, out_dir=args.out, steps=args.steps, log=sys.stderr, restart=args.restart)
     print("%d steps in %.1f s; %d files in %s; diagnostics %s"
           % (res.steps, res.seconds, len(res.files), res.out_dir, res.verdict or "off"))
 

@@ -211,6 +211,7 @@ template <typename R> hipError_t launchScalarStats(int, int, long long, const R 
 hipError_t launchFinalSum(const double *, long long, int, int, int, double *, hipStream_t);
 hipError_t launchFinalSumLL(const long long *, long long, long long *, hipStream_t);
 template <typename R> hipError_t launchToDouble(const R *, double *, long long, hipStream_t);
+template <typename R> hipError_t launchNsDensity(const NsDev<R> &, double *dst, hipStream_t);
 template <typename R> hipError_t launchFromDouble(const double *, R *, long long, hipStream_t);
 hipError_t launchI8ToDouble(const int8_t *, double *, long long, hipStream_t);
 hipError_t launchI32ToDouble(const int *, double *, long long, hipStream_t);

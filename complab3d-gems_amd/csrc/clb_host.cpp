@@ -72,6 +72,10 @@ struct CaseBase {
     virtual void soluteSolid(hipStream_t, int *reopened) = 0;
     virtual void precipConvert(hipStream_t, int *changed) = 0;
     virtual Status dissolConvert(hipStream_t, int *reopened) = 0;
+    // checkpoint / restart of the device-resident state
+    virtual size_t checkpointBytes() = 0;
+    virtual Status checkpointSave(hipStream_t, uint8_t *dst, size_t n) = 0;
+    virtual Status checkpointLoad(hipStream_t, const uint8_t *src, size_t n) = 0;
 };
 
 template <typename T>
@@ -462,6 +466,121 @@ struct Case : CaseBase {
     void setRelaxation(const double *omp, const double *omb, double dt) override {
         for (int s = 0; s < p.n_species; ++s) { p.omega_pore[s] = omp[s]; p.omega_biofilm[s] = omb[s]; }
         p.dt = dt;
+    }
+
+    // ------------------------------------------------------------ checkpoint
+    // Layout: ChkHeader, the parameter block, the compact voxel list (checked,
+    // not restored: a checkpoint loads into a case of the same geometry), then
+    // every persistent device field in sections() order.  Scratch buffers of a
+    // step (reduction partials, CA outbox, dissolution shares, staging) are
+    // not part of it: every step rewrites them before reading.
+    struct ChkHeader {
+        char magic[8];
+        int32_t abi, rBytes, rnBytes, nx, ny, nz, Nopen, Nint, Nleft, Sm, Si, M, flowOn, dissolOn;
+        uint8_t fCurIsA, gCurIsA, adeFirst, adeReset, adeClsDirty, nsFirst, nsClsDirty, pad0;
+        double nsOmega, nsBioOmega, deltaP, gmaxClosed;
+        int64_t iT, oldTotMask, msumClosed, launchNo;
+        int32_t nsSaturate, batchLen;
+        uint64_t caSweep;
+        int64_t cnt[CLB_NCOUNTERS];
+    };
+    struct ChkSection { void *ptr; size_t bytes; };
+    std::vector<ChkSection> chkSections() const {
+        std::vector<ChkSection> v;
+        const size_t r = sizeof(R), rn = sizeof(RN);
+        const size_t n = (size_t) N, no = (size_t) Nopen, ni = (size_t) std::max(Nint, 1);
+        const size_t sm = (size_t) std::max(Sm, 1), si = (size_t) std::max(Si, 1), m = (size_t) std::max(p.n_microbes, 1);
+        v.push_back({mask, n}); v.push_back({geom, n}); v.push_back({dist, n}); v.push_back({phase, n});
+        v.push_back({adeCls, no}); v.push_back({adeClsPull, no}); v.push_back({nsCls, no}); v.push_back({nsClsPull, no});
+        v.push_back({reopened, no});
+        v.push_back({adeKinds, ni * sizeof(uint16_t)});
+        v.push_back({fA, sm * 7 * no * r}); v.push_back({fB, sm * 7 * no * r});
+        v.push_back({imm, si * n * r}); v.push_back({B, m * n * r}); v.push_back({Btot, n * r});
+        v.push_back({age, n * sizeof(int)}); v.push_back({age2, n * sizeof(int)});   // the sweep's pair, in pointer order
+        if (pushed) v.push_back({pushed, n});
+        v.push_back({u, 3 * no * r});
+        v.push_back({rhoN, sm * (size_t) std::max(Nopen - Nint, 1) * sizeof(double)});
+        v.push_back({clampCount, sizeof(unsigned long long)});
+        v.push_back({dcount, 3 * sizeof(unsigned long long)});
+        v.push_back({ctl, sizeof(StepCtl)});
+        if (p.flow_on) {
+            v.push_back({gA, 19 * no * rn}); v.push_back({gB, 19 * no * rn});
+            v.push_back({nsKinds, ni * sizeof(unsigned long long)});
+            v.push_back({tracer, sizeof(NsTracer)});
+            v.push_back({nsDone, sizeof(int)});
+        }
+        return v;
+    }
+    ChkHeader chkHeader() const {
+        ChkHeader h;
+        memset(&h, 0, sizeof(h));
+        memcpy(h.magic, "CLBCHK01", 8);
+        h.abi = CLB_ABI_VERSION; h.rBytes = (int32_t) sizeof(R); h.rnBytes = (int32_t) sizeof(RN);
+        h.nx = nx; h.ny = ny; h.nz = nz; h.Nopen = Nopen; h.Nint = Nint; h.Nleft = Nleft;
+        h.Sm = Sm; h.Si = Si; h.M = p.n_microbes; h.flowOn = p.flow_on; h.dissolOn = p.dissol_on;
+        h.fCurIsA = fCurIsA; h.gCurIsA = gCurIsA; h.adeFirst = adeFirst; h.adeReset = adeReset;
+        h.adeClsDirty = adeClsDirty; h.nsFirst = nsFirst; h.nsClsDirty = nsClsDirty;
+        h.nsOmega = nsOmega; h.nsBioOmega = nsBioOmega; h.deltaP = deltaP; h.gmaxClosed = gmaxClosed;
+        h.iT = iT; h.oldTotMask = oldTotMask; h.msumClosed = msumClosed; h.launchNo = launchNo;
+        h.nsSaturate = nsSaturate; h.batchLen = batchLen; h.caSweep = caSweep_;
+        for (int k = 0; k < CLB_NCOUNTERS; ++k) h.cnt[k] = cnt[k];
+        return h;
+    }
+    size_t checkpointBytes() override {
+        size_t b = sizeof(ChkHeader) + sizeof(clb_case_params) + (size_t) Nopen * sizeof(int);
+        for (const auto &sec : chkSections()) b += sec.bytes;
+        return b;
+    }
+    Status checkpointSave(hipStream_t st, uint8_t *dst, size_t n) override {
+        if (n < checkpointBytes()) return {CLB_E_ARG, "checkpoint buffer too small"};
+        flushTimes(st);
+        const ChkHeader h = chkHeader();
+        uint8_t *w = dst;
+        memcpy(w, &h, sizeof(h)); w += sizeof(h);
+        memcpy(w, &p, sizeof(p)); w += sizeof(p);
+        HIPT(hipMemcpyAsync(w, list, (size_t) Nopen * sizeof(int), hipMemcpyDeviceToHost, st));
+        w += (size_t) Nopen * sizeof(int);
+        for (const auto &sec : chkSections()) {
+            HIPT(hipMemcpyAsync(w, sec.ptr, sec.bytes, hipMemcpyDeviceToHost, st));
+            w += sec.bytes;
+        }
+        HIPT(hipStreamSynchronize(st));
+        return {CLB_OK, ""};
+    }
+    Status checkpointLoad(hipStream_t st, const uint8_t *src, size_t n) override {
+        if (n < sizeof(ChkHeader) + sizeof(clb_case_params)) return {CLB_E_ARG, "not a checkpoint (too short)"};
+        ChkHeader h;
+        memcpy(&h, src, sizeof(h));
+        if (memcmp(h.magic, "CLBCHK01", 8) != 0) return {CLB_E_ARG, "not a checkpoint of this library"};
+        const ChkHeader mine = chkHeader();
+        if (h.abi != mine.abi || h.rBytes != mine.rBytes || h.rnBytes != mine.rnBytes)
+            return {CLB_E_ARG, "checkpoint of another library version or precision"};
+        if (h.nx != nx || h.ny != ny || h.nz != nz || h.Nopen != Nopen || h.Nint != Nint || h.Nleft != Nleft ||
+            h.Sm != Sm || h.Si != Si || h.M != mine.M || h.flowOn != mine.flowOn || h.dissolOn != mine.dissolOn)
+            return {CLB_E_ARG, "checkpoint of a case of another shape (lattice, species, microbes or processes)"};
+        if (n != checkpointBytes()) return {CLB_E_ARG, "checkpoint size does not match the case"};
+        const uint8_t *r = src + sizeof(h);
+        clb_case_params q;
+        memcpy(&q, r, sizeof(q)); r += sizeof(q);
+        std::vector<int> lst(Nopen);
+        HIPT(hipMemcpy(lst.data(), list, (size_t) Nopen * sizeof(int), hipMemcpyDeviceToHost));
+        if (memcmp(lst.data(), r, (size_t) Nopen * sizeof(int)) != 0)
+            return {CLB_E_ARG, "checkpoint of another geometry"};
+        r += (size_t) Nopen * sizeof(int);
+        for (const auto &sec : chkSections()) {
+            HIPT(hipMemcpyAsync(sec.ptr, r, sec.bytes, hipMemcpyHostToDevice, st));
+            r += sec.bytes;
+        }
+        HIPT(hipStreamSynchronize(st));
+        p = q;
+        fCurIsA = h.fCurIsA; gCurIsA = h.gCurIsA; adeFirst = h.adeFirst; adeReset = h.adeReset;
+        adeClsDirty = h.adeClsDirty; nsFirst = h.nsFirst; nsClsDirty = h.nsClsDirty;
+        nsOmega = h.nsOmega; nsBioOmega = h.nsBioOmega; deltaP = h.deltaP; gmaxClosed = h.gmaxClosed;
+        iT = h.iT; oldTotMask = h.oldTotMask; msumClosed = h.msumClosed; launchNo = h.launchNo;
+        nsSaturate = h.nsSaturate; batchLen = h.batchLen; caSweep_ = h.caSweep;
+        for (int k = 0; k < CLB_NCOUNTERS; ++k) cnt[k] = h.cnt[k];
+        kms = 0; klaunch = 0;
+        return {CLB_OK, ""};
     }
 
     // ------------------------------------------------------------------ flow
@@ -955,6 +1074,10 @@ struct Case : CaseBase {
             break;
         case CLB_FIELD_AGE: HIPT(launchI32ToDouble(age, stage, N, st)); break;
         case CLB_FIELD_PHASE: HIPT(launchI8ToDouble(phase, stage, N, st)); break;
+        case CLB_FIELD_NSRHO:
+            if (!p.flow_on) return {CLB_E_STATE, "the case has no flow lattice"};
+            HIPT(launchNsDensity<RN>(ns(), stage, st));
+            break;
         default: return {CLB_E_ARG, "unknown field"};
         }
         HIPT(hipMemcpyAsync(dst, stage, N * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1488,6 +1611,32 @@ int clb_stream_copy(clb_ctx *ctx, size_t bytes, int reps, double *gbps, double *
     return rc;
 }
 
+int clb_checkpoint_size(clb_ctx *ctx, int case_id, size_t *bytes) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (!bytes) return fail(ctx, CLB_E_ARG, "null output");
+    *bytes = c->checkpointBytes();
+    return CLB_OK;
+    GUARD_END
+}
+int clb_checkpoint_save(clb_ctx *ctx, int case_id, void *dst, size_t bytes) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (!dst) return fail(ctx, CLB_E_ARG, "null buffer");
+    Status r = c->checkpointSave(ctx->stream, (uint8_t *) dst, bytes);
+    if (r.code) return fail(ctx, r.code, r.msg);
+    return CLB_OK;
+    GUARD_END
+}
+int clb_checkpoint_load(clb_ctx *ctx, int case_id, const void *src, size_t bytes) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (!src) return fail(ctx, CLB_E_ARG, "null buffer");
+    Status r = c->checkpointLoad(ctx->stream, (const uint8_t *) src, bytes);
+    if (r.code) return fail(ctx, r.code, r.msg);
+    return CLB_OK;
+    GUARD_END
+}
 int clb_set_timing_stride(clb_ctx *ctx, int case_id, int stride) {
     GUARD_BEGIN
     CASE_OR_FAIL(c)

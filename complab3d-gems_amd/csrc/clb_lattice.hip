@@ -1092,6 +1092,28 @@ __global__ void __launch_bounds__(1024) k_ns_tracer(NsTracer *t, const double *p
 
 // velocity of the current flow state (computeVelocity) into the compact field
 // read by the ADE lattices; optionally the box reductions of the Peclet rescale
+// the flow density computeDensity reports, into a dense field: 1 + the sum of
+// the post-stream populations at an open interior voxel, the imposed density
+// on an open pressure plane, 1 at closed voxels
+__global__ void k_fill_d(double *dst, long long n, double v) {
+    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+template <typename R>
+__global__ void __launch_bounds__(256) k_ns_density(NsDev<R> a, double *dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.Nopen || a.cls[i] >= NS_BB) return;
+    double r;
+    if (i >= a.Nint) r = i < a.Nint + a.Nleft ? 0.0 : a.rhoBarRight;
+    else {
+        double fl[19];
+        pull19<R, double>(a, i, fl);
+        r = 0;
+        for (int q = 0; q < 19; ++q) r += fl[q];
+    }
+    dst[a.list[i]] = 1.0 + r;
+}
+
 template <typename R, typename RA>
 __global__ void __launch_bounds__(256) k_ns_velocity(NsDev<R> a, RA *u, double *parts) {
     __shared__ double red[4][4], r4[4];
@@ -1201,6 +1223,14 @@ __global__ void k_gather(const double *src, const int *list, int n, R *dst) {
 // host launchers
 // ------------------------------------------------------------------------------------------
 static inline unsigned int nblk(long long n, int b) { return (unsigned int) ((n + b - 1) / b); }
+template <typename R>
+hipError_t launchNsDensity(const NsDev<R> &a, double *dst, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_d, dim3(nblk(a.N, 256)), dim3(256), 0, st, dst, a.N, 1.0);
+    if (a.Nopen > 0) hipLaunchKernelGGL(k_ns_density<R>, dim3(nblk(a.Nopen, 256)), dim3(256), 0, st, a, dst);
+    return hipGetLastError();
+}
+template hipError_t launchNsDensity<double>(const NsDev<double> &, double *, hipStream_t);
+template hipError_t launchNsDensity<float>(const NsDev<float> &, double *, hipStream_t);
 
 unsigned int nsStepBlocks(int Nopen, int Nint) { return nblk(Nint, 256) + nblk(Nopen - Nint, 256); }
 

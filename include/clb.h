@@ -278,7 +278,7 @@ int  clb_dissol_convert(clb_ctx *ctx, int case_id, int *reopened);
 #define CLB_FIELD_VEL      4  /* velocity component `index`                     */
 #define CLB_FIELD_AGE      5  /* CA age                                         */
 #define CLB_FIELD_PHASE    6  /* solid phase id                                 */
-#define CLB_FIELD_NSRHO    7  /* flow density                                   */
+#define CLB_FIELD_NSRHO    7  /* flow density (1 at closed voxels)              */
 /* copies nx*ny*nz doubles (x fastest) of one field into dst */
 int  clb_read_field(clb_ctx *ctx, int case_id, int field, int index, double *dst);
 /* overwrites a species concentration field with equilibrium populations at
@@ -311,6 +311,19 @@ int  clb_field_stats(clb_ctx *ctx, int case_id, double *out);
  * clamped, [11] voxels reopened by dissolution */
 #define CLB_NCOUNTERS 12
 int  clb_counters(clb_ctx *ctx, int case_id, int64_t out[CLB_NCOUNTERS]);
+
+/* ---- checkpoint / restart (saveBinaryBlock / loadBinaryBlock of every lattice,
+ * complab.cpp:2293-2302 and 3274-3310, read back by <IO><read_ADE_file>) -------
+ * One self-describing block of this library's own format holding everything a
+ * bit-identical continuation needs: the persistent device fields (masks,
+ * phases, both population buffers of every lattice, immobile species, biomass,
+ * ages, velocities, flow state, classes), the step state and the parameter
+ * block (relaxation and dt included).  A checkpoint loads into a case created
+ * with the same parameters and geometry (CLB_E_ARG otherwise); the library's
+ * precision and ABI version must match. */
+int  clb_checkpoint_size(clb_ctx *ctx, int case_id, size_t *bytes);
+int  clb_checkpoint_save(clb_ctx *ctx, int case_id, void *dst, size_t bytes);
+int  clb_checkpoint_load(clb_ctx *ctx, int case_id, const void *src, size_t bytes);
 
 /* ---- profiling ------------------------------------------------------------- */
 /* average device time (ms) of the fused transport kernel of the interior

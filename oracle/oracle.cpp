@@ -1394,7 +1394,17 @@ int orc_read_field(void *h, int field, int index, double *dst) {
     case CLB_FIELD_AGE: for (size_t i = 0; i < N; ++i) dst[i] = c.age[i]; return 0;
     case CLB_FIELD_PHASE: for (size_t i = 0; i < N; ++i) dst[i] = c.phase[i]; return 0;
     case CLB_FIELD_NSRHO:
-        for (size_t i = 0; i < N; ++i) { double r = 0; for (int q = 0; q < Q19; ++q) r += c.g[(size_t) q * N + i]; dst[i] = 1.0 + r; }
+        /* computeDensity: the post-stream populations of a BGK cell, the
+         * imposed density of a pressure-plane cell, 1 at closed cells */
+        for (size_t i = 0; i < N; ++i) {
+            const int x = (int) (i % (size_t) c.nx);
+            double r = 0;
+            if (c.nsCls[i] == NS_BB || c.nsCls[i] == NS_NODYN) r = 0;
+            else if (x == 0) r = nsRhoBarLeft();
+            else if (x == c.nx - 1) r = nsRhoBarRight(c);
+            else for (int q = 0; q < Q19; ++q) r += c.g[(size_t) q * N + i];
+            dst[i] = 1.0 + r;
+        }
         return 0;
     }
     return CLB_E_ARG;

@@ -1,0 +1,97 @@
+"""Checkpoint / restart: a case saved after some steps and loaded into a fresh
+case of the same parameters and geometry continues bit-identically to the
+case that kept running (every field and counter); checkpoints of another case
+are refused."""
+import numpy as np
+import pytest
+
+import complab_amd as clb
+from complab_amd import configs, driver
+from complab_amd import params as P
+
+pytestmark = pytest.mark.gpu
+
+
+def state(c, w):
+    p = w.params
+    out = {"mask": c.read(P.FIELD_MASK), "age": c.read(P.FIELD_AGE), "phase": c.read(P.FIELD_PHASE),
+           "btot": c.read(P.FIELD_BTOTAL), "counters": c.counters(), "stats": c.field_stats()}
+    for s in range(p.n_species):
+        out["c%d" % s] = c.read(P.FIELD_CONC, s)
+    for m in range(p.n_microbes):
+        out["b%d" % m] = c.read(P.FIELD_BIOMASS, m)
+    if p.flow_on:
+        for k in range(3):
+            out["u%d" % k] = c.read(P.FIELD_VEL, k)
+        out["rho"] = c.read(P.FIELD_NSRHO)
+    return out
+
+
+def continue_both(ctx, w, before, after):
+    a = clb.Case(ctx, w.params, w.geometry)
+    rep = driver.setup(a, w.params, w.flow, w.transport)
+    a.set_omegas(rep.omega_pore, rep.omega_biofilm, 1.0)
+    assert a.run_steps(before) == (0, before)
+    at_save = a.counters()
+    blob = a.checkpoint()
+    assert a.run_steps(after) == (0, after)
+    sa = state(a, w)
+    b = clb.Case(ctx, w.params, w.geometry)
+    b.restore(blob)
+    assert b.run_steps(after) == (0, after)
+    sb = state(b, w)
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+    a.close()
+    b.close()
+    return at_save, sa["counters"]
+
+
+def test_biotic_ca_case_resumes_bit_identically(ctx):
+    print("case: biotic CA 0.12-scale pack, checkpoint after 25 steps, 35 more", flush=True)
+    w = configs.config2(scale=0.12)
+    w.transport.stabilize_steps = 30
+    w.params.ns_update_interval = 5
+    w.params.ns_max_iter = 3000
+    w.params.b0[0] = 95.0
+    w.params.mu_max[0] = 2e-2
+    c0, c1 = continue_both(ctx, w, 25, 35)
+    assert c1[6] > c0[6] and c1[5] > c0[5]                          # CA sweeps and mask changes after the save
+
+
+def test_precipitation_dissolution_case_resumes_bit_identically(ctx):
+    print("case: carbonate-calcium 1/16 scale FP64, checkpoint after 500 steps, 500 more", flush=True)
+    w = configs.config3(scale=1 / 16, precision=P.FP64)
+    w.flow.eps = 1e-6
+    w.params.ns_conv_eps = 1e-6
+    c0, c1 = continue_both(ctx, w, 500, 500)
+    print("counters at the save", c0, "at the end", c1, flush=True)
+    assert c1[4] > c0[4] and c1[8] > c0[8]                          # flow re-solves, dissolution after the save
+
+
+def test_fp32_case_resumes_bit_identically(ctx):
+    print("case: biotic FP32 0.1-scale pack, checkpoint after 10 steps, 20 more", flush=True)
+    w = configs.config2(scale=0.1, precision=P.FP32)
+    w.transport.stabilize_steps = 20
+    w.params.ns_max_iter = 2000
+    continue_both(ctx, w, 10, 20)
+
+
+def test_foreign_checkpoints_are_refused(ctx):
+    print("case: refusals", flush=True)
+    w = configs.config2(scale=0.1)
+    w.transport.stabilize_steps = 0
+    w.flow.max_iter = 200
+    a = clb.Case(ctx, w.params, w.geometry)
+    blob = a.checkpoint()
+    with pytest.raises(clb.ClbError):
+        a.restore(blob[:-8])                                           # truncated
+    with pytest.raises(clb.ClbError):
+        a.restore(b"XXXXXXXX" + blob[8:])                              # not a checkpoint
+    other = configs.config2(scale=0.1, seed=7)                         # another geometry, same parameters
+    b = clb.Case(ctx, other.params, other.geometry)
+    with pytest.raises(clb.ClbError):
+        b.restore(blob)
+    a.restore(blob)                                                    # its own loads
+    a.close()
+    b.close()

@@ -41,3 +41,21 @@ def test_device_run_writes_the_oracle_files(ctx, tmp_path):
     rel = np.abs(grows - orows) / np.maximum(np.abs(orows), 1e-300)
     assert float(np.max(rel[:, 3:][np.abs(orows[:, 3:]) > 1e-14])) <= 1e-10
     assert g.verdict == o.verdict == "PASS"
+
+
+def test_runner_restart_from_a_checkpoint(ctx, tmp_path):
+    print("case: runner with checkpoints every 10 steps, restart from step 10", flush=True)
+    r = inputs.load(write_run(str(tmp_path)))
+    r.chk_interval = 10
+    full = runner.run(r, case=Case(ctx, r.params, r.geometry), out_dir=str(tmp_path / "full"))
+    assert [os.path.basename(p) for p in full.checkpoints] == ["checkpoint_0000010.clbchk", "checkpoint_0000020.clbchk"]
+    again = runner.run(r, case=Case(ctx, r.params, r.geometry), out_dir=str(tmp_path / "again"),
+                       restart=full.checkpoints[0])
+    assert again.steps == 25 and again.flow_iters == []
+    names = sorted(n for n in os.listdir(again.out_dir) if n.endswith(".vti"))
+    assert names == sorted("%s_%07d.vti" % (n, it) for n in ("A", "B", "maskL", "nsLattice") for it in (10, 20, 25))
+    for n in names:
+        a, _ = output.read_vti(os.path.join(full.out_dir, n))
+        b, _ = output.read_vti(os.path.join(again.out_dir, n))
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (n, k)

@@ -48,6 +48,10 @@ def test_flow_and_transport_fp64(ctx):
     assert abs(rg.delta_p - ro.delta_p) <= 1e-10 * abs(ro.delta_p)
     for k in range(3):
         assert relerr(g.read(P.FIELD_VEL, k), o.read(P.FIELD_VEL, k)) <= REL
+    rg_, ro_ = g.read(P.FIELD_NSRHO), o.read(P.FIELD_NSRHO)      # computeDensity of the flow lattice
+    assert np.max(np.abs(rg_ - ro_)) <= 1e-12                       # density ~1: absolute
+    assert np.all(rg_[w.geometry < P.PORE] == 1.0)
+    assert np.all(rg_[:, :, 0][w.geometry[:, :, 0] >= P.PORE] == 1.0)   # the imposed inlet density
     for c in (g, o):
         c.set_omegas(ro.omega_pore, ro.omega_biofilm, 1.0)
     rc, n = g.run_steps(150)
