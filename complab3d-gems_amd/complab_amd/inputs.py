@@ -112,6 +112,172 @@ def _btype(s, where):
     raise InputError("%s (%s) should be Dirichlet, Neumann or closed" % (where, s))
 
 
+# ---------------------------------------------------------------------------------------------
+# .thm energetics files (complab3d_thermo.hh:299-508): `key value...` lines, `#` comments;
+# file keys provenance, version, temperature (K), energy_units (kJ | J), gas_constant
+# (kJ/mol/K), concentration_scale; blocks opened by `reaction NAME` with microbe, stoich
+# (name/number pairs), dG0, atp, dGatp, chi, cmin, dGloss and the yield keys
+# ---------------------------------------------------------------------------------------------
+@dataclass
+class ThmReaction:
+    name: str
+    microbe: str = ""
+    species: List[str] = field(default_factory=list)
+    nu: List[float] = field(default_factory=list)
+    dG0: Optional[float] = None
+    atp: float = 0.0
+    dGatp: float = 50.0
+    chi: float = 1.0
+    cmin: float = 1e-9
+    dGloss: float = 0.0
+    yield_on: bool = False
+    numbers: dict = field(default_factory=dict)        # dGana, carbon, reduction, dGdis, yieldmax
+
+
+@dataclass
+class ThmModel:
+    provenance: str = ""
+    temperature: float = 298.15
+    unit_scale: float = 1.0                             # to kJ/mol
+    gas_constant: float = 8.314462618e-3
+    conc_scale: float = 1.0
+    reactions: List[ThmReaction] = field(default_factory=list)
+
+    def rt(self) -> float:
+        return self.gas_constant * self.temperature
+
+
+_THM_NUMBERS = ("dG0", "atp", "dGatp", "chi", "cmin", "dGloss", "dGana", "carbon", "reduction", "dGdis", "yieldmax")
+
+
+def read_thm(path) -> ThmModel:
+    """A .thm file, strictly: an unknown keyword, a malformed line, a block
+    without stoich, chi <= 0, cmin <= 0 or a negative dGloss is an error."""
+    M = ThmModel()
+    cur = None
+    with open(path) as fh:
+        for ln, line in enumerate(fh, 1):
+            t = line.split("#", 1)[0].split()
+            if not t:
+                continue
+            at = " (%s line %d)" % (os.path.basename(path), ln)
+            key = t[0]
+
+            def one_number(positive=False):
+                if len(t) != 2:
+                    raise InputError("%s needs exactly one number%s" % (key, at))
+                try:
+                    v = float(t[1])
+                except ValueError:
+                    raise InputError("%s needs exactly one number%s" % (key, at))
+                if positive and not v > 0:
+                    raise InputError("%s needs one positive number%s" % (key, at))
+                return v
+            if key == "provenance":
+                M.provenance = " ".join(t[1:])
+            elif key == "version":
+                pass
+            elif key == "temperature":
+                M.temperature = one_number(True)
+            elif key == "energy_units":
+                if len(t) != 2 or t[1] not in ("kJ", "kj", "J", "j"):
+                    raise InputError("energy_units is kJ or J%s" % at)
+                M.unit_scale = 1.0 if t[1] in ("kJ", "kj") else 1e-3
+            elif key == "concentration_scale":
+                M.conc_scale = one_number(True)
+            elif key == "gas_constant":
+                M.gas_constant = one_number(True)
+            elif key == "reaction":
+                if len(t) != 2:
+                    raise InputError("reaction needs exactly one name%s" % at)
+                if any(r.name == t[1] for r in M.reactions):
+                    raise InputError("two reactions are both named '%s'%s" % (t[1], at))
+                cur = ThmReaction(t[1])
+                M.reactions.append(cur)
+            elif cur is None:
+                raise InputError("'%s' appears before any reaction block%s" % (key, at))
+            elif key == "stoich":
+                if len(t) < 3 or (len(t) - 1) % 2:
+                    raise InputError("stoich needs name/number pairs%s" % at)
+                for name, num in zip(t[1::2], t[2::2]):
+                    try:
+                        v = float(num)
+                    except ValueError:
+                        raise InputError("'%s' is not a number in the stoich line%s" % (num, at))
+                    if v == 0.0:
+                        raise InputError("species '%s' has a zero coefficient%s" % (name, at))
+                    cur.species.append(name)
+                    cur.nu.append(v)
+            elif key == "microbe":
+                if len(t) != 2:
+                    raise InputError("microbe needs one name, index or 'all'%s" % at)
+                cur.microbe = t[1]
+            elif key == "yield":
+                if len(t) != 2 or t[1].lower() not in set(_TRUE) | {"off", "false", "no", "0"}:
+                    raise InputError("yield needs on or off%s" % at)
+                cur.yield_on = t[1].lower() in _TRUE
+            elif key in _THM_NUMBERS:
+                v = one_number()
+                if key in ("dG0", "atp", "dGatp", "chi", "cmin", "dGloss"):
+                    setattr(cur, key, v)
+                else:
+                    cur.numbers[key] = v
+            else:
+                raise InputError("'%s' is not a keyword this file understands%s" % (key, at))
+    if not M.reactions:
+        raise InputError("%s declares no reaction blocks" % path)
+    for r in M.reactions:
+        if not r.species:
+            raise InputError("reaction '%s' has no stoich line" % r.name)
+        if r.dG0 is None:
+            raise InputError("reaction '%s' has no dG0" % r.name)
+        if not r.chi > 0:
+            raise InputError("reaction '%s' has chi <= 0" % r.name)
+        if not r.cmin > 0:
+            raise InputError("reaction '%s' has cmin <= 0, so ln Q can be -inf" % r.name)
+        if r.dGloss < 0:
+            raise InputError("reaction '%s' has a negative dGloss" % r.name)
+        r.dG0 *= M.unit_scale
+        r.dGatp *= M.unit_scale
+        r.dGloss *= M.unit_scale
+    return M
+
+
+def apply_thm(p, M: ThmModel, species: List[str], microbes: List[str]):
+    """Binds the model to the run (species and microbe names) and sets the
+    parameter block's gate.  The compiled-kinetics path computes one rate
+    vector per voxel, so one reaction block for microbe0 or all of them
+    (complab3d_integration.hh:826-860)."""
+    if len(M.reactions) > 1:
+        raise InputError("the energetics file declares more than one reaction block; the kinetics path "
+                         "computes one combined rate vector per voxel and takes one gate")
+    r = M.reactions[0]
+    if r.microbe not in ("", "all", "*"):
+        idx = microbes.index(r.microbe) if r.microbe in microbes else (
+            int(r.microbe) if r.microbe.isdigit() and int(r.microbe) < len(microbes) else -1)
+        if idx < 0:
+            raise InputError("reaction '%s' names microbe '%s', which is not in <name_of_microbes>"
+                             % (r.name, r.microbe))
+        if idx != 0:
+            raise InputError("reaction '%s' is for an organism other than microbe0; the kinetics path "
+                             "gates only as microbe0 (write `microbe all`)" % r.name)
+    for s in range(len(p.thm_nu)):
+        p.thm_nu[s] = 0.0
+    for name, v in zip(r.species, r.nu):
+        if name not in species:
+            raise InputError("reaction '%s' uses species '%s', which is not in <name_of_substrates>. This run has: %s"
+                             % (r.name, name, " ".join(species)))
+        p.thm_nu[species.index(name)] += v
+    p.thm_on = 1
+    p.thm_rt = M.rt()
+    p.thm_dg0 = r.dG0
+    p.thm_dgloss = r.dGloss
+    p.thm_threshold = r.atp * r.dGatp
+    p.thm_chi = r.chi
+    p.thm_cmin = r.cmin
+    p.thm_conc_scale = M.conc_scale
+
+
 @dataclass
 class Run:
     params: P.CaseParams
@@ -332,6 +498,13 @@ def load(xml_path, geometry_path: Optional[str] = None, precision=P.FP64, order=
             raise InputError("<reopen_fraction> must be strictly between 0 and 1")
         if p.dissol_is_precipitate:
             p.precip_phase_id = 1
+
+    # ---- thermodynamic gate (complab3d_integration.hh:315-326, 810-860)
+    if doc.flag("thermodynamics", "enabled"):
+        f = doc.text("thermodynamics", "energetics_file", default=None)
+        if not f:
+            raise InputError("<thermodynamics> is enabled but <energetics_file> is not set")
+        apply_thm(p, read_thm(f if os.path.isabs(f) else os.path.join(base, f)), names, microbes)
 
     # ---- diagnostics (complab3d_integration.hh:375-423)
     run_diag = doc.flag("diagnostics", "enabled")

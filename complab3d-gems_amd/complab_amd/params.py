@@ -99,6 +99,16 @@ class CaseParams(C.Structure):
         ("dissol_reopen_frac", _d),
         ("dissol_k", _d),
         ("dissol_stoich", _d * MAX_SPECIES),
+        ("thm_on", _i),
+        ("thm_pad", _i),
+        ("thm_rt", _d),
+        ("thm_dg0", _d),
+        ("thm_dgloss", _d),
+        ("thm_threshold", _d),
+        ("thm_chi", _d),
+        ("thm_cmin", _d),
+        ("thm_conc_scale", _d),
+        ("thm_nu", _d * MAX_SPECIES),
     ]
 
 
@@ -152,6 +162,11 @@ def default_params(nx, ny, nz, n_species):
     p.dissol_surface_only = 1
     p.dissol_interval = 1
     p.dissol_reopen_frac = 0.9
+    # the .thm defaults (complab3d_thermo.hh:174-197): 298.15 K, R in kJ/mol/K
+    p.thm_rt = 8.314462618e-3 * 298.15
+    p.thm_chi = 1.0
+    p.thm_cmin = 1e-9
+    p.thm_conc_scale = 1.0
     return p
 
 

@@ -101,6 +101,10 @@ struct KinDev {
     // launch) their abiotic increment is left for the pass after the park;
     // 2 (that pass) only they are visited
     int nearPhase;
+    // thermodynamic gate of the biotic rates (clb_case_params thm_*)
+    int thm;
+    double thmRT, thmDG0, thmLoss, thmThr, thmChi, thmCmin, thmScale;
+    double thmNu[CLB_DEV_MAXS];
 };
 
 template <typename R>

@@ -214,6 +214,10 @@ struct Case : CaseBase {
         k.dSurfaceOnly = p.dissol_surface_only; k.dK = p.dissol_k;
         for (int s = 0; s < CLB_DEV_MAXS; ++s) k.dStoich[s] = p.dissol_stoich[s];
         k.phase = phase; k.share = share; k.dcount = dcount;
+        k.thm = p.thm_on ? 1 : 0;
+        k.thmRT = p.thm_rt; k.thmDG0 = p.thm_dg0; k.thmLoss = p.thm_dgloss; k.thmThr = p.thm_threshold;
+        k.thmChi = p.thm_chi; k.thmCmin = p.thm_cmin; k.thmScale = p.thm_conc_scale;
+        for (int s = 0; s < CLB_DEV_MAXS; ++s) k.thmNu[s] = p.thm_nu[s];
         if (!k.bio && k.nrxn == 0 && !k.dissol) k.on = 0;
         return k;
     }
@@ -1227,6 +1231,13 @@ int checkParams(clb_ctx *ctx, const clb_case_params *p, const int8_t *geometry) 
     if (p->n_microbes < 0 || p->n_microbes > CLB_MAX_MICROBES) return fail(ctx, CLB_E_ARG, "n_microbes out of range");
     if (p->n_rxn < 0 || p->n_rxn > CLB_MAX_RXN) return fail(ctx, CLB_E_ARG, "n_rxn out of range");
     if (p->eq_on && (p->eq_nc < 0 || p->eq_nc > CLB_MAX_COMP)) return fail(ctx, CLB_E_ARG, "eq_nc out of range");
+    if (p->thm_on) {                                               // complab3d_thermo.hh:490-508
+        if (!(p->thm_chi > 0.0)) return fail(ctx, CLB_E_ARG, "thermodynamic gate: chi must be > 0");
+        if (!(p->thm_cmin > 0.0)) return fail(ctx, CLB_E_ARG, "thermodynamic gate: cmin must be > 0, or ln Q can be -inf");
+        if (!(p->thm_rt > 0.0)) return fail(ctx, CLB_E_ARG, "thermodynamic gate: R T must be > 0");
+        if (!(p->thm_conc_scale > 0.0)) return fail(ctx, CLB_E_ARG, "thermodynamic gate: conc_scale must be > 0");
+        if (p->thm_dgloss < 0.0) return fail(ctx, CLB_E_ARG, "thermodynamic gate: dGloss must be >= 0");
+    }
     if (p->eq_on && (p->eq_anderson_depth < 1 || p->eq_anderson_depth > 4)) return fail(ctx, CLB_E_ARG, "eq_anderson_depth must be 1..4");
     for (int s = 0; s < p->n_species; ++s) {
         if (p->left_btype[s] < 0 || p->left_btype[s] > 2 || p->right_btype[s] < 0 || p->right_btype[s] > 2)

@@ -126,7 +126,27 @@ __device__ __forceinline__ void adeDirichlet(T fl[7], int atLeft, T rhoD) {
 // ------------------------------------------------------------------------------------------
 // kinetics (the rate laws of the reference's compiled-kinetics headers)
 // ------------------------------------------------------------------------------------------
-// dual Monod, config/kinetics/defineKinetics.biotic.hh:90-126
+// the thermodynamic factor F_T of complab3d_thermo.hh:215-255 (Jin & Bethke):
+// ln Q over the block's species with activities floored at cmin, dG, the
+// driving force past the ATP threshold, 0 at or past it, 1 where exp underflows
+__device__ double thermoGate(const KinDev &k, const double *C) {
+    double lnQ = 0.0;
+    for (int s = 0; s < k.S; ++s) {
+        if (k.thmNu[s] == 0.0) continue;
+        double a = C[s] * k.thmScale;
+        if (!(a > k.thmCmin)) a = k.thmCmin;
+        lnQ += k.thmNu[s] * log(a);
+    }
+    const double dG = k.thmDG0 + k.thmRT * lnQ + k.thmLoss;
+    const double x = (dG + k.thmThr) / ((k.thmChi > 0.0 ? k.thmChi : 1.0) * k.thmRT);
+    if (x >= 0.0) return 0.0;
+    if (x < -700.0) return 1.0;
+    const double F = 1.0 - exp(x);
+    return F < 0.0 ? 0.0 : (F > 1.0 ? 1.0 : F);
+}
+
+// dual Monod, config/kinetics/defineKinetics.biotic.hh:90-126; with the
+// thermodynamic gate the whole rate vector is scaled by F_T (part1:127-140)
 __device__ void bioticLaw(const KinDev &k, const double *C, const double *B, double *sr, double *br, int mask) {
     for (int s = 0; s < k.S; ++s) sr[s] = 0;
     for (int m = 0; m < k.M; ++m) br[m] = 0;
@@ -144,6 +164,13 @@ __device__ void bioticLaw(const KinDev &k, const double *C, const double *B, dou
     }
     for (int s = 0; s < k.S; ++s) if (!isfinite(sr[s])) sr[s] = 0;
     for (int m = 0; m < k.M; ++m) if (!isfinite(br[m])) br[m] = 0;
+    if (k.thm) {
+        const double F = thermoGate(k, C);
+        if (F < 1.0) {
+            for (int s = 0; s < k.S; ++s) sr[s] *= F;
+            for (int m = 0; m < k.M; ++m) br[m] *= F;
+        }
+    }
 }
 
 // mass action with the ABC per-step cap, ABC_sweep_package/kinetics/defineAbioticKinetics.hh:166-210

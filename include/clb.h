@@ -164,6 +164,21 @@ typedef struct clb_case_params {
     double  dissol_reopen_frac;        /* <reopen_fraction>                                     */
     double  dissol_k;                  /* rate = k max(C[acid],0) max(mineral,0), mineralR = -rate */
     double  dissol_stoich[CLB_MAX_SPECIES]; /* subsR[s] = stoich[s] rate (defineDissolutionRate) */
+    /* thermodynamic gate of the biotic rates, one .thm reaction block
+     * (complab3d_thermo.hh:215-255, applied at part1:127-140): the voxel's
+     * whole biotic rate vector times F_T = max(0, 1 - exp(f / (chi RT))),
+     * f = dG0 + RT ln Q + dGloss + atp dGatp,
+     * ln Q = sum_s nu_s ln max(conc_scale C_s, cmin); energies in kJ/mol */
+    int32_t thm_on;
+    int32_t thm_pad;
+    double  thm_rt;                    /* gas constant x temperature                          */
+    double  thm_dg0;
+    double  thm_dgloss;                /* >= 0                                                 */
+    double  thm_threshold;             /* atp x dGatp                                          */
+    double  thm_chi;                   /* > 0                                                  */
+    double  thm_cmin;                  /* activity floor, > 0                                  */
+    double  thm_conc_scale;            /* lattice concentration -> mol/L                       */
+    double  thm_nu[CLB_MAX_SPECIES];   /* stoichiometry: negative consumed, positive produced */
 } clb_case_params;
 
 typedef struct clb_ctx clb_ctx;

@@ -206,6 +206,28 @@ void adeStream(Case &c, int s) {
 /* ------------------------------------------------------------------------------------
  *  kinetics
  * ------------------------------------------------------------------------------------ */
+/* the thermodynamic factor (complab3d_thermo.hh:215-255, Jin & Bethke 2003):
+ * ln Q = sum nu_s ln max(conc_scale C_s, cmin); dG = dG0 + RT ln Q + dGloss;
+ * f = dG + atp dGatp; F_T = max(0, 1 - exp(f / (chi RT))), 0 for f >= 0,
+ * 1 where f / (chi RT) < -700 */
+double thermoFactor(const clb_case_params &p, const double *C, double *dGout) {
+    double lnQ = 0.0;
+    for (int s = 0; s < p.n_species; ++s) {
+        if (p.thm_nu[s] == 0.0) continue;
+        double a = C[s] * p.thm_conc_scale;
+        if (!(a > p.thm_cmin)) a = p.thm_cmin;
+        lnQ += p.thm_nu[s] * std::log(a);
+    }
+    const double dG = p.thm_dg0 + p.thm_rt * lnQ + p.thm_dgloss;
+    if (dGout) *dGout = dG;
+    const double chi = p.thm_chi > 0.0 ? p.thm_chi : 1.0;
+    const double x = (dG + p.thm_threshold) / (chi * p.thm_rt);
+    if (x >= 0.0) return 0.0;
+    if (x < -700.0) return 1.0;
+    const double F = 1.0 - std::exp(x);
+    return F < 0.0 ? 0.0 : (F > 1.0 ? 1.0 : F);
+}
+
 /* dual-Monod restatement of defineRxnKinetics (config/kinetics/defineKinetics.biotic.hh:90-126):
  * growth = mu_max * S/(Ks+S) [* E/(Ke+E)] * B ; bioR = growth - kd*B ; subsR_s += nu_s*growth */
 void bioticLaw(const clb_case_params &p, const double *C, const double *B, double *subsR, double *bioR, int mask) {
@@ -225,6 +247,13 @@ void bioticLaw(const clb_case_params &p, const double *C, const double *B, doubl
     }
     for (int s = 0; s < p.n_species; ++s) if (!std::isfinite(subsR[s])) subsR[s] = 0;
     for (int m = 0; m < p.n_microbes; ++m) if (!std::isfinite(bioR[m])) bioR[m] = 0;
+    if (p.thm_on) {                                                    // part1:127-140
+        const double F = thermoFactor(p, C, nullptr);
+        if (F < 1.0) {
+            for (int s = 0; s < p.n_species; ++s) subsR[s] *= F;
+            for (int m = 0; m < p.n_microbes; ++m) bioR[m] *= F;
+        }
+    }
 }
 
 /* mass-action restatement of defineAbioticRxnKinetics
@@ -1570,6 +1599,12 @@ int orc_eq_voxel(const clb_case_params *p, const double *cin, double *cout) {
     return 0;
 }
 
+/* the thermodynamic factor and dG of one composition, for the tests */
+int orc_thermo_factor(const clb_case_params *p, const double *C, double *F, double *dG) {
+    *F = thermoFactor(*p, C, dG);
+    return 0;
+}
+
 /* a kinetics evaluation on one voxel (biotic then abiotic law), for the tests */
 int orc_rates(const clb_case_params *p, const double *C, const double *B, int mask, double *subsR, double *bioR, double *abioR) {
     bioticLaw(*p, C, B, subsR, bioR, mask);
@@ -1684,5 +1719,6 @@ extern "C" int orc_params_layout(int64_t *out) {
     out[4] = (int64_t) offsetof(clb_case_params, eq_stoich);
     out[5] = (int64_t) offsetof(clb_case_params, precip_on);
     out[6] = (int64_t) offsetof(clb_case_params, dissol_phase_id);
+    out[7] = (int64_t) offsetof(clb_case_params, thm_nu);
     return 0;
 }

@@ -59,6 +59,7 @@ def lib():
             ("orc_eq_solve", [P, P, P, P, P, P]),
             ("orc_eq_voxel", [P, P, P]),
             ("orc_rates", [P, P, P, C.c_int, P, P, P]),
+            ("orc_thermo_factor", [P, P, P, P]),
             ("orc_rng_draws", [C.c_uint64, C.c_int, P]),
             ("orc_set_threads", [C.c_int]),
             ("orc_geom_spheres", [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64, C.c_int, P]),
@@ -225,6 +226,15 @@ def rates(params, conc, bio, mask):
     sr = np.zeros(params.n_species); br = np.zeros(max(1, params.n_microbes)); ar = np.zeros(params.n_species)
     lib().orc_rates(C.byref(params), _ptr(c), _ptr(b), mask, _ptr(sr), _ptr(br), _ptr(ar))
     return sr, br, ar
+
+
+def thermo_factor(params, conc):
+    """(F_T, dG) of one composition under the parameter block's gate."""
+    c = np.zeros(max(params.n_species, 1))
+    c[:len(conc)] = conc
+    F = C.c_double(0.0); dG = C.c_double(0.0)
+    lib().orc_thermo_factor(C.byref(params), _ptr(c), C.byref(F), C.byref(dG))
+    return F.value, dG.value
 
 
 def rng_draws(seed, n):

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_entry_point():
 
 
 def test_ctypes_params_match_c_layout():
-    out = np.zeros(7, dtype=np.int64)
+    out = np.zeros(8, dtype=np.int64)
     O.lib().orc_params_layout(out.ctypes.data_as(C.c_void_p))
     assert out[0] == C.sizeof(P.CaseParams)
 

@@ -232,3 +232,42 @@ def test_fp32_close_to_fp64(ctx):
         e = relerr(g32.read(P.FIELD_CONC, s), g64.read(P.FIELD_CONC, s), om)
         print("fp32 species", s, e)
         assert e < 1e-3
+
+
+def test_thermodynamic_gate_biotic_fp64(ctx):
+    """The biotic case with the thermodynamic gate on the dual-Monod rates:
+    dG0 set so that F_T starts near 0.86 at the initial composition and falls
+    as the products build up."""
+    print("case: biotic 0.12-scale pack with the thermodynamic gate", flush=True)
+    w = configs.config2(scale=0.12)
+    w.transport.stabilize_steps = 30
+    w.params.ns_max_iter = 3000
+    p = w.params
+    p.mu_max[0] = 2e-2
+    p.thm_on = 1
+    p.thm_rt = 8.314462618e-3 * 298.15
+    p.thm_threshold = 12.5
+    nu = [-1.0, -1.0, 1.0, 1.0]
+    for s, v in enumerate(nu):
+        p.thm_nu[s] = v
+    lnq = sum(v * np.log(max(p.c0[s], p.thm_cmin)) for s, v in enumerate(nu))
+    p.thm_dg0 = -12.5 - 2.0 * p.thm_rt - p.thm_rt * lnq                  # f / RT = -2 at c0
+    g, o = both(ctx, w)
+    ro = driver.setup(o, w.params, w.flow, w.transport)
+    rg = driver.setup(g, w.params, w.flow, w.transport)
+    assert rg.iters == ro.iters
+    for c in (g, o):
+        c.set_omegas(ro.omega_pore, ro.omega_biofilm, 1.0)
+    assert g.run_steps(60) == o.run_steps(60) == (0, 60)
+    assert np.array_equal(g.read(P.FIELD_MASK), o.read(P.FIELD_MASK))
+    om = open_mask(o)
+    assert relerr(g.read(P.FIELD_BIOMASS, 0), o.read(P.FIELD_BIOMASS, 0)) <= REL
+    for s in range(4):
+        assert relerr(g.read(P.FIELD_CONC, s), o.read(P.FIELD_CONC, s), om) <= REL
+    # the gate acted: the same case without it consumes more donor
+    w.params.thm_on = 0
+    u = O.OracleCase(w.params, w.geometry)
+    driver.setup(u, w.params, w.flow, w.transport)
+    u.set_omegas(ro.omega_pore, ro.omega_biofilm, 1.0)
+    u.run_steps(60)
+    assert u.species_totals()[0] < o.species_totals()[0]

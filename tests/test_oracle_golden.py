@@ -191,9 +191,9 @@ def test_pack_builder_conventions():
 
 
 def test_params_layout_matches_c():
-    out = np.zeros(7, dtype=np.int64)
+    out = np.zeros(8, dtype=np.int64)
     O.lib().orc_params_layout.argtypes = [C.c_void_p]
     O.lib().orc_params_layout(out.ctypes.data_as(C.c_void_p))
     CP = P.CaseParams
     assert list(out) == [C.sizeof(CP), CP.dt.offset, CP.bio_stoich.offset, CP.ca_seed.offset,
-                         CP.eq_stoich.offset, CP.precip_on.offset, CP.dissol_phase_id.offset]
+                         CP.eq_stoich.offset, CP.precip_on.offset, CP.dissol_phase_id.offset, CP.thm_nu.offset]
