@@ -118,10 +118,32 @@ struct NsDev {
     R *cur, *out;                  // [19][Nopen]
     const uint8_t *cls, *clsPull;  // [Nopen]
     const unsigned long long *kinds;  // [Nint] source kind of each moving population, 2 bits per direction
+    const int4 *nbr;               // [Nint][3] compact neighbour record (NsNbRecord), the interior's pull
     double om, omBio, rhoBarRight;
     int first;
     int *done;
 };
+
+// z component of D3Q19 direction q (the order of D19C in clb_common.h), for
+// compile-time use on both sides
+__host__ __device__ constexpr int d19z(int q) {
+    return (q == 5 || q == 11 || q == 14 || q == 15 || q == 18) ? 1
+         : ((q == 6 || q == 12 || q == 13 || q == 16 || q == 17) ? -1 : 0);
+}
+
+// The interior flow pull's neighbour indices in 48 bytes instead of 18 x 4:
+// the interior compact order is the dense order of the interior voxels, so
+// the source of direction q (at z - c_qz) is a small offset from one base per
+// z plane -- the voxel itself for c_qz = 0, the compact position of (x, y, z-1)
+// or (x, y, z+1) (the first listed voxel at or after it) otherwise.  A source
+// on an x plane, or an offset beyond int16, sets its bit in `far`, and the
+// pull takes that index from the full table.
+struct NsNbRecord {
+    int32_t baseM, baseP;          // compact position of (x, y, z-1) and (x, y, z+1)
+    int16_t d[18];                 // source index minus its plane's base, direction q = 1..18
+    uint32_t far;                  // bit q-1: index from the full table
+};
+static_assert(sizeof(NsNbRecord) == 48, "three 16-byte loads");
 
 struct NsTracer {
     double ring[NS_TRACER_WINDOW];

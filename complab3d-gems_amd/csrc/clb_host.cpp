@@ -13,6 +13,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "clb.h"
@@ -99,6 +100,7 @@ struct Case : CaseBase {
     // compact open-voxel layout (clb_device.h): list, neighbour tables, classes
     int Nopen = 0, Nint = 0, Nleft = 0;
     int *list = nullptr, *nb7 = nullptr, *nb19 = nullptr;
+    NsNbRecord *nsNbr = nullptr;                // [Nint] the interior flow pull's neighbour records
     uint8_t *adeCls = nullptr, *adeClsPull = nullptr, *nsCls = nullptr, *nsClsPull = nullptr, *reopened = nullptr;
     uint16_t *adeKinds = nullptr;
     unsigned long long *nsKinds = nullptr;
@@ -227,7 +229,7 @@ struct Case : CaseBase {
         a.nx = nx; a.ny = ny; a.nz = nz; a.N = N;
         a.Nopen = Nopen; a.Nint = Nint; a.Nleft = Nleft; a.list = list; a.nb = nb19; a.geom = geom;
         a.cur = gCur(); a.out = gOut();
-        a.cls = nsCls; a.clsPull = nsClsPull; a.kinds = nsKinds;
+        a.cls = nsCls; a.clsPull = nsClsPull; a.kinds = nsKinds; a.nbr = (const int4 *) nsNbr;
         a.om = nsOmega; a.omBio = nsBioOmega; a.rhoBarRight = -deltaP * 3.0;
         a.first = nsFirst ? 1 : 0;
         a.done = nsDone;
@@ -276,6 +278,38 @@ struct Case : CaseBase {
         if (gv > CLB_PORE) return p.perm_ratio > 1e-14 ? NS_BIO : NS_BB;
         return gv == CLB_WALL ? NS_BB : NS_NODYN;
     }
+    // the interior voxels' neighbour records (NsNbRecord) from the full D3Q19
+    // table; every index a record gives is checked against the table here
+    std::vector<NsNbRecord> nsRecords(const std::vector<int> &lst, const std::vector<int> &t19,
+                                      const int (*C)[3]) const {
+        std::vector<NsNbRecord> rec((size_t) std::max(Nint, 1));
+        auto basePos = [&](int x, int y, int z) -> int {       // first listed interior voxel at or after (x, y, z)
+            if (z < 0 || z >= nz) return 0;
+            const long long v = x + (long long) nx * (y + (long long) ny * z);
+            return (int) (std::lower_bound(lst.begin(), lst.begin() + Nint, (int) v) - lst.begin());
+        };
+        for (int i = 0; i < Nint; ++i) {
+            const long long v = lst[i];
+            const int x = (int) (v % nx), y = (int) ((v / nx) % ny), z = (int) (v / ((long long) nx * ny));
+            NsNbRecord &r = rec[i];
+            memset(&r, 0, sizeof(r));
+            r.baseM = basePos(x, y, z - 1);
+            r.baseP = basePos(x, y, z + 1);
+            for (int q = 1; q < 19; ++q) {
+                const int n = t19[(size_t) (q - 1) * Nopen + i];
+                if (n < 0) continue;                                // closed or outside: never decoded
+                const int cz = C[q][2];
+                if (cz != d19z(q)) throw HipFail{hipErrorInvalidValue, "D3Q19 direction table mismatch"};
+                const long long base = cz == 0 ? i : (cz > 0 ? r.baseM : r.baseP);
+                const long long dq = n - base;
+                if (n >= Nint || dq < -32768 || dq > 32767) { r.far |= 1u << (q - 1); continue; }
+                r.d[q - 1] = (int16_t) dq;
+                if (base + r.d[q - 1] != n) throw HipFail{hipErrorInvalidValue, "neighbour record does not decode"};
+            }
+        }
+        return rec;
+    }
+
     void create(const clb_case_params &pp, const int8_t *geometry, const double *biomass0, hipStream_t st) {
         p = pp;
         nx = p.nx; ny = p.ny; nz = p.nz; N = (long long) nx * ny * nz;
@@ -345,6 +379,7 @@ struct Case : CaseBase {
         if (p.flow_on) {
             gA = dalloc<RN>(allocs, 19 * (size_t) Nopen); gB = dalloc<RN>(allocs, 19 * (size_t) Nopen);
             nb19 = dalloc<int>(allocs, (size_t) 18 * Nopen);
+            nsNbr = dalloc<NsNbRecord>(allocs, std::max(Nint, 1));
             nsKinds = dalloc<unsigned long long>(allocs, std::max(Nint, 1));
         }
         u = dalloc<R>(allocs, 3 * (size_t) Nopen);
@@ -400,6 +435,8 @@ struct Case : CaseBase {
             if (p.flow_on) {
                 std::vector<int> t19 = nbTable(19, C19h);
                 HIPT(hipMemcpy(nb19, t19.data(), t19.size() * sizeof(int), hipMemcpyHostToDevice));
+                std::vector<NsNbRecord> rec = nsRecords(lst, t19, C19h);
+                HIPT(hipMemcpy(nsNbr, rec.data(), rec.size() * sizeof(NsNbRecord), hipMemcpyHostToDevice));
             }
         }
         HIPT(hipMemcpyAsync(adeCls, ac.data(), Nopen, hipMemcpyHostToDevice, st));

@@ -917,10 +917,26 @@ __device__ __forceinline__ void pull19k(const NsDev<R> &a, int i, T fl[19]) {
         for (int q = 0; q < 19; ++q) fl[q] = (T) a.cur[q * No + i];
         return;
     }
+    // the compact neighbour record: three 16-byte loads for all 18 indices
+    const int4 r0 = a.nbr[3 * (long long) i], r1 = a.nbr[3 * (long long) i + 1], r2 = a.nbr[3 * (long long) i + 2];
+    const int w[8] = {r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y};
+    const int w8 = r2.z;
+    const unsigned int far = (unsigned int) r2.w;
+    const unsigned long long kd = a.kinds[i];
     int nbq[18];
 #pragma unroll
-    for (int q = 1; q < 19; ++q) nbq[q - 1] = a.nb[(q - 1) * No + i];
-    const unsigned long long kd = a.kinds[i];
+    for (int q = 1; q < 19; ++q) {
+        const int h = q - 1;
+        const int word = h < 16 ? w[h >> 1] : w8;
+        const int dq = (h & 1) ? (word >> 16) : (int) (short) (word & 0xffff);
+        const int cz = d19z(q);
+        nbq[h] = (cz == 0 ? i : (cz > 0 ? r0.x : r0.y)) + dq;
+    }
+    if (far) {
+#pragma unroll
+        for (int q = 1; q < 19; ++q)
+            if ((far >> (q - 1)) & 1u) nbq[q - 1] = a.nb[(q - 1) * No + i];
+    }
     fl[0] = (T) a.cur[i];
 #pragma unroll
     for (int q = 1; q < 19; ++q) {
