@@ -201,6 +201,11 @@ struct EqDev {
 // the interior kernel, then (when given) an event, then the planes
 template <typename R> hipError_t launchAdeStep(const AdeDev<R> &, const KinDev &, hipStream_t, hipEvent_t afterInterior);
 template <typename R> hipError_t launchKinetics(const AdeDev<R> &, const KinDev &, hipStream_t);
+// the kinetics over a device-held list of voxels, and the list of the voxels next to the dissolving phase
+template <typename R> hipError_t launchKineticsListed(const AdeDev<R> &, const KinDev &, const int *sel,
+                                                     const unsigned int *count, hipStream_t);
+template <typename R> hipError_t launchNearPhaseList(const AdeDev<R> &, const KinDev &, int *sel, unsigned int *count,
+                                                    hipStream_t);
 template <typename R> hipError_t launchDissolPark(const AdeDev<R> &, const KinDev &, hipStream_t);
 template <typename R> hipError_t launchDissolConvert(const AdeDev<R> &, int8_t *phase, int phaseId, int mineralSlot,
                                                      double threshold, unsigned int *count, hipStream_t);

@@ -145,6 +145,11 @@ struct Case : CaseBase {
     // collected, clamped)
     double *share = nullptr;
     unsigned long long *dcount = nullptr;
+    // the voxels next to the dissolving phase (the gather's voxels), rebuilt
+    // on the device after the phases change
+    int *nearList = nullptr;
+    unsigned int *nearCnt = nullptr;
+    bool nearDirty = true;
     double gmaxClosed = -1e300;
     long long msumClosed = 0;
     int batchLen = 8;
@@ -395,7 +400,11 @@ struct Case : CaseBase {
         nsDone = dalloc<int>(allocs, 1);
         tracer = dalloc<NsTracer>(allocs, 1);
         ctl = dalloc<StepCtl>(allocs, 1);
-        if (p.dissol_on) share = dalloc<double>(allocs, (size_t) std::max(p.n_species, 1) * Nopen);
+        if (p.dissol_on) {
+            share = dalloc<double>(allocs, (size_t) std::max(p.n_species, 1) * Nopen);
+            nearList = dalloc<int>(allocs, std::max(Nint, 1));
+            nearCnt = dalloc<unsigned int>(allocs, 1);
+        }
         dcount = dalloc<unsigned long long>(allocs, 3);
         caPartMax = dalloc<double>(allocs, nblk(Nint, 256) + nblk(Nopen - Nint, 256) + 1);
         caPartMask = dalloc<long long>(allocs, nblk(Nint, 256) + nblk(Nopen - Nint, 256) + 1);
@@ -621,6 +630,7 @@ struct Case : CaseBase {
         nsSaturate = h.nsSaturate; batchLen = h.batchLen; caSweep_ = h.caSweep;
         for (int k = 0; k < CLB_NCOUNTERS; ++k) cnt[k] = h.cnt[k];
         kms = 0; klaunch = 0;
+        nearDirty = true;
         return {CLB_OK, ""};
     }
 
@@ -750,16 +760,20 @@ struct Case : CaseBase {
     // with the gather at the waiting voxels (the reference's order,
     // complab.cpp:2461-2545).
     void transportLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr, bool maskTotal = false) {
-        if (!kinOn || !p.dissol_on) { fusedLaunch(st, kinOn, halt, true, maskTotal); return; }
-        fusedLaunch(st, true, halt, true, maskTotal, 1);
+        if (!kinOn || !p.dissol_on) { fusedLaunch(st, kinOn, halt, maskTotal); return; }
+        if (nearDirty) {
+            HIPT(launchNearPhaseList<R>(ade(), kin(true), nearList, nearCnt, st));
+            nearDirty = false;
+        }
+        fusedLaunch(st, true, halt, maskTotal, 1);
         KinDev k = kin(true);
         k.bio = 0;
         k.nearPhase = 2;
         HIPT(launchDissolPark<R>(ade(halt), k, st));
-        HIPT(launchKinetics<R>(ade(halt), k, st));
+        HIPT(launchKineticsListed<R>(ade(halt), k, nearList, nearCnt, st));
     }
-    void fusedLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr, bool abioticInFused = true,
-                     bool maskTotal = false, int nearPhase = 0) {
+    void fusedLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr, bool maskTotal = false,
+                     int nearPhase = 0) {
         const bool timed = launchNo % timeStride == 0;
         if (timed && evUsed == evpool.size()) {
             if (evpool.size() >= kEvPool) flushTimes(st);
@@ -774,7 +788,6 @@ struct Case : CaseBase {
         KinDev k = kin(kinOn);
         k.dissol = 0;
         k.nearPhase = nearPhase;
-        if (!abioticInFused) k.nrxn = 0;
         if (!k.bio && k.nrxn == 0) k.on = 0;
         if (timed) HIPT(hipEventRecord(evpool[evUsed].first, st));
         HIPT(launchAdeStep<R>(a, k, st, timed ? evpool[evUsed].second : nullptr));
@@ -827,6 +840,7 @@ struct Case : CaseBase {
                                     p.dissol_reopen_frac * p.dissol_full, ucnt + 3, st));
         HIPT(hipMemcpyAsync(&n, ucnt + 3, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
         HIPT(hipStreamSynchronize(st));
+        if (n) nearDirty = true;
         if (reopenedOut) *reopenedOut = (int) n;
         return {CLB_OK, ""};
     }
@@ -855,6 +869,7 @@ struct Case : CaseBase {
             HIPT(launchPrecip<R>(nx, ny, N, mask, imm + (size_t) slot[p.precip_species] * N, p.precip_max_rho, p.precip_phase_id, phase, ucnt, st));
             HIPT(hipMemcpyAsync(&ch, ucnt, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
             HIPT(hipStreamSynchronize(st));
+            if (ch) nearDirty = true;
         }
         if (changed) *changed = ch ? 1 : 0;
     }
@@ -988,6 +1003,7 @@ struct Case : CaseBase {
             HIPT(hipStreamSynchronize(st));
             if (ch) {
                 cnt[5]++;
+                nearDirty = true;
                 if (p.flow_on && !nsSaturate) { Status r = resolve(st, p.precip_perm_ratio); if (r.code) return r; }
                 unsigned int nre = 0;
                 HIPT(hipMemsetAsync(ucnt + 1, 0, sizeof(unsigned int), st));

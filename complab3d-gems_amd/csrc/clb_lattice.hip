@@ -588,10 +588,7 @@ __global__ void __launch_bounds__(256) k_ade_planes(AdeDev<R> a) {
 // the fused kernel, for callers that apply the reference's processors one by
 // one); bit-identical to the fused kernel's kinetics
 template <typename R, int MAXS, bool DISSOL>
-__global__ void __launch_bounds__(256) k_kinetics(AdeDev<R> a, KinDev k) {
-    if (halted(a.halt)) return;
-    const int i = xcdBlock() * blockDim.x + threadIdx.x;
-    if (i >= a.Nint) return;
+__device__ __forceinline__ void kineticsVoxel(const AdeDev<R> &a, const KinDev &k, int i) {
     if (a.cls[i] == ADE_BB) return;
     const long long v = a.list[i];
     const int mask = a.mask[v];
@@ -615,6 +612,38 @@ __global__ void __launch_bounds__(256) k_kinetics(AdeDev<R> a, KinDev k) {
 #pragma unroll
         for (int q = 0; q < 7; ++q) c[q * No + i] = f[ms][q];
     }
+}
+template <typename R, int MAXS, bool DISSOL>
+__global__ void __launch_bounds__(256) k_kinetics(AdeDev<R> a, KinDev k) {
+    if (halted(a.halt)) return;
+    const int i = xcdBlock() * blockDim.x + threadIdx.x;
+    if (i >= a.Nint) return;
+    kineticsVoxel<R, MAXS, DISSOL>(a, k, i);
+}
+// the same over a list of voxels whose length the device holds (the voxels
+// next to the dissolving phase): a grid-stride loop of a fixed grid
+template <typename R, int MAXS, bool DISSOL>
+__global__ void __launch_bounds__(256) k_kinetics_listed(AdeDev<R> a, KinDev k, const int *sel, const unsigned int *count) {
+    if (halted(a.halt)) return;
+    const unsigned int n = *count;
+    for (unsigned int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+        kineticsVoxel<R, MAXS, DISSOL>(a, k, sel[t]);
+}
+
+// the compact voxels next to the dissolving phase (nextToPhase), appended in
+// any order: one atomic per wave
+template <typename R>
+__global__ void __launch_bounds__(256) k_near_phase_list(AdeDev<R> a, KinDev k, int *sel, unsigned int *count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool near = i < a.Nint && nextToPhase(a, k, i);
+    const unsigned long long b = __ballot(near);
+    if (!b) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long) b) - 1;
+    unsigned int base = 0;
+    if (lane == leader) base = atomicAdd(count, (unsigned int) __popcll(b));
+    base = __shfl(base, leader, 64);
+    if (near) sel[base + __popcll(b & ((1ull << lane) - 1))] = i;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1138,6 +1167,10 @@ __global__ void __launch_bounds__(1024) k_ns_tracer(NsTracer *t, const double *p
 // the flow density computeDensity reports, into a dense field: 1 + the sum of
 // the post-stream populations at an open interior voxel, the imposed density
 // on an open pressure plane, 1 at closed voxels
+__global__ void k_fill_u32(unsigned int *dst, unsigned int n, unsigned int v) {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
 __global__ void k_fill_d(double *dst, long long n, double v) {
     const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = v;
@@ -1297,6 +1330,31 @@ hipError_t launchAdeStep(const AdeDev<R> &a, const KinDev &k, hipStream_t st, hi
     return hipGetLastError();
 }
 template <typename R, bool DISSOL>
+static void kineticsListedVariants(const AdeDev<R> &a, const KinDev &k, const int *sel, const unsigned int *count,
+                                   hipStream_t st) {
+    const unsigned int g = nblk(a.Nint, 256) < 1024u ? nblk(a.Nint, 256) : 1024u;
+    if (a.Sm <= 1) hipLaunchKernelGGL((k_kinetics_listed<R, 1, DISSOL>), dim3(g), dim3(256), 0, st, a, k, sel, count);
+    else if (a.Sm <= 2) hipLaunchKernelGGL((k_kinetics_listed<R, 2, DISSOL>), dim3(g), dim3(256), 0, st, a, k, sel, count);
+    else if (a.Sm <= 3) hipLaunchKernelGGL((k_kinetics_listed<R, 3, DISSOL>), dim3(g), dim3(256), 0, st, a, k, sel, count);
+    else if (a.Sm <= 4) hipLaunchKernelGGL((k_kinetics_listed<R, 4, DISSOL>), dim3(g), dim3(256), 0, st, a, k, sel, count);
+    else if (a.Sm <= 6) hipLaunchKernelGGL((k_kinetics_listed<R, 6, DISSOL>), dim3(g), dim3(256), 0, st, a, k, sel, count);
+    else hipLaunchKernelGGL((k_kinetics_listed<R, 8, DISSOL>), dim3(g), dim3(256), 0, st, a, k, sel, count);
+}
+template <typename R>
+hipError_t launchKineticsListed(const AdeDev<R> &a, const KinDev &k, const int *sel, const unsigned int *count,
+                                hipStream_t st) {
+    if (a.Nint <= 0 || !k.on) return hipSuccess;
+    if (k.dissol) kineticsListedVariants<R, true>(a, k, sel, count, st);
+    else kineticsListedVariants<R, false>(a, k, sel, count, st);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchNearPhaseList(const AdeDev<R> &a, const KinDev &k, int *sel, unsigned int *count, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, count, 1u, 0u);
+    if (a.Nint > 0) hipLaunchKernelGGL(k_near_phase_list<R>, dim3(nblk(a.Nint, 256)), dim3(256), 0, st, a, k, sel, count);
+    return hipGetLastError();
+}
+template <typename R, bool DISSOL>
 static void kineticsVariants(const AdeDev<R> &a, const KinDev &k, hipStream_t st) {
     const unsigned int g = nblk(a.Nint, 256);
     if (a.Sm <= 1) hipLaunchKernelGGL((k_kinetics<R, 1, DISSOL>), dim3(g), dim3(256), 0, st, a, k);
@@ -1414,6 +1472,8 @@ hipError_t launchGatherCompact(const double *src, const int *list, int n, R *dst
 #define INST(R) \
     template hipError_t launchAdeStep<R>(const AdeDev<R> &, const KinDev &, hipStream_t, hipEvent_t); \
     template hipError_t launchKinetics<R>(const AdeDev<R> &, const KinDev &, hipStream_t); \
+    template hipError_t launchKineticsListed<R>(const AdeDev<R> &, const KinDev &, const int *, const unsigned int *, hipStream_t); \
+    template hipError_t launchNearPhaseList<R>(const AdeDev<R> &, const KinDev &, int *, unsigned int *, hipStream_t); \
     template hipError_t launchDissolPark<R>(const AdeDev<R> &, const KinDev &, hipStream_t); \
     template hipError_t launchDissolConvert<R>(const AdeDev<R> &, int8_t *, int, int, double, unsigned int *, hipStream_t); \
     template hipError_t launchAdeDensity<R>(const AdeDev<R> &, int, double *, hipStream_t); \

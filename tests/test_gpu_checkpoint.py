@@ -69,9 +69,10 @@ def test_precipitation_dissolution_case_resumes_bit_identically(ctx):
     assert c1[4] > c0[4] and c1[8] > c0[8]                          # flow re-solves, dissolution after the save
 
 
-def test_fp32_case_resumes_bit_identically(ctx):
-    print("case: biotic FP32 0.1-scale pack, checkpoint after 10 steps, 20 more", flush=True)
-    w = configs.config2(scale=0.1, precision=P.FP32)
+@pytest.mark.parametrize("prec", [P.FP32, P.FP32_ADE])
+def test_reduced_precision_case_resumes_bit_identically(ctx, prec):
+    print("case: biotic 0.1-scale pack precision %d, checkpoint after 10 steps, 20 more" % prec, flush=True)
+    w = configs.config2(scale=0.1, precision=prec)
     w.transport.stabilize_steps = 20
     w.params.ns_max_iter = 2000
     continue_both(ctx, w, 10, 20)
