@@ -299,6 +299,7 @@ class Run:
     chk_interval: int = 1000000                 # <IO><save_CHK_interval>
     mask_filename: str = "maskLattice"
     summary_csv: str = "summary.csv"
+    symbolic: dict = field(default_factory=dict)   # microbe index -> params.SymProgram
 
 
 def load(xml_path, geometry_path: Optional[str] = None, precision=P.FP64, order="slices") -> Run:
@@ -372,14 +373,16 @@ def load(xml_path, geometry_path: Optional[str] = None, precision=P.FP64, order=
     if M > P.MAX_MICROBES:
         raise InputError("%d microbes, this path holds %d" % (M, P.MAX_MICROBES))
     perm = []
+    rtypes = []
     for m in range(M):
         tag = "microbe%d" % m
         solver = doc.text("microbiology", tag, "solver_type").lower()
         if solver not in ("ca", "cellular automata", "cellular_automata"):
             raise InputError("%s: solver_type %s; only cellular-automaton biofilms are on this path" % (tag, solver))
         rt = doc.text("microbiology", tag, "reaction_type", default="kinetics").lower()
-        if rt not in ("kinetics", "none"):
+        if rt not in ("kinetics", "none", "symbolic"):
             raise InputError("%s: reaction_type %s is not on this path" % (tag, rt))
+        rtypes.append(rt)
         microbes.append(doc.text("microbiology", tag, "name_of_microbes", default=tag))
         b0 = doc.nums("microbiology", tag, "initial_densities")
         bio_mats = doc.nums("LB_numerics", "domain", "material_numbers", tag, default=None)
@@ -398,6 +401,31 @@ def load(xml_path, geometry_path: Optional[str] = None, precision=P.FP64, order=
             raise InputError("%s: viscosity_ratio_in_biofilm must be given for a CA biofilm" % tag)
         perm.append(1.0 / ratio if ratio > 1e-14 else -99.0)
     p.n_microbes = M
+
+    # ---- symbolic rate laws (complab3d_integration.hh:264-300; chooseFiles,
+    # complab3d_symbolic.hh:910-965): a microbe's own <expressions_file>, else
+    # the shared <symbolic><expressions_file>
+    symbolic = {}
+    if "symbolic" in rtypes:
+        if "kinetics" in rtypes:
+            raise InputError("symbolic and compiled-kinetics organisms in one run are not combined on this path")
+        if not doc.flag("symbolic", "enabled"):
+            raise InputError("a microbe has reaction_type symbolic but <symbolic><enabled> is not true")
+        if doc.text("symbolic", "abiotic_file", default=None):
+            raise InputError("<symbolic><abiotic_file>: abiotic symbolic laws are not on this path")
+        shared = doc.text("symbolic", "expressions_file", default=None)
+        from . import symbolic as SYM
+        for m, rt in enumerate(rtypes):
+            if rt != "symbolic":
+                continue
+            f = doc.text("microbiology", "microbe%d" % m, "expressions_file", default=None) or shared
+            if not f:
+                raise InputError("%s has reaction_type symbolic but no rate law" % microbes[m])
+            try:
+                law = SYM.read_sym(f if os.path.isabs(f) else os.path.join(base, f))
+                symbolic[m] = SYM.compile_law(law, names, microbes[m])
+            except SYM.SymError as e:
+                raise InputError(str(e))
     if M > 0:
         if len(set(perm)) != 1:
             raise InputError("one biofilm viscosity ratio for every microbe on this path")
@@ -537,4 +565,5 @@ def load(xml_path, geometry_path: Optional[str] = None, precision=P.FP64, order=
                vtk_interval=doc.num("IO", "save_VTK_interval", default=1000, cast=int),
                chk_interval=doc.num("IO", "save_CHK_interval", default=1000000, cast=int),
                mask_filename=doc.text("IO", "mask_filename", default="maskLattice"),
-               summary_csv=doc.text("diagnostics", "summary_csv", default="summary.csv"))
+               summary_csv=doc.text("diagnostics", "summary_csv", default="summary.csv"),
+               symbolic=symbolic)

@@ -100,7 +100,7 @@ class CaseParams(C.Structure):
         ("dissol_k", _d),
         ("dissol_stoich", _d * MAX_SPECIES),
         ("thm_on", _i),
-        ("thm_pad", _i),
+        ("thm_microbe", _i),
         ("thm_rt", _d),
         ("thm_dg0", _d),
         ("thm_dgloss", _d),
@@ -167,7 +167,28 @@ def default_params(nx, ny, nz, n_species):
     p.thm_chi = 1.0
     p.thm_cmin = 1e-9
     p.thm_conc_scale = 1.0
+    p.thm_microbe = -1
     return p
+
+
+SYM_MAXOPS, SYM_MAXCONST, SYM_MAXVARS, SYM_MAXRATES, SYM_STACK = 256, 64, 16, 16, 16
+(SYM_CONST, SYM_VAR, SYM_NEG, SYM_ADD, SYM_SUB, SYM_MUL, SYM_DIV, SYM_POW, SYM_EXP, SYM_LOG, SYM_LOG10,
+ SYM_SQRT, SYM_ABS, SYM_TANH, SYM_MIN, SYM_MAX) = range(1, 17)
+
+
+class SymProgram(C.Structure):
+    """clb_sym_program of include/clb.h."""
+    _fields_ = [
+        ("n_ops", _i), ("n_vars", _i), ("n_rates", _i), ("n_const", _i),
+        ("op", C.c_uint8 * SYM_MAXOPS),
+        ("arg", C.c_int16 * SYM_MAXOPS),
+        ("rate_end", C.c_int16 * SYM_MAXRATES),
+        ("var_species", C.c_int16 * SYM_MAXVARS),
+        ("rate_species", C.c_int16 * SYM_MAXRATES),
+        ("lo", _d * SYM_MAXVARS), ("hi", _d * SYM_MAXVARS),
+        ("konst", _d * SYM_MAXCONST),
+        ("unit_scale", _d),
+    ]
 
 
 def copy_params(p):

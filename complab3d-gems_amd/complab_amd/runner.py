@@ -52,6 +52,8 @@ def run(r: inputs.Run, ctx=None, case=None, out_dir: Optional[str] = None, steps
     out = out_dir or r.output_path
     if case is None:
         case = Case(ctx or Context(0), r.params, r.geometry)
+    for m, prog in r.symbolic.items():
+        case.set_symbolic(m, prog)
     say = (lambda m: print(m, file=log, flush=True)) if log else (lambda m: None)
     iters = []
     if restart:

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "clb.h"
+
 #define CLB_DEV_MAXS 8
 #define CLB_DEV_MAXM 2
 #define CLB_DEV_MAXR 4
@@ -101,8 +103,11 @@ struct KinDev {
     // launch) their abiotic increment is left for the pass after the park;
     // 2 (that pass) only they are visited
     int nearPhase;
+    // symbolic laws of the biotic rates, one program per microbe (device memory)
+    const clb_sym_program *sym;
+    int symOn;
     // thermodynamic gate of the biotic rates (clb_case_params thm_*)
-    int thm;
+    int thm, thmMicrobe;
     double thmRT, thmDG0, thmLoss, thmThr, thmChi, thmCmin, thmScale;
     double thmNu[CLB_DEV_MAXS];
 };

@@ -73,6 +73,7 @@ struct CaseBase {
     virtual void soluteSolid(hipStream_t, int *reopened) = 0;
     virtual void precipConvert(hipStream_t, int *changed) = 0;
     virtual Status dissolConvert(hipStream_t, int *reopened) = 0;
+    virtual Status setSymbolic(int microbe, const clb_sym_program *prog) = 0;
     // checkpoint / restart of the device-resident state
     virtual size_t checkpointBytes() = 0;
     virtual Status checkpointSave(hipStream_t, uint8_t *dst, size_t n) = 0;
@@ -150,6 +151,11 @@ struct Case : CaseBase {
     int *nearList = nullptr;
     unsigned int *nearCnt = nullptr;
     bool nearDirty = true;
+    // symbolic laws: one program per microbe on the device; the biotic rates
+    // come from them when every microbe has one
+    clb_sym_program *symDev = nullptr;
+    uint32_t symMask = 0;
+    bool symOn = false;
     double gmaxClosed = -1e300;
     long long msumClosed = 0;
     int batchLen = 8;
@@ -221,7 +227,9 @@ struct Case : CaseBase {
         k.dSurfaceOnly = p.dissol_surface_only; k.dK = p.dissol_k;
         for (int s = 0; s < CLB_DEV_MAXS; ++s) k.dStoich[s] = p.dissol_stoich[s];
         k.phase = phase; k.share = share; k.dcount = dcount;
-        k.thm = p.thm_on ? 1 : 0;
+        k.sym = symDev; k.symOn = symOn ? 1 : 0;
+        if (symOn && p.n_microbes > 0) k.bio = 1;
+        k.thm = p.thm_on ? 1 : 0; k.thmMicrobe = p.thm_microbe;
         k.thmRT = p.thm_rt; k.thmDG0 = p.thm_dg0; k.thmLoss = p.thm_dgloss; k.thmThr = p.thm_threshold;
         k.thmChi = p.thm_chi; k.thmCmin = p.thm_cmin; k.thmScale = p.thm_conc_scale;
         for (int s = 0; s < CLB_DEV_MAXS; ++s) k.thmNu[s] = p.thm_nu[s];
@@ -400,6 +408,8 @@ struct Case : CaseBase {
         nsDone = dalloc<int>(allocs, 1);
         tracer = dalloc<NsTracer>(allocs, 1);
         ctl = dalloc<StepCtl>(allocs, 1);
+        symDev = dalloc<clb_sym_program>(allocs, (size_t) std::max(p.n_microbes, 1));
+        HIPT(hipMemsetAsync(symDev, 0, (size_t) std::max(p.n_microbes, 1) * sizeof(clb_sym_program), st));
         if (p.dissol_on) {
             share = dalloc<double>(allocs, (size_t) std::max(p.n_species, 1) * Nopen);
             nearList = dalloc<int>(allocs, std::max(Nint, 1));
@@ -518,6 +528,47 @@ struct Case : CaseBase {
         p.dt = dt;
     }
 
+    // ------------------------------------------------------------ symbolic laws
+    Status setSymbolic(int microbe, const clb_sym_program *prog) override {
+        if (microbe < 0 || microbe >= p.n_microbes) return {CLB_E_ARG, "microbe index out of range"};
+        if (prog) {
+            if (p.dissol_on) return {CLB_E_ARG, "symbolic rate laws are not combined with dissolution here"};
+            const clb_sym_program &P = *prog;
+            if (P.n_ops < 0 || P.n_ops > CLB_SYM_MAXOPS || P.n_vars < 0 || P.n_vars > CLB_SYM_MAXVARS ||
+                P.n_rates < 1 || P.n_rates > CLB_SYM_MAXRATES || P.n_const < 0 || P.n_const > CLB_SYM_MAXCONST)
+                return {CLB_E_ARG, "symbolic law: a count is out of range"};
+            if (!(P.unit_scale > 0.0) || !std::isfinite(P.unit_scale)) return {CLB_E_ARG, "symbolic law: unit_scale"};
+            for (int i = 0; i < P.n_vars; ++i)
+                if (P.var_species[i] < -1 || P.var_species[i] >= p.n_species) return {CLB_E_ARG, "symbolic law: variable binding"};
+            int pc = 0;
+            for (int r = 0; r < P.n_rates; ++r) {
+                if (P.rate_species[r] < -1 || P.rate_species[r] >= p.n_species) return {CLB_E_ARG, "symbolic law: rate binding"};
+                if (P.rate_end[r] < pc || P.rate_end[r] > P.n_ops) return {CLB_E_ARG, "symbolic law: rate code bounds"};
+                int sp = 0;
+                for (; pc < P.rate_end[r]; ++pc) {            // the stack each op leaves, checked before any run
+                    const int op = P.op[pc], arg = P.arg[pc];
+                    if (op == CLB_SYM_CONST) { if (arg < 0 || arg >= P.n_const) return {CLB_E_ARG, "symbolic law: constant index"}; ++sp; }
+                    else if (op == CLB_SYM_VAR) { if (arg < 0 || arg >= P.n_vars + r) return {CLB_E_ARG, "symbolic law: value slot"}; ++sp; }
+                    else if (op == CLB_SYM_NEG || (op >= CLB_SYM_EXP && op <= CLB_SYM_TANH)) { if (sp < 1) return {CLB_E_ARG, "symbolic law: stack"}; }
+                    else if ((op >= CLB_SYM_ADD && op <= CLB_SYM_POW) || op == CLB_SYM_MIN || op == CLB_SYM_MAX) {
+                        if (sp < 2) return {CLB_E_ARG, "symbolic law: stack"};
+                        --sp;
+                    } else return {CLB_E_ARG, "symbolic law: unknown op"};
+                    if (sp > CLB_SYM_STACK) return {CLB_E_ARG, "symbolic law: stack deeper than the evaluator's"};
+                }
+                if (sp != 1) return {CLB_E_ARG, "symbolic law: a rate's code must leave one value"};
+            }
+            HIPT(hipMemcpy(symDev + microbe, prog, sizeof(clb_sym_program), hipMemcpyHostToDevice));
+            symMask |= 1u << microbe;
+        } else {
+            HIPT(hipMemset(symDev + microbe, 0, sizeof(clb_sym_program)));
+            symMask &= ~(1u << microbe);
+        }
+        const uint32_t all = p.n_microbes >= 32 ? 0xffffffffu : ((1u << p.n_microbes) - 1u);
+        symOn = p.n_microbes > 0 && symMask == all;
+        return {CLB_OK, ""};
+    }
+
     // ------------------------------------------------------------ checkpoint
     // Layout: ChkHeader, the parameter block, the compact voxel list (checked,
     // not restored: a checkpoint loads into a case of the same geometry), then
@@ -533,6 +584,7 @@ struct Case : CaseBase {
         int32_t nsSaturate, batchLen;
         uint64_t caSweep;
         int64_t cnt[CLB_NCOUNTERS];
+        uint32_t symMask, symOn;
     };
     struct ChkSection { void *ptr; size_t bytes; };
     std::vector<ChkSection> chkSections() const {
@@ -550,6 +602,7 @@ struct Case : CaseBase {
         if (pushed) v.push_back({pushed, n});
         v.push_back({u, 3 * no * r});
         v.push_back({rhoN, sm * (size_t) std::max(Nopen - Nint, 1) * sizeof(double)});
+        v.push_back({symDev, (size_t) std::max(p.n_microbes, 1) * sizeof(clb_sym_program)});
         v.push_back({clampCount, sizeof(unsigned long long)});
         v.push_back({dcount, 3 * sizeof(unsigned long long)});
         v.push_back({ctl, sizeof(StepCtl)});
@@ -574,6 +627,7 @@ struct Case : CaseBase {
         h.iT = iT; h.oldTotMask = oldTotMask; h.msumClosed = msumClosed; h.launchNo = launchNo;
         h.nsSaturate = nsSaturate; h.batchLen = batchLen; h.caSweep = caSweep_;
         for (int k = 0; k < CLB_NCOUNTERS; ++k) h.cnt[k] = cnt[k];
+        h.symMask = symMask; h.symOn = symOn ? 1u : 0u;
         return h;
     }
     size_t checkpointBytes() override {
@@ -631,6 +685,7 @@ struct Case : CaseBase {
         for (int k = 0; k < CLB_NCOUNTERS; ++k) cnt[k] = h.cnt[k];
         kms = 0; klaunch = 0;
         nearDirty = true;
+        symMask = h.symMask; symOn = h.symOn != 0;
         return {CLB_OK, ""};
     }
 
@@ -760,6 +815,14 @@ struct Case : CaseBase {
     // with the gather at the waiting voxels (the reference's order,
     // complab.cpp:2461-2545).
     void transportLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr, bool maskTotal = false) {
+        if (kinOn && symOn) {
+            // the symbolic laws run in the split kinetics kernel, biotic then
+            // abiotic, on the transported state (the biomass changes there, so
+            // the step's mask/total update follows it, not the transport)
+            fusedLaunch(st, false, halt, false);
+            HIPT(launchKinetics<R>(ade(halt), kin(true), st));
+            return;
+        }
         if (!kinOn || !p.dissol_on) { fusedLaunch(st, kinOn, halt, maskTotal); return; }
         if (nearDirty) {
             HIPT(launchNearPhaseList<R>(ade(), kin(true), nearList, nearCnt, st));
@@ -1062,7 +1125,7 @@ struct Case : CaseBase {
         // the transport kernels make the step's mask and total update and its
         // block partials (interior, then planes) when every open voxel passes
         // through them; the final reduction takes the step's decision
-        const bool fused = Sm > 0 && Nint > 0;
+        const bool fused = Sm > 0 && Nint > 0 && !symOn;
         const int nparts = (int) (nblk(Nint, 256) + nblk(Nopen - Nint, 256));
         for (int j = 0; j < len; ++j) {
             transportLaunch(st, true, &ctl->halt, fused);
@@ -1092,6 +1155,9 @@ struct Case : CaseBase {
     Status runSteps(hipStream_t st, int n, int *done) override {
         int k = 0;
         Status s{CLB_OK, ""};
+        if (symMask != 0 && !symOn)
+            return {CLB_E_STATE, "some microbes have a symbolic law and some do not; every microbe needs one "
+                                 "(the compiled and symbolic laws are not mixed here)"};
         if (p.n_microbes > 0) {
             while (k < n) {
                 int fin = 0;
@@ -1675,6 +1741,14 @@ int clb_stream_copy(clb_ctx *ctx, size_t bytes, int reps, double *gbps, double *
     return rc;
 }
 
+int clb_set_symbolic(clb_ctx *ctx, int case_id, int microbe, const clb_sym_program *prog) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    Status r = c->setSymbolic(microbe, prog);
+    if (r.code) return fail(ctx, r.code, r.msg);
+    return CLB_OK;
+    GUARD_END
+}
 int clb_checkpoint_size(clb_ctx *ctx, int case_id, size_t *bytes) {
     GUARD_BEGIN
     CASE_OR_FAIL(c)

@@ -204,6 +204,100 @@ __device__ __forceinline__ double clampInc(int clamp, double d, double cur, unsi
 // ------------------------------------------------------------------------------------------
 // advection-diffusion
 // ------------------------------------------------------------------------------------------
+// one organism's symbolic law at one voxel (clb_sym_program; evalNode and
+// evaluate of complab3d_symbolic.hh:310-348, 766-804): the variables, held in
+// their ranges, then each rate's postfix code in file order
+__device__ void symEvaluate(const clb_sym_program *P, const double *avail, double bmass, double *out) {
+    double v[CLB_SYM_MAXVARS + CLB_SYM_MAXRATES];
+    const int nv = P->n_vars, nr = P->n_rates;
+    for (int i = 0; i < nv; ++i) {
+        const int s = P->var_species[i];
+        double x = s >= 0 ? avail[s] : bmass;
+        if (P->hi[i] > P->lo[i]) {
+            if (x < P->lo[i]) x = P->lo[i];
+            else if (x > P->hi[i]) x = P->hi[i];
+        }
+        v[i] = x;
+    }
+    int pc = 0;
+    for (int r = 0; r < nr; ++r) {
+        double st[CLB_SYM_STACK];
+        int sp = 0;
+        for (; pc < P->rate_end[r]; ++pc) {
+            const int op = P->op[pc], arg = P->arg[pc];
+            if (op == CLB_SYM_CONST) { st[sp++] = P->konst[arg]; continue; }
+            if (op == CLB_SYM_VAR) { st[sp++] = v[arg]; continue; }
+            if (op == CLB_SYM_NEG) { st[sp - 1] = -st[sp - 1]; continue; }
+            if (op <= CLB_SYM_POW || op >= CLB_SYM_MIN) {            // two operands
+                const double y = st[--sp], x = st[sp - 1];
+                double z;
+                switch (op) {
+                case CLB_SYM_ADD: z = x + y; break;
+                case CLB_SYM_SUB: z = x - y; break;
+                case CLB_SYM_MUL: z = x * y; break;
+                case CLB_SYM_DIV: z = fabs(y) < 1e-300 ? 0.0 : x / y; break;
+                case CLB_SYM_POW: z = (x < 0.0 && y != floor(y)) ? 0.0 : pow(x, y); break;
+                case CLB_SYM_MIN: z = x < y ? x : y; break;
+                default: z = x > y ? x : y; break;                    // CLB_SYM_MAX
+                }
+                st[sp - 1] = z;
+                continue;
+            }
+            const double x = st[sp - 1];
+            double z;
+            switch (op) {
+            case CLB_SYM_EXP: z = exp(x); break;
+            case CLB_SYM_LOG: z = x > 0.0 ? log(x) : 0.0; break;
+            case CLB_SYM_LOG10: z = x > 0.0 ? log10(x) : 0.0; break;
+            case CLB_SYM_SQRT: z = x > 0.0 ? sqrt(x) : 0.0; break;
+            case CLB_SYM_ABS: z = fabs(x); break;
+            default: z = tanh(x); break;                              // CLB_SYM_TANH
+            }
+            st[sp - 1] = z;
+        }
+        double y = sp > 0 ? st[sp - 1] : 0.0;
+        if (y != y) y = 0.0;
+        if (y > 1e300) y = 1e300; else if (y < -1e300) y = -1e300;
+        v[nv + r] = y;
+        out[r] = y * P->unit_scale;
+    }
+}
+
+// the symbolic organisms of one voxel (run_symbolic3D,
+// complab3d_processors_symbolic.hh:104-182): every organism present evaluates
+// its law on the voxel's chemistry; substrate rates x gate x dt are requests
+// on one shared budget, a debit held to what the voxel has; the growth rate x
+// gate x biomass x dt is the organism's increment.  dC and dB come out as
+// increments of the step.
+__device__ void symbioticIncrements(const KinDev &k, const double *C, const double *B, double *dC, double *dB) {
+    const double thrd = 1e-12;
+    double avail[CLB_DEV_MAXS], req[CLB_DEV_MAXS], bm[CLB_DEV_MAXM];
+    for (int s = 0; s < k.S; ++s) { avail[s] = C[s] > thrd ? C[s] : 0.0; req[s] = 0.0; dC[s] = 0.0; }
+    bool any = false;
+    for (int m = 0; m < k.M; ++m) { bm[m] = B[m] > thrd ? B[m] : 0.0; dB[m] = 0.0; if (bm[m] > thrd) any = true; }
+    if (!any) return;
+    for (int m = 0; m < k.M; ++m) {
+        if (!(bm[m] > thrd)) continue;
+        const clb_sym_program *P = k.sym + m;
+        if (P->n_rates <= 0) continue;
+        double out[CLB_SYM_MAXRATES];
+        symEvaluate(P, avail, bm[m], out);
+        const double ft = (k.thm && (k.thmMicrobe < 0 || k.thmMicrobe == m)) ? thermoGate(k, avail) : 1.0;
+        for (int r = 0; r < P->n_rates; ++r) {
+            const int rs = P->rate_species[r];
+            if (rs >= 0) req[rs] += out[r] * ft * k.dt;
+            else dB[m] += out[r] * ft * bm[m] * k.dt;
+        }
+    }
+    for (int s = 0; s < k.S; ++s) {
+        double draw = req[s];
+        if (fabs(draw) <= thrd) continue;
+        const double headroom = avail[s] + dC[s];
+        if (draw + headroom < 0.0) draw = -headroom;
+        dC[s] += draw;
+    }
+}
+
 // a face neighbour of compact voxel i in the reaction box belongs to the
 // dissolving phase: i collects that neighbour's parked products (the
 // condition of the gather below and, seen from the phase voxel, the open
@@ -228,7 +322,7 @@ __device__ __forceinline__ bool nextToPhase(const AdeDev<R> &a, const KinDev &k,
 // the clamp, the D3Q7 deposit w_i dC, immobile species and biomass in place
 // (run_kinetics + update_rxnLattices, part1:85-187, 406-497; the abiotic pair
 // part1:199-404; surfaceAbioticKinetics3D's face gate)
-template <typename R, int MAXS, bool DISSOL>
+template <typename R, int MAXS, bool DISSOL, bool SYM = false>
 __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &k, int i, R (&f)[MAXS][7],
                                               long long v, int mask, R (&bpre)[CLB_DEV_MAXM], bool havePre) {
     const long long No = a.Nopen;
@@ -277,17 +371,26 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
                 Bv[m] = havePre ? (double) bpre[m] : (double) a.B[(long long) m * N + v];
                 if (Bv[m] < 1e-12) Bv[m] = 0;
             }
-            bioticLaw(k, C, Bv, sr, br, mask);
+            // the step's increments: rate x dt of the compiled law, or the
+            // symbolic laws' increments after their shared budget
+            if (SYM) symbioticIncrements(k, C, Bv, sr, br);
+            else {
+                bioticLaw(k, C, Bv, sr, br, mask);
+#pragma unroll
+                for (int s = 0; s < CLB_DEV_MAXS; ++s) if (s < S) sr[s] *= k.dt;
+#pragma unroll
+                for (int m = 0; m < CLB_DEV_MAXM; ++m) if (m < k.M) br[m] *= k.dt;
+            }
 #pragma unroll
             for (int s = 0; s < CLB_DEV_MAXS; ++s) {
                 if (s >= S) continue;
-                const double d = sr[s] * k.dt;
+                const double d = sr[s];
                 if (d > 1e-12 || d < -1e-12) depositTo(s, clampInc(k.clamp, d, concOf(s), binds));
             }
 #pragma unroll
             for (int m = 0; m < CLB_DEV_MAXM; ++m) {
                 if (m >= k.M) continue;
-                double d = br[m] * k.dt;
+                double d = br[m];
                 if (d > 1e-12 || d < -1e-12) {
                     R *p = a.B + (long long) m * N + v;
                     const double b = havePre ? (double) bpre[m] : (double) *p;
@@ -587,7 +690,7 @@ __global__ void __launch_bounds__(256) k_ade_planes(AdeDev<R> a) {
 // the kinetics alone on the stored post-collision state (the split form of
 // the fused kernel, for callers that apply the reference's processors one by
 // one); bit-identical to the fused kernel's kinetics
-template <typename R, int MAXS, bool DISSOL>
+template <typename R, int MAXS, bool DISSOL, bool SYM = false>
 __device__ __forceinline__ void kineticsVoxel(const AdeDev<R> &a, const KinDev &k, int i) {
     if (a.cls[i] == ADE_BB) return;
     const long long v = a.list[i];
@@ -604,7 +707,7 @@ __device__ __forceinline__ void kineticsVoxel(const AdeDev<R> &a, const KinDev &
         for (int q = 0; q < 7; ++q) f[ms][q] = c[q * No + i];
     }
     R nopre[CLB_DEV_MAXM] = {};
-    voxelKinetics<R, MAXS, DISSOL>(a, k, i, f, v, mask, nopre, false);
+    voxelKinetics<R, MAXS, DISSOL, SYM>(a, k, i, f, v, mask, nopre, false);
 #pragma unroll
     for (int ms = 0; ms < MAXS; ++ms) {
         if (ms >= a.Sm) continue;
@@ -613,12 +716,12 @@ __device__ __forceinline__ void kineticsVoxel(const AdeDev<R> &a, const KinDev &
         for (int q = 0; q < 7; ++q) c[q * No + i] = f[ms][q];
     }
 }
-template <typename R, int MAXS, bool DISSOL>
+template <typename R, int MAXS, bool DISSOL, bool SYM>
 __global__ void __launch_bounds__(256) k_kinetics(AdeDev<R> a, KinDev k) {
     if (halted(a.halt)) return;
     const int i = xcdBlock() * blockDim.x + threadIdx.x;
     if (i >= a.Nint) return;
-    kineticsVoxel<R, MAXS, DISSOL>(a, k, i);
+    kineticsVoxel<R, MAXS, DISSOL, SYM>(a, k, i);
 }
 // the same over a list of voxels whose length the device holds (the voxels
 // next to the dissolving phase): a grid-stride loop of a fixed grid
@@ -1354,21 +1457,22 @@ hipError_t launchNearPhaseList(const AdeDev<R> &a, const KinDev &k, int *sel, un
     if (a.Nint > 0) hipLaunchKernelGGL(k_near_phase_list<R>, dim3(nblk(a.Nint, 256)), dim3(256), 0, st, a, k, sel, count);
     return hipGetLastError();
 }
-template <typename R, bool DISSOL>
+template <typename R, bool DISSOL, bool SYM>
 static void kineticsVariants(const AdeDev<R> &a, const KinDev &k, hipStream_t st) {
     const unsigned int g = nblk(a.Nint, 256);
-    if (a.Sm <= 1) hipLaunchKernelGGL((k_kinetics<R, 1, DISSOL>), dim3(g), dim3(256), 0, st, a, k);
-    else if (a.Sm <= 2) hipLaunchKernelGGL((k_kinetics<R, 2, DISSOL>), dim3(g), dim3(256), 0, st, a, k);
-    else if (a.Sm <= 3) hipLaunchKernelGGL((k_kinetics<R, 3, DISSOL>), dim3(g), dim3(256), 0, st, a, k);
-    else if (a.Sm <= 4) hipLaunchKernelGGL((k_kinetics<R, 4, DISSOL>), dim3(g), dim3(256), 0, st, a, k);
-    else if (a.Sm <= 6) hipLaunchKernelGGL((k_kinetics<R, 6, DISSOL>), dim3(g), dim3(256), 0, st, a, k);
-    else hipLaunchKernelGGL((k_kinetics<R, 8, DISSOL>), dim3(g), dim3(256), 0, st, a, k);
+    if (a.Sm <= 1) hipLaunchKernelGGL((k_kinetics<R, 1, DISSOL, SYM>), dim3(g), dim3(256), 0, st, a, k);
+    else if (a.Sm <= 2) hipLaunchKernelGGL((k_kinetics<R, 2, DISSOL, SYM>), dim3(g), dim3(256), 0, st, a, k);
+    else if (a.Sm <= 3) hipLaunchKernelGGL((k_kinetics<R, 3, DISSOL, SYM>), dim3(g), dim3(256), 0, st, a, k);
+    else if (a.Sm <= 4) hipLaunchKernelGGL((k_kinetics<R, 4, DISSOL, SYM>), dim3(g), dim3(256), 0, st, a, k);
+    else if (a.Sm <= 6) hipLaunchKernelGGL((k_kinetics<R, 6, DISSOL, SYM>), dim3(g), dim3(256), 0, st, a, k);
+    else hipLaunchKernelGGL((k_kinetics<R, 8, DISSOL, SYM>), dim3(g), dim3(256), 0, st, a, k);
 }
 template <typename R>
 hipError_t launchKinetics(const AdeDev<R> &a, const KinDev &k, hipStream_t st) {
     if (a.Nint <= 0 || !k.on) return hipSuccess;
-    if (k.dissol) kineticsVariants<R, true>(a, k, st);
-    else kineticsVariants<R, false>(a, k, st);
+    if (k.dissol) kineticsVariants<R, true, false>(a, k, st);           // symbolic laws are refused with dissolution
+    else if (k.symOn && k.bio) kineticsVariants<R, false, true>(a, k, st);
+    else kineticsVariants<R, false, false>(a, k, st);
     return hipGetLastError();
 }
 template <typename R>

@@ -170,7 +170,7 @@ typedef struct clb_case_params {
      * f = dG0 + RT ln Q + dGloss + atp dGatp,
      * ln Q = sum_s nu_s ln max(conc_scale C_s, cmin); energies in kJ/mol */
     int32_t thm_on;
-    int32_t thm_pad;
+    int32_t thm_microbe;               /* the organism gated on the symbolic path; -1: every one */
     double  thm_rt;                    /* gas constant x temperature                          */
     double  thm_dg0;
     double  thm_dgloss;                /* >= 0                                                 */
@@ -326,6 +326,45 @@ int  clb_field_stats(clb_ctx *ctx, int case_id, double *out);
  * clamped, [11] voxels reopened by dissolution */
 #define CLB_NCOUNTERS 12
 int  clb_counters(clb_ctx *ctx, int case_id, int64_t out[CLB_NCOUNTERS]);
+
+/* ---- symbolic rate laws (complab3d_symbolic.hh; run_symbolic3D,
+ * complab3d_processors_symbolic.hh:64-200) ------------------------------------
+ * One organism's .sym law compiled to postfix code over a value vector that
+ * holds the declared variables (slots 0..n_vars-1) and then the rate outputs
+ * already computed (slot n_vars + r).  Rate r's code is op[rate_end[r-1] ..
+ * rate_end[r]); its value, NaN -> 0 and held within +-1e300, is stored
+ * unscaled in its slot and leaves the law times unit_scale.  Division by
+ * |q| < 1e-300, a negative base to a fractional power, and the logarithms and
+ * square root of a non-positive argument give 0.  Variables with hi > lo are
+ * held within [lo, hi].  A variable binds a species' concentration or, with
+ * var_species -1, the organism's own biomass; a rate with rate_species s is
+ * species s's rate, with -1 the specific growth rate (times the biomass). */
+#define CLB_SYM_MAXOPS    256
+#define CLB_SYM_MAXCONST  64
+#define CLB_SYM_MAXVARS   16
+#define CLB_SYM_MAXRATES  16
+#define CLB_SYM_STACK     16
+enum {
+    CLB_SYM_CONST = 1, CLB_SYM_VAR, CLB_SYM_NEG, CLB_SYM_ADD, CLB_SYM_SUB, CLB_SYM_MUL, CLB_SYM_DIV,
+    CLB_SYM_POW, CLB_SYM_EXP, CLB_SYM_LOG, CLB_SYM_LOG10, CLB_SYM_SQRT, CLB_SYM_ABS, CLB_SYM_TANH,
+    CLB_SYM_MIN, CLB_SYM_MAX
+};
+typedef struct clb_sym_program {
+    int32_t n_ops, n_vars, n_rates, n_const;
+    uint8_t op[CLB_SYM_MAXOPS];
+    int16_t arg[CLB_SYM_MAXOPS];                /* CONST: constant index; VAR: value slot */
+    int16_t rate_end[CLB_SYM_MAXRATES];
+    int16_t var_species[CLB_SYM_MAXVARS];
+    int16_t rate_species[CLB_SYM_MAXRATES];
+    double  lo[CLB_SYM_MAXVARS], hi[CLB_SYM_MAXVARS];
+    double  konst[CLB_SYM_MAXCONST];
+    double  unit_scale;                         /* 1 per second, 1/3600 per hour */
+} clb_sym_program;
+/* The biotic rates of microbe `microbe` come from `prog` instead of the
+ * compiled law (every microbe of a case takes one, before its first step);
+ * the program is checked (op codes, slots, stack depth) and CLB_E_ARG
+ * returned when it does not run; NULL switches the microbe back. */
+int  clb_set_symbolic(clb_ctx *ctx, int case_id, int microbe, const clb_sym_program *prog);
 
 /* ---- checkpoint / restart (saveBinaryBlock / loadBinaryBlock of every lattice,
  * complab.cpp:2293-2302 and 3274-3310, read back by <IO><read_ADE_file>) -------

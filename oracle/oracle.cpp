@@ -101,6 +101,9 @@ struct Case {
     std::vector<double> share;              // dissolution: per-face product share parked per species, S*N
     uint64_t caSweepCounter = 0;
     std::string err;
+    // symbolic laws, one program per microbe; in use when every microbe has one
+    std::vector<clb_sym_program> sym;
+    uint32_t symMask = 0;
 
     size_t id(int x, int y, int z) const { return (size_t) x + (size_t) nx * ((size_t) y + (size_t) ny * (size_t) z); }
     bool inside(int x, int y, int z) const { return x >= 0 && y >= 0 && z >= 0 && x < nx && y < ny && z < nz; }
@@ -228,6 +231,98 @@ double thermoFactor(const clb_case_params &p, const double *C, double *dGout) {
     return F < 0.0 ? 0.0 : (F > 1.0 ? 1.0 : F);
 }
 
+/* a .sym law compiled to postfix code (clb_sym_program), evaluated as
+ * evalNode / evaluate do (complab3d_symbolic.hh:310-348, 766-804): variables
+ * held in their ranges, then each rate in file order, NaN -> 0, +-1e300,
+ * stored unscaled for later rates, leaving times unit_scale */
+void symEvaluate(const clb_sym_program &P, const double *avail, double bmass, double *out) {
+    double v[CLB_SYM_MAXVARS + CLB_SYM_MAXRATES];
+    for (int i = 0; i < P.n_vars; ++i) {
+        const int s = P.var_species[i];
+        double x = s >= 0 ? avail[s] : bmass;
+        if (P.hi[i] > P.lo[i]) { if (x < P.lo[i]) x = P.lo[i]; else if (x > P.hi[i]) x = P.hi[i]; }
+        v[i] = x;
+    }
+    int pc = 0;
+    for (int r = 0; r < P.n_rates; ++r) {
+        std::vector<double> st;
+        for (; pc < P.rate_end[r]; ++pc) {
+            const int op = P.op[pc], arg = P.arg[pc];
+            if (op == CLB_SYM_CONST) { st.push_back(P.konst[arg]); continue; }
+            if (op == CLB_SYM_VAR) { st.push_back(v[arg]); continue; }
+            double &top = st.back();
+            switch (op) {
+            case CLB_SYM_NEG: top = -top; continue;
+            case CLB_SYM_EXP: top = std::exp(top); continue;
+            case CLB_SYM_LOG: top = top > 0.0 ? std::log(top) : 0.0; continue;
+            case CLB_SYM_LOG10: top = top > 0.0 ? std::log10(top) : 0.0; continue;
+            case CLB_SYM_SQRT: top = top > 0.0 ? std::sqrt(top) : 0.0; continue;
+            case CLB_SYM_ABS: top = std::fabs(top); continue;
+            case CLB_SYM_TANH: top = std::tanh(top); continue;
+            default: break;
+            }
+            const double y = st.back(); st.pop_back();
+            const double x = st.back();
+            double z = 0.0;
+            switch (op) {
+            case CLB_SYM_ADD: z = x + y; break;
+            case CLB_SYM_SUB: z = x - y; break;
+            case CLB_SYM_MUL: z = x * y; break;
+            case CLB_SYM_DIV: z = std::fabs(y) < 1e-300 ? 0.0 : x / y; break;
+            case CLB_SYM_POW: z = (x < 0.0 && y != std::floor(y)) ? 0.0 : std::pow(x, y); break;
+            case CLB_SYM_MIN: z = x < y ? x : y; break;
+            case CLB_SYM_MAX: z = x > y ? x : y; break;
+            }
+            st.back() = z;
+        }
+        double y = st.empty() ? 0.0 : st.back();
+        if (y != y) y = 0.0;
+        if (y > 1e300) y = 1e300; else if (y < -1e300) y = -1e300;
+        v[P.n_vars + r] = y;
+        out[r] = y * P.unit_scale;
+    }
+}
+
+/* run_symbolic3D (complab3d_processors_symbolic.hh:104-182): the increments of
+ * one voxel -- every organism present evaluates its law; substrate rates x
+ * gate x dt are requests on one budget, a debit held to what the voxel has;
+ * growth x gate x biomass x dt is the organism's increment */
+void symbioticIncrements(const Case &c, const double *C, const double *B, double *dC, double *dB) {
+    const clb_case_params &p = c.p;
+    const double thrd = THRD;
+    double avail[CLB_MAX_SPECIES], req[CLB_MAX_SPECIES], bm[CLB_MAX_MICROBES];
+    for (int s = 0; s < p.n_species; ++s) { avail[s] = C[s] > thrd ? C[s] : 0.0; req[s] = 0.0; dC[s] = 0.0; }
+    bool any = false;
+    for (int m = 0; m < p.n_microbes; ++m) { bm[m] = B[m] > thrd ? B[m] : 0.0; dB[m] = 0.0; if (bm[m] > thrd) any = true; }
+    if (!any) return;
+    for (int m = 0; m < p.n_microbes; ++m) {
+        if (!(bm[m] > thrd)) continue;
+        const clb_sym_program &P = c.sym[(size_t) m];
+        if (P.n_rates <= 0) continue;
+        double out[CLB_SYM_MAXRATES];
+        symEvaluate(P, avail, bm[m], out);
+        const double ft = (p.thm_on && (p.thm_microbe < 0 || p.thm_microbe == m)) ? thermoFactor(p, avail, nullptr) : 1.0;
+        for (int r = 0; r < P.n_rates; ++r) {
+            const int rs = P.rate_species[r];
+            if (rs >= 0) req[rs] += out[r] * ft * p.dt;
+            else dB[m] += out[r] * ft * bm[m] * p.dt;
+        }
+    }
+    for (int s = 0; s < p.n_species; ++s) {
+        double draw = req[s];
+        if (std::fabs(draw) <= thrd) continue;
+        const double headroom = avail[s] + dC[s];
+        if (draw + headroom < 0.0) draw = -headroom;
+        dC[s] += draw;
+    }
+}
+
+/* every microbe has a symbolic law (mixing with the compiled law is not built) */
+bool symActive(const Case &c) {
+    const uint32_t all = c.p.n_microbes >= 32 ? 0xffffffffu : ((1u << c.p.n_microbes) - 1u);
+    return c.p.n_microbes > 0 && c.symMask == all;
+}
+
 /* dual-Monod restatement of defineRxnKinetics (config/kinetics/defineKinetics.biotic.hh:90-126):
  * growth = mu_max * S/(Ks+S) [* E/(Ke+E)] * B ; bioR = growth - kd*B ; subsR_s += nu_s*growth */
 void bioticLaw(const clb_case_params &p, const double *C, const double *B, double *subsR, double *bioR, int mask) {
@@ -305,9 +400,14 @@ void bioticStep(Case &c) {
         double C[CLB_MAX_SPECIES], Bv[CLB_MAX_MICROBES], sr[CLB_MAX_SPECIES], br[CLB_MAX_MICROBES];
         for (int s = 0; s < S; ++s) { C[s] = conc(c, s, i); if (C[s] < THRD) C[s] = 0; }
         for (int m = 0; m < M; ++m) { Bv[m] = c.B[m][i]; if (Bv[m] < THRD) Bv[m] = 0; }
-        bioticLaw(p, C, Bv, sr, br, mask);
-        for (int s = 0; s < S; ++s) { double d = sr[s] * p.dt; if (d > THRD || d < -THRD) dC[(size_t) s * c.N + i] += d; }
-        for (int m = 0; m < M; ++m) { double d = br[m] * p.dt; if (d > THRD || d < -THRD) dB[(size_t) m * c.N + i] += d; }
+        if (symActive(c)) symbioticIncrements(c, C, Bv, sr, br);       // increments of the step
+        else {
+            bioticLaw(p, C, Bv, sr, br, mask);
+            for (int s = 0; s < S; ++s) sr[s] *= p.dt;
+            for (int m = 0; m < M; ++m) br[m] *= p.dt;
+        }
+        for (int s = 0; s < S; ++s) { double d = sr[s]; if (d > THRD || d < -THRD) dC[(size_t) s * c.N + i] += d; }
+        for (int m = 0; m < M; ++m) { double d = br[m]; if (d > THRD || d < -THRD) dB[(size_t) m * c.N + i] += d; }
     }
 #pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 1; x < c.nx - 1; ++x) {
@@ -1222,7 +1322,7 @@ int coupledStep(Case &c) {
     const clb_case_params &p = c.p;
     c.adeClsCollide = c.adeCls;
     for (int s = 0; s < p.n_species; ++s) if (p.mobile[s]) adeCollide(c, s);
-    if (p.bio_law != CLB_BIO_NONE && p.n_microbes > 0) bioticStep(c);
+    if ((p.bio_law != CLB_BIO_NONE || symActive(c)) && p.n_microbes > 0) bioticStep(c);
     if (p.n_rxn > 0 || p.dissol_on) abioticStep(c);
     if (p.eq_on) equilibriumStep(c);
     int rc = CLB_OK;
@@ -1279,7 +1379,7 @@ int coupledStep(Case &c) {
 void transportStep(Case &c) {
     const clb_case_params &p = c.p;
     for (int s = 0; s < p.n_species; ++s) if (p.mobile[s]) adeCollide(c, s);
-    if (p.bio_law != CLB_BIO_NONE && p.n_microbes > 0) bioticStep(c);
+    if ((p.bio_law != CLB_BIO_NONE || symActive(c)) && p.n_microbes > 0) bioticStep(c);
     if (p.n_rxn > 0 || p.dissol_on) abioticStep(c);
     for (int s = 0; s < p.n_species; ++s) if (p.mobile[s]) adeStream(c, s);
 }
@@ -1596,6 +1696,22 @@ int orc_eq_voxel(const clb_case_params *p, const double *cin, double *cout) {
         if (C[s] + dC < 1e-20) dC = 1e-20 - C[s];
         cout[s] = cin[s] + (std::fabs(dC) > THRD ? dC : 0.0);
     }
+    return 0;
+}
+
+/* the biotic rates of microbe m from a symbolic law (NULL: back to the compiled law) */
+int orc_set_symbolic(void *h, int m, const clb_sym_program *prog) {
+    Case &c = *(Case *) h;
+    if (m < 0 || m >= c.p.n_microbes) return CLB_E_ARG;
+    if (c.sym.size() < (size_t) c.p.n_microbes) c.sym.resize((size_t) c.p.n_microbes);
+    if (prog) { c.sym[(size_t) m] = *prog; c.symMask |= 1u << m; }
+    else { memset(&c.sym[(size_t) m], 0, sizeof(clb_sym_program)); c.symMask &= ~(1u << m); }
+    return 0;
+}
+
+/* one law's outputs at one composition, for the tests */
+int orc_sym_evaluate(const clb_sym_program *prog, const double *avail, double bmass, double *out) {
+    symEvaluate(*prog, avail, bmass, out);
     return 0;
 }
 

@@ -60,6 +60,8 @@ def lib():
             ("orc_eq_voxel", [P, P, P]),
             ("orc_rates", [P, P, P, C.c_int, P, P, P]),
             ("orc_thermo_factor", [P, P, P, P]),
+            ("orc_set_symbolic", [P, C.c_int, P]),
+            ("orc_sym_evaluate", [P, P, C.c_double, P]),
             ("orc_rng_draws", [C.c_uint64, C.c_int, P]),
             ("orc_set_threads", [C.c_int]),
             ("orc_geom_spheres", [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64, C.c_int, P]),
@@ -121,6 +123,9 @@ class OracleCase:
         if rc < 0:
             raise RuntimeError("oracle error %d: %s" % (rc, lib().orc_last_error(self.h).decode()))
         return rc
+
+    def set_symbolic(self, microbe, prog):
+        self._chk(lib().orc_set_symbolic(self.h, int(microbe), C.byref(prog) if prog is not None else None))
 
     def set_omegas(self, omp, omb, dt):
         a = np.ascontiguousarray(omp, dtype=np.float64)
@@ -226,6 +231,15 @@ def rates(params, conc, bio, mask):
     sr = np.zeros(params.n_species); br = np.zeros(max(1, params.n_microbes)); ar = np.zeros(params.n_species)
     lib().orc_rates(C.byref(params), _ptr(c), _ptr(b), mask, _ptr(sr), _ptr(br), _ptr(ar))
     return sr, br, ar
+
+
+def sym_evaluate(prog, avail, bmass):
+    """Outputs of one symbolic law (a params.SymProgram) at one composition."""
+    a = np.zeros(64)
+    a[:len(avail)] = avail
+    out = np.zeros(16)
+    lib().orc_sym_evaluate(C.byref(prog), _ptr(a), float(bmass), _ptr(out))
+    return out[:prog.n_rates].copy()
 
 
 def thermo_factor(params, conc):
