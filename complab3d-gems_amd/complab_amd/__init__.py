@@ -296,8 +296,9 @@ class Case:
 
     def set_symbolic(self, microbe, prog):
         """Microbe `microbe` takes its biotic rates from `prog` (a
-        params.SymProgram, e.g. from complab_amd.symbolic.compile); None
-        returns it to the compiled law."""
+        params.SymProgram, from complab_amd.symbolic.compile_law); None
+        returns it to the compiled law.  microbe = params.SYM_ABIOTIC sets
+        (or, with None, clears) the abiotic law."""
         self._c(lib().clb_set_symbolic(self.ctx.h, self.id, int(microbe),
                                        C.byref(prog) if prog is not None else None))
 

@@ -299,7 +299,7 @@ class Run:
     chk_interval: int = 1000000                 # <IO><save_CHK_interval>
     mask_filename: str = "maskLattice"
     summary_csv: str = "summary.csv"
-    symbolic: dict = field(default_factory=dict)   # microbe index -> params.SymProgram
+    symbolic: dict = field(default_factory=dict)   # microbe index (params.SYM_ABIOTIC: the abiotic law) -> params.SymProgram
 
 
 def load(xml_path, geometry_path: Optional[str] = None, precision=P.FP64, order="slices") -> Run:
@@ -411,8 +411,6 @@ def load(xml_path, geometry_path: Optional[str] = None, precision=P.FP64, order=
             raise InputError("symbolic and compiled-kinetics organisms in one run are not combined on this path")
         if not doc.flag("symbolic", "enabled"):
             raise InputError("a microbe has reaction_type symbolic but <symbolic><enabled> is not true")
-        if doc.text("symbolic", "abiotic_file", default=None):
-            raise InputError("<symbolic><abiotic_file>: abiotic symbolic laws are not on this path")
         shared = doc.text("symbolic", "expressions_file", default=None)
         from . import symbolic as SYM
         for m, rt in enumerate(rtypes):
@@ -555,6 +553,20 @@ def load(xml_path, geometry_path: Optional[str] = None, precision=P.FP64, order=
     geo = np.empty(raw.shape, dtype=np.int8)
     for src, dst in mats.items():
         geo[raw == src] = dst
+
+    # the abiotic symbolic law (<symbolic><abiotic_file>): every open voxel, no
+    # organism, its own block whether or not mass-action kinetics are on
+    # (complab.cpp:2455-2493)
+    ab = doc.text("symbolic", "abiotic_file", default=None) if doc.flag("symbolic", "enabled") else None
+    if ab:
+        if p.dissol_on:
+            raise InputError("<symbolic><abiotic_file> together with dissolution is not combined on this path")
+        from . import symbolic as SYM
+        try:
+            law = SYM.read_sym(ab if os.path.isabs(ab) else os.path.join(base, ab))
+            symbolic[P.SYM_ABIOTIC] = SYM.compile_law(law, names, None)
+        except SYM.SymError as e:
+            raise InputError(str(e))
 
     return Run(params=p, geometry=geo, flow=flow, transport=transport, steps=steps, species=names,
                microbes=microbes, dx=dx_m, diag_enabled=run_diag,

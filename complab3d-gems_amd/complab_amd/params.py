@@ -172,6 +172,7 @@ def default_params(nx, ny, nz, n_species):
 
 
 SYM_MAXOPS, SYM_MAXCONST, SYM_MAXVARS, SYM_MAXRATES, SYM_STACK = 256, 64, 16, 16, 16
+SYM_ABIOTIC = -1                        # clb_set_symbolic: the abiotic law
 (SYM_CONST, SYM_VAR, SYM_NEG, SYM_ADD, SYM_SUB, SYM_MUL, SYM_DIV, SYM_POW, SYM_EXP, SYM_LOG, SYM_LOG10,
  SYM_SQRT, SYM_ABS, SYM_TANH, SYM_MIN, SYM_MAX) = range(1, 17)
 

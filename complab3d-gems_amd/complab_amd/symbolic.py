@@ -13,14 +13,19 @@ The file: `#` comments, then lines
     yield    acetate 0.4                     lines are derived, coefficient
     biomass  Bug                             (nu_i / |nu_key|) / Y)
 
+An abiotic law (complab3d_symbolic.hh:84-115, 653-684) has no organism: one
+`rate extent = ...` line and a reaction line, from which the species lines
+`nu_i * extent` are derived.  A law is biotic with a "growth" line and
+abiotic with an "extent" line, never both.
+
 Expressions (the reference's grammar, complab3d_symbolic.hh:185-305): + and -
 left-associative, then * and /, then unary - and +, then ^ (right-associative,
 its exponent a unary expression: -2^2 = -4, 2^-1 = 0.5); numbers, the
 declared variables and the rates defined earlier (a name matches the
 variables first), pi, e, exp log log10 sqrt abs tanh (one argument), min max
-pow (two).  A "growth" rate is the specific growth rate; a substrate rate is
-that substrate's rate of change.  An "extent" (abiotic) law is refused: the
-symbolic path here is the biotic one.
+pow (two).  A "growth" rate is the specific growth rate, an "extent" rate the
+rate of the abiotic reaction; a substrate rate is that substrate's rate of
+change.
 """
 import re
 from dataclasses import dataclass, field
@@ -229,13 +234,31 @@ def read_sym(path) -> SymLaw:
             law.biomass = t[0]
         else:
             raise SymError("'%s' is not a keyword of a .sym file%s" % (key, at))
+    if not law.vars:
+        raise SymError("%s has no 'vars' line" % path)
     names = [r[0] for r in law.rates]
-    if "extent" in names:
-        raise SymError("%s: an 'extent' line is an abiotic law; the symbolic path here is the biotic one" % path)
-    if law.reaction:
-        # derived substrate lines (complab3d_symbolic.hh, the reaction/yield/biomass block)
-        if "growth" not in names:
-            raise SymError("%s: a 'reaction' line needs a 'growth' rate line" % path)
+    growth, extent = "growth" in names, "extent" in names
+    if growth and extent:
+        raise SymError("%s: a law has a 'growth' line (an organism's) or an 'extent' line (no organism's), "
+                       "not both" % path)
+    if extent and not law.reaction:
+        raise SymError("%s: an 'extent' line needs the 'reaction' it is the rate of" % path)
+    if law.reaction and extent:
+        # abiotic: nu_i * extent per species (complab3d_symbolic.hh:653-684)
+        if law.yield_of or law.biomass:
+            raise SymError("%s: an 'extent' law belongs to no organism; 'yield' and 'biomass' do not apply" % path)
+        for n, v in law.reaction:
+            if n in names:
+                raise SymError("%s: '%s' has a rate line and a place in the reaction; keep one" % (path, n))
+            text = "%s * extent" % _g10(v)
+            law.code.append(parse_expression(text, law.vars + [r[0] for r in law.rates]))
+            law.rates.append((n, text))
+            law.derived += 1
+    elif law.reaction:
+        # derived substrate lines (complab3d_symbolic.hh:686-745)
+        if not growth:
+            raise SymError("%s: a 'reaction' line needs a 'growth' line (an organism's law) or an "
+                           "'extent' line (no organism's)" % path)
         if not law.yield_of or not law.yield_val > 0 or not law.biomass:
             raise SymError("%s: a biotic 'reaction' needs 'yield' (positive) and 'biomass'" % path)
         key = [v for n, v in law.reaction if n == law.yield_of]
@@ -249,6 +272,8 @@ def read_sym(path) -> SymLaw:
             law.code.append(parse_expression(text, law.vars + [r[0] for r in law.rates]))
             law.rates.append((n, text))
             law.derived += 1
+    elif law.yield_of or law.biomass:
+        raise SymError("%s: 'yield' or 'biomass' without a 'reaction' line derives nothing" % path)
     if not law.rates:
         raise SymError("%s declares no rate" % path)
     return law
@@ -259,9 +284,21 @@ def _g10(x: float) -> str:
     return "%.10g" % x
 
 
-def compile_law(law: SymLaw, species: Sequence[str], microbe: str) -> P.SymProgram:
-    """The law bound to a run: each variable to a substrate or to this
-    organism's biomass (its name), each rate to 'growth' or a substrate."""
+def is_abiotic(law: SymLaw) -> bool:
+    return any(r[0] == "extent" for r in law.rates)
+
+
+def compile_law(law: SymLaw, species: Sequence[str], microbe: Optional[str]) -> P.SymProgram:
+    """The law bound to a run (bindToSubstrates, complab3d_symbolic.hh:812-883):
+    each variable to a substrate or to this organism's biomass (its name),
+    each rate to 'growth' or a substrate.  microbe None binds the abiotic law
+    (<symbolic><abiotic_file>): substrates only, 'extent' instead of
+    'growth'."""
+    abiotic = microbe is None
+    if abiotic and any(r[0] == "growth" for r in law.rates):
+        raise SymError("the abiotic law declares a 'growth' rate; an abiotic law belongs to no organism")
+    if not abiotic and is_abiotic(law):
+        raise SymError("an 'extent' law is abiotic and belongs in <abiotic_file>, not to microbe '%s'" % microbe)
     prog = P.SymProgram()
     if len(law.vars) > P.SYM_MAXVARS or len(law.rates) > P.SYM_MAXRATES:
         raise SymError("the law has more variables or rates than the kernel holds")
@@ -269,8 +306,10 @@ def compile_law(law: SymLaw, species: Sequence[str], microbe: str) -> P.SymProgr
     for i, v in enumerate(law.vars):
         if v in species:
             prog.var_species[i] = list(species).index(v)
-        elif v == microbe:
+        elif not abiotic and v == microbe:
             prog.var_species[i] = -1
+        elif abiotic:
+            raise SymError("variable '%s' of the abiotic law is not a substrate of this run" % v)
         else:
             raise SymError("variable '%s' is neither a substrate nor the microbe '%s'" % (v, microbe))
         lo, hi = law.ranges.get(v, (0.0, 0.0))
@@ -278,12 +317,13 @@ def compile_law(law: SymLaw, species: Sequence[str], microbe: str) -> P.SymProgr
     consts: List[float] = []
     ops: List[Tuple[int, int]] = []
     for r, ((name, _), code) in enumerate(zip(law.rates, law.code)):
-        if name == "growth":
+        if name in ("growth", "extent"):
             prog.rate_species[r] = -1
         elif name in species:
             prog.rate_species[r] = list(species).index(name)
         else:
-            raise SymError("rate '%s' is neither 'growth' nor a substrate of this run" % name)
+            raise SymError("rate '%s' is neither '%s' nor a substrate of this run"
+                           % (name, "extent" if abiotic else "growth"))
         depth = 0
         for op, arg in code:
             if op == P.SYM_CONST:

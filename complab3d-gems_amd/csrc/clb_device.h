@@ -106,6 +106,8 @@ struct KinDev {
     // symbolic laws of the biotic rates, one program per microbe (device memory)
     const clb_sym_program *sym;
     int symOn;
+    // the abiotic symbolic law (device memory), null when there is none
+    const clb_sym_program *symAb;
     // thermodynamic gate of the biotic rates (clb_case_params thm_*)
     int thm, thmMicrobe;
     double thmRT, thmDG0, thmLoss, thmThr, thmChi, thmCmin, thmScale;

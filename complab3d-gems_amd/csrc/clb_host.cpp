@@ -156,6 +156,10 @@ struct Case : CaseBase {
     clb_sym_program *symDev = nullptr;
     uint32_t symMask = 0;
     bool symOn = false;
+    // the abiotic symbolic law, held after the microbes' programs
+    bool symAb = false;
+    size_t symSlots() const { return (size_t) std::max(p.n_microbes, 1) + 1; }
+    clb_sym_program *symAbDev() const { return symDev + symSlots() - 1; }
     double gmaxClosed = -1e300;
     long long msumClosed = 0;
     int batchLen = 8;
@@ -228,12 +232,13 @@ struct Case : CaseBase {
         for (int s = 0; s < CLB_DEV_MAXS; ++s) k.dStoich[s] = p.dissol_stoich[s];
         k.phase = phase; k.share = share; k.dcount = dcount;
         k.sym = symDev; k.symOn = symOn ? 1 : 0;
+        k.symAb = symAb ? symAbDev() : nullptr;
         if (symOn && p.n_microbes > 0) k.bio = 1;
         k.thm = p.thm_on ? 1 : 0; k.thmMicrobe = p.thm_microbe;
         k.thmRT = p.thm_rt; k.thmDG0 = p.thm_dg0; k.thmLoss = p.thm_dgloss; k.thmThr = p.thm_threshold;
         k.thmChi = p.thm_chi; k.thmCmin = p.thm_cmin; k.thmScale = p.thm_conc_scale;
         for (int s = 0; s < CLB_DEV_MAXS; ++s) k.thmNu[s] = p.thm_nu[s];
-        if (!k.bio && k.nrxn == 0 && !k.dissol) k.on = 0;
+        if (!k.bio && k.nrxn == 0 && !k.dissol && !k.symAb) k.on = 0;
         return k;
     }
     NsDev<RN> ns() const {
@@ -408,8 +413,8 @@ struct Case : CaseBase {
         nsDone = dalloc<int>(allocs, 1);
         tracer = dalloc<NsTracer>(allocs, 1);
         ctl = dalloc<StepCtl>(allocs, 1);
-        symDev = dalloc<clb_sym_program>(allocs, (size_t) std::max(p.n_microbes, 1));
-        HIPT(hipMemsetAsync(symDev, 0, (size_t) std::max(p.n_microbes, 1) * sizeof(clb_sym_program), st));
+        symDev = dalloc<clb_sym_program>(allocs, symSlots());
+        HIPT(hipMemsetAsync(symDev, 0, symSlots() * sizeof(clb_sym_program), st));
         if (p.dissol_on) {
             share = dalloc<double>(allocs, (size_t) std::max(p.n_species, 1) * Nopen);
             nearList = dalloc<int>(allocs, std::max(Nint, 1));
@@ -530,7 +535,8 @@ struct Case : CaseBase {
 
     // ------------------------------------------------------------ symbolic laws
     Status setSymbolic(int microbe, const clb_sym_program *prog) override {
-        if (microbe < 0 || microbe >= p.n_microbes) return {CLB_E_ARG, "microbe index out of range"};
+        const bool abiotic = microbe == CLB_SYM_ABIOTIC;
+        if (!abiotic && (microbe < 0 || microbe >= p.n_microbes)) return {CLB_E_ARG, "microbe index out of range"};
         if (prog) {
             if (p.dissol_on) return {CLB_E_ARG, "symbolic rate laws are not combined with dissolution here"};
             const clb_sym_program &P = *prog;
@@ -538,8 +544,10 @@ struct Case : CaseBase {
                 P.n_rates < 1 || P.n_rates > CLB_SYM_MAXRATES || P.n_const < 0 || P.n_const > CLB_SYM_MAXCONST)
                 return {CLB_E_ARG, "symbolic law: a count is out of range"};
             if (!(P.unit_scale > 0.0) || !std::isfinite(P.unit_scale)) return {CLB_E_ARG, "symbolic law: unit_scale"};
-            for (int i = 0; i < P.n_vars; ++i)
+            for (int i = 0; i < P.n_vars; ++i) {
                 if (P.var_species[i] < -1 || P.var_species[i] >= p.n_species) return {CLB_E_ARG, "symbolic law: variable binding"};
+                if (abiotic && P.var_species[i] < 0) return {CLB_E_ARG, "the abiotic law binds no biomass"};
+            }
             int pc = 0;
             for (int r = 0; r < P.n_rates; ++r) {
                 if (P.rate_species[r] < -1 || P.rate_species[r] >= p.n_species) return {CLB_E_ARG, "symbolic law: rate binding"};
@@ -558,8 +566,12 @@ struct Case : CaseBase {
                 }
                 if (sp != 1) return {CLB_E_ARG, "symbolic law: a rate's code must leave one value"};
             }
-            HIPT(hipMemcpy(symDev + microbe, prog, sizeof(clb_sym_program), hipMemcpyHostToDevice));
-            symMask |= 1u << microbe;
+            clb_sym_program *dst = abiotic ? symAbDev() : symDev + microbe;
+            HIPT(hipMemcpy(dst, prog, sizeof(clb_sym_program), hipMemcpyHostToDevice));
+            if (abiotic) symAb = true; else symMask |= 1u << microbe;
+        } else if (abiotic) {
+            HIPT(hipMemset(symAbDev(), 0, sizeof(clb_sym_program)));
+            symAb = false;
         } else {
             HIPT(hipMemset(symDev + microbe, 0, sizeof(clb_sym_program)));
             symMask &= ~(1u << microbe);
@@ -602,7 +614,7 @@ struct Case : CaseBase {
         if (pushed) v.push_back({pushed, n});
         v.push_back({u, 3 * no * r});
         v.push_back({rhoN, sm * (size_t) std::max(Nopen - Nint, 1) * sizeof(double)});
-        v.push_back({symDev, (size_t) std::max(p.n_microbes, 1) * sizeof(clb_sym_program)});
+        v.push_back({symDev, symSlots() * sizeof(clb_sym_program)});
         v.push_back({clampCount, sizeof(unsigned long long)});
         v.push_back({dcount, 3 * sizeof(unsigned long long)});
         v.push_back({ctl, sizeof(StepCtl)});
@@ -627,7 +639,7 @@ struct Case : CaseBase {
         h.iT = iT; h.oldTotMask = oldTotMask; h.msumClosed = msumClosed; h.launchNo = launchNo;
         h.nsSaturate = nsSaturate; h.batchLen = batchLen; h.caSweep = caSweep_;
         for (int k = 0; k < CLB_NCOUNTERS; ++k) h.cnt[k] = cnt[k];
-        h.symMask = symMask; h.symOn = symOn ? 1u : 0u;
+        h.symMask = symMask; h.symOn = (symOn ? 1u : 0u) | (symAb ? 2u : 0u);
         return h;
     }
     size_t checkpointBytes() override {
@@ -685,7 +697,7 @@ struct Case : CaseBase {
         for (int k = 0; k < CLB_NCOUNTERS; ++k) cnt[k] = h.cnt[k];
         kms = 0; klaunch = 0;
         nearDirty = true;
-        symMask = h.symMask; symOn = h.symOn != 0;
+        symMask = h.symMask; symOn = (h.symOn & 1u) != 0; symAb = (h.symOn & 2u) != 0;
         return {CLB_OK, ""};
     }
 
@@ -815,7 +827,7 @@ struct Case : CaseBase {
     // with the gather at the waiting voxels (the reference's order,
     // complab.cpp:2461-2545).
     void transportLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr, bool maskTotal = false) {
-        if (kinOn && symOn) {
+        if (kinOn && (symOn || symAb)) {
             // the symbolic laws run in the split kinetics kernel, biotic then
             // abiotic, on the transported state (the biomass changes there, so
             // the step's mask/total update follows it, not the transport)
@@ -884,8 +896,8 @@ struct Case : CaseBase {
     void kinetics(hipStream_t st, int which) override {
         KinDev k = kin(true);
         if (!(which & CLB_KIN_BIOTIC)) k.bio = 0;
-        if (!(which & CLB_KIN_ABIOTIC)) { k.nrxn = 0; k.dissol = 0; }
-        if (!k.bio && k.nrxn == 0 && !k.dissol) k.on = 0;
+        if (!(which & CLB_KIN_ABIOTIC)) { k.nrxn = 0; k.dissol = 0; k.symAb = nullptr; }
+        if (!k.bio && k.nrxn == 0 && !k.dissol && !k.symAb) k.on = 0;
         if (k.dissol) {
             // the biotic family first, on its own, then park + abiotic + gather
             if (k.bio) { KinDev kb = k; kb.nrxn = 0; kb.dissol = 0; HIPT(launchKinetics<R>(ade(), kb, st)); }
@@ -1125,7 +1137,7 @@ struct Case : CaseBase {
         // the transport kernels make the step's mask and total update and its
         // block partials (interior, then planes) when every open voxel passes
         // through them; the final reduction takes the step's decision
-        const bool fused = Sm > 0 && Nint > 0 && !symOn;
+        const bool fused = Sm > 0 && Nint > 0 && !symOn && !symAb;
         const int nparts = (int) (nblk(Nint, 256) + nblk(Nopen - Nint, 256));
         for (int j = 0; j < len; ++j) {
             transportLaunch(st, true, &ctl->halt, fused);

@@ -298,6 +298,28 @@ __device__ void symbioticIncrements(const KinDev &k, const double *C, const doub
     }
 }
 
+// the abiotic symbolic law at one open voxel (run_symbolic_abiotic3D,
+// complab3d_processors_abiotic_learned.hh:60-135): no biomass; each species'
+// rate x dt is a request held to what the voxel has, added to the voxel's
+// abiotic increment dAb when it is above the threshold
+__device__ void symAbioticIncrements(const KinDev &k, const double *C, double *dAb) {
+    const double thrd = 1e-12;
+    const clb_sym_program *P = k.symAb;
+    double avail[CLB_DEV_MAXS], req[CLB_DEV_MAXS], out[CLB_SYM_MAXRATES];
+    for (int s = 0; s < k.S; ++s) { avail[s] = C[s] > thrd ? C[s] : 0.0; req[s] = 0.0; }
+    symEvaluate(P, avail, 0.0, out);
+    for (int r = 0; r < P->n_rates; ++r) {
+        const int rs = P->rate_species[r];                           // -1: the extent line itself
+        if (rs >= 0) req[rs] += out[r] * k.dt;
+    }
+    for (int s = 0; s < k.S; ++s) {
+        double draw = req[s];
+        if (fabs(draw) <= thrd) continue;
+        if (draw + avail[s] < 0.0) draw = -avail[s];
+        if (fabs(draw) > thrd) dAb[s] += draw;
+    }
+}
+
 // a face neighbour of compact voxel i in the reaction box belongs to the
 // dissolving phase: i collects that neighbour's parked products (the
 // condition of the gather below and, seen from the phase voxel, the open
@@ -373,7 +395,7 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
             }
             // the step's increments: rate x dt of the compiled law, or the
             // symbolic laws' increments after their shared budget
-            if (SYM) symbioticIncrements(k, C, Bv, sr, br);
+            if (SYM && k.symOn) symbioticIncrements(k, C, Bv, sr, br);
             else {
                 bioticLaw(k, C, Bv, sr, br, mask);
 #pragma unroll
@@ -423,7 +445,8 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
             if (k.surface_only) react = surf;
             deferred = react && near;
         }
-        if ((k.nrxn > 0 || DISSOL) && !deferred) {
+        const bool symAb = SYM && k.symAb != nullptr;
+        if ((k.nrxn > 0 || DISSOL || symAb) && !deferred) {
             // one increment per species: the abiotic rates, then (dissolution) the
             // product shares the face neighbours of the phase parked this step,
             // applied together (update_abiotic_rxnLattices, part1:306-404)
@@ -442,6 +465,11 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
                         if (d > 1e-12 || d < -1e-12) dAb[s] += d;
                     }
                 }
+            }
+            if (symAb) {                                                  // complab.cpp:2489-2493, same increments
+#pragma unroll
+                for (int s = 0; s < CLB_DEV_MAXS; ++s) { if (s < S) { C[s] = concOf(s); if (C[s] < 1e-12) C[s] = 0; } }
+                symAbioticIncrements(k, C, dAb);
             }
             if (DISSOL) {                                                 // dissolutionGather3D, dissolutionVOP.hh:607-720
                 unsigned int collected = 0, clamped = 0;
@@ -1471,7 +1499,7 @@ template <typename R>
 hipError_t launchKinetics(const AdeDev<R> &a, const KinDev &k, hipStream_t st) {
     if (a.Nint <= 0 || !k.on) return hipSuccess;
     if (k.dissol) kineticsVariants<R, true, false>(a, k, st);           // symbolic laws are refused with dissolution
-    else if (k.symOn && k.bio) kineticsVariants<R, false, true>(a, k, st);
+    else if ((k.symOn && k.bio) || k.symAb) kineticsVariants<R, false, true>(a, k, st);
     else kineticsVariants<R, false, false>(a, k, st);
     return hipGetLastError();
 }

@@ -338,12 +338,17 @@ int  clb_counters(clb_ctx *ctx, int case_id, int64_t out[CLB_NCOUNTERS]);
  * square root of a non-positive argument give 0.  Variables with hi > lo are
  * held within [lo, hi].  A variable binds a species' concentration or, with
  * var_species -1, the organism's own biomass; a rate with rate_species s is
- * species s's rate, with -1 the specific growth rate (times the biomass). */
+ * species s's rate, with -1 the specific growth rate (times the biomass).
+ * The abiotic law (microbe CLB_SYM_ABIOTIC, run_symbolic_abiotic3D,
+ * complab3d_processors_abiotic_learned.hh:49-150) binds species only; its
+ * rate_species -1 is the extent line, which feeds the species lines derived
+ * from it and moves nothing itself. */
 #define CLB_SYM_MAXOPS    256
 #define CLB_SYM_MAXCONST  64
 #define CLB_SYM_MAXVARS   16
 #define CLB_SYM_MAXRATES  16
 #define CLB_SYM_STACK     16
+#define CLB_SYM_ABIOTIC   (-1)                  /* clb_set_symbolic: the abiotic law */
 enum {
     CLB_SYM_CONST = 1, CLB_SYM_VAR, CLB_SYM_NEG, CLB_SYM_ADD, CLB_SYM_SUB, CLB_SYM_MUL, CLB_SYM_DIV,
     CLB_SYM_POW, CLB_SYM_EXP, CLB_SYM_LOG, CLB_SYM_LOG10, CLB_SYM_SQRT, CLB_SYM_ABS, CLB_SYM_TANH,
@@ -363,7 +368,11 @@ typedef struct clb_sym_program {
 /* The biotic rates of microbe `microbe` come from `prog` instead of the
  * compiled law (every microbe of a case takes one, before its first step);
  * the program is checked (op codes, slots, stack depth) and CLB_E_ARG
- * returned when it does not run; NULL switches the microbe back. */
+ * returned when it does not run; NULL switches the microbe back.  With
+ * microbe = CLB_SYM_ABIOTIC, `prog` is the abiotic law: it runs at every open
+ * voxel of the reaction box after the biotic update, with its own shared
+ * budget, and its increments join the mass-action ones (complab.cpp:2455-2493;
+ * with or without n_rxn > 0).  Symbolic laws are refused with dissolution. */
 int  clb_set_symbolic(clb_ctx *ctx, int case_id, int microbe, const clb_sym_program *prog);
 
 /* ---- checkpoint / restart (saveBinaryBlock / loadBinaryBlock of every lattice,

@@ -104,6 +104,9 @@ struct Case {
     // symbolic laws, one program per microbe; in use when every microbe has one
     std::vector<clb_sym_program> sym;
     uint32_t symMask = 0;
+    // the abiotic symbolic law (<symbolic><abiotic_file>), in use when symAbOn
+    clb_sym_program symAb{};
+    bool symAbOn = false;
 
     size_t id(int x, int y, int z) const { return (size_t) x + (size_t) nx * ((size_t) y + (size_t) ny * (size_t) z); }
     bool inside(int x, int y, int z) const { return x >= 0 && y >= 0 && z >= 0 && x < nx && y < ny && z < nz; }
@@ -314,6 +317,28 @@ void symbioticIncrements(const Case &c, const double *C, const double *B, double
         const double headroom = avail[s] + dC[s];
         if (draw + headroom < 0.0) draw = -headroom;
         dC[s] += draw;
+    }
+}
+
+/* run_symbolic_abiotic3D (complab3d_processors_abiotic_learned.hh:60-135): the
+ * abiotic .sym law at one open voxel, no biomass; each substrate's rate x dt
+ * is a request held to what the voxel has (a budget of its own), added to the
+ * voxel's abiotic increment when it is above the threshold */
+void symAbioticAdd(const Case &c, const double *C, double *dAb) {
+    const clb_case_params &p = c.p;
+    const double thrd = THRD;
+    double avail[CLB_MAX_SPECIES], req[CLB_MAX_SPECIES], out[CLB_SYM_MAXRATES];
+    for (int s = 0; s < p.n_species; ++s) { avail[s] = C[s] > thrd ? C[s] : 0.0; req[s] = 0.0; }
+    symEvaluate(c.symAb, avail, 0.0, out);
+    for (int r = 0; r < c.symAb.n_rates; ++r) {
+        const int rs = c.symAb.rate_species[r];                      // -1: the extent line itself
+        if (rs >= 0) req[rs] += out[r] * p.dt;
+    }
+    for (int s = 0; s < p.n_species; ++s) {
+        double draw = req[s];
+        if (std::fabs(draw) <= thrd) continue;
+        if (draw + avail[s] < 0.0) draw = -avail[s];
+        if (std::fabs(draw) > thrd) dAb[s] += draw;
     }
 }
 
@@ -558,6 +583,17 @@ void abioticStep(Case &c) {
             for (int s = 0; s < S; ++s) { C[s] = conc(c, s, i); if (C[s] < THRD) C[s] = 0; }
             abioticLaw(p, C, sr);
             for (int s = 0; s < S; ++s) { double d = sr[s] * p.dt; if (d > THRD || d < -THRD) dC[(size_t) s * c.N + i] += d; }
+        }
+    }
+    if (c.symAbOn) {                                                    // complab.cpp:2489-2493, same increments
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 1; x < c.nx - 1; ++x) {
+            size_t i = c.id(x, y, z);
+            if (c.solidOrBB(c.mask[i])) continue;
+            double C[CLB_MAX_SPECIES], d[CLB_MAX_SPECIES];
+            for (int s = 0; s < S; ++s) { C[s] = conc(c, s, i); if (C[s] < THRD) C[s] = 0; d[s] = 0.0; }
+            symAbioticAdd(c, C, d);
+            for (int s = 0; s < S; ++s) dC[(size_t) s * c.N + i] += d[s];
         }
     }
     if (p.dissol_on) {
@@ -1323,7 +1359,7 @@ int coupledStep(Case &c) {
     c.adeClsCollide = c.adeCls;
     for (int s = 0; s < p.n_species; ++s) if (p.mobile[s]) adeCollide(c, s);
     if ((p.bio_law != CLB_BIO_NONE || symActive(c)) && p.n_microbes > 0) bioticStep(c);
-    if (p.n_rxn > 0 || p.dissol_on) abioticStep(c);
+    if (p.n_rxn > 0 || p.dissol_on || c.symAbOn) abioticStep(c);
     if (p.eq_on) equilibriumStep(c);
     int rc = CLB_OK;
     if (p.n_microbes > 0) {
@@ -1380,7 +1416,7 @@ void transportStep(Case &c) {
     const clb_case_params &p = c.p;
     for (int s = 0; s < p.n_species; ++s) if (p.mobile[s]) adeCollide(c, s);
     if ((p.bio_law != CLB_BIO_NONE || symActive(c)) && p.n_microbes > 0) bioticStep(c);
-    if (p.n_rxn > 0 || p.dissol_on) abioticStep(c);
+    if (p.n_rxn > 0 || p.dissol_on || c.symAbOn) abioticStep(c);
     for (int s = 0; s < p.n_species; ++s) if (p.mobile[s]) adeStream(c, s);
 }
 
@@ -1699,9 +1735,15 @@ int orc_eq_voxel(const clb_case_params *p, const double *cin, double *cout) {
     return 0;
 }
 
-/* the biotic rates of microbe m from a symbolic law (NULL: back to the compiled law) */
+/* the biotic rates of microbe m from a symbolic law (NULL: back to the compiled
+ * law); m = CLB_SYM_ABIOTIC sets or clears the abiotic law */
 int orc_set_symbolic(void *h, int m, const clb_sym_program *prog) {
     Case &c = *(Case *) h;
+    if (m == CLB_SYM_ABIOTIC) {                                         // the abiotic law
+        if (prog) { c.symAb = *prog; c.symAbOn = true; }
+        else { memset(&c.symAb, 0, sizeof(clb_sym_program)); c.symAbOn = false; }
+        return 0;
+    }
     if (m < 0 || m >= c.p.n_microbes) return CLB_E_ARG;
     if (c.sym.size() < (size_t) c.p.n_microbes) c.sym.resize((size_t) c.p.n_microbes);
     if (prog) { c.sym[(size_t) m] = *prog; c.symMask |= 1u << m; }

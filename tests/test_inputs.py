@@ -193,3 +193,32 @@ def test_reference_example_runs_through_the_oracle_setup():
     assert o.run_steps(5) == (0, 5)
     tot = o.species_totals()
     assert np.all(np.isfinite(tot))
+
+
+def test_abiotic_symbolic_file_hook(tmp_path):
+    """<symbolic><abiotic_file> binds an extent law to the run's substrates
+    (complab3d_integration.hh:264-300, complab.cpp:2455-2493); refused beside
+    dissolution and for a biotic (growth) law."""
+    nx, ny, nz = 6, 4, 3
+    xml = write_case(tmp_path, [5] * (nx * ny * nz))
+    dissol = XML[XML.index("  <dissolution>"):XML.index("  <diagnostics>")]
+    sym = "  <symbolic><enabled>true</enabled><abiotic_file>ab.sym</abiotic_file></symbolic>\n"
+    (tmp_path / "ab.sym").write_text("units per_hour\nvars A B\nreaction A -1 B 2\nrate extent = 3.6 * A * B\n")
+    with open(xml, "w") as fh:
+        fh.write(XML.replace("  <diagnostics>", sym + "  <diagnostics>"))
+    with pytest.raises(inputs.InputError):                     # with dissolution
+        inputs.load(xml)
+    with open(xml, "w") as fh:
+        fh.write(XML.replace(dissol, sym))
+    r = inputs.load(xml)
+    prog = r.symbolic[P.SYM_ABIOTIC]
+    assert prog.n_vars == 2 and list(prog.var_species[:2]) == [0, 1]
+    assert list(prog.rate_species[:3]) == [-1, 0, 1] and prog.unit_scale == 1.0 / 3600
+    out = O.sym_evaluate(prog, [0.5, 0.25, 0.0], 0.0)
+    assert out[1] == pytest.approx(-3.6 * 0.125 / 3600, rel=1e-15) and out[2] == -2 * out[1]
+    (tmp_path / "ab.sym").write_text("vars A Bug\nrate growth = A\n")
+    with pytest.raises(inputs.InputError):
+        inputs.load(xml)
+    with open(xml, "w") as fh:                                 # not read while <symbolic> is off
+        fh.write(XML.replace(dissol, sym.replace("true", "false")))
+    assert P.SYM_ABIOTIC not in inputs.load(xml).symbolic

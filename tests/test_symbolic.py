@@ -69,7 +69,6 @@ def test_reader_refusals(tmp_path):
            "forward reference": "vars A\nrate growth = B2\nrate B2 = A\n",
            "missing paren": "vars A\nrate growth = (A + 1\n",
            "trailing text": "vars A\nrate growth = A A\n",
-           "abiotic extent": "vars A B\nreaction A -1 B 1\nrate extent = A\n",
            "duplicate rate": "vars A\nrate growth = A\nrate growth = 2\n",
            "unknown keyword": "vars A\nratee growth = A\n",
            "bad units": "units per_day\nvars A\nrate growth = A\n",
@@ -145,3 +144,82 @@ def test_symbolic_monod_runs_like_the_compiled_law(tmp_path):
     for s in range(w.params.n_species):
         ca, cb = a.read(P.FIELD_CONC, s), b.read(P.FIELD_CONC, s)
         assert np.max(np.abs(ca - cb)) <= 1e-9 * max(np.max(np.abs(ca)), 1e-300), s
+
+
+# ---- the abiotic form: one extent line (tests/test_sym_stoich.cpp:138-216)
+
+def test_abiotic_extent_matches_hand_written(tmp_path):
+    hand = tmp_path / "ahand.sym"
+    hand.write_text("units per_second\nvars Fe HS\nrate Fe = -1.6e2 * Fe * HS\nrate HS = -1.6e2 * Fe * HS\n")
+    decl = tmp_path / "adecl.sym"
+    decl.write_text("units per_second\nvars Fe HS FeS\nreaction Fe -1  HS -1  FeS 1\nrate extent = 1.6e2 * Fe * HS\n")
+    lh, ld = S.read_sym(str(hand)), S.read_sym(str(decl))
+    assert ld.derived == 3 and S.is_abiotic(ld) and not S.is_abiotic(lh)
+    sp = ["Fe", "HS", "FeS"]
+    ah, ad = S.compile_law(lh, sp, None), S.compile_law(ld, sp, None)
+    assert list(ad.rate_species[:4]) == [-1, 0, 1, 2]
+    for i in range(1, 41):
+        c = [1e-3 * i / 40.0, 1e-3 * (41 - i) / 40.0, 0.0]
+        rh, rd = O.sym_evaluate(ah, c, 0.0), O.sym_evaluate(ad, c, 0.0)
+        assert np.array_equal(rh, rd[1:3])                     # derived Fe and HS equal the written pair
+    rd = O.sym_evaluate(ad, [4e-4, 7e-4, 0.0], 0.0)
+    assert rd[1] == rd[2] and rd[3] == -rd[1] and rd[3] > 0 > rd[1]
+    assert rd[0] == pytest.approx(1.6e2 * 4e-4 * 7e-4, rel=1e-15)
+
+
+def test_abiotic_refusals_and_binding(tmp_path):
+    bad = {"extent with biomass": "vars Fe HS Bug\nreaction Fe -1 HS -1\nbiomass Bug\nrate extent = Fe\n",
+           "extent with yield": "vars Fe HS\nreaction Fe -1 HS -1\nyield Fe 0.4\nrate extent = Fe\n",
+           "growth and extent": "vars Fe HS Bug\nreaction Fe -1 HS -1\nyield Fe 0.4\nbiomass Bug\n"
+                                "rate growth = Fe\nrate extent = Fe\n",
+           "extent, no reaction": "vars Fe HS\nrate extent = Fe * HS\n",
+           "reaction, no rate of it": "vars Fe HS\nreaction Fe -1 HS -1\nrate Fe = -1 * Fe\n",
+           "rate line and reaction place": "vars Fe HS\nreaction Fe -1 HS -1\nrate extent = Fe\nrate Fe = -1\n",
+           "yield without reaction": "vars A Bug\nyield A 0.4\nrate growth = A\n",
+           "no vars": "rate growth = 1\n"}
+    for what, text in bad.items():
+        f = tmp_path / "bad.sym"
+        f.write_text(text)
+        with pytest.raises(S.SymError):
+            S.read_sym(str(f))
+    f = tmp_path / "ab.sym"
+    f.write_text("vars Fe HS FeS\nreaction Fe -1  HS -1  FeS 1\nrate extent = 1.6e2 * Fe * HS\n")
+    ab = S.read_sym(str(f))
+    with pytest.raises(S.SymError):                            # an abiotic law handed to an organism
+        S.compile_law(ab, ["Fe", "HS", "FeS"], "Bug")
+    S.compile_law(ab, ["Fe", "HS", "FeS"], None)
+    f.write_text("vars acetate o2 Bug\nreaction acetate -1 o2 -2\nyield acetate 0.4\nbiomass Bug\n"
+                 "rate growth = 0.35 * acetate\n")
+    bio = S.read_sym(str(f))
+    with pytest.raises(S.SymError):                            # a biotic law through <abiotic_file>
+        S.compile_law(bio, ["acetate", "o2"], None)
+    S.compile_law(bio, ["acetate", "o2"], "Bug")
+    f.write_text("vars Fe X\nrate Fe = -Fe\n")
+    with pytest.raises(S.SymError):                            # X binds no substrate
+        S.compile_law(S.read_sym(str(f)), ["Fe", "HS"], None)
+
+
+def test_abiotic_symbolic_runs_like_mass_action(tmp_path):
+    """A -> B at k = 1e-3 written as a .sym extent law: the oracle's run equals
+    the mass-action run bit for bit (the same products, the same order)."""
+    w = configs.config1(scale=0.15)
+    w.transport.stabilize_steps = 10
+    w.params.ns_max_iter = 2000
+    text = "units per_second\nvars A B\nreaction A -1 B 1\nrate extent = 1e-3 * A\n"
+    prog = law_of(tmp_path, text, species=("A", "B"), microbe=None)
+    runs = []
+    for sym in (False, True):
+        p = P.copy_params(w.params)
+        if sym:
+            p.n_rxn = 0
+        o = O.OracleCase(p, w.geometry)
+        if sym:
+            o.set_symbolic(P.SYM_ABIOTIC, prog)
+        rep = driver.setup(o, p, w.flow, w.transport)
+        o.set_omegas(rep.omega_pore, rep.omega_biofilm, 1.0)
+        o.run_steps(40)
+        runs.append(o)
+    a, b = runs
+    for s in range(2):
+        assert np.array_equal(a.read(P.FIELD_CONC, s), b.read(P.FIELD_CONC, s)), s
+    assert a.read(P.FIELD_CONC, 1).max() > 0                   # the product appeared
