@@ -407,8 +407,6 @@ def load(xml_path, geometry_path: Optional[str] = None, precision=P.FP64, order=
     # the shared <symbolic><expressions_file>
     symbolic = {}
     if "symbolic" in rtypes:
-        if "kinetics" in rtypes:
-            raise InputError("symbolic and compiled-kinetics organisms in one run are not combined on this path")
         if not doc.flag("symbolic", "enabled"):
             raise InputError("a microbe has reaction_type symbolic but <symbolic><enabled> is not true")
         shared = doc.text("symbolic", "expressions_file", default=None)

@@ -105,7 +105,9 @@ struct KinDev {
     int nearPhase;
     // symbolic laws of the biotic rates, one program per microbe (device memory)
     const clb_sym_program *sym;
-    int symOn;
+    int symOn;                     // some microbe has one
+    unsigned int symMask;          // which microbes: the compiled law skips them
+    int compiled;                  // the compiled biotic law is on (bio_law)
     // the abiotic symbolic law (device memory), null when there is none
     const clb_sym_program *symAb;
     // thermodynamic gate of the biotic rates (clb_case_params thm_*)

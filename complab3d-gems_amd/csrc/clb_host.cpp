@@ -231,7 +231,8 @@ struct Case : CaseBase {
         k.dSurfaceOnly = p.dissol_surface_only; k.dK = p.dissol_k;
         for (int s = 0; s < CLB_DEV_MAXS; ++s) k.dStoich[s] = p.dissol_stoich[s];
         k.phase = phase; k.share = share; k.dcount = dcount;
-        k.sym = symDev; k.symOn = symOn ? 1 : 0;
+        k.sym = symDev; k.symOn = symOn ? 1 : 0; k.symMask = symMask;
+        k.compiled = k.bio;
         k.symAb = symAb ? symAbDev() : nullptr;
         if (symOn && p.n_microbes > 0) k.bio = 1;
         k.thm = p.thm_on ? 1 : 0; k.thmMicrobe = p.thm_microbe;
@@ -576,8 +577,7 @@ struct Case : CaseBase {
             HIPT(hipMemset(symDev + microbe, 0, sizeof(clb_sym_program)));
             symMask &= ~(1u << microbe);
         }
-        const uint32_t all = p.n_microbes >= 32 ? 0xffffffffu : ((1u << p.n_microbes) - 1u);
-        symOn = p.n_microbes > 0 && symMask == all;
+        symOn = p.n_microbes > 0 && symMask != 0;
         return {CLB_OK, ""};
     }
 
@@ -1167,9 +1167,6 @@ struct Case : CaseBase {
     Status runSteps(hipStream_t st, int n, int *done) override {
         int k = 0;
         Status s{CLB_OK, ""};
-        if (symMask != 0 && !symOn)
-            return {CLB_E_STATE, "some microbes have a symbolic law and some do not; every microbe needs one "
-                                 "(the compiled and symbolic laws are not mixed here)"};
         if (p.n_microbes > 0) {
             while (k < n) {
                 int fin = 0;

@@ -147,13 +147,16 @@ __device__ double thermoGate(const KinDev &k, const double *C) {
 
 // dual Monod, config/kinetics/defineKinetics.biotic.hh:90-126; with the
 // thermodynamic gate the whole rate vector is scaled by F_T (part1:127-140)
-__device__ void bioticLaw(const KinDev &k, const double *C, const double *B, double *sr, double *br, int mask) {
+// (microbes in `skip` have a symbolic law and are left out)
+__device__ void bioticLaw(const KinDev &k, const double *C, const double *B, double *sr, double *br, int mask,
+                          unsigned int skip = 0u) {
     for (int s = 0; s < k.S; ++s) sr[s] = 0;
     for (int m = 0; m < k.M; ++m) br[m] = 0;
     if (mask < 2) return;
     const double S = k.donor >= 0 ? fmax(C[k.donor], 0.0) : 0.0;
     const double E = k.acceptor >= 0 ? fmax(C[k.acceptor], 0.0) : 1.0;
     for (int m = 0; m < k.M; ++m) {
+        if ((skip >> m) & 1u) continue;
         const double Bm = fmax(B[m], 0.0);
         if (Bm <= 0.0) continue;
         double mu = k.mu_max[m] * (S / (k.ks[m] + S));
@@ -393,10 +396,26 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
                 Bv[m] = havePre ? (double) bpre[m] : (double) a.B[(long long) m * N + v];
                 if (Bv[m] < 1e-12) Bv[m] = 0;
             }
-            // the step's increments: rate x dt of the compiled law, or the
-            // symbolic laws' increments after their shared budget
-            if (SYM && k.symOn) symbioticIncrements(k, C, Bv, sr, br);
-            else {
+            // the step's increments: rate x dt of the compiled law and, in the
+            // symbolic variant, the symbolic organisms' increments after their
+            // shared budget, each kept when above the threshold (run_kinetics
+            // then run_symbolic3D into the same increments, complab.cpp:2348-2393)
+            if (SYM) {
+                double sc[CLB_DEV_MAXS], sb[CLB_DEV_MAXM];
+                if (k.compiled) bioticLaw(k, C, Bv, sr, br, mask, k.symMask);
+                else {
+#pragma unroll
+                    for (int s = 0; s < CLB_DEV_MAXS; ++s) sr[s] = 0.0;
+#pragma unroll
+                    for (int m = 0; m < CLB_DEV_MAXM; ++m) br[m] = 0.0;
+                }
+                symbioticIncrements(k, C, Bv, sc, sb);
+                auto kept = [](double x) { return (x > 1e-12 || x < -1e-12) ? x : 0.0; };
+#pragma unroll
+                for (int s = 0; s < CLB_DEV_MAXS; ++s) if (s < S) sr[s] = kept(sr[s] * k.dt) + kept(sc[s]);
+#pragma unroll
+                for (int m = 0; m < CLB_DEV_MAXM; ++m) if (m < k.M) br[m] = kept(br[m] * k.dt) + kept(sb[m]);
+            } else {
                 bioticLaw(k, C, Bv, sr, br, mask);
 #pragma unroll
                 for (int s = 0; s < CLB_DEV_MAXS; ++s) if (s < S) sr[s] *= k.dt;
@@ -1499,7 +1518,7 @@ template <typename R>
 hipError_t launchKinetics(const AdeDev<R> &a, const KinDev &k, hipStream_t st) {
     if (a.Nint <= 0 || !k.on) return hipSuccess;
     if (k.dissol) kineticsVariants<R, true, false>(a, k, st);           // symbolic laws are refused with dissolution
-    else if ((k.symOn && k.bio) || k.symAb) kineticsVariants<R, false, true>(a, k, st);
+    else if ((k.symOn && k.bio) || k.symAb) kineticsVariants<R, false, true>(a, k, st);   // symbolic laws
     else kineticsVariants<R, false, false>(a, k, st);
     return hipGetLastError();
 }

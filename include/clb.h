@@ -366,7 +366,9 @@ typedef struct clb_sym_program {
     double  unit_scale;                         /* 1 per second, 1/3600 per hour */
 } clb_sym_program;
 /* The biotic rates of microbe `microbe` come from `prog` instead of the
- * compiled law (every microbe of a case takes one, before its first step);
+ * compiled law (set before the first step; the compiled law keeps the
+ * microbes without one, and both add into the step's increments as
+ * run_kinetics and run_symbolic3D do, complab.cpp:2348-2393);
  * the program is checked (op codes, slots, stack depth) and CLB_E_ARG
  * returned when it does not run; NULL switches the microbe back.  With
  * microbe = CLB_SYM_ABIOTIC, `prog` is the abiotic law: it runs at every open

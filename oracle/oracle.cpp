@@ -342,21 +342,20 @@ void symAbioticAdd(const Case &c, const double *C, double *dAb) {
     }
 }
 
-/* every microbe has a symbolic law (mixing with the compiled law is not built) */
-bool symActive(const Case &c) {
-    const uint32_t all = c.p.n_microbes >= 32 ? 0xffffffffu : ((1u << c.p.n_microbes) - 1u);
-    return c.p.n_microbes > 0 && c.symMask == all;
-}
+/* some microbe has a symbolic law */
+bool symActive(const Case &c) { return c.p.n_microbes > 0 && c.symMask != 0; }
 
 /* dual-Monod restatement of defineRxnKinetics (config/kinetics/defineKinetics.biotic.hh:90-126):
  * growth = mu_max * S/(Ks+S) [* E/(Ke+E)] * B ; bioR = growth - kd*B ; subsR_s += nu_s*growth */
-void bioticLaw(const clb_case_params &p, const double *C, const double *B, double *subsR, double *bioR, int mask) {
+void bioticLaw(const clb_case_params &p, const double *C, const double *B, double *subsR, double *bioR, int mask,
+               uint32_t skip = 0) {
     for (int s = 0; s < p.n_species; ++s) subsR[s] = 0;
     for (int m = 0; m < p.n_microbes; ++m) bioR[m] = 0;
     if (mask < 2) return;
     double S = p.donor >= 0 ? std::max(C[p.donor], 0.0) : 0.0;
     double E = p.acceptor >= 0 ? std::max(C[p.acceptor], 0.0) : 1.0;
     for (int m = 0; m < p.n_microbes; ++m) {
+        if ((skip >> m) & 1u) continue;                                 // a microbe with a symbolic law
         double Bm = std::max(B[m], 0.0);
         if (Bm <= 0.0) continue;
         double mu = p.mu_max[m] * (S / (p.ks[m] + S));
@@ -425,14 +424,19 @@ void bioticStep(Case &c) {
         double C[CLB_MAX_SPECIES], Bv[CLB_MAX_MICROBES], sr[CLB_MAX_SPECIES], br[CLB_MAX_MICROBES];
         for (int s = 0; s < S; ++s) { C[s] = conc(c, s, i); if (C[s] < THRD) C[s] = 0; }
         for (int m = 0; m < M; ++m) { Bv[m] = c.B[m][i]; if (Bv[m] < THRD) Bv[m] = 0; }
-        if (symActive(c)) symbioticIncrements(c, C, Bv, sr, br);       // increments of the step
-        else {
-            bioticLaw(p, C, Bv, sr, br, mask);
-            for (int s = 0; s < S; ++s) sr[s] *= p.dt;
-            for (int m = 0; m < M; ++m) br[m] *= p.dt;
+        // the compiled law for the microbes without a symbolic law (run_kinetics),
+        // then the symbolic organisms' increments (run_symbolic3D), into the
+        // same increments (complab.cpp:2348-2393)
+        if (p.bio_law != CLB_BIO_NONE) {
+            bioticLaw(p, C, Bv, sr, br, mask, c.symMask);
+            for (int s = 0; s < S; ++s) { double d = sr[s] * p.dt; if (d > THRD || d < -THRD) dC[(size_t) s * c.N + i] += d; }
+            for (int m = 0; m < M; ++m) { double d = br[m] * p.dt; if (d > THRD || d < -THRD) dB[(size_t) m * c.N + i] += d; }
         }
-        for (int s = 0; s < S; ++s) { double d = sr[s]; if (d > THRD || d < -THRD) dC[(size_t) s * c.N + i] += d; }
-        for (int m = 0; m < M; ++m) { double d = br[m]; if (d > THRD || d < -THRD) dB[(size_t) m * c.N + i] += d; }
+        if (symActive(c)) {
+            symbioticIncrements(c, C, Bv, sr, br);
+            for (int s = 0; s < S; ++s) { double d = sr[s]; if (d > THRD || d < -THRD) dC[(size_t) s * c.N + i] += d; }
+            for (int m = 0; m < M; ++m) { double d = br[m]; if (d > THRD || d < -THRD) dB[(size_t) m * c.N + i] += d; }
+        }
     }
 #pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 1; x < c.nx - 1; ++x) {

@@ -153,3 +153,41 @@ def test_abiotic_symbolic_beside_compiled_biotic_law(ctx, tmp_path):
         _close(g.read(P.FIELD_CONC, s)[om], o.read(P.FIELD_CONC, s)[om], s)
     assert o.read(P.FIELD_CONC, 3)[om].max() > 0
     g.close()
+
+
+def test_compiled_and_symbolic_organisms_match_oracle(ctx, tmp_path):
+    print("case: two organisms at 0.1 scale, microbe 0 compiled, microbe 1 symbolic (gate on microbe 1)", flush=True)
+    w = configs.config2(scale=0.1)
+    p = w.params
+    geo, _ = configs.seed_biofilm(w.geometry, 0.02, 11, code=P.BIOFILM + 1)
+    p.n_microbes = 2
+    p.mu_max[0] = 2e-2; p.mu_max[1] = 1.5e-2
+    p.ks[1] = p.ks[0]; p.ke[1] = p.ke[0]; p.kdecay[1] = 0.0
+    for s in range(p.n_species):
+        p.bio_stoich[1][s] = p.bio_stoich[0][s]
+    p.b0[0] = 95.0; p.b0[1] = 90.0
+    p.flow_on = 0
+    p.thm_on = 1
+    p.thm_microbe = 1
+    p.thm_threshold = 12.5
+    for s, v in enumerate([-1.0, -1.0, 1.0, 1.0]):
+        p.thm_nu[s] = v
+    p.thm_dg0 = -12.5 - 2.0 * p.thm_rt - p.thm_rt * (2 * np.log(p.thm_cmin) - 2 * np.log(0.5))
+    f = tmp_path / "law.sym"
+    f.write_text(LAW)
+    prog = S.compile_law(S.read_sym(str(f)), ["S%d" % s for s in range(p.n_species)], "Bug")
+    g = clb.Case(ctx, p, geo)
+    o = O.OracleCase(p, geo)
+    for c in (g, o):
+        c.set_symbolic(1, prog)
+        c.set_omegas([1.2] * p.n_species, [1.1] * p.n_species, 1.0)
+    assert g.run_steps(60) == o.run_steps(60) == (0, 60)
+    assert np.array_equal(g.read(P.FIELD_MASK), o.read(P.FIELD_MASK))
+    om = o.read(P.FIELD_MASK) >= 2
+    for m in range(2):
+        _close(g.read(P.FIELD_BIOMASS, m), o.read(P.FIELD_BIOMASS, m), "biomass %d" % m)
+    for s in range(4):
+        _close(g.read(P.FIELD_CONC, s)[om], o.read(P.FIELD_CONC, s)[om], s)
+    cg, co = g.counters(), o.counters()
+    assert cg[5] == co[5] and cg[6] == co[6]
+    g.close()

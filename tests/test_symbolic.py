@@ -223,3 +223,64 @@ def test_abiotic_symbolic_runs_like_mass_action(tmp_path):
     for s in range(2):
         assert np.array_equal(a.read(P.FIELD_CONC, s), b.read(P.FIELD_CONC, s)), s
     assert a.read(P.FIELD_CONC, 1).max() > 0                   # the product appeared
+
+
+def two_microbe_case():
+    """config2 at 0.1 scale with a second organism seeded beside the first,
+    both on the dual-Monod law with their own mu_max."""
+    w = configs.config2(scale=0.1)
+    p = w.params
+    geo, _ = configs.seed_biofilm(w.geometry, 0.02, 11, code=P.BIOFILM + 1)
+    p.n_microbes = 2
+    p.mu_max[0] = 2e-2; p.mu_max[1] = 1.5e-2
+    p.ks[1] = p.ks[0]; p.ke[1] = p.ke[0]; p.kdecay[0] = 0.0; p.kdecay[1] = 0.0
+    for s in range(p.n_species):
+        p.bio_stoich[1][s] = p.bio_stoich[0][s]
+    p.b0[0] = 95.0; p.b0[1] = 90.0
+    p.flow_on = 0
+    return w, geo
+
+
+def monod_sym_of(p, m):
+    names = ["S%d" % s for s in range(p.n_species)]
+    lines = ["vars %s Bug" % " ".join(names),
+             "rate growth = %.17g * S%d / (%.17g + S%d) * S%d / (%.17g + S%d)"
+             % (p.mu_max[m], p.donor, p.ks[m], p.donor, p.acceptor, p.ke[m], p.acceptor)]
+    for s in range(p.n_species):
+        if p.bio_stoich[m][s] != 0:
+            lines.append("rate S%d = %.17g * growth * Bug" % (s, p.bio_stoich[m][s]))
+    return "\n".join(lines) + "\n", names
+
+
+def test_compiled_and_symbolic_organisms_in_one_run(tmp_path):
+    """Microbe 0 on the compiled law, microbe 1 on a .sym restatement of its
+    own compiled law: the run equals the all-compiled run within 1e-9, and
+    each organism grows by its own law (a symbolic law left out, or applied
+    to the wrong organism, moves microbe 1's biomass)."""
+    w, geo = two_microbe_case()
+    text, names = monod_sym_of(w.params, 1)
+    prog = law_of(tmp_path, text, species=names)
+    runs = []
+    for mixed in (False, True):
+        o = O.OracleCase(w.params, geo)
+        if mixed:
+            o.set_symbolic(1, prog)
+        o.set_omegas([1.2] * w.params.n_species, [1.1] * w.params.n_species, 1.0)
+        b0 = [o.read(P.FIELD_BIOMASS, m).sum() for m in range(2)]
+        assert o.run_steps(40) == (0, 40)
+        assert all(o.read(P.FIELD_BIOMASS, m).sum() > b0[m] for m in range(2))
+        runs.append(o)
+    a, b = runs
+    assert np.array_equal(a.read(P.FIELD_MASK), b.read(P.FIELD_MASK))
+    for m in range(2):
+        ba, bb = a.read(P.FIELD_BIOMASS, m), b.read(P.FIELD_BIOMASS, m)
+        assert np.max(np.abs(ba - bb)) <= 1e-9 * np.max(np.abs(ba)), m
+    for s in range(w.params.n_species):
+        ca, cb = a.read(P.FIELD_CONC, s), b.read(P.FIELD_CONC, s)
+        assert np.max(np.abs(ca - cb)) <= 1e-9 * max(np.max(np.abs(ca)), 1e-300), s
+    # the same law given to microbe 0 instead runs microbe 0 at microbe 1's rate
+    o = O.OracleCase(w.params, geo)
+    o.set_symbolic(0, prog)
+    o.set_omegas([1.2] * w.params.n_species, [1.1] * w.params.n_species, 1.0)
+    o.run_steps(40)
+    assert not np.allclose(o.read(P.FIELD_BIOMASS, 0), a.read(P.FIELD_BIOMASS, 0), rtol=1e-6, atol=0)
