@@ -75,6 +75,7 @@ def lib():
         "clb_kernel_time": ([V, I, I, C.POINTER(D), C.POINTER(C.c_int64)], I),
         "clb_set_timing_stride": ([V, I, I], I),
         "clb_set_symbolic": ([V, I, I, V], I),
+        "clb_rate_capture": ([V, I, I], I),
         "clb_checkpoint_size": ([V, I, C.POINTER(C.c_size_t)], I),
         "clb_checkpoint_save": ([V, I, V, C.c_size_t], I),
         "clb_checkpoint_load": ([V, I, V, C.c_size_t], I),
@@ -102,7 +103,7 @@ def exported_symbols():
         "clb_ade_collide_stream", "clb_kinetics", "clb_mask_total", "clb_ca_sweep", "clb_age_update",
         "clb_update_solute_dynamics", "clb_update_solute_solid", "clb_precip_convert", "clb_dissol_convert",
         "clb_read_field", "clb_write_field", "clb_species_totals", "clb_field_stats", "clb_counters", "clb_kernel_time", "clb_set_timing_stride",
-        "clb_set_symbolic", "clb_checkpoint_size", "clb_checkpoint_save", "clb_checkpoint_load",
+        "clb_set_symbolic", "clb_rate_capture", "clb_checkpoint_size", "clb_checkpoint_save", "clb_checkpoint_load",
         "clb_stream_copy",
         "clb_rng_draws", "clb_geom_spheres", "clb_geom_pack", "clb_geom_generate",
     ]
@@ -301,6 +302,11 @@ class Case:
         (or, with None, clears) the abiotic law."""
         self._c(lib().clb_set_symbolic(self.ctx.h, self.id, int(microbe),
                                        C.byref(prog) if prog is not None else None))
+
+    def rate_capture(self, on=True):
+        """Record each following step's requested reaction increments / dt
+        (read as params.FIELD_RATE; clb_rate_capture)."""
+        self._c(lib().clb_rate_capture(self.ctx.h, self.id, 1 if on else 0))
 
     def checkpoint(self) -> bytes:
         """The case's state as one block (clb_checkpoint_save)."""

@@ -164,6 +164,13 @@ class Snapshots:
             norm = np.sqrt((u * u).sum(axis=0))
             self._put("nsLattice_", iteration, [("velocityNorm", norm, "Float32"), ("velocity", u, "Float32")])
 
+    def write_rates(self, case, iteration: int):
+        """rate_<species>_<iteration>.vti: the captured step's rate of every
+        species ("Rate", Float64, x = 1..nx-2; writeRateVTI,
+        complab_functions.hh:163-166)."""
+        for s, name in enumerate(self.species):
+            self._put("rate_" + name + "_", iteration, [("Rate", case.read(P.FIELD_RATE, s)[:, :, 1:-1], "Float64")])
+
 
 SUMMARY_HEADER = (
     "# CompLB3D run summary, one row per diagnostic interval\n"

@@ -80,12 +80,23 @@ def run(r: inputs.Run, ctx=None, case=None, out_dir: Optional[str] = None, steps
     def due(it, every):
         return every > 0 and it % every == 0
 
+    # rate snapshots (complab.cpp:2078-2105, 2548-2561, 3284): with a rate law
+    # on, the step after each concentration snapshot is captured and written
+    # as rate_<species>_<iT>.vti under the snapshot's number, and the last
+    # step under the final snapshot's
+    pr = r.params
+    rates = vtk_every > 0 and not pr.dissol_on and (
+        (pr.bio_law != 0 and pr.n_microbes > 0) or pr.n_rxn > 0 or len(r.symbolic) > 0)
+    capture_at = -1
+
     it = int(case.counters()[CHK_STEP]) if restart else 0
     first = it
     while True:
         if it < steps:
             if due(it, vtk_every):
                 snaps.write(case, it)
+                if rates:
+                    capture_at = it
             if r.diag_enabled and due(it, diag_every):
                 row = row_of(case, it, r.species, r.microbes)
                 msg = diag.record(row)
@@ -102,12 +113,28 @@ def run(r: inputs.Run, ctx=None, case=None, out_dir: Optional[str] = None, steps
         for every in (vtk_every, diag_every if r.diag_enabled else 0, chk_every):
             if every > 0:
                 nxt = min(nxt, (it // every + 1) * every)
+        last = rates and it == steps - 1
+        capture = capture_at == it or last
+        if capture:
+            nxt = it + 1
+            case.rate_capture(True)
+        elif rates and nxt == steps:
+            nxt = steps - 1                           # the last step runs on its own
         rc, n = case.run_steps(nxt - it)
+        if capture:
+            if n == 1 and capture_at == it:
+                snaps.write_rates(case, it)
+            if not (n == 1 and last):
+                case.rate_capture(False)
+            capture_at = -1
         it += n
         if rc:
             raise RuntimeError("the run stopped with status %d after %d steps" % (rc, it))
         say("step %d" % it)
     snaps.write(case, it)                           # the final output, after the last step
+    if rates and it == steps and it > first:
+        snaps.write_rates(case, it)
+        case.rate_capture(False)
     return Result(steps=it, out_dir=out, files=list(snaps.written), summary_csv=csv_path if csv else None,
                   verdict=diag.verdict(), flow_iters=iters, seconds=time.perf_counter() - t0,
                   checkpoints=checkpoints)

@@ -110,6 +110,9 @@ struct KinDev {
     int compiled;                  // the compiled biotic law is on (bio_law)
     // the abiotic symbolic law (device memory), null when there is none
     const clb_sym_program *symAb;
+    // rate capture (split kinetics kernel only): the step's requested
+    // increments of every species, before the clamp, / dt, dense [S][N]
+    double *rate;
     // thermodynamic gate of the biotic rates (clb_case_params thm_*)
     int thm, thmMicrobe;
     double thmRT, thmDG0, thmLoss, thmThr, thmChi, thmCmin, thmScale;
@@ -252,6 +255,7 @@ hipError_t launchFinalSum(const double *, long long, int, int, int, double *, hi
 hipError_t launchFinalSumLL(const long long *, long long, long long *, hipStream_t);
 template <typename R> hipError_t launchToDouble(const R *, double *, long long, hipStream_t);
 template <typename R> hipError_t launchNsDensity(const NsDev<R> &, double *dst, hipStream_t);
+hipError_t launchFillD(double *dst, size_t n, double v, hipStream_t);
 template <typename R> hipError_t launchFromDouble(const double *, R *, long long, hipStream_t);
 hipError_t launchI8ToDouble(const int8_t *, double *, long long, hipStream_t);
 hipError_t launchI32ToDouble(const int *, double *, long long, hipStream_t);

@@ -74,6 +74,7 @@ struct CaseBase {
     virtual void precipConvert(hipStream_t, int *changed) = 0;
     virtual Status dissolConvert(hipStream_t, int *reopened) = 0;
     virtual Status setSymbolic(int microbe, const clb_sym_program *prog) = 0;
+    virtual Status rateCapture(hipStream_t, bool on) = 0;
     // checkpoint / restart of the device-resident state
     virtual size_t checkpointBytes() = 0;
     virtual Status checkpointSave(hipStream_t, uint8_t *dst, size_t n) = 0;
@@ -158,6 +159,10 @@ struct Case : CaseBase {
     bool symOn = false;
     // the abiotic symbolic law, held after the microbes' programs
     bool symAb = false;
+    // rate capture: the split kinetics kernel writes each step's requested
+    // increments / dt here ([S][N], allocated at the first capture)
+    double *rateBuf = nullptr;
+    bool rateOn = false;
     size_t symSlots() const { return (size_t) std::max(p.n_microbes, 1) + 1; }
     clb_sym_program *symAbDev() const { return symDev + symSlots() - 1; }
     double gmaxClosed = -1e300;
@@ -234,6 +239,7 @@ struct Case : CaseBase {
         k.sym = symDev; k.symOn = symOn ? 1 : 0; k.symMask = symMask;
         k.compiled = k.bio;
         k.symAb = symAb ? symAbDev() : nullptr;
+        k.rate = rateOn ? rateBuf : nullptr;
         if (symOn && p.n_microbes > 0) k.bio = 1;
         k.thm = p.thm_on ? 1 : 0; k.thmMicrobe = p.thm_microbe;
         k.thmRT = p.thm_rt; k.thmDG0 = p.thm_dg0; k.thmLoss = p.thm_dgloss; k.thmThr = p.thm_threshold;
@@ -581,6 +587,16 @@ struct Case : CaseBase {
         return {CLB_OK, ""};
     }
 
+    Status rateCapture(hipStream_t st, bool on) override {
+        if (on && p.dissol_on) return {CLB_E_ARG, "rate capture is not combined with dissolution here"};
+        if (on && !rateBuf) {
+            rateBuf = dalloc<double>(allocs, (size_t) std::max(p.n_species, 1) * N);
+            HIPT(launchFillD(rateBuf, (size_t) std::max(p.n_species, 1) * N, 0.0, st));
+        }
+        rateOn = on;
+        return {CLB_OK, ""};
+    }
+
     // ------------------------------------------------------------ checkpoint
     // Layout: ChkHeader, the parameter block, the compact voxel list (checked,
     // not restored: a checkpoint loads into a case of the same geometry), then
@@ -827,11 +843,12 @@ struct Case : CaseBase {
     // with the gather at the waiting voxels (the reference's order,
     // complab.cpp:2461-2545).
     void transportLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr, bool maskTotal = false) {
-        if (kinOn && (symOn || symAb)) {
+        if (kinOn && (symOn || symAb || rateOn)) {
             // the symbolic laws run in the split kinetics kernel, biotic then
             // abiotic, on the transported state (the biomass changes there, so
             // the step's mask/total update follows it, not the transport)
             fusedLaunch(st, false, halt, false);
+            if (rateOn) HIPT(launchFillD(rateBuf, (size_t) std::max(p.n_species, 1) * N, 0.0, st));
             HIPT(launchKinetics<R>(ade(halt), kin(true), st));
             return;
         }
@@ -1137,7 +1154,7 @@ struct Case : CaseBase {
         // the transport kernels make the step's mask and total update and its
         // block partials (interior, then planes) when every open voxel passes
         // through them; the final reduction takes the step's decision
-        const bool fused = Sm > 0 && Nint > 0 && !symOn && !symAb;
+        const bool fused = Sm > 0 && Nint > 0 && !symOn && !symAb && !rateOn;
         const int nparts = (int) (nblk(Nint, 256) + nblk(Nopen - Nint, 256));
         for (int j = 0; j < len; ++j) {
             transportLaunch(st, true, &ctl->halt, fused);
@@ -1206,6 +1223,11 @@ struct Case : CaseBase {
             break;
         case CLB_FIELD_AGE: HIPT(launchI32ToDouble(age, stage, N, st)); break;
         case CLB_FIELD_PHASE: HIPT(launchI8ToDouble(phase, stage, N, st)); break;
+        case CLB_FIELD_RATE:
+            if (!rateOn) return {CLB_E_STATE, "rate capture is off"};
+            if (index < 0 || index >= p.n_species) return {CLB_E_ARG, "species index out of range"};
+            HIPT(hipMemcpyAsync(stage, rateBuf + (size_t) index * N, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+            break;
         case CLB_FIELD_NSRHO:
             if (!p.flow_on) return {CLB_E_STATE, "the case has no flow lattice"};
             HIPT(launchNsDensity<RN>(ns(), stage, st));
@@ -1780,6 +1802,14 @@ int clb_checkpoint_load(clb_ctx *ctx, int case_id, const void *src, size_t bytes
     CASE_OR_FAIL(c)
     if (!src) return fail(ctx, CLB_E_ARG, "null buffer");
     Status r = c->checkpointLoad(ctx->stream, (const uint8_t *) src, bytes);
+    if (r.code) return fail(ctx, r.code, r.msg);
+    return CLB_OK;
+    GUARD_END
+}
+int clb_rate_capture(clb_ctx *ctx, int case_id, int on) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    Status r = c->rateCapture(ctx->stream, on != 0);
     if (r.code) return fail(ctx, r.code, r.msg);
     return CLB_OK;
     GUARD_END

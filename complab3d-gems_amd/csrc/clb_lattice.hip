@@ -385,6 +385,12 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
                 *p = (R) ((double) *p + d);
             }
         };
+        // the step's requested increments, kept for the rate capture
+        // (accumulateRates, complab.cpp:2101-2105: the biotic and the abiotic
+        // increments before the clamp; the symbolic variant only)
+        double dReq[CLB_DEV_MAXS];
+#pragma unroll
+        for (int s = 0; s < CLB_DEV_MAXS; ++s) dReq[s] = 0.0;
         if (k.bio && k.M > 0) {
             double Bv[CLB_DEV_MAXM], br[CLB_DEV_MAXM];
             
@@ -426,6 +432,7 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
             for (int s = 0; s < CLB_DEV_MAXS; ++s) {
                 if (s >= S) continue;
                 const double d = sr[s];
+                if (SYM) dReq[s] = d;
                 if (d > 1e-12 || d < -1e-12) depositTo(s, clampInc(k.clamp, d, concOf(s), binds));
             }
 #pragma unroll
@@ -522,8 +529,13 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
             for (int s = 0; s < CLB_DEV_MAXS; ++s) {
                 if (s >= S) continue;
                 const double d = dAb[s];
+                if (SYM) dReq[s] += d;
                 if (d > 1e-12 || d < -1e-12) depositTo(s, clampInc(k.clamp, d, concOf(s), binds));
             }
+        }
+        if (SYM && k.rate) {
+#pragma unroll
+            for (int s = 0; s < CLB_DEV_MAXS; ++s) if (s < S) k.rate[(long long) s * N + v] = dReq[s] * (1.0 / k.dt);
         }
         if (binds) atomicAdd(a.clampCount, (unsigned long long) binds);
     }
@@ -1325,6 +1337,11 @@ __global__ void k_fill_d(double *dst, long long n, double v) {
     const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = v;
 }
+hipError_t launchFillD(double *dst, size_t n, double v, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_d, dim3((unsigned int) ((n + 255) / 256)), dim3(256), 0, st, dst, (long long) n, v);
+    return hipGetLastError();
+}
 template <typename R>
 __global__ void __launch_bounds__(256) k_ns_density(NsDev<R> a, double *dst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1518,7 +1535,7 @@ template <typename R>
 hipError_t launchKinetics(const AdeDev<R> &a, const KinDev &k, hipStream_t st) {
     if (a.Nint <= 0 || !k.on) return hipSuccess;
     if (k.dissol) kineticsVariants<R, true, false>(a, k, st);           // symbolic laws are refused with dissolution
-    else if ((k.symOn && k.bio) || k.symAb) kineticsVariants<R, false, true>(a, k, st);   // symbolic laws
+    else if ((k.symOn && k.bio) || k.symAb || k.rate) kineticsVariants<R, false, true>(a, k, st);   // symbolic laws, rate capture
     else kineticsVariants<R, false, false>(a, k, st);
     return hipGetLastError();
 }

@@ -294,6 +294,7 @@ int  clb_dissol_convert(clb_ctx *ctx, int case_id, int *reopened);
 #define CLB_FIELD_AGE      5  /* CA age                                         */
 #define CLB_FIELD_PHASE    6  /* solid phase id                                 */
 #define CLB_FIELD_NSRHO    7  /* flow density (1 at closed voxels)              */
+#define CLB_FIELD_RATE     8  /* species `index`'s rate of the last captured step (clb_rate_capture) */
 /* copies nx*ny*nz doubles (x fastest) of one field into dst */
 int  clb_read_field(clb_ctx *ctx, int case_id, int field, int index, double *dst);
 /* overwrites a species concentration field with equilibrium populations at
@@ -389,6 +390,15 @@ int  clb_set_symbolic(clb_ctx *ctx, int case_id, int microbe, const clb_sym_prog
 int  clb_checkpoint_size(clb_ctx *ctx, int case_id, size_t *bytes);
 int  clb_checkpoint_save(clb_ctx *ctx, int case_id, void *dst, size_t bytes);
 int  clb_checkpoint_load(clb_ctx *ctx, int case_id, const void *src, size_t bytes);
+
+/* ---- rate snapshots (rateField + accumulateRates, complab.cpp:2078-2105,
+ * written as rate_<species>_<iT>.vti at 2548-2561) -----------------------------
+ * With capture on, every following step records each species' requested
+ * reaction increment of the step -- the biotic and abiotic increments as the
+ * rate laws asked for them, before the clamp, equilibrium not counted -- over
+ * dt, readable as CLB_FIELD_RATE (0 where nothing reacted).  The steps run
+ * the split kinetics kernel while capture is on.  Refused with dissolution. */
+int  clb_rate_capture(clb_ctx *ctx, int case_id, int on);
 
 /* ---- profiling ------------------------------------------------------------- */
 /* average device time (ms) of the fused transport kernel of the interior

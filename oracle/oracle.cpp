@@ -107,6 +107,9 @@ struct Case {
     // the abiotic symbolic law (<symbolic><abiotic_file>), in use when symAbOn
     clb_sym_program symAb{};
     bool symAbOn = false;
+    // rate capture: the step's requested increments (accumulateRates), S*N
+    std::vector<double> rate;
+    bool rateOn = false;
 
     size_t id(int x, int y, int z) const { return (size_t) x + (size_t) nx * ((size_t) y + (size_t) ny * (size_t) z); }
     bool inside(int x, int y, int z) const { return x >= 0 && y >= 0 && z >= 0 && x < nx && y < ny && z < nz; }
@@ -438,6 +441,7 @@ void bioticStep(Case &c) {
             for (int m = 0; m < M; ++m) { double d = br[m]; if (d > THRD || d < -THRD) dB[(size_t) m * c.N + i] += d; }
         }
     }
+    if (c.rateOn) for (size_t k = 0; k < dC.size(); ++k) c.rate[k] += dC[k];     // complab.cpp:2450
 #pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 1; x < c.nx - 1; ++x) {
         size_t i = c.id(x, y, z);
@@ -604,6 +608,7 @@ void abioticStep(Case &c) {
         dissolutionPark(c, dC);
         dissolutionGather(c, dC);
     }
+    if (c.rateOn) for (size_t k = 0; k < dC.size(); ++k) c.rate[k] += dC[k];     // complab.cpp:2523
 #pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 1; x < c.nx - 1; ++x) {
         size_t i = c.id(x, y, z);
@@ -1360,6 +1365,7 @@ int nsResolve(Case &c, double permRatio) {
  * ------------------------------------------------------------------------------------ */
 int coupledStep(Case &c) {
     const clb_case_params &p = c.p;
+    if (c.rateOn) std::fill(c.rate.begin(), c.rate.end(), 0.0);
     c.adeClsCollide = c.adeCls;
     for (int s = 0; s < p.n_species; ++s) if (p.mobile[s]) adeCollide(c, s);
     if ((p.bio_law != CLB_BIO_NONE || symActive(c)) && p.n_microbes > 0) bioticStep(c);
@@ -1418,6 +1424,7 @@ int coupledStep(Case &c) {
 /* only transport + kinetics: collide, biotic, abiotic, stream */
 void transportStep(Case &c) {
     const clb_case_params &p = c.p;
+    if (c.rateOn) std::fill(c.rate.begin(), c.rate.end(), 0.0);
     for (int s = 0; s < p.n_species; ++s) if (p.mobile[s]) adeCollide(c, s);
     if ((p.bio_law != CLB_BIO_NONE || symActive(c)) && p.n_microbes > 0) bioticStep(c);
     if (p.n_rxn > 0 || p.dissol_on || c.symAbOn) abioticStep(c);
@@ -1562,6 +1569,10 @@ int orc_read_field(void *h, int field, int index, double *dst) {
     case CLB_FIELD_VEL: if (index < 0 || index > 2) return CLB_E_ARG; for (size_t i = 0; i < N; ++i) dst[i] = c.u[(size_t) index * N + i]; return 0;
     case CLB_FIELD_AGE: for (size_t i = 0; i < N; ++i) dst[i] = c.age[i]; return 0;
     case CLB_FIELD_PHASE: for (size_t i = 0; i < N; ++i) dst[i] = c.phase[i]; return 0;
+    case CLB_FIELD_RATE:                                                /* writeRateVTI(..., 1/dt) */
+        if (!c.rateOn || index < 0 || index >= c.p.n_species) return CLB_E_ARG;
+        for (size_t i = 0; i < N; ++i) dst[i] = c.rate[(size_t) index * N + i] * (1.0 / c.p.dt);
+        return 0;
     case CLB_FIELD_NSRHO:
         /* computeDensity: the post-stream populations of a BGK cell, the
          * imposed density of a pressure-plane cell, 1 at closed cells */
@@ -1752,6 +1763,15 @@ int orc_set_symbolic(void *h, int m, const clb_sym_program *prog) {
     if (c.sym.size() < (size_t) c.p.n_microbes) c.sym.resize((size_t) c.p.n_microbes);
     if (prog) { c.sym[(size_t) m] = *prog; c.symMask |= 1u << m; }
     else { memset(&c.sym[(size_t) m], 0, sizeof(clb_sym_program)); c.symMask &= ~(1u << m); }
+    return 0;
+}
+
+/* rate capture on or off (the increments of each following step, readable as
+ * CLB_FIELD_RATE) */
+int orc_rate_capture(void *h, int on) {
+    Case &c = *(Case *) h;
+    c.rateOn = on != 0;
+    if (c.rateOn) c.rate.assign((size_t) std::max(c.p.n_species, 1) * c.N, 0.0);
     return 0;
 }
 

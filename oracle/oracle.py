@@ -61,6 +61,7 @@ def lib():
             ("orc_rates", [P, P, P, C.c_int, P, P, P]),
             ("orc_thermo_factor", [P, P, P, P]),
             ("orc_set_symbolic", [P, C.c_int, P]),
+            ("orc_rate_capture", [P, C.c_int]),
             ("orc_sym_evaluate", [P, P, C.c_double, P]),
             ("orc_rng_draws", [C.c_uint64, C.c_int, P]),
             ("orc_set_threads", [C.c_int]),
@@ -123,6 +124,9 @@ class OracleCase:
         if rc < 0:
             raise RuntimeError("oracle error %d: %s" % (rc, lib().orc_last_error(self.h).decode()))
         return rc
+
+    def rate_capture(self, on=True):
+        self._chk(lib().orc_rate_capture(self.h, 1 if on else 0))
 
     def set_symbolic(self, microbe, prog):
         self._chk(lib().orc_set_symbolic(self.h, int(microbe), C.byref(prog) if prog is not None else None))

@@ -59,3 +59,43 @@ def test_runner_restart_from_a_checkpoint(ctx, tmp_path):
         b, _ = output.read_vti(os.path.join(again.out_dir, n))
         for k in a:
             assert np.array_equal(a[k], b[k]), (n, k)
+
+
+def test_rate_capture_matches_oracle(ctx):
+    """The captured rates of a biotic CA case with an abiotic reaction, on the
+    device against the oracle (1e-10), and the steps after capture is turned
+    off continuing on the fused path to the same fields."""
+    from complab_amd import configs, driver
+    from complab_amd import params as P
+    print("case: biotic 0.12-scale pack + S2 -> S3, rates captured over steps 31-35", flush=True)
+    w = configs.config2(scale=0.12)
+    w.transport.stabilize_steps = 20
+    w.params.ns_max_iter = 3000
+    p = w.params
+    p.n_rxn = 1
+    p.rxn_r1[0], p.rxn_r2[0], p.rxn_k[0] = 2, 0, 0.5
+    p.rxn_stoich[0][2], p.rxn_stoich[0][3] = -1.0, 1.0
+    g = Case(ctx, p, w.geometry)
+    o = O.OracleCase(p, w.geometry)
+    rg = driver.setup(g, p, w.flow, w.transport)
+    ro = driver.setup(o, p, w.flow, w.transport)
+    assert rg.iters == ro.iters
+    for c in (g, o):
+        c.set_omegas(ro.omega_pore, ro.omega_biofilm, 1.0)
+        assert c.run_steps(30) == (0, 30)
+        c.rate_capture(True)
+        assert c.run_steps(5) == (0, 5)
+    om = o.read(P.FIELD_MASK) >= 2
+    for s in range(4):
+        a, b = g.read(P.FIELD_RATE, s), o.read(P.FIELD_RATE, s)
+        assert np.max(np.abs(a - b)) <= 1e-10 * max(np.max(np.abs(b)), 1e-300), s
+        assert np.all(a[~om] == 0.0)
+    assert np.max(np.abs(o.read(P.FIELD_RATE, 3))) > 0
+    for c in (g, o):
+        c.rate_capture(False)
+        assert c.run_steps(5) == (0, 5)
+    assert np.array_equal(g.read(P.FIELD_MASK), o.read(P.FIELD_MASK))
+    for s in range(4):
+        a, b = g.read(P.FIELD_CONC, s)[om], o.read(P.FIELD_CONC, s)[om]
+        assert np.max(np.abs(a - b)) <= 1e-10 * max(np.max(np.abs(b)), 1e-300), s
+    g.close()

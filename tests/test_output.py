@@ -185,3 +185,35 @@ def test_runner_writes_the_reference_files(tmp_path):
     flat = [x for f in want.fields for x in (f.total, f.mean, f.minv, f.maxv, f.held)]
     assert list(got[3:]) == flat
     assert res.verdict == "PASS"                                        # B is closed at both planes: conserved
+
+
+def test_runner_writes_rate_snapshots(tmp_path):
+    """rate_<species>_<iT>.vti (writeRateVTI): the step after each
+    concentration snapshot and the last step, "Rate" = the requested
+    increment / dt.  For A -> B at k = 1e-3 the rate of A is -k times A at the
+    start of the step and the rate of B its negative (a sign, a species or a
+    step mixed up fails one of them)."""
+    r = inputs.load(write_run(str(tmp_path)))
+    p = r.params
+    p.n_rxn = 1
+    p.rxn_r1[0], p.rxn_r2[0], p.rxn_k[0] = 0, -1, 1e-3
+    p.rxn_stoich[0][0], p.rxn_stoich[0][1] = -1.0, 1.0
+    o = O.OracleCase(p, r.geometry)
+    res = runner.run(r, case=o)
+    names = sorted(os.listdir(res.out_dir))
+    for n in "AB":
+        for it in (0, 10, 20, 25):
+            assert "rate_%s_%07d.vti" % (n, it) in names
+    box = o.read(P.FIELD_MASK)[:, :, 1:-1] >= P.PORE
+    # the same run stopped before step 21, then that step captured
+    o2 = O.OracleCase(p, r.geometry)
+    runner.run(r, case=o2, out_dir=str(tmp_path / "o2"), steps=20)
+    a0 = o2.read(P.FIELD_CONC, 0)[:, :, 1:-1]
+    o2.rate_capture(True)
+    assert o2.run_steps(1) == (0, 1)
+    ra, _ = output.read_vti(os.path.join(res.out_dir, "rate_A_0000020.vti"))
+    rb, _ = output.read_vti(os.path.join(res.out_dir, "rate_B_0000020.vti"))
+    assert np.array_equal(ra["Rate"], o2.read(P.FIELD_RATE, 0)[:, :, 1:-1])
+    assert np.allclose(ra["Rate"][box], -1e-3 * a0[box] / p.dt, rtol=1e-12, atol=0)
+    assert np.array_equal(rb["Rate"], -ra["Rate"])
+    assert np.all(ra["Rate"][~box] == 0.0) and ra["Rate"][box].min() < 0
