@@ -132,8 +132,10 @@ class Snapshots:
     """Writes one snapshot of every field of a case into `out_dir`."""
 
     def __init__(self, out_dir: str, species: Sequence[str], microbes: Sequence[str] = (),
-                 mask_filename: str = "maskLattice", flow: bool = False, closed_density=(1.0, 0.0)):
+                 mask_filename: str = "maskLattice", flow: bool = False, closed_density=(1.0, 0.0),
+                 energy: Sequence = ()):
         self.out_dir = out_dir
+        self.energy = list(energy)                  # (microbe index, name) of the gated organisms
         self.species = list(species)
         self.microbes = list(microbes)
         self.mask_filename = mask_filename
@@ -163,6 +165,11 @@ class Snapshots:
             u[:, closed] = 0.0
             norm = np.sqrt((u * u).sum(axis=0))
             self._put("nsLattice_", iteration, [("velocityNorm", norm, "Float32"), ("velocity", u, "Float32")])
+        # the energy snapshot (writeEnergyFields, complab.cpp:2152-2163): dG and
+        # F_T of each gated organism at the standing concentrations
+        for m, name in self.energy:
+            self._put("dG_" + name + "_", iteration, [("DeltaG", case.read(P.FIELD_DG, m)[:, :, 1:-1], "Float64")])
+            self._put("FT_" + name + "_", iteration, [("F_T", case.read(P.FIELD_FT, m)[:, :, 1:-1], "Float64")])
 
     def write_rates(self, case, iteration: int):
         """rate_<species>_<iteration>.vti: the captured step's rate of every

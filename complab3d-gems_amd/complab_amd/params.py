@@ -24,7 +24,8 @@ CA_FRACTION, CA_HALF = 0, 1
 KIN_BIOTIC, KIN_ABIOTIC = 1, 2
 EQ_AAPCF, EQ_NEWTON = 0, 1
 
-FIELD_CONC, FIELD_BIOMASS, FIELD_BTOTAL, FIELD_MASK, FIELD_VEL, FIELD_AGE, FIELD_PHASE, FIELD_NSRHO, FIELD_RATE = range(9)
+(FIELD_CONC, FIELD_BIOMASS, FIELD_BTOTAL, FIELD_MASK, FIELD_VEL, FIELD_AGE, FIELD_PHASE, FIELD_NSRHO, FIELD_RATE,
+ FIELD_DG, FIELD_FT) = range(11)
 GEOM_SPHERES, GEOM_RANDOM, GEOM_CYLINDERS, GEOM_FRACTURE, GEOM_CHANNEL = range(5)
 
 _d = C.c_double

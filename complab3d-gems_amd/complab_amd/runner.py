@@ -41,6 +41,10 @@ def checkpoint_name(iteration: int) -> str:
     return "checkpoint_%07d.clbchk" % iteration
 
 
+def vtk_every_of(r) -> int:
+    return max(r.vtk_interval, 0)
+
+
 def run(r: inputs.Run, ctx=None, case=None, out_dir: Optional[str] = None, steps: Optional[int] = None,
         log=None, restart: Optional[str] = None) -> Result:
     """Runs `r` on a new device case (or on `case`, created from the same
@@ -66,7 +70,11 @@ def run(r: inputs.Run, ctx=None, case=None, out_dir: Optional[str] = None, steps
         say("set-up done: flow iterations %s, dt %.6g s" % (iters, rep.dt))
 
     flow = bool(r.params.flow_on) and r.flow.pe > 0
-    snaps = Snapshots(out, r.species, r.microbes, r.mask_filename, flow=flow)
+    pr = r.params
+    gated = []
+    if pr.thm_on and vtk_every_of(r) > 0:                      # gatedMicrobes, complab3d_energyfield.hh:190-197
+        gated = [(m, r.microbes[m]) for m in range(len(r.microbes)) if pr.thm_microbe < 0 or m == pr.thm_microbe]
+    snaps = Snapshots(out, r.species, r.microbes, r.mask_filename, flow=flow, energy=gated)
     diag = Diagnostics(tol=r.diag_tol, enabled=r.diag_enabled)
     for c in r.conserve:
         diag.add_conserve(c)

@@ -256,6 +256,7 @@ hipError_t launchFinalSumLL(const long long *, long long, long long *, hipStream
 template <typename R> hipError_t launchToDouble(const R *, double *, long long, hipStream_t);
 template <typename R> hipError_t launchNsDensity(const NsDev<R> &, double *dst, hipStream_t);
 hipError_t launchFillD(double *dst, size_t n, double v, hipStream_t);
+template <typename R> hipError_t launchEnergyField(const AdeDev<R> &, const KinDev &, double *dst, int which, hipStream_t);
 template <typename R> hipError_t launchFromDouble(const double *, R *, long long, hipStream_t);
 hipError_t launchI8ToDouble(const int8_t *, double *, long long, hipStream_t);
 hipError_t launchI32ToDouble(const int *, double *, long long, hipStream_t);

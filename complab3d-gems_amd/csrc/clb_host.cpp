@@ -1223,6 +1223,14 @@ struct Case : CaseBase {
             break;
         case CLB_FIELD_AGE: HIPT(launchI32ToDouble(age, stage, N, st)); break;
         case CLB_FIELD_PHASE: HIPT(launchI8ToDouble(phase, stage, N, st)); break;
+        case CLB_FIELD_DG:
+        case CLB_FIELD_FT:                                   // computeEnergyField3D, complab3d_energyfield.hh:88-150
+            if (!p.thm_on) return {CLB_E_STATE, "the case has no thermodynamic block"};
+            if (index < 0 || index >= p.n_microbes || (p.thm_microbe >= 0 && index != p.thm_microbe))
+                return {CLB_E_ARG, "that microbe has no thermodynamic block"};
+            HIPT(launchFillD(stage, (size_t) N, 0.0, st));
+            HIPT(launchEnergyField<R>(ade(), kin(true), stage, field == CLB_FIELD_FT ? 1 : 0, st));
+            break;
         case CLB_FIELD_RATE:
             if (!rateOn) return {CLB_E_STATE, "rate capture is off"};
             if (index < 0 || index >= p.n_species) return {CLB_E_ARG, "species index out of range"};

@@ -129,7 +129,7 @@ __device__ __forceinline__ void adeDirichlet(T fl[7], int atLeft, T rhoD) {
 // the thermodynamic factor F_T of complab3d_thermo.hh:215-255 (Jin & Bethke):
 // ln Q over the block's species with activities floored at cmin, dG, the
 // driving force past the ATP threshold, 0 at or past it, 1 where exp underflows
-__device__ double thermoGate(const KinDev &k, const double *C) {
+__device__ double thermoGate(const KinDev &k, const double *C, double *dGout = nullptr) {
     double lnQ = 0.0;
     for (int s = 0; s < k.S; ++s) {
         if (k.thmNu[s] == 0.0) continue;
@@ -138,6 +138,7 @@ __device__ double thermoGate(const KinDev &k, const double *C) {
         lnQ += k.thmNu[s] * log(a);
     }
     const double dG = k.thmDG0 + k.thmRT * lnQ + k.thmLoss;
+    if (dGout) *dGout = dG;
     const double x = (dG + k.thmThr) / ((k.thmChi > 0.0 ? k.thmChi : 1.0) * k.thmRT);
     if (x >= 0.0) return 0.0;
     if (x < -700.0) return 1.0;
@@ -911,6 +912,41 @@ __global__ void k_dissol_convert(AdeDev<R> a, int8_t *phase, int phaseId, int mi
 
 // the x planes: pull + regularized Dirichlet (or Neumann) completion + BGK;
 // density after the stream into a dense field (closed voxels keep 0)
+// the energy snapshot of one interior voxel (computeEnergyField3D,
+// complab3d_energyfield.hh:104-150): the gate's dG (which 0) or F_T (1) at the
+// standing densities (negative ones read 0); open interior voxels only
+template <typename R>
+__global__ void __launch_bounds__(256) k_energy_field(AdeDev<R> a, KinDev k, double *dst, int which) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.Nint) return;
+    const long long v = a.list[i];
+    const int m = a.mask[v];
+    if (m == 0 || m == 1) return;
+    double C[CLB_DEV_MAXS];
+    for (int s = 0; s < a.S; ++s) {
+        double r = 0;
+        if (!a.mobile[s]) r = (double) a.imm[(long long) a.slot[s] * a.N + v];
+        else if (a.reset) {
+            r = a.c0[s];
+            if (r < 1e-12 && r > -1e-12) r = 0;
+        } else if (!closedFor(a.cls, i, a.Nint, a.Nleft, a.lbt[s], a.rbt[s])) {
+            const int ms = a.slot[s];
+            double fl[7];
+            pull7<R, double>(a, a.cur + (long long) ms * 7 * a.Nopen, a.out + (long long) ms * 7 * a.Nopen, i, a.lbt[s], a.rbt[s], fl);
+            for (int q = 0; q < 7; ++q) r += fl[q];
+        }
+        C[s] = r > 0.0 ? r : 0.0;
+    }
+    double dG = 0.0;
+    const double F = thermoGate(k, C, &dG);
+    dst[v] = which ? F : dG;
+}
+template <typename R>
+hipError_t launchEnergyField(const AdeDev<R> &a, const KinDev &k, double *dst, int which, hipStream_t st) {
+    if (a.Nint > 0) hipLaunchKernelGGL(k_energy_field<R>, dim3((unsigned int) ((a.Nint + 255) / 256)), dim3(256), 0, st, a, k, dst, which);
+    return hipGetLastError();
+}
+
 template <typename R>
 __global__ void k_ade_density(AdeDev<R> a, int ms, double *dst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1640,6 +1676,7 @@ hipError_t launchGatherCompact(const double *src, const int *list, int n, R *dst
 #define INST(R) \
     template hipError_t launchAdeStep<R>(const AdeDev<R> &, const KinDev &, hipStream_t, hipEvent_t); \
     template hipError_t launchKinetics<R>(const AdeDev<R> &, const KinDev &, hipStream_t); \
+    template hipError_t launchEnergyField<R>(const AdeDev<R> &, const KinDev &, double *, int, hipStream_t); \
     template hipError_t launchKineticsListed<R>(const AdeDev<R> &, const KinDev &, const int *, const unsigned int *, hipStream_t); \
     template hipError_t launchNearPhaseList<R>(const AdeDev<R> &, const KinDev &, int *, unsigned int *, hipStream_t); \
     template hipError_t launchDissolPark<R>(const AdeDev<R> &, const KinDev &, hipStream_t); \

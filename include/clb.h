@@ -295,6 +295,9 @@ int  clb_dissol_convert(clb_ctx *ctx, int case_id, int *reopened);
 #define CLB_FIELD_PHASE    6  /* solid phase id                                 */
 #define CLB_FIELD_NSRHO    7  /* flow density (1 at closed voxels)              */
 #define CLB_FIELD_RATE     8  /* species `index`'s rate of the last captured step (clb_rate_capture) */
+#define CLB_FIELD_DG       9  /* the gate's dG (kJ/mol) of gated microbe `index` at the standing
+                                 concentrations, 0 off the open interior (dG_<microbe>_*.vti) */
+#define CLB_FIELD_FT      10  /* the gate's F_T of gated microbe `index`, likewise (FT_<microbe>_*.vti) */
 /* copies nx*ny*nz doubles (x fastest) of one field into dst */
 int  clb_read_field(clb_ctx *ctx, int case_id, int field, int index, double *dst);
 /* overwrites a species concentration field with equilibrium populations at

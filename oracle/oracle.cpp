@@ -1569,6 +1569,19 @@ int orc_read_field(void *h, int field, int index, double *dst) {
     case CLB_FIELD_VEL: if (index < 0 || index > 2) return CLB_E_ARG; for (size_t i = 0; i < N; ++i) dst[i] = c.u[(size_t) index * N + i]; return 0;
     case CLB_FIELD_AGE: for (size_t i = 0; i < N; ++i) dst[i] = c.age[i]; return 0;
     case CLB_FIELD_PHASE: for (size_t i = 0; i < N; ++i) dst[i] = c.phase[i]; return 0;
+    case CLB_FIELD_DG:
+    case CLB_FIELD_FT:                                                  /* computeEnergyField3D */
+        if (!c.p.thm_on || index < 0 || index >= c.p.n_microbes) return CLB_E_ARG;
+        for (size_t i = 0; i < N; ++i) {
+            dst[i] = 0.0;
+            const int x = (int) (i % (size_t) c.nx);
+            if (x == 0 || x == c.nx - 1 || c.solidOrBB(c.mask[i])) continue;
+            double C[CLB_MAX_SPECIES], dG = 0.0;
+            for (int s = 0; s < c.p.n_species; ++s) { const double v = conc(c, s, i); C[s] = v > 0.0 ? v : 0.0; }
+            const double F = thermoFactor(c.p, C, &dG);
+            dst[i] = field == CLB_FIELD_FT ? F : dG;
+        }
+        return 0;
     case CLB_FIELD_RATE:                                                /* writeRateVTI(..., 1/dt) */
         if (!c.rateOn || index < 0 || index >= c.p.n_species) return CLB_E_ARG;
         for (size_t i = 0; i < N; ++i) dst[i] = c.rate[(size_t) index * N + i] * (1.0 / c.p.dt);

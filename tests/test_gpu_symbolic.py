@@ -58,6 +58,11 @@ def test_symbolic_law_matches_oracle(ctx, tmp_path):
         assert np.max(np.abs(a - c)) <= 1e-10 * max(np.max(np.abs(c)), 1e-300), s
     cg, co = g.counters(), o.counters()
     assert cg[5] == co[5] and cg[6] == co[6]
+    # the energy snapshot of the gated organism at the standing concentrations
+    dg, do = g.read(P.FIELD_DG, 0), o.read(P.FIELD_DG, 0)
+    assert np.max(np.abs(dg - do)) <= 1e-10 * np.max(np.abs(do)) and np.max(np.abs(do)) > 0
+    fg, fo = g.read(P.FIELD_FT, 0), o.read(P.FIELD_FT, 0)
+    assert np.max(np.abs(fg - fo)) <= 1e-10 and 0.0 <= fo.min() and fo.max() <= 1.0
 
 
 def test_bad_programs_are_refused(ctx, tmp_path):

@@ -159,3 +159,37 @@ def test_reference_example_19_gate():
     assert p.thm_on == 1 and p.thm_rt == pytest.approx(RGAS * 277.15, rel=1e-15)
     assert (p.thm_dg0, p.thm_dgloss, p.thm_threshold, p.thm_chi, p.thm_cmin) == (-33.12, 25.7, 12.5, 1.0, 1e-9)
     assert list(p.thm_nu[:4]) == [-1, -1, 1, 1]
+
+
+def test_energy_snapshot_files(tmp_path):
+    """dG_<microbe>/FT_<microbe> (computeEnergyField3D): at every open voxel of
+    x = 1..nx-2 the gate's dG and F_T at the standing concentrations (negative
+    ones read 0), 0 elsewhere; the files carry them under "DeltaG" and "F_T"."""
+    from complab_amd import configs, output
+    w = configs.config2(scale=0.08)
+    p = w.params
+    p.thm_on, p.thm_microbe = 1, 0
+    p.thm_rt = RGAS * 298.15
+    p.thm_dg0, p.thm_threshold = -20.0, 12.5
+    for s, v in enumerate([-1.0, -1.0, 1.0, 1.0]):
+        p.thm_nu[s] = v
+    p.flow_on = 0
+    o = O.OracleCase(p, w.geometry)
+    o.set_omegas([1.2] * 4, [1.1] * 4, 1.0)
+    o.run_steps(20)
+    snaps = output.Snapshots(str(tmp_path), ["S%d" % s for s in range(4)], ["Bug"], energy=[(0, "Bug")])
+    snaps.write(o, 20)
+    g, _ = output.read_vti(str(tmp_path / "dG_Bug_0000020.vti"))
+    f, _ = output.read_vti(str(tmp_path / "FT_Bug_0000020.vti"))
+    mask = o.read(P.FIELD_MASK)[:, :, 1:-1]
+    conc = [o.read(P.FIELD_CONC, s)[:, :, 1:-1] for s in range(4)]
+    open_ = mask >= P.PORE
+    assert np.all(g["DeltaG"][~open_] == 0.0) and np.all(f["F_T"][~open_] == 0.0)
+    idx = np.argwhere(open_)
+    for k in np.linspace(0, len(idx) - 1, 25).astype(int):
+        z, y, x = idx[k]
+        c = [max(conc[s][z, y, x], 0.0) for s in range(4)]
+        F, dG = O.thermo_factor(p, c)
+        lnQ = sum(v * math.log(max(ci, p.thm_cmin)) for v, ci in zip([-1, -1, 1, 1], c))
+        assert dG == pytest.approx(p.thm_dg0 + p.thm_rt * lnQ, rel=1e-12)     # the equation, by hand
+        assert g["DeltaG"][z, y, x] == dG and f["F_T"][z, y, x] == F
