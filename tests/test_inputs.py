@@ -147,7 +147,8 @@ needs_ref = pytest.mark.skipif(not os.path.isdir(os.path.join(REF, "examples")),
 @needs_ref
 @pytest.mark.parametrize("ex", ["01_flow_only", "02_diffusion_only", "03_abiotic_kinetics", "04_equilibrium",
                                 "05_biotic_cellular_automaton", "13_precipitation", "14_dissolution",
-                                "15_precip_and_dissolution"])
+                                "15_precip_and_dissolution", "17_symbolic_law", "19_thermodynamic_gate",
+                                "20_upscaling"])
 def test_reference_examples_load(ex):
     r = inputs.load(os.path.join(REF, "examples", ex, "CompLaB.xml"))
     p = r.params
