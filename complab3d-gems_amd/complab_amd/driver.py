@@ -52,6 +52,40 @@ class SetupReport:
     dt: float = 0.0
     omega_pore: List[float] = field(default_factory=list)
     omega_biofilm: List[float] = field(default_factory=list)
+    tau_notes: List[str] = field(default_factory=list)   # the tau screen's warnings
+
+
+TAU_REJECT = 0.51     # complab.cpp:1318: below this the lattice rings
+TAU_WARN = 0.55       # complab.cpp:1319
+
+
+class TauError(ValueError):
+    """A relaxation time below TAU_REJECT: the reference terminates (complab.cpp:1358-1377)."""
+
+
+def tau_screen(p, omega_pore, omega_biofilm):
+    """The relaxation times every streaming species will use, checked before
+    the run as complab.cpp:1315-1378 does: the pore and biofilm tau of each
+    mobile species (an immobile one never streams).  Returns the warnings
+    (tau in [0.51, 0.55), or above 2); raises TauError below 0.51."""
+    low, marginal, high = [], [], []
+    for s in range(p.n_species):
+        if not p.mobile[s]:
+            continue
+        for where, om in (("pore", omega_pore[s]), ("biofilm", omega_biofilm[s])):
+            tau = 1.0 / om
+            label = "substrate%d in %s: tau = %.6g" % (s, where, tau)
+            if tau < TAU_REJECT:
+                low.append(label)
+            elif tau < TAU_WARN:
+                marginal.append(label)
+            elif tau > 2.0:
+                high.append(label)
+    if low:
+        raise TauError("a relaxation time below %.2f oscillates instead of diffusing: %s"
+                       % (TAU_REJECT, "; ".join(low)))
+    return (["above 2 (over-diffusive): " + m for m in high]
+            + ["close to 0.5 (may ring): " + m for m in marginal])
 
 
 def _solve(case, fs: FlowSetup, chunk=0, progress=None):
@@ -107,7 +141,8 @@ def solve_flow(case, fs: FlowSetup, rep: SetupReport, chunk=0, progress=None):
 
 
 def ade_relaxation(p, pore_mean_u, fs: FlowSetup, ts: TransportSetup, rep: SetupReport):
-    """Lattice viscosity, relaxation and time step per species (complab.cpp:1220-1252)."""
+    """Lattice viscosity, relaxation and time step per species (complab.cpp:1220-1252),
+    then the tau screen (complab.cpp:1315-1378)."""
     if fs.pe > 1e-12:
         ref_nu = pore_mean_u * fs.char_length / fs.pe
         ref_tau = ref_nu / CS2_ADE + 0.5
@@ -128,6 +163,7 @@ def ade_relaxation(p, pore_mean_u, fs: FlowSetup, ts: TransportSetup, rep: Setup
     rep.dt = dt
     rep.omega_pore = omp
     rep.omega_biofilm = omb
+    rep.tau_notes = tau_screen(p, omp, omb)
     return omp, omb, dt
 
 

@@ -243,24 +243,32 @@ def read_thm(path) -> ThmModel:
     return M
 
 
-def apply_thm(p, M: ThmModel, species: List[str], microbes: List[str]):
+def apply_thm(p, M: ThmModel, species: List[str], microbes: List[str], kinetics_in_use: bool = True):
     """Binds the model to the run (species and microbe names) and sets the
     parameter block's gate.  The compiled-kinetics path computes one rate
-    vector per voxel, so one reaction block for microbe0 or all of them
-    (complab3d_integration.hh:826-860)."""
+    vector per voxel, so while an organism takes its rates from it the file
+    holds one reaction block, for microbe0 or all of them
+    (complab3d_integration.hh:838-858).  A block naming one organism gates
+    that organism alone (specOfMicrobe, complab3d_thermo.hh:640-648); this
+    path carries one gate, so one block in every case."""
     if len(M.reactions) > 1:
-        raise InputError("the energetics file declares more than one reaction block; the kinetics path "
-                         "computes one combined rate vector per voxel and takes one gate")
+        if kinetics_in_use:
+            raise InputError("the energetics file declares more than one reaction block; the kinetics path "
+                             "computes one combined rate vector per voxel and takes one gate")
+        raise InputError("the energetics file declares more than one reaction block; this path carries one gate")
     r = M.reactions[0]
+    p.thm_microbe = -1
     if r.microbe not in ("", "all", "*"):
         idx = microbes.index(r.microbe) if r.microbe in microbes else (
             int(r.microbe) if r.microbe.isdigit() and int(r.microbe) < len(microbes) else -1)
         if idx < 0:
             raise InputError("reaction '%s' names microbe '%s', which is not in <name_of_microbes>"
                              % (r.name, r.microbe))
-        if idx != 0:
-            raise InputError("reaction '%s' is for an organism other than microbe0; the kinetics path "
-                             "gates only as microbe0 (write `microbe all`)" % r.name)
+        if idx != 0 and kinetics_in_use:
+            raise InputError("reaction '%s' is for an organism other than microbe0, but an organism takes its "
+                             "rates from the kinetics path, which gates only as microbe0 (write `microbe all`)"
+                             % r.name)
+        p.thm_microbe = idx
     for s in range(len(p.thm_nu)):
         p.thm_nu[s] = 0.0
     for name, v in zip(r.species, r.nu):
@@ -528,7 +536,8 @@ def load(xml_path, geometry_path: Optional[str] = None, precision=P.FP64, order=
         f = doc.text("thermodynamics", "energetics_file", default=None)
         if not f:
             raise InputError("<thermodynamics> is enabled but <energetics_file> is not set")
-        apply_thm(p, read_thm(f if os.path.isabs(f) else os.path.join(base, f)), names, microbes)
+        kinetics_in_use = biotic and any(rt == "kinetics" for rt in rtypes)
+        apply_thm(p, read_thm(f if os.path.isabs(f) else os.path.join(base, f)), names, microbes, kinetics_in_use)
 
     # ---- diagnostics (complab3d_integration.hh:375-423)
     run_diag = doc.flag("diagnostics", "enabled")
@@ -566,6 +575,20 @@ def load(xml_path, geometry_path: Optional[str] = None, precision=P.FP64, order=
         except SYM.SymError as e:
             raise InputError(str(e))
 
+    if symbolic:
+        # the symbolic processors skip fixed substrates and can read speciation
+        # totals (complab3d_processors_symbolic.hh:124, 167, from
+        # complab3d_metabolic.hh:829-859, 640-655); neither is carried here
+        fix = doc.text("chemistry", "fix_concentration", default=None)
+        if fix is not None and any(v.lower() in ("yes", "true", "1") for v in fix.split()):
+            raise InputError("<chemistry><fix_concentration> holds a substrate fixed; the symbolic rate "
+                             "path here does not hold substrates fixed")
+        basis = doc.text("simulation_mode", "fba_concentration_basis", default="free").strip().lower()
+        if basis == "total":
+            raise InputError("<fba_concentration_basis> total: the symbolic rate path here reads free "
+                             "concentrations only")
+        if basis != "free":
+            raise InputError("fba_concentration_basis \"%s\" is not understood. Use free or total" % basis)
     return Run(params=p, geometry=geo, flow=flow, transport=transport, steps=steps, species=names,
                microbes=microbes, dx=dx_m, diag_enabled=run_diag,
                diag_interval=doc.num("diagnostics", "interval", default=0, cast=int),

@@ -12,6 +12,7 @@ MAX_RXN = 4
 MAX_COMP = 8
 
 OK = 0
+CLOGGED = 1           # not an error: the run stopped, the pore space no longer percolates
 E_ARG, E_HIP, E_NOMEM, E_STATE, E_NONFINITE, E_CA_STALL, E_CA_SWEEPS, E_AGE_SWEEPS, E_GEOMETRY = (
     -1, -2, -3, -4, -5, -6, -7, -8, -9)
 

@@ -45,6 +45,17 @@ def vtk_every_of(r) -> int:
     return max(r.vtk_interval, 0)
 
 
+
+def gated_microbes(r):
+    """The organisms that have a reaction block of the .thm file, as
+    (index, name): one pair of dG_/FT_ snapshot files each
+    (gatedMicrobes, complab3d_energyfield.hh:190-197)."""
+    pr = r.params
+    if not pr.thm_on:
+        return []
+    return [(m, r.microbes[m]) for m in range(len(r.microbes)) if pr.thm_microbe < 0 or m == pr.thm_microbe]
+
+
 def run(r: inputs.Run, ctx=None, case=None, out_dir: Optional[str] = None, steps: Optional[int] = None,
         log=None, restart: Optional[str] = None) -> Result:
     """Runs `r` on a new device case (or on `case`, created from the same
@@ -71,9 +82,7 @@ def run(r: inputs.Run, ctx=None, case=None, out_dir: Optional[str] = None, steps
 
     flow = bool(r.params.flow_on) and r.flow.pe > 0
     pr = r.params
-    gated = []
-    if pr.thm_on and vtk_every_of(r) > 0:                      # gatedMicrobes, complab3d_energyfield.hh:190-197
-        gated = [(m, r.microbes[m]) for m in range(len(r.microbes)) if pr.thm_microbe < 0 or m == pr.thm_microbe]
+    gated = gated_microbes(r) if vtk_every_of(r) > 0 else []
     snaps = Snapshots(out, r.species, r.microbes, r.mask_filename, flow=flow, energy=gated)
     diag = Diagnostics(tol=r.diag_tol, enabled=r.diag_enabled)
     for c in r.conserve:

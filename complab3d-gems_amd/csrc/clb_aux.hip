@@ -355,6 +355,63 @@ __global__ void __launch_bounds__(256) k_ca_age_inplace(CaDev<R> c, int *age, co
     if ((threadIdx.x & 63) == 0 && b) atomicAdd(raised + k, (unsigned int) __popcll(b));
 }
 
+// The single in-place sweep of updateAgeDistance3D that follows a mask change
+// (complab.cpp:3079).  The processor walks x outermost and z innermost
+// (part2:862-867), so a voxel reads the ages its -x, -y and -z face neighbours
+// took earlier in the same sweep and the old ages of its +x, +y and +z
+// neighbours; the rule reads face neighbours only.  The sweep's result is
+// therefore the unique solution of new[v] = rule(old[v], new[-faces], old[+faces])
+// over the voxels holding biomass: triangular in the order x + y + z.  It is
+// solved by relaxation: every launch re-evaluates each such voxel from the
+// current `ageNew` of its -face neighbours (in place; each value a launch
+// reads is an earlier launch's or this one's), and a launch that changes
+// nothing stands on the solution -- the planes x + y + z = k are exact after
+// k + 1 launches at the latest, so it terminates.  `ageNew` starts as a copy
+// of `ageOld` (every other voxel keeps its age).
+template <typename R>
+__device__ __forceinline__ int ageRuleOrdered(const CaDev<R> &c, const int *ageOld, const int *ageNew, long long i) {
+    const int a = ageOld[i];
+    const double B = (double) c.Btot[i];
+    if (!(B > 1e-14)) return a;
+    int x, y, z;
+    coords(i, c.nx, c.ny, x, y, z);
+    long long del[6]; bool earlier[6]; int nd = 0;
+    for (int q = 1; q < 7; ++q) {
+        const int X = x + D7C[q][0], Y = y + D7C[q][1], Z = z + D7C[q][2];
+        if (X < 0 || Y < 0 || Z < 0 || X >= c.nx || Y >= c.ny || Z >= c.nz) continue;
+        const long long j = X + (long long) c.nx * (Y + (long long) c.ny * Z);
+        if (c.dist[j] > 0) { del[nd] = j; earlier[nd] = (D7C[q][0] + D7C[q][1] + D7C[q][2]) < 0; ++nd; }
+    }
+    if (a == 0) return 1;
+    if (a == 1 && (B - c.bmax) > -1e-14) {
+        for (int t = 0; t < nd; ++t) if ((earlier[t] ? ageNew[del[t]] : ageOld[del[t]]) == 0) return a;
+        return 2;
+    }
+    int cnt = 0;
+    for (int t = 0; t < nd; ++t) {
+        const int an = earlier[t] ? ageNew[del[t]] : ageOld[del[t]];
+        if (an >= a && ((double) c.Btot[del[t]] - c.bmax) > -1e-14) ++cnt;
+    }
+    return cnt == nd ? a + 1 : a;
+}
+// one relaxation launch over the voxels holding biomass; launch k counts the
+// voxels it changed in changed[k], and a launch queued after one that changed
+// nothing does nothing
+template <typename R>
+__global__ void __launch_bounds__(256) k_ca_age_ordered(CaDev<R> c, const int *ageOld, int *ageNew, const int *blist, int n,
+                                                        unsigned int *changed, int k) {
+    if (k > 0 && changed[k - 1] == 0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool ch = false;
+    if (i < n) {
+        const long long v = blist[i];
+        const int na = ageRuleOrdered(c, ageOld, ageNew, v);
+        if (na != ageNew[v]) { ageNew[v] = na; ch = true; }
+    }
+    const unsigned long long b = __ballot(ch);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(changed + k, (unsigned int) __popcll(b));
+}
+
 // ------------------------------------------------------------------------------------------
 // mask-driven switches (compact ADE classes from the dense live mask) and precipitation
 // ------------------------------------------------------------------------------------------
@@ -549,6 +606,12 @@ hipError_t launchCaAgeInplace(const CaDev<R> &c, int *age, const int *blist, int
     if (n > 0) hipLaunchKernelGGL(k_ca_age_inplace<R>, dim3(nblk(n, 256)), dim3(256), 0, st, c, age, blist, n, raised, k);
     return hipGetLastError();
 }
+template <typename R>
+hipError_t launchCaAgeOrdered(const CaDev<R> &c, const int *ageOld, int *ageNew, const int *blist, int n,
+                              unsigned int *changed, int k, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_ca_age_ordered<R>, dim3(nblk(n, 256)), dim3(256), 0, st, c, ageOld, ageNew, blist, n, changed, k);
+    return hipGetLastError();
+}
 hipError_t launchOmegaSwitch(const int8_t *mask, const int *list, uint8_t *cls, int Nopen, hipStream_t st) {
     hipLaunchKernelGGL(k_ade_omega_switch, dim3(nblk(Nopen, 256)), dim3(256), 0, st, mask, list, cls, Nopen);
     return hipGetLastError();
@@ -595,6 +658,15 @@ hipError_t launchI32ToDouble(const int *src, double *dst, long long n, hipStream
     hipLaunchKernelGGL(k_int_to_double<int>, dim3(nblk(n, 256)), dim3(256), 0, st, src, dst, n);
     return hipGetLastError();
 }
+// nearest integer of each value (an age field written by the caller)
+__global__ void k_double_to_i32(const double *src, int *dst, long long n) {
+    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (int) llround(src[i]);
+}
+hipError_t launchDoubleToI32(const double *src, int *dst, long long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_double_to_i32, dim3(nblk(n, 256)), dim3(256), 0, st, src, dst, n);
+    return hipGetLastError();
+}
 
 #define INST(R) \
     template hipError_t launchCaMaskTotal<R>(const CaDev<R> &, int, hipStream_t); \
@@ -603,6 +675,7 @@ hipError_t launchI32ToDouble(const int *src, double *dst, long long n, hipStream
     template hipError_t launchCaAge<R>(const CaDev<R> &, const int *, int *, long long *, hipStream_t); \
     template hipError_t launchAgeCompact<R>(const CaDev<R> &, int *, unsigned int *, unsigned int *, hipStream_t); \
     template hipError_t launchCaAgeInplace<R>(const CaDev<R> &, int *, const int *, int, unsigned int *, int, hipStream_t); \
+    template hipError_t launchCaAgeOrdered<R>(const CaDev<R> &, const int *, int *, const int *, int, unsigned int *, int, hipStream_t); \
     template hipError_t launchPrecip<R>(int, int, long long, int8_t *, const R *, double, int, int8_t *, unsigned int *, hipStream_t); \
     template hipError_t launchScalarStats<R>(int, int, long long, const R *, double *, hipStream_t); \
     template hipError_t launchToDouble<R>(const R *, double *, long long, hipStream_t); \

@@ -247,6 +247,8 @@ template <typename R> hipError_t launchAgeCompact(const CaDev<R> &, int *blist, 
 template <typename R> hipError_t launchCaAgeInplace(const CaDev<R> &, int *age, const int *blist, int n, unsigned int *raised,
                                                     int k, hipStream_t);
 template <typename R> hipError_t launchCaAge(const CaDev<R> &, const int *, int *, long long *, hipStream_t);
+template <typename R> hipError_t launchCaAgeOrdered(const CaDev<R> &, const int *ageOld, int *ageNew, const int *blist, int n,
+                                                    unsigned int *changed, int k, hipStream_t);
 hipError_t launchOmegaSwitch(const int8_t *, const int *, uint8_t *, int, hipStream_t);
 hipError_t launchSolidSwitch(const int8_t *, const int *, uint8_t *, uint8_t *, int, unsigned int *, hipStream_t);
 template <typename R> hipError_t launchPrecip(int, int, long long, int8_t *, const R *, double, int, int8_t *, unsigned int *, hipStream_t);
@@ -260,6 +262,7 @@ template <typename R> hipError_t launchEnergyField(const AdeDev<R> &, const KinD
 template <typename R> hipError_t launchFromDouble(const double *, R *, long long, hipStream_t);
 hipError_t launchI8ToDouble(const int8_t *, double *, long long, hipStream_t);
 hipError_t launchI32ToDouble(const int *, double *, long long, hipStream_t);
+hipError_t launchDoubleToI32(const double *, int *, long long, hipStream_t);
 hipError_t launchCopy16(const void *, void *, long long, hipStream_t);
 
 }  // namespace clb

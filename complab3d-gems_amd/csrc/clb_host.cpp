@@ -7,6 +7,7 @@
 // the mask-change trigger, the flow re-solve and its convergence).
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -38,6 +39,182 @@ struct Status {
     int code;
     std::string msg;
 };
+
+// ------------------------------------------------------------------ checks
+// the parameter block's own consistency (everything but the geometry)
+Status checkParamBlock(const clb_case_params *p) {
+    if (!p) return {CLB_E_ARG, "null parameters"};
+    if (p->nx < 3 || p->ny < 1 || p->nz < 1) return {CLB_E_ARG, "lattice must be at least 3 x 1 x 1"};
+    // voxel indices are 32-bit on the device and in the compact tables
+    if ((long long) p->nx * p->ny * p->nz > (long long) INT_MAX) return {CLB_E_ARG, "lattice holds more than 2^31 - 1 voxels"};
+    if (p->precision != CLB_FP64 && p->precision != CLB_FP32 && p->precision != CLB_FP32_ADE)
+        return {CLB_E_ARG, "unknown precision"};
+    if (p->n_species < 0 || p->n_species > CLB_MAX_SPECIES) return {CLB_E_ARG, "n_species out of range"};
+    if (p->n_microbes < 0 || p->n_microbes > CLB_MAX_MICROBES) return {CLB_E_ARG, "n_microbes out of range"};
+    if (p->n_rxn < 0 || p->n_rxn > CLB_MAX_RXN) return {CLB_E_ARG, "n_rxn out of range"};
+    for (int r = 0; r < p->n_rxn; ++r)
+        if (p->rxn_r1[r] < -1 || p->rxn_r1[r] >= p->n_species || p->rxn_r2[r] < -1 || p->rxn_r2[r] >= p->n_species)
+            return {CLB_E_ARG, "reaction reactant index out of range"};
+    if (p->eq_on && (p->eq_nc < 0 || p->eq_nc > CLB_MAX_COMP)) return {CLB_E_ARG, "eq_nc out of range"};
+    if (p->eq_on)
+        for (int j = 0; j < p->eq_nc; ++j)
+            if (p->eq_comp_species[j] < -1 || p->eq_comp_species[j] >= p->n_species)
+                return {CLB_E_ARG, "eq_comp_species index out of range"};
+    if (p->thm_on) {                                               // complab3d_thermo.hh:490-508
+        if (!(p->thm_chi > 0.0)) return {CLB_E_ARG, "thermodynamic gate: chi must be > 0"};
+        if (!(p->thm_cmin > 0.0)) return {CLB_E_ARG, "thermodynamic gate: cmin must be > 0, or ln Q can be -inf"};
+        if (!(p->thm_rt > 0.0)) return {CLB_E_ARG, "thermodynamic gate: R T must be > 0"};
+        if (!(p->thm_conc_scale > 0.0)) return {CLB_E_ARG, "thermodynamic gate: conc_scale must be > 0"};
+        if (p->thm_dgloss < 0.0) return {CLB_E_ARG, "thermodynamic gate: dGloss must be >= 0"};
+        if (p->thm_microbe < -1 || p->thm_microbe >= p->n_microbes) return {CLB_E_ARG, "thm_microbe out of range"};
+    }
+    if (p->eq_on && (p->eq_anderson_depth < 1 || p->eq_anderson_depth > 4)) return {CLB_E_ARG, "eq_anderson_depth must be 1..4"};
+    for (int s = 0; s < p->n_species; ++s) {
+        if (p->left_btype[s] < 0 || p->left_btype[s] > 2 || p->right_btype[s] < 0 || p->right_btype[s] > 2)
+            return {CLB_E_ARG, "boundary type must be 0, 1 or 2"};
+    }
+    if (p->precip_on && (p->precip_species < 0 || p->precip_species >= p->n_species || p->mobile[p->precip_species]))
+        return {CLB_E_ARG, "precip_species must be an immobile species"};
+    if (p->dissol_on) {
+        const int sM = p->dissol_species;
+        if (sM < 0 || sM >= p->n_species || p->mobile[sM]) return {CLB_E_ARG, "dissol_species must be an immobile species"};
+        if (p->dissol_phase_id < 1 || p->dissol_phase_id > 127) return {CLB_E_ARG, "dissol_phase_id must be 1..127"};
+        if (!(p->dissol_reopen_frac > 0.0) || p->dissol_reopen_frac >= 1.0)
+            return {CLB_E_ARG, "dissol_reopen_frac must lie in (0, 1)"};             // complab.cpp:539-543
+        if (p->dissol_interval < 1) return {CLB_E_ARG, "dissol_interval must be >= 1"};
+        if (p->dissol_acid < -1 || p->dissol_acid >= p->n_species) return {CLB_E_ARG, "dissol_acid out of range"};
+        if (p->dissol_is_precipitate) {
+            if (!p->precip_on || p->precip_phase_id != p->dissol_phase_id || p->precip_species != sM)
+                return {CLB_E_ARG, "a precipitate phase needs precipitation on, stamping dissol_phase_id into dissol_species"};
+        } else if (p->dissol_material < 0 || p->dissol_material >= CLB_PORE) {
+            return {CLB_E_ARG, "a grain phase needs a closed material number (0 or 1)"};
+        }
+        for (int s = 0; s < p->n_species; ++s)
+            if (!p->mobile[s] && s != sM && p->dissol_stoich[s] != 0.0)
+                return {CLB_E_ARG, "dissolution products must be mobile species"};
+    }
+    if (p->bio_law != CLB_BIO_NONE) {
+        if (p->donor >= p->n_species || p->acceptor >= p->n_species) return {CLB_E_ARG, "donor/acceptor index out of range"};
+    }
+    return {CLB_OK, ""};
+}
+
+// a symbolic law's program against the case it binds to: counts, bindings,
+// op codes, slots and the stack depth every op leaves, before it can run
+Status checkSymProgram(const clb_case_params &p, const clb_sym_program &P, bool abiotic) {
+    if (P.n_ops < 0 || P.n_ops > CLB_SYM_MAXOPS || P.n_vars < 0 || P.n_vars > CLB_SYM_MAXVARS ||
+        P.n_rates < 1 || P.n_rates > CLB_SYM_MAXRATES || P.n_const < 0 || P.n_const > CLB_SYM_MAXCONST)
+        return {CLB_E_ARG, "symbolic law: a count is out of range"};
+    if (!(P.unit_scale > 0.0) || !std::isfinite(P.unit_scale)) return {CLB_E_ARG, "symbolic law: unit_scale"};
+    for (int i = 0; i < P.n_vars; ++i) {
+        if (P.var_species[i] < -1 || P.var_species[i] >= p.n_species) return {CLB_E_ARG, "symbolic law: variable binding"};
+        if (abiotic && P.var_species[i] < 0) return {CLB_E_ARG, "the abiotic law binds no biomass"};
+    }
+    int pc = 0;
+    for (int r = 0; r < P.n_rates; ++r) {
+        if (P.rate_species[r] < -1 || P.rate_species[r] >= p.n_species) return {CLB_E_ARG, "symbolic law: rate binding"};
+        if (P.rate_end[r] < pc || P.rate_end[r] > P.n_ops) return {CLB_E_ARG, "symbolic law: rate code bounds"};
+        int sp = 0;
+        for (; pc < P.rate_end[r]; ++pc) {            // the stack each op leaves, checked before any run
+            const int op = P.op[pc], arg = P.arg[pc];
+            if (op == CLB_SYM_CONST) { if (arg < 0 || arg >= P.n_const) return {CLB_E_ARG, "symbolic law: constant index"}; ++sp; }
+            else if (op == CLB_SYM_VAR) { if (arg < 0 || arg >= P.n_vars + r) return {CLB_E_ARG, "symbolic law: value slot"}; ++sp; }
+            else if (op == CLB_SYM_NEG || (op >= CLB_SYM_EXP && op <= CLB_SYM_TANH)) { if (sp < 1) return {CLB_E_ARG, "symbolic law: stack"}; }
+            else if ((op >= CLB_SYM_ADD && op <= CLB_SYM_POW) || op == CLB_SYM_MIN || op == CLB_SYM_MAX) {
+                if (sp < 2) return {CLB_E_ARG, "symbolic law: stack"};
+                --sp;
+            } else return {CLB_E_ARG, "symbolic law: unknown op"};
+            if (sp > CLB_SYM_STACK) return {CLB_E_ARG, "symbolic law: stack deeper than the evaluator's"};
+        }
+        if (sp != 1) return {CLB_E_ARG, "symbolic law: a rate's code must leave one value"};
+    }
+    return {CLB_OK, ""};
+}
+
+// ------------------------------------------------------------------ checkpoint layout
+// A checkpoint block: ChkHeader, the parameter block, the compact voxel list
+// (checked, not restored: a checkpoint loads into a case of the same
+// geometry), then every persistent device field in chkSectionSizes() order.
+// Scratch buffers of a step (reduction partials, CA outbox, dissolution
+// shares, staging) are not part of it: every step rewrites them before reading.
+struct ChkHeader {
+    char magic[8];
+    int32_t abi, rBytes, rnBytes, nx, ny, nz, Nopen, Nint, Nleft, Sm, Si, M, flowOn, dissolOn;
+    uint8_t fCurIsA, gCurIsA, adeFirst, adeReset, adeClsDirty, nsFirst, nsClsDirty, pad0;
+    double nsOmega, nsBioOmega, deltaP, gmaxClosed;
+    int64_t iT, oldTotMask, msumClosed, launchNo;
+    int32_t nsSaturate, batchLen, clogged, resolveFresh;
+    uint64_t caSweep;
+    int64_t cnt[CLB_NCOUNTERS];
+    uint32_t symMask, symOn;
+};
+static const char kChkMagic[8] = {'C', 'L', 'B', 'C', 'H', 'K', '0', '2'};
+enum { CHK_SYM_SECTION = 20 };                     // index of the symbolic programs in the section list
+// the byte sizes of the sections, from the shape the header states
+std::vector<size_t> chkSectionSizes(const ChkHeader &h) {
+    std::vector<size_t> v;
+    const size_t r = (size_t) h.rBytes, rn = (size_t) h.rnBytes;
+    const size_t n = (size_t) h.nx * h.ny * h.nz, no = (size_t) h.Nopen, ni = (size_t) std::max(h.Nint, 1);
+    const size_t sm = (size_t) std::max(h.Sm, 1), si = (size_t) std::max(h.Si, 1), m = (size_t) std::max(h.M, 1);
+    const size_t np = (size_t) std::max(h.Nopen - h.Nint, 1);
+    v.push_back(n); v.push_back(n); v.push_back(n); v.push_back(n);          // mask geom dist phase
+    v.push_back(no); v.push_back(no); v.push_back(no); v.push_back(no);      // adeCls adeClsPull nsCls nsClsPull
+    v.push_back(no);                                                          // reopened
+    v.push_back(ni * sizeof(uint16_t));                                       // adeKinds
+    v.push_back(sm * 7 * no * r); v.push_back(sm * 7 * no * r);               // fA fB
+    v.push_back(si * n * r); v.push_back(m * n * r); v.push_back(n * r);      // imm B Btot
+    v.push_back(n * sizeof(int)); v.push_back(n * sizeof(int));               // age age2 (pointer order)
+    v.push_back(h.M > 0 ? n : 0);                                             // pushed
+    v.push_back(3 * no * r);                                                  // u
+    v.push_back(sm * np * sizeof(double));                                    // rhoN
+    v.push_back(((size_t) std::max(h.M, 1) + 1) * sizeof(clb_sym_program));   // symDev  (CHK_SYM_SECTION)
+    v.push_back(sizeof(unsigned long long));                                  // clampCount
+    v.push_back(3 * sizeof(unsigned long long));                              // dcount
+    v.push_back(sizeof(StepCtl));                                             // ctl
+    if (h.flowOn) {
+        v.push_back(19 * no * rn); v.push_back(19 * no * rn);                 // gA gB
+        v.push_back(ni * sizeof(unsigned long long));                         // nsKinds
+        v.push_back(sizeof(NsTracer));                                        // tracer
+        v.push_back(sizeof(int));                                             // nsDone
+    }
+    return v;
+}
+// a checkpoint block checked from its bytes alone: magic, the parameter
+// block, the shape the header states against it, the total size, and every
+// symbolic program it would upload
+Status checkCheckpointBlock(const uint8_t *src, size_t n, ChkHeader *hOut, clb_case_params *pOut) {
+    if (!src || n < sizeof(ChkHeader) + sizeof(clb_case_params)) return {CLB_E_ARG, "not a checkpoint (too short)"};
+    ChkHeader h;
+    memcpy(&h, src, sizeof(h));
+    if (memcmp(h.magic, kChkMagic, 8) != 0) return {CLB_E_ARG, "not a checkpoint of this library"};
+    clb_case_params q;
+    memcpy(&q, src + sizeof(h), sizeof(q));
+    Status s = checkParamBlock(&q);
+    if (s.code) return {s.code, "checkpoint parameter block: " + s.msg};
+    if (h.nx != q.nx || h.ny != q.ny || h.nz != q.nz || h.M != q.n_microbes || h.flowOn != q.flow_on ||
+        h.dissolOn != q.dissol_on || h.Nopen < 1 || h.Nint < 0 || h.Nleft < 0 || h.Nint + h.Nleft > h.Nopen ||
+        (long long) h.Nopen > (long long) h.nx * h.ny * h.nz || h.Sm < 0 || h.Si < 0 || h.Sm + h.Si != q.n_species ||
+        (h.rBytes != 4 && h.rBytes != 8) || (h.rnBytes != 4 && h.rnBytes != 8))
+        return {CLB_E_ARG, "checkpoint header does not match its parameter block"};
+    const std::vector<size_t> sz = chkSectionSizes(h);
+    size_t total = sizeof(ChkHeader) + sizeof(clb_case_params) + (size_t) h.Nopen * sizeof(int), symOff = 0;
+    for (size_t k = 0; k < sz.size(); ++k) { if (k == CHK_SYM_SECTION) symOff = total; total += sz[k]; }
+    if (n != total) return {CLB_E_ARG, "checkpoint size does not match the case"};
+    const int slots = std::max(q.n_microbes, 1) + 1;
+    for (int k = 0; k < slots; ++k) {
+        const bool abiotic = k == slots - 1;
+        const bool used = abiotic ? (h.symOn & 2u) != 0 : (k < q.n_microbes && ((h.symMask >> k) & 1u));
+        if (!used) continue;
+        clb_sym_program P;
+        memcpy(&P, src + symOff + (size_t) k * sizeof(clb_sym_program), sizeof(P));
+        Status t = checkSymProgram(q, P, abiotic);
+        if (t.code) return {t.code, "checkpoint " + t.msg};
+    }
+    if (h.symMask >> std::max(q.n_microbes, 0)) return {CLB_E_ARG, "checkpoint symbolic mask names a microbe the case lacks"};
+    if (hOut) *hOut = h;
+    if (pOut) *pOut = q;
+    return {CLB_OK, ""};
+}
 
 struct CaseBase {
     clb_case_params p;
@@ -127,6 +304,8 @@ struct Case : CaseBase {
     double nsOmega = 1, nsBioOmega = 1, deltaP = 0;
     long long iT = 0;
     int nsSaturate = 0;
+    int clogged = 0;                            // percolationFlag: the run stopped for good
+    bool resolveFresh = true;                   // the in-step re-solves' tracer is still empty
     long long oldTotMask = 0;
     uint64_t caSweep_ = 0;
     int64_t cnt[CLB_NCOUNTERS] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -546,33 +725,8 @@ struct Case : CaseBase {
         if (!abiotic && (microbe < 0 || microbe >= p.n_microbes)) return {CLB_E_ARG, "microbe index out of range"};
         if (prog) {
             if (p.dissol_on) return {CLB_E_ARG, "symbolic rate laws are not combined with dissolution here"};
-            const clb_sym_program &P = *prog;
-            if (P.n_ops < 0 || P.n_ops > CLB_SYM_MAXOPS || P.n_vars < 0 || P.n_vars > CLB_SYM_MAXVARS ||
-                P.n_rates < 1 || P.n_rates > CLB_SYM_MAXRATES || P.n_const < 0 || P.n_const > CLB_SYM_MAXCONST)
-                return {CLB_E_ARG, "symbolic law: a count is out of range"};
-            if (!(P.unit_scale > 0.0) || !std::isfinite(P.unit_scale)) return {CLB_E_ARG, "symbolic law: unit_scale"};
-            for (int i = 0; i < P.n_vars; ++i) {
-                if (P.var_species[i] < -1 || P.var_species[i] >= p.n_species) return {CLB_E_ARG, "symbolic law: variable binding"};
-                if (abiotic && P.var_species[i] < 0) return {CLB_E_ARG, "the abiotic law binds no biomass"};
-            }
-            int pc = 0;
-            for (int r = 0; r < P.n_rates; ++r) {
-                if (P.rate_species[r] < -1 || P.rate_species[r] >= p.n_species) return {CLB_E_ARG, "symbolic law: rate binding"};
-                if (P.rate_end[r] < pc || P.rate_end[r] > P.n_ops) return {CLB_E_ARG, "symbolic law: rate code bounds"};
-                int sp = 0;
-                for (; pc < P.rate_end[r]; ++pc) {            // the stack each op leaves, checked before any run
-                    const int op = P.op[pc], arg = P.arg[pc];
-                    if (op == CLB_SYM_CONST) { if (arg < 0 || arg >= P.n_const) return {CLB_E_ARG, "symbolic law: constant index"}; ++sp; }
-                    else if (op == CLB_SYM_VAR) { if (arg < 0 || arg >= P.n_vars + r) return {CLB_E_ARG, "symbolic law: value slot"}; ++sp; }
-                    else if (op == CLB_SYM_NEG || (op >= CLB_SYM_EXP && op <= CLB_SYM_TANH)) { if (sp < 1) return {CLB_E_ARG, "symbolic law: stack"}; }
-                    else if ((op >= CLB_SYM_ADD && op <= CLB_SYM_POW) || op == CLB_SYM_MIN || op == CLB_SYM_MAX) {
-                        if (sp < 2) return {CLB_E_ARG, "symbolic law: stack"};
-                        --sp;
-                    } else return {CLB_E_ARG, "symbolic law: unknown op"};
-                    if (sp > CLB_SYM_STACK) return {CLB_E_ARG, "symbolic law: stack deeper than the evaluator's"};
-                }
-                if (sp != 1) return {CLB_E_ARG, "symbolic law: a rate's code must leave one value"};
-            }
+            Status chk = checkSymProgram(p, *prog, abiotic);
+            if (chk.code) return chk;
             clb_sym_program *dst = abiotic ? symAbDev() : symDev + microbe;
             HIPT(hipMemcpy(dst, prog, sizeof(clb_sym_program), hipMemcpyHostToDevice));
             if (abiotic) symAb = true; else symMask |= 1u << microbe;
@@ -598,54 +752,21 @@ struct Case : CaseBase {
     }
 
     // ------------------------------------------------------------ checkpoint
-    // Layout: ChkHeader, the parameter block, the compact voxel list (checked,
-    // not restored: a checkpoint loads into a case of the same geometry), then
-    // every persistent device field in sections() order.  Scratch buffers of a
-    // step (reduction partials, CA outbox, dissolution shares, staging) are
-    // not part of it: every step rewrites them before reading.
-    struct ChkHeader {
-        char magic[8];
-        int32_t abi, rBytes, rnBytes, nx, ny, nz, Nopen, Nint, Nleft, Sm, Si, M, flowOn, dissolOn;
-        uint8_t fCurIsA, gCurIsA, adeFirst, adeReset, adeClsDirty, nsFirst, nsClsDirty, pad0;
-        double nsOmega, nsBioOmega, deltaP, gmaxClosed;
-        int64_t iT, oldTotMask, msumClosed, launchNo;
-        int32_t nsSaturate, batchLen;
-        uint64_t caSweep;
-        int64_t cnt[CLB_NCOUNTERS];
-        uint32_t symMask, symOn;
-    };
+    // the device pointers of the sections, sized by chkSectionSizes
     struct ChkSection { void *ptr; size_t bytes; };
     std::vector<ChkSection> chkSections() const {
+        void *ptrs[] = {mask, geom, dist, phase, adeCls, adeClsPull, nsCls, nsClsPull, reopened, adeKinds,
+                        fA, fB, imm, B, Btot, age, age2, pushed, u, rhoN, symDev, clampCount, dcount, ctl,
+                        gA, gB, nsKinds, tracer, nsDone};
+        const std::vector<size_t> sz = chkSectionSizes(chkHeader());
         std::vector<ChkSection> v;
-        const size_t r = sizeof(R), rn = sizeof(RN);
-        const size_t n = (size_t) N, no = (size_t) Nopen, ni = (size_t) std::max(Nint, 1);
-        const size_t sm = (size_t) std::max(Sm, 1), si = (size_t) std::max(Si, 1), m = (size_t) std::max(p.n_microbes, 1);
-        v.push_back({mask, n}); v.push_back({geom, n}); v.push_back({dist, n}); v.push_back({phase, n});
-        v.push_back({adeCls, no}); v.push_back({adeClsPull, no}); v.push_back({nsCls, no}); v.push_back({nsClsPull, no});
-        v.push_back({reopened, no});
-        v.push_back({adeKinds, ni * sizeof(uint16_t)});
-        v.push_back({fA, sm * 7 * no * r}); v.push_back({fB, sm * 7 * no * r});
-        v.push_back({imm, si * n * r}); v.push_back({B, m * n * r}); v.push_back({Btot, n * r});
-        v.push_back({age, n * sizeof(int)}); v.push_back({age2, n * sizeof(int)});   // the sweep's pair, in pointer order
-        if (pushed) v.push_back({pushed, n});
-        v.push_back({u, 3 * no * r});
-        v.push_back({rhoN, sm * (size_t) std::max(Nopen - Nint, 1) * sizeof(double)});
-        v.push_back({symDev, symSlots() * sizeof(clb_sym_program)});
-        v.push_back({clampCount, sizeof(unsigned long long)});
-        v.push_back({dcount, 3 * sizeof(unsigned long long)});
-        v.push_back({ctl, sizeof(StepCtl)});
-        if (p.flow_on) {
-            v.push_back({gA, 19 * no * rn}); v.push_back({gB, 19 * no * rn});
-            v.push_back({nsKinds, ni * sizeof(unsigned long long)});
-            v.push_back({tracer, sizeof(NsTracer)});
-            v.push_back({nsDone, sizeof(int)});
-        }
+        for (size_t k = 0; k < sz.size(); ++k) if (sz[k]) v.push_back({ptrs[k], sz[k]});
         return v;
     }
     ChkHeader chkHeader() const {
         ChkHeader h;
         memset(&h, 0, sizeof(h));
-        memcpy(h.magic, "CLBCHK01", 8);
+        memcpy(h.magic, kChkMagic, 8);
         h.abi = CLB_ABI_VERSION; h.rBytes = (int32_t) sizeof(R); h.rnBytes = (int32_t) sizeof(RN);
         h.nx = nx; h.ny = ny; h.nz = nz; h.Nopen = Nopen; h.Nint = Nint; h.Nleft = Nleft;
         h.Sm = Sm; h.Si = Si; h.M = p.n_microbes; h.flowOn = p.flow_on; h.dissolOn = p.dissol_on;
@@ -654,6 +775,7 @@ struct Case : CaseBase {
         h.nsOmega = nsOmega; h.nsBioOmega = nsBioOmega; h.deltaP = deltaP; h.gmaxClosed = gmaxClosed;
         h.iT = iT; h.oldTotMask = oldTotMask; h.msumClosed = msumClosed; h.launchNo = launchNo;
         h.nsSaturate = nsSaturate; h.batchLen = batchLen; h.caSweep = caSweep_;
+        h.clogged = clogged; h.resolveFresh = resolveFresh ? 1 : 0;
         for (int k = 0; k < CLB_NCOUNTERS; ++k) h.cnt[k] = cnt[k];
         h.symMask = symMask; h.symOn = (symOn ? 1u : 0u) | (symAb ? 2u : 0u);
         return h;
@@ -680,20 +802,20 @@ struct Case : CaseBase {
         return {CLB_OK, ""};
     }
     Status checkpointLoad(hipStream_t st, const uint8_t *src, size_t n) override {
-        if (n < sizeof(ChkHeader) + sizeof(clb_case_params)) return {CLB_E_ARG, "not a checkpoint (too short)"};
         ChkHeader h;
-        memcpy(&h, src, sizeof(h));
-        if (memcmp(h.magic, "CLBCHK01", 8) != 0) return {CLB_E_ARG, "not a checkpoint of this library"};
+        clb_case_params q;
+        Status chk = checkCheckpointBlock(src, n, &h, &q);      // nothing reaches the device unchecked
+        if (chk.code) return chk;
         const ChkHeader mine = chkHeader();
         if (h.abi != mine.abi || h.rBytes != mine.rBytes || h.rnBytes != mine.rnBytes)
             return {CLB_E_ARG, "checkpoint of another library version or precision"};
         if (h.nx != nx || h.ny != ny || h.nz != nz || h.Nopen != Nopen || h.Nint != Nint || h.Nleft != Nleft ||
             h.Sm != Sm || h.Si != Si || h.M != mine.M || h.flowOn != mine.flowOn || h.dissolOn != mine.dissolOn)
             return {CLB_E_ARG, "checkpoint of a case of another shape (lattice, species, microbes or processes)"};
+        for (int s2 = 0; s2 < p.n_species; ++s2)
+            if ((q.mobile[s2] != 0) != (p.mobile[s2] != 0)) return {CLB_E_ARG, "checkpoint of a case with other mobile species"};
         if (n != checkpointBytes()) return {CLB_E_ARG, "checkpoint size does not match the case"};
-        const uint8_t *r = src + sizeof(h);
-        clb_case_params q;
-        memcpy(&q, r, sizeof(q)); r += sizeof(q);
+        const uint8_t *r = src + sizeof(h) + sizeof(q);
         std::vector<int> lst(Nopen);
         HIPT(hipMemcpy(lst.data(), list, (size_t) Nopen * sizeof(int), hipMemcpyDeviceToHost));
         if (memcmp(lst.data(), r, (size_t) Nopen * sizeof(int)) != 0)
@@ -710,6 +832,7 @@ struct Case : CaseBase {
         nsOmega = h.nsOmega; nsBioOmega = h.nsBioOmega; deltaP = h.deltaP; gmaxClosed = h.gmaxClosed;
         iT = h.iT; oldTotMask = h.oldTotMask; msumClosed = h.msumClosed; launchNo = h.launchNo;
         nsSaturate = h.nsSaturate; batchLen = h.batchLen; caSweep_ = h.caSweep;
+        clogged = h.clogged; resolveFresh = h.resolveFresh != 0;
         for (int k = 0; k < CLB_NCOUNTERS; ++k) cnt[k] = h.cnt[k];
         kms = 0; klaunch = 0;
         nearDirty = true;
@@ -986,11 +1109,7 @@ struct Case : CaseBase {
     // of one batch are read back together: the first sweep that raised nothing
     // stands on the fixed point.  More than 1000 sweeps is an error.
     Status ageRelax(hipStream_t st, bool toFixedPoint) {
-        if (!toFixedPoint) {
-            HIPT(launchCaAge<R>(ca(), age, age2, partsLL, st));
-            std::swap(age, age2);
-            return {CLB_OK, ""};
-        }
+        if (!toFixedPoint) return ageOrderedSweep(st);
         const int K = 8;
         unsigned int *cntDev = ageCnt;
         int *blist = ageList;
@@ -1009,6 +1128,34 @@ struct Case : CaseBase {
                 ++swept;
                 if (raised[k] == 0) return {CLB_OK, ""};
                 if (swept > 1000) return {CLB_E_AGE_SWEEPS, "age relaxation did not settle"};
+            }
+        }
+    }
+    // the single in-place sweep after a mask change (complab.cpp:3079), solved
+    // as its triangular system over the voxels holding biomass (k_ca_age_ordered):
+    // relaxation launches queued eight at a time until one changes nothing;
+    // the planes x + y + z = k are exact after k + 1 launches, which bounds them
+    Status ageOrderedSweep(hipStream_t st) {
+        const int K = 8;
+        HIPT(launchAgeCompact<R>(ca(), ageList, reinterpret_cast<unsigned int *>(partsLL), ageCnt, st));
+        unsigned int n = 0;
+        HIPT(hipMemcpyAsync(&n, ageCnt, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        HIPT(hipStreamSynchronize(st));
+        if (n == 0) return {CLB_OK, ""};
+        HIPT(hipMemcpyAsync(age2, age, (size_t) N * sizeof(int), hipMemcpyDeviceToDevice, st));
+        unsigned int *chg = ageCnt + 16;
+        const long long cap = (long long) nx + ny + nz + 2;
+        long long launches = 0;
+        while (true) {
+            HIPT(hipMemsetAsync(chg, 0, K * sizeof(unsigned int), st));
+            for (int k = 0; k < K; ++k) HIPT(launchCaAgeOrdered<R>(ca(), age, age2, ageList, (int) n, chg, k, st));
+            unsigned int changed[K];
+            HIPT(hipMemcpyAsync(changed, chg, sizeof(changed), hipMemcpyDeviceToHost, st));
+            HIPT(hipStreamSynchronize(st));
+            for (int k = 0; k < K; ++k) {
+                ++launches;
+                if (changed[k] == 0) { std::swap(age, age2); return {CLB_OK, ""}; }
+                if (launches > cap) return {CLB_E_AGE_SWEEPS, "ordered age sweep did not settle"};
             }
         }
     }
@@ -1045,19 +1192,35 @@ struct Case : CaseBase {
     }
     long long lastMaskSum = 0;
 
-    Status resolve(hipStream_t st, double perm) {
+    // an in-step flow re-solve (complab.cpp:3087-3093, 3162-3168, 3221-3226)
+    // with the main loop's own tracer (ns_convg2, created empty before the
+    // loop and never reset, complab.cpp:2166), the re-solve tolerance and cap
+    Status resolveSolve(hipStream_t st, int *cv) {
+        int it = 0;
+        Status s = nsSolve(st, p.ns_max_iter, p.ns_conv_rule, p.ns_conv_eps, resolveFresh ? 1 : 0, &it, cv);
+        resolveFresh = false;
+        return s;
+    }
+    // re-solve after a mask change (maskPath, complab.cpp:3085-3130) or a
+    // precipitation seal (3159-3178).  A re-solve that runs out of iterations
+    // with no flow through the outlet plane x = nx-2 clogs the case
+    // (percolationFlag); so does one whose energy is not finite on the
+    // mask-change path, where the velocity is then not coupled (3119-3121)
+    Status resolve(hipStream_t st, double perm, bool maskPath) {
         nsUpdateDynamics(st, perm);
-        int it = 0, cv = 0;
-        Status s = nsSolve(st, p.ns_max_iter, p.ns_conv_rule, p.ns_conv_eps, 0, &it, &cv);
+        int cv = 0;
+        Status s = resolveSolve(st, &cv);
+        if (s.code == CLB_E_NONFINITE && maskPath) { cnt[4]++; clogged = 1; return {CLB_OK, ""}; }
         if (s.code) return s;
         cnt[4]++;
         if (!cv) {
             nsSaturate = 1;
             double stt[4];
             nsStats(st, stt);
-            if (stt[2] > 1e-12) nsSaturate = 0;
+            if (stt[2] > 1e-12) nsSaturate = 0;            // thrd, complab3d_processors.hh:60
+            else clogged = 1;
         }
-        nsCouple(st);
+        if (!(maskPath && clogged)) nsCouple(st);
         return {CLB_OK, ""};
     }
 
@@ -1082,7 +1245,7 @@ struct Case : CaseBase {
                 if (a.code) return a;
                 if (p.ade_update_interval > 0 && iT % p.ade_update_interval == 0) HIPT(launchOmegaSwitch(mask, list, adeCls, Nopen, st));
                 if (p.ns_update_interval > 0 && iT % p.ns_update_interval == 0 && p.flow_on && !nsSaturate) {
-                    Status r = resolve(st, p.perm_ratio);
+                    Status r = resolve(st, p.perm_ratio, true);
                     if (r.code) return r;
                 }
             }
@@ -1096,7 +1259,7 @@ struct Case : CaseBase {
             if (ch) {
                 cnt[5]++;
                 nearDirty = true;
-                if (p.flow_on && !nsSaturate) { Status r = resolve(st, p.precip_perm_ratio); if (r.code) return r; }
+                if (p.flow_on && !nsSaturate) { Status r = resolve(st, p.precip_perm_ratio, false); if (r.code) return r; }
                 unsigned int nre = 0;
                 HIPT(hipMemsetAsync(ucnt + 1, 0, sizeof(unsigned int), st));
                 HIPT(launchSolidSwitch(mask, list, adeCls, reopened, Nopen, ucnt + 1, st));
@@ -1116,16 +1279,18 @@ struct Case : CaseBase {
                 if (p.flow_on) {
                     nsUpdateDynamics(st, p.perm_ratio);
                     nsSaturate = 0;
-                    int it = 0, cv = 0;
-                    Status r = nsSolve(st, p.ns_max_iter, p.ns_conv_rule, p.ns_conv_eps, 0, &it, &cv);
+                    int cv = 0;
+                    Status r = resolveSolve(st, &cv);
                     if (r.code) return r;
                     cnt[4]++;
                     nsCouple(st);
+                    clogged = 0;                                 // reopening can undo a clog (complab.cpp:3234)
                 }
                 soluteSolid(st, nullptr);
             }
         }
         ++iT; cnt[7]++;
+        if (clogged) return {CLB_CLOGGED, "the pore space no longer percolates"};   // complab.cpp:3261
         return {CLB_OK, ""};
     }
 
@@ -1175,6 +1340,7 @@ struct Case : CaseBase {
         batchLen = halt ? std::max(1, fin + 1) : std::min(kBatchMax, 2 * batchLen);
         if (halt) {
             Status s = stepTail(st, ctlHost->gmax, ctlHost->msum);
+            if (s.code == CLB_CLOGGED) *finished = fin + 1;      // the step ran to its end
             if (s.code) return s;
             *finished = fin + 1;
         }
@@ -1184,6 +1350,7 @@ struct Case : CaseBase {
     Status runSteps(hipStream_t st, int n, int *done) override {
         int k = 0;
         Status s{CLB_OK, ""};
+        if (clogged) { if (done) *done = 0; return {CLB_CLOGGED, "the pore space no longer percolates"}; }
         if (p.n_microbes > 0) {
             while (k < n) {
                 int fin = 0;
@@ -1192,7 +1359,7 @@ struct Case : CaseBase {
                 if (s.code) break;
             }
         } else {
-            for (; k < n; ++k) { s = coupledStep(st); if (s.code) break; }
+            for (; k < n; ++k) { s = coupledStep(st); if (s.code) { if (s.code == CLB_CLOGGED) ++k; break; } }
         }
         flushTimes(st);
         if (done) *done = k;
@@ -1269,21 +1436,27 @@ struct Case : CaseBase {
         } else if (field == CLB_FIELD_VEL) {
             if (index < 0 || index > 2) return {CLB_E_ARG, "velocity component out of range"};
             HIPT(launchGatherCompact<R>(stage, list, Nopen, u + (size_t) index * Nopen, st));
+        } else if (field == CLB_FIELD_AGE) {
+            HIPT(launchDoubleToI32(stage, age, N, st));
+            HIPT(hipMemcpyAsync(age2, age, (size_t) N * sizeof(int), hipMemcpyDeviceToDevice, st));
         } else return {CLB_E_ARG, "field cannot be written"};
         HIPT(hipStreamSynchronize(st));
         return {CLB_OK, ""};
     }
-    void speciesTotals(hipStream_t st, double *out) override {
+    void speciesTotals(hipStream_t st, double *out) override { speciesTotalsWith(st, parts, scal, out); }
+    // the reductions write every block partial before the final sums read
+    // them, so neither buffer needs clearing
+    void speciesTotalsWith(hipStream_t st, double *partBuf, double *sums, double *out) {
         for (int s = 0; s < p.n_species; ++s) {
             if (p.mobile[s]) {
-                HIPT(launchAdeTotal<R>(ade(), slot[s], parts, st));
-                HIPT(launchFinalSum(parts, nblk(Nint, 256), 1, 0, 0, scal + s, st));
+                HIPT(launchAdeTotal<R>(ade(), slot[s], partBuf, st));
+                HIPT(launchFinalSum(partBuf, nblk(Nint, 256), 1, 0, 0, sums + s, st));
             } else {
-                HIPT(launchScalarStats<R>(nx, ny, N, imm + (size_t) slot[s] * N, parts, st));
-                HIPT(launchFinalSum(parts, nblk(N, 256), 4, 0, 0, scal + s, st));
+                HIPT(launchScalarStats<R>(nx, ny, N, imm + (size_t) slot[s] * N, partBuf, st));
+                HIPT(launchFinalSum(partBuf, nblk(N, 256), 4, 0, 0, sums + s, st));
             }
         }
-        HIPT(hipMemcpyAsync(out, scal, p.n_species * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPT(hipMemcpyAsync(out, sums, p.n_species * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPT(hipStreamSynchronize(st));
     }
     // per species, then per microbe: total, minimum, maximum, held
@@ -1382,48 +1555,8 @@ CaseBase *getCase(clb_ctx *ctx, int id) {
 
 int checkParams(clb_ctx *ctx, const clb_case_params *p, const int8_t *geometry) {
     if (!p || !geometry) return fail(ctx, CLB_E_ARG, "null parameters or geometry");
-    if (p->nx < 3 || p->ny < 1 || p->nz < 1) return fail(ctx, CLB_E_ARG, "lattice must be at least 3 x 1 x 1");
-    if (p->precision != CLB_FP64 && p->precision != CLB_FP32 && p->precision != CLB_FP32_ADE)
-        return fail(ctx, CLB_E_ARG, "unknown precision");
-    if (p->n_species < 0 || p->n_species > CLB_MAX_SPECIES) return fail(ctx, CLB_E_ARG, "n_species out of range");
-    if (p->n_microbes < 0 || p->n_microbes > CLB_MAX_MICROBES) return fail(ctx, CLB_E_ARG, "n_microbes out of range");
-    if (p->n_rxn < 0 || p->n_rxn > CLB_MAX_RXN) return fail(ctx, CLB_E_ARG, "n_rxn out of range");
-    if (p->eq_on && (p->eq_nc < 0 || p->eq_nc > CLB_MAX_COMP)) return fail(ctx, CLB_E_ARG, "eq_nc out of range");
-    if (p->thm_on) {                                               // complab3d_thermo.hh:490-508
-        if (!(p->thm_chi > 0.0)) return fail(ctx, CLB_E_ARG, "thermodynamic gate: chi must be > 0");
-        if (!(p->thm_cmin > 0.0)) return fail(ctx, CLB_E_ARG, "thermodynamic gate: cmin must be > 0, or ln Q can be -inf");
-        if (!(p->thm_rt > 0.0)) return fail(ctx, CLB_E_ARG, "thermodynamic gate: R T must be > 0");
-        if (!(p->thm_conc_scale > 0.0)) return fail(ctx, CLB_E_ARG, "thermodynamic gate: conc_scale must be > 0");
-        if (p->thm_dgloss < 0.0) return fail(ctx, CLB_E_ARG, "thermodynamic gate: dGloss must be >= 0");
-    }
-    if (p->eq_on && (p->eq_anderson_depth < 1 || p->eq_anderson_depth > 4)) return fail(ctx, CLB_E_ARG, "eq_anderson_depth must be 1..4");
-    for (int s = 0; s < p->n_species; ++s) {
-        if (p->left_btype[s] < 0 || p->left_btype[s] > 2 || p->right_btype[s] < 0 || p->right_btype[s] > 2)
-            return fail(ctx, CLB_E_ARG, "boundary type must be 0, 1 or 2");
-    }
-    if (p->precip_on && (p->precip_species < 0 || p->precip_species >= p->n_species || p->mobile[p->precip_species]))
-        return fail(ctx, CLB_E_ARG, "precip_species must be an immobile species");
-    if (p->dissol_on) {
-        const int sM = p->dissol_species;
-        if (sM < 0 || sM >= p->n_species || p->mobile[sM]) return fail(ctx, CLB_E_ARG, "dissol_species must be an immobile species");
-        if (p->dissol_phase_id < 1 || p->dissol_phase_id > 127) return fail(ctx, CLB_E_ARG, "dissol_phase_id must be 1..127");
-        if (!(p->dissol_reopen_frac > 0.0) || p->dissol_reopen_frac >= 1.0)
-            return fail(ctx, CLB_E_ARG, "dissol_reopen_frac must lie in (0, 1)");             // complab.cpp:539-543
-        if (p->dissol_interval < 1) return fail(ctx, CLB_E_ARG, "dissol_interval must be >= 1");
-        if (p->dissol_acid < -1 || p->dissol_acid >= p->n_species) return fail(ctx, CLB_E_ARG, "dissol_acid out of range");
-        if (p->dissol_is_precipitate) {
-            if (!p->precip_on || p->precip_phase_id != p->dissol_phase_id || p->precip_species != sM)
-                return fail(ctx, CLB_E_ARG, "a precipitate phase needs precipitation on, stamping dissol_phase_id into dissol_species");
-        } else if (p->dissol_material < 0 || p->dissol_material >= CLB_PORE) {
-            return fail(ctx, CLB_E_ARG, "a grain phase needs a closed material number (0 or 1)");
-        }
-        for (int s = 0; s < p->n_species; ++s)
-            if (!p->mobile[s] && s != sM && p->dissol_stoich[s] != 0.0)
-                return fail(ctx, CLB_E_ARG, "dissolution products must be mobile species");
-    }
-    if (p->bio_law != CLB_BIO_NONE) {
-        if (p->donor >= p->n_species || p->acceptor >= p->n_species) return fail(ctx, CLB_E_ARG, "donor/acceptor index out of range");
-    }
+    Status blk = checkParamBlock(p);
+    if (blk.code) return fail(ctx, blk.code, blk.msg);
     const long long N = (long long) p->nx * p->ny * p->nz;
     // a no-dynamics grain voxel must not touch open water through a face
     long long open = 0;

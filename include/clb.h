@@ -38,6 +38,10 @@ extern "C" {
 
 /* status codes */
 #define CLB_OK              0
+#define CLB_CLOGGED         1   /* not an error: clb_run_steps stopped after a step whose flow
+                                   re-solve ran out of iterations with no outlet flow, or
+                                   diverged (percolationFlag, complab.cpp:3113, 3119, 3176,
+                                   3261); the case refuses further steps                     */
 #define CLB_E_ARG          -1   /* bad argument / inconsistent parameters    */
 #define CLB_E_HIP          -2   /* HIP runtime failure                       */
 #define CLB_E_NOMEM        -3   /* device allocation failed                  */
@@ -301,7 +305,8 @@ int  clb_dissol_convert(clb_ctx *ctx, int case_id, int *reopened);
 /* copies nx*ny*nz doubles (x fastest) of one field into dst */
 int  clb_read_field(clb_ctx *ctx, int case_id, int field, int index, double *dst);
 /* overwrites a species concentration field with equilibrium populations at
- * rest (or a biomass field); for tests and restarts */
+ * rest, a biomass field (the total follows), a velocity component, or the CA
+ * age field (rounded to integers); for tests and restarts */
 int  clb_write_field(clb_ctx *ctx, int case_id, int field, int index, const double *src);
 
 /* ---- reductions (PopulationSum3D, MaskedScalarCounts3D, computeSum) --------- */

@@ -44,6 +44,7 @@ struct TransportSetup {
 struct Report {
     double delta_p = 0.0, pore_mean_u = 0.0, max_ux = 0.0, outlet_ux = 0.0;
     double permeability = 0.0, pe_achieved = 0.0, ref_tau = 0.0, dt = 0.0;
+    int tau_marginal = 0, tau_high = 0;      /* the tau screen's warnings */
     std::vector<int> iters;
     std::vector<int> converged;
     std::vector<double> omega_pore, omega_biofilm;
@@ -82,9 +83,30 @@ inline int solveFlow(clb_ctx *ctx, int id, const FlowSetup &fs, Report &rep, dou
     return CLB_OK;
 }
 
+/* the tau screen of complab.cpp:1315-1378 over the relaxation rates of the
+ * streaming (mobile) species, pore and biofilm: CLB_E_ARG below 0.51 (the
+ * reference terminates); warnings for [0.51, 0.55) and above 2 counted in
+ * rep.tau_marginal / rep.tau_high */
+inline int tauScreen(int nSpecies, const int32_t *mobile, Report &rep) {
+    rep.tau_marginal = rep.tau_high = 0;
+    int low = 0;
+    for (int s = 0; s < nSpecies; ++s) {
+        if (mobile && !mobile[s]) continue;
+        const double taus[2] = {1.0 / rep.omega_pore[s], 1.0 / rep.omega_biofilm[s]};
+        for (double tau : taus) {
+            if (tau < 0.51) ++low;
+            else if (tau < 0.55) ++rep.tau_marginal;
+            else if (tau > 2.0) ++rep.tau_high;
+        }
+    }
+    return low ? CLB_E_ARG : CLB_OK;
+}
+
 /* lattice viscosity, relaxation and time step per species
- * (complab.cpp:1220-1252); CLB_E_ARG where the reference stops */
-inline int adeRelaxation(int nSpecies, double poreMeanU, const FlowSetup &fs, const TransportSetup &ts, Report &rep) {
+ * (complab.cpp:1220-1252) and the tau screen; CLB_E_ARG where the reference
+ * stops.  mobile: n_species flags (NULL: every species streams) */
+inline int adeRelaxation(int nSpecies, double poreMeanU, const FlowSetup &fs, const TransportSetup &ts, Report &rep,
+                         const int32_t *mobile = nullptr) {
     double refTau, refNu;
     if (fs.pe > 1e-12) {
         refNu = poreMeanU * fs.char_length / fs.pe;
@@ -105,7 +127,7 @@ inline int adeRelaxation(int nSpecies, double poreMeanU, const FlowSetup &fs, co
         rep.omega_pore[s] = s > 0 ? 1.0 / (nuS / CS2_ADE + 0.5) : 1.0 / refTau;
         rep.omega_biofilm[s] = 1.0 / (refNu * ts.d_biofilm[s] / ts.d_pore[0] / CS2_ADE + 0.5);
     }
-    return CLB_OK;
+    return tauScreen(nSpecies, mobile, rep);
 }
 
 /* the whole set-up on a created case */
@@ -114,7 +136,7 @@ inline int setup(clb_ctx *ctx, int id, const clb_case_params &p, const FlowSetup
     int rc;
     const bool flow = fs.pe > 0 && p.flow_on;
     if (flow && (rc = solveFlow(ctx, id, fs, rep, &poreMean))) return rc;
-    if ((rc = adeRelaxation(p.n_species, poreMean, fs, ts, rep))) return rc;
+    if ((rc = adeRelaxation(p.n_species, poreMean, fs, ts, rep, p.mobile))) return rc;
     if ((rc = clb_set_ade_relaxation(ctx, id, rep.omega_pore.data(), rep.omega_biofilm.data(), rep.dt))) return rc;
     if (flow) {
         if ((rc = clb_ns_couple(ctx, id))) return rc;

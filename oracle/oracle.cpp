@@ -95,6 +95,8 @@ struct Case {
     Tracer tracer;
     double lastE = -1.0;
     int nsSaturate = 0;
+    int clogged = 0;                        // percolationFlag (complab.cpp:3113, 3119, 3176)
+    bool resolveFresh = true;               // the in-step re-solves' tracer ns_convg2 (complab.cpp:2166) is still empty
     long long oldTotMask = 0;
     long long iT = 0;
     int64_t counters[CLB_NCOUNTERS] = {0,0,0,0,0,0,0,0,0,0,0,0};
@@ -990,14 +992,13 @@ void caSweep(Case &c) {
     }
 }
 
-/* updateAgeDistance3D (part2:840-965), one sweep against a snapshot of the
- * ages (the reference updates in place; the relaxation loop below reaches
- * the same fixed point either way, since the rule only ever raises an age
- * and is monotone in the neighbours' ages). */
+/* updateAgeDistance3D (part2:840-965): one sweep in place, in the
+ * processor's loop order (x outermost, z innermost, part2:862-867), so a voxel
+ * reads the ages its -x, -y and -z face neighbours already took this sweep and
+ * the old ages of its +x, +y and +z neighbours. */
 void ageSweep(Case &c) {
     const double Bmax = c.p.bmax;
-    std::vector<int> old = c.age;
-    for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 0; x < c.nx; ++x) {
+    for (int x = 0; x < c.nx; ++x) for (int y = 0; y < c.ny; ++y) for (int z = 0; z < c.nz; ++z) {
         size_t i = c.id(x, y, z);
         double B = c.Btot[i];
         if (!(B > COMPLAB_THRD)) continue;
@@ -1007,20 +1008,20 @@ void ageSweep(Case &c) {
             if (!c.inside(X, Y, Z)) continue;
             if (c.dist[c.id(X, Y, Z)] > 0) del[nd++] = q;
         }
-        int a = old[i];
+        int a = c.age[i];
         int na = a; bool upd = true;
         if (a == 0) na = 1;
         else if (a == 1 && (B - Bmax) > -COMPLAB_THRD) {
             for (int t = 0; t < nd; ++t) {
                 size_t j = c.id(x + C7[del[t]][0], y + C7[del[t]][1], z + C7[del[t]][2]);
-                if (c.dist[j] > 0 && old[j] == 0) { upd = false; break; }
+                if (c.dist[j] > 0 && c.age[j] == 0) { upd = false; break; }
             }
             if (upd) na = 2;
         } else {
             int cnt = 0;
             for (int t = 0; t < nd; ++t) {
                 size_t j = c.id(x + C7[del[t]][0], y + C7[del[t]][1], z + C7[del[t]][2]);
-                if (old[j] >= a && (c.Btot[j] - Bmax) > -COMPLAB_THRD) ++cnt;
+                if (c.age[j] >= a && (c.Btot[j] - Bmax) > -COMPLAB_THRD) ++cnt;
             }
             if (cnt == nd) na = a + 1; else upd = false;
         }
@@ -1344,19 +1345,35 @@ int dissolConvert(Case &c) {
     return n;
 }
 
-/* re-solve after a mask change (complab.cpp:3085-3130) */
-int nsResolve(Case &c, double permRatio) {
+/* an in-step flow re-solve (complab.cpp:3087-3093, 3162-3168, 3221-3226): the
+ * loop's own tracer ns_convg2, created empty before the main loop and never
+ * reset, with the re-solve tolerance and cap */
+int nsResolveSolve(Case &c, int *conv) {
+    int it;
+    const int rc = nsSolve(c, c.p.ns_max_iter, c.p.ns_conv_rule, c.p.ns_conv_eps, c.resolveFresh ? 1 : 0, &it, conv);
+    c.resolveFresh = false;
+    return rc;
+}
+
+/* re-solve after a mask change (maskPath, complab.cpp:3085-3130) or a
+ * precipitation seal (3159-3178).  A re-solve that runs out of iterations
+ * with no flow through the outlet plane x = nx-2 clogs the case; one whose
+ * energy is not finite clogs it too on the mask-change path, where the
+ * velocity is then not coupled (3119-3121) */
+int nsResolve(Case &c, double permRatio, bool maskPath) {
     nsUpdateDynamics(c, permRatio);
-    int it, conv;
-    int rc = nsSolve(c, c.p.ns_max_iter, c.p.ns_conv_rule, c.p.ns_conv_eps, 0, &it, &conv);
+    int conv = 0;
+    int rc = nsResolveSolve(c, &conv);
+    if (rc == CLB_E_NONFINITE && maskPath) { c.counters[4]++; c.clogged = 1; c.err.clear(); return CLB_OK; }
     if (rc) return rc;
     c.counters[4]++;
     if (!conv) {
         c.nsSaturate = 1;
         double st[4]; nsStats(c, st);
         if (st[2] > THRD) c.nsSaturate = 0;
+        else c.clogged = 1;
     }
-    nsCouple(c);
+    if (!(maskPath && c.clogged)) nsCouple(c);
     return CLB_OK;
 }
 
@@ -1383,7 +1400,7 @@ int coupledStep(Case &c) {
             ageSweep(c);
             if (p.ade_update_interval > 0 && c.iT % p.ade_update_interval == 0) soluteOmegaSwitch(c);
             if (p.ns_update_interval > 0 && c.iT % p.ns_update_interval == 0 && p.flow_on && !c.nsSaturate) {
-                rc = nsResolve(c, p.perm_ratio);
+                rc = nsResolve(c, p.perm_ratio, true);
                 if (rc) return rc;
             }
         }
@@ -1394,7 +1411,7 @@ int coupledStep(Case &c) {
         long long post = sumMask(c);
         if (pre != post) {
             c.counters[5]++;
-            if (p.flow_on && !c.nsSaturate) { rc = nsResolve(c, p.precip_perm_ratio); if (rc) return rc; }
+            if (p.flow_on && !c.nsSaturate) { rc = nsResolve(c, p.precip_perm_ratio, false); if (rc) return rc; }
             soluteSolidSwitch(c);
         }
     }
@@ -1406,11 +1423,12 @@ int coupledStep(Case &c) {
             if (p.flow_on) {
                 nsUpdateDynamics(c, p.perm_ratio);
                 c.nsSaturate = 0;
-                int it, conv;
-                rc = nsSolve(c, p.ns_max_iter, p.ns_conv_rule, p.ns_conv_eps, 0, &it, &conv);
+                int conv;
+                rc = nsResolveSolve(c, &conv);
                 if (rc) return rc;
                 c.counters[4]++;
                 nsCouple(c);
+                c.clogged = 0;                          // reopening can undo a clog (complab.cpp:3234)
             }
             soluteSolidSwitch(c);
         }
@@ -1418,7 +1436,7 @@ int coupledStep(Case &c) {
     for (int s = 0; s < p.n_species; ++s) if (p.mobile[s]) { adeStream(c, s); rebuildReopened(c, s); }
     c.reopened.clear();
     c.iT++; c.counters[7]++;
-    return CLB_OK;
+    return c.clogged ? CLB_CLOGGED : CLB_OK;            // the loop ends after this step's stream (complab.cpp:3261)
 }
 
 /* only transport + kinetics: collide, biotic, abiotic, stream */
@@ -1546,7 +1564,8 @@ int orc_ade_stabilize(void *h, int steps) {
 int orc_run_steps(void *h, int n, int *done) {
     Case &c = *(Case *) h;
     int k = 0, rc = 0;
-    for (; k < n; ++k) { rc = coupledStep(c); if (rc) break; }
+    if (c.clogged) { if (done) *done = 0; return CLB_CLOGGED; }
+    for (; k < n; ++k) { rc = coupledStep(c); if (rc) { if (rc == CLB_CLOGGED) ++k; break; } }
     if (done) *done = k;
     return rc;
 }

@@ -112,7 +112,8 @@ def test_biotic_coupled_fp64(ctx):
         assert relerr(g.read(P.FIELD_CONC, s), o.read(P.FIELD_CONC, s), om) <= REL
     cg, co = g.counters(), o.counters()
     print("counters gpu", cg, "oracle", co)
-    assert cg[4] == co[4] and cg[5] == co[5] and cg[6] == co[6]
+    assert cg[4] == co[4] and cg[5] == co[5] and cg[6] == co[6] and cg[7] == co[7]
+    assert cg[2] == co[2], "clamp binds differ"
 
 
 def test_ca_spill_invariants_and_parity(ctx):

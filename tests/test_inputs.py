@@ -223,3 +223,82 @@ def test_abiotic_symbolic_file_hook(tmp_path):
     with open(xml, "w") as fh:                                 # not read while <symbolic> is off
         fh.write(XML.replace(dissol, sym.replace("true", "false")))
     assert P.SYM_ABIOTIC not in inputs.load(xml).symbolic
+
+
+THM_BLOCK = """temperature 298.15
+energy_units kJ
+reaction R1
+  stoich A -1 B 1
+  dG0 -20
+  microbe {microbe}
+"""
+
+
+def two_symbolic_microbes(tmp_path, microbe, kinds=("symbolic", "symbolic")):
+    nx, ny, nz = 6, 4, 3
+    xml = write_case(tmp_path, [5] * (nx * ny * nz))
+    (tmp_path / "g0.sym").write_text("vars A Bug\nrate growth = 0.1 * A\n")
+    (tmp_path / "g1.sym").write_text("vars A Bug2\nrate growth = 0.2 * A\n")
+    (tmp_path / "r.thm").write_text(THM_BLOCK.format(microbe=microbe))
+    text = XML.replace("<number_of_microbes>1</number_of_microbes>", "<number_of_microbes>2</number_of_microbes>")
+    text = text.replace("<microbe0>7</microbe0>", "<microbe0>7</microbe0><microbe1>8</microbe1>")
+    m0 = text[text.index("    <microbe0><name_of_microbes>"):text.index("  </microbiology>")]
+    m1 = m0.replace("microbe0>", "microbe1>").replace(">Bug<", ">Bug2<")
+    m0s = m0.replace("<reaction_type>kinetics</reaction_type>",
+                     "<reaction_type>%s</reaction_type><expressions_file>g0.sym</expressions_file>" % kinds[0])
+    m1s = m1.replace("<reaction_type>kinetics</reaction_type>",
+                     "<reaction_type>%s</reaction_type><expressions_file>g1.sym</expressions_file>" % kinds[1])
+    text = text.replace(m0, m0s + m1s)
+    text = text.replace("  <diagnostics>", "  <symbolic><enabled>true</enabled></symbolic>\n"
+                        "  <thermodynamics><enabled>true</enabled><energetics_file>r.thm</energetics_file></thermodynamics>\n"
+                        "  <diagnostics>")
+    with open(xml, "w") as fh:
+        fh.write(text)
+    return xml, text
+
+
+def test_thm_block_gates_the_organism_it_names(tmp_path):
+    """A block naming one organism gates that organism alone
+    (specOfMicrobe, complab3d_thermo.hh:640-648), and only it gets the
+    dG_/FT_ snapshot pair (gatedMicrobes, complab3d_energyfield.hh:190-197).
+    An organism other than microbe0 is accepted when no organism uses the
+    compiled kinetics path (complab3d_integration.hh:838)."""
+    from complab_amd import runner
+    xml, _ = two_symbolic_microbes(tmp_path, "Bug")
+    r = inputs.load(xml)
+    assert r.params.thm_on == 1 and r.params.thm_microbe == 0
+    assert runner.gated_microbes(r) == [(0, "Bug")]
+    (tmp_path / "b").mkdir()
+    xml, _ = two_symbolic_microbes(tmp_path / "b", "Bug2")
+    r = inputs.load(xml)
+    assert r.params.thm_microbe == 1 and runner.gated_microbes(r) == [(1, "Bug2")]
+    (tmp_path / "c").mkdir()
+    xml, _ = two_symbolic_microbes(tmp_path / "c", "all")
+    r = inputs.load(xml)
+    assert r.params.thm_microbe == -1 and runner.gated_microbes(r) == [(0, "Bug"), (1, "Bug2")]
+    (tmp_path / "d").mkdir()
+    xml, _ = two_symbolic_microbes(tmp_path / "d", "Bug2", kinds=("kinetics", "symbolic"))
+    with pytest.raises(inputs.InputError, match="microbe0"):          # the compiled path gates as microbe0 only
+        inputs.load(xml)
+
+
+def test_symbolic_runs_refuse_fixed_substrates_and_totals(tmp_path):
+    """The symbolic processors skip substrates held fixed and can read
+    speciation totals (complab3d_processors_symbolic.hh:124, 167); this path
+    does neither, so a symbolic run that asks for them is refused rather than
+    run differently."""
+    xml, text = two_symbolic_microbes(tmp_path, "all")
+    with open(xml, "w") as fh:
+        fh.write(text.replace("<chemistry>", "<chemistry><fix_concentration>no yes no</fix_concentration>"))
+    with pytest.raises(inputs.InputError, match="fix_concentration"):
+        inputs.load(xml)
+    with open(xml, "w") as fh:
+        fh.write(text.replace("<chemistry>", "<chemistry><fix_concentration>no no no</fix_concentration>"))
+    inputs.load(xml)
+    with open(xml, "w") as fh:
+        fh.write(text.replace("</simulation_mode>", "<fba_concentration_basis>total</fba_concentration_basis></simulation_mode>"))
+    with pytest.raises(inputs.InputError, match="total"):
+        inputs.load(xml)
+    with open(xml, "w") as fh:
+        fh.write(text.replace("</simulation_mode>", "<fba_concentration_basis>Free</fba_concentration_basis></simulation_mode>"))
+    inputs.load(xml)

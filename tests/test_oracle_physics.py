@@ -216,9 +216,10 @@ def test_ca_spill_invariants():
     assert np.all(m[(a < thr) & openv] == P.PORE)
 
 
-def gauss_seidel_age(age, dist, btot, bmax):
+def gauss_seidel_age(age, dist, btot, bmax, once=False):
     """updateAgeDistance3D as the reference runs it: in place, x outermost,
-    then y, then z (part2:868-955); repeated until the age sum stops changing."""
+    then y, then z (part2:868-955); repeated until the age sum stops changing,
+    or one sweep (`once`, the pass after a mask change, complab.cpp:3079)."""
     nz, ny, nx = age.shape
     age = age.copy()
     T = 1e-14
@@ -244,14 +245,73 @@ def gauss_seidel_age(age, dist, btot, bmax):
                     else:
                         if all(age[q] >= a and (btot[q] - bmax) > -T for q in nb):
                             age[z, y, x] = a + 1
-        if age.sum() == s0:
+        if once or age.sum() == s0:
             return age
 
 
+def jacobi_age_once(age, dist, btot, bmax):
+    """The same rule against a snapshot of the ages: what a parallel sweep
+    without ordering would give."""
+    nz, ny, nx = age.shape
+    old = age.copy()
+    new = age.copy()
+    T = 1e-14
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                B = btot[z, y, x]
+                if not B > T:
+                    continue
+                nb = [(z + dz, y + dy, x + dx) for dx, dy, dz in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1))
+                      if 0 <= x + dx < nx and 0 <= y + dy < ny and 0 <= z + dz < nz and dist[z + dz, y + dy, x + dx] > 0]
+                a = old[z, y, x]
+                if a == 0:
+                    new[z, y, x] = 1
+                elif a == 1 and (B - bmax) > -T:
+                    if not any(old[q] == 0 for q in nb):
+                        new[z, y, x] = 2
+                elif all(old[q] >= a and (btot[q] - bmax) > -T for q in nb):
+                    new[z, y, x] = a + 1
+    return new
+
+
+def test_single_age_sweep_is_the_in_place_pass():
+    """The sweep after a mask change is ONE in-place pass in the processor's
+    loop order (x outer, z inner): a voxel reads the ages its -x/-y/-z
+    neighbours already took.  Checked against a pure-Python pass of the
+    reference loop on a seeded 10x7x6 box whose outcome differs from a sweep
+    against a snapshot (so an unordered sweep would fail here), and from
+    several starting age fields."""
+    nx, ny, nz = 10, 7, 6
+    geom = np.full((nz, ny, nx), P.PORE, np.int8)
+    geom[:, 0, :] = P.WALL; geom[:, -1, :] = P.WALL; geom[0, :, :] = P.WALL; geom[-1, :, :] = P.WALL
+    p = P.default_params(nx, ny, nz, 1)
+    p.n_microbes = 1
+    p.bmax = 10.0
+    dist = np.zeros(geom.shape, int)
+    inner = np.zeros(geom.shape, bool); inner[:-1, :-1, :-1] = True
+    dist[(geom >= 2) & inner] = 1
+    differs = 0
+    for seed in range(6):
+        rng = np.random.default_rng(40 + seed)
+        b = np.where(geom >= 2, rng.choice([0.0, 5.0, 10.0, 10.0, 10.0], size=geom.shape), 0.0)
+        age0 = np.where(geom >= 2, rng.integers(0, 3, size=geom.shape), -1)
+        o = O.OracleCase(p, geom)
+        o.write(P.FIELD_BIOMASS, 0, b)
+        o.write(P.FIELD_AGE, 0, age0)
+        o.age_relax(False)
+        got = o.read(P.FIELD_AGE).astype(int)
+        o.close()
+        want = gauss_seidel_age(age0, dist, b, p.bmax, once=True)
+        assert np.array_equal(got, want), seed
+        differs += int(not np.array_equal(want, jacobi_age_once(age0, dist, b, p.bmax)))
+    assert differs >= 3
+
+
 def test_age_relaxation_reaches_the_reference_fixed_point():
-    """The restatement sweeps against a snapshot; the reference sweeps in
-    place.  The rule only raises ages and is monotone in the neighbours'
-    ages, so both reach the same fixed point."""
+    """To the fixed point (the CA loop's relaxation, complab.cpp:3025-3033)
+    the restatement matches the reference's in-place sweeps repeated until
+    the age sum stops changing."""
     nx, ny, nz = 10, 7, 6
     rng = np.random.default_rng(4)
     geom = np.full((nz, ny, nx), P.PORE, np.int8)
