@@ -70,6 +70,7 @@ def lib():
         "clb_read_field": ([V, I, I, I, V], I),
         "clb_write_field": ([V, I, I, I, V], I),
         "clb_species_totals": ([V, I, V], I),
+        "clb_species_totals_noalloc": ([V, I, V], I),
         "clb_field_stats": ([V, I, V], I),
         "clb_counters": ([V, I, V], I),
         "clb_kernel_time": ([V, I, I, C.POINTER(D), C.POINTER(C.c_int64)], I),
@@ -106,6 +107,7 @@ def exported_symbols():
         "clb_set_symbolic", "clb_rate_capture", "clb_checkpoint_size", "clb_checkpoint_save", "clb_checkpoint_load",
         "clb_stream_copy",
         "clb_rng_draws", "clb_geom_spheres", "clb_geom_pack", "clb_geom_generate",
+        "clb_species_totals_noalloc",
     ]
 
 

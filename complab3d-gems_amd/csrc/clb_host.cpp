@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <string>
 #include <algorithm>
@@ -235,6 +236,7 @@ struct CaseBase {
     virtual Status readField(hipStream_t, int field, int index, double *dst) = 0;
     virtual Status writeField(hipStream_t, int field, int index, const double *src) = 0;
     virtual void speciesTotals(hipStream_t, double *out) = 0;
+    virtual Status speciesTotalsScratch(hipStream_t, double *buf, size_t cap, double *out) = 0;
     virtual void fieldStats(hipStream_t, double *out) = 0;
     virtual void counters(hipStream_t, int64_t out[CLB_NCOUNTERS]) = 0;
     virtual void kernelTime(hipStream_t, int reset, double *ms, int64_t *n) = 0;
@@ -1459,6 +1461,21 @@ struct Case : CaseBase {
         HIPT(hipMemcpyAsync(out, sums, p.n_species * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPT(hipStreamSynchronize(st));
     }
+    // doubles of partials + sums the totals need in a caller's buffer
+    size_t totalsScratchDoubles() const {
+        size_t need = 0;
+        for (int s = 0; s < p.n_species; ++s)
+            need = std::max(need, p.mobile[s] ? (size_t) nblk(Nint, 256) : (size_t) 4 * nblk(N, 256));
+        return need + CLB_MAX_SPECIES;
+    }
+    Status speciesTotalsScratch(hipStream_t st, double *buf, size_t cap, double *out) override {
+        const size_t need = totalsScratchDoubles();
+        if (need > cap)
+            return {CLB_E_ARG, "the case's reductions need " + std::to_string(need * sizeof(double)) +
+                               " bytes of scratch; the static scratch holds " + std::to_string(cap * sizeof(double))};
+        speciesTotalsWith(st, buf + CLB_MAX_SPECIES, buf, out);
+        return {CLB_OK, ""};
+    }
     // per species, then per microbe: total, minimum, maximum, held
     void fieldStats(hipStream_t st, double *out) override {
         const int S = p.n_species;
@@ -1519,6 +1536,33 @@ struct Case : CaseBase {
 };
 
 }  // namespace
+
+// ------------------------------------------------------------------ static scratch
+// clb_species_totals_noalloc keeps its reduction partials and sums in this
+// file-scope device array instead of the case's buffers: fixed size, one copy
+// per GPU at that GPU's own address, held until the process ends.  Host code
+// never passes the array's name to a kernel or a copy (on the host it is a
+// stand-in, not a device address): clb_ctx_create looks each device's copy up
+// once with hipGetSymbolAddress, with that device current, and keeps the
+// pointer in kScratchDev.  Every call on one GPU shares the array.
+constexpr size_t kScratchDoubles = CLB_NOALLOC_SCRATCH_BYTES / sizeof(double);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static std::mutex kScratchMu;
+static double *kScratchDev[kMaxDevices] = {};
+// the device's copy of `scratch` (looked up on the first call for a device)
+static hipError_t scratchOn(int device, double **out) {
+    std::lock_guard<std::mutex> lk(kScratchMu);
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    if (!kScratchDev[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        kScratchDev[device] = static_cast<double *>(ptr);
+    }
+    *out = kScratchDev[device];
+    return hipSuccess;
+}
 
 struct clb_ctx {
     int device = 0;
@@ -1596,6 +1640,8 @@ int clb_ctx_create(int device, clb_ctx **out) {
     clb_ctx *c = new clb_ctx();
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) { delete c; return CLB_E_HIP; }
+    double *scr = nullptr;                       // this device's static scratch, once per device
+    if (scratchOn(device, &scr) != hipSuccess) { (void) hipStreamDestroy(c->stream); delete c; return CLB_E_HIP; }
     *out = c;
     return CLB_OK;
 }
@@ -1858,6 +1904,17 @@ int clb_species_totals(clb_ctx *ctx, int case_id, double *out) {
     CASE_OR_FAIL(c)
     if (!out) return fail(ctx, CLB_E_ARG, "null output");
     c->speciesTotals(ctx->stream, out);
+    return CLB_OK;
+    GUARD_END
+}
+int clb_species_totals_noalloc(clb_ctx *ctx, int case_id, double *out) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    if (!out) return fail(ctx, CLB_E_ARG, "null output");
+    double *scr = nullptr;
+    HIPT(scratchOn(ctx->device, &scr));
+    Status r = c->speciesTotalsScratch(ctx->stream, scr, kScratchDoubles, out);
+    if (r.code) return fail(ctx, r.code, r.msg);
     return CLB_OK;
     GUARD_END
 }

@@ -315,6 +315,17 @@ int  clb_write_field(clb_ctx *ctx, int case_id, int field, int index, const doub
  * over every voxel (a mineral keeps its inventory in the voxels it sealed,
  * complab3d_diagnostics.hh:255-261); out has n_species entries */
 int  clb_species_totals(clb_ctx *ctx, int case_id, double *out);
+/* clb_species_totals, allocating nothing: the block partials and sums live in
+ * a static device array of CLB_NOALLOC_SCRATCH_BYTES that the library holds on
+ * every GPU it runs on, not in the case's buffers.  No counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  CLB_E_ARG when the case's reductions
+ * need more than the array holds (an immobile species needs 32 bytes per 256
+ * voxels of the lattice: about 41.9 M voxels at most).  Calls on one GPU share
+ * the array: calls that overlap on one GPU (several host threads or streams)
+ * race; nothing in it needs clearing between calls. */
+#define CLB_NOALLOC_SCRATCH_BYTES (5u << 20)
+int  clb_species_totals_noalloc(clb_ctx *ctx, int case_id, double *out);
 /* diagnostics row (complab.cpp:2736-2785, complab3d_diagnostics.hh:57-69): for
  * each species, then each microbe, four values -- total, minimum, maximum and
  * held.  total is clb_species_totals' (a microbe: its biomass over every voxel
