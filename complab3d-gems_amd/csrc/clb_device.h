@@ -25,6 +25,19 @@ enum { BC_DIRICHLET_DEV = 0, BC_NEUMANN_DEV = 1, BC_CLOSED_DEV = 2 };
 #define NB_CLOSED -1
 #define NB_OUTSIDE -2
 
+// Populations are stored in tiles of 64 voxels (one wavefront): population q
+// of compact voxel i of a Q-population lattice lives at
+// ((i / 64) * Q + q) * 64 + i % 64.  The Q loads or stores of one wave then
+// fall in one contiguous stretch of Q x 64 values of a tile instead of Q
+// arrays Nopen apart; on gfx950 the 19-array pattern of one thread per voxel
+// streams at 4.7 TB/s and the tiled one at 5.6 TB/s.  A lattice holds
+// popPad(Nopen) voxels' worth (the tail of the last tile is never touched).
+__host__ __device__ __forceinline__ long long popPad(long long n) { return (n + 63) & ~63LL; }
+template <int Q>
+__host__ __device__ __forceinline__ long long pidx(int q, long long i) {
+    return (((i >> 6) * Q + q) << 6) | (i & 63);
+}
+
 // step-batch control block: a batch of coupled steps is queued without a host
 // round trip; the mask/total reduction of each step decides on the device
 // whether the host must act (CA spill, mask change, non-finite biomass), and
@@ -43,13 +56,14 @@ struct AdeDev {
     int nx, ny, nz;
     long long N;
     int Nopen, Nint, Nleft;
+    long long NoP;                 // popPad(Nopen): the voxels one species' lattice holds
     const int *list;               // [Nopen]
     const int *nb;                 // [6][Nopen]
     int S, Sm, Smax, M;
     int mobile[CLB_DEV_MAXS];      // per species
     int slot[CLB_DEV_MAXS];        // per species: mobile slot or immobile slot
     int mobSpecies[CLB_DEV_MAXS];  // per mobile slot: species index
-    R *cur, *out;                  // [Smax][7][Nopen]
+    R *cur, *out;                  // [Smax][7 x NoP] tiled (pidx<7>)
     R *imm;                        // [Si][N] dense
     R *B;                          // [M][N] dense
     const R *u;                    // [3][Nopen]
@@ -124,10 +138,11 @@ struct NsDev {
     int nx, ny, nz;
     long long N;
     int Nopen, Nint, Nleft;
+    long long NoP;                 // popPad(Nopen)
     const int *list;               // [Nopen]
     const int8_t *geom;            // dense geometry codes (the pore-space statistics)
     const int *nb;                 // [18][Nopen]
-    R *cur, *out;                  // [19][Nopen]
+    R *cur, *out;                  // [19 x NoP] tiled (pidx<19>)
     const uint8_t *cls, *clsPull;  // [Nopen]
     const unsigned long long *kinds;  // [Nint] source kind of each moving population, 2 bits per direction
     const int4 *nbr;               // [Nint][3] compact neighbour record (NsNbRecord), the interior's pull

@@ -43,10 +43,10 @@ __global__ void __launch_bounds__(128, (eqWavesPerSimd<NS, NC, SOLVER>())) k_equ
         if (s >= S) break;
         double r;
         if (a.mobile[s]) {
-            const R *o = a.cur + (long long) a.slot[s] * 7 * No + i;
+            const R *o = a.cur + (long long) a.slot[s] * 7 * a.NoP;
             r = 0;
 #pragma unroll
-            for (int q = 0; q < 7; ++q) r += (double) o[q * No];
+            for (int q = 0; q < 7; ++q) r += (double) o[pidx<7>(q, i)];
         } else r = (double) a.imm[(long long) a.slot[s] * N + v];
         C[s] = fmax(r, EQ_MIN_CONC);
     }
@@ -61,9 +61,9 @@ __global__ void __launch_bounds__(128, (eqWavesPerSimd<NS, NC, SOLVER>())) k_equ
         if (C[s] + dC < 1e-20) dC = 1e-20 - C[s];
         if (fabs(dC) > 1e-12) {
             if (a.mobile[s]) {
-                R *o = a.cur + (long long) a.slot[s] * 7 * No + i;
+                R *o = a.cur + (long long) a.slot[s] * 7 * a.NoP;
 #pragma unroll
-                for (int q = 0; q < 7; ++q) o[q * No] = (R) ((double) o[q * No] + w7(q) * dC);
+                for (int q = 0; q < 7; ++q) o[pidx<7>(q, i)] = (R) ((double) o[pidx<7>(q, i)] + w7(q) * dC);
             } else {
                 R *p = a.imm + (long long) a.slot[s] * N + v;
                 *p = (R) ((double) *p + dC);

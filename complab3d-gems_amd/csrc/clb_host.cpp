@@ -162,7 +162,8 @@ std::vector<size_t> chkSectionSizes(const ChkHeader &h) {
     v.push_back(no); v.push_back(no); v.push_back(no); v.push_back(no);      // adeCls adeClsPull nsCls nsClsPull
     v.push_back(no);                                                          // reopened
     v.push_back(ni * sizeof(uint16_t));                                       // adeKinds
-    v.push_back(sm * 7 * no * r); v.push_back(sm * 7 * no * r);               // fA fB
+    const size_t nop = (size_t) popPad(h.Nopen);                              // tiled population lattices
+    v.push_back(sm * 7 * nop * r); v.push_back(sm * 7 * nop * r);             // fA fB
     v.push_back(si * n * r); v.push_back(m * n * r); v.push_back(n * r);      // imm B Btot
     v.push_back(n * sizeof(int)); v.push_back(n * sizeof(int));               // age age2 (pointer order)
     v.push_back(h.M > 0 ? n : 0);                                             // pushed
@@ -173,7 +174,7 @@ std::vector<size_t> chkSectionSizes(const ChkHeader &h) {
     v.push_back(3 * sizeof(unsigned long long));                              // dcount
     v.push_back(sizeof(StepCtl));                                             // ctl
     if (h.flowOn) {
-        v.push_back(19 * no * rn); v.push_back(19 * no * rn);                 // gA gB
+        v.push_back(19 * nop * rn); v.push_back(19 * nop * rn);               // gA gB
         v.push_back(ni * sizeof(unsigned long long));                         // nsKinds
         v.push_back(sizeof(NsTracer));                                        // tracer
         v.push_back(sizeof(int));                                             // nsDone
@@ -367,7 +368,7 @@ struct Case : CaseBase {
         AdeDev<R> a;
         memset(&a, 0, sizeof(a));
         a.nx = nx; a.ny = ny; a.nz = nz; a.N = N;
-        a.Nopen = Nopen; a.Nint = Nint; a.Nleft = Nleft; a.list = list; a.nb = nb7;
+        a.Nopen = Nopen; a.Nint = Nint; a.Nleft = Nleft; a.NoP = popPad(Nopen); a.list = list; a.nb = nb7;
         a.S = p.n_species; a.Sm = Sm; a.Smax = Sm > 0 ? Sm : 1; a.M = p.n_microbes;
         for (int s = 0; s < CLB_DEV_MAXS; ++s) {
             a.mobile[s] = s < p.n_species ? p.mobile[s] : 0;
@@ -433,7 +434,7 @@ struct Case : CaseBase {
         NsDev<RN> a;
         memset(&a, 0, sizeof(a));
         a.nx = nx; a.ny = ny; a.nz = nz; a.N = N;
-        a.Nopen = Nopen; a.Nint = Nint; a.Nleft = Nleft; a.list = list; a.nb = nb19; a.geom = geom;
+        a.Nopen = Nopen; a.Nint = Nint; a.Nleft = Nleft; a.NoP = popPad(Nopen); a.list = list; a.nb = nb19; a.geom = geom;
         a.cur = gCur(); a.out = gOut();
         a.cls = nsCls; a.clsPull = nsClsPull; a.kinds = nsKinds; a.nbr = (const int4 *) nsNbr;
         a.om = nsOmega; a.omBio = nsBioOmega; a.rhoBarRight = -deltaP * 3.0;
@@ -570,8 +571,8 @@ struct Case : CaseBase {
         adeKinds = dalloc<uint16_t>(allocs, std::max(Nint, 1));
         nsCls = dalloc<uint8_t>(allocs, Nopen); nsClsPull = dalloc<uint8_t>(allocs, Nopen);
         reopened = dalloc<uint8_t>(allocs, Nopen);
-        fA = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * 7 * Nopen);
-        fB = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * 7 * Nopen);
+        fA = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * 7 * popPad(Nopen));
+        fB = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * 7 * popPad(Nopen));
         imm = dalloc<R>(allocs, (size_t) std::max(Si, 1) * N);
         const int M = p.n_microbes;
         B = dalloc<R>(allocs, (size_t) std::max(M, 1) * N);
@@ -583,7 +584,7 @@ struct Case : CaseBase {
         age = dalloc<int>(allocs, N); age2 = dalloc<int>(allocs, N); ageList = dalloc<int>(allocs, std::max(Nopen, 1));
         ageCnt = dalloc<unsigned int>(allocs, 32);
         if (p.flow_on) {
-            gA = dalloc<RN>(allocs, 19 * (size_t) Nopen); gB = dalloc<RN>(allocs, 19 * (size_t) Nopen);
+            gA = dalloc<RN>(allocs, 19 * (size_t) popPad(Nopen)); gB = dalloc<RN>(allocs, 19 * (size_t) popPad(Nopen));
             nb19 = dalloc<int>(allocs, (size_t) 18 * Nopen);
             nsNbr = dalloc<NsNbRecord>(allocs, std::max(Nint, 1));
             nsKinds = dalloc<unsigned long long>(allocs, std::max(Nint, 1));
@@ -662,15 +663,16 @@ struct Case : CaseBase {
         HIPT(hipMemsetAsync(reopened, 0, Nopen, st));
         // populations at rest with c0 (initializeAtEquilibrium + stabilizeADElattice3D seeding)
         {
-            std::vector<R> fl((size_t) 7 * Nopen);
+            const long long nop = popPad(Nopen);
+            std::vector<R> fl((size_t) 7 * nop);
             for (int ms = 0; ms < Sm; ++ms) {
                 int s = mobSpecies[ms];
                 for (int q = 0; q < 7; ++q) {
                     R v = (R) ((q == 0 ? 0.25 : 0.125) * p.c0[s]);
-                    std::fill(fl.begin() + (size_t) q * Nopen, fl.begin() + (size_t) (q + 1) * Nopen, v);
+                    for (long long i = 0; i < nop; ++i) fl[(size_t) pidx<7>(q, i)] = v;
                 }
-                HIPT(hipMemcpy(fA + (size_t) ms * 7 * Nopen, fl.data(), fl.size() * sizeof(R), hipMemcpyHostToDevice));
-                HIPT(hipMemcpy(fB + (size_t) ms * 7 * Nopen, fl.data(), fl.size() * sizeof(R), hipMemcpyHostToDevice));
+                HIPT(hipMemcpy(fA + (size_t) ms * 7 * nop, fl.data(), fl.size() * sizeof(R), hipMemcpyHostToDevice));
+                HIPT(hipMemcpy(fB + (size_t) ms * 7 * nop, fl.data(), fl.size() * sizeof(R), hipMemcpyHostToDevice));
             }
             std::vector<R> sc(N);
             for (int s = 0; s < p.n_species; ++s) {
@@ -1674,13 +1676,16 @@ size_t clb_case_footprint(const clb_case_params *p) {
     int sm = 0, si = 0;
     for (int s = 0; s < p->n_species && s < CLB_MAX_SPECIES; ++s) (p->mobile[s] ? sm : si)++;
     size_t b = 0;
+    const size_t NP = (size_t) popPad((long long) N);  // tiled population lattices
     b += N * 9;                                         // mask, geom, dist, phase, classes, reopened
-    b += 2 * (size_t) std::max(sm, 1) * 7 * N * r;      // ADE populations
+    b += 2 * (size_t) std::max(sm, 1) * 7 * NP * r;     // ADE populations
+    b += N * (sizeof(int) + 6 * sizeof(int) + sizeof(uint16_t));   // voxel list, D3Q7 table, pull kinds
     b += (size_t) std::max(si, 1) * N * r;
     b += (size_t) std::max(p->n_microbes, 1) * N * r + N * r;
     if (p->n_microbes > 0) b += (size_t) p->n_microbes * 6 * N * r + N;
-    b += 2 * N * sizeof(int);
-    if (p->flow_on) b += 2 * 19 * N * rn;
+    b += 3 * N * sizeof(int);                           // ages (two) and the biomass voxel list
+    if (p->flow_on) b += 2 * 19 * NP * rn + N * (18 * sizeof(int) + sizeof(NsNbRecord) + sizeof(unsigned long long));
+    if (p->dissol_on) b += (size_t) std::max(p->n_species, 1) * N * sizeof(double) + N * sizeof(int);
     b += 3 * N * r;
     b += N * sizeof(double);                            // staging
     b += ((N + 127) / 128) * (5 * sizeof(double) + sizeof(long long));

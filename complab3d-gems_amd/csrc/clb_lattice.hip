@@ -52,17 +52,17 @@ __device__ __forceinline__ void pull7(const AdeDev<R> &a, const R *__restrict__ 
     const long long No = a.Nopen;
     if (a.first) {
 #pragma unroll
-        for (int q = 0; q < 7; ++q) fl[q] = (T) cur[q * No + i];
+        for (int q = 0; q < 7; ++q) fl[q] = (T) cur[pidx<7>(q, i)];
         return;
     }
-    fl[0] = (T) cur[i];
+    fl[0] = (T) cur[pidx<7>(0, i)];
 #pragma unroll
     for (int q = 1; q < 7; ++q) {
         const int n = a.nb[(q - 1) * No + i];
-        if (n == NB_OUTSIDE) fl[q] = (T) cur[opp7(q) * No + i];               // edge: own opposite
+        if (n == NB_OUTSIDE) fl[q] = (T) cur[pidx<7>(opp7(q), i)];            // edge: own opposite
         else if (n == NB_CLOSED || closedFor(a.clsPull, n, a.Nint, a.Nleft, lbt, rbt))
-            fl[q] = (T) old[opp7(q) * No + i];                                // bounce-back
-        else fl[q] = (T) cur[q * No + n];
+            fl[q] = (T) old[pidx<7>(opp7(q), i)];                             // bounce-back
+        else fl[q] = (T) cur[pidx<7>(q, n)];
     }
 }
 
@@ -81,7 +81,7 @@ __device__ __forceinline__ void firstHandOver(const AdeDev<R> &a, R *cur, int i,
         const int n = a.nb[(q - 1) * No + i];
         const bool closed = planesOnly ? (n >= a.Nint && closedFor(a.clsPull, n, a.Nint, a.Nleft, lbt, rbt))
                                        : (n == NB_CLOSED || (n >= 0 && closedFor(a.clsPull, n, a.Nint, a.Nleft, lbt, rbt)));
-        if (closed) cur[opp7(q) * No + i] = (R) (W7F * c0);
+        if (closed) cur[pidx<7>(opp7(q), i)] = (R) (W7F * c0);
     }
 }
 
@@ -197,11 +197,13 @@ __device__ void abioticLaw(const KinDev &k, const double *C, double *sr) {
     if (k.rate_scale != 1.0) for (int s = 0; s < k.S; ++s) sr[s] *= k.rate_scale;
 }
 
-// the clamp dC = max(R dt, -C), only for a debit and only against what is there
+// the clamp dC = max(R dt, -C), only for a debit and only against what is
+// there; a bind is counted when the debit exceeds the availability by more
+// than 1e-12 of it (one equal to it but for rounding is capped, not counted)
 __device__ __forceinline__ double clampInc(int clamp, double d, double cur, unsigned int &binds) {
     if (!clamp || d >= 0) return d;
     const double lim = -fmax(cur, 0.0);
-    if (d < lim) { ++binds; return lim; }
+    if (d < lim) { if (d < lim - 1e-12 * fabs(lim)) ++binds; return lim; }
     return d;
 }
 
@@ -600,8 +602,8 @@ __device__ __forceinline__ void adePlaneVoxel(const AdeDev<R> &a, int i, int ms)
     const int s = a.mobSpecies[ms];
     const int bt = left ? a.lbt[s] : a.rbt[s];
     if (bt == BC_CLOSED_DEV) return;
-    R *cur = a.cur + (long long) ms * 7 * No;
-    R *old = a.out + (long long) ms * 7 * No;
+    R *cur = a.cur + (long long) ms * 7 * a.NoP;
+    R *old = a.out + (long long) ms * 7 * a.NoP;
     R f[7];
     pull7<R, R>(a, cur, old, i, a.lbt[s], a.rbt[s], f);
     if (a.first) firstHandOver<R>(a, cur, i, a.lbt[s], a.rbt[s], a.c0[s], false);   // a plane voxel has no plane neighbour
@@ -618,7 +620,7 @@ __device__ __forceinline__ void adePlaneVoxel(const AdeDev<R> &a, int i, int ms)
     }
     adeDirichlet<R>(f, left ? 1 : 0, (R) rhoD);
     bgk7<R>(f, (R) (cls == ADE_BFILM ? a.omB[s] : a.omP[s]), ux, uy, uz);
-    for (int q = 0; q < 7; ++q) old[q * No + i] = f[q];
+    for (int q = 0; q < 7; ++q) old[pidx<7>(q, i)] = f[q];
 }
 
 // interior: pull + BGK collide + biotic and abiotic kinetics with the clamp
@@ -658,21 +660,21 @@ __device__ __forceinline__ void adeBulkVoxel(const AdeDev<R> &a, const KinDev &k
 #pragma unroll
     for (int ms = 0; ms < MAXS; ++ms) {
         if (ms >= a.Sm) continue;
-        const R *__restrict__ cur = a.cur + (long long) ms * 7 * No;
-        const R *__restrict__ old = a.out + (long long) ms * 7 * No;
+        const R *__restrict__ cur = a.cur + (long long) ms * 7 * a.NoP;
+        const R *__restrict__ old = a.out + (long long) ms * 7 * a.NoP;
         if (a.first) {
 #pragma unroll
-            for (int q = 0; q < 7; ++q) f[ms][q] = cur[q * No + i];
+            for (int q = 0; q < 7; ++q) f[ms][q] = cur[pidx<7>(q, i)];
         } else {
             const bool lClosed = a.mlbt[ms] == BC_CLOSED_DEV, rClosed = a.mrbt[ms] == BC_CLOSED_DEV;
-            f[ms][0] = cur[i];
+            f[ms][0] = cur[pidx<7>(0, i)];
 #pragma unroll
             for (int q = 1; q < 7; ++q) {
                 const int n = nbq[q - 1];
                 int kd = kind[q - 1];
                 if (kd == 3) kd = ((n < a.Nint + a.Nleft) ? lClosed : rClosed) ? 2 : 0;
                 const R *base = kd == 2 ? old : cur;
-                const long long off = kd == 0 ? (long long) q * No + n : (long long) opp7(q) * No + i;
+                const long long off = kd == 0 ? pidx<7>(q, n) : pidx<7>(opp7(q), i);
                 f[ms][q] = base[off];
             }
         }
@@ -693,11 +695,11 @@ __device__ __forceinline__ void adeBulkVoxel(const AdeDev<R> &a, const KinDev &k
             for (int q = 0; q < 7; ++q) r += (double) f[ms][q];
             a.rhoN[(long long) ms * Np + (nbq[1] - a.Nint)] = r;
         }
-        if (a.first) firstHandOver<R>(a, a.cur + (long long) ms * 7 * No, i, a.mlbt[ms], a.mrbt[ms], a.mc0[ms], false);
+        if (a.first) firstHandOver<R>(a, a.cur + (long long) ms * 7 * a.NoP, i, a.mlbt[ms], a.mrbt[ms], a.mc0[ms], false);
         if (a.reset) {
             double c0 = a.mc0[ms];
             if (c0 < 1e-12 && c0 > -1e-12) c0 = 0;
-            if (!a.first) firstHandOver<R>(a, a.cur + (long long) ms * 7 * No, i, a.mlbt[ms], a.mrbt[ms], c0, true);
+            if (!a.first) firstHandOver<R>(a, a.cur + (long long) ms * 7 * a.NoP, i, a.mlbt[ms], a.mrbt[ms], c0, true);
 #pragma unroll
             for (int q = 0; q < 7; ++q) f[ms][q] = (R) (w7(q) * c0);
         }
@@ -709,9 +711,9 @@ __device__ __forceinline__ void adeBulkVoxel(const AdeDev<R> &a, const KinDev &k
 #pragma unroll
     for (int ms = 0; ms < MAXS; ++ms) {
         if (ms >= a.Sm) continue;
-        R *o = a.out + (long long) ms * 7 * No;
+        R *o = a.out + (long long) ms * 7 * a.NoP;
 #pragma unroll
-        for (int q = 0; q < 7; ++q) o[q * No + i] = f[ms][q];
+        for (int q = 0; q < 7; ++q) o[pidx<7>(q, i)] = f[ms][q];
     }
     if (a.maskTotal) voxelMaskTotal<R>(a, v, vmask, bpre, bt, mt);
 }
@@ -762,18 +764,18 @@ __device__ __forceinline__ void kineticsVoxel(const AdeDev<R> &a, const KinDev &
 #pragma unroll
     for (int ms = 0; ms < MAXS; ++ms) {
         if (ms >= a.Sm) continue;
-        const R *c = a.cur + (long long) ms * 7 * No;
+        const R *c = a.cur + (long long) ms * 7 * a.NoP;
 #pragma unroll
-        for (int q = 0; q < 7; ++q) f[ms][q] = c[q * No + i];
+        for (int q = 0; q < 7; ++q) f[ms][q] = c[pidx<7>(q, i)];
     }
     R nopre[CLB_DEV_MAXM] = {};
     voxelKinetics<R, MAXS, DISSOL, SYM>(a, k, i, f, v, mask, nopre, false);
 #pragma unroll
     for (int ms = 0; ms < MAXS; ++ms) {
         if (ms >= a.Sm) continue;
-        R *c = a.cur + (long long) ms * 7 * No;
+        R *c = a.cur + (long long) ms * 7 * a.NoP;
 #pragma unroll
-        for (int q = 0; q < 7; ++q) c[q * No + i] = f[ms][q];
+        for (int q = 0; q < 7; ++q) c[pidx<7>(q, i)] = f[ms][q];
     }
 }
 template <typename R, int MAXS, bool DISSOL, bool SYM>
@@ -817,10 +819,10 @@ template <typename R>
 __device__ __forceinline__ double storedConc(const AdeDev<R> &a, int s, int n) {
     const int sl = a.slot[s];
     if (a.mobile[s]) {
-        const R *c = a.cur + (long long) sl * 7 * a.Nopen + n;
+        const R *c = a.cur + (long long) sl * 7 * a.NoP;
         double r = 0;
 #pragma unroll
-        for (int q = 0; q < 7; ++q) r += (double) c[(long long) q * a.Nopen];
+        for (int q = 0; q < 7; ++q) r += (double) c[pidx<7>(q, n)];
         return r;
     }
     return (double) a.imm[(long long) sl * a.N + a.list[n]];
@@ -932,7 +934,7 @@ __global__ void __launch_bounds__(256) k_energy_field(AdeDev<R> a, KinDev k, dou
         } else if (!closedFor(a.cls, i, a.Nint, a.Nleft, a.lbt[s], a.rbt[s])) {
             const int ms = a.slot[s];
             double fl[7];
-            pull7<R, double>(a, a.cur + (long long) ms * 7 * a.Nopen, a.out + (long long) ms * 7 * a.Nopen, i, a.lbt[s], a.rbt[s], fl);
+            pull7<R, double>(a, a.cur + (long long) ms * 7 * a.NoP, a.out + (long long) ms * 7 * a.NoP, i, a.lbt[s], a.rbt[s], fl);
             for (int q = 0; q < 7; ++q) r += fl[q];
         }
         C[s] = r > 0.0 ? r : 0.0;
@@ -960,7 +962,7 @@ __global__ void k_ade_density(AdeDev<R> a, int ms, double *dst) {
         if (r < 1e-12 && r > -1e-12) r = 0;
     } else {
         double fl[7];
-        pull7<R, double>(a, a.cur + (long long) ms * 7 * a.Nopen, a.out + (long long) ms * 7 * a.Nopen, i, a.lbt[s], a.rbt[s], fl);
+        pull7<R, double>(a, a.cur + (long long) ms * 7 * a.NoP, a.out + (long long) ms * 7 * a.NoP, i, a.lbt[s], a.rbt[s], fl);
         for (int q = 0; q < 7; ++q) r += fl[q];
     }
     dst[v] = r;
@@ -997,7 +999,7 @@ __global__ void __launch_bounds__(256) k_ade_held(AdeDev<R> a, int ms, double *p
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = a.mobSpecies[ms];
     const long long No = a.Nopen;
-    const R *cur = a.cur + (long long) ms * 7 * No;
+    const R *cur = a.cur + (long long) ms * 7 * a.NoP;
     double h = 0, lo = -1e300, hi = -1e300;
     if (i < a.Nopen) {
         const int lbt = a.lbt[s], rbt = a.rbt[s];
@@ -1006,7 +1008,7 @@ __global__ void __launch_bounds__(256) k_ade_held(AdeDev<R> a, int ms, double *p
             if (a.reset) { v = a.c0[s]; if (v < 1e-12 && v > -1e-12) v = 0; }
             else {
                 double fl[7];
-                pull7<R, double>(a, cur, a.out + (long long) ms * 7 * No, i, lbt, rbt, fl);
+                pull7<R, double>(a, cur, a.out + (long long) ms * 7 * a.NoP, i, lbt, rbt, fl);
                 for (int q = 0; q < 7; ++q) v += fl[q];
             }
             if (i >= a.Nint) h += v;
@@ -1025,7 +1027,7 @@ __global__ void __launch_bounds__(256) k_ade_held(AdeDev<R> a, int ms, double *p
                 if (d == NB_CLOSED || (d >= 0 && closedFor(a.cls, d, a.Nint, a.Nleft, lbt, rbt))) {
                     if (a.first) h += W7F * a.c0[s];
                     else if (a.reset && d >= a.Nint) h += W7F * c0r;
-                    else h += (double) cur[q * No + i];
+                    else h += (double) cur[pidx<7>(q, i)];
                 }
             }
         }
@@ -1051,9 +1053,9 @@ __global__ void k_ade_set(AdeDev<R> a, int ms, const double *src) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.Nopen) return;
     const int s = a.mobSpecies[ms];
-    R *cur = a.cur + (long long) ms * 7 * a.Nopen;
+    R *cur = a.cur + (long long) ms * 7 * a.NoP;
     const double c = closedFor(a.cls, i, a.Nint, a.Nleft, a.lbt[s], a.rbt[s]) ? a.c0[s] : src[a.list[i]];
-    for (int q = 0; q < 7; ++q) cur[q * (long long) a.Nopen + i] = (R) (w7(q) * c);
+    for (int q = 0; q < 7; ++q) cur[pidx<7>(q, i)] = (R) (w7(q) * c);
 }
 
 // a voxel reopened this step: rest population, and the rest value on every
@@ -1073,17 +1075,17 @@ __global__ void k_ade_reopen(AdeDev<R> a, const uint8_t *reopened) {
     const long long No = a.Nopen;
     for (int ms = 0; ms < a.Sm; ++ms) {
         const int s = a.mobSpecies[ms];
-        R *cur = a.cur + (long long) ms * 7 * No;
-        R *old = a.out + (long long) ms * 7 * No;
-        cur[i] = (R) (W7R * a.c0[s]);
+        R *cur = a.cur + (long long) ms * 7 * a.NoP;
+        R *old = a.out + (long long) ms * 7 * a.NoP;
+        cur[pidx<7>(0, i)] = (R) (W7R * a.c0[s]);
         for (int q = 1; q < 7; ++q) {
             const int n = a.nb[(q - 1) * No + i];
             const bool closedSrc = n < 0 || closedFor(a.clsPull, n, a.Nint, a.Nleft, a.lbt[s], a.rbt[s]);
             if (closedSrc) {
-                old[opp7(q) * No + i] = (R) (W7F * a.c0[s]);
-                cur[opp7(q) * No + i] = (R) (W7F * a.c0[s]);
+                old[pidx<7>(opp7(q), i)] = (R) (W7F * a.c0[s]);
+                cur[pidx<7>(opp7(q), i)] = (R) (W7F * a.c0[s]);
             } else {
-                cur[opp7(q) * No + i] = old[q * No + n];
+                cur[pidx<7>(opp7(q), i)] = old[pidx<7>(q, n)];
             }
         }
     }
@@ -1102,7 +1104,7 @@ __global__ void __launch_bounds__(256) k_ade_total(AdeDev<R> a, int ms, double *
             if (a.reset) { v = a.c0[s]; if (v < 1e-12 && v > -1e-12) v = 0; }
             else {
                 double fl[7];
-                pull7<R, double>(a, a.cur + (long long) ms * 7 * a.Nopen, a.out + (long long) ms * 7 * a.Nopen, i, a.lbt[s], a.rbt[s], fl);
+                pull7<R, double>(a, a.cur + (long long) ms * 7 * a.NoP, a.out + (long long) ms * 7 * a.NoP, i, a.lbt[s], a.rbt[s], fl);
                 for (int q = 0; q < 7; ++q) v += fl[q];
             }
         }
@@ -1122,16 +1124,16 @@ __device__ __forceinline__ void pull19(const NsDev<R> &a, int i, T fl[19]) {
     const long long No = a.Nopen;
     if (a.first) {
 #pragma unroll
-        for (int q = 0; q < 19; ++q) fl[q] = (T) a.cur[q * No + i];
+        for (int q = 0; q < 19; ++q) fl[q] = (T) a.cur[pidx<19>(q, i)];
         return;
     }
-    fl[0] = (T) a.cur[i];
+    fl[0] = (T) a.cur[pidx<19>(0, i)];
 #pragma unroll
     for (int q = 1; q < 19; ++q) {
         const int n = a.nb[(q - 1) * No + i];
-        if (n == NB_OUTSIDE) fl[q] = (T) a.cur[opp19(q) * No + i];
-        else if (n == NB_CLOSED || a.clsPull[n] >= NS_BB) fl[q] = (T) a.out[opp19(q) * No + i];
-        else fl[q] = (T) a.cur[q * No + n];
+        if (n == NB_OUTSIDE) fl[q] = (T) a.cur[pidx<19>(opp19(q), i)];
+        else if (n == NB_CLOSED || a.clsPull[n] >= NS_BB) fl[q] = (T) a.out[pidx<19>(opp19(q), i)];
+        else fl[q] = (T) a.cur[pidx<19>(q, n)];
     }
 }
 // the interior's pull from the precomputed source kinds: every neighbour
@@ -1141,7 +1143,7 @@ __device__ __forceinline__ void pull19k(const NsDev<R> &a, int i, T fl[19]) {
     const long long No = a.Nopen;
     if (a.first) {
 #pragma unroll
-        for (int q = 0; q < 19; ++q) fl[q] = (T) a.cur[q * No + i];
+        for (int q = 0; q < 19; ++q) fl[q] = (T) a.cur[pidx<19>(q, i)];
         return;
     }
     // the compact neighbour record: three 16-byte loads for all 18 indices
@@ -1164,12 +1166,12 @@ __device__ __forceinline__ void pull19k(const NsDev<R> &a, int i, T fl[19]) {
         for (int q = 1; q < 19; ++q)
             if ((far >> (q - 1)) & 1u) nbq[q - 1] = a.nb[(q - 1) * No + i];
     }
-    fl[0] = (T) a.cur[i];
+    fl[0] = (T) a.cur[pidx<19>(0, i)];
 #pragma unroll
     for (int q = 1; q < 19; ++q) {
         const unsigned int k = (unsigned int) (kd >> (2 * (q - 1))) & 3u;
         const R *base = k == 2 ? a.out : a.cur;
-        const long long off = k == 0 ? (long long) q * No + nbq[q - 1] : (long long) opp19(q) * No + i;
+        const long long off = k == 0 ? pidx<19>(q, nbq[q - 1]) : pidx<19>(opp19(q), i);
         fl[q] = (T) base[off];
     }
 }
@@ -1180,7 +1182,7 @@ __device__ __forceinline__ void nsFirstHandOver(const NsDev<R> &a, int i) {
 #pragma unroll
     for (int q = 1; q < 19; ++q) {
         const int n = a.nb[(q - 1) * No + i];
-        if (n == NB_CLOSED || (n >= 0 && a.clsPull[n] >= NS_BB)) a.cur[opp19(q) * No + i] = (R) 0;
+        if (n == NB_CLOSED || (n >= 0 && a.clsPull[n] >= NS_BB)) a.cur[pidx<19>(opp19(q), i)] = (R) 0;
     }
 }
 
@@ -1249,7 +1251,7 @@ __device__ __forceinline__ void nsBulkBlock(const NsDev<R> &a, double *partE, do
             const R om = (R) (cls == NS_BIO ? a.omBio : a.om);
             const long long No = a.Nopen;
 #pragma unroll
-            for (int q = 0; q < 19; ++q) a.out[q * No + i] = fl[q] + om * (feqInc<R>(q, rb, jx, jy, jz, jsq) - fl[q]);
+            for (int q = 0; q < 19; ++q) a.out[pidx<19>(q, i)] = fl[q] + om * (feqInc<R>(q, rb, jx, jy, jz, jsq) - fl[q]);
         }
     }
     blockEnergy(e, cnt, partE, partN, blk);
@@ -1274,7 +1276,7 @@ __device__ __forceinline__ void nsPlaneBlock(const NsDev<R> &a, double *partE, d
             e = 0.5 * jsq; cnt = 1;
             const double om = cls == NS_BIO ? a.omBio : a.om;
             const long long No = a.Nopen;
-            for (int q = 0; q < 19; ++q) a.out[q * No + i] = (R) (fl[q] + om * (feqInc<double>(q, rb, jx, 0.0, 0.0, jsq) - fl[q]));
+            for (int q = 0; q < 19; ++q) a.out[pidx<19>(q, i)] = (R) (fl[q] + om * (feqInc<double>(q, rb, jx, 0.0, 0.0, jsq) - fl[q]));
         }
     }
     blockEnergy(e, cnt, partE, partN, slot0 + b);
@@ -1451,8 +1453,8 @@ __global__ void k_ns_init(NsDev<R> a, double deltaP) {
     const bool closed = a.cls[i] >= NS_BB;
     for (int q = 0; q < 19; ++q) {
         const R v = closed ? (R) 0 : (R) (w19(q) * rb);
-        a.cur[q * (long long) a.Nopen + i] = v;
-        a.out[q * (long long) a.Nopen + i] = v;
+        a.cur[pidx<19>(q, i)] = v;
+        a.out[pidx<19>(q, i)] = v;
     }
 }
 
@@ -1478,11 +1480,11 @@ __global__ void k_ns_reopen(NsDev<R> a) {
     if (i >= a.Nopen) return;
     if (!(a.clsPull[i] >= NS_BB && a.cls[i] < NS_BB)) return;
     const long long No = a.Nopen;
-    a.cur[i] = (R) 0;
+    a.cur[pidx<19>(0, i)] = (R) 0;
     for (int q = 1; q < 19; ++q) {
         const int n = a.nb[(q - 1) * No + i];
-        if (n < 0 || a.clsPull[n] >= NS_BB) { a.out[opp19(q) * No + i] = (R) 0; a.cur[opp19(q) * No + i] = (R) 0; }
-        else a.cur[opp19(q) * No + i] = a.out[q * No + n];
+        if (n < 0 || a.clsPull[n] >= NS_BB) { a.out[pidx<19>(opp19(q), i)] = (R) 0; a.cur[pidx<19>(opp19(q), i)] = (R) 0; }
+        else a.cur[pidx<19>(opp19(q), i)] = a.out[pidx<19>(q, n)];
     }
 }
 

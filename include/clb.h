@@ -338,7 +338,8 @@ int  clb_species_totals_noalloc(clb_ctx *ctx, int case_id, double *out);
  * it.  An immobile species: 0 (nothing streams).  A microbe: its biomass in the
  * x = 0 and x = nx-1 planes.  out has 4 * (n_species + n_microbes) entries. */
 int  clb_field_stats(clb_ctx *ctx, int case_id, double *out);
-/* counters: [0] open voxels, [1] biofilm voxels, [2] clamp binds,
+/* counters: [0] open voxels, [1] biofilm voxels, [2] clamp binds (debits the
+ * clamp capped that exceeded the availability by more than 1e-12 of it),
  * [3] equilibrium non-converged voxels, [4] flow re-solves, [5] mask changes,
  * [6] CA sweeps, [7] steps run, [8] dissolution product shares parked,
  * [9] shares collected (equal to [8]: the deposit invariant of

@@ -403,13 +403,18 @@ void abioticLaw(const clb_case_params &p, const double *C, double *subsR) {
 }
 
 /* the clamp dC = max(R dt, -C) (README.md:87-88; symbolic processor 161-176),
- * only for a debit and only against what the voxel holds */
+ * only for a debit and only against what the voxel holds.  The bind counter
+ * counts debits beyond the availability by more than 1e-12 of it: a debit that
+ * equals the availability but for rounding (the dissolution gather's own
+ * positivity rule leaves exactly such debits) is capped but not counted. */
 double clampInc(Case &c, double dC, double cur) {
     if (!c.p.clamp || dC >= 0) return dC;
     double lim = -std::max(cur, 0.0);
     if (dC < lim) {
+        if (dC < lim - 1e-12 * std::fabs(lim)) {
 #pragma omp atomic
-        c.counters[2]++;
+            c.counters[2]++;
+        }
         return lim;
     }
     return dC;
@@ -1572,6 +1577,8 @@ int orc_run_steps(void *h, int n, int *done) {
 int orc_ade_steps(void *h, int n) { Case &c = *(Case *) h; for (int k = 0; k < n; ++k) transportStep(c); return 0; }
 int orc_equilibrium(void *h) { equilibriumStep(*(Case *) h); return 0; }
 int orc_ca_spill(void *h, int *sweeps) { return caBlock(*(Case *) h, sweeps); }
+/* one push + gather sweep of the CA rule, without the mask/total update */
+int orc_ca_sweep(void *h) { caSweep(*(Case *) h); return 0; }
 int orc_precip_convert(void *h) { return precipConvert(*(Case *) h) ? 1 : 0; }
 
 int orc_read_field(void *h, int field, int index, double *dst) {

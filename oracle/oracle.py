@@ -48,6 +48,7 @@ def lib():
             ("orc_ade_steps", [P, C.c_int]),
             ("orc_equilibrium", [P]),
             ("orc_ca_spill", [P, P]),
+            ("orc_ca_sweep", [P]),
             ("orc_precip_convert", [P]),
             ("orc_age_relax", [P, C.c_int, P]),
             ("orc_params_layout", [P]),
@@ -76,7 +77,8 @@ def lib():
 
 
 def ref():
-    """The reference's own code (libref.so) or None when it was not built."""
+    """The reference's equilibrium solver built against the Palabos stub of its
+    own tests (libref.so), or None when it was not built: a cross-check."""
     global _ref
     if _ref is None:
         path = os.path.join(HERE, "_ref", "libref.so")
@@ -86,12 +88,29 @@ def ref():
         P = C.c_void_p
         L.ref_eq_solve.argtypes = [C.c_int, C.c_int, P, P, P, C.c_double, C.c_int, C.c_int, P, P, P, P, P]
         L.ref_eq_voxel.argtypes = [C.c_int, C.c_int, P, P, P, C.c_double, C.c_int, C.c_int, P, P]
+        _ref = L
+    return _ref
+
+
+_ref_geom = None
+
+
+def ref_geom():
+    """The reference's geometry generator compiled from its own source with
+    nothing standing in for anything (libref_geom.so), or None."""
+    global _ref_geom
+    if _ref_geom is None:
+        path = os.path.join(HERE, "_ref", "libref_geom.so")
+        if not os.path.exists(path):
+            return None
+        L = C.CDLL(path)
+        P = C.c_void_p
         L.ref_rng_draws.argtypes = [C.c_uint64, C.c_int, P]
         L.ref_geom_generate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
                                         C.c_uint64, C.c_char_p, P]
         L.ref_geom_inspect.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P, P, P, P]
-        _ref = L
-    return _ref
+        _ref_geom = L
+    return _ref_geom
 
 
 def _ptr(a):
@@ -168,6 +187,10 @@ class OracleCase:
 
     def equilibrium(self):
         self._chk(lib().orc_equilibrium(self.h))
+
+    def ca_sweep(self):
+        """One push + gather sweep (no mask/total update)."""
+        self._chk(lib().orc_ca_sweep(self.h))
 
     def ca_spill(self):
         sw = C.c_int()
@@ -304,13 +327,13 @@ def ref_eq_voxel(params, conc):
 
 def ref_rng_draws(seed, n):
     out = np.zeros(n, dtype=np.uint64)
-    ref().ref_rng_draws(seed, n, _ptr(out))
+    ref_geom().ref_rng_draws(seed, n, _ptr(out))
     return out
 
 
 def ref_geom_generate(nx, ny, nz, kind, radius, count, porosity, seed, walls):
     out = np.zeros(nx * ny * nz, dtype=np.int8)
-    rc = ref().ref_geom_generate(nx, ny, nz, kind, radius, count, porosity, seed, walls.encode(), _ptr(out))
+    rc = ref_geom().ref_geom_generate(nx, ny, nz, kind, radius, count, porosity, seed, walls.encode(), _ptr(out))
     assert rc == 0
     return out.reshape(nz, ny, nx)
 
@@ -319,5 +342,5 @@ def ref_geom_inspect(codes):
     nz, ny, nx = codes.shape
     a = np.ascontiguousarray(codes, dtype=np.int8)
     phi = C.c_double(); op = C.c_long(); cl = C.c_int(); sp = C.c_int(); pc = C.c_int()
-    ref().ref_geom_inspect(nx, ny, nz, _ptr(a), C.byref(phi), C.byref(op), C.byref(cl), C.byref(sp), C.byref(pc))
+    ref_geom().ref_geom_inspect(nx, ny, nz, _ptr(a), C.byref(phi), C.byref(op), C.byref(cl), C.byref(sp), C.byref(pc))
     return dict(porosity=phi.value, open=op.value, clusters=cl.value, spanning=sp.value, percolates=bool(pc.value))

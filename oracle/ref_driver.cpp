@@ -1,20 +1,20 @@
 /*
- * ref_driver.cpp -- TEST INFRASTRUCTURE.  Builds the reference's own
- * self-contained sources (the equilibrium solver of
- * src/complab3d_processors_part4_eqsolver.hh and the geometry generator of
- * src/complab3d_geometry.hh) against the reference's own Palabos stub
- * (tests/palabos_stub.hh), exactly as the reference's tests/run_tests.sh
- * compiles its processors, and exposes them through extern "C" so the tests
- * can pin the oracle against the real reference code.  Built into
- * oracle/_ref/ by oracle/Makefile from the sources where they lie; nothing of
- * the reference is copied into this repository.
+ * ref_driver.cpp -- TEST INFRASTRUCTURE.  Builds the reference's equilibrium
+ * solver (src/complab3d_processors_part4_eqsolver.hh) against the Palabos
+ * stub the reference's own tests use (tests/palabos_stub.hh), as its
+ * tests/run_tests.sh compiles its processors, and exposes it through
+ * extern "C".  A build against a stand-in for Palabos is a cross-check of the
+ * oracle's solver, not a pin of it: the oracle's speciation is pinned by the
+ * mathematics (closed forms, mass balance and mass action at the stated
+ * tolerances, tests/test_oracle_golden.py).  Built into oracle/_ref/ by
+ * oracle/Makefile from the sources where they lie; nothing of the reference is
+ * copied into this repository.
  */
 #include "palabos_stub.hh"
 
 namespace plb { PlbOut pcout; }
 
 #include "complab3d_processors_part4_eqsolver.hh"
-#include "complab3d_geometry.hh"
 
 #include <cstdint>
 #include <string>
@@ -76,40 +76,6 @@ int ref_eq_voxel(int ns, int nc, const double *stoich, const double *logk, const
     Box3D box = {1, 1, 0, 0, 0, 0};
     proc.process(box, ptr);
     for (int i = 0; i < ns; ++i) cout[i] = lat[i].c.computeDensity();
-    return 0;
-}
-
-/* complab_geom::Rng draws */
-int ref_rng_draws(uint64_t seed, int n, uint64_t *out) {
-    complab_geom::Rng r(seed);
-    for (int k = 0; k < n; ++k) out[k] = r.next();
-    return 0;
-}
-
-/* complab_geom::generate; kind 0 spheres, 1 random, 2 cylinders, 3 fracture.
- * out: nx*ny*nz codes, index x + nx*(y + ny*z) */
-int ref_geom_generate(int nx, int ny, int nz, int kind, double radius, int count, double porosity,
-                      uint64_t seed, const char *walls, int8_t *out) {
-    complab_geom::GenOptions o;
-    const char *kinds[4] = {"spheres", "random", "cylinders", "fracture"};
-    if (kind < 0 || kind > 3) return -1;
-    o.kind = kinds[kind];
-    o.radius = radius; o.count = count; o.porosity = porosity; o.seed = seed;
-    o.walls = walls ? walls : "none";
-    complab_geom::Grid g = complab_geom::generate(nx, ny, nz, o);
-    if (g.size() == 0) return -1;
-    for (size_t i = 0; i < g.size(); ++i) out[i] = (int8_t) g.v[i];
-    return 0;
-}
-
-/* complab_geom::inspect: porosity, open voxels, clusters, spanning clusters, percolates */
-int ref_geom_inspect(int nx, int ny, int nz, const int8_t *codes, double *porosity, long *open,
-                     int *clusters, int *spanning, int *percolates) {
-    complab_geom::Grid g(nx, ny, nz, 0);
-    for (size_t i = 0; i < g.size(); ++i) g.v[i] = codes[i];
-    complab_geom::Inspection R = complab_geom::inspect(g);
-    *porosity = R.porosity; *open = R.openVoxels; *clusters = R.clusters;
-    *spanning = R.spanningClusters; *percolates = R.percolates ? 1 : 0;
     return 0;
 }
 

@@ -96,3 +96,64 @@ def test_foreign_checkpoints_are_refused(ctx):
     a.restore(blob)                                                    # its own loads
     a.close()
     b.close()
+
+
+def restarted_against_oracle(ctx, w, before, after, rel=1e-10):
+    """The device runs `before` steps, checkpoints, and a fresh device case
+    loaded from that block runs `after` more; the oracle runs the same
+    before + after steps without interruption.  The restarted run must match
+    it as an uninterrupted device run does: fields within `rel`, masks, ages,
+    phases and counters exact."""
+    import oracle as O
+    a = clb.Case(ctx, w.params, w.geometry)
+    o = O.OracleCase(w.params, w.geometry)
+    ra = driver.setup(a, w.params, w.flow, w.transport)
+    ro = driver.setup(o, w.params, w.flow, w.transport)
+    assert ra.iters == ro.iters
+    for c in (a, o):
+        c.set_omegas(ro.omega_pore, ro.omega_biofilm, 1.0)
+    assert a.run_steps(before) == (0, before)
+    blob = a.checkpoint()
+    a.close()
+    b = clb.Case(ctx, w.params, w.geometry)
+    b.restore(blob)
+    assert b.run_steps(after) == (0, after)
+    assert o.run_steps(before + after) == (0, before + after)
+    for f in (P.FIELD_MASK, P.FIELD_AGE, P.FIELD_PHASE):
+        assert np.array_equal(b.read(f), o.read(f)), f
+    cb, co = b.counters(), o.counters()
+    print("restarted counters", list(cb), "oracle", list(co), flush=True)
+    for k in range(2, 12):
+        assert cb[k] == co[k], (k, cb[k], co[k])
+    om = o.read(P.FIELD_MASK) >= 2
+    om[:, :, 0] = False; om[:, :, -1] = False
+    for s in range(w.params.n_species):
+        x, y = b.read(P.FIELD_CONC, s), o.read(P.FIELD_CONC, s)
+        sel = om if w.params.mobile[s] else np.ones_like(om)
+        e = float(np.max(np.abs(x[sel] - y[sel])) / max(np.max(np.abs(y[sel])), 1e-300))
+        assert e <= rel, (s, e)
+    for m in range(w.params.n_microbes):
+        x, y = b.read(P.FIELD_BIOMASS, m), o.read(P.FIELD_BIOMASS, m)
+        assert np.max(np.abs(x - y)) <= rel * max(np.max(np.abs(y)), 1e-300)
+    b.close(); o.close()
+    return cb
+
+
+def test_restarted_biotic_case_matches_uninterrupted_oracle(ctx):
+    print("case: biotic CA 0.12-scale pack, restart after 25 steps, 35 more, against the oracle", flush=True)
+    w = configs.config2(scale=0.12)
+    w.transport.stabilize_steps = 30
+    w.params.ns_update_interval = 5
+    w.params.ns_max_iter = 3000
+    w.params.b0[0] = 95.0
+    w.params.mu_max[0] = 2e-2
+    c = restarted_against_oracle(ctx, w, 25, 35)
+    assert c[6] > 0 and c[5] > 0 and c[4] > 0               # sweeps, mask changes, re-solves
+
+
+def test_restarted_precipitation_case_matches_uninterrupted_oracle(ctx):
+    print("case: carbonate-calcium 1/16 scale, restart after 400 steps, 400 more, against the oracle", flush=True)
+    w = configs.config3(scale=1 / 16)
+    w.flow.eps = 1e-6
+    c = restarted_against_oracle(ctx, w, 400, 400)
+    assert c[4] > 0 and c[8] > 0                            # re-solves, dissolution shares

@@ -429,3 +429,89 @@ def test_threads_do_not_change_results():
         assert np.array_equal(x, y)
     for k in (2, 3, 4, 5):
         assert np.array_equal(a[k], b[k])
+
+
+def ca_two_microbe_case(method, seed):
+    """A 9x7x7 box (walls on y and z), two CA microbes, random biomass with
+    some voxels above the cap; ages random so the age fallback varies."""
+    nx, ny, nz = 9, 7, 7
+    geom = np.full((nz, ny, nx), P.PORE, np.int8)
+    geom[:, 0, :] = P.WALL; geom[:, -1, :] = P.WALL; geom[0, :, :] = P.WALL; geom[-1, :, :] = P.WALL
+    rng = np.random.default_rng(seed)
+    geom[1:-1, 1:-1, 1:-1][rng.uniform(size=(nz - 2, ny - 2, nx - 2)) < 0.15] = P.WALL
+    p = P.default_params(nx, ny, nz, 1)
+    p.n_microbes = 2
+    p.bmax = 10.0
+    p.thrd_bfilm_frac = 0.1
+    p.ca_method = method
+    p.ca_seed = 77 + seed
+    openv = geom >= 2
+    inner = openv.copy(); inner[:, :, 0] = False; inner[:, :, -1] = False
+    b = [np.where(inner, rng.uniform(0, 6, geom.shape), 0.0) for _ in range(2)]
+    hot = inner & (rng.uniform(size=geom.shape) < 0.3)
+    b[0][hot] += rng.uniform(3, 12, hot.sum())
+    b[1][hot] += rng.uniform(3, 12, hot.sum())
+    geom[hot & (rng.uniform(size=geom.shape) < 0.7)] = P.BIOFILM     # most of the full voxels are biofilm
+    o = O.OracleCase(p, geom)
+    for m in range(2):
+        o.write(P.FIELD_BIOMASS, m, b[m])
+    o.write(P.FIELD_AGE, 0, np.where(openv, rng.integers(0, 4, geom.shape), -1).astype(float))
+    return p, geom, o, b
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_ca_fraction_sweep_moves_the_excess_in_species_shares(seed):
+    """One sweep of the fraction rule (pushExcessBiomass3D, part1:500-732):
+    every over-full voxel with an open face neighbour inside the CA box ends
+    at exactly the cap; the amount that leaves is sum (B - Bmax)+ over those
+    voxels; each microbe leaves in its share B_m / B_tot of the donor (so
+    B_m ends at B_m Bmax / B_tot); what leaves arrives in open face
+    neighbours only, and the biomass of every microbe is conserved."""
+    p, geom, o, b = ca_two_microbe_case(P.CA_FRACTION, seed)
+    tot = b[0] + b[1]
+    o.ca_sweep()
+    a = [o.read(P.FIELD_BIOMASS, m) for m in range(2)]
+    for m in range(2):
+        assert abs(a[m].sum() - b[m].sum()) <= 1e-12 * b[m].sum()
+        assert np.all(a[m][geom < 2] == 0)
+    nz, ny, nx = geom.shape
+    donors = np.zeros(geom.shape, bool)
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(1, nx - 1):
+                if not (tot[z, y, x] > p.bmax and tot[z, y, x] - p.bmax > 1e-12):
+                    continue
+                nbs = [(x + 1, y, z), (x - 1, y, z), (x, y + 1, z), (x, y - 1, z), (x, y, z + 1), (x, y, z - 1)]
+                gaps = [x == nx - 2, x == 1, y == ny - 1, y == 0, z == nz - 1, z == 0]
+                if any(not g and geom[Z, Y, X] >= 2 for (X, Y, Z), g in zip(nbs, gaps)):
+                    donors[z, y, x] = True
+    assert donors.sum() > 5
+    moved = np.sum(tot[donors] - p.bmax)
+    # a donor receives from over-full neighbours too: its outflow is B - Bmax
+    # exactly, so its own share ratio fixes what it kept before the gather
+    kept = [b[m] * p.bmax / np.where(tot > 0, tot, 1.0) for m in range(2)]
+    gained = [a[m] - np.where(donors, kept[m], b[m]) for m in range(2)]
+    assert np.all(gained[0] >= -1e-12) and np.all(gained[1] >= -1e-12)
+    assert abs(gained[0].sum() + gained[1].sum() - moved) <= 1e-10 * moved
+    for m in range(2):
+        out_m = np.sum(b[m][donors] - kept[m][donors])
+        assert abs(out_m - np.sum(gained[m])) <= 1e-10 * moved
+
+
+def test_ca_half_sweep_keeps_half_in_biofilm_donors():
+    """The half rule (halfPushExcessBiomass3D, part1:735-970): an over-full
+    biofilm voxel sends half of its total away; pore voxels never push."""
+    p, geom, o, b = ca_two_microbe_case(P.CA_HALF, 9)
+    m0 = o.read(P.FIELD_MASK)
+    tot = b[0] + b[1]
+    o.ca_sweep()
+    a = [o.read(P.FIELD_BIOMASS, m) for m in range(2)]
+    for m in range(2):
+        assert abs(a[m].sum() - b[m].sum()) <= 1e-12 * b[m].sum()
+    donors = (tot > p.bmax) & (m0 >= P.BIOFILM)
+    pore_over = (tot > p.bmax) & (m0 == P.PORE)
+    after = a[0] + a[1]
+    # a donor keeps half unless a neighbour sent it something
+    assert np.sum(donors) > 0
+    assert np.all(after[donors] >= 0.5 * tot[donors] - 1e-12)
+    assert np.all(after[pore_over] >= tot[pore_over] - 1e-12)
