@@ -1,12 +1,18 @@
 """Writes the golden fixtures of tests/golden from the reference's own code.
 
-Runs in a container that holds the CompLaB3D-GEMS tree at /root/reference:
-`make -C oracle ref` compiles the reference's equilibrium solver
-(src/complab3d_processors_part4_eqsolver.hh) and geometry generator
-(src/complab3d_geometry.hh) against the reference's own Palabos stub
-(tests/palabos_stub.hh) into oracle/_ref/libref.so; this script calls that
-library on seeded inputs and stores inputs and outputs as .npz data.  The ABC
-sweep geometry fixtures are read from the reference tree as data.
+Runs in a container that holds the CompLaB3D-GEMS tree at /root/reference.
+`make -C oracle ref` compiles from the sources where they lie:
+  * the geometry generator (src/complab3d_geometry.hh, standard library only)
+    into oracle/_ref/libref_geom.so -- the reference compiled here: the
+    geometry fixtures are its outputs;
+  * the equilibrium solver (src/complab3d_processors_part4_eqsolver.hh)
+    against the Palabos stub of the reference's own tests into
+    oracle/_ref/libref.so -- a build on a stand-in for Palabos, so the
+    equilibrium fixtures are a cross-check, not a pin (the oracle's
+    speciation is pinned by tests/test_oracle_eq_laws.py).
+This script calls those libraries on seeded inputs and stores inputs and
+outputs as .npz data.  The ABC sweep geometry fixtures are read from the
+reference tree as data.
 """
 import json
 import os
@@ -85,8 +91,8 @@ def geometry_fracture():
 
 
 def main():
-    if O.ref() is None:
-        raise SystemExit("oracle/_ref/libref.so is missing: run `make -C oracle ref` where the reference tree exists")
+    if O.ref() is None or O.ref_geom() is None:
+        raise SystemExit("oracle/_ref is missing: run `make -C oracle ref` where the reference tree exists")
     # ---- equilibrium: carbonate (2 components) and calcium carbonate (6 components)
     p = P.default_params(3, 1, 1, 4)
     carbonate(p)
