@@ -71,6 +71,7 @@ def lib():
         "clb_write_field": ([V, I, I, I, V], I),
         "clb_species_totals": ([V, I, V], I),
         "clb_species_totals_noalloc": ([V, I, V], I),
+        "clb_set_tuning": ([V, I, I, I], I),
         "clb_field_stats": ([V, I, V], I),
         "clb_counters": ([V, I, V], I),
         "clb_kernel_time": ([V, I, I, C.POINTER(D), C.POINTER(C.c_int64)], I),
@@ -107,7 +108,7 @@ def exported_symbols():
         "clb_set_symbolic", "clb_rate_capture", "clb_checkpoint_size", "clb_checkpoint_save", "clb_checkpoint_load",
         "clb_stream_copy",
         "clb_rng_draws", "clb_geom_spheres", "clb_geom_pack", "clb_geom_generate",
-        "clb_species_totals_noalloc",
+        "clb_species_totals_noalloc", "clb_set_tuning",
     ]
 
 
@@ -333,6 +334,10 @@ class Case:
     def load_checkpoint(self, path):
         with open(path, "rb") as f:
             self.restore(f.read())
+
+    def set_tuning(self, knob, value):
+        """A launch shape of the hot kernels (P.TUNE_*); results do not change."""
+        self._c(lib().clb_set_tuning(self.ctx.h, self.id, int(knob), int(value)))
 
     def set_timing_stride(self, stride):
         """Time only every stride-th transport launch (event markers cost the

@@ -143,12 +143,18 @@ struct NsDev {
     const int8_t *geom;            // dense geometry codes (the pore-space statistics)
     const int *nb;                 // [18][Nopen]
     R *cur, *out;                  // [19 x NoP] tiled (pidx<19>)
+    // the dense bounce-back stores paired with cur and out (nsBBSlot): the
+    // interior pull reads the populations that come back from closed
+    // neighbours from bbOut and writes the new ones there
+    R *bbCur, *bbOut;              // [popPad(Nint) x 18]
     const uint8_t *cls, *clsPull;  // [Nopen]
     const unsigned long long *kinds;  // [Nint] source kind of each moving population, 2 bits per direction
     const int4 *nbr;               // [Nint][3] compact neighbour record (NsNbRecord), the interior's pull
     double om, omBio, rhoBarRight;
     int first;
     int *done;
+    int shape;                     // interior launch shape: 0 a block per chunk of 256 voxels,
+                                   // > 0 that many blocks looping over chunks, loads a chunk ahead
 };
 
 // z component of D3Q19 direction q (the order of D19C in clb_common.h), for
@@ -248,10 +254,13 @@ template <typename R, typename RA> hipError_t launchNsVelocity(const NsDev<R> &,
 template <typename R> hipError_t launchNsInit(const NsDev<R> &, double, hipStream_t);
 hipError_t launchNsUpdateDyn(const int8_t *, const int *, uint8_t *, int, double, unsigned int *, hipStream_t);
 template <typename R> hipError_t launchNsReopen(const NsDev<R> &, const uint8_t *, hipStream_t);
+// the dense bounce-back store `bb` rebuilt from the populations `g` it pairs
+// with, for the current pull kinds
+template <typename R> hipError_t launchNsBBFill(const NsDev<R> &, const R *g, R *bb, hipStream_t);
 template <typename R> hipError_t launchEquilibrium(const AdeDev<R> &, const EqDev &, unsigned int *, hipStream_t);
 template <typename R> hipError_t launchScatterCompact(const R *, const int *, int, double *, hipStream_t);
 template <typename R> hipError_t launchGatherCompact(const double *, const int *, int, R *, hipStream_t);
-unsigned int nsStepBlocks(int Nopen, int Nint);
+unsigned int nsStepBlocks(int Nopen, int Nint, int shape);
 // host launchers: dense grids (clb_aux.hip)
 template <typename R> hipError_t launchCaMaskTotal(const CaDev<R> &, int check, hipStream_t);
 template <typename R> hipError_t launchCaMaskFinal(const CaDev<R> &, int nparts, int check, hipStream_t);

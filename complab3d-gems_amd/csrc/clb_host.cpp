@@ -242,6 +242,7 @@ struct CaseBase {
     virtual void counters(hipStream_t, int64_t out[CLB_NCOUNTERS]) = 0;
     virtual void kernelTime(hipStream_t, int reset, double *ms, int64_t *n) = 0;
     virtual void setTimeStride(int stride) = 0;
+    virtual Status setTuning(int knob, int value) = 0;
     virtual void setRelaxation(const double *omp, const double *omb, double dt) = 0;
     // the processors one at a time (host facade of the reference's processor API)
     virtual void adeCollideStream(hipStream_t, int n) = 0;
@@ -291,6 +292,7 @@ struct Case : CaseBase {
     int *age = nullptr, *age2 = nullptr, *ageList = nullptr;
     unsigned int *ageCnt = nullptr;             // [0] biomass voxels, [1..8] voxels raised per queued sweep
     RN *gA = nullptr, *gB = nullptr;
+    RN *bbA = nullptr, *bbB = nullptr;          // dense bounce-back stores paired with gA / gB
     R *u = nullptr;
     double *rhoN = nullptr, *parts = nullptr, *scal = nullptr, *stage = nullptr;
     long long *partsLL = nullptr, *scalLL = nullptr;
@@ -307,6 +309,7 @@ struct Case : CaseBase {
     double nsOmega = 1, nsBioOmega = 1, deltaP = 0;
     long long iT = 0;
     int nsSaturate = 0;
+    int nsShape = 0;                            // flow interior launch shape (NsDev::shape; clb_set_tuning)
     int clogged = 0;                            // percolationFlag: the run stopped for good
     bool resolveFresh = true;                   // the in-step re-solves' tracer is still empty
     long long oldTotMask = 0;
@@ -436,10 +439,12 @@ struct Case : CaseBase {
         a.nx = nx; a.ny = ny; a.nz = nz; a.N = N;
         a.Nopen = Nopen; a.Nint = Nint; a.Nleft = Nleft; a.NoP = popPad(Nopen); a.list = list; a.nb = nb19; a.geom = geom;
         a.cur = gCur(); a.out = gOut();
+        a.bbCur = gCurIsA ? bbA : bbB; a.bbOut = gCurIsA ? bbB : bbA;
         a.cls = nsCls; a.clsPull = nsClsPull; a.kinds = nsKinds; a.nbr = (const int4 *) nsNbr;
         a.om = nsOmega; a.omBio = nsBioOmega; a.rhoBarRight = -deltaP * 3.0;
         a.first = nsFirst ? 1 : 0;
         a.done = nsDone;
+        a.shape = nsShape;
         return a;
     }
     CaDev<R> ca() const {
@@ -585,6 +590,7 @@ struct Case : CaseBase {
         ageCnt = dalloc<unsigned int>(allocs, 32);
         if (p.flow_on) {
             gA = dalloc<RN>(allocs, 19 * (size_t) popPad(Nopen)); gB = dalloc<RN>(allocs, 19 * (size_t) popPad(Nopen));
+            bbA = dalloc<RN>(allocs, 18 * (size_t) std::max(popPad(Nint), 64LL)); bbB = dalloc<RN>(allocs, 18 * (size_t) std::max(popPad(Nint), 64LL));
             nb19 = dalloc<int>(allocs, (size_t) 18 * Nopen);
             nsNbr = dalloc<NsNbRecord>(allocs, std::max(Nint, 1));
             nsKinds = dalloc<unsigned long long>(allocs, std::max(Nint, 1));
@@ -841,10 +847,20 @@ struct Case : CaseBase {
         kms = 0; klaunch = 0;
         nearDirty = true;
         symMask = h.symMask; symOn = (h.symOn & 1u) != 0; symAb = (h.symOn & 2u) != 0;
+        nsBBRebuild(st);                                 // derived from the loaded populations
+        HIPT(hipStreamSynchronize(st));
         return {CLB_OK, ""};
     }
 
     // ------------------------------------------------------------------ flow
+    // both dense bounce-back stores rebuilt from their population buffers:
+    // after the pull kinds change, voxels reopen, or the populations are set
+    void nsBBRebuild(hipStream_t st) {
+        if (!p.flow_on) return;
+        NsDev<RN> a = ns();
+        HIPT(launchNsBBFill<RN>(a, gA, bbA, st));
+        HIPT(launchNsBBFill<RN>(a, gB, bbB, st));
+    }
     void nsInit(hipStream_t st, double dP) override {
         if (!p.flow_on) return;
         deltaP = dP;
@@ -861,6 +877,7 @@ struct Case : CaseBase {
         nsBioOmega = 1.0 / (p.perm_ratio * (1.0 / nsOmega - 0.5) + 0.5);
         gCurIsA = true;
         HIPT(launchNsInit<RN>(ns(), dP, st));
+        nsBBRebuild(st);
         nsFirst = true;
         nsClsDirty = false;
         HIPT(hipStreamSynchronize(st));
@@ -877,7 +894,7 @@ struct Case : CaseBase {
         long long base = th.iters;
         th.converged = 0; th.nonfinite = 0;
         HIPT(hipMemcpyAsync(tracer, &th, sizeof(NsTracer), hipMemcpyHostToDevice, st));
-        const unsigned int g = nsStepBlocks(Nopen, Nint);
+        const unsigned int g = nsStepBlocks(Nopen, Nint, nsShape);
         double *partE = parts, *partN = parts + g;
         int done = 0;
         long long ran = 0;
@@ -889,11 +906,13 @@ struct Case : CaseBase {
             for (int k = 0; k < batch; ++k) {
                 NsDev<RN> a = ns();
                 a.cur = curA ? gA : gB; a.out = curA ? gB : gA;
+                a.bbCur = curA ? bbA : bbB; a.bbOut = curA ? bbB : bbA;
                 HIPT(launchNsStep<RN>(a, partE, partN, tracer, rule, eps, st));
                 nsFirst = false;
                 if (!launched && nsClsDirty) {
                     HIPT(hipMemcpyAsync(nsClsPull, nsCls, Nopen, hipMemcpyDeviceToDevice, st));
                     HIPT(launchNsKinds(nb19, nsClsPull, Nopen, Nint, nsKinds, st));
+                    nsBBRebuild(st);
                     nsClsDirty = false;
                 }
                 launched = true;
@@ -944,6 +963,7 @@ struct Case : CaseBase {
         HIPT(hipStreamSynchronize(st));
         if (ch) {
             HIPT(launchNsReopen<RN>(ns(), nullptr, st));
+            nsBBRebuild(st);
             nsClsDirty = true;
         }
     }
@@ -1530,6 +1550,14 @@ struct Case : CaseBase {
         out[8] = (int64_t) dc[0]; out[9] = (int64_t) dc[1]; out[10] = (int64_t) dc[2];
     }
     void setTimeStride(int stride) override { timeStride = stride; }
+    Status setTuning(int knob, int value) override {
+        if (knob == CLB_TUNE_NS_SHAPE) {
+            if (value < 0) return {CLB_E_ARG, "flow launch shape must be >= 0"};
+            nsShape = value;
+            return {CLB_OK, ""};
+        }
+        return {CLB_E_ARG, "unknown tuning knob"};
+    }
     void kernelTime(hipStream_t, int reset, double *ms, int64_t *n) override {
         if (ms) *ms = klaunch ? kms / (double) klaunch : 0.0;
         if (n) *n = klaunch;
@@ -1684,7 +1712,7 @@ size_t clb_case_footprint(const clb_case_params *p) {
     b += (size_t) std::max(p->n_microbes, 1) * N * r + N * r;
     if (p->n_microbes > 0) b += (size_t) p->n_microbes * 6 * N * r + N;
     b += 3 * N * sizeof(int);                           // ages (two) and the biomass voxel list
-    if (p->flow_on) b += 2 * 19 * NP * rn + N * (18 * sizeof(int) + sizeof(NsNbRecord) + sizeof(unsigned long long));
+    if (p->flow_on) b += 2 * 19 * NP * rn + 2 * 18 * NP * rn + N * (18 * sizeof(int) + sizeof(NsNbRecord) + sizeof(unsigned long long));
     if (p->dissol_on) b += (size_t) std::max(p->n_species, 1) * N * sizeof(double) + N * sizeof(int);
     b += 3 * N * r;
     b += N * sizeof(double);                            // staging
@@ -1902,6 +1930,14 @@ int clb_write_field(clb_ctx *ctx, int case_id, int field, int index, const doubl
     if (!src) return fail(ctx, CLB_E_ARG, "null source");
     Status s = c->writeField(ctx->stream, field, index, src);
     return s.code ? fail(ctx, s.code, s.msg) : CLB_OK;
+    GUARD_END
+}
+int clb_set_tuning(clb_ctx *ctx, int case_id, int knob, int value) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    Status r = c->setTuning(knob, value);
+    if (r.code) return fail(ctx, r.code, r.msg);
+    return CLB_OK;
     GUARD_END
 }
 int clb_species_totals(clb_ctx *ctx, int case_id, double *out) {

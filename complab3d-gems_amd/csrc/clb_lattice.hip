@@ -544,6 +544,165 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
     }
 }
 
+// the kinetics of the fused transport kernel: the increments of
+// voxelKinetics<R, MAXS, false, false> bit for bit (the same operations in
+// the same order), with each species' rate formed as it is deposited instead
+// of in arrays over every species, so that the populations are about all the
+// registers the kernel keeps live.  The thermodynamic gate reads the
+// concentrations through `conc`.
+template <typename F>
+__device__ __forceinline__ double thermoGateOf(const KinDev &k, F conc) {
+    double lnQ = 0.0;
+#pragma unroll
+    for (int s = 0; s < CLB_DEV_MAXS; ++s) {
+        if (s >= k.S || k.thmNu[s] == 0.0) continue;
+        double a = conc(s) * k.thmScale;
+        if (!(a > k.thmCmin)) a = k.thmCmin;
+        lnQ += k.thmNu[s] * log(a);
+    }
+    const double dG = k.thmDG0 + k.thmRT * lnQ + k.thmLoss;
+    const double x = (dG + k.thmThr) / ((k.thmChi > 0.0 ? k.thmChi : 1.0) * k.thmRT);
+    if (x >= 0.0) return 0.0;
+    if (x < -700.0) return 1.0;
+    const double Fv = 1.0 - exp(x);
+    return Fv < 0.0 ? 0.0 : (Fv > 1.0 ? 1.0 : Fv);
+}
+template <typename R, int MAXS>
+__device__ __forceinline__ void voxelKineticsFused(const AdeDev<R> &a, const KinDev &k, int i, R (&f)[MAXS][7],
+                                                   long long v, int mask, R (&bpre)[CLB_DEV_MAXM]) {
+    if (mask == 0 || mask == 1) return;
+    const long long No = a.Nopen;
+    unsigned int binds = 0;
+    const int S = k.S;
+    const long long N = a.N;
+    auto concOf = [&](int s) -> double {
+        const int sl = a.slot[s];
+        if (a.mobile[s]) {
+            double r = 0;
+#pragma unroll
+            for (int m2 = 0; m2 < MAXS; ++m2) if (m2 == sl) {
+#pragma unroll
+                for (int q = 0; q < 7; ++q) r += (double) f[m2][q];
+            }
+            return r;
+        }
+        return (double) a.imm[(long long) sl * N + v];
+    };
+    auto depositTo = [&](int s, double d) {
+        const int sl = a.slot[s];
+        if (a.mobile[s]) {
+#pragma unroll
+            for (int m2 = 0; m2 < MAXS; ++m2) if (m2 == sl) {
+#pragma unroll
+                for (int q = 0; q < 7; ++q) f[m2][q] = (R) ((double) f[m2][q] + w7(q) * d);
+            }
+        } else {
+            R *p = a.imm + (long long) sl * N + v;
+            *p = (R) ((double) *p + d);
+        }
+    };
+    auto thr = [](double c) { return c < 1e-12 ? 0.0 : c; };
+    if (k.bio && k.M > 0) {
+        // bioticLaw: the growth and net biomass rate of each microbe
+        double g[CLB_DEV_MAXM], br[CLB_DEV_MAXM];
+        const double Sx = k.donor >= 0 ? fmax(thr(concOf(k.donor)), 0.0) : 0.0;
+        const double Ex = k.acceptor >= 0 ? fmax(thr(concOf(k.acceptor)), 0.0) : 1.0;
+#pragma unroll
+        for (int m = 0; m < CLB_DEV_MAXM; ++m) {
+            g[m] = 0.0; br[m] = 0.0;
+            if (m >= k.M) continue;
+            double Bv = (double) bpre[m];
+            if (Bv < 1e-12) Bv = 0;
+            const double Bm = fmax(Bv, 0.0);
+            if (Bm <= 0.0) continue;
+            double mu = k.mu_max[m] * (Sx / (k.ks[m] + Sx));
+            if (k.acceptor >= 0) mu *= Ex / (k.ke[m] + Ex);
+            const double growth = mu * Bm;
+            br[m] += growth - k.kdecay[m] * Bm;
+            g[m] = growth;
+        }
+        const double F = k.thm ? thermoGateOf(k, [&](int s) { return thr(concOf(s)); }) : 1.0;
+#pragma unroll
+        for (int s = 0; s < CLB_DEV_MAXS; ++s) {
+            if (s >= S) continue;
+            double sr = 0;
+#pragma unroll
+            for (int m = 0; m < CLB_DEV_MAXM; ++m) if (m < k.M) sr += k.bio_stoich[m][s] * g[m];
+            if (!isfinite(sr)) sr = 0;
+            if (k.thm && F < 1.0) sr *= F;
+            const double d = sr * k.dt;
+            if (d > 1e-12 || d < -1e-12) depositTo(s, clampInc(k.clamp, d, concOf(s), binds));
+        }
+#pragma unroll
+        for (int m = 0; m < CLB_DEV_MAXM; ++m) {
+            if (m >= k.M) continue;
+            double b0 = br[m];
+            if (!isfinite(b0)) b0 = 0;
+            if (k.thm && F < 1.0) b0 *= F;
+            double d = b0 * k.dt;
+            if (d > 1e-12 || d < -1e-12) {
+                R *p = a.B + (long long) m * N + v;
+                const double b = (double) bpre[m];
+                if (d < -b) d = -b;                                   // biomass never negative
+                *p = (R) (b + d);
+                bpre[m] = (R) (b + d);
+            }
+        }
+    }
+    // the surface gate of the abiotic rates and, in the transport launch of a
+    // dissolution step, whether the increment waits for the parked products
+    const bool wantNear = k.nearPhase == 1;
+    bool react = k.nrxn > 0, deferred = false;
+    if (react && (k.surface_only || wantNear)) {
+        bool surf = false, near = false;
+        if (wantNear) { const int own = k.phase[v]; near = own > 0 && own == k.dPhase; }
+#pragma unroll
+        for (int q = 1; q < 7; ++q) {
+            const int n = a.nb[(opp7(q) - 1) * No + i];                // the voxel at +c_q
+            if (n == NB_OUTSIDE) continue;
+            if (n == NB_CLOSED) { surf = true; continue; }
+            const long long vn = a.list[n];
+            const int nm = a.mask[vn];
+            if (nm == 0 || nm == 1) surf = true;
+            if (wantNear && n < a.Nint) { const int pid = k.phase[vn]; if (pid > 0 && pid == k.dPhase) near = true; }
+        }
+        if (k.surface_only) react = surf;
+        deferred = react && near;
+    }
+    if (k.nrxn > 0 && !deferred && react) {
+        // abioticLaw: each reaction's rate (an excluded one contributes +-0)
+        double Rr[CLB_DEV_MAXR];
+#pragma unroll
+        for (int r = 0; r < CLB_DEV_MAXR; ++r) {
+            Rr[r] = 0.0;
+            if (r >= k.nrxn) continue;
+            const int r1 = k.r1[r], r2 = k.r2[r];
+            const double av = r1 >= 0 ? fmax(thr(concOf(r1)), 0.0) : 1.0;
+            const double bv = r2 >= 0 ? fmax(thr(concOf(r2)), 0.0) : 1.0;
+            if (k.min_conc > 0 && ((r1 >= 0 && av <= k.min_conc) || (r2 >= 0 && bv <= k.min_conc))) continue;
+            double Rv = k.rk[r] * av * bv;
+            if (k.cap_dt > 0 && k.cap_frac > 0) {
+                const double sc = fmin(r1 >= 0 ? av : 1e300, r2 >= 0 ? bv : 1e300);
+                const double cap = sc * k.cap_frac / k.cap_dt;
+                if (Rv > cap) Rv = cap;
+            }
+            Rr[r] = Rv;
+        }
+#pragma unroll
+        for (int s = 0; s < CLB_DEV_MAXS; ++s) {
+            if (s >= S) continue;
+            double sr = 0;
+#pragma unroll
+            for (int r = 0; r < CLB_DEV_MAXR; ++r) if (r < k.nrxn) sr += k.rxn_stoich[r][s] * Rr[r];
+            if (!isfinite(sr)) sr = 0;
+            if (k.rate_scale != 1.0) sr *= k.rate_scale;
+            const double d = sr * k.dt;
+            if (d > 1e-12 || d < -1e-12) depositTo(s, clampInc(k.clamp, d, concOf(s), binds));
+        }
+    }
+    if (binds) atomicAdd(a.clampCount, (unsigned long long) binds);
+}
+
 // the step's mask and total update of one voxel from its biomass b
 // (updateLocalMaskNtotalLattices3D: the total, then pore <-> biofilm code at
 // the threshold), and the total and mask it leaves for the reductions
@@ -706,7 +865,7 @@ __device__ __forceinline__ void adeBulkVoxel(const AdeDev<R> &a, const KinDev &k
         bgk7<R>(f[ms], (R) (cls == ADE_BFILM ? a.momB[ms] : a.momP[ms]), ux, uy, uz);
     }
 
-    if (k.on) voxelKinetics<R, MAXS, false>(a, k, i, f, v, vmask, bpre, true);
+    if (k.on) voxelKineticsFused<R, MAXS>(a, k, i, f, v, vmask, bpre);
 
 #pragma unroll
     for (int ms = 0; ms < MAXS; ++ms) {
@@ -1136,22 +1295,48 @@ __device__ __forceinline__ void pull19(const NsDev<R> &a, int i, T fl[19]) {
         else fl[q] = (T) a.cur[pidx<19>(q, n)];
     }
 }
-// the interior's pull from the precomputed source kinds: every neighbour
-// index first, then every population, so that the loads are in flight together
+// The dense bounce-back store of the interior flow pull.  A population that
+// comes back from a closed neighbour is the voxel's own opposite population
+// from two launches back; the pull used to read it from the voxel's own slot
+// of `out`, which fetches whole cache lines for the few lanes of a wave with
+// that closed link.  Instead every tile of 64 interior voxels (one wave) owns
+// 64 x 18 slots of a store paired with each population buffer: voxel i's
+// closed links (pull kind 2), q ascending, take consecutive slots after those
+// of the tile's earlier voxels, so a wave reads and writes its bounce-back
+// values densely.  The slot layout depends on the pull kinds alone; the store
+// is rebuilt from the own slots of the buffer (still written every launch)
+// whenever the kinds change or voxels reopen (k_ns_bb_fill).
+__device__ __forceinline__ unsigned long long nsKind2Mask(unsigned long long kd) {
+    return (kd >> 1) & ~kd & 0x555555555ull;                       // 2-bit fields equal to 2
+}
+// first slot of voxel i (every lane of the wave takes part: a prefix sum)
+__device__ __forceinline__ long long nsBBBase(int i, unsigned long long kd, bool valid) {
+    const int c = valid ? __popcll(nsKind2Mask(kd)) : 0;
+    int x = c;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return (long long) (i >> 6) * (64 * 18) + (x - c);
+}
+
+// the interior's pull from the precomputed source kinds and the voxel's
+// neighbour record (loaded beforehand, a tile ahead: nsBulkBlocks)
 template <typename R, typename T>
-__device__ __forceinline__ void pull19k(const NsDev<R> &a, int i, T fl[19]) {
+__device__ __forceinline__ void pull19k(const NsDev<R> &a, int i, T fl[19], unsigned long long kd, long long bb,
+                                        const int4 r0, const int4 r1, const int4 r2) {
     const long long No = a.Nopen;
     if (a.first) {
 #pragma unroll
         for (int q = 0; q < 19; ++q) fl[q] = (T) a.cur[pidx<19>(q, i)];
         return;
     }
-    // the compact neighbour record: three 16-byte loads for all 18 indices
-    const int4 r0 = a.nbr[3 * (long long) i], r1 = a.nbr[3 * (long long) i + 1], r2 = a.nbr[3 * (long long) i + 2];
+    // the compact neighbour record: 18 indices in three 16-byte words
     const int w[8] = {r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y};
     const int w8 = r2.z;
     const unsigned int far = (unsigned int) r2.w;
-    const unsigned long long kd = a.kinds[i];
     int nbq[18];
 #pragma unroll
     for (int q = 1; q < 19; ++q) {
@@ -1167,12 +1352,14 @@ __device__ __forceinline__ void pull19k(const NsDev<R> &a, int i, T fl[19]) {
             if ((far >> (q - 1)) & 1u) nbq[q - 1] = a.nb[(q - 1) * No + i];
     }
     fl[0] = (T) a.cur[pidx<19>(0, i)];
+    long long j = bb;
 #pragma unroll
     for (int q = 1; q < 19; ++q) {
         const unsigned int k = (unsigned int) (kd >> (2 * (q - 1))) & 3u;
-        const R *base = k == 2 ? a.out : a.cur;
-        const long long off = k == 0 ? pidx<19>(q, nbq[q - 1]) : pidx<19>(opp19(q), i);
+        const R *base = k == 2 ? a.bbOut : a.cur;
+        const long long off = k == 0 ? pidx<19>(q, nbq[q - 1]) : (k == 2 ? j : pidx<19>(opp19(q), i));
         fl[q] = (T) base[off];
+        if (k == 2) ++j;
     }
 }
 
@@ -1183,6 +1370,18 @@ __device__ __forceinline__ void nsFirstHandOver(const NsDev<R> &a, int i) {
     for (int q = 1; q < 19; ++q) {
         const int n = a.nb[(q - 1) * No + i];
         if (n == NB_CLOSED || (n >= 0 && a.clsPull[n] >= NS_BB)) a.cur[pidx<19>(opp19(q), i)] = (R) 0;
+    }
+}
+// the same for an interior voxel, whose bounce-back values of the launch
+// after next come from the store paired with cur
+template <typename R>
+__device__ __forceinline__ void nsFirstHandOverBB(const NsDev<R> &a, int i, unsigned long long kd, long long bb) {
+    long long j = bb;
+#pragma unroll
+    for (int q = 1; q < 19; ++q) {
+        if (((kd >> (2 * (q - 1))) & 3ull) != 2ull) continue;
+        a.cur[pidx<19>(opp19(q), i)] = (R) 0;
+        a.bbCur[j++] = (R) 0;
     }
 }
 
@@ -1229,32 +1428,85 @@ __device__ __forceinline__ void blockEnergy(double e, double cnt, double *partE,
     }
 }
 
-// interior: pull + BGK; the energy statistic |j|^2/2 per block
+// interior: pull + BGK of the 256 voxels of chunk c (one per thread), whose
+// class, pull kinds and neighbour record were loaded a chunk ahead; the
+// energy statistic |j|^2/2 accumulated into e, cnt
+struct NsAhead {
+    int cls;
+    unsigned long long kd;
+    int4 r0, r1, r2;
+};
 template <typename R>
-__device__ __forceinline__ void nsBulkBlock(const NsDev<R> &a, double *partE, double *partN, int blk) {
-    const int i = blk * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void nsLoadAhead(const NsDev<R> &a, int c, NsAhead &h) {
+    const int i = c * 256 + (int) threadIdx.x;
+    const bool in = i < a.Nint;
+    h.cls = in ? (int) a.cls[i] : NS_BB;
+    h.kd = in ? a.kinds[i] : 0ull;
+    if (in && !a.first) {
+        h.r0 = a.nbr[3 * (long long) i]; h.r1 = a.nbr[3 * (long long) i + 1]; h.r2 = a.nbr[3 * (long long) i + 2];
+    } else {
+        h.r0 = h.r1 = h.r2 = make_int4(0, 0, 0, 0);
+    }
+}
+template <typename R>
+__device__ __forceinline__ void nsBulkChunk(const NsDev<R> &a, int c, const NsAhead &h, double &e, double &cnt) {
+    const int i = c * 256 + (int) threadIdx.x;
+    const bool in = i < a.Nint;
+    const long long bb = nsBBBase(i, h.kd, in);           // every lane of the wave
+    if (!in || h.cls >= NS_BB) return;
+    R fl[19];                                           // compute type = storage type
+    pull19k<R, R>(a, i, fl, h.kd, bb, h.r0, h.r1, h.r2);
+    if (a.first) nsFirstHandOverBB<R>(a, i, h.kd, bb);
+    R rb = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+        rb += fl[q];
+        jx += (R) D19C[q][0] * fl[q]; jy += (R) D19C[q][1] * fl[q]; jz += (R) D19C[q][2] * fl[q];
+    }
+    const R jsq = jx * jx + jy * jy + jz * jz;
+    e += 0.5 * (double) jsq; cnt += 1;
+    const R om = (R) (h.cls == NS_BIO ? a.omBio : a.om);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) fl[q] = fl[q] + om * (feqInc<R>(q, rb, jx, jy, jz, jsq) - fl[q]);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) a.out[pidx<19>(q, i)] = fl[q];
+    // what goes back to this voxel from its closed neighbours two launches
+    // on: the new population opposite each closed link
+    long long j = bb;
+#pragma unroll
+    for (int q = 1; q < 19; ++q)
+        if (((h.kd >> (2 * (q - 1))) & 3ull) == 2ull) a.bbOut[j++] = fl[opp19(q)];
+}
+// The interior in G blocks of 256 threads, block lb taking the contiguous
+// chunks [lb C / G, (lb + 1) C / G) of the C chunks of 256 interior voxels.
+// A thread loads the class, kinds and neighbour record of its voxel in the
+// next chunk before it pulls in the current one, so those loads are in
+// flight beside the current gathers: one dependent round of memory per chunk
+// instead of two.
+// Without AHEAD the loads of a chunk come right before its pull (fewer live
+// registers: more waves per SIMD); with G = C that is one chunk per block.
+template <typename R, bool AHEAD>
+__device__ __forceinline__ void nsBulkBlocks(const NsDev<R> &a, double *partE, double *partN, int lb, int G) {
+    const int C = (a.Nint + 255) / 256;
+    const int c0 = (int) ((long long) lb * C / G), c1 = (int) ((long long) (lb + 1) * C / G);
     double e = 0, cnt = 0;
-    if (i < a.Nint) {
-        const int cls = a.cls[i];
-        if (cls < NS_BB) {
-            R fl[19];                                  // compute type = storage type
-            pull19k<R, R>(a, i, fl);
-            if (a.first) nsFirstHandOver<R>(a, i);
-            R rb = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-            for (int q = 0; q < 19; ++q) {
-                rb += fl[q];
-                jx += (R) D19C[q][0] * fl[q]; jy += (R) D19C[q][1] * fl[q]; jz += (R) D19C[q][2] * fl[q];
-            }
-            const R jsq = jx * jx + jy * jy + jz * jz;
-            e = 0.5 * (double) jsq; cnt = 1;
-            const R om = (R) (cls == NS_BIO ? a.omBio : a.om);
-            const long long No = a.Nopen;
-#pragma unroll
-            for (int q = 0; q < 19; ++q) a.out[pidx<19>(q, i)] = fl[q] + om * (feqInc<R>(q, rb, jx, jy, jz, jsq) - fl[q]);
+    if (AHEAD) {
+        NsAhead h;
+        if (c0 < c1) nsLoadAhead<R>(a, c0, h);
+        for (int c = c0; c < c1; ++c) {
+            NsAhead nx = h;
+            if (c + 1 < c1) nsLoadAhead<R>(a, c + 1, nx);
+            nsBulkChunk<R>(a, c, h, e, cnt);
+            h = nx;
+        }
+    } else {
+        for (int c = c0; c < c1; ++c) {
+            NsAhead h;
+            nsLoadAhead<R>(a, c, h);
+            nsBulkChunk<R>(a, c, h, e, cnt);
         }
     }
-    blockEnergy(e, cnt, partE, partN, blk);
+    blockEnergy(e, cnt, partE, partN, lb);
 }
 
 // the pressure planes
@@ -1286,12 +1538,12 @@ __device__ __forceinline__ void nsPlaneBlock(const NsDev<R> &a, double *partE, d
 // planes (few blocks with a long serial chain each, so they start first and
 // run beside the interior), the rest the interior in XCD order; block partials
 // at [interior blocks][plane blocks]
-template <typename R>
+template <typename R, bool AHEAD>
 __global__ void __launch_bounds__(256) k_ns_step(NsDev<R> a, double *partE, double *partN, int gP) {
     if (*a.done) return;
     const int gB = (int) gridDim.x - gP;
     if ((int) blockIdx.x < gP) nsPlaneBlock<R>(a, partE, partN, blockIdx.x, gB);
-    else nsBulkBlock<R>(a, partE, partN, xcdBlockOf((int) blockIdx.x - gP, gB));
+    else nsBulkBlocks<R, AHEAD>(a, partE, partN, xcdBlockOf((int) blockIdx.x - gP, gB), gB);
 }
 
 // the convergence rule, evaluated on the device after every iteration.  The
@@ -1513,7 +1765,13 @@ hipError_t launchNsDensity(const NsDev<R> &a, double *dst, hipStream_t st) {
 template hipError_t launchNsDensity<double>(const NsDev<double> &, double *, hipStream_t);
 template hipError_t launchNsDensity<float>(const NsDev<float> &, double *, hipStream_t);
 
-unsigned int nsStepBlocks(int Nopen, int Nint) { return nblk(Nint, 256) + nblk(Nopen - Nint, 256); }
+// interior blocks of a flow launch: one per chunk of 256 voxels, or in the
+// prefetching mode (NsDev::shape > 0) that many blocks looping over chunks
+static unsigned int nsInteriorBlocks(int Nint, int shape) {
+    const unsigned int g = nblk(Nint, 256);
+    return shape > 0 && (unsigned int) shape < g ? (unsigned int) shape : g;
+}
+unsigned int nsStepBlocks(int Nopen, int Nint, int shape) { return nsInteriorBlocks(Nint, shape) + nblk(Nopen - Nint, 256); }
 
 template <typename R>
 hipError_t launchAdeStep(const AdeDev<R> &a, const KinDev &k, hipStream_t st, hipEvent_t afterInterior) {
@@ -1620,6 +1878,23 @@ __global__ void __launch_bounds__(256) k_ns_kinds(const int *nb, const uint8_t *
     }
     kinds[i] = code;
 }
+// the dense bounce-back store `bb` from the own slots of the population
+// buffer `g` it pairs with (nsBBBase), for the current pull kinds
+template <typename R>
+__global__ void __launch_bounds__(256) k_ns_bb_fill(NsDev<R> a, const R *g, R *bb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = i < a.Nint;
+    const unsigned long long kd = in ? a.kinds[i] : 0ull;
+    long long j = nsBBBase(i, kd, in);
+    if (!in) return;
+    for (int q = 1; q < 19; ++q)
+        if (((kd >> (2 * (q - 1))) & 3ull) == 2ull) bb[j++] = g[pidx<19>(opp19(q), i)];
+}
+template <typename R>
+hipError_t launchNsBBFill(const NsDev<R> &a, const R *g, R *bb, hipStream_t st) {
+    if (a.Nint > 0) hipLaunchKernelGGL(k_ns_bb_fill<R>, dim3(nblk(a.Nint, 256)), dim3(256), 0, st, a, g, bb);
+    return hipGetLastError();
+}
 hipError_t launchNsKinds(const int *nb, const uint8_t *clsPull, int Nopen, int Nint, unsigned long long *kinds, hipStream_t st) {
     if (Nint > 0) hipLaunchKernelGGL(k_ns_kinds, dim3(nblk(Nint, 256)), dim3(256), 0, st, nb, clsPull, Nopen, Nint, kinds);
     return hipGetLastError();
@@ -1640,8 +1915,11 @@ hipError_t launchAdeTotal(const AdeDev<R> &a, int ms, double *parts, hipStream_t
 }
 template <typename R>
 hipError_t launchNsStep(const NsDev<R> &a, double *partE, double *partN, NsTracer *t, int rule, double eps, hipStream_t st) {
-    const unsigned int gB = nblk(a.Nint, 256), gP = nblk(a.Nopen - a.Nint, 256);
-    if (gB + gP) hipLaunchKernelGGL(k_ns_step<R>, dim3(gB + gP), dim3(256), 0, st, a, partE, partN, (int) gP);
+    const unsigned int gB = nsInteriorBlocks(a.Nint, a.shape), gP = nblk(a.Nopen - a.Nint, 256);
+    if (gB + gP) {
+        if (a.shape > 0) hipLaunchKernelGGL((k_ns_step<R, true>), dim3(gB + gP), dim3(256), 0, st, a, partE, partN, (int) gP);
+        else hipLaunchKernelGGL((k_ns_step<R, false>), dim3(gB + gP), dim3(256), 0, st, a, partE, partN, (int) gP);
+    }
     hipLaunchKernelGGL(k_ns_tracer, dim3(1), dim3(1024), 0, st, t, partE, partN, (int) (gB + gP), rule, eps, a.done);
     return hipGetLastError();
 }
@@ -1692,6 +1970,7 @@ hipError_t launchGatherCompact(const double *src, const int *list, int n, R *dst
     template hipError_t launchNsVelocity<R, R>(const NsDev<R> &, R *, double *, hipStream_t); \
     template hipError_t launchNsInit<R>(const NsDev<R> &, double, hipStream_t); \
     template hipError_t launchNsReopen<R>(const NsDev<R> &, const uint8_t *, hipStream_t); \
+    template hipError_t launchNsBBFill<R>(const NsDev<R> &, const R *, R *, hipStream_t); \
     template hipError_t launchScatterCompact<R>(const R *, const int *, int, double *, hipStream_t); \
     template hipError_t launchGatherCompact<R>(const double *, const int *, int, R *, hipStream_t);
 INST(double)

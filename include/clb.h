@@ -431,6 +431,14 @@ int  clb_kernel_time(clb_ctx *ctx, int case_id, int reset, double *avg_ms, int64
  * on.  No counterpart in the reference. */
 int  clb_set_timing_stride(clb_ctx *ctx, int case_id, int stride);
 
+/* launch shapes of the hot kernels, the knobs of the autotuner
+ * (complab_amd.tune); results do not depend on them.  No counterpart in the
+ * reference.  CLB_TUNE_NS_SHAPE: the flow interior launch, 0 one workgroup per
+ * 256 voxels, n > 0 n workgroups each looping over a contiguous run of them and
+ * loading the next one's neighbour records ahead. */
+#define CLB_TUNE_NS_SHAPE 0
+int  clb_set_tuning(clb_ctx *ctx, int case_id, int knob, int value);
+
 /* device copy bandwidth of a plain vectorised copy kernel over `bytes` bytes,
  * best of `reps` launches (HIP events); the measured ceiling the roofline
  * fractions are read against and the calibration of the HBM byte counters.
