@@ -213,6 +213,8 @@ def main():
     ap.add_argument("--serve-steps", type=int, default=5000)
     ap.add_argument("--serve-size", type=int, default=128, help="largest edge of a serving request")
     ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--max-to-spill", type=int, default=3000,
+                    help="untimed steps allowed after the warm-up to reach the CA spill")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -240,6 +242,16 @@ def main():
     rc, n = g.run_steps(args.warmup)
     if rc:
         raise SystemExit("warm-up stopped with status %d after %d steps" % (rc, n))
+    # untimed: advance until the CA spill is under way, so that any timed
+    # window holds the whole coupled step (spill sweeps, mask changes) and not
+    # only the transport of the biofilm's first growth
+    to_spill = 0
+    while g.counters()[6] == 0 and to_spill < args.max_to_spill:
+        rc, n = g.run_steps(100)
+        if rc:
+            raise SystemExit("advance to the spill stopped with status %d" % rc)
+        to_spill += n
+    print("spill under way after %d steps past the warm-up" % to_spill, file=sys.stderr, flush=True)
     ctx.sync()
     g.set_timing_stride(TIMING_STRIDE)
     g.kernel_time(reset=True)
@@ -253,6 +265,20 @@ def main():
     c1 = g.counters()
     kms, launches = g.kernel_time()
 
+    # the flow: a re-solve as the main loop runs one after a mask change
+    # (its own tracer starts empty: complab.cpp:2166), timed alone, and its
+    # cost per coupled step at the configured re-solve interval; then the
+    # per-iteration time of the flow kernel and tracer
+    ctx.sync()
+    t1 = time.perf_counter()
+    g.ns_update_dynamics(w.params.perm_ratio)
+    rs_it, rs_cv = g.ns_solve(w.params.ns_max_iter, w.params.ns_conv_rule, w.params.ns_conv_eps, reset=True)
+    ctx.sync()
+    rs_s = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    fl_it, _ = g.ns_solve(500, P.CONV_TRACER, 1e-300, reset=True)      # a tolerance no window meets
+    ctx.sync()
+    fl_us = (time.perf_counter() - t1) / max(fl_it, 1) * 1e6
     copy_gbs, copy_ms = ctx.stream_copy(1 << 30, 20)
     p = w.params
     V = p.nx * p.ny * p.nz
@@ -263,6 +289,8 @@ def main():
     bytes_launch, per_open = algorithmic_bytes(p, n_int, (p.nx - 2) * p.ny * p.nz)
     achieved = bytes_launch / (kms * 1e-3) / 1e9
     mlups = V * args.steps / t / 1e6
+    ns_bytes = 19 * 2 * (8 if p.precision != P.FP32 else 4) * n_open + V    # per flow iteration
+    resolve_ms_per_step = rs_s * 1e3 / max(p.ns_update_interval, 1)
 
     cpu = None
     if not args.no_cpu:
@@ -295,7 +323,16 @@ def main():
         "config": {"workload": w.name, "lattice": [p.nx, p.ny, p.nz], "open_voxels": n_open, "open_interior": n_int,
                    "species": p.n_species, "device": dev,
                    "flow_resolves": int(c1[4] - c0[4]), "ca_sweeps": int(c1[6] - c0[6]),
-                   "mask_changes": int(c1[5] - c0[5])},
+                   "mask_changes": int(c1[5] - c0[5]), "untimed_steps_to_spill": to_spill},
+        "flow": {"resolve_iterations": int(rs_it), "resolve_converged": bool(rs_cv), "resolve_s": round(rs_s, 3),
+                 "resolve_interval_steps": int(p.ns_update_interval),
+                 "resolve_ms_per_step_amortised": round(resolve_ms_per_step, 4),
+                 "mlups_with_a_resolve_every_interval":
+                     round(V * args.steps / (t + resolve_ms_per_step * 1e-3 * args.steps) / 1e6, 2),
+                 "iteration_us": round(fl_us, 2), "bytes_per_iteration": int(ns_bytes),
+                 "iteration_gbs": round(ns_bytes / (fl_us * 1e-6) / 1e9, 1),
+                 "iteration_frac": round(ns_bytes / (fl_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                 "note": "k_ns_step + k_ns_tracer per iteration, wall clock over 500 iterations"},
         "units": {"mflups_open": round(n_open * args.steps / t / 1e6, 2),
                   "lattice_mlups": round(V * args.steps * sum(1 for s in range(p.n_species) if p.mobile[s]) / t / 1e6, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

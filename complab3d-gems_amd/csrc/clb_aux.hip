@@ -61,7 +61,28 @@ __global__ void __launch_bounds__(256) k_ca_mask_total(CaDev<R> c, int check) {
 // the CA must run (max total above Bmax), the mask moved, or the total is not
 // finite (check 2: the step always needs the host, a precipitation step)
 template <typename R>
+__device__ __forceinline__ void caMaskFinalBlock(const CaDev<R> &c, int nparts, int check);
+template <typename R>
 __global__ void __launch_bounds__(1024) k_ca_mask_final(CaDev<R> c, int nparts, int check) {
+    caMaskFinalBlock<R>(c, nparts, check);
+}
+// the batched form: member blockIdx.x's decision, check 2 on the steps its
+// host must finish (precipitation / dissolution checks), 1 otherwise
+template <typename R>
+__global__ void __launch_bounds__(1024) k_ca_mask_final_group(const AdeMember<R> *__restrict__ E) {
+    const AdeMember<R> &g = E[blockIdx.x];
+    if (!g.ca) return;
+    const long long it = g.iT0 + g.c.ctl->done;
+    const bool due = (g.precipEvery > 0 && it % g.precipEvery == 0) || (g.dissolEvery > 0 && it % g.dissolEvery == 0);
+    caMaskFinalBlock<R>(g.c, g.gInt + g.gPln, due ? 2 : 1);
+}
+template <typename R>
+hipError_t launchCaMaskFinalGroup(const AdeMember<R> *E, int nMembers, hipStream_t st) {
+    if (nMembers > 0) hipLaunchKernelGGL(k_ca_mask_final_group<R>, dim3(nMembers), dim3(1024), 0, st, E);
+    return hipGetLastError();
+}
+template <typename R>
+__device__ __forceinline__ void caMaskFinalBlock(const CaDev<R> &c, int nparts, int check) {
     __shared__ double sMax[16];
     __shared__ long long sMask[16];
     StepCtl *ctl = c.ctl;
@@ -671,6 +692,7 @@ hipError_t launchDoubleToI32(const double *src, int *dst, long long n, hipStream
 #define INST(R) \
     template hipError_t launchCaMaskTotal<R>(const CaDev<R> &, int, hipStream_t); \
     template hipError_t launchCaMaskFinal<R>(const CaDev<R> &, int, int, hipStream_t); \
+    template hipError_t launchCaMaskFinalGroup<R>(const AdeMember<R> *, int, hipStream_t); \
     template hipError_t launchCaPushGather<R>(const CaDev<R> &, uint64_t, hipStream_t); \
     template hipError_t launchCaAge<R>(const CaDev<R> &, const int *, int *, long long *, hipStream_t); \
     template hipError_t launchAgeCompact<R>(const CaDev<R> &, int *, unsigned int *, unsigned int *, hipStream_t); \

@@ -230,6 +230,32 @@ struct EqDev {
     int maxIter, depth;
 };
 
+// ---- the batched form: the members of a batch in single launches --------
+// A launch covers every member: a table of member descriptors in device
+// memory and a map of the launch's workgroups to (member, workgroup of the
+// member's own launch), so each member computes exactly what its own launch
+// computes (the same block partials in the same order).
+template <typename R>
+struct NsMember {
+    NsDev<R> a;
+    double *partE, *partN;         // the member's block partials (interior, then planes)
+    NsTracer *tracer;
+    int gP, gB;                    // plane blocks, interior blocks (one per 256 voxels)
+    int rule;
+    double eps;
+    int active;                    // 0: the member sits this solve out
+};
+template <typename R>
+struct AdeMember {
+    AdeDev<R> a;
+    KinDev k;
+    CaDev<R> c;
+    int gInt, gPln;                // interior and plane workgroups at the batch's tile size
+    int ca;                        // 1: a CA microbe, the step's mask/total decision on the device
+    long long iT0;                 // the member's steps at the start of the batch
+    int precipEvery, dissolEvery;  // a step on these intervals needs the host: it halts the member
+};
+
 // host launchers: compact lattices (clb_lattice.hip)
 // the interior kernel, then (when given) an event, then the planes
 template <typename R> hipError_t launchAdeStep(const AdeDev<R> &, const KinDev &, hipStream_t, hipEvent_t afterInterior);
@@ -261,6 +287,14 @@ template <typename R> hipError_t launchEquilibrium(const AdeDev<R> &, const EqDe
 template <typename R> hipError_t launchScatterCompact(const R *, const int *, int, double *, hipStream_t);
 template <typename R> hipError_t launchGatherCompact(const double *, const int *, int, R *, hipStream_t);
 unsigned int nsStepBlocks(int Nopen, int Nint, int shape);
+// batched form: one flow iteration (and each member's tracer) of every active
+// member; one transport launch (interior, then planes) of every member at
+// tile size `tile`, with the largest member's species bound
+template <typename R> hipError_t launchNsStepGroup(const NsMember<R> *E, int nMembers, const int2 *map, int nBlocks,
+                                                   hipStream_t);
+template <typename R> hipError_t launchAdeStepGroup(const AdeMember<R> *E, const int2 *mapInt, int nInt,
+                                                    const int2 *mapPln, int nPln, int maxSm, int tile, hipStream_t);
+template <typename R> hipError_t launchCaMaskFinalGroup(const AdeMember<R> *E, int nMembers, hipStream_t);
 // host launchers: dense grids (clb_aux.hip)
 template <typename R> hipError_t launchCaMaskTotal(const CaDev<R> &, int check, hipStream_t);
 template <typename R> hipError_t launchCaMaskFinal(const CaDev<R> &, int nparts, int check, hipStream_t);

@@ -888,6 +888,24 @@ __global__ void __launch_bounds__(256) k_ade_bulk(AdeDev<R> a, KinDev k) {
 
 // the planes after the interior, one thread per plane voxel and mobile species
 // (blockIdx.y), so the short dependent load chains of the few plane voxels overlap
+// one plane block (pb) for mobile slot ms; the planes' mask/total partials
+// follow the interior's gInt block partials
+template <typename R>
+__device__ __forceinline__ void adePlanesBlock(const AdeDev<R> &a, int pb, int ms, int gInt) {
+    const int i = a.Nint + pb * blockDim.x + threadIdx.x;
+    if (i < a.Nopen && ms < a.Sm) adePlaneVoxel<R>(a, i, ms);
+    if (a.maskTotal && ms == 0) {
+        double bt = -1e300; long long mt = 0;
+        if (i < a.Nopen) {
+            const long long v = a.list[i];
+            R b[CLB_DEV_MAXM];
+#pragma unroll
+            for (int m = 0; m < CLB_DEV_MAXM; ++m) b[m] = m < a.M ? a.B[(long long) m * a.N + v] : (R) 0;
+            voxelMaskTotal<R>(a, v, a.mask[v], b, bt, mt);
+        }
+        blockMaskTotal(bt, mt, a.partMax, a.partMask, gInt + pb);
+    }
+}
 template <typename R>
 __global__ void __launch_bounds__(256) k_ade_planes(AdeDev<R> a) {
     if (halted(a.halt)) return;
@@ -1550,7 +1568,8 @@ __global__ void __launch_bounds__(256) k_ns_step(NsDev<R> a, double *partE, doub
 // partials are summed by 1024 threads, thread t taking t, t + 1024, ... (all
 // of its loads issued together), then in a fixed tree; the window statistics
 // run in the ring's order (oldest first) from a copy in LDS.
-__global__ void __launch_bounds__(1024) k_ns_tracer(NsTracer *t, const double *partE, const double *partN, int nparts, int rule, double eps, int *done) {
+__device__ __forceinline__ void nsTracerBlock(NsTracer *t, const double *partE, const double *partN, int nparts, int rule,
+                                              double eps, int *done) {
     __shared__ double sE[1024], sN[1024];
     __shared__ double win[NS_TRACER_WINDOW];
     __shared__ double sVal;
@@ -1612,6 +1631,9 @@ __global__ void __launch_bounds__(1024) k_ns_tracer(NsTracer *t, const double *p
         const double r = sd / fabs(m);
         if (isnan(r) || r < eps) { t->converged = 1; *done = 1; }
     }
+}
+__global__ void __launch_bounds__(1024) k_ns_tracer(NsTracer *t, const double *partE, const double *partN, int nparts, int rule, double eps, int *done) {
+    nsTracerBlock(t, partE, partN, nparts, rule, eps, done);
 }
 
 // velocity of the current flow state (computeVelocity) into the compact field
@@ -1953,6 +1975,61 @@ hipError_t launchGatherCompact(const double *src, const int *list, int n, R *dst
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------------------------------
+// the batched form: every member of a batch in one launch
+// ------------------------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256) k_ns_step_group(const NsMember<R> *__restrict__ E, const int2 *__restrict__ map) {
+    const int2 e = map[xcdBlockOf((int) blockIdx.x, (int) gridDim.x)];
+    const NsMember<R> &g = E[e.x];
+    if (!g.active || *g.a.done) return;
+    if (e.y < g.gP) nsPlaneBlock<R>(g.a, g.partE, g.partN, e.y, g.gB);
+    else nsBulkBlocks<R, false>(g.a, g.partE, g.partN, e.y - g.gP, g.gB);
+}
+template <typename R>
+__global__ void __launch_bounds__(1024) k_ns_tracer_group(const NsMember<R> *__restrict__ E) {
+    const NsMember<R> &g = E[blockIdx.x];
+    if (!g.active) return;
+    nsTracerBlock(g.tracer, g.partE, g.partN, g.gB + g.gP, g.rule, g.eps, g.a.done);
+}
+template <typename R>
+hipError_t launchNsStepGroup(const NsMember<R> *E, int nMembers, const int2 *map, int nBlocks, hipStream_t st) {
+    if (nBlocks > 0) hipLaunchKernelGGL(k_ns_step_group<R>, dim3(nBlocks), dim3(256), 0, st, E, map);
+    if (nMembers > 0) hipLaunchKernelGGL(k_ns_tracer_group<R>, dim3(nMembers), dim3(1024), 0, st, E);
+    return hipGetLastError();
+}
+template <typename R, int MAXS>
+__global__ void __launch_bounds__(256) k_ade_bulk_group(const AdeMember<R> *__restrict__ E, const int2 *__restrict__ map) {
+    const int2 e = map[xcdBlockOf((int) blockIdx.x, (int) gridDim.x)];
+    const AdeMember<R> &g = E[e.x];
+    if (halted(g.a.halt)) return;
+    double bt = -1e300; long long mt = 0;
+    adeBulkVoxel<R, MAXS>(g.a, g.k, e.y * (int) blockDim.x + (int) threadIdx.x, bt, mt);
+    if (g.a.maskTotal) blockMaskTotal(bt, mt, g.a.partMax, g.a.partMask, e.y);
+}
+template <typename R>
+__global__ void __launch_bounds__(256) k_ade_planes_group(const AdeMember<R> *__restrict__ E, const int2 *__restrict__ map) {
+    const int2 e = map[blockIdx.x];
+    const AdeMember<R> &g = E[e.x];
+    if (halted(g.a.halt)) return;
+    adePlanesBlock<R>(g.a, e.y & 0xffff, e.y >> 16, g.gInt);
+}
+template <typename R>
+hipError_t launchAdeStepGroup(const AdeMember<R> *E, const int2 *mapInt, int nInt, const int2 *mapPln, int nPln, int maxSm,
+                              int tile, hipStream_t st) {
+    if (nInt > 0) {
+        const dim3 g(nInt), b(tile);
+        if (maxSm <= 1) hipLaunchKernelGGL((k_ade_bulk_group<R, 1>), g, b, 0, st, E, mapInt);
+        else if (maxSm <= 2) hipLaunchKernelGGL((k_ade_bulk_group<R, 2>), g, b, 0, st, E, mapInt);
+        else if (maxSm <= 3) hipLaunchKernelGGL((k_ade_bulk_group<R, 3>), g, b, 0, st, E, mapInt);
+        else if (maxSm <= 4) hipLaunchKernelGGL((k_ade_bulk_group<R, 4>), g, b, 0, st, E, mapInt);
+        else if (maxSm <= 6) hipLaunchKernelGGL((k_ade_bulk_group<R, 6>), g, b, 0, st, E, mapInt);
+        else hipLaunchKernelGGL((k_ade_bulk_group<R, 8>), g, b, 0, st, E, mapInt);
+    }
+    if (nPln > 0) hipLaunchKernelGGL(k_ade_planes_group<R>, dim3(nPln), dim3(tile), 0, st, E, mapPln);
+    return hipGetLastError();
+}
+
 #define INST(R) \
     template hipError_t launchAdeStep<R>(const AdeDev<R> &, const KinDev &, hipStream_t, hipEvent_t); \
     template hipError_t launchKinetics<R>(const AdeDev<R> &, const KinDev &, hipStream_t); \
@@ -1971,6 +2048,8 @@ hipError_t launchGatherCompact(const double *src, const int *list, int n, R *dst
     template hipError_t launchNsInit<R>(const NsDev<R> &, double, hipStream_t); \
     template hipError_t launchNsReopen<R>(const NsDev<R> &, const uint8_t *, hipStream_t); \
     template hipError_t launchNsBBFill<R>(const NsDev<R> &, const R *, R *, hipStream_t); \
+    template hipError_t launchNsStepGroup<R>(const NsMember<R> *, int, const int2 *, int, hipStream_t); \
+    template hipError_t launchAdeStepGroup<R>(const AdeMember<R> *, const int2 *, int, const int2 *, int, int, int, hipStream_t); \
     template hipError_t launchScatterCompact<R>(const R *, const int *, int, double *, hipStream_t); \
     template hipError_t launchGatherCompact<R>(const double *, const int *, int, R *, hipStream_t);
 INST(double)
