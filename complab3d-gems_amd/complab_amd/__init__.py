@@ -86,6 +86,16 @@ def lib():
         "clb_geom_spheres": ([I, I, I, D, I, C.c_uint64, I, V], I),
         "clb_geom_pack": ([I, I, I, D, D, C.c_uint64, I, V, C.POINTER(D)], I),
         "clb_geom_generate": ([I, I, I, I, D, I, D, D, D, C.c_uint64, I, V], I),
+        "clb_batch_create": ([V, I, V, I, C.POINTER(I)], I),
+        "clb_batch_destroy": ([V, I], I),
+        "clb_batch_add": ([V, I, I], I),
+        "clb_batch_remove": ([V, I, I], I),
+        "clb_batch_members": ([V, I, C.POINTER(I), V, I], I),
+        "clb_batch_ns_solve": ([V, I, I, I, D, I, V, V, V], I),
+        "clb_batch_ade_stabilize": ([V, I, I, V], I),
+        "clb_batch_run_steps": ([V, I, I, V, V], I),
+        "clb_batch_set_tile": ([V, I, I, I], I),
+        "clb_batch_tuning": ([V, I, I, C.POINTER(I), V], I),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -109,6 +119,9 @@ def exported_symbols():
         "clb_stream_copy",
         "clb_rng_draws", "clb_geom_spheres", "clb_geom_pack", "clb_geom_generate",
         "clb_species_totals_noalloc", "clb_set_tuning",
+        "clb_batch_create", "clb_batch_destroy", "clb_batch_add", "clb_batch_remove", "clb_batch_members",
+        "clb_batch_ns_solve", "clb_batch_ade_stabilize", "clb_batch_run_steps", "clb_batch_set_tile",
+        "clb_batch_tuning",
     ]
 
 
@@ -383,3 +396,75 @@ def rng_draws(seed, n):
     out = np.zeros(n, dtype=np.uint64)
     lib().clb_rng_draws(seed, n, _p(out))
     return out
+
+
+class Batch:
+    """Cases of one precision stepped together (clb_batch_*): each kernel of a
+    flow iteration or a transport step covers every member, and every member
+    ends where its own calls would take it.  Per-member results come back in
+    member order (self.cases)."""
+
+    def __init__(self, ctx, cases=(), precision=None):
+        cases = list(cases)
+        if precision is None:
+            if not cases:
+                raise ValueError("an empty batch needs its precision")
+            precision = cases[0].p.precision
+        self.ctx = ctx
+        self.precision = precision
+        ids = np.array([c.id for c in cases], dtype=np.int32)
+        bid = C.c_int()
+        ctx.check(lib().clb_batch_create(ctx.h, precision, _p(ids) if len(ids) else None, len(ids), C.byref(bid)))
+        self.id = bid.value
+        self.cases = cases
+
+    def close(self):
+        if getattr(self, "id", None) is not None and self.ctx.h:
+            lib().clb_batch_destroy(self.ctx.h, self.id)
+            self.id = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.cases)
+
+    def add(self, case):
+        self.ctx.check(lib().clb_batch_add(self.ctx.h, self.id, case.id))
+        self.cases.append(case)
+
+    def remove(self, case):
+        self.ctx.check(lib().clb_batch_remove(self.ctx.h, self.id, case.id))
+        self.cases = [c for c in self.cases if c is not case]
+
+    def ns_solve(self, max_iter, rule, eps, reset=False):
+        """Per member (iterations, converged, status) of clb_ns_solve."""
+        n = len(self.cases)
+        it = np.zeros(n, np.int32); cv = np.zeros(n, np.int32); st = np.zeros(n, np.int32)
+        self.ctx.check(lib().clb_batch_ns_solve(self.ctx.h, self.id, max_iter, rule, eps, int(reset),
+                                                _p(it), _p(cv), _p(st)))
+        return [(int(a), bool(b), int(c)) for a, b, c in zip(it, cv, st)]
+
+    def ade_stabilize(self, steps):
+        st = np.zeros(len(self.cases), np.int32)
+        self.ctx.check(lib().clb_batch_ade_stabilize(self.ctx.h, self.id, steps, _p(st)))
+        return [int(x) for x in st]
+
+    def run_steps(self, n):
+        """Per member (status, steps done) of clb_run_steps."""
+        k = len(self.cases)
+        d = np.zeros(k, np.int32); st = np.zeros(k, np.int32)
+        self.ctx.check(lib().clb_batch_run_steps(self.ctx.h, self.id, n, _p(d), _p(st)))
+        return [(int(a), int(b)) for a, b in zip(st, d)]
+
+    def set_tile(self, mode, tile):
+        self.ctx.check(lib().clb_batch_set_tile(self.ctx.h, self.id, mode, tile))
+
+    def tuning(self, mode=P.BATCH_STEPS):
+        """(picked workgroup size or 0, {candidate: microseconds per step})."""
+        t = C.c_int(); us = np.zeros(3)
+        self.ctx.check(lib().clb_batch_tuning(self.ctx.h, self.id, mode, C.byref(t), _p(us)))
+        return t.value, {256: float(us[0]), 128: float(us[1]), 64: float(us[2])}

@@ -107,37 +107,49 @@ def _solve(case, fs: FlowSetup, chunk=0, progress=None):
     return total, cv
 
 
-def solve_flow(case, fs: FlowSetup, rep: SetupReport, chunk=0, progress=None):
-    """NS solve + Pe rescale (complab.cpp:1098-1183)."""
+def _after_first_solve(case, fs: FlowSetup):
+    """The Peclet rescale after the first solve (complab.cpp:1124-1160): the
+    state of the report, and the new pressure drop when a second solve is
+    due (None otherwise)."""
     nu = CS2_NS * (fs.ns_tau - 0.5)
     dp = fs.delta_p
-    case.ns_init(dp)
-    it, cv = _solve(case, fs, chunk, progress)
-    rep.iters.append(it); rep.converged.append(cv)
     st = case.ns_stats()
-    pore_mean = st[3] if fs.has_biofilm else st[0]
-    outlet = st[2]
     d_lat = CS2_ADE * (fs.tau_ade_fixed - 0.5)
-    perm = outlet * nu * fs.char_length / dp
+    perm = st[2] * nu * fs.char_length / dp
     u_t = fs.pe * d_lat / fs.char_length
     dp_new = u_t * nu * fs.char_length / perm if perm > 0 else dp
-    max_ux = st[1]
-    if abs(dp_new - dp) / dp > 0.01:
+    s = dict(dp=dp, pore_mean=st[3] if fs.has_biofilm else st[0], outlet=st[2], max_ux=st[1],
+             perm=perm, d_lat=d_lat)
+    return s, (dp_new if abs(dp_new - dp) / dp > 0.01 else None)
+
+
+def _after_second_solve(case, s, dp_new):
+    st = case.ns_stats()
+    s.update(pore_mean=st[0], max_ux=st[1], outlet=st[2], dp=dp_new)   # complab.cpp:1175 averages the box
+
+
+def _flow_report(s, fs: FlowSetup, rep: SetupReport):
+    rep.delta_p = s["dp"]
+    rep.pore_mean_u = s["pore_mean"]
+    rep.max_ux = s["max_ux"]
+    rep.outlet_ux = s["outlet"]
+    rep.permeability = s["perm"]
+    rep.pe_achieved = s["outlet"] * fs.char_length / s["d_lat"]
+    return s["pore_mean"]
+
+
+def solve_flow(case, fs: FlowSetup, rep: SetupReport, chunk=0, progress=None):
+    """NS solve + Pe rescale (complab.cpp:1098-1183)."""
+    case.ns_init(fs.delta_p)
+    it, cv = _solve(case, fs, chunk, progress)
+    rep.iters.append(it); rep.converged.append(cv)
+    s, dp_new = _after_first_solve(case, fs)
+    if dp_new is not None:
         case.ns_init(dp_new)
         it, cv = _solve(case, fs, chunk, progress)
         rep.iters.append(it); rep.converged.append(cv)
-        st = case.ns_stats()
-        pore_mean = st[0]                      # complab.cpp:1175 averages the box
-        max_ux = st[1]
-        outlet = st[2]
-        dp = dp_new
-    rep.delta_p = dp
-    rep.pore_mean_u = pore_mean
-    rep.max_ux = max_ux
-    rep.outlet_ux = outlet
-    rep.permeability = perm
-    rep.pe_achieved = outlet * fs.char_length / d_lat
-    return pore_mean
+        _after_second_solve(case, s, dp_new)
+    return _flow_report(s, fs, rep)
 
 
 def ade_relaxation(p, pore_mean_u, fs: FlowSetup, ts: TransportSetup, rep: SetupReport):
@@ -181,3 +193,59 @@ def setup(case, p, fs: FlowSetup, ts: TransportSetup, chunk=0, progress=None):
         if ts.stabilize_steps > 0:
             case.ade_stabilize(ts.stabilize_steps)
     return rep
+
+
+def setup_batch(ctx, cases, workloads):
+    """setup() of many GPU cases (workloads[i] belongs to cases[i]) with their
+    flow solves and stabilizations in batch launches (complab_amd.Batch):
+    every case ends where setup() would take it.  Returns the reports."""
+    from . import Batch, ClbError
+    n = len(cases)
+    reps = [SetupReport() for _ in range(n)]
+    flow = [i for i in range(n) if workloads[i].flow.pe > 0 and workloads[i].params.flow_on]
+
+    def solve(dps):
+        for i, dp in dps.items():
+            cases[i].ns_init(dp)
+        groups = {}
+        for i in dps:
+            fs = workloads[i].flow
+            groups.setdefault((fs.max_iter, fs.rule, fs.eps), []).append(i)
+        for (max_iter, rule, eps), ids in groups.items():
+            b = Batch(ctx, [cases[i] for i in ids])
+            try:
+                res = b.ns_solve(max_iter, rule, eps, reset=True)
+            finally:
+                b.close()
+            for i, (it, cv, st) in zip(ids, res):
+                if st < 0:
+                    raise ClbError(st, "flow solve of %s" % workloads[i].name)
+                reps[i].iters.append(it); reps[i].converged.append(cv)
+
+    solve({i: workloads[i].flow.delta_p for i in flow})
+    state, again = {}, {}
+    for i in flow:
+        state[i], dp_new = _after_first_solve(cases[i], workloads[i].flow)
+        if dp_new is not None:
+            again[i] = dp_new
+    if again:
+        solve(again)
+        for i, dp_new in again.items():
+            _after_second_solve(cases[i], state[i], dp_new)
+    stab = {}
+    for i in range(n):
+        w = workloads[i]
+        pore_mean = _flow_report(state[i], w.flow, reps[i]) if i in state else 0.0
+        omp, omb, dt = ade_relaxation(w.params, pore_mean, w.flow, w.transport, reps[i])
+        cases[i].set_omegas(omp, omb, dt)
+        if i in state:
+            cases[i].ns_couple()
+            if w.transport.stabilize_steps > 0:
+                stab.setdefault(w.transport.stabilize_steps, []).append(i)
+    for steps, ids in stab.items():
+        b = Batch(ctx, [cases[i] for i in ids])
+        try:
+            b.ade_stabilize(steps)
+        finally:
+            b.close()
+    return reps

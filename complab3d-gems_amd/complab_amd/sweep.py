@@ -189,6 +189,67 @@ def serve(workloads, rate_per_s=2.0, workers=8, device=0, budget=200e9, steady_t
     return _pool(list(enumerate(workloads)), workers, device, budget, steady_tol, keep_fields, arrivals=list(arrivals))
 
 
+def _results_of(c, w, r, keep_fields):
+    if keep_fields:
+        r.fields = [c.read(P.FIELD_CONC, s) for s in range(w.params.n_species) if w.params.mobile[s]]
+        m = c.read(P.FIELD_MASK)
+        om = m >= P.PORE
+        om[:, :, 0] = False
+        om[:, :, -1] = False
+        r.open_mask = om
+
+
+def run_group(workloads, device=0, keep_fields=True, create_threads=8):
+    """The batched form: the cases are created together (their host tables
+    built in parallel threads), their set-up solves and stabilizations run in
+    batch launches (driver.setup_batch), then their steps (clb_batch_run_steps,
+    each case's own `steps`).  Returns the results and the wall time; a
+    result's `seconds` is the wall time at which its case finished."""
+    from concurrent.futures import ThreadPoolExecutor
+    from . import Batch
+    start = time.perf_counter()
+    ctx = Context(device)
+    results = [CaseResult(w.name) for w in workloads]
+    cases = []
+    try:
+        with ThreadPoolExecutor(max(1, create_threads)) as ex:
+            cases = list(ex.map(lambda w: Case(ctx, w.params, w.geometry), workloads))
+        reps = driver.setup_batch(ctx, cases, workloads)
+        for c, rep, r in zip(cases, reps, results):
+            c.set_omegas(rep.omega_pore, rep.omega_biofilm, 1.0)
+            r.flow_iters = list(rep.iters)
+        ctx.sync()
+        t_setup = time.perf_counter() - start
+        b = Batch(ctx, cases)
+        live = list(range(len(cases)))
+        try:
+            while live:
+                k = min(workloads[i].steps - results[i].steps for i in live)
+                out = b.run_steps(k)
+                still = []
+                for i, (rc, d) in zip(live, out):
+                    r = results[i]
+                    r.steps += d
+                    if rc:
+                        r.status = rc
+                    if rc or r.steps >= workloads[i].steps:
+                        b.remove(cases[i])
+                        _results_of(cases[i], workloads[i], r, keep_fields)
+                        r.setup_seconds = t_setup
+                        r.seconds = r.latency = time.perf_counter() - start
+                    else:
+                        still.append(i)
+                live = still
+        finally:
+            b.close()
+        ctx.sync()
+    finally:
+        for c in cases:
+            c.close()
+        ctx.close()
+    return results, time.perf_counter() - start
+
+
 def field_error(a, b):
     """Largest relative difference of the final mobile fields of two runs of a
     case, over the open voxels of the reaction box, scaled per species by the

@@ -82,6 +82,29 @@ hipError_t launchCaMaskFinalGroup(const AdeMember<R> *E, int nMembers, hipStream
     return hipGetLastError();
 }
 template <typename R>
+__global__ void k_ctl_reset_group(const AdeMember<R> *__restrict__ E, int n) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || !E[j].ca) return;
+    StepCtl *c = E[j].c.ctl;
+    c->halt = 0; c->done = 0; c->oldMask = E[j].oldMask;
+}
+template <typename R>
+__global__ void k_ctl_gather_group(const AdeMember<R> *__restrict__ E, int n, StepCtl *out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || !E[j].ca) return;
+    out[j] = *E[j].c.ctl;
+}
+template <typename R>
+hipError_t launchCtlResetGroup(const AdeMember<R> *E, int nMembers, hipStream_t st) {
+    if (nMembers > 0) hipLaunchKernelGGL(k_ctl_reset_group<R>, dim3((nMembers + 63) / 64), dim3(64), 0, st, E, nMembers);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launchCtlGatherGroup(const AdeMember<R> *E, int nMembers, StepCtl *out, hipStream_t st) {
+    if (nMembers > 0) hipLaunchKernelGGL(k_ctl_gather_group<R>, dim3((nMembers + 63) / 64), dim3(64), 0, st, E, nMembers, out);
+    return hipGetLastError();
+}
+template <typename R>
 __device__ __forceinline__ void caMaskFinalBlock(const CaDev<R> &c, int nparts, int check) {
     __shared__ double sMax[16];
     __shared__ long long sMask[16];
@@ -693,6 +716,8 @@ hipError_t launchDoubleToI32(const double *src, int *dst, long long n, hipStream
     template hipError_t launchCaMaskTotal<R>(const CaDev<R> &, int, hipStream_t); \
     template hipError_t launchCaMaskFinal<R>(const CaDev<R> &, int, int, hipStream_t); \
     template hipError_t launchCaMaskFinalGroup<R>(const AdeMember<R> *, int, hipStream_t); \
+    template hipError_t launchCtlResetGroup<R>(const AdeMember<R> *, int, hipStream_t); \
+    template hipError_t launchCtlGatherGroup<R>(const AdeMember<R> *, int, StepCtl *, hipStream_t); \
     template hipError_t launchCaPushGather<R>(const CaDev<R> &, uint64_t, hipStream_t); \
     template hipError_t launchCaAge<R>(const CaDev<R> &, const int *, int *, long long *, hipStream_t); \
     template hipError_t launchAgeCompact<R>(const CaDev<R> &, int *, unsigned int *, unsigned int *, hipStream_t); \

@@ -253,6 +253,7 @@ struct AdeMember {
     int gInt, gPln;                // interior and plane workgroups at the batch's tile size
     int ca;                        // 1: a CA microbe, the step's mask/total decision on the device
     long long iT0;                 // the member's steps at the start of the batch
+    long long oldMask;             // the mask sum its host accepted last
     int precipEvery, dissolEvery;  // a step on these intervals needs the host: it halts the member
 };
 
@@ -295,6 +296,14 @@ template <typename R> hipError_t launchNsStepGroup(const NsMember<R> *E, int nMe
 template <typename R> hipError_t launchAdeStepGroup(const AdeMember<R> *E, const int2 *mapInt, int nInt,
                                                     const int2 *mapPln, int nPln, int maxSm, int tile, hipStream_t);
 template <typename R> hipError_t launchCaMaskFinalGroup(const AdeMember<R> *E, int nMembers, hipStream_t);
+// the members' convergence state after a run of flow iterations: per member
+// {tracer iterations, converged, non-finite, done}
+template <typename R> hipError_t launchNsGroupStatus(const NsMember<R> *E, int nMembers, long long *out, hipStream_t);
+// the CA members' step controls: a new batch of steps (no halt, nothing
+// finished, the accepted mask sum), and a copy of each member's control
+// (slot j for member j) for one read-back
+template <typename R> hipError_t launchCtlResetGroup(const AdeMember<R> *E, int nMembers, hipStream_t);
+template <typename R> hipError_t launchCtlGatherGroup(const AdeMember<R> *E, int nMembers, StepCtl *out, hipStream_t);
 // host launchers: dense grids (clb_aux.hip)
 template <typename R> hipError_t launchCaMaskTotal(const CaDev<R> &, int check, hipStream_t);
 template <typename R> hipError_t launchCaMaskFinal(const CaDev<R> &, int nparts, int check, hipStream_t);

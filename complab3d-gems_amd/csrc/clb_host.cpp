@@ -220,6 +220,7 @@ Status checkCheckpointBlock(const uint8_t *src, size_t n, ChkHeader *hOut, clb_c
 
 struct CaseBase {
     clb_case_params p;
+    int inBatches = 0;                          // batches holding the case (it cannot be destroyed then)
     int nx = 0, ny = 0, nz = 0;
     long long N = 0;
     std::vector<void *> allocs;
@@ -616,8 +617,10 @@ struct Case : CaseBase {
             nearCnt = dalloc<unsigned int>(allocs, 1);
         }
         dcount = dalloc<unsigned long long>(allocs, 3);
-        caPartMax = dalloc<double>(allocs, nblk(Nint, 256) + nblk(Nopen - Nint, 256) + 1);
-        caPartMask = dalloc<long long>(allocs, nblk(Nint, 256) + nblk(Nopen - Nint, 256) + 1);
+        // one mask/total partial per transport workgroup, at the smallest
+        // workgroup the batched form uses (64)
+        caPartMax = dalloc<double>(allocs, nblk(Nint, 64) + nblk(Nopen - Nint, 64) + 1);
+        caPartMask = dalloc<long long>(allocs, nblk(Nint, 64) + nblk(Nopen - Nint, 64) + 1);
         {
             void *h = nullptr;
             HIPT(hipHostMalloc(&h, sizeof(StepCtl), hipHostMallocDefault));
@@ -1565,6 +1568,532 @@ struct Case : CaseBase {
     }
 };
 
+// ------------------------------------------------------------------ the batched form
+// Many cases stepped together.  Each kernel of a flow iteration or of a
+// transport step covers every member of the batch, through a table of member
+// descriptors and a map of the launch's workgroups to (member, workgroup of
+// the member's own launch), so each member computes exactly what its own
+// launches compute: the same voxel arithmetic, the same block partials summed
+// in the same order, the same step decisions on the device.  What a step
+// needs the host for (a CA spill, a mask change and its flow re-solve,
+// precipitation) runs on the member's own path between runs of steps.  Runs
+// of steps and of flow iterations are captured once in HIP graphs and
+// replayed; the transport workgroup size is the autotuner's pick.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { if (p) (void) hipFree(p); }
+    // room for n elements (the contents are not kept); true when the buffer moved
+    bool ensure(size_t n) {
+        if (p && n <= cap) return false;
+        if (p) { HIPT(hipFree(p)); p = nullptr; cap = 0; }
+        const size_t c = std::max<size_t>(n, 1);
+        HIPT(hipMalloc(&p, c * sizeof(T)));
+        cap = c;
+        return true;
+    }
+};
+template <typename T>
+struct PinBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    PinBuf() = default;
+    PinBuf(const PinBuf &) = delete;
+    PinBuf &operator=(const PinBuf &) = delete;
+    ~PinBuf() { if (p) (void) hipHostFree(p); }
+    void ensure(size_t n) {
+        if (p && n <= cap) return;
+        if (p) { HIPT(hipHostFree(p)); p = nullptr; cap = 0; }
+        const size_t c = std::max<size_t>(n, 1);
+        void *h = nullptr;
+        HIPT(hipHostMalloc(&h, c * sizeof(T), hipHostMallocDefault));
+        p = static_cast<T *>(h);
+        cap = c;
+    }
+};
+// an instantiated graph and what its kernel nodes were captured with (device
+// pointers, grid sizes): replayed while that stays the same
+struct GraphExec {
+    hipGraphExec_t exec = nullptr;
+    std::vector<uintptr_t> key;
+    GraphExec() = default;
+    GraphExec(const GraphExec &) = delete;
+    GraphExec &operator=(const GraphExec &) = delete;
+    ~GraphExec() { reset(); }
+    void reset() {
+        if (exec) (void) hipGraphExecDestroy(exec);
+        exec = nullptr;
+        key.clear();
+    }
+    template <class F>
+    void ensure(hipStream_t st, const std::vector<uintptr_t> &k, F &&body) {
+        if (exec && key == k) return;
+        reset();
+        HIPT(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        hipGraph_t g = nullptr;
+        try {
+            body();
+        } catch (...) {
+            (void) hipStreamEndCapture(st, &g);
+            if (g) (void) hipGraphDestroy(g);
+            throw;
+        }
+        HIPT(hipStreamEndCapture(st, &g));
+        const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void) hipGraphDestroy(g);
+        if (e != hipSuccess) { exec = nullptr; throw HipFail{e, "hipGraphInstantiate"}; }
+        key = k;
+    }
+    void launch(hipStream_t st) { HIPT(hipGraphLaunch(exec, st)); }
+};
+
+struct BatchBase {
+    std::vector<int> ids;
+    std::vector<CaseBase *> members;
+    virtual ~BatchBase() { for (CaseBase *c : members) c->inBatches--; }
+    virtual bool accepts(CaseBase *c) const = 0;
+    virtual void attach(CaseBase *c) = 0;
+    virtual void detach(size_t k) = 0;
+    virtual Status nsSolve(hipStream_t, int maxIter, int rule, double eps, int reset, int *iters, int *conv, int *status) = 0;
+    virtual Status stabilize(hipStream_t, int steps, int *status) = 0;
+    virtual Status runSteps(hipStream_t, int n, int *done, int *status) = 0;
+    // the transport workgroup sizes the autotuner tries, and per mode (0 the
+    // coupled steps, 1 the stabilization) the pick (-1: not tuned yet) and the
+    // measured time per step of each candidate
+    static constexpr int kTiles[3] = {256, 128, 64};
+    int pick[2] = {-1, -1};
+    bool fixed[2] = {false, false};
+    double tuneUs[2][3] = {{0, 0, 0}, {0, 0, 0}};
+};
+constexpr int BatchBase::kTiles[3];
+
+template <typename R, typename RN>
+struct Batch : BatchBase {
+    using C = Case<R, RN>;
+    static constexpr int kFlowRun = 100;        // flow iterations per replay (even: members keep their parity)
+    static constexpr int kSegSteps = 64;        // coupled steps per run between host checks
+    static constexpr int kSegStab = 512;        // stabilization steps per run
+    std::vector<C *> m;
+    // flow: descriptors [parity][member], workgroup map, convergence read-back
+    PinBuf<NsMember<RN>> nsHost;
+    DevBuf<NsMember<RN>> nsTab;
+    PinBuf<int2> nsMapHost;
+    DevBuf<int2> nsMap;
+    DevBuf<long long> nsStat;
+    PinBuf<long long> nsStatHost;
+    GraphExec nsGraph;
+    // transport, per candidate workgroup size: descriptors [parity][member] and
+    // the map (interior workgroups by species class, then the plane workgroups)
+    struct Plan {
+        struct Range { int cls, off, cnt; };
+        std::vector<Range> bulk;
+        int plnOff = 0, plnCnt = 0, members = 0;
+        bool anyCa = false;
+    };
+    PinBuf<AdeMember<R>> adeHost[3];
+    DevBuf<AdeMember<R>> adeTab[3];
+    PinBuf<int2> mapHost[3];
+    DevBuf<int2> mapDev[3];
+    Plan plan[3];
+    GraphExec pair[3];                          // two steps (both parities) per candidate
+    DevBuf<StepCtl> ctlDev;
+    PinBuf<StepCtl> ctlHost;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    ~Batch() override {
+        if (ev0) (void) hipEventDestroy(ev0);
+        if (ev1) (void) hipEventDestroy(ev1);
+    }
+    bool accepts(CaseBase *c) const override { return dynamic_cast<C *>(c) != nullptr; }
+    void attach(CaseBase *c) override { m.push_back(static_cast<C *>(c)); members.push_back(c); c->inBatches++; }
+    void detach(size_t k) override {
+        members[k]->inBatches--;
+        members.erase(members.begin() + (long) k);
+        m.erase(m.begin() + (long) k);
+        ids.erase(ids.begin() + (long) k);
+    }
+
+    // ---- flow
+    static int nsPlanes(const C *c) { return (int) nblk(c->Nopen - c->Nint, 256); }
+    static int nsInterior(const C *c) { return (int) nblk(c->Nint, 256); }
+    // descriptors and map of the members `act` (parity cur[j] now); returns the workgroups
+    int nsBuild(hipStream_t st, const std::vector<int> &act, const std::vector<char> &cur, int rule, double eps) {
+        HIPT(hipStreamSynchronize(st));                 // the staging buffers are free
+        const int n = (int) act.size();
+        nsHost.ensure(2 * (size_t) n);
+        nsTab.ensure(2 * (size_t) n);
+        size_t nb = 0;
+        for (int k = 0; k < n; ++k) {
+            C *c = m[act[k]];
+            for (int t = 0; t < 2; ++t) {
+                const bool a = (t == 0) == (cur[act[k]] != 0);
+                NsMember<RN> &e = nsHost.p[(size_t) t * n + k];
+                memset(&e, 0, sizeof(e));
+                e.a = c->ns();
+                e.a.cur = a ? c->gA : c->gB;
+                e.a.out = a ? c->gB : c->gA;
+                e.a.bbCur = a ? c->bbA : c->bbB;
+                e.a.bbOut = a ? c->bbB : c->bbA;
+                e.a.first = 0;
+                e.a.shape = 0;
+                e.gB = nsInterior(c);
+                e.gP = nsPlanes(c);
+                e.partE = c->parts;
+                e.partN = c->parts + (e.gB + e.gP);
+                e.tracer = c->tracer;
+                e.rule = rule;
+                e.eps = eps;
+                e.active = 1;
+            }
+            nb += (size_t) nsInterior(c) + nsPlanes(c);
+        }
+        if (nb > (size_t) INT_MAX) throw HipFail{hipErrorInvalidValue, "batch flow grid too large"};
+        nsMapHost.ensure(nb);
+        nsMap.ensure(nb);
+        size_t q = 0;
+        for (int k = 0; k < n; ++k) {
+            const int g = nsInterior(m[act[k]]) + nsPlanes(m[act[k]]);
+            for (int b = 0; b < g; ++b) nsMapHost.p[q++] = make_int2(k, b);
+        }
+        HIPT(hipMemcpyAsync(nsTab.p, nsHost.p, 2 * (size_t) n * sizeof(NsMember<RN>), hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpyAsync(nsMap.p, nsMapHost.p, nb * sizeof(int2), hipMemcpyHostToDevice, st));
+        return (int) nb;
+    }
+    void nsReadStatus(hipStream_t st, int n) {
+        nsStat.ensure(4 * (size_t) n);
+        nsStatHost.ensure(4 * (size_t) n);
+        HIPT(launchNsGroupStatus<RN>(nsTab.p, n, nsStat.p, st));
+        HIPT(hipMemcpyAsync(nsStatHost.p, nsStat.p, 4 * (size_t) n * sizeof(long long), hipMemcpyDeviceToHost, st));
+        HIPT(hipStreamSynchronize(st));
+    }
+    // Every member's flow solve as its own nsSolve does it: the first iteration
+    // on the member's own path (the hand-over after an initialisation, the
+    // refresh of changed classes), the rest in launches over every member
+    // still iterating, each stopping on its own tracer.
+    Status nsSolve(hipStream_t st, int maxIter, int rule, double eps, int reset, int *iters, int *conv, int *status) override {
+        const int n = (int) m.size();
+        std::vector<int> act;
+        std::vector<char> cur0(n, 1), cur(n, 1);
+        for (int j = 0; j < n; ++j) {
+            C *c = m[j];
+            int it = 0, cv = 0;
+            Status s = c->nsSolve(st, std::min(maxIter, 1), rule, eps, reset, &it, &cv);
+            iters[j] = it; conv[j] = cv; status[j] = s.code;
+            if (s.code || !c->p.flow_on || maxIter <= 1 || it < 1 || cv) continue;
+            cur0[j] = cur[j] = c->gCurIsA ? 1 : 0;
+            act.push_back(j);
+        }
+        if (act.empty()) return {CLB_OK, ""};
+        const long long want = (long long) maxIter - 1;
+        int na = (int) act.size();
+        int nb = nsBuild(st, act, cur, rule, eps);
+        nsReadStatus(st, na);
+        std::vector<long long> base(n, 0);
+        for (int k = 0; k < na; ++k) base[act[k]] = nsStatHost.p[4 * k];
+        auto finish = [&](int j, long long eff, const long long *sv) {
+            m[j]->gCurIsA = (eff & 1) ? !cur0[j] : (cur0[j] != 0);
+            iters[j] = 1 + (int) eff;
+            conv[j] = sv[1] ? 1 : 0;
+            if (sv[2]) status[j] = CLB_E_NONFINITE;
+        };
+        long long ran = 0;
+        std::vector<long long> last(4 * (size_t) n, 0);
+        while (ran < want && na > 0) {
+            const int run = (int) std::min<long long>(kFlowRun, want - ran);
+            if (run == kFlowRun) {
+                const std::vector<uintptr_t> key = {(uintptr_t) nsTab.p, (uintptr_t) nsMap.p, (uintptr_t) nb, (uintptr_t) na};
+                nsGraph.ensure(st, key, [&] {
+                    for (int k = 0; k < kFlowRun; ++k)
+                        HIPT(launchNsStepGroup<RN>(nsTab.p + (size_t) (k & 1) * na, na, nsMap.p, nb, st));
+                });
+                nsGraph.launch(st);
+            } else {
+                for (int k = 0; k < run; ++k) HIPT(launchNsStepGroup<RN>(nsTab.p + (size_t) (k & 1) * na, na, nsMap.p, nb, st));
+            }
+            nsReadStatus(st, na);
+            std::vector<int> still;
+            for (int k = 0; k < na; ++k) {
+                const int j = act[k];
+                const long long *sv = nsStatHost.p + 4 * k;
+                memcpy(&last[4 * (size_t) j], sv, 4 * sizeof(long long));
+                const long long eff = sv[0] - base[j];
+                if (!sv[3] && eff < ran + run) return {CLB_E_HIP, "flow iterations were lost"};
+                if (sv[3]) finish(j, eff, sv);
+                else still.push_back(j);
+            }
+            ran += run;
+            if ((int) still.size() != na) {
+                act.swap(still);
+                na = (int) act.size();
+                if (na > 0) {
+                    // tables at each member's parity now; the count and the
+                    // parity at the end still run from the start of the solve
+                    for (int j : act) cur[j] = (ran & 1) ? !cur0[j] : cur0[j];
+                    nb = nsBuild(st, act, cur, rule, eps);
+                }
+            }
+        }
+        for (int j : act) finish(j, ran, &last[4 * (size_t) j]);
+        return {CLB_OK, ""};
+    }
+
+    // ---- transport
+    static int sClass(int Sm) { return Sm <= 1 ? 1 : Sm <= 2 ? 2 : Sm <= 3 ? 3 : Sm <= 4 ? 4 : Sm <= 6 ? 6 : 8; }
+    static bool stepsGroupable(const C *c) {
+        return c->Sm > 0 && c->Nint > 0 && !c->symOn && !c->symAb && !c->rateOn && !c->p.eq_on && !c->p.dissol_on;
+    }
+    static bool stabGroupable(const C *c) { return c->Sm > 0 && c->Nint > 0; }
+    // descriptors and map of group G for candidate ti: the coupled step
+    // (kinetics, CA decision) or the stabilization step (transport alone)
+    void adeBuild(hipStream_t st, const std::vector<int> &G, int ti, bool steps) {
+        HIPT(hipStreamSynchronize(st));
+        const int T = kTiles[ti], n = (int) G.size();
+        adeHost[ti].ensure(2 * (size_t) n);
+        adeTab[ti].ensure(2 * (size_t) n);
+        Plan &P = plan[ti];
+        P = Plan();
+        P.members = n;
+        size_t nInt = 0, nPln = 0;
+        for (int k = 0; k < n; ++k) {
+            C *c = m[G[k]];
+            const bool ca = steps && c->p.n_microbes > 0;
+            P.anyCa = P.anyCa || ca;
+            for (int t = 0; t < 2; ++t) {
+                const bool a = (t == 0) == c->fCurIsA;
+                AdeMember<R> &e = adeHost[ti].p[(size_t) t * n + k];
+                memset(&e, 0, sizeof(e));
+                e.a = c->ade(ca ? &c->ctl->halt : nullptr);
+                e.a.cur = a ? c->fA : c->fB;
+                e.a.out = a ? c->fB : c->fA;
+                e.a.first = 0;
+                e.a.reset = 0;
+                e.a.maskTotal = ca ? 1 : 0;
+                e.k = c->kin(steps);
+                e.k.dissol = 0;
+                e.k.nearPhase = 0;
+                if (!e.k.bio && e.k.nrxn == 0) e.k.on = 0;
+                e.c = c->ca();
+                e.gInt = (int) nblk(c->Nint, T);
+                e.gPln = (int) nblk(c->Nopen - c->Nint, T);
+                e.ca = ca ? 1 : 0;
+                e.iT0 = c->iT;
+                e.oldMask = c->oldTotMask;
+                e.precipEvery = c->p.precip_on && c->p.precip_interval > 0 && c->p.precip_species >= 0 ? c->p.precip_interval : 0;
+                e.dissolEvery = c->p.dissol_on && c->p.dissol_interval > 0 ? c->p.dissol_interval : 0;
+            }
+            const AdeMember<R> &e = adeHost[ti].p[k];
+            if (e.gPln >= 65536) throw HipFail{hipErrorInvalidValue, "too many plane workgroups for the batch map"};
+            nInt += (size_t) e.gInt;
+            nPln += (size_t) e.gPln * (size_t) c->Sm;
+        }
+        if (nInt + nPln > (size_t) INT_MAX) throw HipFail{hipErrorInvalidValue, "batch transport grid too large"};
+        mapHost[ti].ensure(nInt + nPln);
+        mapDev[ti].ensure(nInt + nPln);
+        int q = 0;
+        const int classes[6] = {1, 2, 3, 4, 6, 8};
+        for (int cl : classes) {
+            const int off = q;
+            for (int k = 0; k < n; ++k) {
+                if (sClass(m[G[k]]->Sm) != cl) continue;
+                const int g = adeHost[ti].p[k].gInt;
+                for (int b = 0; b < g; ++b) mapHost[ti].p[q++] = make_int2(k, b);
+            }
+            if (q > off) P.bulk.push_back({cl, off, q - off});
+        }
+        P.plnOff = q;
+        for (int k = 0; k < n; ++k) {
+            const int g = adeHost[ti].p[k].gPln, sm = m[G[k]]->Sm;
+            for (int ms = 0; ms < sm; ++ms)
+                for (int b = 0; b < g; ++b) mapHost[ti].p[q++] = make_int2(k, b | (ms << 16));
+        }
+        P.plnCnt = q - P.plnOff;
+        HIPT(hipMemcpyAsync(adeTab[ti].p, adeHost[ti].p, 2 * (size_t) n * sizeof(AdeMember<R>), hipMemcpyHostToDevice, st));
+        HIPT(hipMemcpyAsync(mapDev[ti].p, mapHost[ti].p, (size_t) q * sizeof(int2), hipMemcpyHostToDevice, st));
+    }
+    // one step of every member at parity t with candidate ti
+    void adeStep(hipStream_t st, int ti, int t) {
+        const Plan &P = plan[ti];
+        const AdeMember<R> *E = adeTab[ti].p + (size_t) t * P.members;
+        const int T = kTiles[ti];
+        for (const auto &b : P.bulk) HIPT(launchAdeStepGroup<R>(E, mapDev[ti].p + b.off, b.cnt, nullptr, 0, b.cls, T, st));
+        if (P.plnCnt) HIPT(launchAdeStepGroup<R>(E, nullptr, 0, mapDev[ti].p + P.plnOff, P.plnCnt, 1, T, st));
+        if (P.anyCa) HIPT(launchCaMaskFinalGroup<R>(E, P.members, st));
+    }
+    void adePair(hipStream_t st, int ti) {
+        const Plan &P = plan[ti];
+        std::vector<uintptr_t> key = {(uintptr_t) adeTab[ti].p, (uintptr_t) mapDev[ti].p, (uintptr_t) P.members,
+                                      (uintptr_t) P.anyCa, (uintptr_t) P.plnOff, (uintptr_t) P.plnCnt};
+        for (const auto &b : P.bulk) { key.push_back((uintptr_t) b.cls); key.push_back((uintptr_t) b.off); key.push_back((uintptr_t) b.cnt); }
+        pair[ti].ensure(st, key, [&] { adeStep(st, ti, 0); adeStep(st, ti, 1); });
+        pair[ti].launch(st);
+    }
+    // L steps of group G, pairs of steps replayed from a graph (each member
+    // back at its parity after a pair).  The first run of a mode with room for
+    // it captures the pair of every candidate workgroup size, replays each
+    // once untimed and once timed, and keeps the fastest: every candidate
+    // computes the same steps, so the pick changes only the time.  Returns
+    // the candidate that ran the last steps.
+    int adeRun(hipStream_t st, const std::vector<int> &G, int L, int mode) {
+        const bool steps = mode == 0;
+        const int ng = (int) G.size();
+        int left = L;
+        int ti = pick[mode] >= 0 ? pick[mode] : 0;
+        if (pick[mode] < 0 && L >= 12) {
+            if (!ev0) { HIPT(hipEventCreate(&ev0)); HIPT(hipEventCreate(&ev1)); }
+            for (int c = 0; c < 3; ++c) adeBuild(st, G, c, steps);
+            if (steps) HIPT(launchCtlResetGroup<R>(adeTab[0].p, ng, st));
+            for (int c = 0; c < 3; ++c) adePair(st, c);
+            for (int c = 0; c < 3; ++c) {
+                HIPT(hipEventRecord(ev0, st));
+                pair[c].launch(st);
+                HIPT(hipEventRecord(ev1, st));
+                HIPT(hipEventSynchronize(ev1));
+                float ms = 0;
+                HIPT(hipEventElapsedTime(&ms, ev0, ev1));
+                tuneUs[mode][c] = 500.0 * ms;              // microseconds per step
+            }
+            left -= 12;
+            ti = 0;
+            for (int c = 1; c < 3; ++c) if (tuneUs[mode][c] < tuneUs[mode][ti]) ti = c;
+            pick[mode] = ti;
+        } else {
+            adeBuild(st, G, ti, steps);
+            if (steps) HIPT(launchCtlResetGroup<R>(adeTab[ti].p, ng, st));
+        }
+        for (int p = 0; p < left / 2; ++p) adePair(st, ti);
+        if (left & 1) adeStep(st, ti, 0);
+        return ti;
+    }
+
+    // Every member's stabilization as its own adeStabilize does it: a step
+    // that needs the member's own path first (hand-over, reset, changed
+    // classes), the rest together, then the reset mark.
+    Status stabilize(hipStream_t st, int steps, int *status) override {
+        const int nm = (int) m.size();
+        std::vector<int> G, left(nm, 0);
+        for (int j = 0; j < nm; ++j) {
+            status[j] = CLB_OK;
+            C *c = m[j];
+            if (!stabGroupable(c) || steps <= 0) { c->adeStabilize(st, steps); continue; }
+            int todo = steps;
+            if (c->adeFirst || c->adeReset || c->adeClsDirty) { c->adeCollideStream(st, 1); --todo; }
+            left[j] = todo;
+            if (todo > 0) G.push_back(j);
+            else c->adeReset = true;
+        }
+        while (!G.empty()) {
+            int L = kSegStab;
+            for (int j : G) L = std::min(L, left[j]);
+            adeRun(st, G, L, 1);
+            std::vector<int> next;
+            for (int j : G) {
+                C *c = m[j];
+                if (L & 1) c->fCurIsA = !c->fCurIsA;
+                left[j] -= L;
+                if (left[j] > 0) next.push_back(j);
+                else c->adeReset = true;
+            }
+            G.swap(next);
+        }
+        HIPT(hipStreamSynchronize(st));
+        return {CLB_OK, ""};
+    }
+
+    // Every member's coupled steps as its own runSteps takes them.  Members
+    // the group launches do not cover (equilibrium, dissolution, symbolic
+    // laws, rate capture, no mobile species) run on their own path.  The
+    // others run together in runs that end at the first step any member
+    // needs the host for; within a run a CA member's own step decision stops
+    // it on the device (its later launches return at once), and the host
+    // then finishes that step on the member's path.
+    Status runSteps(hipStream_t st, int n, int *done, int *status) override {
+        const int nm = (int) m.size();
+        std::vector<int> G;
+        for (int j = 0; j < nm; ++j) {
+            done[j] = 0;
+            status[j] = CLB_OK;
+            C *c = m[j];
+            if (n <= 0) continue;
+            if (!stepsGroupable(c) || c->clogged) {
+                int d = 0;
+                Status s = c->runSteps(st, n, &d);
+                done[j] = d;
+                status[j] = s.code;
+                continue;
+            }
+            G.push_back(j);
+        }
+        while (!G.empty()) {
+            std::vector<int> ready;
+            for (int j : G) {
+                C *c = m[j];
+                if (c->adeFirst || c->adeReset || c->adeClsDirty) {
+                    int d = 0;
+                    Status s = c->runSteps(st, 1, &d);
+                    done[j] += d;
+                    if (s.code) { status[j] = s.code; continue; }
+                }
+                if (done[j] < n) ready.push_back(j);
+            }
+            G.swap(ready);
+            if (G.empty()) break;
+            int L = kSegSteps;
+            for (int j : G) {
+                C *c = m[j];
+                L = std::min(L, n - done[j]);
+                for (int d = 1; d <= L; ++d)
+                    if (c->hostDue(c->iT + d - 1)) { L = d; break; }
+            }
+            const int ti = adeRun(st, G, L, 0);
+            const int ng = (int) G.size();
+            if (plan[ti].anyCa) {
+                ctlDev.ensure(ng);
+                ctlHost.ensure(ng);
+                HIPT(launchCtlGatherGroup<R>(adeTab[ti].p, ng, ctlDev.p, st));
+                HIPT(hipMemcpyAsync(ctlHost.p, ctlDev.p, (size_t) ng * sizeof(StepCtl), hipMemcpyDeviceToHost, st));
+            }
+            HIPT(hipStreamSynchronize(st));
+            std::vector<int> next;
+            for (int k = 0; k < ng; ++k) {
+                const int j = G[k];
+                C *c = m[j];
+                Status s{CLB_OK, ""};
+                if (c->p.n_microbes > 0) {
+                    const StepCtl cs = ctlHost.p[k];
+                    const int fin = cs.done;
+                    const bool halt = cs.halt != 0;
+                    const int ranK = fin + (halt ? 1 : 0);            // launches that did work
+                    if (fin < 0 || ranK > L) return {CLB_E_HIP, "step batch bookkeeping failed"};
+                    if (ranK & 1) c->fCurIsA = !c->fCurIsA;
+                    c->iT += fin;
+                    c->cnt[7] += fin;
+                    done[j] += fin;
+                    if (halt) {
+                        s = c->stepTail(st, cs.gmax, cs.msum);
+                        if (s.code == CLB_OK || s.code == CLB_CLOGGED) done[j] += 1;
+                    }
+                } else {
+                    // no step of the run but its last can need the host
+                    if (L & 1) c->fCurIsA = !c->fCurIsA;
+                    c->iT += L - 1;
+                    c->cnt[7] += L - 1;
+                    done[j] += L - 1;
+                    s = c->stepTail(st, 0.0, 0);
+                    if (s.code == CLB_OK || s.code == CLB_CLOGGED) done[j] += 1;
+                }
+                if (s.code) { status[j] = s.code; continue; }
+                if (done[j] < n) next.push_back(j);
+            }
+            G.swap(next);
+        }
+        return {CLB_OK, ""};
+    }
+};
+
 }  // namespace
 
 // ------------------------------------------------------------------ static scratch
@@ -1598,8 +2127,11 @@ struct clb_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    std::mutex mu;                              // guards the maps (cases may be created from several threads)
     std::map<int, std::unique_ptr<CaseBase>> cases;
+    std::map<int, std::unique_ptr<BatchBase>> batches;
     int next = 1;
+    int nextBatch = 1;
 };
 
 namespace {
@@ -1608,8 +2140,14 @@ int fail(clb_ctx *ctx, int code, const std::string &msg) {
     return code;
 }
 CaseBase *getCase(clb_ctx *ctx, int id) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
     auto it = ctx->cases.find(id);
     return it == ctx->cases.end() ? nullptr : it->second.get();
+}
+BatchBase *getBatch(clb_ctx *ctx, int id) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->batches.find(id);
+    return it == ctx->batches.end() ? nullptr : it->second.get();
 }
 
 #define GUARD_BEGIN try {
@@ -1678,6 +2216,7 @@ int clb_ctx_create(int device, clb_ctx **out) {
 int clb_ctx_destroy(clb_ctx *ctx) {
     if (!ctx) return CLB_E_ARG;
     (void) hipSetDevice(ctx->device);
+    ctx->batches.clear();
     ctx->cases.clear();
     if (ctx->stream) (void) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1717,6 +2256,7 @@ size_t clb_case_footprint(const clb_case_params *p) {
     b += 3 * N * r;
     b += N * sizeof(double);                            // staging
     b += ((N + 127) / 128) * (5 * sizeof(double) + sizeof(long long));
+    b += ((N + 63) / 64 + 2) * (sizeof(double) + sizeof(long long));   // mask/total partials
     return b;
 }
 
@@ -1730,6 +2270,7 @@ int clb_case_create(clb_ctx *ctx, const clb_case_params *p, const int8_t *geomet
     if (p->precision == CLB_FP32) { auto *k = new Case<float>(); c.reset(k); k->create(*p, geometry, biomass0, ctx->stream); }
     else if (p->precision == CLB_FP32_ADE) { auto *k = new Case<float, double>(); c.reset(k); k->create(*p, geometry, biomass0, ctx->stream); }
     else { auto *k = new Case<double>(); c.reset(k); k->create(*p, geometry, biomass0, ctx->stream); }
+    std::lock_guard<std::mutex> lk(ctx->mu);
     int id = ctx->next++;
     ctx->cases[id] = std::move(c);
     *case_id = id;
@@ -1738,10 +2279,17 @@ int clb_case_create(clb_ctx *ctx, const clb_case_params *p, const int8_t *geomet
 }
 int clb_case_destroy(clb_ctx *ctx, int case_id) {
     if (!ctx) return CLB_E_ARG;
-    auto it = ctx->cases.find(case_id);
-    if (it == ctx->cases.end()) return fail(ctx, CLB_E_ARG, "unknown case id");
+    std::unique_ptr<CaseBase> gone;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        auto it = ctx->cases.find(case_id);
+        if (it == ctx->cases.end()) return fail(ctx, CLB_E_ARG, "unknown case id");
+        if (it->second->inBatches > 0) return fail(ctx, CLB_E_STATE, "the case belongs to a batch: remove it first");
+        gone = std::move(it->second);
+        ctx->cases.erase(it);
+    }
     (void) hipSetDevice(ctx->device);
-    ctx->cases.erase(it);
+    gone.reset();
     return CLB_OK;
 }
 
@@ -2065,6 +2613,125 @@ int clb_kernel_time(clb_ctx *ctx, int case_id, int reset, double *avg_ms, int64_
     GUARD_BEGIN
     CASE_OR_FAIL(c)
     c->kernelTime(ctx->stream, reset, avg_ms, launches);
+    return CLB_OK;
+    GUARD_END
+}
+
+// ---- the batched form
+#define BATCH_OR_FAIL(var)                                                                     \
+    if (!ctx) return CLB_E_ARG;                                                                \
+    BatchBase *var = getBatch(ctx, batch_id);                                                  \
+    if (!var) return fail(ctx, CLB_E_ARG, "unknown batch id");                                 \
+    HIPT(hipSetDevice(ctx->device));
+
+int clb_batch_create(clb_ctx *ctx, int precision, const int *case_ids, int n, int *batch_id) {
+    if (!ctx || !batch_id || n < 0 || (n > 0 && !case_ids)) return CLB_E_ARG;
+    GUARD_BEGIN
+    std::unique_ptr<BatchBase> b;
+    if (precision == CLB_FP64) b.reset(new Batch<double, double>());
+    else if (precision == CLB_FP32) b.reset(new Batch<float, float>());
+    else if (precision == CLB_FP32_ADE) b.reset(new Batch<float, double>());
+    else return fail(ctx, CLB_E_ARG, "unknown precision");
+    for (int k = 0; k < n; ++k) {
+        CaseBase *c = getCase(ctx, case_ids[k]);
+        if (!c) return fail(ctx, CLB_E_ARG, "unknown case id");
+        if (!b->accepts(c)) return fail(ctx, CLB_E_ARG, "a batch holds cases of its own precision only");
+        for (int q = 0; q < k; ++q)
+            if (case_ids[q] == case_ids[k]) return fail(ctx, CLB_E_ARG, "a case appears twice in the batch");
+    }
+    for (int k = 0; k < n; ++k) { b->attach(getCase(ctx, case_ids[k])); b->ids.push_back(case_ids[k]); }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const int id = ctx->nextBatch++;
+    ctx->batches[id] = std::move(b);
+    *batch_id = id;
+    return CLB_OK;
+    GUARD_END
+}
+int clb_batch_destroy(clb_ctx *ctx, int batch_id) {
+    if (!ctx) return CLB_E_ARG;
+    std::unique_ptr<BatchBase> gone;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        auto it = ctx->batches.find(batch_id);
+        if (it == ctx->batches.end()) return fail(ctx, CLB_E_ARG, "unknown batch id");
+        gone = std::move(it->second);
+        ctx->batches.erase(it);
+    }
+    (void) hipSetDevice(ctx->device);
+    gone.reset();
+    return CLB_OK;
+}
+int clb_batch_add(clb_ctx *ctx, int batch_id, int case_id) {
+    GUARD_BEGIN
+    BATCH_OR_FAIL(b)
+    CaseBase *c = getCase(ctx, case_id);
+    if (!c) return fail(ctx, CLB_E_ARG, "unknown case id");
+    if (!b->accepts(c)) return fail(ctx, CLB_E_ARG, "a batch holds cases of its own precision only");
+    for (int id : b->ids) if (id == case_id) return fail(ctx, CLB_E_ARG, "the case is in the batch already");
+    b->attach(c);
+    b->ids.push_back(case_id);
+    return CLB_OK;
+    GUARD_END
+}
+int clb_batch_remove(clb_ctx *ctx, int batch_id, int case_id) {
+    GUARD_BEGIN
+    BATCH_OR_FAIL(b)
+    for (size_t k = 0; k < b->ids.size(); ++k)
+        if (b->ids[k] == case_id) { b->detach(k); return CLB_OK; }
+    return fail(ctx, CLB_E_ARG, "the case is not in the batch");
+    GUARD_END
+}
+int clb_batch_members(clb_ctx *ctx, int batch_id, int *n, int *case_ids, int cap) {
+    GUARD_BEGIN
+    BATCH_OR_FAIL(b)
+    if (!n) return fail(ctx, CLB_E_ARG, "null output");
+    *n = (int) b->ids.size();
+    if (case_ids) for (int k = 0; k < *n && k < cap; ++k) case_ids[k] = b->ids[k];
+    return CLB_OK;
+    GUARD_END
+}
+int clb_batch_ns_solve(clb_ctx *ctx, int batch_id, int max_iter, int rule, double eps, int reset_tracer, int *iters,
+                       int *converged, int *status) {
+    GUARD_BEGIN
+    BATCH_OR_FAIL(b)
+    if (!iters || !converged || !status) return fail(ctx, CLB_E_ARG, "null output arrays");
+    if (rule != CLB_CONV_TRACER && rule != CLB_CONV_RELDELTA) return fail(ctx, CLB_E_ARG, "unknown convergence rule");
+    Status s = b->nsSolve(ctx->stream, max_iter, rule, eps, reset_tracer, iters, converged, status);
+    return s.code ? fail(ctx, s.code, s.msg) : CLB_OK;
+    GUARD_END
+}
+int clb_batch_ade_stabilize(clb_ctx *ctx, int batch_id, int steps, int *status) {
+    GUARD_BEGIN
+    BATCH_OR_FAIL(b)
+    if (!status) return fail(ctx, CLB_E_ARG, "null output array");
+    Status s = b->stabilize(ctx->stream, steps, status);
+    return s.code ? fail(ctx, s.code, s.msg) : CLB_OK;
+    GUARD_END
+}
+int clb_batch_run_steps(clb_ctx *ctx, int batch_id, int nsteps, int *done, int *status) {
+    GUARD_BEGIN
+    BATCH_OR_FAIL(b)
+    if (!done || !status) return fail(ctx, CLB_E_ARG, "null output arrays");
+    Status s = b->runSteps(ctx->stream, nsteps, done, status);
+    return s.code ? fail(ctx, s.code, s.msg) : CLB_OK;
+    GUARD_END
+}
+int clb_batch_set_tile(clb_ctx *ctx, int batch_id, int mode, int tile) {
+    GUARD_BEGIN
+    BATCH_OR_FAIL(b)
+    if (mode != CLB_BATCH_STEPS && mode != CLB_BATCH_STABILIZE) return fail(ctx, CLB_E_ARG, "unknown batch mode");
+    if (tile == 0) { b->pick[mode] = -1; return CLB_OK; }
+    for (int c = 0; c < 3; ++c)
+        if (BatchBase::kTiles[c] == tile) { b->pick[mode] = c; return CLB_OK; }
+    return fail(ctx, CLB_E_ARG, "workgroup size must be 0 (tune), 64, 128 or 256");
+    GUARD_END
+}
+int clb_batch_tuning(clb_ctx *ctx, int batch_id, int mode, int *tile, double us_per_step[3]) {
+    GUARD_BEGIN
+    BATCH_OR_FAIL(b)
+    if (mode != CLB_BATCH_STEPS && mode != CLB_BATCH_STABILIZE) return fail(ctx, CLB_E_ARG, "unknown batch mode");
+    if (tile) *tile = b->pick[mode] >= 0 ? BatchBase::kTiles[b->pick[mode]] : 0;
+    if (us_per_step) for (int c = 0; c < 3; ++c) us_per_step[c] = b->tuneUs[mode][c];
     return CLB_OK;
     GUARD_END
 }

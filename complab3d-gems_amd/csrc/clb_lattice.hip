@@ -1998,6 +1998,21 @@ hipError_t launchNsStepGroup(const NsMember<R> *E, int nMembers, const int2 *map
     if (nMembers > 0) hipLaunchKernelGGL(k_ns_tracer_group<R>, dim3(nMembers), dim3(1024), 0, st, E);
     return hipGetLastError();
 }
+template <typename R>
+__global__ void k_ns_group_status(const NsMember<R> *__restrict__ E, int n, long long *out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const NsTracer *t = E[j].tracer;
+    out[4 * j + 0] = t->iters;
+    out[4 * j + 1] = t->converged;
+    out[4 * j + 2] = t->nonfinite;
+    out[4 * j + 3] = *E[j].a.done;
+}
+template <typename R>
+hipError_t launchNsGroupStatus(const NsMember<R> *E, int nMembers, long long *out, hipStream_t st) {
+    if (nMembers > 0) hipLaunchKernelGGL(k_ns_group_status<R>, dim3((nMembers + 63) / 64), dim3(64), 0, st, E, nMembers, out);
+    return hipGetLastError();
+}
 template <typename R, int MAXS>
 __global__ void __launch_bounds__(256) k_ade_bulk_group(const AdeMember<R> *__restrict__ E, const int2 *__restrict__ map) {
     const int2 e = map[xcdBlockOf((int) blockIdx.x, (int) gridDim.x)];
@@ -2049,6 +2064,7 @@ hipError_t launchAdeStepGroup(const AdeMember<R> *E, const int2 *mapInt, int nIn
     template hipError_t launchNsReopen<R>(const NsDev<R> &, const uint8_t *, hipStream_t); \
     template hipError_t launchNsBBFill<R>(const NsDev<R> &, const R *, R *, hipStream_t); \
     template hipError_t launchNsStepGroup<R>(const NsMember<R> *, int, const int2 *, int, hipStream_t); \
+    template hipError_t launchNsGroupStatus<R>(const NsMember<R> *, int, long long *, hipStream_t); \
     template hipError_t launchAdeStepGroup<R>(const AdeMember<R> *, const int2 *, int, const int2 *, int, int, int, hipStream_t); \
     template hipError_t launchScatterCompact<R>(const R *, const int *, int, double *, hipStream_t); \
     template hipError_t launchGatherCompact<R>(const double *, const int *, int, R *, hipStream_t);

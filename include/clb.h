@@ -439,6 +439,43 @@ int  clb_set_timing_stride(clb_ctx *ctx, int case_id, int stride);
 #define CLB_TUNE_NS_SHAPE 0
 int  clb_set_tuning(clb_ctx *ctx, int case_id, int knob, int value);
 
+/* ---- the batched form ------------------------------------------------------
+ * No counterpart in the reference, which runs one case per process
+ * (complab.cpp main); a parameter sweep there is many processes
+ * (ABC_sweep_package/README_1_SCENARIO.md:552-557).  A batch steps many cases
+ * of one precision together: each kernel of a flow iteration or of a transport
+ * step covers every member (a table of member descriptors and a map of the
+ * launch's workgroups to member workgroups), runs of steps and of flow
+ * iterations replay HIP graphs, and the transport workgroup size is picked by
+ * timing the candidates (64, 128, 256) on the batch's first steps.  Every
+ * member ends exactly where its own calls (clb_ns_solve, clb_ade_stabilize,
+ * clb_run_steps) would take it: the same voxel arithmetic, the same block
+ * partials summed in the same order, the same step decisions.  Members the
+ * group launches do not cover (equilibrium, dissolution, symbolic laws, rate
+ * capture, no mobile species) run on their own path inside the same call.
+ * Per-member outputs are in member order (creation order, then additions).
+ * A member's own failure (clog, non-finite state, ...) is its status entry,
+ * not the call's return code.  A case in a batch cannot be destroyed. */
+#define CLB_BATCH_STEPS     0
+#define CLB_BATCH_STABILIZE 1
+int  clb_batch_create(clb_ctx *ctx, int precision, const int *case_ids, int n, int *batch_id);
+int  clb_batch_destroy(clb_ctx *ctx, int batch_id);
+/* continuous batching: members join and leave between calls */
+int  clb_batch_add(clb_ctx *ctx, int batch_id, int case_id);
+int  clb_batch_remove(clb_ctx *ctx, int batch_id, int case_id);
+int  clb_batch_members(clb_ctx *ctx, int batch_id, int *n, int *case_ids, int cap);
+/* clb_ns_solve of every member; iters / converged / status per member */
+int  clb_batch_ns_solve(clb_ctx *ctx, int batch_id, int max_iter, int rule, double eps, int reset_tracer,
+                        int *iters, int *converged, int *status);
+/* clb_ade_stabilize of every member */
+int  clb_batch_ade_stabilize(clb_ctx *ctx, int batch_id, int steps, int *status);
+/* clb_run_steps of every member: done / status per member */
+int  clb_batch_run_steps(clb_ctx *ctx, int batch_id, int nsteps, int *done, int *status);
+/* the transport workgroup size of a mode: 64, 128, 256, or 0 to tune again */
+int  clb_batch_set_tile(clb_ctx *ctx, int batch_id, int mode, int tile);
+/* the pick (0: not tuned yet) and each candidate's measured microseconds per step */
+int  clb_batch_tuning(clb_ctx *ctx, int batch_id, int mode, int *tile, double us_per_step[3]);
+
 /* device copy bandwidth of a plain vectorised copy kernel over `bytes` bytes,
  * best of `reps` launches (HIP events); the measured ceiling the roofline
  * fractions are read against and the calibration of the HBM byte counters.
