@@ -142,56 +142,78 @@ def cpu_model():
     return "unknown"
 
 
+def _progress(tag):
+    def log(r):
+        print("%s: %s done, %d steps, %.2f s" % (tag, r.name, r.steps, r.seconds), file=sys.stderr, flush=True)
+    return log
+
+
 def sweep_measurement(args):
+    """The batched form against the plain path: the same seeded cases, each
+    through its full set-up and `--sweep-steps` coupled steps, one case at a
+    time in FP64 (the plain per-case path) and all together through the
+    batched entry points (sweep.run_group: one launch per flow iteration and
+    per transport step over every member, graph-replayed runs, the autotuned
+    transport workgroup) in FP64, FP32 transport with an FP64 flow, and FP32."""
     from complab_amd import configs, sweep
     from complab_amd import params as P
-    n, steps, W = args.sweep_cases, args.sweep_steps, args.workers
+    n, steps = args.sweep_cases, args.sweep_steps
     plain = [configs.sweep_case(s, P.FP64, steps=steps) for s in range(n)]
-    r64, t64 = sweep.run_sequential(plain)
+    r64, t64 = sweep.run_sequential(plain, log=_progress("sweep plain fp64"))
     if any(r.status for r in r64):
         raise SystemExit("a plain sweep case failed: %s" % [r.error or r.status for r in r64])
     cph64 = n / t64 * 3600.0
-    res = {"cases": n, "steps": steps, "workers": W,
+    vox = sum(w.params.nx * w.params.ny * w.params.nz for w in plain)
+    res = {"cases": n, "steps": steps, "voxels_total": int(vox),
            "workload": "seeded batched-form cases: edges 32..160, aspect <= 4, spheres/cylinders/fracture, "
                        "porosity 0.25-0.5, 2-6 species, flow tau 0.6-1.2, Monod growth; full set-up per case",
            "plain_fp64": {"wall_s": round(t64, 3), "cases_per_hour": round(cph64, 1)}}
-    for label, prec in (("fp32_ade", P.FP32_ADE), ("fp32", P.FP32)):
+    for label, prec in (("fp64", P.FP64), ("fp32_ade", P.FP32_ADE), ("fp32", P.FP32)):
         ws = [configs.sweep_case(s, prec, steps=steps) for s in range(n)]
-        r, t = sweep.run_batched(ws, workers=W)
+        r, t = sweep.run_group(ws, log=_progress("sweep batched " + label))
         if any(x.status for x in r):
             raise SystemExit("a batched sweep case failed: %s" % [x.error or x.status for x in r])
         err = max(sweep.field_error(a, b) for a, b in zip(r, r64))
         res["batched_" + label] = {"wall_s": round(t, 3), "cases_per_hour": round(n / t * 3600.0, 1),
-                                   "speedup_vs_plain_fp64": round(t64 / t, 3), "max_rel_err": float("%.3g" % err)}
-    print("sweep: plain %.1f s, batched fp32_ade %.1f s, fp32 %.1f s" %
-          (t64, res["batched_fp32_ade"]["wall_s"], res["batched_fp32"]["wall_s"]), file=sys.stderr, flush=True)
+                                   "speedup_vs_plain_fp64": round(t64 / t, 3), "max_rel_err": float("%.3g" % err),
+                                   "setup_s": round(max(x.setup_seconds for x in r), 3)}
+        print("sweep: batched %s %.1f s (plain %.1f s), error %.3g" % (label, t, t64, err), file=sys.stderr, flush=True)
     return res
 
 
 def serving_measurement(args):
-    """Serving runs FP64: precipitation seals a voxel at a threshold, so any
-    lower precision can seal it a check earlier or later and change the pore
-    space; in FP64 every request's result equals its plain one-at-a-time run."""
+    """Continuous batching (sweep.serve_group) of ABC-style requests in FP32
+    transport with an FP64 flow, against the plain path: the same requests in
+    FP64, one at a time.  Precipitation seals a voxel at a threshold, so a
+    reduced precision can seal it a check earlier or later and change the pore
+    space: the line reports the error of every request and how many are within
+    1e-5."""
     from complab_amd import configs, sweep
     from complab_amd import params as P
-    n, W = args.serve_requests, args.workers
+    n = args.serve_requests
     size = (48, max(48, args.serve_size))
-    reqs = [configs.serving_case(100 + s, P.FP64, size=size, max_steps=args.serve_steps) for s in range(n)]
-    r, wall = sweep.serve(reqs, rate_per_s=2.0, workers=W, steady_tol=1e-6, seed=7)
-    if any(x.status for x in r):
+    reqs = [configs.serving_case(100 + s, P.FP32_ADE, size=size, max_steps=args.serve_steps) for s in range(n)]
+    r, wall = sweep.serve_group(reqs, rate_per_s=args.serve_rate, steady_tol=1e-6, seed=7,
+                                log=_progress("serving"))
+    if any(x.status not in (0, 1) for x in r):
         raise SystemExit("a serving request failed: %s" % [x.error or x.status for x in r])
     lat = [x.latency for x in r]
-    plain, t_plain = sweep.run_sequential(reqs, steady_tol=1e-6)
-    err = max(sweep.field_error(a, b) for a, b in zip(r, plain))
-    print("serving: wall %.1f s, p50 %.2f s, p99 %.2f s, plain one-at-a-time %.1f s" %
-          (wall, sweep.percentile(lat, 50), sweep.percentile(lat, 99), t_plain), file=sys.stderr, flush=True)
-    return {"requests": n, "rate_per_s": 2.0, "workers": W, "precision": "fp64",
+    plain_w = [configs.serving_case(100 + s, P.FP64, size=size, max_steps=args.serve_steps) for s in range(n)]
+    plain, t_plain = sweep.run_sequential(plain_w, steady_tol=1e-6, log=_progress("serving plain fp64"))
+    errs = [sweep.field_error(a, b) for a, b in zip(r, plain)]
+    print("serving: wall %.1f s, p50 %.2f s, p99 %.2f s, plain one-at-a-time %.1f s, worst error %.3g" %
+          (wall, sweep.percentile(lat, 50), sweep.percentile(lat, 99), t_plain, max(errs)), file=sys.stderr, flush=True)
+    return {"requests": n, "rate_per_s": args.serve_rate, "precision": "fp32 transport, fp64 flow",
             "workload": "ABC-style A+B->mineral, precipitation with flow re-solve every 100 steps, edges %d..%d, "
                         "3-5 species, to steady state (1e-6) or %d steps" % (size[0], size[1], args.serve_steps),
+            "edges": [[w.params.nx - 2, w.params.ny, w.params.nz] for w in reqs],
             "p50_s": round(sweep.percentile(lat, 50), 3), "p99_s": round(sweep.percentile(lat, 99), 3),
             "wall_s": round(wall, 3), "cases_per_hour": round(n / wall * 3600.0, 1),
             "plain_fp64_wall_s": round(t_plain, 3), "plain_fp64_cases_per_hour": round(n / t_plain * 3600.0, 1),
-            "steps": [x.steps for x in r], "max_rel_err_vs_plain_fp64": float("%.3g" % err)}
+            "speedup_vs_plain_fp64": round(t_plain / wall, 3),
+            "steps": [x.steps for x in r], "plain_steps": [x.steps for x in plain],
+            "rel_err_vs_plain_fp64": [float("%.3g" % e) for e in errs],
+            "within_1e-5": int(sum(e <= 1e-5 for e in errs))}
 
 
 def main():
@@ -207,12 +229,12 @@ def main():
     ap.add_argument("--ns-max-iter", type=int, default=100000,
                     help="cap of the set-up flow solves (profiling runs shorten the untimed set-up)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched and serving measurements")
-    ap.add_argument("--sweep-cases", type=int, default=16)
+    ap.add_argument("--sweep-cases", type=int, default=64)
     ap.add_argument("--sweep-steps", type=int, default=1000)
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-steps", type=int, default=5000)
-    ap.add_argument("--serve-size", type=int, default=128, help="largest edge of a serving request")
-    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--serve-size", type=int, default=256, help="largest edge of a serving request")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="request arrivals per second")
     ap.add_argument("--max-to-spill", type=int, default=3000,
                     help="untimed steps allowed after the warm-up to reach the CA spill")
     args = ap.parse_args()

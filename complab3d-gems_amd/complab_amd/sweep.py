@@ -113,9 +113,10 @@ class _Admission:
             self.cv.notify_all()
 
 
-def _pool(jobs, workers, device, budget, steady_tol, keep_fields, arrivals=None):
+def _pool(jobs, workers, device, budget, steady_tol, keep_fields, arrivals=None, log=None):
     """Runs (index, workload) jobs on `workers` threads; with `arrivals` each
-    job enters the queue at its arrival time (seconds after the start)."""
+    job enters the queue at its arrival time (seconds after the start).  `log`,
+    when given, is called with each finished result."""
     q = queue.Queue()
     results = [None] * len(jobs)
     adm = _Admission(budget)
@@ -144,6 +145,8 @@ def _pool(jobs, workers, device, budget, steady_tol, keep_fields, arrivals=None)
                     adm.release(fp)
                 r.latency = time.perf_counter() - start - t_arr
                 results[i] = r
+                if log is not None:
+                    log(r)
         finally:
             ctx.close()
 
@@ -168,9 +171,9 @@ def _pool(jobs, workers, device, budget, steady_tol, keep_fields, arrivals=None)
     return results, wall
 
 
-def run_sequential(workloads, device=0, steady_tol=None, keep_fields=True):
+def run_sequential(workloads, device=0, steady_tol=None, keep_fields=True, log=None):
     """The plain per-case path: one case at a time on one stream."""
-    return _pool(list(enumerate(workloads)), 1, device, float("inf"), steady_tol, keep_fields)
+    return _pool(list(enumerate(workloads)), 1, device, float("inf"), steady_tol, keep_fields, log=log)
 
 
 def run_batched(workloads, workers=8, device=0, budget=200e9, steady_tol=None, keep_fields=True):
@@ -199,7 +202,7 @@ def _results_of(c, w, r, keep_fields):
         r.open_mask = om
 
 
-def run_group(workloads, device=0, keep_fields=True, create_threads=8):
+def run_group(workloads, device=0, keep_fields=True, create_threads=8, log=None):
     """The batched form: the cases are created together (their host tables
     built in parallel threads), their set-up solves and stabilizations run in
     batch launches (driver.setup_batch), then their steps (clb_batch_run_steps,
@@ -237,6 +240,8 @@ def run_group(workloads, device=0, keep_fields=True, create_threads=8):
                         _results_of(cases[i], workloads[i], r, keep_fields)
                         r.setup_seconds = t_setup
                         r.seconds = r.latency = time.perf_counter() - start
+                        if log is not None:
+                            log(r)
                     else:
                         still.append(i)
                 live = still
@@ -246,6 +251,104 @@ def run_group(workloads, device=0, keep_fields=True, create_threads=8):
     finally:
         for c in cases:
             c.close()
+        ctx.close()
+    return results, time.perf_counter() - start
+
+
+def _steady(tot, prev, mobile, tol):
+    scale = np.maximum(np.abs(tot[mobile]), 1e-300)
+    return bool(np.max(np.abs(tot[mobile] - prev[mobile]) / scale) < tol)
+
+
+def serve_group(workloads, rate_per_s=2.0, device=0, budget=200e9, steady_tol=1e-6, check_every=100, seed=0,
+                keep_fields=True, create_threads=8, log=None):
+    """The serving loop over the batched form (continuous batching): requests
+    arrive as a Poisson stream of `rate_per_s`; between runs of `check_every`
+    steps the requests that have arrived and whose footprint fits in what the
+    running ones leave of `budget` are created, set up together
+    (driver.setup_batch) and join the running batch, and members that reached
+    steady state (mobile totals changing by less than `steady_tol`, relative,
+    over the run), their step limit or a stop leave it.  Every member checks at
+    multiples of `check_every` of its own steps, as run_workload does, so each
+    result equals its one-at-a-time run.  Returns the results (latency =
+    arrival to last step) and the wall time."""
+    from collections import deque
+    from concurrent.futures import ThreadPoolExecutor
+    from . import Batch
+    n = len(workloads)
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate_per_s, size=n))
+    arrivals -= arrivals[0]
+    results = [CaseResult(w.name) for w in workloads]
+    pending = deque(range(n))
+    running = {}                     # request -> [case, totals at the last check, footprint]
+    used = 0
+    start = time.perf_counter()
+    ctx = Context(device)
+    b = Batch(ctx, precision=workloads[0].params.precision) if n else None
+    try:
+        with ThreadPoolExecutor(max(1, create_threads)) as ex:
+            while pending or running:
+                now = time.perf_counter() - start
+                new = []
+                while pending and arrivals[pending[0]] <= now:
+                    fp = footprint(workloads[pending[0]].params)
+                    if (running or new) and used + fp > budget:
+                        break
+                    used += fp
+                    new.append((pending.popleft(), fp))
+                if not running and not new:
+                    time.sleep(max(0.0, arrivals[pending[0]] - now))
+                    continue
+                if new:
+                    ids = [i for i, _ in new]
+                    cases = list(ex.map(lambda i: Case(ctx, workloads[i].params, workloads[i].geometry), ids))
+                    reps = driver.setup_batch(ctx, cases, [workloads[i] for i in ids])
+                    for (i, fp), c, rep in zip(new, cases, reps):
+                        c.set_omegas(rep.omega_pore, rep.omega_biofilm, 1.0)
+                        results[i].flow_iters = list(rep.iters)
+                        results[i].setup_seconds = time.perf_counter() - start - arrivals[i]
+                        b.add(c)
+                        running[i] = [c, c.species_totals(), fp]
+                order = [next(i for i, v in running.items() if v[0] is c) for c in b.cases]
+
+                def next_check(i):
+                    s = results[i].steps
+                    return min((s // check_every + 1) * check_every, workloads[i].steps)
+                k = min(next_check(i) - results[i].steps for i in order)
+                due = {i: next_check(i) for i in order}
+                out = b.run_steps(k)
+                for i, (rc, d) in zip(order, out):
+                    c, prev, fp = running[i]
+                    r, w = results[i], workloads[i]
+                    r.steps += d
+                    done = rc != 0 or r.steps >= w.steps
+                    if rc:
+                        r.status = rc
+                    if not done and r.steps == due[i]:
+                        tot = c.species_totals()
+                        mob = [s for s in range(w.params.n_species) if w.params.mobile[s]]
+                        done = _steady(tot, prev, mob, steady_tol)
+                        running[i][1] = tot
+                    if done:
+                        b.remove(c)
+                        _results_of(c, w, r, keep_fields)
+                        c.close()
+                        del running[i]
+                        used -= fp
+                        r.latency = time.perf_counter() - start - arrivals[i]
+                        r.seconds = r.latency
+                        if log is not None:
+                            log(r)
+    finally:
+        for v in running.values():
+            try:
+                b.remove(v[0])
+            except Exception:       # noqa: BLE001 - tearing down after an error
+                pass
+            v[0].close()
+        if b is not None:
+            b.close()
         ctx.close()
     return results, time.perf_counter() - start
 
