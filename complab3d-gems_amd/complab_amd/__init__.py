@@ -101,6 +101,9 @@ def lib():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = res
+    if L.clb_abi_version() != P.ABI_VERSION:
+        raise RuntimeError("libclb.so has ABI version %d, this package expects %d: rebuild it"
+                           % (L.clb_abi_version(), P.ABI_VERSION))
     _lib = L
     return L
 

@@ -287,18 +287,11 @@ def config3(scale=1.0, precision=P.FP64, seed=3, steps=5000):
     p.precip_on = 1; p.precip_species = CALCITE_; p.precip_max_rho = 0.05
     p.precip_interval = 100; p.precip_phase_id = 1; p.precip_perm_ratio = 0.0
     # CaCO3 + H -> Ca + HCO3 on the wetted faces of the sealed voxels
-    p.dissol_on = 1
-    p.dissol_species = CALCITE_
-    p.dissol_material = -1
-    p.dissol_is_precipitate = 1
-    p.dissol_phase_id = 1
-    p.dissol_full = 0.05
+    P.set_phase(p, 0, CALCITE_, 0.05, precipitate=True, k_rate=20.0, acid=H_,
+                stoich={H_: -1.0, CA_: 1.0, HCO3_: 1.0})
     p.dissol_reopen_frac = 0.9
     p.dissol_surface_only = 1
     p.dissol_interval = 100
-    p.dissol_k = 20.0
-    p.dissol_acid = H_
-    p.dissol_stoich[H_] = -1.0; p.dissol_stoich[CA_] = 1.0; p.dissol_stoich[HCO3_] = 1.0
     p.ns_tau = 0.8
     p.flow_on = 1
     # the set-up solve to 1e-8 and the in-step re-solves to 1e-6: the

@@ -10,10 +10,11 @@ parameter block, the flow and transport set-up and the geometry.
 Two things the files do not carry: the rate laws (the reference compiles them
 from defineKinetics.hh / defineAbioticKinetics.hh / defineDissolutionRate into
 the executable), which the caller sets on `params` (bio_law, mu_max, rxn_*,
-dissol_k, ...); and the kinetics families this build runs (dual Monod,
-mass action, the examples' dissolution law) bound what those can be.  A run the
-path cannot take -- a finite-difference or LB biomass solver, FBA or surrogate
-reactions, more than one dissolvable phase -- is refused with the reason.
+dissol_k[phase], ...); and the kinetics families this build runs (dual Monod,
+mass action, the dissolution power law of each phase) bound what those can
+be.  A run the path cannot take -- a finite-difference or LB biomass solver,
+FBA or surrogate reactions, more than four dissolvable phases -- is refused
+with the reason.
 
 readGeometry reads one x slice at a time through Palabos' stream operator,
 which walks y and then z fastest; the reference's generator (writeDat,
@@ -497,7 +498,7 @@ def load(xml_path, geometry_path: Optional[str] = None, precision=P.FP64, order=
         p.precip_perm_ratio = doc.num("precipitation", "perm_ratio", default=0.0)
         p.precip_interval = doc.num("precipitation", "update_interval", default=200, cast=int)
 
-    # ---- dissolution (complab.cpp:516-574): one declared phase on this path
+    # ---- dissolution (complab.cpp:516-574): declared phases <phase0>, <phase1>, ...
     if doc.flag("dissolution", "enabled"):
         phases = []
         for k in range(64):
@@ -506,30 +507,33 @@ def load(xml_path, geometry_path: Optional[str] = None, precision=P.FP64, order=
             phases.append(k)
         if not phases:
             raise InputError("<dissolution> is enabled but no <phase0> block was given")
-        if len(phases) > 1:
-            raise InputError("%d dissolvable phases; this path takes one" % len(phases))
-        ph = ("dissolution", "phase0")
+        if len(phases) > P.MAX_PHASES:
+            raise InputError("%d dissolvable phases; this path takes up to %d" % (len(phases), P.MAX_PHASES))
         p.dissol_on = 1
-        mat = doc.num(*ph, "material_number", cast=int)
-        p.dissol_is_precipitate = 1 if doc.flag(*ph, "is_precipitate") else 0
-        if not p.dissol_is_precipitate and mats.get(mat) not in (P.SOLID, P.WALL):
-            raise InputError("a grain phase lies on the solid or bounce-back material on this path")
-        p.dissol_material = mats.get(mat, -1)
-        p.dissol_species = doc.num(*ph, "substrate", default=-1, cast=int)
-        p.dissol_full = doc.num(*ph, "full_density", default=0.0)
-        p.dissol_initial = doc.num(*ph, "initial_fill", default=0.0)
-        p.dissol_phase_id = 1
+        p.dissol_n_phases = len(phases)
         p.dissol_reopen_frac = doc.num("dissolution", "reopen_fraction", default=0.9)
         p.dissol_surface_only = doc.num("dissolution", "surface_only", default=1, cast=int)
         p.dissol_interval = doc.num("dissolution", "update_interval", default=100, cast=int)
-        if not 0 <= p.dissol_species < S:
-            raise InputError("dissolution substrate %d is outside 0..%d" % (p.dissol_species, S - 1))
-        if not p.dissol_full > 0:
-            raise InputError("<full_density> must be positive")
         if not 0 < p.dissol_reopen_frac < 1:
             raise InputError("<reopen_fraction> must be strictly between 0 and 1")
-        if p.dissol_is_precipitate:
-            p.precip_phase_id = 1
+        for k in phases:
+            ph = ("dissolution", "phase%d" % k)
+            mat = doc.num(*ph, "material_number", cast=int)
+            precipitate = doc.flag(*ph, "is_precipitate")
+            if not precipitate and mats.get(mat) not in (P.SOLID, P.WALL):
+                raise InputError("grain phase %d lies on material %d; on this path a grain phase lies on the "
+                                 "solid or bounce-back material" % (k, mat))
+            p.dissol_is_precipitate[k] = 1 if precipitate else 0
+            p.dissol_material[k] = -1 if precipitate else mats[mat]
+            p.dissol_species[k] = doc.num(*ph, "substrate", default=-1, cast=int)
+            p.dissol_full[k] = doc.num(*ph, "full_density", default=0.0)
+            p.dissol_initial[k] = 0.0 if precipitate else doc.num(*ph, "initial_fill", default=0.0)
+            if not 0 <= p.dissol_species[k] < S:
+                raise InputError("dissolution substrate %d is outside 0..%d" % (p.dissol_species[k], S - 1))
+            if not p.dissol_full[k] > 0:
+                raise InputError("<full_density> of phase %d must be positive" % k)
+            if precipitate:
+                p.precip_phase_id = k + 1          # the last precipitate phase declared (complab.cpp:530)
 
     # ---- thermodynamic gate (complab3d_integration.hh:315-326, 810-860)
     if doc.flag("thermodynamics", "enabled"):

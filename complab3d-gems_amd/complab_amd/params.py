@@ -6,10 +6,12 @@ load time.
 """
 import ctypes as C
 
+ABI_VERSION = 2       # CLB_ABI_VERSION of include/clb.h
 MAX_SPECIES = 8
 MAX_MICROBES = 2
 MAX_RXN = 4
 MAX_COMP = 8
+MAX_PHASES = 4
 
 OK = 0
 CLOGGED = 1           # not an error: the run stopped, the pore space no longer percolates
@@ -91,18 +93,20 @@ class CaseParams(C.Structure):
         ("precip_interval", _i),
         ("precip_phase_id", _i),
         ("dissol_on", _i),
-        ("dissol_species", _i),
-        ("dissol_material", _i),
-        ("dissol_is_precipitate", _i),
-        ("dissol_phase_id", _i),
+        ("dissol_n_phases", _i),
         ("dissol_surface_only", _i),
         ("dissol_interval", _i),
-        ("dissol_acid", _i),
-        ("dissol_full", _d),
-        ("dissol_initial", _d),
         ("dissol_reopen_frac", _d),
-        ("dissol_k", _d),
-        ("dissol_stoich", _d * MAX_SPECIES),
+        ("dissol_species", _i * MAX_PHASES),
+        ("dissol_material", _i * MAX_PHASES),
+        ("dissol_is_precipitate", _i * MAX_PHASES),
+        ("dissol_acid", _i * MAX_PHASES),
+        ("dissol_full", _d * MAX_PHASES),
+        ("dissol_initial", _d * MAX_PHASES),
+        ("dissol_k", _d * MAX_PHASES),
+        ("dissol_acid_order", _d * MAX_PHASES),
+        ("dissol_mineral_order", _d * MAX_PHASES),
+        ("dissol_stoich", (_d * MAX_SPECIES) * MAX_PHASES),
         ("thm_on", _i),
         ("thm_microbe", _i),
         ("thm_rt", _d),
@@ -159,10 +163,12 @@ def default_params(nx, ny, nz, n_species):
     p.eq_anderson_depth = 4
     p.precip_species = -1
     p.precip_interval = 1
-    p.dissol_species = -1
-    p.dissol_material = -1
-    p.dissol_acid = -1
-    p.dissol_phase_id = 1
+    for k in range(MAX_PHASES):
+        p.dissol_species[k] = -1
+        p.dissol_material[k] = -1
+        p.dissol_acid[k] = -1
+        p.dissol_acid_order[k] = 1.0
+        p.dissol_mineral_order[k] = 1.0
     p.dissol_surface_only = 1
     p.dissol_interval = 1
     p.dissol_reopen_frac = 0.9
@@ -200,3 +206,32 @@ def copy_params(p):
     q = CaseParams()
     C.memmove(C.byref(q), C.byref(p), C.sizeof(CaseParams))
     return q
+
+
+def set_phase(p, k, species, full, material=-1, initial=0.0, precipitate=False, k_rate=0.0, acid=-1,
+              acid_order=1.0, mineral_order=1.0, stoich=None):
+    """Declares dissolvable phase k (id k + 1, <phase k> of <dissolution>) in
+    the parameter block and turns dissolution on; the rate law is
+    rate = k_rate max(C[acid],0)^acid_order max(mineral,0)^mineral_order with
+    products stoich[s] * rate (defineDissolutionRate of the reference's
+    examples: both orders 1).  A precipitate phase is also made the one
+    precipitation stamps."""
+    if not 0 <= k < MAX_PHASES:
+        raise ValueError("phase index %d outside 0..%d" % (k, MAX_PHASES - 1))
+    p.dissol_on = 1
+    p.dissol_n_phases = max(p.dissol_n_phases, k + 1)
+    p.dissol_species[k] = species
+    p.dissol_full[k] = full
+    p.dissol_material[k] = -1 if precipitate else material
+    p.dissol_initial[k] = 0.0 if precipitate else initial
+    p.dissol_is_precipitate[k] = 1 if precipitate else 0
+    p.dissol_k[k] = k_rate
+    p.dissol_acid[k] = acid
+    p.dissol_acid_order[k] = acid_order
+    p.dissol_mineral_order[k] = mineral_order
+    for s in range(MAX_SPECIES):
+        p.dissol_stoich[k][s] = 0.0
+    for s, v in (stoich or {}).items():
+        p.dissol_stoich[k][s] = v
+    if precipitate:
+        p.precip_phase_id = k + 1

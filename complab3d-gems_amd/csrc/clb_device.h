@@ -8,6 +8,7 @@
 #define CLB_DEV_MAXS 8
 #define CLB_DEV_MAXM 2
 #define CLB_DEV_MAXR 4
+#define CLB_DEV_MAXP 4
 #define NS_TRACER_WINDOW 500
 
 namespace clb {
@@ -92,6 +93,14 @@ struct AdeDev {
 };
 
 // rate-law parameters (biotic dual Monod, abiotic mass action)
+// the reopen check of the declared phases (dissolNodeConversion3D): phase id
+// q + 1 reopens below threshold[q] of the immobile slot slot[q]
+struct DissolConv {
+    int n;
+    int slot[CLB_DEV_MAXP];
+    double threshold[CLB_DEV_MAXP];
+};
+
 struct KinDev {
     int on, bio, S, M;
     double dt;
@@ -105,15 +114,19 @@ struct KinDev {
     double rxn_stoich[CLB_DEV_MAXR][CLB_DEV_MAXS];
     double cap_frac, cap_dt, min_conc, rate_scale;
     int surface_only;
-    // dissolution of one declared solid phase (dissolutionVOP.hh)
+    // dissolution of the declared solid phases (dissolutionVOP.hh): phase
+    // id p + 1 keeps its inventory in species dMineral[p] and dissolves by
+    // rate = dK max(C[dAcid],0)^dAo max(mineral,0)^dMo, products dStoich
     int dissol;                    // the gather runs with the abiotic increment
-    int dPhase, dMineral, dAcid, dSurfaceOnly;
-    double dK, dStoich[CLB_DEV_MAXS];
+    int dPhases, dSurfaceOnly;
+    int dMineral[CLB_DEV_MAXP], dAcid[CLB_DEV_MAXP];
+    double dK[CLB_DEV_MAXP], dAo[CLB_DEV_MAXP], dMo[CLB_DEV_MAXP];
+    double dStoich[CLB_DEV_MAXP][CLB_DEV_MAXS];
     const int8_t *phase;           // dense phase ids
     double *share;                 // [S][Nopen] per-face product share parked this step
     unsigned long long *dcount;    // [0] shares parked, [1] collected, [2] collected and clamped
     // the voxels next to the phase (a face neighbour in the reaction box of
-    // phase dPhase) take the gather: 0 every voxel as usual; 1 (the transport
+    // a declared phase) take the gather: 0 every voxel as usual; 1 (the transport
     // launch) their abiotic increment is left for the pass after the park;
     // 2 (that pass) only they are visited
     int nearPhase;
@@ -267,8 +280,8 @@ template <typename R> hipError_t launchKineticsListed(const AdeDev<R> &, const K
 template <typename R> hipError_t launchNearPhaseList(const AdeDev<R> &, const KinDev &, int *sel, unsigned int *count,
                                                     hipStream_t);
 template <typename R> hipError_t launchDissolPark(const AdeDev<R> &, const KinDev &, hipStream_t);
-template <typename R> hipError_t launchDissolConvert(const AdeDev<R> &, int8_t *phase, int phaseId, int mineralSlot,
-                                                     double threshold, unsigned int *count, hipStream_t);
+template <typename R> hipError_t launchDissolConvert(const AdeDev<R> &, int8_t *phase, const DissolConv &,
+                                                     unsigned int *count, hipStream_t);
 template <typename R> hipError_t launchAdeDensity(const AdeDev<R> &, int, double *, hipStream_t);
 template <typename R> hipError_t launchAdeSet(const AdeDev<R> &, int, const double *, hipStream_t);
 template <typename R> hipError_t launchAdeReopen(const AdeDev<R> &, const uint8_t *, hipStream_t);

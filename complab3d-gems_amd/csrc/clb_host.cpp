@@ -43,6 +43,16 @@ struct Status {
 
 // ------------------------------------------------------------------ checks
 // the parameter block's own consistency (everything but the geometry)
+// the grain phase seeded on a voxel of geometry code gv: the id (k + 1) of the
+// first declared grain phase on that material, 0 for none (initPhaseLattice3D,
+// dissolutionVOP.hh:300-326, over the grain phases only, complab.cpp:1672-1683)
+int grainPhaseOf(const clb_case_params &p, int gv) {
+    if (!p.dissol_on) return 0;
+    for (int k = 0; k < p.dissol_n_phases && k < CLB_MAX_PHASES; ++k)
+        if (!p.dissol_is_precipitate[k] && p.dissol_material[k] == gv) return k + 1;
+    return 0;
+}
+
 Status checkParamBlock(const clb_case_params *p) {
     if (!p) return {CLB_E_ARG, "null parameters"};
     if (p->nx < 3 || p->ny < 1 || p->nz < 1) return {CLB_E_ARG, "lattice must be at least 3 x 1 x 1"};
@@ -76,23 +86,35 @@ Status checkParamBlock(const clb_case_params *p) {
     }
     if (p->precip_on && (p->precip_species < 0 || p->precip_species >= p->n_species || p->mobile[p->precip_species]))
         return {CLB_E_ARG, "precip_species must be an immobile species"};
+    if (p->precip_on && (p->precip_phase_id < 0 || p->precip_phase_id > 127)) return {CLB_E_ARG, "precip_phase_id must be 0..127"};
     if (p->dissol_on) {
-        const int sM = p->dissol_species;
-        if (sM < 0 || sM >= p->n_species || p->mobile[sM]) return {CLB_E_ARG, "dissol_species must be an immobile species"};
-        if (p->dissol_phase_id < 1 || p->dissol_phase_id > 127) return {CLB_E_ARG, "dissol_phase_id must be 1..127"};
+        if (p->dissol_n_phases < 1 || p->dissol_n_phases > CLB_MAX_PHASES)
+            return {CLB_E_ARG, "dissol_n_phases must be 1..CLB_MAX_PHASES"};       // complab.cpp:533-537
         if (!(p->dissol_reopen_frac > 0.0) || p->dissol_reopen_frac >= 1.0)
             return {CLB_E_ARG, "dissol_reopen_frac must lie in (0, 1)"};             // complab.cpp:539-543
         if (p->dissol_interval < 1) return {CLB_E_ARG, "dissol_interval must be >= 1"};
-        if (p->dissol_acid < -1 || p->dissol_acid >= p->n_species) return {CLB_E_ARG, "dissol_acid out of range"};
-        if (p->dissol_is_precipitate) {
-            if (!p->precip_on || p->precip_phase_id != p->dissol_phase_id || p->precip_species != sM)
-                return {CLB_E_ARG, "a precipitate phase needs precipitation on, stamping dissol_phase_id into dissol_species"};
-        } else if (p->dissol_material < 0 || p->dissol_material >= CLB_PORE) {
-            return {CLB_E_ARG, "a grain phase needs a closed material number (0 or 1)"};
+        for (int k = 0; k < p->dissol_n_phases; ++k) {
+            const int sM = p->dissol_species[k];
+            if (sM < 0 || sM >= p->n_species || p->mobile[sM])                       // complab.cpp:556-560
+                return {CLB_E_ARG, "a phase's dissol_species must be an immobile species"};
+            if (!(p->dissol_full[k] > 0.0)) return {CLB_E_ARG, "a phase's dissol_full must be positive"};   // 561-566
+            if (p->dissol_acid[k] < -1 || p->dissol_acid[k] >= p->n_species) return {CLB_E_ARG, "dissol_acid out of range"};
+            if (!std::isfinite(p->dissol_k[k]) || !(p->dissol_acid_order[k] >= 0.0) || !(p->dissol_mineral_order[k] >= 0.0) ||
+                !std::isfinite(p->dissol_acid_order[k]) || !std::isfinite(p->dissol_mineral_order[k]))
+                return {CLB_E_ARG, "a phase's rate constant must be finite and its orders finite and >= 0"};
+            if (!p->dissol_is_precipitate[k] && (p->dissol_material[k] < 0 || p->dissol_material[k] >= CLB_PORE))
+                return {CLB_E_ARG, "a grain phase needs a closed material number (0 or 1)"};
+            if (!p->dissol_is_precipitate[k] && !(p->dissol_initial[k] >= 0.0))
+                return {CLB_E_ARG, "a grain phase's dissol_initial must be >= 0"};
+            for (int s = 0; s < p->n_species; ++s)
+                if (!p->mobile[s] && s != sM && p->dissol_stoich[k][s] != 0.0)
+                    return {CLB_E_ARG, "dissolution products must be mobile species"};
         }
-        for (int s = 0; s < p->n_species; ++s)
-            if (!p->mobile[s] && s != sM && p->dissol_stoich[s] != 0.0)
-                return {CLB_E_ARG, "dissolution products must be mobile species"};
+        if (p->precip_on && p->precip_phase_id > 0) {
+            const int k = p->precip_phase_id - 1;
+            if (k >= p->dissol_n_phases || !p->dissol_is_precipitate[k] || p->dissol_species[k] != p->precip_species)
+                return {CLB_E_ARG, "precip_phase_id must name a precipitate phase whose substrate is precip_species"};
+        }
     }
     if (p->bio_law != CLB_BIO_NONE) {
         if (p->donor >= p->n_species || p->acceptor >= p->n_species) return {CLB_E_ARG, "donor/acceptor index out of range"};
@@ -418,9 +440,13 @@ struct Case : CaseBase {
         k.cap_frac = p.rxn_cap_frac; k.cap_dt = p.rxn_cap_dt; k.min_conc = p.rxn_min_conc;
         k.rate_scale = p.abiotic_rate_scale; k.surface_only = p.surface_only;
         k.dissol = p.dissol_on ? 1 : 0;
-        k.dPhase = p.dissol_phase_id; k.dMineral = p.dissol_species; k.dAcid = p.dissol_acid;
-        k.dSurfaceOnly = p.dissol_surface_only; k.dK = p.dissol_k;
-        for (int s = 0; s < CLB_DEV_MAXS; ++s) k.dStoich[s] = p.dissol_stoich[s];
+        k.dPhases = p.dissol_on ? p.dissol_n_phases : 0;
+        k.dSurfaceOnly = p.dissol_surface_only;
+        for (int q = 0; q < CLB_DEV_MAXP; ++q) {
+            k.dMineral[q] = p.dissol_species[q]; k.dAcid[q] = p.dissol_acid[q];
+            k.dK[q] = p.dissol_k[q]; k.dAo[q] = p.dissol_acid_order[q]; k.dMo[q] = p.dissol_mineral_order[q];
+            for (int s = 0; s < CLB_DEV_MAXS; ++s) k.dStoich[q][s] = p.dissol_stoich[q][s];
+        }
         k.phase = phase; k.share = share; k.dcount = dcount;
         k.sym = symDev; k.symOn = symOn ? 1 : 0; k.symMask = symMask;
         k.compiled = k.bio;
@@ -537,8 +563,7 @@ struct Case : CaseBase {
         std::vector<int> lst, d2c(N, -1);
         // voxels that can carry solute: open geometry, and the grain of a
         // dissolvable phase (closed until it dissolves and reopens)
-        const bool grainPhase = p.dissol_on && !p.dissol_is_precipitate && p.dissol_material >= 0;
-        auto listed = [&](int gv) { return gv >= CLB_PORE || (grainPhase && gv == p.dissol_material); };
+        auto listed = [&](int gv) { return gv >= CLB_PORE || grainPhaseOf(p, gv) > 0; };
         for (int pass = 0; pass < 3; ++pass)
             for (long long v = 0; v < N; ++v) {
                 const int x = (int) (v % nx);
@@ -644,8 +669,7 @@ struct Case : CaseBase {
             ac[i] = gv == CLB_PORE ? ADE_PORE : (gv > CLB_PORE ? ADE_BFILM : ADE_BB);
             nc[i] = nsClassOf(gv);
         }
-        if (grainPhase)
-            for (long long i = 0; i < N; ++i) if (g[i] == p.dissol_material) ph[i] = (int8_t) p.dissol_phase_id;
+        for (long long i = 0; i < N; ++i) ph[i] = (int8_t) grainPhaseOf(p, g[i]);      // initPhaseLattice3D
         HIPT(hipMemcpyAsync(mask, g.data(), N, hipMemcpyHostToDevice, st));
         HIPT(hipMemcpyAsync(geom, g.data(), N, hipMemcpyHostToDevice, st));
         HIPT(hipMemcpyAsync(dist, dh.data(), N, hipMemcpyHostToDevice, st));
@@ -687,8 +711,11 @@ struct Case : CaseBase {
             for (int s = 0; s < p.n_species; ++s) {
                 if (p.mobile[s]) continue;
                 std::fill(sc.begin(), sc.end(), (R) p.c0[s]);
-                if (grainPhase && s == p.dissol_species && p.dissol_initial > 0.0)   // complab.cpp:1684-1698
-                    for (long long i = 0; i < N; ++i) if (g[i] == p.dissol_material) sc[i] = (R) p.dissol_initial;
+                // each grain phase's inventory on its material, in phase order (complab.cpp:1687-1698)
+                for (int q = 0; p.dissol_on && q < p.dissol_n_phases; ++q) {
+                    if (p.dissol_is_precipitate[q] || !(p.dissol_initial[q] > 0.0) || p.dissol_species[q] != s) continue;
+                    for (long long i = 0; i < N; ++i) if (g[i] == p.dissol_material[q]) sc[i] = (R) p.dissol_initial[q];
+                }
                 HIPT(hipMemcpy(imm + (size_t) slot[s] * N, sc.data(), N * sizeof(R), hipMemcpyHostToDevice));
             }
             std::vector<double> tot(N, 0.0);
@@ -1076,10 +1103,15 @@ struct Case : CaseBase {
     Status dissolConvert(hipStream_t st, int *reopenedOut) override {
         unsigned int n = 0;
         if (reopenedOut) *reopenedOut = 0;
-        if (!p.dissol_on || !(p.dissol_full > 0.0)) return {CLB_OK, ""};
+        if (!p.dissol_on) return {CLB_OK, ""};
+        DissolConv dc;
+        dc.n = p.dissol_n_phases;
+        for (int q = 0; q < CLB_DEV_MAXP; ++q) {
+            dc.slot[q] = q < dc.n ? slot[p.dissol_species[q]] : 0;
+            dc.threshold[q] = q < dc.n ? p.dissol_reopen_frac * p.dissol_full[q] : 0.0;
+        }
         HIPT(hipMemsetAsync(ucnt + 3, 0, sizeof(unsigned int), st));
-        HIPT(launchDissolConvert<R>(ade(), phase, p.dissol_phase_id, slot[p.dissol_species],
-                                    p.dissol_reopen_frac * p.dissol_full, ucnt + 3, st));
+        HIPT(launchDissolConvert<R>(ade(), phase, dc, ucnt + 3, st));
         HIPT(hipMemcpyAsync(&n, ucnt + 3, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
         HIPT(hipStreamSynchronize(st));
         if (n) nearDirty = true;
@@ -2179,7 +2211,7 @@ int checkParams(clb_ctx *ctx, const clb_case_params *p, const int8_t *geometry) 
     }
     if (open == 0) return fail(ctx, CLB_E_GEOMETRY, "the geometry has no open voxel");
     // (a dissolvable grain of solid material is meant to touch the water)
-    const bool solidGrainPhase = p->dissol_on && !p->dissol_is_precipitate && p->dissol_material == CLB_SOLID;
+    const bool solidGrainPhase = grainPhaseOf(*p, CLB_SOLID) > 0;
     for (int z = 0; z < p->nz && !solidGrainPhase; ++z) for (int y = 0; y < p->ny; ++y) for (int x = 0; x < p->nx; ++x) {
         long long i = x + (long long) p->nx * (y + (long long) p->ny * z);
         if (geometry[i] != CLB_SOLID) continue;

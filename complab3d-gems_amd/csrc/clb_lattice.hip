@@ -326,21 +326,21 @@ __device__ void symAbioticIncrements(const KinDev &k, const double *C, double *d
     }
 }
 
-// a face neighbour of compact voxel i in the reaction box belongs to the
-// dissolving phase: i collects that neighbour's parked products (the
+// a face neighbour of compact voxel i in the reaction box belongs to a
+// declared phase: i collects that neighbour's parked products (the
 // condition of the gather below and, seen from the phase voxel, the open
-// neighbours whose water the park pass averages).  A voxel of the phase
+// neighbours whose water the park pass averages).  A voxel of a phase
 // itself counts too: the park reads its mineral.
 template <typename R>
 __device__ __forceinline__ bool nextToPhase(const AdeDev<R> &a, const KinDev &k, int i) {
     const int own = k.phase[a.list[i]];
-    bool near = own > 0 && own == k.dPhase;
+    bool near = own > 0 && own <= k.dPhases;
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
         const int n = a.nb[(opp7(d + 1) - 1) * a.Nopen + i];
         if (n < 0 || n >= a.Nint) continue;
         const int pid = k.phase[a.list[n]];
-        if (pid > 0 && pid == k.dPhase) near = true;
+        if (pid > 0 && pid <= k.dPhases) near = true;
     }
     return near;
 }
@@ -460,7 +460,7 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
         bool react = k.nrxn > 0, deferred = false;
         if (react && (k.surface_only || wantNear)) {
             bool surf = false, near = false;
-            if (wantNear) { const int own = k.phase[v]; near = own > 0 && own == k.dPhase; }
+            if (wantNear) { const int own = k.phase[v]; near = own > 0 && own <= k.dPhases; }
 #pragma unroll
             for (int q = 1; q < 7; ++q) {
                 const int n = a.nb[(opp7(q) - 1) * No + i];                // the voxel at +c_q
@@ -469,7 +469,7 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
                 const long long vn = a.list[n];
                 const int nm = a.mask[vn];
                 if (nm == 0 || nm == 1) surf = true;
-                if (wantNear && n < a.Nint) { const int pid = k.phase[vn]; if (pid > 0 && pid == k.dPhase) near = true; }
+                if (wantNear && n < a.Nint) { const int pid = k.phase[vn]; if (pid > 0 && pid <= k.dPhases) near = true; }
             }
             if (k.surface_only) react = surf;
             deferred = react && near;
@@ -507,10 +507,11 @@ __device__ __forceinline__ void voxelKinetics(const AdeDev<R> &a, const KinDev &
                     const int n = a.nb[(opp7(d + 1) - 1) * No + i];        // the face neighbour at +dloc[d]
                     if (n < 0 || n >= a.Nint) continue;                    // outside the reaction box or closed geometry
                     const int pid = k.phase[a.list[n]];
-                    if (pid <= 0 || pid != k.dPhase) continue;
+                    if (pid <= 0 || pid > k.dPhases) continue;            // a declared phase parks
+                    const int sM = k.dMineral[pid - 1];
 #pragma unroll
                     for (int s = 0; s < CLB_DEV_MAXS; ++s) {
-                        if (s >= S || s == k.dMineral) continue;
+                        if (s >= S || s == sM) continue;                   // the mineral stays where it is
                         const double share = k.share[(long long) s * No + n];
                         if (fabs(share) <= 1e-14) continue;
                         double give = share;
@@ -655,7 +656,7 @@ __device__ __forceinline__ void voxelKineticsFused(const AdeDev<R> &a, const Kin
     bool react = k.nrxn > 0, deferred = false;
     if (react && (k.surface_only || wantNear)) {
         bool surf = false, near = false;
-        if (wantNear) { const int own = k.phase[v]; near = own > 0 && own == k.dPhase; }
+        if (wantNear) { const int own = k.phase[v]; near = own > 0 && own <= k.dPhases; }
 #pragma unroll
         for (int q = 1; q < 7; ++q) {
             const int n = a.nb[(opp7(q) - 1) * No + i];                // the voxel at +c_q
@@ -664,7 +665,7 @@ __device__ __forceinline__ void voxelKineticsFused(const AdeDev<R> &a, const Kin
             const long long vn = a.list[n];
             const int nm = a.mask[vn];
             if (nm == 0 || nm == 1) surf = true;
-            if (wantNear && n < a.Nint) { const int pid = k.phase[vn]; if (pid > 0 && pid == k.dPhase) near = true; }
+            if (wantNear && n < a.Nint) { const int pid = k.phase[vn]; if (pid > 0 && pid <= k.dPhases) near = true; }
         }
         if (k.surface_only) react = surf;
         deferred = react && near;
@@ -1005,13 +1006,19 @@ __device__ __forceinline__ double storedConc(const AdeDev<R> &a, int s, int n) {
     return (double) a.imm[(long long) sl * a.N + a.list[n]];
 }
 
-// surfaceDissolutionKinetics3D (dissolutionVOP.hh:378-569): a voxel of the
+// x^e for the dissolution rate law: exactly x for order 1 and 1 for order 0
+__device__ __forceinline__ double orderPow(double x, double e) {
+    return e == 1.0 ? x : (e == 0.0 ? 1.0 : pow(x, e));
+}
+
+// surfaceDissolutionKinetics3D (dissolutionVOP.hh:378-569): a voxel of a
 // declared phase with mineral left averages the water of its open face
-// neighbours in the reaction box, takes the rate law of defineDissolutionRate
-// (rate = k max(C[acid],0) max(mineral,0)), removes the mineral (at most half
-// of it per step; products scaled alike) and parks one share per open face of
-// every product for the gather.  Shares of a phase voxel are rewritten every
-// step, so a voxel that stops dissolving parks zeros.
+// neighbours in the reaction box, takes its phase's rate law
+// (defineDissolutionRate: rate = k max(C[acid],0)^a max(mineral,0)^m),
+// removes the mineral (at most half of it per step; products scaled alike)
+// and parks one share per open face of every product for the gather.  Shares
+// of a phase voxel are rewritten every step, so a voxel that stops dissolving
+// parks zeros.
 template <typename R>
 __global__ void __launch_bounds__(256) k_dissol_park(AdeDev<R> a, KinDev k) {
     if (halted(a.halt)) return;
@@ -1019,11 +1026,13 @@ __global__ void __launch_bounds__(256) k_dissol_park(AdeDev<R> a, KinDev k) {
     if (i >= a.Nint) return;
     const long long v = a.list[i];
     const int pid = k.phase[v];
-    if (pid <= 0 || pid != k.dPhase) return;
+    if (pid <= 0 || pid > k.dPhases) return;
+    const int q = pid - 1;
+    const int sM = k.dMineral[q];
     const long long No = a.Nopen;
     const int S = k.S;
     for (int s = 0; s < S; ++s) k.share[(long long) s * No + i] = 0.0;
-    R *pm = a.imm + (long long) a.slot[k.dMineral] * a.N + v;
+    R *pm = a.imm + (long long) a.slot[sM] * a.N + v;
     const double mineral = (double) *pm;
     if (!(mineral > 1e-12)) return;
     int openN[6];
@@ -1045,10 +1054,10 @@ __global__ void __launch_bounds__(256) k_dissol_park(AdeDev<R> a, KinDev k) {
         for (int s = 0; s < S; ++s) water[s] /= (double) nOpen;
     }
     for (int s = 0; s < S; ++s) if (water[s] < 0.0) water[s] = 0.0;
-    const double H = k.dAcid >= 0 ? fmax(water[k.dAcid], 0.0) : 1.0;
-    const double rate = k.dK * H * fmax(mineral, 0.0);
+    const double H = k.dAcid[q] >= 0 ? orderPow(fmax(water[k.dAcid[q]], 0.0), k.dAo[q]) : 1.0;
+    const double rate = k.dK[q] * H * orderPow(fmax(mineral, 0.0), k.dMo[q]);
     double mineralR = -rate;
-    for (int s = 0; s < S; ++s) { subsR[s] = k.dStoich[s] * rate; if (!isfinite(subsR[s])) subsR[s] = 0.0; }
+    for (int s = 0; s < S; ++s) { subsR[s] = k.dStoich[q][s] * rate; if (!isfinite(subsR[s])) subsR[s] = 0.0; }
     if (!isfinite(mineralR)) mineralR = 0.0;
     if (!(mineralR < 0.0)) return;
     double dMineral = mineralR * k.dt;
@@ -1061,7 +1070,7 @@ __global__ void __launch_bounds__(256) k_dissol_park(AdeDev<R> a, KinDev k) {
     if (nOpen > 0) {
         unsigned long long parked = 0;
         for (int s = 0; s < S; ++s) {
-            if (s == k.dMineral) continue;
+            if (s == sM) continue;
             const double released = subsR[s] * k.dt * scale;
             if (fabs(released) <= 1e-12) continue;
             k.share[(long long) s * No + i] = released / (double) nOpen;
@@ -1071,18 +1080,17 @@ __global__ void __launch_bounds__(256) k_dissol_park(AdeDev<R> a, KinDev k) {
     }
 }
 
-// dissolNodeConversion3D (dissolutionVOP.hh:744-812): a voxel of the phase
-// whose inventory is below reopen_fraction x full_density turns pore and
-// loses its phase
+// dissolNodeConversion3D (dissolutionVOP.hh:744-812): a voxel of a declared
+// phase whose inventory is below reopen_fraction x its full_density turns
+// pore and loses its phase
 template <typename R>
-__global__ void k_dissol_convert(AdeDev<R> a, int8_t *phase, int phaseId, int mineralSlot, double threshold,
-                                 unsigned int *count) {
+__global__ void k_dissol_convert(AdeDev<R> a, int8_t *phase, DissolConv c, unsigned int *count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.Nint) return;
     const long long v = a.list[i];
     const int pid = phase[v];
-    if (pid <= 0 || pid != phaseId) return;
-    if ((double) a.imm[(long long) mineralSlot * a.N + v] < threshold) {
+    if (pid <= 0 || pid > c.n) return;
+    if ((double) a.imm[(long long) c.slot[pid - 1] * a.N + v] < c.threshold[pid - 1]) {
         a.mask[v] = 2;
         phase[v] = 0;
         atomicAdd(count, 1u);
@@ -1864,11 +1872,9 @@ hipError_t launchDissolPark(const AdeDev<R> &a, const KinDev &k, hipStream_t st)
     return hipGetLastError();
 }
 template <typename R>
-hipError_t launchDissolConvert(const AdeDev<R> &a, int8_t *phase, int phaseId, int mineralSlot, double threshold,
-                               unsigned int *count, hipStream_t st) {
+hipError_t launchDissolConvert(const AdeDev<R> &a, int8_t *phase, const DissolConv &c, unsigned int *count, hipStream_t st) {
     if (a.Nint <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_dissol_convert<R>, dim3(nblk(a.Nint, 256)), dim3(256), 0, st, a, phase, phaseId, mineralSlot,
-                       threshold, count);
+    hipLaunchKernelGGL(k_dissol_convert<R>, dim3(nblk(a.Nint, 256)), dim3(256), 0, st, a, phase, c, count);
     return hipGetLastError();
 }
 template <typename R>
@@ -2052,7 +2058,7 @@ hipError_t launchAdeStepGroup(const AdeMember<R> *E, const int2 *mapInt, int nIn
     template hipError_t launchKineticsListed<R>(const AdeDev<R> &, const KinDev &, const int *, const unsigned int *, hipStream_t); \
     template hipError_t launchNearPhaseList<R>(const AdeDev<R> &, const KinDev &, int *, unsigned int *, hipStream_t); \
     template hipError_t launchDissolPark<R>(const AdeDev<R> &, const KinDev &, hipStream_t); \
-    template hipError_t launchDissolConvert<R>(const AdeDev<R> &, int8_t *, int, int, double, unsigned int *, hipStream_t); \
+    template hipError_t launchDissolConvert<R>(const AdeDev<R> &, int8_t *, const DissolConv &, unsigned int *, hipStream_t); \
     template hipError_t launchAdeDensity<R>(const AdeDev<R> &, int, double *, hipStream_t); \
     template hipError_t launchAdeSet<R>(const AdeDev<R> &, int, const double *, hipStream_t); \
     template hipError_t launchAdeReopen<R>(const AdeDev<R> &, const uint8_t *, hipStream_t); \

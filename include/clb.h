@@ -29,12 +29,13 @@
 extern "C" {
 #endif
 
-#define CLB_ABI_VERSION 1
+#define CLB_ABI_VERSION 2
 
 #define CLB_MAX_SPECIES   8   /* substrates (mobile + immobile)              */
 #define CLB_MAX_MICROBES  2   /* CA biofilm populations                      */
 #define CLB_MAX_RXN       4   /* abiotic mass-action reactions               */
 #define CLB_MAX_COMP      8   /* equilibrium components                      */
+#define CLB_MAX_PHASES    4   /* declared dissolvable solid phases           */
 
 /* status codes */
 #define CLB_OK              0
@@ -154,20 +155,31 @@ typedef struct clb_case_params {
     int32_t precip_interval;
     int32_t precip_phase_id;
 
-    /* ---- dissolution (dissolutionVOP.hh): one declared solid phase --------- */
+    /* ---- dissolution (dissolutionVOP.hh): declared solid phases ----------
+     * Phase k (<phase k>) has id k + 1 (complab.cpp:516-531).  A grain phase
+     * is seeded on every voxel of its material at creation, the first phase
+     * naming a material taking its voxels (initPhaseLattice3D,
+     * complab.cpp:1672-1698); a precipitate phase starts nowhere and
+     * precipitation stamps precip_phase_id, which must name it.  Rate law of
+     * phase k (defineDissolutionRate for phase id k + 1):
+     *   rate = k * max(C[acid],0)^acid_order * max(mineral,0)^mineral_order,
+     *   mineralR = -rate, subsR[s] = stoich[s] * rate
+     * (acid < 0: factor 1; the examples' law is both orders 1). */
     int32_t dissol_on;
-    int32_t dissol_species;            /* <substrate>: immobile species holding the inventory   */
-    int32_t dissol_material;           /* <material_number> of a grain phase; -1: none          */
-    int32_t dissol_is_precipitate;     /* <is_precipitate>: precipitation stamps this phase     */
-    int32_t dissol_phase_id;           /* the phase id (1 for <phase0>)                         */
+    int32_t dissol_n_phases;           /* 1..CLB_MAX_PHASES                                      */
     int32_t dissol_surface_only;       /* <surface_only>: enclosed voxels do not react          */
     int32_t dissol_interval;           /* <update_interval> of the reopen check                 */
-    int32_t dissol_acid;               /* rate law: the species C[acid]; -1: none (factor 1)    */
-    double  dissol_full;               /* <full_density>                                        */
-    double  dissol_initial;            /* <initial_fill> seeded on dissol_material voxels       */
     double  dissol_reopen_frac;        /* <reopen_fraction>                                     */
-    double  dissol_k;                  /* rate = k max(C[acid],0) max(mineral,0), mineralR = -rate */
-    double  dissol_stoich[CLB_MAX_SPECIES]; /* subsR[s] = stoich[s] rate (defineDissolutionRate) */
+    int32_t dissol_species[CLB_MAX_PHASES];        /* <substrate>: immobile inventory species */
+    int32_t dissol_material[CLB_MAX_PHASES];       /* grain phase: CLB_SOLID or CLB_WALL; -1 none */
+    int32_t dissol_is_precipitate[CLB_MAX_PHASES]; /* <is_precipitate>                          */
+    int32_t dissol_acid[CLB_MAX_PHASES];           /* rate law: species C[acid]; -1 none        */
+    double  dissol_full[CLB_MAX_PHASES];           /* <full_density>                            */
+    double  dissol_initial[CLB_MAX_PHASES];        /* <initial_fill> of a grain phase           */
+    double  dissol_k[CLB_MAX_PHASES];
+    double  dissol_acid_order[CLB_MAX_PHASES];
+    double  dissol_mineral_order[CLB_MAX_PHASES];
+    double  dissol_stoich[CLB_MAX_PHASES][CLB_MAX_SPECIES];
     /* thermodynamic gate of the biotic rates, one .thm reaction block
      * (complab3d_thermo.hh:215-255, applied at part1:127-140): the voxel's
      * whole biotic rate vector times F_T = max(0, 1 - exp(f / (chi RT))),

@@ -356,8 +356,10 @@ private:
     int anyChange;
 };
 
-/* dissolutionVOP.hh:309-342: the phase ids of the grain phase are seeded from
- * the geometry when the case is created (dissol_material, dissol_phase_id) */
+/* dissolutionVOP.hh:309-342: the phase ids of the grain phases are seeded from
+ * the geometry when the case is created; this checks that the (material, id)
+ * pairs the caller passes (complab.cpp:1672-1683, grain phases only) are the
+ * case's: phase id k + 1 on dissol_material[k] */
 template <typename T>
 class initPhaseLattice3D {
 public:
@@ -367,8 +369,12 @@ public:
         clb_case_params p;
         int rc = detail::params(t, p);
         if (rc) return rc;
-        for (size_t k = 0; k < materials.size() && k < ids.size(); ++k)
-            if (!p.dissol_on || materials[k] != p.dissol_material || ids[k] != p.dissol_phase_id) return detail::mismatch(t);
+        if (!p.dissol_on || materials.size() != ids.size()) return detail::mismatch(t);
+        for (size_t k = 0; k < materials.size(); ++k) {
+            const plint q = ids[k] - 1;
+            if (q < 0 || q >= p.dissol_n_phases || p.dissol_is_precipitate[q] || materials[k] != p.dissol_material[q])
+                return detail::mismatch(t);
+        }
         return CLB_OK;
     }
 private:

@@ -469,19 +469,25 @@ void bioticStep(Case &c) {
 
 /* run_abiotic_kinetics (part1:199-297) or surfaceAbioticKinetics3D
  * (precipitationVOP.hh:53-153), then update_abiotic_rxnLattices (part1:306-404) */
-/* the dissolution rate law of the reference's examples (examples/14_dissolution,
- * examples/15_precip_and_dissolution kinetics/defineAbioticKinetics.hh,
- * defineDissolutionRate): rate = k max(C[acid],0) max(mineral,0),
- * mineralR = -rate, subsR[s] = stoich[s] rate; non-finite rates read 0 */
-void dissolutionLaw(const clb_case_params &p, const double *water, double mineral, double *subsR, double &mineralR) {
+/* the dissolution rate law of declared phase q (defineDissolutionRate for
+ * phase id q + 1; the reference's examples 14 and 15,
+ * kinetics/defineAbioticKinetics.hh, have orders 1 and 1):
+ * rate = k max(C[acid],0)^a max(mineral,0)^m, mineralR = -rate,
+ * subsR[s] = stoich[s] rate; non-finite rates read 0.  x^1 is x and x^0 is 1
+ * exactly, other orders std::pow. */
+static double orderPow(double x, double e) { return e == 1.0 ? x : (e == 0.0 ? 1.0 : std::pow(x, e)); }
+void dissolutionLaw(const clb_case_params &p, int q, const double *water, double mineral, double *subsR, double &mineralR) {
     const int S = p.n_species;
-    const double H = p.dissol_acid >= 0 ? std::max(water[p.dissol_acid], 0.0) : 1.0;
-    const double rate = p.dissol_k * H * std::max(mineral, 0.0);
+    const double H = p.dissol_acid[q] >= 0 ? orderPow(std::max(water[p.dissol_acid[q]], 0.0), p.dissol_acid_order[q]) : 1.0;
+    const double rate = p.dissol_k[q] * H * orderPow(std::max(mineral, 0.0), p.dissol_mineral_order[q]);
     mineralR = -rate;
-    for (int s = 0; s < S; ++s) subsR[s] = p.dissol_stoich[s] * rate;
+    for (int s = 0; s < S; ++s) subsR[s] = p.dissol_stoich[q][s] * rate;
     if (!std::isfinite(mineralR)) mineralR = 0.0;
     for (int s = 0; s < S; ++s) if (!std::isfinite(subsR[s])) subsR[s] = 0.0;
 }
+
+/* the declared phase of phase id pid, -1 for none (DissolutionConfig::byId) */
+static int phaseIndex(const clb_case_params &p, int pid) { return (pid >= 1 && pid <= p.dissol_n_phases) ? pid - 1 : -1; }
 
 static const int DLOC[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
 
@@ -492,12 +498,14 @@ static const int DLOC[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1
  * and parks one share per open face of every product */
 void dissolutionPark(Case &c, std::vector<double> &dC) {
     const clb_case_params &p = c.p;
-    const int S = p.n_species, sM = p.dissol_species;
+    const int S = p.n_species;
     std::fill(c.share.begin(), c.share.end(), 0.0);
     double water[CLB_MAX_SPECIES], subsR[CLB_MAX_SPECIES];
     for (int x = 1; x < c.nx - 1; ++x) for (int y = 0; y < c.ny; ++y) for (int z = 0; z < c.nz; ++z) {
         size_t i = c.id(x, y, z);
-        if (c.phase[i] <= 0 || c.phase[i] != p.dissol_phase_id) continue;
+        const int q = phaseIndex(p, c.phase[i]);
+        if (q < 0) continue;
+        const int sM = p.dissol_species[q];
         const double mineral = c.imm[sM][i];
         if (!(mineral > THRD)) continue;
         size_t openN[6]; int nOpen = 0;
@@ -516,7 +524,7 @@ void dissolutionPark(Case &c, std::vector<double> &dC) {
         }
         for (int s = 0; s < S; ++s) if (water[s] < 0.0) water[s] = 0.0;
         double mineralR = 0.0;
-        dissolutionLaw(p, water, mineral, subsR, mineralR);
+        dissolutionLaw(p, q, water, mineral, subsR, mineralR);
         if (!(mineralR < 0.0)) continue;
         double dMineral = mineralR * p.dt;
         double scale = 1.0;
@@ -543,7 +551,7 @@ void dissolutionPark(Case &c, std::vector<double> &dC) {
  * pending (the positivity rule, 669-683) */
 void dissolutionGather(Case &c, std::vector<double> &dC) {
     const clb_case_params &p = c.p;
-    const int S = p.n_species, sM = p.dissol_species;
+    const int S = p.n_species;
     const double thrdG = 1e-14;
     for (int x = 1; x < c.nx - 1; ++x) for (int y = 0; y < c.ny; ++y) for (int z = 0; z < c.nz; ++z) {
         size_t i = c.id(x, y, z);
@@ -552,7 +560,9 @@ void dissolutionGather(Case &c, std::vector<double> &dC) {
             int X = x + DLOC[d][0], Y = y + DLOC[d][1], Z = z + DLOC[d][2];
             if (X < 1 || X > c.nx - 2 || Y < 0 || Y > c.ny - 1 || Z < 0 || Z > c.nz - 1) continue;
             size_t j = c.id(X, Y, Z);
-            if (c.phase[j] <= 0 || c.phase[j] != p.dissol_phase_id) continue;
+            const int q = phaseIndex(p, c.phase[j]);
+            if (q < 0) continue;
+            const int sM = p.dissol_species[q];
             for (int s = 0; s < S; ++s) {
                 if (s == sM) continue;
                 const double share = c.share[(size_t) s * c.N + j];
@@ -1336,12 +1346,12 @@ bool precipConvert(Case &c) {
  * turns pore and loses its phase; returns the voxels reopened */
 int dissolConvert(Case &c) {
     const clb_case_params &p = c.p;
-    if (!(p.dissol_full > 0.0)) return 0;
     int n = 0;
     for (int z = 0; z < c.nz; ++z) for (int y = 0; y < c.ny; ++y) for (int x = 1; x < c.nx - 1; ++x) {
         size_t i = c.id(x, y, z);
-        if (c.phase[i] <= 0 || c.phase[i] != p.dissol_phase_id) continue;
-        if (c.imm[p.dissol_species][i] < p.dissol_reopen_frac * p.dissol_full) {
+        const int q = phaseIndex(p, c.phase[i]);
+        if (q < 0 || !(p.dissol_full[q] > 0.0)) continue;
+        if (c.imm[p.dissol_species[q]][i] < p.dissol_reopen_frac * p.dissol_full[q]) {
             c.mask[i] = CLB_PORE;
             c.phase[i] = 0;
             ++n;
@@ -1505,15 +1515,18 @@ void *orc_case_create(const clb_case_params *p, const int8_t *geometry, const do
         c->rhoL[s].assign((size_t) c->ny * c->nz, p->left_btype[s] == CLB_BC_DIRICHLET ? p->left_bc[s] : c0);
         c->rhoR[s].assign((size_t) c->ny * c->nz, p->right_btype[s] == CLB_BC_DIRICHLET ? p->right_bc[s] : c0);
     }
-    // dissolution: the grain phase and its inventory (complab.cpp:1667-1699)
+    // dissolution (complab.cpp:1667-1699): each voxel takes the id of the first
+    // grain phase on its material (initPhaseLattice3D over the grain phases),
+    // then every grain phase seeds its inventory on its material, in order
     if (p->dissol_on) {
         c->share.assign((size_t) p->n_species * N, 0.0);
-        if (!p->dissol_is_precipitate && p->dissol_material >= 0) {
-            for (size_t i = 0; i < N; ++i) {
-                if (geometry[i] != p->dissol_material) continue;
-                c->phase[i] = (int8_t) p->dissol_phase_id;
-                if (p->dissol_initial > 0.0) c->imm[p->dissol_species][i] = p->dissol_initial;
-            }
+        for (size_t i = 0; i < N; ++i)
+            for (int q = 0; q < p->dissol_n_phases; ++q)
+                if (!p->dissol_is_precipitate[q] && geometry[i] == p->dissol_material[q]) { c->phase[i] = (int8_t) (q + 1); break; }
+        for (int q = 0; q < p->dissol_n_phases; ++q) {
+            if (p->dissol_is_precipitate[q] || !(p->dissol_initial[q] > 0.0)) continue;
+            for (size_t i = 0; i < N; ++i)
+                if (geometry[i] == p->dissol_material[q]) c->imm[p->dissol_species[q]][i] = p->dissol_initial[q];
         }
     }
     c->B.resize(p->n_microbes);
@@ -1939,7 +1952,7 @@ extern "C" int orc_params_layout(int64_t *out) {
     out[3] = (int64_t) offsetof(clb_case_params, ca_seed);
     out[4] = (int64_t) offsetof(clb_case_params, eq_stoich);
     out[5] = (int64_t) offsetof(clb_case_params, precip_on);
-    out[6] = (int64_t) offsetof(clb_case_params, dissol_phase_id);
+    out[6] = (int64_t) offsetof(clb_case_params, dissol_stoich);
     out[7] = (int64_t) offsetof(clb_case_params, thm_nu);
     return 0;
 }

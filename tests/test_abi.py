@@ -29,7 +29,7 @@ def test_library_exports_every_declared_entry_point():
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
     assert sorted(clb.exported_symbols()) == names
-    assert L.clb_abi_version() == 1
+    assert L.clb_abi_version() == P.ABI_VERSION
 
 
 def test_ctypes_params_match_c_layout():

@@ -9,7 +9,7 @@ import complab_amd as clb
 from complab_amd import params as P
 import oracle as O
 
-from test_oracle_dissolution import calcite_case, CA, CAL, H
+from test_oracle_dissolution import calcite_case, two_phase_case, CA, CAL, DOL, H, MG
 
 pytestmark = pytest.mark.gpu
 
@@ -23,7 +23,7 @@ def relerr(a, b, mask=None):
     return float(np.max(np.abs(a - b)) / scale)
 
 
-def compare(g, o, n_species=3):
+def compare(g, o, n_species=3, immobile=(CAL,)):
     mg, mo = g.read(P.FIELD_MASK), o.read(P.FIELD_MASK)
     assert np.array_equal(mg, mo), "mask differs"
     assert np.array_equal(g.read(P.FIELD_PHASE), o.read(P.FIELD_PHASE)), "phase differs"
@@ -33,7 +33,7 @@ def compare(g, o, n_species=3):
     om[:, :, 0] = False
     om[:, :, -1] = False
     for s in range(n_species):
-        mask = om if s != CAL else None
+        mask = om if s not in immobile else None
         e = relerr(g.read(P.FIELD_CONC, s), o.read(P.FIELD_CONC, s), mask)
         print("species", s, "rel err", e)
         assert e <= REL
@@ -106,4 +106,55 @@ def test_reopening_with_flow_resolve(ctx):
     assert cg[11] > 0 and cg[4] >= 1, "no reopening / no re-solve happened"
     for k in range(3):
         assert relerr(g.read(P.FIELD_VEL, k), o.read(P.FIELD_VEL, k)) <= REL
+    g.close(); o.close()
+
+
+def test_two_grain_phases_match_oracle(ctx):
+    """Two declared grain phases with their own inventories and rate laws
+    (first order in acid and mineral; second order in acid, zero order in
+    mineral), reopening at their own thresholds."""
+    p, geom = two_phase_case(k1=0.4, k2=0.02, h0=1.0)
+    p.dissol_reopen_frac = 0.6
+    p.dissol_interval = 3
+    g = clb.Case(ctx, p, geom)
+    o = O.OracleCase(p, geom)
+    for c in (g, o):
+        c.set_omegas([1.2, 1.2, 1.0, 1.2, 1.0], [1.2, 1.2, 1.0, 1.2, 1.0], 1.0)
+    rg, ng = g.run_steps(12)
+    ro, no = o.run_steps(12)
+    assert rg == ro == 0 and ng == no == 12
+    compare(g, o, n_species=5, immobile=(CAL, DOL))
+    c = g.counters()
+    assert c[8] > 0 and c[8] == c[9] and c[11] > 0
+    g.close(); o.close()
+
+
+def test_precipitate_phase_beside_a_grain_phase_matches_oracle(ctx):
+    """A grain phase (calcite, id 1) and a precipitate phase (id 2, a mineral
+    made from Ca and Mg on the grain surface, sealing voxels that then
+    dissolve back in acid): each voxel's gather skips the mineral of the
+    phase it collects from."""
+    p, geom = two_phase_case(k1=0.05, k2=0.0, h0=1.0)
+    geom[geom == P.WALL] = P.PORE
+    p.dissol_n_phases = 1
+    p.c0[MG] = 0.5
+    p.n_rxn = 1
+    p.rxn_r1[0] = CA; p.rxn_r2[0] = MG; p.rxn_k[0] = 5.0
+    p.rxn_stoich[0][CA] = -1.0; p.rxn_stoich[0][MG] = -1.0; p.rxn_stoich[0][DOL] = 1.0
+    p.surface_only = 1
+    P.set_phase(p, 1, DOL, 0.02, precipitate=True, k_rate=3.0, acid=H, stoich={CA: 1.0, MG: 1.0, H: -2.0})
+    p.precip_on = 1; p.precip_species = DOL; p.precip_max_rho = 0.02; p.precip_interval = 2
+    p.dissol_reopen_frac = 0.9
+    p.dissol_interval = 2
+    g = clb.Case(ctx, p, geom)
+    o = O.OracleCase(p, geom)
+    for c in (g, o):
+        c.set_omegas([1.2, 1.2, 1.0, 1.2, 1.0], [1.2, 1.2, 1.0, 1.2, 1.0], 1.0)
+    rg, ng = g.run_steps(20)
+    ro, no = o.run_steps(20)
+    assert rg == ro == 0 and ng == no == 20
+    compare(g, o, n_species=5, immobile=(CAL, DOL))
+    c = o.counters()
+    print("oracle counters", c)
+    assert c[8] > 0 and c[5] > 0, "no dissolution / no seal happened"
     g.close(); o.close()

@@ -102,8 +102,8 @@ def test_tags_map_onto_the_parameter_block(tmp_path):
     assert list(p.eq_logk[:3]) == [0, 0, 3.5] and list(p.eq_comp_species[:2]) == [0, 1]
     assert (p.eq_tol, p.eq_max_iter, p.eq_anderson_depth) == (1e-10, 200, 4)
     assert (p.precip_on, p.precip_species, p.precip_max_rho, p.precip_interval, p.precip_perm_ratio) == (1, 2, 8, 50, 0)
-    assert (p.dissol_on, p.dissol_species, p.dissol_is_precipitate, p.dissol_phase_id, p.precip_phase_id) == (1, 2, 1, 1, 1)
-    assert (p.dissol_full, p.dissol_reopen_frac, p.dissol_interval, p.dissol_surface_only) == (8, 0.8, 25, 1)
+    assert (p.dissol_on, p.dissol_n_phases, p.dissol_species[0], p.dissol_is_precipitate[0], p.precip_phase_id) == (1, 1, 2, 1, 1)
+    assert (p.dissol_full[0], p.dissol_reopen_frac, p.dissol_interval, p.dissol_surface_only) == (8, 0.8, 25, 1)
     assert r.diag_enabled and r.diag_interval == 10 and r.diag_tol == 1e-8 and r.conserve == ["A+M", "B+M"]
     # materials: 5, 6 pore; 9 bounce-back; 7 the microbe's biofilm; 0 solid
     raw = inputs.read_geometry_dat(str(tmp_path / "in" / "g.dat"), nx, ny, nz)
@@ -302,3 +302,34 @@ def test_symbolic_runs_refuse_fixed_substrates_and_totals(tmp_path):
     with open(xml, "w") as fh:
         fh.write(text.replace("</simulation_mode>", "<fba_concentration_basis>Free</fba_concentration_basis></simulation_mode>"))
     inputs.load(xml)
+
+
+def test_several_dissolution_phases(tmp_path):
+    """<phase0>, <phase1>, ... take ids 1, 2, ... in order; a grain phase maps
+    its material number onto the solid or bounce-back code, a precipitate
+    phase takes none and becomes the phase precipitation stamps (the last one
+    declared, complab.cpp:516-531); more than four phases are refused, as is a
+    grain phase on a pore material."""
+    two = ("<phase0><material_number>9</material_number><substrate>2</substrate><full_density>20</full_density>"
+           "<initial_fill>20</initial_fill></phase0>"
+           "<phase1><material_number>0</material_number><substrate>2</substrate><full_density>8</full_density>"
+           "<is_precipitate>true</is_precipitate></phase1>")
+    one = XML[XML.index("    <phase0>"):XML.index("</dissolution>")]
+    vals = np.random.default_rng(1).choice([0, 5, 6, 7, 9], size=6 * 4 * 3)
+    path = write_case(tmp_path, vals)
+    with open(path, "w") as fh:
+        fh.write(XML.replace(one, two))
+    p = inputs.load(path).params
+    assert (p.dissol_n_phases, p.precip_phase_id) == (2, 2)
+    assert list(p.dissol_material[:2]) == [P.WALL, -1] and list(p.dissol_is_precipitate[:2]) == [0, 1]
+    assert list(p.dissol_full[:2]) == [20, 8] and list(p.dissol_initial[:2]) == [20, 0]
+    five = "".join("<phase%d><material_number>0</material_number><substrate>2</substrate><full_density>1</full_density>"
+                   "</phase%d>" % (k, k) for k in range(5))
+    with open(path, "w") as fh:
+        fh.write(XML.replace(one, five))
+    with pytest.raises(inputs.InputError):
+        inputs.load(path)
+    with open(path, "w") as fh:
+        fh.write(XML.replace(one, two.replace("<material_number>9</material_number>", "<material_number>5</material_number>")))
+    with pytest.raises(inputs.InputError):
+        inputs.load(path)

@@ -196,4 +196,4 @@ def test_params_layout_matches_c():
     O.lib().orc_params_layout(out.ctypes.data_as(C.c_void_p))
     CP = P.CaseParams
     assert list(out) == [C.sizeof(CP), CP.dt.offset, CP.bio_stoich.offset, CP.ca_seed.offset,
-                         CP.eq_stoich.offset, CP.precip_on.offset, CP.dissol_phase_id.offset, CP.thm_nu.offset]
+                         CP.eq_stoich.offset, CP.precip_on.offset, CP.dissol_stoich.offset, CP.thm_nu.offset]
