@@ -8,10 +8,14 @@ stabilization) is untimed; W warm-up steps, then K coupled steps are timed
 between two device synchronisations.  Prints one JSON line.
 
 value: coupled MLUPS = lattice voxels x K / seconds / 1e6.
-roofline: the fused transport + kinetics kernel, algorithmic bytes per launch
-(per open voxel S_m*14*p + 3p + (S_i+M)*2p, plus 1 byte of mask per voxel)
-over its HIP-event-timed average duration, against 8 TB/s; `traffic` is the
-HBM bytes per launch of the committed rocprofv3 counter summary in profiles/.
+roofline: the kernel with the most device time in the timed window -- the
+flow kernel of the in-step re-solves (19 x 2 x p bytes per open voxel, plus
+1 byte of mask per voxel) or the fused transport + kinetics kernel (per open
+voxel S_m*14*p + 3p + (S_i+M)*2p, plus 1 byte of mask per voxel) --
+algorithmic bytes per launch over its HIP-event-timed average duration,
+against 8 TB/s; `traffic` is the HBM bytes per launch of the committed
+rocprofv3 counter summary in profiles/.  Both kernels' figures are in
+roofline_flow and roofline_transport.
 cpu_baseline: the CPU oracle (a scalar C++ restatement) on the same case for a
 few steps, velocity copied from the GPU flow solve.
 sweep: the batched form -- seeded sweep cases of the batched workload (cuboids
@@ -43,17 +47,18 @@ HBM_PEAK_GBS = 8000.0
 # the timed region brackets every 8th transport launch with HIP events: each
 # event pair puts markers on the stream that cost the device a dispatch gap
 TIMING_STRIDE = 8
-PROFILE = os.path.join(ROOT, "profiles", "k_ade_bulk_biotic_258x128x128_fp64.txt")
+PROFILES = {"k_ade_bulk": os.path.join(ROOT, "profiles", "k_ade_bulk_biotic_258x128x128_fp64.txt"),
+            "k_ns_step": os.path.join(ROOT, "profiles", "k_ns_step_biotic_258x128x128_fp64.txt")}
 
 
-def profiled_traffic(p, n_open):
-    """HBM bytes per launch of the fused kernel, from the committed rocprofv3
+def profiled_traffic(p, n_open, kernel):
+    """HBM bytes per launch of `kernel`, from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a copy
     kernel of known size); None for any other workload."""
     if p.precision != 0 or (p.nx, p.ny, p.nz) != (258, 128, 128) or n_open != 1485001:
         return None
     try:
-        for line in open(PROFILE):
+        for line in open(PROFILES[kernel]):
             if line.startswith("traffic_bytes_per_launch:"):
                 return int(line.split(":")[1])
     except OSError:
@@ -277,6 +282,7 @@ def main():
     ctx.sync()
     g.set_timing_stride(TIMING_STRIDE)
     g.kernel_time(reset=True)
+    g.flow_kernel_time(reset=True)
     c0 = g.counters()
     # a small copy-kernel launch on each side of the timed region marks it in
     # kernel traces (k_copy16 runs nowhere else before the copy measurement)
@@ -291,6 +297,7 @@ def main():
         raise SystemExit("timed run stopped with status %d after %d steps" % (rc, n))
     c1 = g.counters()
     kms, launches = g.kernel_time()
+    fms, flaunches = g.flow_kernel_time()
 
     # the flow: a re-solve as the main loop runs one after a mask change
     # (its own tracer starts empty: complab.cpp:2166), timed alone, and its
@@ -317,6 +324,28 @@ def main():
     achieved = bytes_launch / (kms * 1e-3) / 1e9
     mlups = V * args.steps / t / 1e6
     ns_bytes = 19 * 2 * (8 if p.precision != P.FP32 else 4) * n_open + V    # per flow iteration
+    transport = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(p, n_open, "k_ade_bulk"),
+                 "kernel": "k_ade_bulk (fused stream+collide+kinetics of the interior voxels)",
+                 "kernel_avg_ms": round(kms, 5), "launches": int(launches), "timed_every": TIMING_STRIDE,
+                 "bytes_per_open_voxel": per_open, "bytes_per_launch": int(bytes_launch),
+                 "device_ms_in_window": round(kms * args.steps, 2)}
+    # the flow kernel: working iterations of the in-step re-solves, every
+    # TIMING_STRIDE-th timed; its device time in the window is the timed mean
+    # times the working iterations (timed count x stride)
+    flow = None
+    if flaunches > 0:
+        f_ach = ns_bytes / (fms * 1e-3) / 1e9
+        flow = {"bound": "hbm", "achieved": round(f_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(f_ach / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(p, n_open, "k_ns_step"),
+                "kernel": "k_ns_step (one D3Q19 iteration: pull, incompressible BGK, bounce-back, pressure planes)",
+                "kernel_avg_ms": round(fms, 5), "launches": int(flaunches), "timed_every": TIMING_STRIDE,
+                "bytes_per_open_voxel": 19 * 2 * (8 if p.precision != P.FP32 else 4),
+                "bytes_per_launch": int(ns_bytes),
+                "device_ms_in_window": round(fms * flaunches * TIMING_STRIDE, 2)}
+    dominant = transport
+    if flow is not None and flow["device_ms_in_window"] > transport["device_ms_in_window"]:
+        dominant = flow
     resolve_ms_per_step = rs_s * 1e3 / max(p.ns_update_interval, 1)
 
     cpu = None
@@ -362,12 +391,9 @@ def main():
                  "note": "k_ns_step + k_ns_tracer per iteration, wall clock over 500 iterations"},
         "units": {"mflups_open": round(n_open * args.steps / t / 1e6, 2),
                   "lattice_mlups": round(V * args.steps * sum(1 for s in range(p.n_species) if p.mobile[s]) / t / 1e6, 2)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(p, n_open),
-                     "kernel": "k_ade_bulk (fused stream+collide+kinetics of the interior voxels)",
-                     "kernel_avg_ms": round(kms, 5), "launches": int(launches), "timed_every": TIMING_STRIDE,
-                     "bytes_per_open_voxel": per_open, "bytes_per_launch": int(bytes_launch),
-                     "copy_kernel_gbs": round(copy_gbs, 1)},
+        "roofline": dict(dominant, copy_kernel_gbs=round(copy_gbs, 1)),
+        "roofline_transport": transport,
+        "roofline_flow": flow,
         "cpu_baseline": cpu,
         "sweep": sweep_out,
         "serving": serve_out,

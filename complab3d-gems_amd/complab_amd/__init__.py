@@ -96,6 +96,7 @@ def lib():
         "clb_batch_run_steps": ([V, I, I, V, V], I),
         "clb_batch_set_tile": ([V, I, I, I], I),
         "clb_batch_tuning": ([V, I, I, C.POINTER(I), V], I),
+        "clb_flow_kernel_time": ([V, I, I, C.POINTER(D), C.POINTER(C.c_int64)], I),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -124,7 +125,7 @@ def exported_symbols():
         "clb_species_totals_noalloc", "clb_set_tuning",
         "clb_batch_create", "clb_batch_destroy", "clb_batch_add", "clb_batch_remove", "clb_batch_members",
         "clb_batch_ns_solve", "clb_batch_ade_stabilize", "clb_batch_run_steps", "clb_batch_set_tile",
-        "clb_batch_tuning",
+        "clb_batch_tuning", "clb_flow_kernel_time",
     ]
 
 
@@ -363,6 +364,13 @@ class Case:
     def kernel_time(self, reset=False):
         ms = C.c_double(); n = C.c_int64()
         self._c(lib().clb_kernel_time(self.ctx.h, self.id, int(reset), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def flow_kernel_time(self, reset=False):
+        """Average device time (ms) of the timed flow iterations (k_ns_step
+        alone) and their count."""
+        ms = C.c_double(); n = C.c_int64()
+        self._c(lib().clb_flow_kernel_time(self.ctx.h, self.id, int(reset), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
 

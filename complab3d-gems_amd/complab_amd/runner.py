@@ -102,8 +102,11 @@ def run(r: inputs.Run, ctx=None, case=None, out_dir: Optional[str] = None, steps
     # as rate_<species>_<iT>.vti under the snapshot's number, and the last
     # step under the final snapshot's
     pr = r.params
-    rates = vtk_every > 0 and not pr.dissol_on and (
-        (pr.bio_law != 0 and pr.n_microbes > 0) or pr.n_rxn > 0 or len(r.symbolic) > 0)
+    has_law = (pr.bio_law != 0 and pr.n_microbes > 0) or pr.n_rxn > 0 or len(r.symbolic) > 0
+    rates = vtk_every > 0 and not pr.dissol_on and has_law
+    if vtk_every > 0 and pr.dissol_on and has_law:
+        say("rate snapshots are not written for a run with dissolution: the rate capture "
+            "does not cover the dissolution kinetics on this path")
     capture_at = -1
 
     it = int(case.counters()[CHK_STEP]) if restart else 0

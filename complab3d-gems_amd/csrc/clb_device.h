@@ -289,7 +289,9 @@ template <typename R> hipError_t launchAdeTotal(const AdeDev<R> &, int, double *
 template <typename R> hipError_t launchAdeHeld(const AdeDev<R> &, int, double *, hipStream_t);
 hipError_t launchAdeKinds(const int *nb, const uint8_t *clsPull, int Nopen, int Nint, uint16_t *kinds, hipStream_t);
 hipError_t launchNsKinds(const int *nb, const uint8_t *clsPull, int Nopen, int Nint, unsigned long long *kinds, hipStream_t);
-template <typename R> hipError_t launchNsStep(const NsDev<R> &, double *, double *, NsTracer *, int, double, hipStream_t);
+// evStart / evEnd, when given, are recorded around the flow kernel alone (not its tracer)
+template <typename R> hipError_t launchNsStep(const NsDev<R> &, double *, double *, NsTracer *, int, double, hipStream_t,
+                                              hipEvent_t evStart = nullptr, hipEvent_t evEnd = nullptr);
 template <typename R, typename RA> hipError_t launchNsVelocity(const NsDev<R> &, RA *, double *, hipStream_t);
 template <typename R> hipError_t launchNsInit(const NsDev<R> &, double, hipStream_t);
 hipError_t launchNsUpdateDyn(const int8_t *, const int *, uint8_t *, int, double, unsigned int *, hipStream_t);

@@ -264,6 +264,7 @@ struct CaseBase {
     virtual void fieldStats(hipStream_t, double *out) = 0;
     virtual void counters(hipStream_t, int64_t out[CLB_NCOUNTERS]) = 0;
     virtual void kernelTime(hipStream_t, int reset, double *ms, int64_t *n) = 0;
+    virtual void flowKernelTime(int reset, double *ms, int64_t *n) = 0;
     virtual void setTimeStride(int stride) = 0;
     virtual Status setTuning(int knob, int value) = 0;
     virtual void setRelaxation(const double *omp, const double *omb, double dt) = 0;
@@ -345,10 +346,25 @@ struct Case : CaseBase {
     int timeStride = 1;                         // time every timeStride-th launch
     double kms = 0;
     int64_t klaunch = 0;
+    // the same for the flow kernel (k_ns_step, its tracer left out): every
+    // timeStride-th iteration of nsSolve, working iterations only
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> nsEvPool;
+    std::vector<long long> nsEvIter;            // the solve's iteration each pair brackets
+    size_t nsEvUsed = 0;
+    long long nsLaunchNo = 0;
+    double nsKms = 0;
+    int64_t nsKlaunch = 0;
     unsigned int nparts = 0;
     // queued step batches (CA cases): the control block on the device, its
     // pinned host mirror, the mask/total partials and the closed-voxel constants
     StepCtl *ctl = nullptr, *ctlHost = nullptr;
+    // small read-backs of the host's decisions land in pinned memory, and the
+    // host waits on them by polling an event rather than sleeping in a
+    // stream synchronisation (each round trip of the CA loop and the age
+    // sweeps pays that wake-up)
+    unsigned int *pinU = nullptr;               // kPinU counters
+    hipEvent_t evWait = nullptr;
+    static constexpr int kPinU = 32;
     double *caPartMax = nullptr;
     long long *caPartMask = nullptr;
     // dissolution: per-face product shares [S][Nopen], device counters (parked,
@@ -381,8 +397,24 @@ struct Case : CaseBase {
 
     ~Case() override {
         for (auto &e : evpool) { (void) hipEventDestroy(e.first); (void) hipEventDestroy(e.second); }
+        for (auto &e : nsEvPool) { (void) hipEventDestroy(e.first); (void) hipEventDestroy(e.second); }
         for (void *q : allocs) (void) hipFree(q);
         if (ctlHost) (void) hipHostFree(ctlHost);
+        if (pinU) (void) hipHostFree(pinU);
+        if (evWait) (void) hipEventDestroy(evWait);
+    }
+    // waits for everything queued on st so far, polling
+    void waitHost(hipStream_t st) {
+        HIPT(hipEventRecord(evWait, st));
+        hipError_t e;
+        while ((e = hipEventQuery(evWait)) == hipErrorNotReady) {}
+        if (e != hipSuccess) throw HipFail{e, "hipEventQuery"};
+    }
+    // n counters of the device at src, read back into pinU[slot..] and waited on
+    const unsigned int *readU(hipStream_t st, const unsigned int *src, int n, int slot = 0) {
+        HIPT(hipMemcpyAsync(pinU + slot, src, (size_t) n * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        waitHost(st);
+        return pinU + slot;
     }
 
     R *fCur() const { return fCurIsA ? fA : fB; }
@@ -651,6 +683,9 @@ struct Case : CaseBase {
             HIPT(hipHostMalloc(&h, sizeof(StepCtl), hipHostMallocDefault));
             ctlHost = (StepCtl *) h;
             memset(ctlHost, 0, sizeof(StepCtl));
+            HIPT(hipHostMalloc(&h, kPinU * sizeof(unsigned int), hipHostMallocDefault));
+            pinU = (unsigned int *) h;
+            HIPT(hipEventCreateWithFlags(&evWait, hipEventDisableTiming));
         }
 
         // voxel bytes and scalar fields
@@ -937,7 +972,16 @@ struct Case : CaseBase {
                 NsDev<RN> a = ns();
                 a.cur = curA ? gA : gB; a.out = curA ? gB : gA;
                 a.bbCur = curA ? bbA : bbB; a.bbOut = curA ? bbB : bbA;
-                HIPT(launchNsStep<RN>(a, partE, partN, tracer, rule, eps, st));
+                const bool timed = nsLaunchNo++ % timeStride == 0;
+                if (timed && nsEvUsed == nsEvPool.size()) {
+                    hipEvent_t e0, e1;
+                    HIPT(hipEventCreate(&e0)); HIPT(hipEventCreate(&e1));
+                    nsEvPool.push_back({e0, e1}); nsEvIter.push_back(0);
+                }
+                if (timed) nsEvIter[nsEvUsed] = ran + k;
+                HIPT(launchNsStep<RN>(a, partE, partN, tracer, rule, eps, st, timed ? nsEvPool[nsEvUsed].first : nullptr,
+                                      timed ? nsEvPool[nsEvUsed].second : nullptr));
+                if (timed) ++nsEvUsed;
                 nsFirst = false;
                 if (!launched && nsClsDirty) {
                     HIPT(hipMemcpyAsync(nsClsPull, nsCls, Nopen, hipMemcpyDeviceToDevice, st));
@@ -952,6 +996,13 @@ struct Case : CaseBase {
             HIPT(hipMemcpyAsync(&done, nsDone, sizeof(int), hipMemcpyDeviceToHost, st));
             HIPT(hipStreamSynchronize(st));
             const long long eff = th.iters - base;   // launches that did work
+            for (size_t q = 0; q < nsEvUsed; ++q) {
+                if (nsEvIter[q] >= eff) continue;    // a launch after the stop returns at once
+                float ms = 0;
+                HIPT(hipEventElapsedTime(&ms, nsEvPool[q].first, nsEvPool[q].second));
+                nsKms += ms; ++nsKlaunch;
+            }
+            nsEvUsed = 0;
             parity = ((eff & 1) == 0) ? gCurIsA : !gCurIsA;
             if (!done && eff < ran + batch) return {CLB_E_HIP, "flow iterations were lost"};
             ran = eff;
@@ -988,9 +1039,7 @@ struct Case : CaseBase {
         nsBioOmega = 1.0 / (perm * (1.0 / nsOmega - 0.5) + 0.5);
         HIPT(hipMemsetAsync(ucnt, 0, sizeof(unsigned int), st));
         HIPT(launchNsUpdateDyn(mask, list, nsCls, Nopen, perm, ucnt, st));
-        unsigned int ch = 0;
-        HIPT(hipMemcpyAsync(&ch, ucnt, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-        HIPT(hipStreamSynchronize(st));
+        const unsigned int ch = readU(st, ucnt, 1)[0];
         if (ch) {
             HIPT(launchNsReopen<RN>(ns(), nullptr, st));
             nsBBRebuild(st);
@@ -1112,8 +1161,7 @@ struct Case : CaseBase {
         }
         HIPT(hipMemsetAsync(ucnt + 3, 0, sizeof(unsigned int), st));
         HIPT(launchDissolConvert<R>(ade(), phase, dc, ucnt + 3, st));
-        HIPT(hipMemcpyAsync(&n, ucnt + 3, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-        HIPT(hipStreamSynchronize(st));
+        n = readU(st, ucnt + 3, 1)[0];
         if (n) nearDirty = true;
         if (reopenedOut) *reopenedOut = (int) n;
         return {CLB_OK, ""};
@@ -1130,8 +1178,7 @@ struct Case : CaseBase {
         unsigned int nre = 0;
         HIPT(hipMemsetAsync(ucnt + 1, 0, sizeof(unsigned int), st));
         HIPT(launchSolidSwitch(mask, list, adeCls, reopened, Nopen, ucnt + 1, st));
-        HIPT(hipMemcpyAsync(&nre, ucnt + 1, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-        HIPT(hipStreamSynchronize(st));
+        nre = readU(st, ucnt + 1, 1)[0];
         if (nre) HIPT(launchAdeReopen<R>(ade(), reopened, st));
         adeClsDirty = true;
         if (reopenedOut) *reopenedOut = (int) nre;
@@ -1141,8 +1188,7 @@ struct Case : CaseBase {
         if (p.precip_species >= 0 && p.precip_species < p.n_species && !p.mobile[p.precip_species]) {
             HIPT(hipMemsetAsync(ucnt, 0, sizeof(unsigned int), st));
             HIPT(launchPrecip<R>(nx, ny, N, mask, imm + (size_t) slot[p.precip_species] * N, p.precip_max_rho, p.precip_phase_id, phase, ucnt, st));
-            HIPT(hipMemcpyAsync(&ch, ucnt, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-            HIPT(hipStreamSynchronize(st));
+            ch = readU(st, ucnt, 1)[0];
             if (ch) nearDirty = true;
         }
         if (changed) *changed = ch ? 1 : 0;
@@ -1156,7 +1202,7 @@ struct Case : CaseBase {
     void maskTotal(hipStream_t st, double *gmax, long long *msum) {
         HIPT(launchCaMaskTotal<R>(ca(), 0, st));
         HIPT(hipMemcpyAsync(ctlHost, ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, st));
-        HIPT(hipStreamSynchronize(st));
+        waitHost(st);
         *gmax = ctlHost->gmax;
         *msum = ctlHost->msum;
     }
@@ -1173,16 +1219,12 @@ struct Case : CaseBase {
         unsigned int *cntDev = ageCnt;
         int *blist = ageList;
         HIPT(launchAgeCompact<R>(ca(), blist, reinterpret_cast<unsigned int *>(partsLL), cntDev, st));
-        unsigned int n = 0;
-        HIPT(hipMemcpyAsync(&n, cntDev, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-        HIPT(hipStreamSynchronize(st));
+        const unsigned int n = readU(st, cntDev, 1)[0];
         int swept = 0;
         while (true) {
             HIPT(hipMemsetAsync(cntDev + 1, 0, K * sizeof(unsigned int), st));
             for (int k = 0; k < K; ++k) HIPT(launchCaAgeInplace<R>(ca(), age, blist, (int) n, cntDev + 1, k, st));
-            unsigned int raised[K];
-            HIPT(hipMemcpyAsync(raised, cntDev + 1, sizeof(raised), hipMemcpyDeviceToHost, st));
-            HIPT(hipStreamSynchronize(st));
+            const unsigned int *raised = readU(st, cntDev + 1, K);
             for (int k = 0; k < K; ++k) {
                 ++swept;
                 if (raised[k] == 0) return {CLB_OK, ""};
@@ -1197,9 +1239,7 @@ struct Case : CaseBase {
     Status ageOrderedSweep(hipStream_t st) {
         const int K = 8;
         HIPT(launchAgeCompact<R>(ca(), ageList, reinterpret_cast<unsigned int *>(partsLL), ageCnt, st));
-        unsigned int n = 0;
-        HIPT(hipMemcpyAsync(&n, ageCnt, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-        HIPT(hipStreamSynchronize(st));
+        const unsigned int n = readU(st, ageCnt, 1)[0];
         if (n == 0) return {CLB_OK, ""};
         HIPT(hipMemcpyAsync(age2, age, (size_t) N * sizeof(int), hipMemcpyDeviceToDevice, st));
         unsigned int *chg = ageCnt + 16;
@@ -1208,9 +1248,7 @@ struct Case : CaseBase {
         while (true) {
             HIPT(hipMemsetAsync(chg, 0, K * sizeof(unsigned int), st));
             for (int k = 0; k < K; ++k) HIPT(launchCaAgeOrdered<R>(ca(), age, age2, ageList, (int) n, chg, k, st));
-            unsigned int changed[K];
-            HIPT(hipMemcpyAsync(changed, chg, sizeof(changed), hipMemcpyDeviceToHost, st));
-            HIPT(hipStreamSynchronize(st));
+            const unsigned int *changed = readU(st, chg, K);
             for (int k = 0; k < K; ++k) {
                 ++launches;
                 if (changed[k] == 0) { std::swap(age, age2); return {CLB_OK, ""}; }
@@ -1312,18 +1350,14 @@ struct Case : CaseBase {
         if (precipDue(iT)) {
             HIPT(hipMemsetAsync(ucnt, 0, 2 * sizeof(unsigned int), st));
             HIPT(launchPrecip<R>(nx, ny, N, mask, imm + (size_t) slot[p.precip_species] * N, p.precip_max_rho, p.precip_phase_id, phase, ucnt, st));
-            unsigned int ch = 0;
-            HIPT(hipMemcpyAsync(&ch, ucnt, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-            HIPT(hipStreamSynchronize(st));
+            const unsigned int ch = readU(st, ucnt, 1)[0];
             if (ch) {
                 cnt[5]++;
                 nearDirty = true;
                 if (p.flow_on && !nsSaturate) { Status r = resolve(st, p.precip_perm_ratio, false); if (r.code) return r; }
-                unsigned int nre = 0;
                 HIPT(hipMemsetAsync(ucnt + 1, 0, sizeof(unsigned int), st));
                 HIPT(launchSolidSwitch(mask, list, adeCls, reopened, Nopen, ucnt + 1, st));
-                HIPT(hipMemcpyAsync(&nre, ucnt + 1, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-                HIPT(hipStreamSynchronize(st));
+                const unsigned int nre = readU(st, ucnt + 1, 1)[0];
                 if (nre) HIPT(launchAdeReopen<R>(ade(), reopened, st));
                 adeClsDirty = true;
             }
@@ -1387,7 +1421,7 @@ struct Case : CaseBase {
             else HIPT(launchCaMaskTotal<R>(ca(), hostDue(iT + j) ? 2 : 1, st));
         }
         HIPT(hipMemcpyAsync(ctlHost, ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, st));
-        HIPT(hipStreamSynchronize(st));
+        waitHost(st);
         const int fin = ctlHost->done;
         const bool halt = ctlHost->halt != 0;
         const int ran = fin + (halt ? 1 : 0);               // transport launches that did work
@@ -1597,6 +1631,11 @@ struct Case : CaseBase {
         if (ms) *ms = klaunch ? kms / (double) klaunch : 0.0;
         if (n) *n = klaunch;
         if (reset) { kms = 0; klaunch = 0; }
+    }
+    void flowKernelTime(int reset, double *ms, int64_t *n) override {
+        if (ms) *ms = nsKlaunch ? nsKms / (double) nsKlaunch : 0.0;
+        if (n) *n = nsKlaunch;
+        if (reset) { nsKms = 0; nsKlaunch = 0; }
     }
 };
 
@@ -2645,6 +2684,13 @@ int clb_kernel_time(clb_ctx *ctx, int case_id, int reset, double *avg_ms, int64_
     GUARD_BEGIN
     CASE_OR_FAIL(c)
     c->kernelTime(ctx->stream, reset, avg_ms, launches);
+    return CLB_OK;
+    GUARD_END
+}
+int clb_flow_kernel_time(clb_ctx *ctx, int case_id, int reset, double *avg_ms, int64_t *launches) {
+    GUARD_BEGIN
+    CASE_OR_FAIL(c)
+    c->flowKernelTime(reset, avg_ms, launches);
     return CLB_OK;
     GUARD_END
 }

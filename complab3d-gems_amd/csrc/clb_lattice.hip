@@ -1578,8 +1578,8 @@ __global__ void __launch_bounds__(256) k_ns_step(NsDev<R> a, double *partE, doub
 // run in the ring's order (oldest first) from a copy in LDS.
 __device__ __forceinline__ void nsTracerBlock(NsTracer *t, const double *partE, const double *partN, int nparts, int rule,
                                               double eps, int *done) {
-    __shared__ double sE[1024], sN[1024];
-    __shared__ double win[NS_TRACER_WINDOW];
+    __shared__ double sE[16], sN[16];
+    __shared__ double win[16];
     __shared__ double sVal;
     __shared__ int sHead, sFull;
     if (*done) return;
@@ -1596,14 +1596,12 @@ __device__ __forceinline__ void nsTracerBlock(NsTracer *t, const double *partE, 
 #pragma unroll
         for (int k = 0; k < 8; ++k) { E += e[k]; C += c[k]; }
     }
-    sE[tid] = E; sN[tid] = C;
+    for (int off = 32; off > 0; off >>= 1) { E += __shfl_down(E, off, 64); C += __shfl_down(C, off, 64); }
+    if ((tid & 63) == 0) { sE[tid >> 6] = E; sN[tid >> 6] = C; }
     __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if (tid < w) { sE[tid] += sE[tid + w]; sN[tid] += sN[tid + w]; }
-        __syncthreads();
-    }
     if (tid == 0) {
-        const double Es = sE[0], Cs = sN[0];
+        double Es = 0, Cs = 0;
+        for (int w = 0; w < 16; ++w) { Es += sE[w]; Cs += sN[w]; }
         const double val = Cs > 0 ? Es / Cs : 0.0;
         t->iters += 1;
         t->last = val;
@@ -1625,18 +1623,35 @@ __device__ __forceinline__ void nsTracerBlock(NsTracer *t, const double *partE, 
     }
     __syncthreads();
     if (!sFull) return;
+    // the window's mean and population standard deviation, both sums as a
+    // tree over the first 512 threads (wave sums, then the eight wave
+    // partials in order; threads 512 and up add zeros): rounding-level apart
+    // from the sequential sums
     const int W = NS_TRACER_WINDOW;
-    for (int k = tid; k < W - 1; k += 1024) win[k] = t->ring[(sHead + k) % W];
-    if (tid == 0) win[W - 1] = sVal;
+    static_assert(NS_TRACER_WINDOW <= 512, "one window sample per thread of the first eight waves");
+    const double x = tid < W - 1 ? t->ring[(sHead + tid) % W] : (tid == W - 1 ? sVal : 0.0);
+    const int lane = tid & 63, wv = tid >> 6;
+    double sx = x;
+    for (int off = 32; off > 0; off >>= 1) sx += __shfl_down(sx, off, 64);
+    if (lane == 0 && wv < 8) win[wv] = sx;
     __syncthreads();
     if (tid == 0) {
         double m = 0;
-        for (int i = 0; i < W; ++i) m += win[i];
-        m /= (double) W;
-        double sd = 0;
-        for (int i = 0; i < W; ++i) { const double d = win[i] - m; sd += d * d; }
-        sd = sqrt(sd / (double) W);
-        const double r = sd / fabs(m);
+        for (int w = 0; w < 8; ++w) m += win[w];
+        sVal = m / (double) W;
+    }
+    __syncthreads();
+    const double m = sVal;
+    const double d = tid < W ? x - m : 0.0;
+    double sd = d * d;
+    for (int off = 32; off > 0; off >>= 1) sd += __shfl_down(sd, off, 64);
+    __syncthreads();
+    if (lane == 0 && wv < 8) win[8 + wv] = sd;
+    __syncthreads();
+    if (tid == 0) {
+        double v = 0;
+        for (int w = 0; w < 8; ++w) v += win[8 + w];
+        const double r = sqrt(v / (double) W) / fabs(m);
         if (isnan(r) || r < eps) { t->converged = 1; *done = 1; }
     }
 }
@@ -1942,12 +1957,15 @@ hipError_t launchAdeTotal(const AdeDev<R> &a, int ms, double *parts, hipStream_t
     return hipGetLastError();
 }
 template <typename R>
-hipError_t launchNsStep(const NsDev<R> &a, double *partE, double *partN, NsTracer *t, int rule, double eps, hipStream_t st) {
+hipError_t launchNsStep(const NsDev<R> &a, double *partE, double *partN, NsTracer *t, int rule, double eps, hipStream_t st,
+                        hipEvent_t evStart, hipEvent_t evEnd) {
     const unsigned int gB = nsInteriorBlocks(a.Nint, a.shape), gP = nblk(a.Nopen - a.Nint, 256);
+    if (evStart) { const hipError_t e = hipEventRecord(evStart, st); if (e != hipSuccess) return e; }
     if (gB + gP) {
         if (a.shape > 0) hipLaunchKernelGGL((k_ns_step<R, true>), dim3(gB + gP), dim3(256), 0, st, a, partE, partN, (int) gP);
         else hipLaunchKernelGGL((k_ns_step<R, false>), dim3(gB + gP), dim3(256), 0, st, a, partE, partN, (int) gP);
     }
+    if (evEnd) { const hipError_t e = hipEventRecord(evEnd, st); if (e != hipSuccess) return e; }
     hipLaunchKernelGGL(k_ns_tracer, dim3(1), dim3(1024), 0, st, t, partE, partN, (int) (gB + gP), rule, eps, a.done);
     return hipGetLastError();
 }
@@ -2064,7 +2082,7 @@ hipError_t launchAdeStepGroup(const AdeMember<R> *E, const int2 *mapInt, int nIn
     template hipError_t launchAdeReopen<R>(const AdeDev<R> &, const uint8_t *, hipStream_t); \
     template hipError_t launchAdeTotal<R>(const AdeDev<R> &, int, double *, hipStream_t); \
     template hipError_t launchAdeHeld<R>(const AdeDev<R> &, int, double *, hipStream_t); \
-    template hipError_t launchNsStep<R>(const NsDev<R> &, double *, double *, NsTracer *, int, double, hipStream_t); \
+    template hipError_t launchNsStep<R>(const NsDev<R> &, double *, double *, NsTracer *, int, double, hipStream_t, hipEvent_t, hipEvent_t); \
     template hipError_t launchNsVelocity<R, R>(const NsDev<R> &, R *, double *, hipStream_t); \
     template hipError_t launchNsInit<R>(const NsDev<R> &, double, hipStream_t); \
     template hipError_t launchNsReopen<R>(const NsDev<R> &, const uint8_t *, hipStream_t); \

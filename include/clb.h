@@ -438,6 +438,12 @@ int  clb_rate_capture(clb_ctx *ctx, int case_id, int on);
  * their own) over the timed launches since the last reset, measured with HIP
  * events on the case's stream */
 int  clb_kernel_time(clb_ctx *ctx, int case_id, int reset, double *avg_ms, int64_t *launches);
+/* the same for the flow kernel (one D3Q19 iteration, k_ns_step, its
+ * convergence reduction left out) over the timed iterations of every flow
+ * solve since the last reset that did work (iterations queued after the stop
+ * return at once and are not counted); timed at the transport stride.  No
+ * counterpart in the reference. */
+int  clb_flow_kernel_time(clb_ctx *ctx, int case_id, int reset, double *avg_ms, int64_t *launches);
 /* time every stride-th transport launch only (default 1: every launch); each
  * timed launch puts two event markers on the stream, which the device waits
  * on.  No counterpart in the reference. */

@@ -131,3 +131,22 @@ def test_closed_x_planes(ctx):
     st = g.field_stats()
     tot = st[:2, 0] + st[:2, 3]
     assert np.allclose(tot, o.field_stats()[:2, 0] + o.field_stats()[:2, 3], rtol=1e-12)
+
+
+def test_flow_kernel_timing_counts_working_iterations(ctx):
+    """clb_flow_kernel_time: with every iteration timed, the timed launches of
+    a solve that stops on its rule equal the iterations it reports (launches
+    queued after the stop return at once and are not counted), and the mean
+    is a positive device time."""
+    w = configs.config1(scale=0.25)
+    c = clb.Case(ctx, w.params, w.geometry)
+    c.set_timing_stride(1)
+    c.flow_kernel_time(reset=True)
+    c.ns_init(w.flow.delta_p)
+    it, cv = c.ns_solve(5000, P.CONV_RELDELTA, 1e-7, reset=True)
+    ms, n = c.flow_kernel_time()
+    assert cv and 0 < it < 5000
+    assert n == it and ms > 0
+    c.flow_kernel_time(reset=True)
+    assert c.flow_kernel_time() == (0.0, 0)
+    c.close()
