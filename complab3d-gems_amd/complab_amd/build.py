@@ -41,14 +41,18 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
-    # the C++ example that drives a case through clb_driver.hh alone
-    exe = os.path.join(ROOT, "clb_run_case")
-    cmd = ["g++", "-O2", "-std=c++11", "-Wall", "-I" + os.path.join(REPO, "include"),
-           os.path.join(CSRC, "clb_run_case.cpp"), "-o", exe + ".tmp", "-L" + ROOT, "-lclb", "-Wl,-rpath,$ORIGIN"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("clb_run_case failed to build:\n" + r.stdout + r.stderr)
-    os.replace(exe + ".tmp", exe)
+    # the C++ programs that drive a case through clb_driver.hh (the set-up and
+    # whole steps) and through clb_processors.hh (the processor form) alone
+    exes = []
+    for name in ("clb_run_case", "clb_run_processors"):
+        exe = os.path.join(ROOT, name)
+        cmd = ["g++", "-O2", "-std=c++11", "-Wall", "-I" + os.path.join(REPO, "include"),
+               os.path.join(CSRC, name + ".cpp"), "-o", exe + ".tmp", "-L" + ROOT, "-lclb", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("%s failed to build:\n%s%s" % (name, r.stdout, r.stderr))
+        os.replace(exe + ".tmp", exe)
+        exes.append(exe)
     if verbose:
-        print("built", LIB, "and", exe)
+        print("built", LIB, "and", ", ".join(exes))
     return LIB

@@ -383,20 +383,27 @@ __global__ void __launch_bounds__(256) k_age_scatter(CaDev<R> c, const unsigned 
 // the start as the Jacobi sweeps, and a sweep that raises nothing stands on
 // it.  Sweep k counts the voxels it raised in raised[k]; a sweep queued after
 // one that raised nothing does nothing.
+// The voxel count is the compaction's, read on the device (*nDev), so the
+// host queues the sweeps right behind the compaction; the fixed grid strides
+// over the voxels.
 template <typename R>
-__global__ void __launch_bounds__(256) k_ca_age_inplace(CaDev<R> c, int *age, const int *blist, int n,
+__global__ void __launch_bounds__(256) k_ca_age_inplace(CaDev<R> c, int *age, const int *blist, const unsigned int *nDev,
                                                         unsigned int *raised, int k) {
     if (k > 0 && raised[k - 1] == 0) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool up = false;
-    if (i < n) {
-        const long long v = blist[i];
-        const int a = age[v];
-        const int na = ageRule(c, age, v, a);
-        if (na != a) { age[v] = na; up = true; }
+    const int n = (int) *nDev;
+    unsigned int up = 0;
+    for (int i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        const int i = i0 + (int) threadIdx.x;
+        bool u = false;
+        if (i < n) {
+            const long long v = blist[i];
+            const int a = age[v];
+            const int na = ageRule(c, age, v, a);
+            if (na != a) { age[v] = na; u = true; }
+        }
+        up += (unsigned int) __popcll(__ballot(u));
     }
-    const unsigned long long b = __ballot(up);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(raised + k, (unsigned int) __popcll(b));
+    if ((threadIdx.x & 63) == 0 && up) atomicAdd(raised + k, up);
 }
 
 // The single in-place sweep of updateAgeDistance3D that follows a mask change
@@ -442,18 +449,22 @@ __device__ __forceinline__ int ageRuleOrdered(const CaDev<R> &c, const int *ageO
 // voxels it changed in changed[k], and a launch queued after one that changed
 // nothing does nothing
 template <typename R>
-__global__ void __launch_bounds__(256) k_ca_age_ordered(CaDev<R> c, const int *ageOld, int *ageNew, const int *blist, int n,
-                                                        unsigned int *changed, int k) {
+__global__ void __launch_bounds__(256) k_ca_age_ordered(CaDev<R> c, const int *ageOld, int *ageNew, const int *blist,
+                                                        const unsigned int *nDev, unsigned int *changed, int k) {
     if (k > 0 && changed[k - 1] == 0) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool ch = false;
-    if (i < n) {
-        const long long v = blist[i];
-        const int na = ageRuleOrdered(c, ageOld, ageNew, v);
-        if (na != ageNew[v]) { ageNew[v] = na; ch = true; }
+    const int n = (int) *nDev;
+    unsigned int chg = 0;
+    for (int i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        const int i = i0 + (int) threadIdx.x;
+        bool ch = false;
+        if (i < n) {
+            const long long v = blist[i];
+            const int na = ageRuleOrdered(c, ageOld, ageNew, v);
+            if (na != ageNew[v]) { ageNew[v] = na; ch = true; }
+        }
+        chg += (unsigned int) __popcll(__ballot(ch));
     }
-    const unsigned long long b = __ballot(ch);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(changed + k, (unsigned int) __popcll(b));
+    if ((threadIdx.x & 63) == 0 && chg) atomicAdd(changed + k, chg);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -644,16 +655,19 @@ hipError_t launchAgeCompact(const CaDev<R> &c, int *blist, unsigned int *blockSc
     hipLaunchKernelGGL(k_age_scatter<R>, dim3(nb), dim3(256), 0, st, c, blockScratch, blist);
     return hipGetLastError();
 }
+// the age sweeps' grid: enough workgroups for every voxel of the case to
+// have a thread, at most 1024 (the compacted voxels hold biomass, a fraction)
+static inline unsigned int ageGrid(int Nopen) { return std::max(1u, std::min(nblk(Nopen, 256), 1024u)); }
 template <typename R>
-hipError_t launchCaAgeInplace(const CaDev<R> &c, int *age, const int *blist, int n, unsigned int *raised, int k,
-                              hipStream_t st) {
-    if (n > 0) hipLaunchKernelGGL(k_ca_age_inplace<R>, dim3(nblk(n, 256)), dim3(256), 0, st, c, age, blist, n, raised, k);
+hipError_t launchCaAgeInplace(const CaDev<R> &c, int *age, const int *blist, const unsigned int *nDev, unsigned int *raised,
+                              int k, hipStream_t st) {
+    hipLaunchKernelGGL(k_ca_age_inplace<R>, dim3(ageGrid(c.Nopen)), dim3(256), 0, st, c, age, blist, nDev, raised, k);
     return hipGetLastError();
 }
 template <typename R>
-hipError_t launchCaAgeOrdered(const CaDev<R> &c, const int *ageOld, int *ageNew, const int *blist, int n,
+hipError_t launchCaAgeOrdered(const CaDev<R> &c, const int *ageOld, int *ageNew, const int *blist, const unsigned int *nDev,
                               unsigned int *changed, int k, hipStream_t st) {
-    if (n > 0) hipLaunchKernelGGL(k_ca_age_ordered<R>, dim3(nblk(n, 256)), dim3(256), 0, st, c, ageOld, ageNew, blist, n, changed, k);
+    hipLaunchKernelGGL(k_ca_age_ordered<R>, dim3(ageGrid(c.Nopen)), dim3(256), 0, st, c, ageOld, ageNew, blist, nDev, changed, k);
     return hipGetLastError();
 }
 hipError_t launchOmegaSwitch(const int8_t *mask, const int *list, uint8_t *cls, int Nopen, hipStream_t st) {
@@ -721,8 +735,8 @@ hipError_t launchDoubleToI32(const double *src, int *dst, long long n, hipStream
     template hipError_t launchCaPushGather<R>(const CaDev<R> &, uint64_t, hipStream_t); \
     template hipError_t launchCaAge<R>(const CaDev<R> &, const int *, int *, long long *, hipStream_t); \
     template hipError_t launchAgeCompact<R>(const CaDev<R> &, int *, unsigned int *, unsigned int *, hipStream_t); \
-    template hipError_t launchCaAgeInplace<R>(const CaDev<R> &, int *, const int *, int, unsigned int *, int, hipStream_t); \
-    template hipError_t launchCaAgeOrdered<R>(const CaDev<R> &, const int *, int *, const int *, int, unsigned int *, int, hipStream_t); \
+    template hipError_t launchCaAgeInplace<R>(const CaDev<R> &, int *, const int *, const unsigned int *, unsigned int *, int, hipStream_t); \
+    template hipError_t launchCaAgeOrdered<R>(const CaDev<R> &, const int *, int *, const int *, const unsigned int *, unsigned int *, int, hipStream_t); \
     template hipError_t launchPrecip<R>(int, int, long long, int8_t *, const R *, double, int, int8_t *, unsigned int *, hipStream_t); \
     template hipError_t launchScalarStats<R>(int, int, long long, const R *, double *, hipStream_t); \
     template hipError_t launchToDouble<R>(const R *, double *, long long, hipStream_t); \

@@ -326,11 +326,11 @@ hipError_t launchStepCtlReset(StepCtl *, long long oldMask, hipStream_t);
 template <typename R> hipError_t launchCaPushGather(const CaDev<R> &, uint64_t, hipStream_t);
 template <typename R> hipError_t launchAgeCompact(const CaDev<R> &, int *blist, unsigned int *blockScratch, unsigned int *count,
                                                   hipStream_t);
-template <typename R> hipError_t launchCaAgeInplace(const CaDev<R> &, int *age, const int *blist, int n, unsigned int *raised,
-                                                    int k, hipStream_t);
+template <typename R> hipError_t launchCaAgeInplace(const CaDev<R> &, int *age, const int *blist, const unsigned int *nDev,
+                                                    unsigned int *raised, int k, hipStream_t);
 template <typename R> hipError_t launchCaAge(const CaDev<R> &, const int *, int *, long long *, hipStream_t);
-template <typename R> hipError_t launchCaAgeOrdered(const CaDev<R> &, const int *ageOld, int *ageNew, const int *blist, int n,
-                                                    unsigned int *changed, int k, hipStream_t);
+template <typename R> hipError_t launchCaAgeOrdered(const CaDev<R> &, const int *ageOld, int *ageNew, const int *blist,
+                                                    const unsigned int *nDev, unsigned int *changed, int k, hipStream_t);
 hipError_t launchOmegaSwitch(const int8_t *, const int *, uint8_t *, int, hipStream_t);
 hipError_t launchSolidSwitch(const int8_t *, const int *, uint8_t *, uint8_t *, int, unsigned int *, hipStream_t);
 template <typename R> hipError_t launchPrecip(int, int, long long, int8_t *, const R *, double, int, int8_t *, unsigned int *, hipStream_t);

@@ -1212,18 +1212,19 @@ struct Case : CaseBase {
     // the Jacobi fixed point, and faster).  They are queued eight at a time,
     // each skipping itself once the one before raised nothing, and the counts
     // of one batch are read back together: the first sweep that raised nothing
-    // stands on the fixed point.  More than 1000 sweeps is an error.
+    // stands on the fixed point.  More than 1000 sweeps is an error.  The
+    // sweeps read the compacted count on the device, so they are queued
+    // behind the compaction without a round trip to the host.
     Status ageRelax(hipStream_t st, bool toFixedPoint) {
         if (!toFixedPoint) return ageOrderedSweep(st);
         const int K = 8;
         unsigned int *cntDev = ageCnt;
         int *blist = ageList;
         HIPT(launchAgeCompact<R>(ca(), blist, reinterpret_cast<unsigned int *>(partsLL), cntDev, st));
-        const unsigned int n = readU(st, cntDev, 1)[0];
         int swept = 0;
         while (true) {
             HIPT(hipMemsetAsync(cntDev + 1, 0, K * sizeof(unsigned int), st));
-            for (int k = 0; k < K; ++k) HIPT(launchCaAgeInplace<R>(ca(), age, blist, (int) n, cntDev + 1, k, st));
+            for (int k = 0; k < K; ++k) HIPT(launchCaAgeInplace<R>(ca(), age, blist, cntDev, cntDev + 1, k, st));
             const unsigned int *raised = readU(st, cntDev + 1, K);
             for (int k = 0; k < K; ++k) {
                 ++swept;
@@ -1239,15 +1240,13 @@ struct Case : CaseBase {
     Status ageOrderedSweep(hipStream_t st) {
         const int K = 8;
         HIPT(launchAgeCompact<R>(ca(), ageList, reinterpret_cast<unsigned int *>(partsLL), ageCnt, st));
-        const unsigned int n = readU(st, ageCnt, 1)[0];
-        if (n == 0) return {CLB_OK, ""};
         HIPT(hipMemcpyAsync(age2, age, (size_t) N * sizeof(int), hipMemcpyDeviceToDevice, st));
         unsigned int *chg = ageCnt + 16;
         const long long cap = (long long) nx + ny + nz + 2;
         long long launches = 0;
         while (true) {
             HIPT(hipMemsetAsync(chg, 0, K * sizeof(unsigned int), st));
-            for (int k = 0; k < K; ++k) HIPT(launchCaAgeOrdered<R>(ca(), age, age2, ageList, (int) n, chg, k, st));
+            for (int k = 0; k < K; ++k) HIPT(launchCaAgeOrdered<R>(ca(), age, age2, ageList, ageCnt, chg, k, st));
             const unsigned int *changed = readU(st, chg, K);
             for (int k = 0; k < K; ++k) {
                 ++launches;

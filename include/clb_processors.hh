@@ -62,6 +62,13 @@ public:
             return detail::mismatch(t);
         for (size_t m = 0; m < mu.size() && m < (size_t) CLB_MAX_MICROBES; ++m)
             if (!detail::same(mu[m], p.mu_max[m])) return detail::mismatch(t);
+        // the half-saturation constants, Kc[microbe][substrate], of the donor and the acceptor
+        for (size_t m = 0; m < Kc.size() && m < (size_t) CLB_MAX_MICROBES; ++m) {
+            if (p.donor >= 0 && (size_t) p.donor < Kc[m].size() && !detail::same(Kc[m][(size_t) p.donor], p.ks[m]))
+                return detail::mismatch(t);
+            if (p.acceptor >= 0 && (size_t) p.acceptor < Kc[m].size() && !detail::same(Kc[m][(size_t) p.acceptor], p.ke[m]))
+                return detail::mismatch(t);
+        }
         return clb_kinetics(t.ctx, t.case_id, CLB_KIN_BIOTIC);
     }
 private:

@@ -172,3 +172,63 @@ def test_precip_convert_and_solute_solid_match_oracle(ctx):
     assert np.all(conc[solid] == 0.0)
     assert not g.precip_convert()
     g.close(); o.close()
+
+
+def test_cpp_processor_form_matches_whole_steps_and_oracle(ctx, tmp_path):
+    """The reference's loop in processor form, driven from C++ through
+    include/clb_processors.hh alone (complab3d-gems_amd/clb_run_processors,
+    built with the library): collide and stream, run_kinetics +
+    update_rxnLattices, run_abiotic_kinetics + update_abiotic_rxnLattices, and
+    the CA block with updateLocalMaskNtotalLattices3D, pushExcessBiomass3D and
+    updateAgeDistance3D, on a biotic case with an abiotic reaction whose
+    biomass starts above Bmax in places.  Equals the same steps through the
+    fused transport launch and the CA block from Python bit for bit, and the
+    oracle at 1e-10 (masks and ages exact)."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(clb.LIB_PATH), "clb_run_processors")
+    if not os.path.exists(exe):
+        pytest.fail("clb_run_processors is not built (run the build)")
+    w, u, conc = reactive_case()
+    p = w.params
+    b = spill_inputs(w)
+    omp, omb = [1.2, 1.25, 1.3, 1.1], [1.6, 1.65, 1.7, 1.5]
+    steps = 12
+    (tmp_path / "p.bin").write_bytes(bytes(p))
+    (tmp_path / "g.bin").write_bytes(np.ascontiguousarray(w.geometry, dtype=np.int8).tobytes())
+    init = np.concatenate([np.ravel(x) for x in list(u) + conc + [b]] + [np.array(omp + omb)]).astype(np.float64)
+    init.tofile(str(tmp_path / "init.bin"))
+    r = subprocess.run([exe, str(tmp_path / "p.bin"), str(tmp_path / "g.bin"), str(tmp_path / "init.bin"), str(steps),
+                        str(tmp_path / "out.bin")], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    sweeps_c = int(r.stdout.split()[1])
+    out = np.fromfile(str(tmp_path / "out.bin"), dtype=np.float64).reshape(-1, *w.geometry.shape)
+    S = p.n_species
+    cpp = {"conc": out[:S], "bio": out[S], "mask": out[S + 1], "age": out[S + 2]}
+
+    g = clb.Case(ctx, p, w.geometry)
+    o = O.OracleCase(p, w.geometry)
+    for c in (g, o):
+        prepare(c, u, conc)
+        c.write(P.FIELD_BIOMASS, 0, b)
+    sweeps_g = sweeps_o = 0
+    for _ in range(steps):
+        g.ade_steps(1)
+        o.ade_steps(1)
+        rc, n = g.ca_spill()
+        assert rc == 0
+        sweeps_g += n
+        rc, n = o.ca_spill()
+        assert rc == 0
+        sweeps_o += n
+    print("CA sweeps: C++", sweeps_c, "python", sweeps_g, "oracle", sweeps_o)
+    assert sweeps_c == sweeps_g == sweeps_o and sweeps_c > 0
+    om = o.read(P.FIELD_MASK) >= 2
+    for s in range(S):
+        assert np.array_equal(cpp["conc"][s], g.read(P.FIELD_CONC, s)), s
+        assert relerr(cpp["conc"][s], o.read(P.FIELD_CONC, s), om) <= REL, s
+    assert np.array_equal(cpp["bio"], g.read(P.FIELD_BIOMASS, 0))
+    assert relerr(cpp["bio"], o.read(P.FIELD_BIOMASS, 0)) <= REL
+    for f, key in ((P.FIELD_MASK, "mask"), (P.FIELD_AGE, "age")):
+        assert np.array_equal(cpp[key], g.read(f)) and np.array_equal(cpp[key], o.read(f)), key
+    g.close(); o.close()
