@@ -20,28 +20,28 @@ namespace clb {
 // Voxels outside the compact list are closed geometry: their mask and total
 // never move, so their max and sum are constants of the case.  In a queued
 // batch (`check`) the kernel does nothing once an earlier step halted it.
+// the update of one open voxel: its total from the microbes' biomass, its
+// mask from the total; returns the total and the mask for the reductions
 template <typename R>
-__global__ void __launch_bounds__(256) k_ca_mask_total(CaDev<R> c, int check) {
+__device__ __forceinline__ void caMaskTotalVoxel(const CaDev<R> &c, long long idx, double &bt, long long &ms) {
+    int mask = c.mask[idx];
+    if (mask != 0 && mask != 1) {
+        double bm = 0; int nm = 0;
+        for (int m = 0; m < c.M; ++m) { const double v = (double) c.B[(long long) m * c.N + idx]; if (v > 1e-14) { bm += v; nm += 3 + m; } }
+        double t = (double) c.Btot[idx];
+        if (fabs(bm - t) > 1e-14) { c.Btot[idx] = (R) bm; t = bm; }
+        const bool pore = (mask == 2);
+        if (!pore && bm < c.thrdB) { mask = 2; c.mask[idx] = 2; }
+        else if (pore && bm >= c.thrdB && nm > 0) { mask = nm; c.mask[idx] = (int8_t) nm; }
+    }
+    bt = (double) c.Btot[idx];
+    ms = mask;
+}
+// the block's max total and mask sum into the partials
+template <typename R>
+__device__ __forceinline__ void caMaskTotalPartials(const CaDev<R> &c, double bt, long long ms) {
     __shared__ double sMax[4];
     __shared__ long long sMask[4];
-    if (check && c.ctl->halt) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double bt = -1e300; long long ms = 0;
-    if (i < c.Nopen) {
-        const long long idx = c.list[i];
-        int mask = c.mask[idx];
-        if (mask != 0 && mask != 1) {
-            double bm = 0; int nm = 0;
-            for (int m = 0; m < c.M; ++m) { const double v = (double) c.B[(long long) m * c.N + idx]; if (v > 1e-14) { bm += v; nm += 3 + m; } }
-            double t = (double) c.Btot[idx];
-            if (fabs(bm - t) > 1e-14) { c.Btot[idx] = (R) bm; t = bm; }
-            const bool pore = (mask == 2);
-            if (!pore && bm < c.thrdB) { mask = 2; c.mask[idx] = 2; }
-            else if (pore && bm >= c.thrdB && nm > 0) { mask = nm; c.mask[idx] = (int8_t) nm; }
-        }
-        bt = (double) c.Btot[idx];
-        ms = mask;
-    }
     for (int off = 32; off > 0; off >>= 1) {
         bt = nanMax(bt, __shfl_down(bt, off, 64));
         ms += __shfl_down(ms, off, 64);
@@ -54,6 +54,14 @@ __global__ void __launch_bounds__(256) k_ca_mask_total(CaDev<R> c, int check) {
         for (int w = 0; w < (int) (blockDim.x >> 6); ++w) { m = nanMax(m, sMax[w]); s += sMask[w]; }
         c.partMax[blockIdx.x] = m; c.partMask[blockIdx.x] = s;
     }
+}
+template <typename R>
+__global__ void __launch_bounds__(256) k_ca_mask_total(CaDev<R> c, int check) {
+    if (check && c.ctl->halt) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double bt = -1e300; long long ms = 0;
+    if (i < c.Nopen) caMaskTotalVoxel<R>(c, c.list[i], bt, ms);
+    caMaskTotalPartials<R>(c, bt, ms);
 }
 
 // the global max and mask sum from the partials, published in the control
@@ -164,11 +172,14 @@ __device__ __forceinline__ uint32_t caDraw(uint64_t seed, uint64_t sweep, uint64
     return (uint32_t) (h >> 32);
 }
 
+// The spill sweep (pushExcessBiomass3D, part1:500-732, as a push and a
+// gather over a snapshot), over the compact open voxels (a closed voxel
+// neither pushes nor receives).  A voxel over Bmax pushes its excess into an
+// outbox slot per face and takes the sweep's stamp (sweep + 1), so nothing
+// is cleared between sweeps; then every voxel collects, in face order, the
+// slots of the face neighbours holding the current stamp.
 template <typename R>
-__global__ void __launch_bounds__(256) k_ca_push(CaDev<R> c, uint64_t sweep) {
-    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.N) return;
-    c.pushed[i] = 0;
+__device__ __forceinline__ void caPushVoxel(const CaDev<R> &c, uint64_t sweep, long long i, uint32_t stamp) {
     const int mask = c.mask[i];
     if (mask == 0 || mask == 1) return;
     if (c.half && mask == 2) return;
@@ -243,30 +254,50 @@ __global__ void __launch_bounds__(256) k_ca_push(CaDev<R> c, uint64_t sweep) {
         c.B[(long long) m * c.N + i] = (R) Bl[m];
         for (int d = 0; d < 6; ++d) c.outbox[((long long) m * 6 + d) * c.N + i] = (R) ob[m][d];
     }
-    c.pushed[i] = 1;
+    c.pushed[i] = stamp;
+}
+template <typename R>
+__global__ void __launch_bounds__(256) k_ca_push(CaDev<R> c, uint64_t sweep, uint32_t stamp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < c.Nopen) caPushVoxel<R>(c, sweep, c.list[i], stamp);
 }
 
+// the gather of voxel i from its pushing face neighbours, face order
 template <typename R>
-__global__ void __launch_bounds__(256) k_ca_gather(CaDev<R> c) {
-    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.N) return;
+__device__ __forceinline__ void caGatherVoxel(const CaDev<R> &c, long long i, uint32_t stamp) {
     int x, y, z;
     coords(i, c.nx, c.ny, x, y, z);
     double add[CLB_DEV_MAXM];
     for (int m = 0; m < c.M; ++m) add[m] = 0;
-    bool any = false;
     for (int d = 0; d < 6; ++d) {
         const int q = d + 1;
         const int X = x - D7C[q][0], Y = y - D7C[q][1], Z = z - D7C[q][2];
         if (X < 0 || Y < 0 || Z < 0 || X >= c.nx || Y >= c.ny || Z >= c.nz) continue;
         const long long j = X + (long long) c.nx * (Y + (long long) c.ny * Z);
-        if (!c.pushed[j]) continue;
-        any = true;
+        if (c.pushed[j] != stamp) continue;
         for (int m = 0; m < c.M; ++m) add[m] += (double) c.outbox[((long long) m * 6 + d) * c.N + j];
     }
-    if (!any) return;
     for (int m = 0; m < c.M; ++m)
         if (add[m] != 0.0) c.B[(long long) m * c.N + i] = (R) ((double) c.B[(long long) m * c.N + i] + add[m]);
+}
+template <typename R>
+__global__ void __launch_bounds__(256) k_ca_gather(CaDev<R> c, uint32_t stamp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < c.Nopen) caGatherVoxel<R>(c, c.list[i], stamp);
+}
+// the gather and then the mask/total update of the same voxel
+// (updateLocalMaskNtotalLattices3D after the sweep, complab.cpp:2800-2810),
+// with the block partials of the max total and the mask sum
+template <typename R>
+__global__ void __launch_bounds__(256) k_ca_gather_total(CaDev<R> c, uint32_t stamp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double bt = -1e300; long long ms = 0;
+    if (i < c.Nopen) {
+        const long long v = c.list[i];
+        caGatherVoxel<R>(c, v, stamp);
+        caMaskTotalVoxel<R>(c, v, bt, ms);
+    }
+    caMaskTotalPartials<R>(c, bt, ms);
 }
 
 // updateAgeDistance3D's rule at one voxel (every voxel that can hold biomass
@@ -635,10 +666,21 @@ hipError_t launchStepCtlReset(StepCtl *ctl, long long oldMask, hipStream_t st) {
     hipLaunchKernelGGL(k_step_ctl_reset, dim3(1), dim3(1), 0, st, ctl, oldMask);
     return hipGetLastError();
 }
+// one spill sweep (push, gather); with `total` the gather also makes the
+// mask/total update and the final reduction publishes the max total and the
+// mask sum (what updateLocalMaskNtotalLattices3D after the sweep gives)
 template <typename R>
-hipError_t launchCaPushGather(const CaDev<R> &c, uint64_t sweep, hipStream_t st) {
-    hipLaunchKernelGGL(k_ca_push<R>, dim3(nblk(c.N, 256)), dim3(256), 0, st, c, sweep);
-    hipLaunchKernelGGL(k_ca_gather<R>, dim3(nblk(c.N, 256)), dim3(256), 0, st, c);
+hipError_t launchCaPushGather(const CaDev<R> &c, uint64_t sweep, hipStream_t st, bool total) {
+    if (c.Nopen <= 0) return hipSuccess;
+    const uint32_t stamp = (uint32_t) (sweep + 1);
+    const unsigned int g = nblk(c.Nopen, 256);
+    hipLaunchKernelGGL(k_ca_push<R>, dim3(g), dim3(256), 0, st, c, sweep, stamp);
+    if (total) {
+        hipLaunchKernelGGL(k_ca_gather_total<R>, dim3(g), dim3(256), 0, st, c, stamp);
+        hipLaunchKernelGGL(k_ca_mask_final<R>, dim3(1), dim3(1024), 0, st, c, (int) g, 0);
+    } else {
+        hipLaunchKernelGGL(k_ca_gather<R>, dim3(g), dim3(256), 0, st, c, stamp);
+    }
     return hipGetLastError();
 }
 template <typename R>
@@ -732,7 +774,7 @@ hipError_t launchDoubleToI32(const double *src, int *dst, long long n, hipStream
     template hipError_t launchCaMaskFinalGroup<R>(const AdeMember<R> *, int, hipStream_t); \
     template hipError_t launchCtlResetGroup<R>(const AdeMember<R> *, int, hipStream_t); \
     template hipError_t launchCtlGatherGroup<R>(const AdeMember<R> *, int, StepCtl *, hipStream_t); \
-    template hipError_t launchCaPushGather<R>(const CaDev<R> &, uint64_t, hipStream_t); \
+    template hipError_t launchCaPushGather<R>(const CaDev<R> &, uint64_t, hipStream_t, bool); \
     template hipError_t launchCaAge<R>(const CaDev<R> &, const int *, int *, long long *, hipStream_t); \
     template hipError_t launchAgeCompact<R>(const CaDev<R> &, int *, unsigned int *, unsigned int *, hipStream_t); \
     template hipError_t launchCaAgeInplace<R>(const CaDev<R> &, int *, const int *, const unsigned int *, unsigned int *, int, hipStream_t); \

@@ -217,7 +217,9 @@ struct CaDev {
     const int *age;
     const int8_t *dist;
     R *outbox;                     // [M][6][N]
-    uint8_t *pushed;
+    // the sweep that last pushed from each voxel (its stamp, sweep + 1): a
+    // voxel pushed in the current sweep holds the current stamp
+    uint32_t *pushed;
     double bmax, thrdB;
     int half;
     uint64_t seed;
@@ -323,7 +325,7 @@ template <typename R> hipError_t launchCtlGatherGroup(const AdeMember<R> *E, int
 template <typename R> hipError_t launchCaMaskTotal(const CaDev<R> &, int check, hipStream_t);
 template <typename R> hipError_t launchCaMaskFinal(const CaDev<R> &, int nparts, int check, hipStream_t);
 hipError_t launchStepCtlReset(StepCtl *, long long oldMask, hipStream_t);
-template <typename R> hipError_t launchCaPushGather(const CaDev<R> &, uint64_t, hipStream_t);
+template <typename R> hipError_t launchCaPushGather(const CaDev<R> &, uint64_t, hipStream_t, bool total);
 template <typename R> hipError_t launchAgeCompact(const CaDev<R> &, int *blist, unsigned int *blockScratch, unsigned int *count,
                                                   hipStream_t);
 template <typename R> hipError_t launchCaAgeInplace(const CaDev<R> &, int *age, const int *blist, const unsigned int *nDev,

@@ -188,7 +188,7 @@ std::vector<size_t> chkSectionSizes(const ChkHeader &h) {
     v.push_back(sm * 7 * nop * r); v.push_back(sm * 7 * nop * r);             // fA fB
     v.push_back(si * n * r); v.push_back(m * n * r); v.push_back(n * r);      // imm B Btot
     v.push_back(n * sizeof(int)); v.push_back(n * sizeof(int));               // age age2 (pointer order)
-    v.push_back(h.M > 0 ? n : 0);                                             // pushed
+    v.push_back(h.M > 0 ? n * sizeof(uint32_t) : 0);                          // pushed (sweep stamps)
     v.push_back(3 * no * r);                                                  // u
     v.push_back(sm * np * sizeof(double));                                    // rhoN
     v.push_back(((size_t) std::max(h.M, 1) + 1) * sizeof(clb_sym_program));   // symDev  (CHK_SYM_SECTION)
@@ -312,7 +312,7 @@ struct Case : CaseBase {
     uint16_t *adeKinds = nullptr;
     unsigned long long *nsKinds = nullptr;
     R *fA = nullptr, *fB = nullptr, *imm = nullptr, *B = nullptr, *Btot = nullptr, *outbox = nullptr;
-    uint8_t *pushed = nullptr;
+    uint32_t *pushed = nullptr;                 // the stamp of the sweep that last pushed from a voxel
     int *age = nullptr, *age2 = nullptr, *ageList = nullptr;
     unsigned int *ageCnt = nullptr;             // [0] biomass voxels, [1..8] voxels raised per queued sweep
     RN *gA = nullptr, *gB = nullptr;
@@ -642,7 +642,8 @@ struct Case : CaseBase {
         Btot = dalloc<R>(allocs, N);
         if (M > 0) {
             outbox = dalloc<R>(allocs, (size_t) M * 6 * N);
-            pushed = dalloc<uint8_t>(allocs, N);
+            pushed = dalloc<uint32_t>(allocs, N);
+            HIPT(hipMemsetAsync(pushed, 0, N * sizeof(uint32_t), st));      // no sweep has pushed
         }
         age = dalloc<int>(allocs, N); age2 = dalloc<int>(allocs, N); ageList = dalloc<int>(allocs, std::max(Nopen, 1));
         ageCnt = dalloc<unsigned int>(allocs, 32);
@@ -1169,7 +1170,7 @@ struct Case : CaseBase {
     void maskTotalNow(hipStream_t st, double *gmax, long long *msum) override { maskTotal(st, gmax, msum); }
     void caSweep(hipStream_t st) override {
         if (p.n_microbes <= 0) return;
-        HIPT(launchCaPushGather<R>(ca(), caSweep_++, st));
+        HIPT(launchCaPushGather<R>(ca(), caSweep_++, st, false));
         cnt[6]++;
     }
     Status ageUpdate(hipStream_t st, bool toFixedPoint) override { return ageRelax(st, toFixedPoint); }
@@ -1201,6 +1202,10 @@ struct Case : CaseBase {
     // mask/total update + global max of the total + mask sum
     void maskTotal(hipStream_t st, double *gmax, long long *msum) {
         HIPT(launchCaMaskTotal<R>(ca(), 0, st));
+        readCtl(st, gmax, msum);
+    }
+    // the max total and mask sum the last final reduction published
+    void readCtl(hipStream_t st, double *gmax, long long *msum) {
         HIPT(hipMemcpyAsync(ctlHost, ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, st));
         waitHost(st);
         *gmax = ctlHost->gmax;
@@ -1270,8 +1275,8 @@ struct Case : CaseBase {
         if (gmax - p.bmax > 1e-12) {
             double prev = gmax; int stall = 0, wc = 0;
             while (gmax - p.bmax > 1e-12) {
-                HIPT(launchCaPushGather<R>(ca(), caSweep_++, st));
-                maskTotal(st, &gmax, &msum);
+                HIPT(launchCaPushGather<R>(ca(), caSweep_++, st, true));     // the sweep and the mask/total update
+                readCtl(st, &gmax, &msum);
                 ++sweeps; cnt[6]++;
                 if (!std::isfinite(gmax)) { if (sweepsOut) *sweepsOut = sweeps; return {CLB_E_NONFINITE, "non-finite biomass"}; }
                 if (gmax > prev - 1e-12) { if (++stall >= 100) { if (sweepsOut) *sweepsOut = sweeps; return {CLB_E_CA_STALL, "CA spill stalled"}; } }

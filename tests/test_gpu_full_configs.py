@@ -191,3 +191,47 @@ def test_config3_full_size_fp32_transport_against_fp64(ctx):
     print("fp32_ade max rel err per species", errs, "mask voxels differing", int(np.sum(ml != mh)), flush=True)
     assert max(errs) <= 1e-4
     assert int(np.sum(ml != mh)) <= 100
+
+
+def test_config4_all_64_cases_in_one_batch():
+    """The batched form at its stated size: the 64 seeded cases (edges 32 to
+    160, aspect up to 4, spheres / cylinders / fracture, 2-6 species, tau
+    0.6-1.2, Monod parameters over a decade) set up and stepped together in
+    FP32 transport with an FP64 flow; each of eight cases spread over the
+    sizes ends bit-identical to its own one-at-a-time run, and every case
+    runs its steps."""
+    from complab_amd import sweep
+    steps = 200
+    ws = [configs.sweep_case(s, P.FP32_ADE, steps=steps, stabilize=50) for s in range(64)]
+    vox = [w.params.nx * w.params.ny * w.params.nz for w in ws]
+    grp, wall = sweep.run_group(ws)
+    print("64 cases, %.1f M voxels, %.1f s" % (sum(vox) / 1e6, wall), flush=True)
+    assert all(r.status == 0 and r.steps == steps for r in grp)
+    pick = list(np.argsort(vox)[np.linspace(0, 63, 8).astype(int)])
+    alone, _ = sweep.run_sequential([configs.sweep_case(int(s), P.FP32_ADE, steps=steps, stabilize=50) for s in pick])
+    for s, a in zip(pick, alone):
+        b = grp[s]
+        assert a.steps == b.steps and a.flow_iters == b.flow_iters, a.name
+        for fa, fb in zip(a.fields, b.fields):
+            assert np.array_equal(fa, fb), a.name
+
+
+def test_config5_serving_up_to_256_cubed():
+    """The serving form at its stated sizes: 12 ABC-style requests with edges
+    drawn in 48..256 (up to 255 and down to 63), arriving as a Poisson stream and
+    admitted into the running batch between runs of 100 steps; each ends with
+    the steps, flow iterations and fields of its one-at-a-time run, bit for
+    bit, whatever ran beside it."""
+    from complab_amd import sweep
+    reqs = [configs.serving_case(100 + s, P.FP32_ADE, size=(48, 256), max_steps=400, stabilize=50) for s in range(12)]
+    edges = [e for w in reqs for e in (w.params.nx - 2, w.params.ny, w.params.nz)]
+    assert max(edges) > 250 and min(edges) < 70
+    served, wall = sweep.serve_group(reqs, rate_per_s=4.0, steady_tol=1e-6, seed=5)
+    alone, _ = sweep.run_sequential([configs.serving_case(100 + s, P.FP32_ADE, size=(48, 256), max_steps=400, stabilize=50)
+                                     for s in range(12)], steady_tol=1e-6)
+    print("12 requests, wall %.1f s, p50 %.2f s" % (wall, sweep.percentile([r.latency for r in served], 50)), flush=True)
+    for a, b in zip(alone, served):
+        assert a.status == b.status and a.status in (0, P.CLOGGED), a.name
+        assert a.steps == b.steps and a.flow_iters == b.flow_iters, a.name
+        for fa, fb in zip(a.fields, b.fields):
+            assert np.array_equal(fa, fb), a.name
