@@ -242,6 +242,50 @@ def serving_case(seed, precision=P.FP32, size=(48, 256), max_steps=20000, stabil
 H_, CO3_, CA_, HCO3_, CACO3AQ_, CALCITE_ = range(6)
 
 
+def config3_six_components(scale=1.0, precision=P.FP64, seed=3, steps=5000):
+    """The precision form's grain field and inflow with a six-component
+    carbonate-calcium tableau: components H+, CO3--, Ca++, Mg++, Na+, Cl-,
+    complexes HCO3- (H + CO3) and CaCO3(aq) (Ca + CO3), log K drawn within +-2
+    of 10.33 and 3.22 from the seed; eight mobile D3Q7 lattices (the species
+    limit leaves no slot for a mineral, so no precipitation or dissolution);
+    per-voxel FP64 Newton to 1e-12."""
+    rng = np.random.default_rng(seed)
+    nxf, ny, nz = max(12, int(512 * scale)), max(8, int(256 * scale)), max(8, int(256 * scale))
+    lat, phi = _pack(nxf, ny, nz, 3.0, 0.3, seed, kind=P.GEOM_RANDOM)
+    p = P.default_params(nxf + 2, ny, nz, 8)
+    p.precision = precision
+    MG, NA, CL, HCO3, CAAQ = 3, 4, 5, 6, 7
+    c0 = [1e-7, 1e-5, 1e-3, 5e-4, 1e-2, 1.2e-2, 2.1e-2, 1.7e-5]
+    inlet = [1e-5, 1e-7, 1e-3, 5e-4, 1e-2, 1.2e-2, 2.1e-2, 1.7e-7]
+    for s in range(8):
+        p.c0[s] = c0[s]
+        p.left_btype[s] = P.BC_DIRICHLET; p.left_bc[s] = inlet[s]
+        p.right_btype[s] = P.BC_NEUMANN
+    p.eq_on = 1
+    p.eq_solver = P.EQ_NEWTON
+    p.eq_nc = 6
+    for j in range(6):
+        p.eq_stoich[j][j] = 1.0
+        p.eq_comp_species[j] = j
+    p.eq_stoich[HCO3][H_] = 1.0; p.eq_stoich[HCO3][CO3_] = 1.0
+    p.eq_stoich[CAAQ][CA_] = 1.0; p.eq_stoich[CAAQ][CO3_] = 1.0
+    p.eq_logk[HCO3] = 10.33 + float(rng.uniform(-2, 2))
+    p.eq_logk[CAAQ] = 3.22 + float(rng.uniform(-2, 2))
+    p.eq_tol = 1e-12
+    p.eq_max_iter = 200
+    p.ns_tau = 0.8
+    p.flow_on = 1
+    p.ns_conv_rule = P.CONV_RELDELTA
+    p.ns_conv_eps = 1e-6
+    p.ns_max_iter = 100000
+    fs = FlowSetup(pe=1.0, char_length=float(nxf), delta_p=1e-5 * (nxf + 1), ns_tau=0.8,
+                   rule=P.CONV_RELDELTA, eps=1e-8, max_iter=100000)
+    ts = TransportSetup(d_pore=[0.1] * 8, d_biofilm=[0.1] * 8, stabilize_steps=0)
+    return Workload("precision6-%s %dx%dx%d random phi%.2f six-component carbonate-calcium eq"
+                    % ({P.FP64: "fp64", P.FP32: "fp32", P.FP32_ADE: "fp32ade"}[precision], nxf, ny, nz, phi),
+                    p, lat, phi, fs, ts, steps)
+
+
 def config3(scale=1.0, precision=P.FP64, seed=3, steps=5000):
     """Precision form: 512x256x256 seeded random grain field at porosity 0.3;
     the carbonate-calcium system as six species -- H+, CO3--, Ca++ (the

@@ -87,6 +87,9 @@ def main():
     ap.add_argument("--scale", type=float, default=0.25)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--components", type=int, choices=(3, 6), default=3,
+                    help="the tableau: 3 components with calcite precipitation and dissolution (the precision "
+                         "form), or 6 components on eight aqueous lattices without a mineral")
     ap.add_argument("--precisions", default="fp32_ade,fp32",
                     help="reduced precisions to compare, of fp32_ade and fp32 (empty: the FP64 run alone)")
     ap.add_argument("--flow-max-iter", type=int, default=0,
@@ -95,8 +98,10 @@ def main():
                          "while it bounds the run (the full size needs ~100 000 iterations per set-up solve)")
     args = ap.parse_args()
 
+    build = configs.config3 if args.components == 3 else configs.config3_six_components
+
     def workload(prec):
-        w = configs.config3(args.scale, prec, args.seed)
+        w = build(args.scale, prec, args.seed)
         if args.flow_max_iter > 0:
             w.flow.max_iter = args.flow_max_iter
             w.params.ns_max_iter = args.flow_max_iter
@@ -106,7 +111,7 @@ def main():
     res = {"steps": args.steps, "scale": args.scale, "flow_max_iter": args.flow_max_iter}
     hi = run(ctx, workload(P.FP64), args.steps, log=sys.stderr)
     print("fp64 done: setup %.1f s, steps %.1f s" % (hi["setup_s"], hi["steps_s"]), file=sys.stderr, flush=True)
-    res["workload"] = configs.config3(args.scale, P.FP64, args.seed).name
+    res["workload"] = build(args.scale, P.FP64, args.seed).name
     res["fp64"] = {"setup_s": round(hi["setup_s"], 2), "steps_s": round(hi["steps_s"], 2),
                    "flow_iters": hi["flow_iters"], "counters": [int(x) for x in hi["counters"]]}
     chosen = [x for x in args.precisions.split(",") if x]

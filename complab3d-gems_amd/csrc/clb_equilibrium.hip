@@ -19,7 +19,9 @@ namespace clb {
 
 // NS >= ns and NC >= nc bound the per-voxel arrays at compile time (the
 // solver's arrays then stay in registers); the launcher picks the smallest
-// bounds that hold the case, and the kernel of its solver.  The iteration is
+// bounds that hold the case (4x2, 6x3, 8x4, 8x6, 8x8), and the kernel of its
+// solver.  Up to 6 components the register file holds the solver (8x6
+// Newton: no scratch, AA-PCF: 76 bytes a lane; 8x8: 1160 and 624).  The iteration is
 // a chain of dependent FP64 operations, so waves in flight hide its latency:
 // Newton with nc <= 3 is held to 168 VGPRs for 3 waves per SIMD (a 36-byte
 // spill, 17 % faster than 2 waves at 220 VGPRs on the 512x256x256 carbonate
@@ -80,6 +82,7 @@ static void launchEqSolver(const AdeDev<R> &a, const EqDev &e, unsigned int *non
     if (e.ns <= 4 && e.nc <= 2) hipLaunchKernelGGL((k_equilibrium<R, 4, 2, SOLVER>), g, b, 0, st, a, e, nonconv);
     else if (e.ns <= 6 && e.nc <= 3) hipLaunchKernelGGL((k_equilibrium<R, 6, 3, SOLVER>), g, b, 0, st, a, e, nonconv);
     else if (e.nc <= 4) hipLaunchKernelGGL((k_equilibrium<R, EQ_MAXS, 4, SOLVER>), g, b, 0, st, a, e, nonconv);
+    else if (e.nc <= 6) hipLaunchKernelGGL((k_equilibrium<R, EQ_MAXS, 6, SOLVER>), g, b, 0, st, a, e, nonconv);
     else hipLaunchKernelGGL((k_equilibrium<R, EQ_MAXS, EQ_MAXC, SOLVER>), g, b, 0, st, a, e, nonconv);
 }
 
