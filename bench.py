@@ -181,7 +181,8 @@ def sweep_measurement(args):
         err = max(sweep.field_error(a, b) for a, b in zip(r, r64))
         res["batched_" + label] = {"wall_s": round(t, 3), "cases_per_hour": round(n / t * 3600.0, 1),
                                    "speedup_vs_plain_fp64": round(t64 / t, 3), "max_rel_err": float("%.3g" % err),
-                                   "setup_s": round(max(x.setup_seconds for x in r), 3)}
+                                   "setup_s": round(max(x.setup_seconds for x in r), 3),
+                                   "create_s": round(max(x.create_seconds for x in r), 3)}
         print("sweep: batched %s %.1f s (plain %.1f s), error %.3g" % (label, t, t64, err), file=sys.stderr, flush=True)
     return res
 

@@ -36,6 +36,7 @@ class CaseResult:
     name: str
     seconds: float = 0.0            # set-up + steps, on its worker
     setup_seconds: float = 0.0      # of which the set-up (flow solves, stabilization)
+    create_seconds: float = 0.0     # of which the case creation (host tables, uploads)
     latency: float = 0.0            # serving: arrival to completion
     steps: int = 0
     flow_iters: List[int] = field(default_factory=list)
@@ -217,6 +218,8 @@ def run_group(workloads, device=0, keep_fields=True, create_threads=8, log=None)
     try:
         with ThreadPoolExecutor(max(1, create_threads)) as ex:
             cases = list(ex.map(lambda w: Case(ctx, w.params, w.geometry), workloads))
+        ctx.sync()
+        t_create = time.perf_counter() - start
         reps = driver.setup_batch(ctx, cases, workloads)
         for c, rep, r in zip(cases, reps, results):
             c.set_omegas(rep.omega_pore, rep.omega_biofilm, 1.0)
@@ -239,6 +242,7 @@ def run_group(workloads, device=0, keep_fields=True, create_threads=8, log=None)
                         b.remove(cases[i])
                         _results_of(cases[i], workloads[i], r, keep_fields)
                         r.setup_seconds = t_setup
+                        r.create_seconds = t_create
                         r.seconds = r.latency = time.perf_counter() - start
                         if log is not None:
                             log(r)

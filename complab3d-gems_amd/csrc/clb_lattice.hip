@@ -1480,28 +1480,31 @@ __device__ __forceinline__ void nsBulkChunk(const NsDev<R> &a, int c, const NsAh
     const bool in = i < a.Nint;
     const long long bb = nsBBBase(i, h.kd, in);           // every lane of the wave
     if (!in || h.cls >= NS_BB) return;
-    R fl[19];                                           // compute type = storage type
-    pull19k<R, R>(a, i, fl, h.kd, bb, h.r0, h.r1, h.r2);
+    // the collision in FP64 whatever the storage: an FP32 lattice rounds only
+    // where it stores (the kernel is bound by its memory traffic, not its
+    // arithmetic)
+    double fl[19];
+    pull19k<R, double>(a, i, fl, h.kd, bb, h.r0, h.r1, h.r2);
     if (a.first) nsFirstHandOverBB<R>(a, i, h.kd, bb);
-    R rb = 0, jx = 0, jy = 0, jz = 0;
+    double rb = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
         rb += fl[q];
-        jx += (R) D19C[q][0] * fl[q]; jy += (R) D19C[q][1] * fl[q]; jz += (R) D19C[q][2] * fl[q];
+        jx += (double) D19C[q][0] * fl[q]; jy += (double) D19C[q][1] * fl[q]; jz += (double) D19C[q][2] * fl[q];
     }
-    const R jsq = jx * jx + jy * jy + jz * jz;
-    e += 0.5 * (double) jsq; cnt += 1;
-    const R om = (R) (h.cls == NS_BIO ? a.omBio : a.om);
+    const double jsq = jx * jx + jy * jy + jz * jz;
+    e += 0.5 * jsq; cnt += 1;
+    const double om = h.cls == NS_BIO ? a.omBio : a.om;
 #pragma unroll
-    for (int q = 0; q < 19; ++q) fl[q] = fl[q] + om * (feqInc<R>(q, rb, jx, jy, jz, jsq) - fl[q]);
+    for (int q = 0; q < 19; ++q) fl[q] = fl[q] + om * (feqInc<double>(q, rb, jx, jy, jz, jsq) - fl[q]);
 #pragma unroll
-    for (int q = 0; q < 19; ++q) a.out[pidx<19>(q, i)] = fl[q];
+    for (int q = 0; q < 19; ++q) a.out[pidx<19>(q, i)] = (R) fl[q];
     // what goes back to this voxel from its closed neighbours two launches
     // on: the new population opposite each closed link
     long long j = bb;
 #pragma unroll
     for (int q = 1; q < 19; ++q)
-        if (((h.kd >> (2 * (q - 1))) & 3ull) == 2ull) a.bbOut[j++] = fl[opp19(q)];
+        if (((h.kd >> (2 * (q - 1))) & 3ull) == 2ull) a.bbOut[j++] = (R) fl[opp19(q)];
 }
 // The interior in G blocks of 256 threads, block lb taking the contiguous
 // chunks [lb C / G, (lb + 1) C / G) of the C chunks of 256 interior voxels.
