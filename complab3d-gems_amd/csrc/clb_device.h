@@ -162,7 +162,7 @@ struct NsDev {
     R *bbCur, *bbOut;              // [popPad(Nint) x 18]
     const uint8_t *cls, *clsPull;  // [Nopen]
     const unsigned long long *kinds;  // [Nint] source kind of each moving population, 2 bits per direction
-    const int4 *nbr;               // [Nint][3] compact neighbour record (NsNbRecord), the interior's pull
+    const int2 *nbr;               // [3][Nint] compact neighbour record (NsNbRecord words), the interior's pull
     double om, omBio, rhoBarRight;
     int first;
     int *done;
@@ -170,26 +170,62 @@ struct NsDev {
                                    // > 0 that many blocks looping over chunks, loads a chunk ahead
 };
 
-// z component of D3Q19 direction q (the order of D19C in clb_common.h), for
+// components of D3Q19 direction q (the order of D19C in clb_common.h), for
 // compile-time use on both sides
+__host__ __device__ constexpr int d19x(int q) {
+    return (q == 1 || q == 7 || q == 9 || q == 11 || q == 13) ? 1
+         : ((q == 2 || q == 8 || q == 10 || q == 12 || q == 14) ? -1 : 0);
+}
+__host__ __device__ constexpr int d19y(int q) {
+    return (q == 3 || q == 7 || q == 10 || q == 15 || q == 17) ? 1
+         : ((q == 4 || q == 8 || q == 9 || q == 16 || q == 18) ? -1 : 0);
+}
 __host__ __device__ constexpr int d19z(int q) {
     return (q == 5 || q == 11 || q == 14 || q == 15 || q == 18) ? 1
          : ((q == 6 || q == 12 || q == 13 || q == 16 || q == 17) ? -1 : 0);
 }
 
-// The interior flow pull's neighbour indices in 48 bytes instead of 18 x 4:
-// the interior compact order is the dense order of the interior voxels, so
-// the source of direction q (at z - c_qz) is a small offset from one base per
-// z plane -- the voxel itself for c_qz = 0, the compact position of (x, y, z-1)
-// or (x, y, z+1) (the first listed voxel at or after it) otherwise.  A source
-// on an x plane, or an offset beyond int16, sets its bit in `far`, and the
-// pull takes that index from the full table.
+// The interior flow pull's neighbour indices in 24 bytes instead of 18 x 4.
+// The interior compact order is the dense order of the interior voxels, so
+// within one x row of the lattice the listed voxels are consecutive: with
+// P(y', z') the compact position of (x, y', z') (the listed interior voxels
+// before it in dense order), the source (x + sx, y', z') of a direction is
+// P - 1 for sx = -1, P for sx = 0 and P + [(x, y', z') listed] for sx = +1,
+// and i + sx in the voxel's own row.  The record holds the positions of the
+// eight other rows a D3Q19 pull reaches: P(y, z-1) and P(y, z+1) whole, the
+// rows at y -/+ 1 as int16 offsets from i (z) or from those two (z -/+ 1),
+// four "listed" bits for the rows whose x -/+ 1 voxels are sources, and one
+// bit per direction whose index comes from the full table instead (a source
+// on an x plane, or an offset beyond int16).  Every index the record gives is
+// checked at creation against the full table with the decoder below.
+// Device layout: three int2 words per interior voxel, word k of voxel i at
+// [k * Nint + i]: (rowM, rowP), (d01, d23), (d45, flags).
 struct NsNbRecord {
-    int32_t baseM, baseP;          // compact position of (x, y, z-1) and (x, y, z+1)
-    int16_t d[18];                 // source index minus its plane's base, direction q = 1..18
-    uint32_t far;                  // bit q-1: index from the full table
+    int32_t rowM, rowP;            // P(y, z-1), P(y, z+1)
+    int32_t d01, d23, d45;         // int16 pairs: P(y-1, z) - i, P(y+1, z) - i; P(y-1, z-1) - rowM, P(y+1, z-1) - rowM;
+                                   // P(y-1, z+1) - rowP, P(y+1, z+1) - rowP
+    uint32_t flags;                // bits 0-17: direction q = 1..18 from the full table; bits 18-21: (x, y-1, z),
+                                   // (x, y+1, z), (x, y, z-1), (x, y, z+1) listed
 };
-static_assert(sizeof(NsNbRecord) == 48, "three 16-byte loads");
+static_assert(sizeof(NsNbRecord) == 24, "three 8-byte words");
+#define NS_REC_FAR_MASK 0x3ffffu
+
+__host__ __device__ inline int nsRecLo(int w) { return (int) (short) (w & 0xffff); }
+__host__ __device__ inline int nsRecHi(int w) { return w >> 16; }
+// the compact index of the source of direction q (1..18) of interior voxel i,
+// the pull's neighbour at (x, y, z) - c_q
+__host__ __device__ inline int nsRecSource(int q, int i, int rowM, int rowP, int d01, int d23, int d45, unsigned int flags) {
+    const int sx = -d19x(q), sy = -d19y(q), sz = -d19z(q);
+    if (sy == 0 && sz == 0) return i + sx;
+    int P, bit;
+    if (sz == 0) { P = i + (sy < 0 ? nsRecLo(d01) : nsRecHi(d01)); bit = sy < 0 ? 18 : 19; }
+    else if (sy == 0) { P = sz < 0 ? rowM : rowP; bit = sz < 0 ? 20 : 21; }
+    else {
+        P = sz < 0 ? rowM + (sy < 0 ? nsRecLo(d23) : nsRecHi(d23)) : rowP + (sy < 0 ? nsRecLo(d45) : nsRecHi(d45));
+        bit = 0;                                           // D3Q19 has no source at x -/+ 1 in these rows
+    }
+    return P + (sx < 0 ? -1 : (sx > 0 ? (int) ((flags >> bit) & 1u) : 0));
+}
 
 struct NsTracer {
     double ring[NS_TRACER_WINDOW];

@@ -307,7 +307,7 @@ struct Case : CaseBase {
     // compact open-voxel layout (clb_device.h): list, neighbour tables, classes
     int Nopen = 0, Nint = 0, Nleft = 0;
     int *list = nullptr, *nb7 = nullptr, *nb19 = nullptr;
-    NsNbRecord *nsNbr = nullptr;                // [Nint] the interior flow pull's neighbour records
+    int2 *nsNbr = nullptr;                      // [3][Nint] the interior flow pull's neighbour record words
     uint8_t *adeCls = nullptr, *adeClsPull = nullptr, *nsCls = nullptr, *nsClsPull = nullptr, *reopened = nullptr;
     uint16_t *adeKinds = nullptr;
     unsigned long long *nsKinds = nullptr;
@@ -499,7 +499,7 @@ struct Case : CaseBase {
         a.Nopen = Nopen; a.Nint = Nint; a.Nleft = Nleft; a.NoP = popPad(Nopen); a.list = list; a.nb = nb19; a.geom = geom;
         a.cur = gCur(); a.out = gOut();
         a.bbCur = gCurIsA ? bbA : bbB; a.bbOut = gCurIsA ? bbB : bbA;
-        a.cls = nsCls; a.clsPull = nsClsPull; a.kinds = nsKinds; a.nbr = (const int4 *) nsNbr;
+        a.cls = nsCls; a.clsPull = nsClsPull; a.kinds = nsKinds; a.nbr = nsNbr;
         a.om = nsOmega; a.omBio = nsBioOmega; a.rhoBarRight = -deltaP * 3.0;
         a.first = nsFirst ? 1 : 0;
         a.done = nsDone;
@@ -549,36 +549,53 @@ struct Case : CaseBase {
         if (gv > CLB_PORE) return p.perm_ratio > 1e-14 ? NS_BIO : NS_BB;
         return gv == CLB_WALL ? NS_BB : NS_NODYN;
     }
-    // the interior voxels' neighbour records (NsNbRecord) from the full D3Q19
-    // table; every index a record gives is checked against the table here
-    std::vector<NsNbRecord> nsRecords(const std::vector<int> &lst, const std::vector<int> &t19,
-                                      const int (*C)[3]) const {
-        std::vector<NsNbRecord> rec((size_t) std::max(Nint, 1));
-        auto basePos = [&](int x, int y, int z) -> int {       // first listed interior voxel at or after (x, y, z)
-            if (z < 0 || z >= nz) return 0;
+    // the interior voxels' neighbour records (NsNbRecord, one word array after
+    // another) from the full D3Q19 table; every index a record gives is
+    // decoded here as the pull decodes it and checked against the table
+    std::vector<int32_t> nsRecords(const std::vector<int> &lst, const std::vector<int> &t19,
+                                   const int (*C)[3]) const {
+        const size_t M = (size_t) std::max(Nint, 1);
+        std::vector<int32_t> words(6 * M, 0);
+        for (int q = 1; q < 19; ++q)
+            if (C[q][0] != d19x(q) || C[q][1] != d19y(q) || C[q][2] != d19z(q))
+                throw HipFail{hipErrorInvalidValue, "D3Q19 direction table mismatch"};
+        // compact position of (x, y, z): the listed interior voxels before it
+        auto pos = [&](int x, int y, int z) -> long long {
+            if (y < 0 || y >= ny || z < 0 || z >= nz) return 0;
             const long long v = x + (long long) nx * (y + (long long) ny * z);
-            return (int) (std::lower_bound(lst.begin(), lst.begin() + Nint, (int) v) - lst.begin());
+            return std::lower_bound(lst.begin(), lst.begin() + Nint, (int) v) - lst.begin();
         };
+        auto listedAt = [&](int y, int z, int x) -> unsigned int {
+            if (y < 0 || y >= ny || z < 0 || z >= nz) return 0u;
+            const long long v = x + (long long) nx * (y + (long long) ny * z);
+            const long long k = pos(x, y, z);
+            return (k < Nint && lst[(size_t) k] == v) ? 1u : 0u;
+        };
+        auto fit16 = [](long long d) -> int { return (d < -32768 || d > 32767) ? 0 : (int) d; };
+        auto pack = [](int lo, int hi) -> int32_t { return (int32_t) ((uint32_t) (uint16_t) lo | ((uint32_t) (uint16_t) hi << 16)); };
         for (int i = 0; i < Nint; ++i) {
             const long long v = lst[i];
             const int x = (int) (v % nx), y = (int) ((v / nx) % ny), z = (int) (v / ((long long) nx * ny));
-            NsNbRecord &r = rec[i];
+            NsNbRecord r;
             memset(&r, 0, sizeof(r));
-            r.baseM = basePos(x, y, z - 1);
-            r.baseP = basePos(x, y, z + 1);
+            const long long pM = pos(x, y, z - 1), pP = pos(x, y, z + 1);
+            r.rowM = (int32_t) pM; r.rowP = (int32_t) pP;
+            r.d01 = pack(fit16(pos(x, y - 1, z) - i), fit16(pos(x, y + 1, z) - i));
+            r.d23 = pack(fit16(pos(x, y - 1, z - 1) - pM), fit16(pos(x, y + 1, z - 1) - pM));
+            r.d45 = pack(fit16(pos(x, y - 1, z + 1) - pP), fit16(pos(x, y + 1, z + 1) - pP));
+            r.flags = (listedAt(y - 1, z, x) << 18) | (listedAt(y + 1, z, x) << 19) |
+                      (listedAt(y, z - 1, x) << 20) | (listedAt(y, z + 1, x) << 21);
             for (int q = 1; q < 19; ++q) {
                 const int n = t19[(size_t) (q - 1) * Nopen + i];
                 if (n < 0) continue;                                // closed or outside: never decoded
-                const int cz = C[q][2];
-                if (cz != d19z(q)) throw HipFail{hipErrorInvalidValue, "D3Q19 direction table mismatch"};
-                const long long base = cz == 0 ? i : (cz > 0 ? r.baseM : r.baseP);
-                const long long dq = n - base;
-                if (n >= Nint || dq < -32768 || dq > 32767) { r.far |= 1u << (q - 1); continue; }
-                r.d[q - 1] = (int16_t) dq;
-                if (base + r.d[q - 1] != n) throw HipFail{hipErrorInvalidValue, "neighbour record does not decode"};
+                if (n >= Nint || nsRecSource(q, i, r.rowM, r.rowP, r.d01, r.d23, r.d45, r.flags) != n)
+                    r.flags |= 1u << (q - 1);
             }
+            words[2 * (size_t) i] = r.rowM; words[2 * (size_t) i + 1] = r.rowP;
+            words[2 * (M + i)] = r.d01; words[2 * (M + i) + 1] = r.d23;
+            words[2 * (2 * M + i)] = r.d45; words[2 * (2 * M + i) + 1] = (int32_t) r.flags;
         }
-        return rec;
+        return words;
     }
 
     void create(const clb_case_params &pp, const int8_t *geometry, const double *biomass0, hipStream_t st) {
@@ -651,7 +668,7 @@ struct Case : CaseBase {
             gA = dalloc<RN>(allocs, 19 * (size_t) popPad(Nopen)); gB = dalloc<RN>(allocs, 19 * (size_t) popPad(Nopen));
             bbA = dalloc<RN>(allocs, 18 * (size_t) std::max(popPad(Nint), 64LL)); bbB = dalloc<RN>(allocs, 18 * (size_t) std::max(popPad(Nint), 64LL));
             nb19 = dalloc<int>(allocs, (size_t) 18 * Nopen);
-            nsNbr = dalloc<NsNbRecord>(allocs, std::max(Nint, 1));
+            nsNbr = dalloc<int2>(allocs, 3 * (size_t) std::max(Nint, 1));
             nsKinds = dalloc<unsigned long long>(allocs, std::max(Nint, 1));
         }
         u = dalloc<R>(allocs, 3 * (size_t) Nopen);
@@ -717,8 +734,8 @@ struct Case : CaseBase {
             if (p.flow_on) {
                 std::vector<int> t19 = nbTable(19, C19h);
                 HIPT(hipMemcpy(nb19, t19.data(), t19.size() * sizeof(int), hipMemcpyHostToDevice));
-                std::vector<NsNbRecord> rec = nsRecords(lst, t19, C19h);
-                HIPT(hipMemcpy(nsNbr, rec.data(), rec.size() * sizeof(NsNbRecord), hipMemcpyHostToDevice));
+                std::vector<int32_t> rec = nsRecords(lst, t19, C19h);
+                HIPT(hipMemcpy(nsNbr, rec.data(), rec.size() * sizeof(int32_t), hipMemcpyHostToDevice));
             }
         }
         HIPT(hipMemcpyAsync(adeCls, ac.data(), Nopen, hipMemcpyHostToDevice, st));

@@ -1349,33 +1349,24 @@ __device__ __forceinline__ long long nsBBBase(int i, unsigned long long kd, bool
 }
 
 // the interior's pull from the precomputed source kinds and the voxel's
-// neighbour record (loaded beforehand, a tile ahead: nsBulkBlocks)
+// neighbour record (loaded beforehand, a chunk ahead: nsBulkBlocks)
 template <typename R, typename T>
 __device__ __forceinline__ void pull19k(const NsDev<R> &a, int i, T fl[19], unsigned long long kd, long long bb,
-                                        const int4 r0, const int4 r1, const int4 r2) {
+                                        const int2 w0, const int2 w1, const int2 w2) {
     const long long No = a.Nopen;
     if (a.first) {
 #pragma unroll
         for (int q = 0; q < 19; ++q) fl[q] = (T) a.cur[pidx<19>(q, i)];
         return;
     }
-    // the compact neighbour record: 18 indices in three 16-byte words
-    const int w[8] = {r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y};
-    const int w8 = r2.z;
-    const unsigned int far = (unsigned int) r2.w;
+    const unsigned int flags = (unsigned int) w2.y;
     int nbq[18];
 #pragma unroll
-    for (int q = 1; q < 19; ++q) {
-        const int h = q - 1;
-        const int word = h < 16 ? w[h >> 1] : w8;
-        const int dq = (h & 1) ? (word >> 16) : (int) (short) (word & 0xffff);
-        const int cz = d19z(q);
-        nbq[h] = (cz == 0 ? i : (cz > 0 ? r0.x : r0.y)) + dq;
-    }
-    if (far) {
+    for (int q = 1; q < 19; ++q) nbq[q - 1] = nsRecSource(q, i, w0.x, w0.y, w1.x, w1.y, w2.x, flags);
+    if (flags & NS_REC_FAR_MASK) {
 #pragma unroll
         for (int q = 1; q < 19; ++q)
-            if ((far >> (q - 1)) & 1u) nbq[q - 1] = a.nb[(q - 1) * No + i];
+            if ((flags >> (q - 1)) & 1u) nbq[q - 1] = a.nb[(q - 1) * No + i];
     }
     fl[0] = (T) a.cur[pidx<19>(0, i)];
     long long j = bb;
@@ -1460,7 +1451,7 @@ __device__ __forceinline__ void blockEnergy(double e, double cnt, double *partE,
 struct NsAhead {
     int cls;
     unsigned long long kd;
-    int4 r0, r1, r2;
+    int2 w0, w1, w2;
 };
 template <typename R>
 __device__ __forceinline__ void nsLoadAhead(const NsDev<R> &a, int c, NsAhead &h) {
@@ -1469,9 +1460,9 @@ __device__ __forceinline__ void nsLoadAhead(const NsDev<R> &a, int c, NsAhead &h
     h.cls = in ? (int) a.cls[i] : NS_BB;
     h.kd = in ? a.kinds[i] : 0ull;
     if (in && !a.first) {
-        h.r0 = a.nbr[3 * (long long) i]; h.r1 = a.nbr[3 * (long long) i + 1]; h.r2 = a.nbr[3 * (long long) i + 2];
+        h.w0 = a.nbr[i]; h.w1 = a.nbr[(long long) a.Nint + i]; h.w2 = a.nbr[2 * (long long) a.Nint + i];
     } else {
-        h.r0 = h.r1 = h.r2 = make_int4(0, 0, 0, 0);
+        h.w0 = h.w1 = h.w2 = make_int2(0, 0);
     }
 }
 template <typename R>
@@ -1484,7 +1475,7 @@ __device__ __forceinline__ void nsBulkChunk(const NsDev<R> &a, int c, const NsAh
     // where it stores (the kernel is bound by its memory traffic, not its
     // arithmetic)
     double fl[19];
-    pull19k<R, double>(a, i, fl, h.kd, bb, h.r0, h.r1, h.r2);
+    pull19k<R, double>(a, i, fl, h.kd, bb, h.w0, h.w1, h.w2);
     if (a.first) nsFirstHandOverBB<R>(a, i, h.kd, bb);
     double rb = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
