@@ -65,6 +65,12 @@ struct AdeDev {
     int slot[CLB_DEV_MAXS];        // per species: mobile slot or immobile slot
     int mobSpecies[CLB_DEV_MAXS];  // per mobile slot: species index
     R *cur, *out;                  // [Smax][7 x NoP] tiled (pidx<7>)
+    // the bounce-back store paired with each population buffer: per mobile
+    // slot (stride bbStride) and tile of 64 interior voxels, 64 x 6 slots
+    // holding the voxels' own populations opposite their closed links (pull
+    // kind 2), in lane order; the pull reads and writes bbOut
+    R *bbCur, *bbOut;              // [Smax][bbStride]
+    long long bbStride;            // popPad(Nint) x 6
     R *imm;                        // [Si][N] dense
     R *B;                          // [M][N] dense
     const R *u;                    // [3][Nopen]
@@ -337,6 +343,7 @@ template <typename R> hipError_t launchNsReopen(const NsDev<R> &, const uint8_t 
 // the dense bounce-back store `bb` rebuilt from the populations `g` it pairs
 // with, for the current pull kinds
 template <typename R> hipError_t launchNsBBFill(const NsDev<R> &, const R *g, R *bb, hipStream_t);
+template <typename R> hipError_t launchAdeBBFill(const AdeDev<R> &, const R *g, R *bb, hipStream_t);
 template <typename R> hipError_t launchEquilibrium(const AdeDev<R> &, const EqDev &, unsigned int *, hipStream_t);
 template <typename R> hipError_t launchScatterCompact(const R *, const int *, int, double *, hipStream_t);
 template <typename R> hipError_t launchGatherCompact(const double *, const int *, int, R *, hipStream_t);

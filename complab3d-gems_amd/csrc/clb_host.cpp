@@ -312,6 +312,7 @@ struct Case : CaseBase {
     uint16_t *adeKinds = nullptr;
     unsigned long long *nsKinds = nullptr;
     R *fA = nullptr, *fB = nullptr, *imm = nullptr, *B = nullptr, *Btot = nullptr, *outbox = nullptr;
+    R *fbbA = nullptr, *fbbB = nullptr;         // transport bounce-back stores paired with fA / fB
     uint32_t *pushed = nullptr;                 // the stamp of the sweep that last pushed from a voxel
     int *age = nullptr, *age2 = nullptr, *ageList = nullptr;
     unsigned int *ageCnt = nullptr;             // [0] biomass voxels, [1..8] voxels raised per queued sweep
@@ -419,6 +420,7 @@ struct Case : CaseBase {
 
     R *fCur() const { return fCurIsA ? fA : fB; }
     R *fOut() const { return fCurIsA ? fB : fA; }
+    long long adeBBStride() const { return std::max(popPad(Nint), 64LL) * 6; }
     RN *gCur() const { return gCurIsA ? gA : gB; }
     RN *gOut() const { return gCurIsA ? gB : gA; }
 
@@ -442,6 +444,7 @@ struct Case : CaseBase {
             a.mc0[ms] = p.c0[s]; a.momP[ms] = p.omega_pore[s]; a.momB[ms] = p.omega_biofilm[s];
         }
         a.cur = fCur(); a.out = fOut(); a.imm = imm; a.B = B;
+        a.bbCur = fCurIsA ? fbbA : fbbB; a.bbOut = fCurIsA ? fbbB : fbbA; a.bbStride = adeBBStride();
         a.u = u; a.mask = mask; a.cls = adeCls; a.clsPull = adeClsPull; a.kinds = adeKinds;
         a.rhoN = rhoN;
         a.first = adeFirst ? 1 : 0; a.reset = adeReset ? 1 : 0;
@@ -653,6 +656,8 @@ struct Case : CaseBase {
         reopened = dalloc<uint8_t>(allocs, Nopen);
         fA = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * 7 * popPad(Nopen));
         fB = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * 7 * popPad(Nopen));
+        fbbA = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * adeBBStride());
+        fbbB = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * adeBBStride());
         imm = dalloc<R>(allocs, (size_t) std::max(Si, 1) * N);
         const int M = p.n_microbes;
         B = dalloc<R>(allocs, (size_t) std::max(M, 1) * N);
@@ -931,8 +936,24 @@ struct Case : CaseBase {
         nearDirty = true;
         symMask = h.symMask; symOn = (h.symOn & 1u) != 0; symAb = (h.symOn & 2u) != 0;
         nsBBRebuild(st);                                 // derived from the loaded populations
+        adeBBRebuild(st);
         HIPT(hipStreamSynchronize(st));
         return {CLB_OK, ""};
+    }
+
+    // the transport's bounce-back stores rebuilt from the own slots of their
+    // population buffers (k_ade_bb_fill): both after the pull kinds change,
+    // voxels reopen or a restart; the current one after a kernel that changed
+    // the current populations between launches
+    void adeBBRebuild(hipStream_t st) {
+        if (Sm <= 0 || Nint <= 0) return;
+        AdeDev<R> a = ade();
+        HIPT(launchAdeBBFill<R>(a, fA, fbbA, st));
+        HIPT(launchAdeBBFill<R>(a, fB, fbbB, st));
+    }
+    void adeBBRebuildCur(hipStream_t st) {
+        if (Sm <= 0 || Nint <= 0) return;
+        HIPT(launchAdeBBFill<R>(ade(), fCur(), fCurIsA ? fbbA : fbbB, st));
     }
 
     // ------------------------------------------------------------------ flow
@@ -1094,6 +1115,7 @@ struct Case : CaseBase {
             fusedLaunch(st, false, halt, false);
             if (rateOn) HIPT(launchFillD(rateBuf, (size_t) std::max(p.n_species, 1) * N, 0.0, st));
             HIPT(launchKinetics<R>(ade(halt), kin(true), st));
+            adeBBRebuildCur(st);
             return;
         }
         if (!kinOn || !p.dissol_on) { fusedLaunch(st, kinOn, halt, maskTotal); return; }
@@ -1107,6 +1129,7 @@ struct Case : CaseBase {
         k.nearPhase = 2;
         HIPT(launchDissolPark<R>(ade(halt), k, st));
         HIPT(launchKineticsListed<R>(ade(halt), k, nearList, nearCnt, st));
+        adeBBRebuildCur(st);
     }
     void fusedLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr, bool maskTotal = false,
                      int nearPhase = 0) {
@@ -1130,12 +1153,18 @@ struct Case : CaseBase {
         if (timed) { evLaunch[evUsed] = launchNo; ++evUsed; }
         ++launchNo;
         fCurIsA = !fCurIsA;
+        const bool wasFirst = adeFirst;
         adeFirst = false;
         adeReset = false;
         if (adeClsDirty) {
             HIPT(hipMemcpyAsync(adeClsPull, adeCls, Nopen, hipMemcpyDeviceToDevice, st));
             HIPT(launchAdeKinds(nb7, adeClsPull, Nopen, Nint, adeKinds, st));
             adeClsDirty = false;
+            adeBBRebuild(st);
+        } else if (wasFirst) {
+            // the first launch handed the closed neighbours' rest populations
+            // to the own slots of the buffer it read (now the output buffer)
+            HIPT(launchAdeBBFill<R>(ade(), fOut(), fCurIsA ? fbbB : fbbA, st));
         }
     }
     void adeStabilize(hipStream_t st, int steps) override {
@@ -1166,6 +1195,7 @@ struct Case : CaseBase {
             HIPT(launchDissolPark<R>(ade(), k, st));
         }
         HIPT(launchKinetics<R>(ade(), k, st));
+        adeBBRebuildCur(st);
     }
     Status dissolConvert(hipStream_t st, int *reopenedOut) override {
         unsigned int n = 0;
@@ -1197,7 +1227,7 @@ struct Case : CaseBase {
         HIPT(hipMemsetAsync(ucnt + 1, 0, sizeof(unsigned int), st));
         HIPT(launchSolidSwitch(mask, list, adeCls, reopened, Nopen, ucnt + 1, st));
         nre = readU(st, ucnt + 1, 1)[0];
-        if (nre) HIPT(launchAdeReopen<R>(ade(), reopened, st));
+        if (nre) { HIPT(launchAdeReopen<R>(ade(), reopened, st)); adeBBRebuild(st); }
         adeClsDirty = true;
         if (reopenedOut) *reopenedOut = (int) nre;
     }
@@ -1214,6 +1244,7 @@ struct Case : CaseBase {
     void equilibriumQ(hipStream_t st, const int *halt) {
         if (!p.eq_on || p.eq_nc <= 0) return;
         HIPT(launchEquilibrium<R>(ade(halt), eqdev(), ucnt + 2, st));
+        adeBBRebuildCur(st);
     }
 
     // mask/total update + global max of the total + mask sum
@@ -1379,7 +1410,7 @@ struct Case : CaseBase {
                 HIPT(hipMemsetAsync(ucnt + 1, 0, sizeof(unsigned int), st));
                 HIPT(launchSolidSwitch(mask, list, adeCls, reopened, Nopen, ucnt + 1, st));
                 const unsigned int nre = readU(st, ucnt + 1, 1)[0];
-                if (nre) HIPT(launchAdeReopen<R>(ade(), reopened, st));
+                if (nre) { HIPT(launchAdeReopen<R>(ade(), reopened, st)); adeBBRebuild(st); }
                 adeClsDirty = true;
             }
         }
@@ -1534,6 +1565,7 @@ struct Case : CaseBase {
             if (p.mobile[index]) {
                 if (!adeFirst) return {CLB_E_STATE, "a concentration field can only be written before the first step"};
                 HIPT(launchAdeSet<R>(ade(), slot[index], stage, st));
+                adeBBRebuild(st);
             } else HIPT(launchFromDouble<R>(stage, imm + (size_t) slot[index] * N, N, st));
         } else if (field == CLB_FIELD_BIOMASS) {
             if (index < 0 || index >= p.n_microbes) return {CLB_E_ARG, "microbe index out of range"};
@@ -1961,6 +1993,8 @@ struct Batch : BatchBase {
                 e.a = c->ade(ca ? &c->ctl->halt : nullptr);
                 e.a.cur = a ? c->fA : c->fB;
                 e.a.out = a ? c->fB : c->fA;
+                e.a.bbCur = a ? c->fbbA : c->fbbB;
+                e.a.bbOut = a ? c->fbbB : c->fbbA;
                 e.a.first = 0;
                 e.a.reset = 0;
                 e.a.maskTotal = ca ? 1 : 0;
@@ -2338,6 +2372,7 @@ size_t clb_case_footprint(const clb_case_params *p) {
     const size_t NP = (size_t) popPad((long long) N);  // tiled population lattices
     b += N * 9;                                         // mask, geom, dist, phase, classes, reopened
     b += 2 * (size_t) std::max(sm, 1) * 7 * NP * r;     // ADE populations
+    b += 2 * (size_t) std::max(sm, 1) * 6 * std::max(NP, (size_t) 64) * r;   // their bounce-back stores
     b += N * (sizeof(int) + 6 * sizeof(int) + sizeof(uint16_t));   // voxel list, D3Q7 table, pull kinds
     b += (size_t) std::max(si, 1) * N * r;
     b += (size_t) std::max(p->n_microbes, 1) * N * r + N * r;

@@ -783,11 +783,40 @@ __device__ __forceinline__ void adePlaneVoxel(const AdeDev<R> &a, int i, int ms)
     for (int q = 0; q < 7; ++q) old[pidx<7>(q, i)] = f[q];
 }
 
+// The transport pull's full-way bounce-back: what comes back to a voxel from
+// a closed neighbour is its own opposite population from two launches back,
+// which sits in its own slot of `out` until the thread overwrites it; reading
+// it there fetches whole cache lines for the few lanes of a wave with that
+// closed link.  Instead each population buffer has a store beside it: per
+// mobile slot and tile of 64 interior voxels (one wave), 64 x 6 slots in which
+// voxel i's closed links (pull kind 2), q ascending, take consecutive slots
+// after those of the tile's earlier voxels, so a wave reads and writes its
+// bounce-back values densely.  The store always mirrors the own slots of its
+// buffer (still written every launch): a kernel that changes populations
+// between launches is followed by k_ade_bb_fill, as is every change of the
+// pull kinds.  Links to a closed x plane (kind 3) keep the own-slot read.
+__device__ __forceinline__ unsigned int adeKind2Mask(unsigned int kd) { return (kd >> 1) & ~kd & 0x555u; }
+// first slot of voxel i (every lane of the wave takes part: a prefix sum)
+__device__ __forceinline__ long long adeBBBase(int i, unsigned int kd, bool valid) {
+    const int c = valid ? __popc(adeKind2Mask(kd)) : 0;
+    int x = c;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return (long long) (i >> 6) * (64 * 6) + (x - c);
+}
+
 // interior: pull + BGK collide + biotic and abiotic kinetics with the clamp
 // and the D3Q7 deposit, every mobile species of a voxel in one thread
 template <typename R, int MAXS>
 __device__ __forceinline__ void adeBulkVoxel(const AdeDev<R> &a, const KinDev &k, int i, double &bt, long long &mt) {
-    if (i >= a.Nint) return;
+    const bool in = i < a.Nint;
+    const unsigned int kinds = in ? a.kinds[i] : 0u;
+    const long long bb = adeBBBase(i, kinds, in);          // every lane of the wave
+    if (!in) return;
     const int cls = a.cls[i];
     if (cls == ADE_BB) {
         if (a.maskTotal) { const long long v = a.list[i]; bt = (double) a.Btot[v]; mt = a.mask[v]; }
@@ -795,13 +824,12 @@ __device__ __forceinline__ void adeBulkVoxel(const AdeDev<R> &a, const KinDev &k
     }
     const long long No = a.Nopen;
     // where each moving population comes from, shared by every species: a
-    // neighbour's cur slot, the voxel's own opposite cur slot (grid edge), or
-    // its own opposite old slot (bounce-back); a neighbour on an x plane is
-    // closed only for the species whose plane is closed
+    // neighbour's cur slot, the voxel's own opposite cur slot (grid edge), the
+    // bounce-back store (closed neighbour), or its own opposite old slot (a
+    // neighbour on an x plane, closed only for the species whose plane is)
     int nbq[6];
 #pragma unroll
     for (int q = 1; q < 7; ++q) nbq[q - 1] = a.nb[(q - 1) * No + i];
-    const unsigned int kinds = a.kinds[i];
     int kind[6];                       // 0 neighbour, 1 edge, 2 bounce-back, 3 plane neighbour (species rule)
 #pragma unroll
     for (int q = 1; q < 7; ++q) kind[q - 1] = (kinds >> (2 * (q - 1))) & 3;
@@ -827,15 +855,18 @@ __device__ __forceinline__ void adeBulkVoxel(const AdeDev<R> &a, const KinDev &k
             for (int q = 0; q < 7; ++q) f[ms][q] = cur[pidx<7>(q, i)];
         } else {
             const bool lClosed = a.mlbt[ms] == BC_CLOSED_DEV, rClosed = a.mrbt[ms] == BC_CLOSED_DEV;
+            const R *__restrict__ bbo = a.bbOut + (long long) ms * a.bbStride;
             f[ms][0] = cur[pidx<7>(0, i)];
+            long long j = bb;
 #pragma unroll
             for (int q = 1; q < 7; ++q) {
                 const int n = nbq[q - 1];
-                int kd = kind[q - 1];
-                if (kd == 3) kd = ((n < a.Nint + a.Nleft) ? lClosed : rClosed) ? 2 : 0;
-                const R *base = kd == 2 ? old : cur;
-                const long long off = kd == 0 ? pidx<7>(q, n) : pidx<7>(opp7(q), i);
+                const int k0 = kind[q - 1];
+                const int kd = k0 == 3 ? (((n < a.Nint + a.Nleft) ? lClosed : rClosed) ? 2 : 0) : k0;
+                const R *base = k0 == 2 ? bbo : (kd == 2 ? old : cur);
+                const long long off = k0 == 2 ? j : (kd == 0 ? pidx<7>(q, n) : pidx<7>(opp7(q), i));
                 f[ms][q] = base[off];
+                if (k0 == 2) ++j;
             }
         }
     }
@@ -877,6 +908,12 @@ __device__ __forceinline__ void adeBulkVoxel(const AdeDev<R> &a, const KinDev &k
         R *o = a.out + (long long) ms * 7 * a.NoP;
 #pragma unroll
         for (int q = 0; q < 7; ++q) __builtin_nontemporal_store(f[ms][q], &o[pidx<7>(q, i)]);
+        // what goes back to this voxel from its closed neighbours two launches on
+        R *bo = a.bbOut + (long long) ms * a.bbStride;
+        long long j = bb;
+#pragma unroll
+        for (int q = 1; q < 7; ++q)
+            if (kind[q - 1] == 2) bo[j++] = f[ms][opp7(q)];
     }
     if (a.maskTotal) voxelMaskTotal<R>(a, v, vmask, bpre, bt, mt);
 }
@@ -1939,6 +1976,29 @@ hipError_t launchNsKinds(const int *nb, const uint8_t *clsPull, int Nopen, int N
     if (Nint > 0) hipLaunchKernelGGL(k_ns_kinds, dim3(nblk(Nint, 256)), dim3(256), 0, st, nb, clsPull, Nopen, Nint, kinds);
     return hipGetLastError();
 }
+// the bounce-back store `bb` of every mobile slot from the own slots of the
+// population buffer `g` it pairs with (adeBBBase), for the current pull kinds
+template <typename R>
+__global__ void __launch_bounds__(256) k_ade_bb_fill(AdeDev<R> a, const R *g, R *bb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = i < a.Nint;
+    const unsigned int kd = in ? a.kinds[i] : 0u;
+    const long long j0 = adeBBBase(i, kd, in);
+    if (!in) return;
+    for (int ms = 0; ms < a.Sm; ++ms) {
+        const R *gs = g + (long long) ms * 7 * a.NoP;
+        R *bs = bb + (long long) ms * a.bbStride;
+        long long j = j0;
+#pragma unroll
+        for (int q = 1; q < 7; ++q)
+            if (((kd >> (2 * (q - 1))) & 3u) == 2u) bs[j++] = gs[pidx<7>(opp7(q), i)];
+    }
+}
+template <typename R>
+hipError_t launchAdeBBFill(const AdeDev<R> &a, const R *g, R *bb, hipStream_t st) {
+    if (a.Nint > 0 && a.Sm > 0) hipLaunchKernelGGL(k_ade_bb_fill<R>, dim3(nblk(a.Nint, 256)), dim3(256), 0, st, a, g, bb);
+    return hipGetLastError();
+}
 hipError_t launchAdeKinds(const int *nb, const uint8_t *clsPull, int Nopen, int Nint, uint16_t *kinds, hipStream_t st) {
     if (Nint > 0) hipLaunchKernelGGL(k_ade_kinds, dim3(nblk(Nint, 256)), dim3(256), 0, st, nb, clsPull, Nopen, Nint, kinds);
     return hipGetLastError();
@@ -2084,6 +2144,7 @@ hipError_t launchAdeStepGroup(const AdeMember<R> *E, const int2 *mapInt, int nIn
     template hipError_t launchNsInit<R>(const NsDev<R> &, double, hipStream_t); \
     template hipError_t launchNsReopen<R>(const NsDev<R> &, const uint8_t *, hipStream_t); \
     template hipError_t launchNsBBFill<R>(const NsDev<R> &, const R *, R *, hipStream_t); \
+    template hipError_t launchAdeBBFill<R>(const AdeDev<R> &, const R *, R *, hipStream_t); \
     template hipError_t launchNsStepGroup<R>(const NsMember<R> *, int, const int2 *, int, hipStream_t); \
     template hipError_t launchNsGroupStatus<R>(const NsMember<R> *, int, long long *, hipStream_t); \
     template hipError_t launchAdeStepGroup<R>(const AdeMember<R> *, const int2 *, int, const int2 *, int, int, int, hipStream_t); \
