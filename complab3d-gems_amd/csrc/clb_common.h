@@ -27,6 +27,29 @@ __device__ __forceinline__ bool halted(const int *h) {
     return h && *h;
 }
 __device__ __forceinline__ int opp7(int q) { return q == 0 ? 0 : ((q & 1) ? q + 1 : q - 1); }
+
+// a voxel's first slot in the transport bounce-back store follows the closed
+// links of its tile's earlier voxels: counted here from the kinds (any lane
+// order, for the kernels that change populations between launches; the
+// transport kernel takes a wave prefix sum instead)
+__device__ inline long long adeBBSlotOf(const uint16_t *kinds, int i) {
+    const int t0 = i & ~63;
+    int c = 0;
+    for (int l = t0; l < i; ++l) c += __popc(adeKind2Mask(kinds[l]));
+    return (long long) (i >> 6) * (64 * 6) + c;
+}
+// voxel i's new own populations f of mobile slot ms into the store paired with
+// cur, for its closed links (first slot j0)
+template <typename R, typename T>
+__device__ inline void adeBBPut(const AdeDev<R> &a, unsigned int kd, long long j0, int ms, const T f[7]) {
+    if (!adeKind2Mask(kd)) return;
+    R *b = a.bbCur + (long long) ms * a.bbStride;
+    long long j = j0;
+#pragma unroll
+    for (int q = 1; q < 7; ++q)
+        if (((kd >> (2 * (q - 1))) & 3u) == 2u) b[j++] = (R) f[opp7(q)];
+}
+
 __device__ __forceinline__ int opp19(int q) { return q == 0 ? 0 : ((q & 1) ? q + 1 : q - 1); }
 __device__ __forceinline__ double w19(int q) { return q == 0 ? (1.0 / 3.0) : (q < 7 ? (1.0 / 18.0) : (1.0 / 36.0)); }
 

@@ -98,6 +98,10 @@ struct AdeDev {
     long long *partMask;
 };
 
+// the transport bounce-back store (AdeDev::bbCur/bbOut): a voxel's closed
+// links are its pull kinds equal to 2 (2 bits per direction q = 1..6)
+__host__ __device__ inline unsigned int adeKind2Mask(unsigned int kd) { return (kd >> 1) & ~kd & 0x555u; }
+
 // rate-law parameters (biotic dual Monod, abiotic mass action)
 // the reopen check of the declared phases (dissolNodeConversion3D): phase id
 // q + 1 reopens below threshold[q] of the immobile slot slot[q]

@@ -54,6 +54,10 @@ __global__ void __launch_bounds__(128, (eqWavesPerSimd<NS, NC, SOLVER>())) k_equ
     }
     const bool conv = eqSolve<NS, NC, SOLVER>(e, C, E);
     if (!conv) atomicAdd(nonconv, 1u);
+    // the voxel's closed links: the transport's bounce-back store mirrors the
+    // own slots this kernel changes
+    const unsigned int kd = a.kinds[i];
+    const long long j0 = adeKind2Mask(kd) ? adeBBSlotOf(a.kinds, i) : 0;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         if (s >= S) break;
@@ -64,8 +68,13 @@ __global__ void __launch_bounds__(128, (eqWavesPerSimd<NS, NC, SOLVER>())) k_equ
         if (fabs(dC) > 1e-12) {
             if (a.mobile[s]) {
                 R *o = a.cur + (long long) a.slot[s] * 7 * a.NoP;
+                R nv[7];
 #pragma unroll
-                for (int q = 0; q < 7; ++q) o[pidx<7>(q, i)] = (R) ((double) o[pidx<7>(q, i)] + w7(q) * dC);
+                for (int q = 0; q < 7; ++q) {
+                    nv[q] = (R) ((double) o[pidx<7>(q, i)] + w7(q) * dC);
+                    o[pidx<7>(q, i)] = nv[q];
+                }
+                adeBBPut<R>(a, kd, j0, a.slot[s], nv);
             } else {
                 R *p = a.imm + (long long) a.slot[s] * N + v;
                 *p = (R) ((double) *p + dC);

@@ -942,18 +942,15 @@ struct Case : CaseBase {
     }
 
     // the transport's bounce-back stores rebuilt from the own slots of their
-    // population buffers (k_ade_bb_fill): both after the pull kinds change,
-    // voxels reopen or a restart; the current one after a kernel that changed
-    // the current populations between launches
+    // population buffers (k_ade_bb_fill): after the pull kinds change, voxels
+    // reopen, a field is written or a restart.  The kinetics and equilibrium
+    // kernels that change the current populations between launches update
+    // the store beside the own slots themselves (adeBBPut)
     void adeBBRebuild(hipStream_t st) {
         if (Sm <= 0 || Nint <= 0) return;
         AdeDev<R> a = ade();
         HIPT(launchAdeBBFill<R>(a, fA, fbbA, st));
         HIPT(launchAdeBBFill<R>(a, fB, fbbB, st));
-    }
-    void adeBBRebuildCur(hipStream_t st) {
-        if (Sm <= 0 || Nint <= 0) return;
-        HIPT(launchAdeBBFill<R>(ade(), fCur(), fCurIsA ? fbbA : fbbB, st));
     }
 
     // ------------------------------------------------------------------ flow
@@ -1115,7 +1112,6 @@ struct Case : CaseBase {
             fusedLaunch(st, false, halt, false);
             if (rateOn) HIPT(launchFillD(rateBuf, (size_t) std::max(p.n_species, 1) * N, 0.0, st));
             HIPT(launchKinetics<R>(ade(halt), kin(true), st));
-            adeBBRebuildCur(st);
             return;
         }
         if (!kinOn || !p.dissol_on) { fusedLaunch(st, kinOn, halt, maskTotal); return; }
@@ -1129,7 +1125,6 @@ struct Case : CaseBase {
         k.nearPhase = 2;
         HIPT(launchDissolPark<R>(ade(halt), k, st));
         HIPT(launchKineticsListed<R>(ade(halt), k, nearList, nearCnt, st));
-        adeBBRebuildCur(st);
     }
     void fusedLaunch(hipStream_t st, bool kinOn, const int *halt = nullptr, bool maskTotal = false,
                      int nearPhase = 0) {
@@ -1195,7 +1190,6 @@ struct Case : CaseBase {
             HIPT(launchDissolPark<R>(ade(), k, st));
         }
         HIPT(launchKinetics<R>(ade(), k, st));
-        adeBBRebuildCur(st);
     }
     Status dissolConvert(hipStream_t st, int *reopenedOut) override {
         unsigned int n = 0;
@@ -1244,7 +1238,6 @@ struct Case : CaseBase {
     void equilibriumQ(hipStream_t st, const int *halt) {
         if (!p.eq_on || p.eq_nc <= 0) return;
         HIPT(launchEquilibrium<R>(ade(halt), eqdev(), ucnt + 2, st));
-        adeBBRebuildCur(st);
     }
 
     // mask/total update + global max of the total + mask sum

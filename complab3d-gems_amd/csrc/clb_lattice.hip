@@ -795,7 +795,6 @@ __device__ __forceinline__ void adePlaneVoxel(const AdeDev<R> &a, int i, int ms)
 // buffer (still written every launch): a kernel that changes populations
 // between launches is followed by k_ade_bb_fill, as is every change of the
 // pull kinds.  Links to a closed x plane (kind 3) keep the own-slot read.
-__device__ __forceinline__ unsigned int adeKind2Mask(unsigned int kd) { return (kd >> 1) & ~kd & 0x555u; }
 // first slot of voxel i (every lane of the wave takes part: a prefix sum)
 __device__ __forceinline__ long long adeBBBase(int i, unsigned int kd, bool valid) {
     const int c = valid ? __popc(adeKind2Mask(kd)) : 0;
@@ -988,12 +987,15 @@ __device__ __forceinline__ void kineticsVoxel(const AdeDev<R> &a, const KinDev &
     }
     R nopre[CLB_DEV_MAXM] = {};
     voxelKinetics<R, MAXS, DISSOL, SYM>(a, k, i, f, v, mask, nopre, false);
+    const unsigned int kd = a.kinds[i];
+    const long long j0 = adeKind2Mask(kd) ? adeBBSlotOf(a.kinds, i) : 0;
 #pragma unroll
     for (int ms = 0; ms < MAXS; ++ms) {
         if (ms >= a.Sm) continue;
         R *c = a.cur + (long long) ms * 7 * a.NoP;
 #pragma unroll
         for (int q = 0; q < 7; ++q) c[pidx<7>(q, i)] = f[ms][q];
+        adeBBPut<R>(a, kd, j0, ms, f[ms]);                     // the store mirrors the own slots
     }
 }
 template <typename R, int MAXS, bool DISSOL, bool SYM>
