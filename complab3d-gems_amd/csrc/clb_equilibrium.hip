@@ -21,7 +21,7 @@ namespace clb {
 // solver's arrays then stay in registers); the launcher picks the smallest
 // bounds that hold the case (4x2, 6x3, 8x4, 8x6, 8x8), and the kernel of its
 // solver.  Up to 6 components the register file holds the solver (8x6
-// Newton: no scratch, AA-PCF: 76 bytes a lane; 8x8: 1160 and 624).  The iteration is
+// Newton: no scratch, AA-PCF: 116 bytes a lane; 8x8: 1156 and 648).  The iteration is
 // a chain of dependent FP64 operations, so waves in flight hide its latency:
 // Newton with nc <= 3 is held to 168 VGPRs for 3 waves per SIMD (a 36-byte
 // spill, 17 % faster than 2 waves at 220 VGPRs on the 512x256x256 carbonate
