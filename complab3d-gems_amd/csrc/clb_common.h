@@ -32,11 +32,38 @@ __device__ __forceinline__ int opp7(int q) { return q == 0 ? 0 : ((q & 1) ? q + 
 // links of its tile's earlier voxels: counted here from the kinds (any lane
 // order, for the kernels that change populations between launches; the
 // transport kernel takes a wave prefix sum instead)
+// (the kinds array is padded to whole tiles: the tile's 64 entries are one
+// 128-byte line, read as eight independent 16-byte loads)
 __device__ inline long long adeBBSlotOf(const uint16_t *kinds, int i) {
-    const int t0 = i & ~63;
+    const uint4 *kv = reinterpret_cast<const uint4 *>(kinds + (i & ~63));
+    const int lane = i & 63;
+    uint4 w[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[k] = kv[k];
     int c = 0;
-    for (int l = t0; l < i; ++l) c += __popc(adeKind2Mask(kinds[l]));
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const unsigned int h[4] = {w[k].x, w[k].y, w[k].z, w[k].w};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const unsigned int kd = (h[e >> 1] >> (16 * (e & 1))) & 0xffffu;
+            if (8 * k + e < lane) c += __popc(adeKind2Mask(kd));
+        }
+    }
     return (long long) (i >> 6) * (64 * 6) + c;
+}
+// the same as a prefix sum over the wave (every lane of the wave takes part;
+// the wave's voxels are the 64 of one tile)
+__device__ __forceinline__ long long adeBBBase(int i, unsigned int kd, bool valid) {
+    const int c = valid ? __popc(adeKind2Mask(kd)) : 0;
+    int x = c;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return (long long) (i >> 6) * (64 * 6) + (x - c);
 }
 // voxel i's new own populations f of mobile slot ms into the store paired with
 // cur, for its closed links (first slot j0)

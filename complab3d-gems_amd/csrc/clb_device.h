@@ -71,6 +71,7 @@ struct AdeDev {
     // kind 2), in lane order; the pull reads and writes bbOut
     R *bbCur, *bbOut;              // [Smax][bbStride]
     long long bbStride;            // popPad(Nint) x 6
+    int bbOn;                      // the store is in use (else the pull reads the own slots of out)
     R *imm;                        // [Si][N] dense
     R *B;                          // [M][N] dense
     const R *u;                    // [3][Nopen]

@@ -21,7 +21,7 @@ namespace clb {
 // solver's arrays then stay in registers); the launcher picks the smallest
 // bounds that hold the case (4x2, 6x3, 8x4, 8x6, 8x8), and the kernel of its
 // solver.  Up to 6 components the register file holds the solver (8x6
-// Newton: no scratch, AA-PCF: 116 bytes a lane; 8x8: 1156 and 648).  The iteration is
+// Newton: no scratch, AA-PCF: 76 bytes a lane; 8x8: 1160 and 624).  The iteration is
 // a chain of dependent FP64 operations, so waves in flight hide its latency:
 // Newton with nc <= 3 is held to 168 VGPRs for 3 waves per SIMD (a 36-byte
 // spill, 17 % faster than 2 waves at 220 VGPRs on the 512x256x256 carbonate
@@ -54,10 +54,6 @@ __global__ void __launch_bounds__(128, (eqWavesPerSimd<NS, NC, SOLVER>())) k_equ
     }
     const bool conv = eqSolve<NS, NC, SOLVER>(e, C, E);
     if (!conv) atomicAdd(nonconv, 1u);
-    // the voxel's closed links: the transport's bounce-back store mirrors the
-    // own slots this kernel changes
-    const unsigned int kd = a.kinds[i];
-    const long long j0 = adeKind2Mask(kd) ? adeBBSlotOf(a.kinds, i) : 0;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         if (s >= S) break;
@@ -68,13 +64,8 @@ __global__ void __launch_bounds__(128, (eqWavesPerSimd<NS, NC, SOLVER>())) k_equ
         if (fabs(dC) > 1e-12) {
             if (a.mobile[s]) {
                 R *o = a.cur + (long long) a.slot[s] * 7 * a.NoP;
-                R nv[7];
 #pragma unroll
-                for (int q = 0; q < 7; ++q) {
-                    nv[q] = (R) ((double) o[pidx<7>(q, i)] + w7(q) * dC);
-                    o[pidx<7>(q, i)] = nv[q];
-                }
-                adeBBPut<R>(a, kd, j0, a.slot[s], nv);
+                for (int q = 0; q < 7; ++q) o[pidx<7>(q, i)] = (R) ((double) o[pidx<7>(q, i)] + w7(q) * dC);
             } else {
                 R *p = a.imm + (long long) a.slot[s] * N + v;
                 *p = (R) ((double) *p + dC);

@@ -313,6 +313,7 @@ struct Case : CaseBase {
     unsigned long long *nsKinds = nullptr;
     R *fA = nullptr, *fB = nullptr, *imm = nullptr, *B = nullptr, *Btot = nullptr, *outbox = nullptr;
     R *fbbA = nullptr, *fbbB = nullptr;         // transport bounce-back stores paired with fA / fB
+    bool adeBBOn = false;                       // the stores are in use (AdeDev::bbOn)
     uint32_t *pushed = nullptr;                 // the stamp of the sweep that last pushed from a voxel
     int *age = nullptr, *age2 = nullptr, *ageList = nullptr;
     unsigned int *ageCnt = nullptr;             // [0] biomass voxels, [1..8] voxels raised per queued sweep
@@ -445,6 +446,7 @@ struct Case : CaseBase {
         }
         a.cur = fCur(); a.out = fOut(); a.imm = imm; a.B = B;
         a.bbCur = fCurIsA ? fbbA : fbbB; a.bbOut = fCurIsA ? fbbB : fbbA; a.bbStride = adeBBStride();
+        a.bbOn = adeBBOn ? 1 : 0;
         a.u = u; a.mask = mask; a.cls = adeCls; a.clsPull = adeClsPull; a.kinds = adeKinds;
         a.rhoN = rhoN;
         a.first = adeFirst ? 1 : 0; a.reset = adeReset ? 1 : 0;
@@ -563,16 +565,23 @@ struct Case : CaseBase {
             if (C[q][0] != d19x(q) || C[q][1] != d19y(q) || C[q][2] != d19z(q))
                 throw HipFail{hipErrorInvalidValue, "D3Q19 direction table mismatch"};
         // compact position of (x, y, z): the listed interior voxels before it
+        // (the interior list is in dense order: one pass counts them all)
+        std::vector<int> before((size_t) N + 1);
+        {
+            int k = 0;
+            for (long long v = 0; v <= N; ++v) {
+                while (k < Nint && lst[(size_t) k] < v) ++k;
+                before[(size_t) v] = k;
+            }
+        }
         auto pos = [&](int x, int y, int z) -> long long {
             if (y < 0 || y >= ny || z < 0 || z >= nz) return 0;
-            const long long v = x + (long long) nx * (y + (long long) ny * z);
-            return std::lower_bound(lst.begin(), lst.begin() + Nint, (int) v) - lst.begin();
+            return before[(size_t) (x + (long long) nx * (y + (long long) ny * z))];
         };
         auto listedAt = [&](int y, int z, int x) -> unsigned int {
             if (y < 0 || y >= ny || z < 0 || z >= nz) return 0u;
-            const long long v = x + (long long) nx * (y + (long long) ny * z);
-            const long long k = pos(x, y, z);
-            return (k < Nint && lst[(size_t) k] == v) ? 1u : 0u;
+            const size_t v = (size_t) (x + (long long) nx * (y + (long long) ny * z));
+            return before[v + 1] > before[v] ? 1u : 0u;
         };
         auto fit16 = [](long long d) -> int { return (d < -32768 || d > 32767) ? 0 : (int) d; };
         auto pack = [](int lo, int hi) -> int32_t { return (int32_t) ((uint32_t) (uint16_t) lo | ((uint32_t) (uint16_t) hi << 16)); };
@@ -651,13 +660,13 @@ struct Case : CaseBase {
         list = dalloc<int>(allocs, Nopen);
         nb7 = dalloc<int>(allocs, (size_t) 6 * Nopen);
         adeCls = dalloc<uint8_t>(allocs, Nopen); adeClsPull = dalloc<uint8_t>(allocs, Nopen);
-        adeKinds = dalloc<uint16_t>(allocs, std::max(Nint, 1));
+        // whole tiles of 64 (adeBBSlotOf reads a tile's line); the tail stays 0
+        adeKinds = dalloc<uint16_t>(allocs, (size_t) std::max(popPad(Nint), 64LL));
+        HIPT(hipMemsetAsync(adeKinds, 0, (size_t) std::max(popPad(Nint), 64LL) * sizeof(uint16_t), st));
         nsCls = dalloc<uint8_t>(allocs, Nopen); nsClsPull = dalloc<uint8_t>(allocs, Nopen);
         reopened = dalloc<uint8_t>(allocs, Nopen);
         fA = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * 7 * popPad(Nopen));
         fB = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * 7 * popPad(Nopen));
-        fbbA = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * adeBBStride());
-        fbbB = dalloc<R>(allocs, (size_t) std::max(Sm, 1) * adeBBStride());
         imm = dalloc<R>(allocs, (size_t) std::max(Si, 1) * N);
         const int M = p.n_microbes;
         B = dalloc<R>(allocs, (size_t) std::max(M, 1) * N);
@@ -736,6 +745,21 @@ struct Case : CaseBase {
         {
             std::vector<int> t7 = nbTable(7, C7h);
             HIPT(hipMemcpy(nb7, t7.data(), t7.size() * sizeof(int), hipMemcpyHostToDevice));
+            // the transport bounce-back store pays where closed links are few
+            // (measured: 0.73 per interior voxel 5 % faster, 1.6 per voxel 3 %
+            // slower than the own-slot reads) and no equilibrium changes every
+            // voxel's own slots each step
+            long long closedLinks = 0;
+            for (int i = 0; i < Nint; ++i)
+                for (int q = 1; q < 7; ++q) {
+                    const int n = t7[(size_t) (q - 1) * Nopen + i];
+                    if (n == NB_CLOSED || (n >= 0 && ac[(size_t) n] == ADE_BB)) ++closedLinks;
+                }
+            adeBBOn = Sm > 0 && Nint > 0 && !p.eq_on && closedLinks <= (long long) Nint;
+            if (adeBBOn) {
+                fbbA = dalloc<R>(allocs, (size_t) Sm * adeBBStride());
+                fbbB = dalloc<R>(allocs, (size_t) Sm * adeBBStride());
+            }
             if (p.flow_on) {
                 std::vector<int> t19 = nbTable(19, C19h);
                 HIPT(hipMemcpy(nb19, t19.data(), t19.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -947,7 +971,7 @@ struct Case : CaseBase {
     // kernels that change the current populations between launches update
     // the store beside the own slots themselves (adeBBPut)
     void adeBBRebuild(hipStream_t st) {
-        if (Sm <= 0 || Nint <= 0) return;
+        if (!adeBBOn) return;
         AdeDev<R> a = ade();
         HIPT(launchAdeBBFill<R>(a, fA, fbbA, st));
         HIPT(launchAdeBBFill<R>(a, fB, fbbB, st));
@@ -1156,7 +1180,7 @@ struct Case : CaseBase {
             HIPT(launchAdeKinds(nb7, adeClsPull, Nopen, Nint, adeKinds, st));
             adeClsDirty = false;
             adeBBRebuild(st);
-        } else if (wasFirst) {
+        } else if (wasFirst && adeBBOn) {
             // the first launch handed the closed neighbours' rest populations
             // to the own slots of the buffer it read (now the output buffer)
             HIPT(launchAdeBBFill<R>(ade(), fOut(), fCurIsA ? fbbB : fbbA, st));

@@ -792,29 +792,24 @@ __device__ __forceinline__ void adePlaneVoxel(const AdeDev<R> &a, int i, int ms)
 // voxel i's closed links (pull kind 2), q ascending, take consecutive slots
 // after those of the tile's earlier voxels, so a wave reads and writes its
 // bounce-back values densely.  The store always mirrors the own slots of its
-// buffer (still written every launch): a kernel that changes populations
-// between launches is followed by k_ade_bb_fill, as is every change of the
-// pull kinds.  Links to a closed x plane (kind 3) keep the own-slot read.
-// first slot of voxel i (every lane of the wave takes part: a prefix sum)
-__device__ __forceinline__ long long adeBBBase(int i, unsigned int kd, bool valid) {
-    const int c = valid ? __popc(adeKind2Mask(kd)) : 0;
-    int x = c;
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
-    return (long long) (i >> 6) * (64 * 6) + (x - c);
-}
+// buffer (still written every launch): the split kinetics kernels write it
+// beside the own slots they change, and k_ade_bb_fill rebuilds it after every
+// other change of the populations between launches and of the pull kinds.
+// Links to a closed x plane (kind 3) keep the own-slot read.  A case uses the
+// store (AdeDev::bbOn) only where it pays: few closed links per voxel and no
+// per-step equilibrium (whose update of every voxel's own slots would have
+// to be mirrored each step).
 
 // interior: pull + BGK collide + biotic and abiotic kinetics with the clamp
-// and the D3Q7 deposit, every mobile species of a voxel in one thread
-template <typename R, int MAXS>
+// and the D3Q7 deposit, every mobile species of a voxel in one thread.  BB:
+// the bounce-back store may be in use (then AdeDev::bbOn decides); without it
+// the kernel carries none of the store's code
+template <typename R, int MAXS, bool BB>
 __device__ __forceinline__ void adeBulkVoxel(const AdeDev<R> &a, const KinDev &k, int i, double &bt, long long &mt) {
+    const bool useBB = BB && a.bbOn;
     const bool in = i < a.Nint;
     const unsigned int kinds = in ? a.kinds[i] : 0u;
-    const long long bb = adeBBBase(i, kinds, in);          // every lane of the wave
+    const long long bb = useBB ? adeBBBase(i, kinds, in) : 0;   // every lane of the wave
     if (!in) return;
     const int cls = a.cls[i];
     if (cls == ADE_BB) {
@@ -862,10 +857,11 @@ __device__ __forceinline__ void adeBulkVoxel(const AdeDev<R> &a, const KinDev &k
                 const int n = nbq[q - 1];
                 const int k0 = kind[q - 1];
                 const int kd = k0 == 3 ? (((n < a.Nint + a.Nleft) ? lClosed : rClosed) ? 2 : 0) : k0;
-                const R *base = k0 == 2 ? bbo : (kd == 2 ? old : cur);
-                const long long off = k0 == 2 ? j : (kd == 0 ? pidx<7>(q, n) : pidx<7>(opp7(q), i));
+                const bool st = useBB && k0 == 2;
+                const R *base = st ? bbo : (kd == 2 ? old : cur);
+                const long long off = st ? j : (kd == 0 ? pidx<7>(q, n) : pidx<7>(opp7(q), i));
                 f[ms][q] = base[off];
-                if (k0 == 2) ++j;
+                if (st) ++j;
             }
         }
     }
@@ -908,21 +904,23 @@ __device__ __forceinline__ void adeBulkVoxel(const AdeDev<R> &a, const KinDev &k
 #pragma unroll
         for (int q = 0; q < 7; ++q) __builtin_nontemporal_store(f[ms][q], &o[pidx<7>(q, i)]);
         // what goes back to this voxel from its closed neighbours two launches on
-        R *bo = a.bbOut + (long long) ms * a.bbStride;
-        long long j = bb;
+        if (useBB) {
+            R *bo = a.bbOut + (long long) ms * a.bbStride;
+            long long j = bb;
 #pragma unroll
-        for (int q = 1; q < 7; ++q)
-            if (kind[q - 1] == 2) bo[j++] = f[ms][opp7(q)];
+            for (int q = 1; q < 7; ++q)
+                if (kind[q - 1] == 2) bo[j++] = f[ms][opp7(q)];
+        }
     }
     if (a.maskTotal) voxelMaskTotal<R>(a, v, vmask, bpre, bt, mt);
 }
 
-template <typename R, int MAXS>
+template <typename R, int MAXS, bool BB>
 __global__ void __launch_bounds__(256) k_ade_bulk(AdeDev<R> a, KinDev k) {
     if (halted(a.halt)) return;
     const int blk = xcdBlock();
     double bt = -1e300; long long mt = 0;
-    adeBulkVoxel<R, MAXS>(a, k, blk * blockDim.x + threadIdx.x, bt, mt);
+    adeBulkVoxel<R, MAXS, BB>(a, k, blk * blockDim.x + threadIdx.x, bt, mt);
     if (a.maskTotal) blockMaskTotal(bt, mt, a.partMax, a.partMask, blk);
 }
 
@@ -987,7 +985,7 @@ __device__ __forceinline__ void kineticsVoxel(const AdeDev<R> &a, const KinDev &
     }
     R nopre[CLB_DEV_MAXM] = {};
     voxelKinetics<R, MAXS, DISSOL, SYM>(a, k, i, f, v, mask, nopre, false);
-    const unsigned int kd = a.kinds[i];
+    const unsigned int kd = a.bbOn ? a.kinds[i] : 0u;
     const long long j0 = adeKind2Mask(kd) ? adeBBSlotOf(a.kinds, i) : 0;
 #pragma unroll
     for (int ms = 0; ms < MAXS; ++ms) {
@@ -1854,17 +1852,22 @@ static unsigned int nsInteriorBlocks(int Nint, int shape) {
 }
 unsigned int nsStepBlocks(int Nopen, int Nint, int shape) { return nsInteriorBlocks(Nint, shape) + nblk(Nopen - Nint, 256); }
 
+template <typename R, bool BB>
+static void launchAdeBulk(const AdeDev<R> &a, const KinDev &k, unsigned int g, hipStream_t st) {
+    if (a.Sm <= 1) hipLaunchKernelGGL((k_ade_bulk<R, 1, BB>), dim3(g), dim3(256), 0, st, a, k);
+    else if (a.Sm <= 2) hipLaunchKernelGGL((k_ade_bulk<R, 2, BB>), dim3(g), dim3(256), 0, st, a, k);
+    else if (a.Sm <= 3) hipLaunchKernelGGL((k_ade_bulk<R, 3, BB>), dim3(g), dim3(256), 0, st, a, k);
+    else if (a.Sm <= 4) hipLaunchKernelGGL((k_ade_bulk<R, 4, BB>), dim3(g), dim3(256), 0, st, a, k);
+    else if (a.Sm <= 6) hipLaunchKernelGGL((k_ade_bulk<R, 6, BB>), dim3(g), dim3(256), 0, st, a, k);
+    else hipLaunchKernelGGL((k_ade_bulk<R, 8, BB>), dim3(g), dim3(256), 0, st, a, k);
+}
 template <typename R>
 hipError_t launchAdeStep(const AdeDev<R> &a, const KinDev &k, hipStream_t st, hipEvent_t afterInterior) {
     const int Np = a.Nopen - a.Nint;
     if (a.Nint > 0) {
         const unsigned int g = nblk(a.Nint, 256);
-        if (a.Sm <= 1) hipLaunchKernelGGL((k_ade_bulk<R, 1>), dim3(g), dim3(256), 0, st, a, k);
-        else if (a.Sm <= 2) hipLaunchKernelGGL((k_ade_bulk<R, 2>), dim3(g), dim3(256), 0, st, a, k);
-        else if (a.Sm <= 3) hipLaunchKernelGGL((k_ade_bulk<R, 3>), dim3(g), dim3(256), 0, st, a, k);
-        else if (a.Sm <= 4) hipLaunchKernelGGL((k_ade_bulk<R, 4>), dim3(g), dim3(256), 0, st, a, k);
-        else if (a.Sm <= 6) hipLaunchKernelGGL((k_ade_bulk<R, 6>), dim3(g), dim3(256), 0, st, a, k);
-        else hipLaunchKernelGGL((k_ade_bulk<R, 8>), dim3(g), dim3(256), 0, st, a, k);
+        if (a.bbOn) launchAdeBulk<R, true>(a, k, g, st);
+        else launchAdeBulk<R, false>(a, k, g, st);
     }
     if (afterInterior) {
         const hipError_t e = hipEventRecord(afterInterior, st);
@@ -1998,7 +2001,7 @@ __global__ void __launch_bounds__(256) k_ade_bb_fill(AdeDev<R> a, const R *g, R 
 }
 template <typename R>
 hipError_t launchAdeBBFill(const AdeDev<R> &a, const R *g, R *bb, hipStream_t st) {
-    if (a.Nint > 0 && a.Sm > 0) hipLaunchKernelGGL(k_ade_bb_fill<R>, dim3(nblk(a.Nint, 256)), dim3(256), 0, st, a, g, bb);
+    if (a.bbOn && a.Nint > 0 && a.Sm > 0) hipLaunchKernelGGL(k_ade_bb_fill<R>, dim3(nblk(a.Nint, 256)), dim3(256), 0, st, a, g, bb);
     return hipGetLastError();
 }
 hipError_t launchAdeKinds(const int *nb, const uint8_t *clsPull, int Nopen, int Nint, uint16_t *kinds, hipStream_t st) {
@@ -2102,7 +2105,7 @@ __global__ void __launch_bounds__(256) k_ade_bulk_group(const AdeMember<R> *__re
     const AdeMember<R> &g = E[e.x];
     if (halted(g.a.halt)) return;
     double bt = -1e300; long long mt = 0;
-    adeBulkVoxel<R, MAXS>(g.a, g.k, e.y * (int) blockDim.x + (int) threadIdx.x, bt, mt);
+    adeBulkVoxel<R, MAXS, true>(g.a, g.k, e.y * (int) blockDim.x + (int) threadIdx.x, bt, mt);
     if (g.a.maskTotal) blockMaskTotal(bt, mt, g.a.partMax, g.a.partMask, e.y);
 }
 template <typename R>
