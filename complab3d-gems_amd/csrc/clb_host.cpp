@@ -1287,20 +1287,34 @@ struct Case : CaseBase {
     // behind the compaction without a round trip to the host.
     Status ageRelax(hipStream_t st, bool toFixedPoint) {
         if (!toFixedPoint) return ageOrderedSweep(st);
-        const int K = 8;
-        unsigned int *cntDev = ageCnt;
-        int *blist = ageList;
-        HIPT(launchAgeCompact<R>(ca(), blist, reinterpret_cast<unsigned int *>(partsLL), cntDev, st));
+        ageRelaxQueue(st);
+        waitHost(st);
+        return ageRelaxFinish(st);
+    }
+    static constexpr int kAgeBatch = 8;
+    // the compaction and the first batch of in-place sweeps, with the copy of
+    // their counts to pinned memory queued behind them (the caller waits, for
+    // this and whatever else it queued)
+    void ageRelaxQueue(hipStream_t st) {
+        HIPT(launchAgeCompact<R>(ca(), ageList, reinterpret_cast<unsigned int *>(partsLL), ageCnt, st));
+        HIPT(hipMemsetAsync(ageCnt + 1, 0, kAgeBatch * sizeof(unsigned int), st));
+        for (int k = 0; k < kAgeBatch; ++k) HIPT(launchCaAgeInplace<R>(ca(), age, ageList, ageCnt, ageCnt + 1, k, st));
+        HIPT(hipMemcpyAsync(pinU, ageCnt + 1, kAgeBatch * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+    }
+    // after that wait: the first batch's counts, then further batches until a
+    // sweep raised nothing
+    Status ageRelaxFinish(hipStream_t st) {
+        const unsigned int *raised = pinU;
         int swept = 0;
         while (true) {
-            HIPT(hipMemsetAsync(cntDev + 1, 0, K * sizeof(unsigned int), st));
-            for (int k = 0; k < K; ++k) HIPT(launchCaAgeInplace<R>(ca(), age, blist, cntDev, cntDev + 1, k, st));
-            const unsigned int *raised = readU(st, cntDev + 1, K);
-            for (int k = 0; k < K; ++k) {
+            for (int k = 0; k < kAgeBatch; ++k) {
                 ++swept;
                 if (raised[k] == 0) return {CLB_OK, ""};
                 if (swept > 1000) return {CLB_E_AGE_SWEEPS, "age relaxation did not settle"};
             }
+            HIPT(hipMemsetAsync(ageCnt + 1, 0, kAgeBatch * sizeof(unsigned int), st));
+            for (int k = 0; k < kAgeBatch; ++k) HIPT(launchCaAgeInplace<R>(ca(), age, ageList, ageCnt, ageCnt + 1, k, st));
+            raised = readU(st, ageCnt + 1, kAgeBatch);
         }
     }
     // the single in-place sweep after a mask change (complab.cpp:3079), solved
@@ -1341,13 +1355,17 @@ struct Case : CaseBase {
             double prev = gmax; int stall = 0, wc = 0;
             while (gmax - p.bmax > 1e-12) {
                 HIPT(launchCaPushGather<R>(ca(), caSweep_++, st, true));     // the sweep and the mask/total update
+                // the age relaxation of every 50th sweep does not depend on
+                // the sweep's reduction: it is queued before the wait for it
+                const bool ageNow = wc % 50 == 0;
+                if (ageNow) ageRelaxQueue(st);
                 readCtl(st, &gmax, &msum);
                 ++sweeps; cnt[6]++;
                 if (!std::isfinite(gmax)) { if (sweepsOut) *sweepsOut = sweeps; return {CLB_E_NONFINITE, "non-finite biomass"}; }
                 if (gmax > prev - 1e-12) { if (++stall >= 100) { if (sweepsOut) *sweepsOut = sweeps; return {CLB_E_CA_STALL, "CA spill stalled"}; } }
                 else stall = 0;
                 prev = gmax;
-                if (wc % 50 == 0) { Status s = ageRelax(st, true); if (s.code) return s; }
+                if (ageNow) { Status s = ageRelaxFinish(st); if (s.code) return s; }
                 if (wc > 1000) { if (sweepsOut) *sweepsOut = sweeps; return {CLB_E_CA_SWEEPS, "CA spill did not settle"}; }
                 ++wc;
             }
@@ -1407,9 +1425,11 @@ struct Case : CaseBase {
             if (lastMaskSum != oldTotMask) {
                 oldTotMask = lastMaskSum;
                 cnt[5]++;
+                // the solute switch does not depend on the ages: queued first,
+                // it runs before the age sweep's wait
+                if (p.ade_update_interval > 0 && iT % p.ade_update_interval == 0) HIPT(launchOmegaSwitch(mask, list, adeCls, Nopen, st));
                 Status a = ageRelax(st, false);
                 if (a.code) return a;
-                if (p.ade_update_interval > 0 && iT % p.ade_update_interval == 0) HIPT(launchOmegaSwitch(mask, list, adeCls, Nopen, st));
                 if (p.ns_update_interval > 0 && iT % p.ns_update_interval == 0 && p.flow_on && !nsSaturate) {
                     Status r = resolve(st, p.perm_ratio, true);
                     if (r.code) return r;
