@@ -55,11 +55,18 @@ def test_precision_form_fp64_matches_oracle(ctx):
 
 
 def test_reduced_precision_against_fp64(ctx):
+    # field bounds about four times, drift bounds about ten times the measured
+    # values of this seeded case (FP32 transport: 5.4e-6 field, 1.0e-5 drift;
+    # full FP32 with its FP64-arithmetic flow collision: 5.9e-5 field, 1.1e-3
+    # drift; no mask voxel differs)
     steps = 600
     hi = study.run(ctx, small(P.FP64), steps)
+    bounds = {P.FP32_ADE: (2e-5, 1e-4), P.FP32: (2e-4, 1e-2)}
     for prec in (P.FP32_ADE, P.FP32):
         lo = study.run(ctx, small(prec), steps)
         r = study.compare(lo, hi, steps)
         print(prec, r)
-        assert all(np.isfinite(r["field_max_rel_err"]))
-        assert max(r["total_drift_per_1e4_steps"]) < 1.0
+        field, drift = bounds[prec]
+        assert r["mask_voxels_differ"] == 0
+        assert max(r["field_max_rel_err"]) <= field
+        assert max(r["total_drift_per_1e4_steps"]) <= drift
