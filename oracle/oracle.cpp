@@ -1943,6 +1943,19 @@ int orc_geom_pack(int nxf, int ny, int nz, double radius, double porosity, uint6
 
 }  // extern "C"
 
+/* the convergence rule alone, for its test: the values fed one at a time to a
+ * Tracer as the solve feeds the mean energies; *stop = the 1-based count at
+ * which it first reports convergence, 0 if it never does */
+extern "C" int orc_tracer_stop(const double *vals, int n, double eps, int *stop) {
+    Tracer t;
+    *stop = 0;
+    for (int k = 0; k < n; ++k) {
+        t.take(vals[k]);
+        if (t.converged(eps)) { *stop = k + 1; break; }
+    }
+    return 0;
+}
+
 /* layout of clb_case_params as this compiler lays it out, for the ctypes mirror */
 #include <cstddef>
 extern "C" int orc_params_layout(int64_t *out) {

@@ -52,6 +52,7 @@ def lib():
             ("orc_precip_convert", [P]),
             ("orc_age_relax", [P, C.c_int, P]),
             ("orc_params_layout", [P]),
+            ("orc_tracer_stop", [P, C.c_int, C.c_double, P]),
             ("orc_read_field", [P, C.c_int, C.c_int, P]),
             ("orc_write_field", [P, C.c_int, C.c_int, P]),
             ("orc_species_totals", [P, P]),
