@@ -1609,15 +1609,25 @@ __global__ void __launch_bounds__(256) k_ns_step(NsDev<R> a, double *partE, doub
 // the convergence rule, evaluated on the device after every iteration.  The
 // partials are summed by 1024 threads, thread t taking t, t + 1024, ... (all
 // of its loads issued together), then in a fixed tree; the window statistics
-// run in the ring's order (oldest first) from a copy in LDS.
+// run in the ring's order (oldest first).  The ring's head and fill and each
+// thread's sample of the window's W - 1 older values are loaded beside the
+// partials (thread 0 then overwrites only the head slot, which no thread
+// reads), so the statistics wait on no further memory round trip.
 __device__ __forceinline__ void nsTracerBlock(NsTracer *t, const double *partE, const double *partN, int nparts, int rule,
                                               double eps, int *done) {
     __shared__ double sE[16], sN[16];
     __shared__ double win[16];
     __shared__ double sVal;
-    __shared__ int sHead, sFull;
     if (*done) return;
     const int tid = threadIdx.x;
+    const int W = NS_TRACER_WINDOW;
+    static_assert(NS_TRACER_WINDOW <= 512, "one window sample per thread of the first eight waves");
+    int h0 = 0, c0 = 0;
+    double xr = 0.0;
+    if (rule == 0) {
+        h0 = t->head; c0 = t->count;
+        if (tid < W - 1) xr = t->ring[(h0 + 1 + tid) % W];
+    }
     double E = 0, C = 0;
     for (int i0 = tid; i0 < nparts; i0 += 1024 * 8) {
         double e[8], c[8];
@@ -1640,30 +1650,24 @@ __device__ __forceinline__ void nsTracerBlock(NsTracer *t, const double *partE, 
         t->iters += 1;
         t->last = val;
         sVal = val;
-        sFull = 0;
         if (!isfinite(val)) { t->nonfinite = 1; *done = 1; }
         else if (rule == 0) {                 // ValueTracer(1, 1000, eps): window 500
-            const int W = NS_TRACER_WINDOW;
-            const int h = t->head;
-            t->ring[h] = val;
-            t->head = (h + 1) % W;
-            if (t->count < W) t->count += 1;
-            sHead = (h + 1) % W;
-            sFull = t->count >= W;
+            t->ring[h0] = val;
+            t->head = (h0 + 1) % W;
+            if (c0 < W) t->count = c0 + 1;
         } else {
             if (t->prevE > 0 && fabs(val - t->prevE) <= eps * fabs(val)) { t->converged = 1; *done = 1; }
             t->prevE = val;
         }
     }
     __syncthreads();
-    if (!sFull) return;
+    const double val = sVal;
+    if (rule != 0 || !isfinite(val) || c0 + 1 < W) return;
     // the window's mean and population standard deviation, both sums as a
     // tree over the first 512 threads (wave sums, then the eight wave
     // partials in order; threads 512 and up add zeros): rounding-level apart
     // from the sequential sums
-    const int W = NS_TRACER_WINDOW;
-    static_assert(NS_TRACER_WINDOW <= 512, "one window sample per thread of the first eight waves");
-    const double x = tid < W - 1 ? t->ring[(sHead + tid) % W] : (tid == W - 1 ? sVal : 0.0);
+    const double x = tid < W - 1 ? xr : (tid == W - 1 ? val : 0.0);
     const int lane = tid & 63, wv = tid >> 6;
     double sx = x;
     for (int off = 32; off > 0; off >>= 1) sx += __shfl_down(sx, off, 64);
