@@ -23,6 +23,7 @@ FP64, FP32, FP32_ADE = 0, 1, 2
 BC_DIRICHLET, BC_NEUMANN, BC_CLOSED = 0, 1, 2
 CONV_TRACER, CONV_RELDELTA = 0, 1
 TUNE_NS_SHAPE = 0     # clb_set_tuning knobs
+TUNE_ADE_BB = 1       # the transport bounce-back store: 1 on, 0 off (before the first transport launch)
 BATCH_STEPS, BATCH_STABILIZE = 0, 1   # clb_batch_set_tile / clb_batch_tuning modes
 BIO_NONE, BIO_DUAL_MONOD = 0, 1
 CA_FRACTION, CA_HALF = 0, 1

@@ -1715,6 +1715,17 @@ struct Case : CaseBase {
             nsShape = value;
             return {CLB_OK, ""};
         }
+        if (knob == CLB_TUNE_ADE_BB) {
+            if (value != 0 && value != 1) return {CLB_E_ARG, "the transport bounce-back store knob takes 0 or 1"};
+            if (launchNo > 0) return {CLB_E_STATE, "the transport bounce-back store is chosen before the first transport launch"};
+            if (value == 1 && p.eq_on) return {CLB_E_ARG, "the equilibrium kernel does not keep the transport bounce-back store in step"};
+            if (value == 1 && Sm > 0 && Nint > 0 && !fbbA) {
+                fbbA = dalloc<R>(allocs, (size_t) Sm * adeBBStride());
+                fbbB = dalloc<R>(allocs, (size_t) Sm * adeBBStride());
+            }
+            adeBBOn = value == 1 && Sm > 0 && Nint > 0;
+            return {CLB_OK, ""};
+        }
         return {CLB_E_ARG, "unknown tuning knob"};
     }
     void kernelTime(hipStream_t, int reset, double *ms, int64_t *n) override {

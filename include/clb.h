@@ -453,8 +453,14 @@ int  clb_set_timing_stride(clb_ctx *ctx, int case_id, int stride);
  * (complab_amd.tune); results do not depend on them.  No counterpart in the
  * reference.  CLB_TUNE_NS_SHAPE: the flow interior launch, 0 one workgroup per
  * 256 voxels, n > 0 n workgroups each looping over a contiguous run of them and
- * loading the next one's neighbour records ahead. */
+ * loading the next one's neighbour records ahead.  CLB_TUNE_ADE_BB: 1 the
+ * transport pull reads the populations returning from closed neighbours from
+ * a dense store beside each population buffer, 0 from the voxel's own slots;
+ * chosen at creation (the store where closed links average at most one per
+ * interior voxel and no equilibrium runs), settable before the first
+ * transport launch only, and 1 is refused with equilibrium. */
 #define CLB_TUNE_NS_SHAPE 0
+#define CLB_TUNE_ADE_BB   1
 int  clb_set_tuning(clb_ctx *ctx, int case_id, int knob, int value);
 
 /* ---- the batched form ------------------------------------------------------
