@@ -23,7 +23,7 @@ namespace clb {
 // solver.  Up to 6 components the register file holds the solver (8x6
 // Newton: no scratch, AA-PCF: 76 bytes a lane; 8x8: 1160 and 624).  The iteration is
 // a chain of dependent FP64 operations, so waves in flight hide its latency:
-// Newton with nc <= 3 is held to 168 VGPRs for 3 waves per SIMD (a 36-byte
+// Newton with nc <= 3 is held to 168 VGPRs for 3 waves per SIMD (a 28-byte
 // spill, 17 % faster than 2 waves at 220 VGPRs on the 512x256x256 carbonate
 // case); the larger solvers keep every register they need.
 template <int NS, int NC, int SOLVER>
