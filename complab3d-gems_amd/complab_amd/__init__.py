@@ -353,7 +353,9 @@ class Case:
             self.restore(f.read())
 
     def set_tuning(self, knob, value):
-        """A launch shape of the hot kernels (P.TUNE_*); results do not change."""
+        """A knob of the hot kernels (P.TUNE_*: the flow launch shape, the
+        transport bounce-back store before the first transport launch);
+        results do not change."""
         self._c(lib().clb_set_tuning(self.ctx.h, self.id, int(knob), int(value)))
 
     def set_timing_stride(self, stride):
